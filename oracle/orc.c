@@ -1,0 +1,957 @@
+/* orc.c -- CPU restatement of one SLOSH time step.  TEST INFRASTRUCTURE ONLY:
+ * the checker for the HIP kernels, never linked into the product.
+ *
+ * Serial, in the reference's loop order, following (file:line under
+ * sorc/slosh.fd):
+ *   TMSTPC1  tmstpc1.f:55-92   CONTTY contty.f:85-221   SMPT2G smpt2g.f:23-175
+ *            FLT2G flt2g.f:38-132   BDRYHT bdryht.f:72-86
+ *   TMSTPC2  tmstpc2.f:54-118  HMXSV hmxsv.f:44-136   FILTER filter.f:121-211
+ *            XFILTER xfilter.f:176-713
+ *   TMSTPM   tmstpm.f:20        MOMNTM momntm.f:78-470  FRCPNT frcpnt.f:63-200
+ *            MNTMBD mntmbd.f:58-117
+ * TTIMER/STMVAL (the storm side) are supplied by the caller in orc_storm.
+ * REAL is float or double; literals are cast to REAL as the reference's
+ * default-real constants are.
+ */
+#include <math.h>
+#include <string.h>
+#include "orc.h"
+
+typedef REAL R;
+#define RC(x) ((R) (x))
+#define F2(a, i, j) (a)[(size_t) ((j) - 1) * imxb + ((i) - 1)]
+#define HB(i, j) F2(st->hb, i, j)
+#define UB(i, j) F2(st->ub, i, j)
+#define VB(i, j) F2(st->vb, i, j)
+#define HMX(i, j) F2(st->hmx, i, j)
+#define HSUB(i, j) F2(st->hsub, i, j)
+#define ZB(i, j) F2(b->zb, i, j)
+#define ZBM(i, j) F2(b->zbm, i, j)
+#define YLT(i, j) F2(b->ylt, i, j)
+#define ITREE(i, j) F2(b->itree, i, j)
+#define KSKP(i, j) F2(b->kskp, i, j)
+#define IMANN(i, j) F2(b->imann, i, j)
+#define IS(j) b->is[(j) - 1]
+#define IE(j) b->ie[(j) - 1]
+#define MS(j) b->ms[(j) - 1]
+#define ME(j) b->me[(j) - 1]
+#define ELPCT(i) b->elpct[(i) - 1]
+#define ELPDT(i) b->elpdt[(i) - 1]
+#define ELPCL(i) b->elpcl[(i) - 1]
+#define ELPDL(i) b->elpdl[(i) - 1]
+#define ELPCL2(i) b->elpcl2[(i) - 1]
+#define ELPDL2(i) b->elpdl2[(i) - 1]
+#define SINT(j) b->sint[(j) - 1]
+#define COST(j) b->cost[(j) - 1]
+#define SINL(j) b->sinl[(j) - 1]
+#define COSL(j) b->cosl[(j) - 1]
+#define SINL2(j) b->sinl2[(j) - 1]
+#define TAB(t, ii, n) b->t[(size_t) ((n) - 1) * 245 + ((ii) - 1)]
+#define AMAX1(a, b) ((a) > (b) ? (a) : (b))
+#define AMIN1(a, b) ((a) < (b) ? (a) : (b))
+
+/* BLOCK DATA shift tables (flw1dm_block_c.f:37-38) */
+static const int IP[5] = {0, 0, 1, 1, 0}, JP[5] = {0, 0, 0, 1, 1};
+static const int IH[5] = {0, -1, 0, -1, 0}, JH[5] = {0, -1, -1, 0, 0};
+static const int IIH[5] = {0, 0, 1, 0, -1}, JJH[5] = {0, -1, 0, 1, 0};
+
+#define M2G_ 30000
+#define MCT_ 8000
+
+/* ------------------------------------------------------------------ CONTTY */
+void orc_contty(const orc_basin *b, const orc_storm *s, orc_state *st, orc_diag *d)
+{
+   const int imxb = b->imxb, jmxb = b->jmxb, imxb1 = imxb - 1, jmxb1 = jmxb - 1;
+   int i, j, k, ib, ifn, jump, leap, iset = 0, ipass = 0;
+   R dux, dvy, hpst, z, fct, hbj, chng, denm, anum, rfct;
+
+   for (j = 1; j <= jmxb; j++)
+      for (i = 1; i <= imxb1; i++) HSUB(i, j) = HB(i, j);
+   for (j = 2; j <= jmxb1; j++)
+      for (i = 2; i <= imxb1; i++)
+         if (fabs(UB(i, j)) <= RC(1.E-10)) {
+            UB(i, j) = 0;
+            VB(i, j) = 0;
+         }
+   j = 1;
+   ib = IS(j);
+   ifn = IE(j);
+   jump = 4;
+   leap = 1;
+L130:
+   if (ifn >= ib) {
+      for (i = ib; i <= ifn; i++) {
+         if (KSKP(i, j) == '0') continue;
+         if (leap == 2 && i == ipass) {
+            HB(i, j) = -ZB(i, j);
+            leap = 1;
+            continue;
+         }
+         dux = -UB(i, j) + UB(i + 1, j) + UB(i + 1, j + 1) - UB(i, j + 1);
+         if (ZB(i, j) <= RC(30.)) {
+            if (dux == 0) continue;
+         }
+         hpst = HSUB(i, j);
+         z = ELPDL2(i) + (ELPCL2(i) - ELPDL2(i)) * SINL2(j);
+         fct = s->x1b5 / z;
+         dvy = -VB(i, j) - VB(i + 1, j) + VB(i + 1, j + 1) + VB(i, j + 1);
+         hbj = hpst - fct * (dux + dvy);
+         chng = hbj + ZB(i, j);
+         if (chng < 0) {
+            if (-chng < RC(1E-3)) {
+               HB(i, j) = -ZB(i, j);
+               continue;
+            }
+         } else {
+            HB(i, j) = hbj;
+            continue;
+         }
+         if (HSUB(i, j) == -ZB(i, j)) continue;
+         if (d) d->drying_events++;
+         denm = hbj - hpst;
+         denm = copysign(AMAX1(RC(1.E-5), fabs(denm)), denm);
+         anum = -ZB(i, j) - hpst;
+         rfct = anum / denm;
+         if (fabs(rfct) < RC(1.E-5)) rfct = 0;
+         for (k = 1; k <= 4; k++) {
+            int ia = i + IP[k], ja = j + JP[k];
+            UB(ia, ja) = rfct * UB(ia, ja);
+            VB(ia, ja) = rfct * VB(ia, ja);
+         }
+         j = j - 1;
+         jump = 1;
+         iset = i;
+         leap = 1;
+         if (j == 0) goto L330;
+         if (KSKP(i, j) == '0') goto L330;
+         if (IS(j) > i) goto L330;
+         ib = (i - 1 > IS(j)) ? i - 1 : IS(j);
+         ifn = (i + 1 < IE(j)) ? i + 1 : IE(j);
+         goto L130;
+      }
+   }
+L330:
+   jump = jump + 1;
+   leap = 1;
+   if (jump == 2) {
+      j = j + 1;
+      leap = 2;
+      ib = (iset - 1 > IS(j)) ? iset - 1 : IS(j);
+      ifn = IE(j);
+      ipass = iset;
+   } else {
+      j = j + 1;
+      if (j > jmxb1) goto L350;
+      ib = IS(j);
+      ifn = IE(j);
+   }
+   goto L130;
+L350:
+   for (i = 1; i <= imxb1; i++) HB(i, jmxb) = HB(i, 1);
+}
+
+/* ------------------------------------------------------------ SMPT2G/FLT2G */
+static short i2g[M2G_ + 1], j2g[M2G_ + 1];
+static short mct[MCT_ + 1], i2gg[MCT_ + 1], j2gg[MCT_ + 1];
+
+static void flt2g(const orc_basin *b, orc_state *st, int mm2g, int mcmx)
+{
+   const int imxb = b->imxb, jmxb = b->jmxb;
+   int m, m1, mmm, nnn, i, j, k, ii, jj, kk, ia, ibb, ja, jb;
+   R hct, hsm, hdif, gam0, gamf, gamfk, z, zz, zzz, hzz, hbtemp;
+   if (mcmx != 0)
+      for (m = 1; m <= mcmx; m++) HSUB(i2g[mm2g + m], j2g[mm2g + m]) = 999;
+   hct = 1;
+   m1 = 1;
+   mmm = mm2g;
+   nnn = 0;
+L3000:
+   nnn = nnn + 1;
+   if (nnn > 2) return;
+   for (m = m1; m <= mmm; m++) {
+      i = i2g[m];
+      j = j2g[m];
+      if (HSUB(i, j) + ZB(i, j) <= hct) continue;
+      hsm = 0;
+      hdif = 0;
+      gam0 = ELPDL2(i) + (ELPCL2(i) - ELPDL2(i)) * SINL2(j);
+      for (k = 1; k <= 4; k++) {
+         ii = i + IIH[k];
+         jj = j + JJH[k];
+         if (ii == 0 || jj == 0) {
+            gamf = 0;
+            goto L160;
+         }
+         if (b->ebsn == '$' || b->ebsn == '+') {
+            gamfk = ELPDL2(ii) + (ELPCL2(ii) - ELPDL2(ii)) * SINL2(jj);
+            z = gamfk / gam0;
+            gamf = RC(2.) * z / (RC(1.) + z) - RC(1.);
+         } else {
+            gamf = b->gamfp[k - 1];
+         }
+         if (ii == imxb || jj == jmxb) goto L160;
+         if (HSUB(ii, jj) == 999) goto L160;
+         if (HSUB(ii, jj) + ZB(ii, jj) <= hct) goto L160;
+         kk = k % 4 + 1;
+         ia = i + IP[k];
+         ibb = i + IP[kk];
+         ja = j + JP[k];
+         jb = j + JP[kk];
+         z = ZBM(ia, ja);
+         zz = ZBM(ibb, jb);
+         zzz = AMIN1(z, zz) + hct;
+         if (nnn == 2) {
+            if (HSUB(i, j) < zzz && HSUB(ii, jj) < zzz) {
+               hsm = hsm + HSUB(i, j);
+               hzz = HSUB(i, j);
+            } else {
+               hsm = hsm + HSUB(ii, jj);
+               hzz = HSUB(ii, jj);
+            }
+         } else {
+            if (HSUB(i, j) < zzz || HSUB(ii, jj) < zzz) {
+               hsm = hsm + HSUB(i, j);
+               hzz = HSUB(i, j);
+            } else {
+               hsm = hsm + HSUB(ii, jj);
+               hzz = HSUB(ii, jj);
+            }
+         }
+         goto L210;
+      L160:
+         hsm = hsm + HSUB(i, j);
+         hzz = HSUB(i, j);
+      L210:
+         hdif = hdif + hzz * gamf;
+      }
+      hbtemp = (RC(1.) / RC(3.)) * HSUB(i, j) + (hsm + hdif) / RC(6.);
+      HB(i, j) = AMAX1(hbtemp, -ZB(i, j));
+   }
+   if (mcmx == 0) return;
+   for (m = 1; m <= mm2g; m++) HSUB(i2g[m], j2g[m]) = 999;
+   hct = 0;
+   m1 = mm2g + 1;
+   mmm = mm2g + mcmx;
+   for (m = m1; m <= mmm; m++) HSUB(i2g[m], j2g[m]) = HB(i2g[m], j2g[m]);
+   goto L3000;
+}
+
+int orc_smpt2g(const orc_basin *b, const orc_storm *s, orc_state *st, orc_diag *d)
+{
+   const int imxb = b->imxb, jmxb = b->jmxb, imxb1 = imxb - 1, jmxb1 = jmxb - 1;
+   const int imxb2 = imxb - 2, jmxb2 = jmxb - 2;
+   int m, mc, mm, mm2g, mcmx, i, j, k, ii, jj;
+   (void) s;
+   m = 0;
+   for (j = 2; j <= jmxb2; j++)
+      for (i = 2; i <= imxb2; i++)
+         if (UB(i, j) != 0 && ZBM(i, j) > RC(-30.))
+            if (UB(i - 1, j) == 0 || UB(i, j - 1) == 0 || UB(i, j + 1) == 0 || UB(i + 1, j) == 0) {
+               if (m + 1 >= M2G_) {
+                  if (d) d->smooth_overflow = 1;
+                  return -1;
+               }
+               m = m + 1;
+               i2g[m] = i;
+               j2g[m] = j;
+            }
+   mm2g = m;
+   for (j = 1; j <= jmxb1; j++)
+      for (i = 1; i <= imxb1; i++) HSUB(i, j) = 999;
+   mc = 0;
+   for (m = 1; m <= mm2g; m++) {
+      ii = i2g[m];
+      if (ii > 1 && ii < imxb) {
+         jj = j2g[m];
+         if (jj > 1 && jj < jmxb)
+            for (k = 1; k <= 4; k++) {
+               i = ii + IH[k];
+               j = jj + JH[k];
+               if (HB(i, j) + ZB(i, j) != 0) HSUB(i, j) = HB(i, j);
+               else {
+                  if (mc + 1 >= MCT_) {
+                     if (d) d->smooth_overflow = 1;
+                     return -1;
+                  }
+                  mc = mc + 1;
+                  mct[mc] = m;
+               }
+            }
+      }
+   }
+   mcmx = mc;
+   if (mcmx != 0)
+      for (m = 1; m <= mcmx; m++) {
+         mm = mct[m];
+         ii = i2g[mm];
+         jj = j2g[mm];
+         for (k = 1; k <= 4; k++) {
+            i = ii + IH[k];
+            j = jj + JH[k];
+            if (HB(i, j) + ZB(i, j) != 0) HSUB(i, j) = 888;
+         }
+      }
+   m = 0;
+   mm = 0;
+   for (j = 1; j <= jmxb1; j++)
+      for (i = 1; i <= imxb1; i++)
+         if (HSUB(i, j) != 999) {
+            if (HSUB(i, j) != 888) {
+               if (m + mm >= M2G_) goto L140;
+               m = m + 1;
+               i2g[m] = i;
+               j2g[m] = j;
+            } else {
+               if (mm >= MCT_) goto L140;
+               mm = mm + 1;
+               i2gg[mm] = i;
+               j2gg[mm] = j;
+            }
+         }
+L140:
+   mm2g = m;
+   mcmx = mm;
+   if (d) {
+      d->smooth_pts = mm2g;
+      d->smooth_special = mcmx;
+   }
+   if (mm2g >= M2G_ || mcmx >= MCT_) {
+      if (d) d->smooth_overflow = 1;
+      return -1;
+   }
+   for (m = 1; m <= mcmx; m++) {
+      i2g[m + mm2g] = i2gg[m];
+      j2g[m + mm2g] = j2gg[m];
+   }
+   flt2g(b, st, mm2g, mcmx);
+   return 0;
+}
+
+/* ------------------------------------------------------------------ BDRYHT */
+void orc_bdryht(const orc_basin *b, const orc_storm *s, orc_state *st)
+{
+   const int imxb = b->imxb;
+   int n, i, j, k;
+   for (n = 0; n < b->nbcpt; n++) {
+      R xr, yr, x, y, rsq, r1, r2, dr;
+      i = b->ish[n];
+      j = b->jsh[n];
+      xr = ELPCL(i) * COSL(j);
+      yr = ELPDL(i) * SINL(j);
+      x = xr - s->c1 - s->ax;
+      y = yr - s->c2 - s->ay;
+      rsq = x * x + y * y;
+      r1 = sqrt(rsq) / RC(5280.) + RC(1.);
+      k = (int) r1;
+      r2 = k;
+      dr = r1 - r2;
+      if (k > 790) k = 790;
+      HB(i, j) = s->DELP[k] + dr * (s->DELP[k + 1] - s->DELP[k]) + s->seadtm + RC(0.);
+   }
+}
+
+/* ------------------------------------------------------------------- HMXSV */
+static void seg_bounds(const orc_basin *b, int m, int *jl, int *jm, int *jn, int *il,
+                       int *im, int *in)
+{
+   /* IBB(6,3) of hmxsv.f:49-63 / filter.f:101-115 */
+   const int imxb1 = b->imxb - 1, jmxb1 = b->jmxb - 1, imxb2 = imxb1 - 1, jmxb2 = jmxb1 - 1;
+   if (m == 1) {
+      *jl = 2; *jm = jmxb2; *jn = 1; *il = 2; *im = imxb2; *in = 1;
+   } else if (m == 2) {
+      *jl = 1; *jm = jmxb1; *jn = jmxb2; *il = 1; *im = imxb1; *in = 1;
+   } else {
+      *jl = 2; *jm = jmxb2; *jn = 1; *il = 1; *im = imxb1; *in = imxb2;
+   }
+}
+
+void orc_hmxsv(const orc_basin *b, const orc_storm *s, orc_state *st)
+{
+   const int imxb = b->imxb, jmxb = b->jmxb;
+   int m, i, j, k, jl, jm, jn, il, im, in;
+   (void) s;
+   for (m = 1; m <= 3; m++) {
+      seg_bounds(b, m, &jl, &jm, &jn, &il, &im, &in);
+      for (j = jl; j <= jm; j += jn)
+         for (i = il; i <= im; i += in) {
+            R hx = HMX(i, j);
+            if (hx < HB(i, j)) {
+               if (ZB(i, j) > RC(40.) || b->ebsn1 == '&') {
+                  HMX(i, j) = HB(i, j);
+               } else {
+                  R hst = HB(i, j) + ZB(i, j);
+                  if (hst < RC(1.)) {
+                     HMX(i, j) = HB(i, j);
+                  } else {
+                     R hsm = 0, hdif = 0, gam0, gamf, hzz, htemp;
+                     gam0 = ELPDL2(i) + (ELPCL2(i) - ELPDL2(i)) * SINL2(j);
+                     for (k = 1; k <= 4; k++) {
+                        int ii = i + IIH[k], jj = j + JJH[k];
+                        if (m != 1 && (ii == 0 || jj == 0)) {
+                           gamf = 0;
+                           hsm = hsm + HB(i, j);
+                           hzz = HB(i, j);
+                        } else {
+                           R gamfk = ELPDL2(ii) + (ELPCL2(ii) - ELPDL2(ii)) * SINL2(jj);
+                           R z = gamfk / gam0;
+                           gamf = RC(2.) * z / (RC(1.) + z) - RC(1.);
+                           if ((jj != jmxb && ii != imxb) && HB(ii, jj) + ZB(ii, jj) >= RC(1.)) {
+                              int kk = k % 4 + 1;
+                              int ia = i + IP[k], ibb = i + IP[kk], ja = j + JP[k], jb = j + JP[kk];
+                              R zz1 = ZBM(ia, ja), zz2 = ZBM(ibb, jb);
+                              R zzz = AMIN1(zz1, zz2) + RC(1.);
+                              if (HB(i, j) < zzz || HB(ii, jj) < zzz) {
+                                 hsm = hsm + HB(i, j);
+                                 hzz = HB(i, j);
+                              } else {
+                                 hsm = hsm + HB(ii, jj);
+                                 hzz = HB(ii, jj);
+                              }
+                           } else {
+                              hsm = hsm + HB(i, j);
+                              hzz = HB(i, j);
+                           }
+                        }
+                        hdif = hdif + hzz * gamf;
+                     }
+                     htemp = (RC(4.) * HB(i, j) + hsm + hdif) / RC(8.);
+                     HMX(i, j) = AMAX1(hx, htemp);
+                  }
+               }
+            }
+         }
+   }
+}
+
+/* ------------------------------------------------------------------ FILTER */
+void orc_filter(const orc_basin *b, const orc_storm *s, orc_state *st)
+{
+   const int imxb = b->imxb, jmxb = b->jmxb, imxb1 = imxb - 1, jmxb1 = jmxb - 1;
+   const R hcrt = RC(0.5);
+   int m, i, j, k, jl, jm, jn, il, im, in;
+   (void) s;
+   for (j = 1; j <= jmxb1; j++)
+      for (i = 1; i <= imxb1; i++) HSUB(i, j) = HB(i, j);
+   for (m = 1; m <= 3; m++) {
+      seg_bounds(b, m, &jl, &jm, &jn, &il, &im, &in);
+      for (j = jl; j <= jm; j += jn)
+         for (i = il; i <= im; i += in) {
+            R hst, hsm, hdif, gam0, gamf, hzz, htemp;
+            if (ZB(i, j) <= -b->hter) continue;
+            hst = HSUB(i, j) + ZB(i, j);
+            if (hst < hcrt) continue;
+            hsm = 0;
+            hdif = 0;
+            gam0 = ELPDL2(i) + (ELPCL2(i) - ELPDL2(i)) * SINL2(j);
+            for (k = 1; k <= 4; k++) {
+               int ii = i + IIH[k], jj = j + JJH[k];
+               if (m != 1) {
+                  if (ii == 0 || jj == 0) {
+                     gamf = 0;
+                     hsm = hsm + HSUB(i, j);
+                     hzz = HSUB(i, j);
+                     hdif = hdif + hzz * gamf;
+                     continue;
+                  }
+               }
+               if (b->ebsn == '$' || b->ebsn == '+') {
+                  R gamfk = ELPDL2(ii) + (ELPCL2(ii) - ELPDL2(ii)) * SINL2(jj);
+                  R z = gamfk / gam0;
+                  gamf = RC(2.) * z / (RC(1.) + z) - RC(1.);
+               } else {
+                  gamf = b->gamfp[k - 1];
+               }
+               if (ii == imxb || jj == jmxb) {
+                  hsm = hsm + HSUB(i, j);
+                  hzz = HSUB(i, j);
+                  hdif = hdif + hzz * gamf;
+                  continue;
+               }
+               if (HSUB(ii, jj) + ZB(ii, jj) < hcrt) {
+                  hsm = hsm + HSUB(i, j);
+                  hzz = HSUB(i, j);
+                  hdif = hdif + hzz * gamf;
+                  continue;
+               }
+               {
+                  int kk = k % 4 + 1;
+                  int ia = i + IP[k], ibb = i + IP[kk], ja = j + JP[k], jb = j + JP[kk];
+                  R z1 = ZBM(ia, ja), zz = ZBM(ibb, jb), zzz = AMIN1(z1, zz) + hcrt;
+                  if (HSUB(i, j) < zzz || HSUB(ii, jj) < zzz) {
+                     hsm = hsm + HSUB(i, j);
+                     hzz = HSUB(i, j);
+                  } else {
+                     hsm = hsm + HSUB(ii, jj);
+                     hzz = HSUB(ii, jj);
+                  }
+                  hdif = hdif + hzz * gamf;
+               }
+            }
+            htemp = RC(.5) * HSUB(i, j) + (hsm + hdif) / RC(8.);
+            HB(i, j) = AMAX1(-ZB(i, j), htemp);
+         }
+   }
+}
+
+/* ----------------------------------------------------------------- XFILTER */
+void orc_xfilter(const orc_basin *b, const orc_storm *s, orc_state *st, int iversion)
+{
+   const int imxb = b->imxb, jmxb = b->jmxb, imxb1 = imxb - 1, jmxb1 = jmxb - 1;
+   static const int IIX[5] = {0, -1, 1, 1, -1}, JJX[5] = {0, -1, -1, 1, 1};
+   static const int IP1[5] = {0, -1, 1, 2, 0}, JP1[5] = {0, 0, -1, 1, 2};
+   static const int IP2[5] = {0, 0, 2, 1, -1}, JP2[5] = {0, -1, 0, 2, 1};
+   R hcrt = 0, exclude = 0;
+   int m, i, j, k, jl, jm, jn, il, im, in;
+   (void) s;
+   switch (b->ismth) {
+   case 2: hcrt = RC(0.5); exclude = hcrt; break;
+   case 3: hcrt = RC(0.1); exclude = hcrt; break;
+   case 4: hcrt = RC(0.2); exclude = hcrt; break;
+   case 5: hcrt = RC(0.3); exclude = hcrt; break;
+   case 6: hcrt = RC(0.4); exclude = hcrt; break;
+   case 7: hcrt = RC(1.E-5); exclude = RC(1.E-5); break;
+   case 8: hcrt = RC(0.2); exclude = RC(1.E-5); break;
+   case 9: hcrt = RC(0.3); exclude = RC(1.E-5); break;
+   case 10: hcrt = RC(0.4); exclude = RC(1.E-5); break;
+   case 11: hcrt = RC(0.5); exclude = RC(1.E-5); break;
+   }
+   for (j = 1; j <= jmxb1; j++)
+      for (i = 1; i <= imxb1; i++) HSUB(i, j) = HB(i, j);
+   for (m = 1; m <= 3; m++) {
+      seg_bounds(b, m, &jl, &jm, &jn, &il, &im, &in);
+      for (j = jl; j <= jm; j += jn)
+         for (i = il; i <= im; i += in) {
+            int maskii[5], mbarri[5], mbouni[5], maskix[5], mbarrx[5], mbounx[5], km;
+            const int maskc = 1;
+            R hsm2 = 0, xsm2 = 0, hst, htemp;
+            if (ZB(i, j) <= -b->hter) continue;
+            hst = HSUB(i, j) + ZB(i, j);
+            if (hst < exclude) continue;
+            for (k = 1; k <= 4; k++) {
+               int ii, jj, ix, jx, kk, km1, ia, ibb, ja, jb, ia2, ja2, ia1, ja1, iam1, jam1;
+               maskii[k] = mbarri[k] = mbouni[k] = maskix[k] = mbarrx[k] = mbounx[k] = 0;
+               ii = i + IIH[k];
+               jj = j + JJH[k];
+               ix = i + IIX[k];
+               jx = j + JJX[k];
+               if (ii == 0 || jj == 0 || ix == 0 || jx == 0 || i == 0 || j == 0) continue;
+               if (ii == imxb || jj == jmxb || HSUB(ii, jj) + ZB(ii, jj) < hcrt) {
+                  mbouni[k] = 1;
+               } else {
+                  int kk2 = k % 4 + 1;
+                  R z = ZBM(i + IP[k], j + JP[k]), zz = ZBM(i + IP[kk2], j + JP[kk2]);
+                  R zzz = AMIN1(z, zz) + hcrt;
+                  if (HSUB(i, j) < zzz && HSUB(ii, jj) < zzz) mbarri[k] = 1;
+                  else maskii[k] = 1;
+               }
+               kk = k % 4 + 1;
+               km1 = (k + 2) % 4 + 1;
+               ia = i + IP[k];
+               ibb = i + IP[kk];
+               ja = j + JP[k];
+               jb = j + JP[kk];
+               ia2 = i + IP2[k];
+               ja2 = j + JP2[k];
+               ia1 = i + IP1[k];
+               ja1 = j + JP1[k];
+               iam1 = i + IP[km1];
+               jam1 = j + JP[km1];
+               if (ix == imxb || jx == jmxb || HSUB(ix, jx) + ZB(ix, jx) < hcrt) {
+                  mbounx[k] = 1;
+               } else {
+                  R z = ZBM(ia, ja), zz = ZBM(ibb, jb), zzz = AMIN1(z, zz) + hcrt;
+                  R yy = ZBM(ia2, ja2), yyy = AMIN1(z, yy) + hcrt;
+                  R xx = ZBM(ia1, ja1), xxx = AMIN1(z, xx) + hcrt;
+                  R ww = ZBM(iam1, jam1), www = AMIN1(z, ww) + hcrt;
+                  if (((HSUB(i, j) >= zzz && HSUB(ii, jj) >= yyy) ||
+                       (HSUB(ix, jx) >= yyy && HSUB(ii, jj) >= zzz)) ||
+                      ((HSUB(i, j) >= www && HSUB(ia1, ja1) >= xxx) ||
+                       (HSUB(ix, jx) >= xxx && HSUB(ia1, ja1) >= www)))
+                     maskix[k] = 1;
+                  else
+                     mbarrx[k] = 1;
+               }
+            }
+            for (km = 1; km <= 4; km++) {
+               int kkm = km % 4 + 1, km1m = (km + 2) % 4 + 1;
+               int ix = i + IIX[km], jx = j + JJX[km], ii = i + IIH[km], jj = j + JJH[km];
+               int ix2 = i + IIX[kkm], jx2 = j + JJX[kkm], ii2 = i + IIH[kkm], jj2 = j + JJH[kkm];
+               int ix3 = i + IIX[km1m], jx3 = j + JJX[km1m], ii3 = i + IIH[km1m], jj3 = j + JJH[km1m];
+               if (ii == 0 || jj == 0 || ix == 0 || jx == 0 || i == 0 || j == 0 || ix2 == 0 ||
+                   jx2 == 0 || ix3 == 0 || jx3 == 0 || ii2 == 0 || jj2 == 0 || ii3 == 0 || jj3 == 0)
+                  continue;
+               hsm2 = hsm2 + (R) (maskc * maskii[km]) * HSUB(ii, jj) -
+                      (R) (maskc * maskii[km]) * HSUB(i, j);
+               if (iversion == 1) {
+                  R w = (R) (maskix[km] * maskii[km] * maskc * maskix[kkm]);
+                  xsm2 = xsm2 + RC(0.5) * (w * HSUB(ix, jx) - w * HSUB(i, j));
+               } else {
+                  xsm2 = xsm2 + RC(0.5) * (R) maskc * (R) maskix[km] * (R) maskii[km] *
+                                (R) maskii[km1m] * (HSUB(ix, jx) - HSUB(i, j));
+               }
+            }
+            htemp = HSUB(i, j) + RC(1.) / RC(8.) * (hsm2 - RC(1.) * (xsm2));
+            HB(i, j) = AMAX1(-ZB(i, j), htemp);
+         }
+   }
+}
+
+/* ------------------------------------------------------------------ FRCPNT */
+static int iik_of(const orc_basin *b, int i, int j)
+{
+   const int imxb = b->imxb;
+   if (ZB(i, j) >= 0) return (b->iver == 201903) ? 4 : 1;
+   if (b->iver == 201903) {
+      if (IMANN(i, j) >= 100) {
+         int iik = IMANN(i, j) - 6 + 1;
+         return iik > 245 ? 245 : iik;
+      }
+      return 95;
+   }
+   return !strncmp(b->sta, "SOUTH FL", 8) ? 245 : 1;
+}
+
+static void frcpnt(const orc_basin *b, const orc_storm *s, R x, R y, R *fxb, R *fyb, int ncatg,
+                   R *cshltr, int npls, int i, int j, R dph, int id)
+{
+   R xp = x - s->c1 - s->ax, yp = y - s->c2 - s->ay;
+   R rsq = xp * xp + yp * yp, rs = sqrt(rsq), r1 = rs / RC(5280.) + RC(1.), r2, dr;
+   R ccn, x1218, ck1, sk1, a, bb, fctt, fxx, fyy, aa, pxx, pyy;
+   int k = (int) r1, iik = iik_of(b, i, j);
+   r2 = k;
+   dr = r1 - r2;
+   if (k > 790) k = 790;
+   ccn = s->x12[15] + rsq;
+   x1218 = s->w1218[ncatg - 1];
+   if (ncatg == 1) {
+      ck1 = s->CW[k] + dr * (s->CW[k + 1] - s->CW[k]);
+      sk1 = s->SW[k] + dr * (s->SW[k + 1] - s->SW[k]);
+   } else {
+      *cshltr = 1;
+      ck1 = s->C[k] + dr * (s->C[k + 1] - s->C[k]);
+      sk1 = s->S[k] + dr * (s->S[k + 1] - s->S[k]);
+   }
+   a = rs * s->x12[9] - x1218 * (yp * ck1 + xp * sk1);
+   bb = rs * s->x12[10] + x1218 * (xp * ck1 - yp * sk1);
+   if (ncatg == 1) {
+      R rhol = fabs(xp * s->x12[20] + yp * s->x12[21]);
+      R ccc = ccn * rhol / (s->x12[15] + rhol * rhol);
+      a = a + ccc * s->x12[23];
+      bb = bb + ccc * s->x12[24];
+   }
+   a = a * s->x12[4] / ccn;
+   bb = bb * s->x12[4] / ccn;
+   fctt = RC(3.E-6) * sqrt(a * a + bb * bb);
+   fctt = fctt * *cshltr * s->bt;
+   fxx = fctt * a;
+   fyy = fctt * bb;
+   *fxb = TAB(cr2, iik, id) * fxx - TAB(ci2, iik, id) * fyy;
+   *fyb = TAB(cr2, iik, id) * fyy + TAB(ci2, iik, id) * fxx;
+   if (npls != 0) {
+      aa = -dph * (s->P[k] + dr * (s->P[k + 1] - s->P[k]));
+      pxx = aa * xp;
+      pyy = aa * yp;
+      *fxb = *fxb + TAB(br2, iik, id) * pxx - TAB(bi2, iik, id) * pyy;
+      *fyb = *fyb + TAB(br2, iik, id) * pyy + TAB(bi2, iik, id) * pxx;
+   }
+}
+
+/* ------------------------------------------------------------------ MNTMBD */
+static void mntmbd(const orc_basin *b, const orc_storm *s, orc_state *st, int jump, int ijump,
+                   int isb, int jsb, R arj, R aii, int i, int j, R ubj, R vbj, R *fxb, R *fyb,
+                   int ncatg, R *cshltr, int *npls, R dph, int id)
+{
+   const int imxb = b->imxb;
+   R aiul = aii * (UB(isb, jsb) - ubj), aivb = aii * (VB(isb, jsb) - vbj);
+   R xttx, xtty, fxd, fyd, xr, yr, fxbp, fybp;
+   if (jump == 2) {
+      xttx = 0;
+      xtty = 0;
+      UB(isb, jsb) = UB(i, j) + aiul + arj * (VB(isb, jsb) - vbj) + xttx;
+      VB(isb, jsb) = VB(i, j) + aivb - arj * (UB(isb, jsb) - UB(i, j)) + xtty;
+      return;
+   }
+   if (ijump != 2) {
+      fxd = *fxb;
+      fyd = *fyb;
+      xr = ELPCT(isb) * COST(jsb);
+      yr = ELPDT(isb) * SINT(jsb);
+      *npls = 1;
+      frcpnt(b, s, xr, yr, &fxbp, &fybp, ncatg, cshltr, *npls, i, j, dph, id);
+      xr = ELPDT(isb) * COST(jsb);
+      yr = ELPCT(isb) * SINT(jsb);
+      *fxb = xr * fxbp + yr * fybp;
+      *fyb = xr * fybp - yr * fxbp;
+      xttx = s->delt * (*fxb - fxd);
+      xtty = s->delt * (*fyb - fyd);
+      UB(isb, jsb) = UB(i, j) + aiul + arj * (VB(isb, jsb) - vbj) + xttx;
+      VB(isb, jsb) = VB(i, j) + aivb - arj * (UB(isb, jsb) - UB(i, j)) + xtty;
+      return;
+   }
+   *npls = 0;
+   *cshltr = 1;
+   xr = ELPCT(isb) * COST(jsb);
+   yr = ELPDT(isb) * SINT(jsb);
+   frcpnt(b, s, xr, yr, &fxbp, &fybp, ncatg, cshltr, *npls, i, j, dph, id);
+   xr = ELPDT(isb) * COST(jsb);
+   yr = ELPCT(isb) * SINT(jsb);
+   *fxb = xr * fxbp + yr * fybp;
+   *fyb = xr * fybp - yr * fxbp;
+   UB(isb, jsb) = aii * UB(isb, jsb) + arj * VB(isb, jsb) + s->delt * *fxb;
+   VB(isb, jsb) = aii * VB(isb, jsb) - arj * UB(isb, jsb) + s->delt * *fyb;
+}
+
+/* ------------------------------------------------------------------ MOMNTM */
+void orc_momntm(const orc_basin *b, const orc_storm *s, orc_state *st)
+{
+   const int imxb = b->imxb, jmxb = b->jmxb, jmxb1 = jmxb - 1;
+   static const int ISBS[5] = {0, 0, -1, 0, 1}, JSBS[5] = {0, -1, 0, 1, 0};
+   static const int ISHF[5][4] = {{0}, {0, 1, 2, 4}, {0, 1, 2, 3}, {0, 1, 3, 4}, {0, 2, 3, 4}};
+   const R shgt = RC(50.);
+   R slpai[601];
+   int i, j, jend, iskip, icor = 0, ist, ifn;
+   /* persistent across points like the reference's locals and COMMON /MF/ */
+   R dph = 0, hpd[5] = {0, 0, 0, 0, 0}, arj = 0, aii = 0, fxb = 0, fyb = 0, cshltr = 0;
+   int id = 0, ncatg = 1, npls = 1;
+
+   for (i = 1; i <= 599; i++)
+      slpai[i] = (b->iver == 201903) ? TAB(ai2, 4, i + 1) - TAB(ai2, 4, i)
+                                     : TAB(ai2, 1, i + 1) - TAB(ai2, 1, i);
+   slpai[600] = slpai[599];
+   jend = jmxb1;
+   if (b->dollar[0] == '$' && b->dollar[1] == ' ') jend = jmxb;
+   for (j = 2; j <= jend; j++) {
+      iskip = 2;
+      if (!(b->dollar[0] == '$' && b->dollar[1] == ' ')) {
+         if (j == 2) iskip = 1;
+         if (j == jend) iskip = 3;
+      }
+      ist = MS(j);
+      ifn = ME(j);
+      if (ifn - ist < 0) continue;
+      for (i = ist; i <= ifn; i++) {
+         int iik, jump, ijump, kk;
+         R dhx = 0, dhy = 0, aintrp, xttx, xtty, aiu, aiv, fct, ubj, vbj;
+         iik = iik_of(b, i, j);
+         jump = 1;
+         if (ITREE(i, j) == '4' || ITREE(i, j) == '6') continue;
+         ncatg = (ITREE(i, j) == '2' || ITREE(i, j) == '5') ? 2 : 1;
+         if (ncatg != 1 && ITREE(i, j) == '2' && ZBM(i, j) < RC(-50.)) {
+            R dphz, dphi;
+            jump = 1;
+            dhx = -HB(i - 1, j - 1) + HB(i, j - 1) - HB(i - 1, j) + HB(i, j);
+            dhy = -HB(i - 1, j - 1) - HB(i, j - 1) + HB(i - 1, j) + HB(i, j);
+            dph = RC(.25) * (HB(i - 1, j - 1) + HB(i, j - 1) + HB(i - 1, j) + HB(i, j) +
+                             ZB(i - 1, j - 1) + ZB(i, j - 1) + ZB(i - 1, j) + ZB(i, j));
+            dphz = AMIN1(RC(599.), dph);
+            id = (int) AMIN1(RC(599.), dphz);
+            if (id <= 0) id = 1;
+            dphi = id;
+            aintrp = TAB(ai2, iik, id) + slpai[id] * (dphz - dphi);
+            goto L450;
+         }
+         {
+            R hmn = AMIN1(AMIN1(AMIN1(HB(i, j), HB(i - 1, j)), HB(i, j - 1)), HB(i - 1, j - 1));
+            if (hmn <= ZBM(i, j)) {
+               R hmxx = AMAX1(AMAX1(AMAX1(HB(i, j), HB(i - 1, j)), HB(i, j - 1)), HB(i - 1, j - 1));
+               if (hmxx <= ZBM(i, j)) {
+                  UB(i, j) = 0;
+                  VB(i, j) = 0;
+                  goto L500;
+               } else {
+                  R hf1 = AMAX1(HB(i - 1, j - 1), ZBM(i, j)), hf2 = AMAX1(HB(i, j - 1), ZBM(i, j));
+                  R hf3 = AMAX1(HB(i - 1, j), ZBM(i, j)), hf4 = AMAX1(HB(i, j), ZBM(i, j));
+                  jump = 2;
+                  dhx = -hf1 + hf2 - hf3 + hf4;
+                  dhy = -hf1 - hf2 + hf3 + hf4;
+               }
+            } else {
+               jump = 1;
+               dhx = -HB(i - 1, j - 1) + HB(i, j - 1) - HB(i - 1, j) + HB(i, j);
+               dhy = -HB(i - 1, j - 1) - HB(i, j - 1) + HB(i - 1, j) + HB(i, j);
+            }
+         }
+         {
+            R dphz;
+            int k;
+            hpd[1] = HB(i - 1, j - 1) + ZB(i - 1, j - 1);
+            hpd[2] = HB(i, j - 1) + ZB(i, j - 1);
+            hpd[3] = HB(i - 1, j) + ZB(i - 1, j);
+            hpd[4] = HB(i, j) + ZB(i, j);
+            dph = RC(.25) * (hpd[1] + hpd[2] + hpd[3] + hpd[4]);
+            dphz = dph;
+            id = (int) AMIN1(RC(599.), dphz);
+            if (dph <= RC(5.)) {
+               if (jump != 2) {
+                  for (k = 1; k <= 4; k++) {
+                     if (hpd[k] < RC(5.))
+                        hpd[k] = hpd[k] * (RC(.4) + RC(.12) * hpd[k] - RC(.008) * hpd[k] * hpd[k]);
+                     else
+                        hpd[k] = hpd[k] - RC(1.);
+                  }
+                  dph = RC(.25) * (hpd[1] + hpd[2] + hpd[3] + hpd[4]);
+                  dphz = dph;
+                  id = (int) AMIN1(RC(599.), dphz);
+               }
+               if (id < 1) id = 1;
+            }
+            aintrp = 0;
+            for (k = 1; k <= 4; k++) {
+               R dz = hpd[k], dphi, aitr;
+               int idd = (int) AMIN1(RC(599.), dz);
+               if (idd < 1) idd = 1;
+               dphi = idd;
+               aitr = TAB(ai2, iik, idd) + slpai[idd] * (dz - dphi);
+               aintrp = aintrp + aitr;
+            }
+            aintrp = RC(.25) * aintrp;
+         }
+      L450:
+         aii = RC(1.) + s->x1b4 * aintrp;
+         {
+            int llat = (int) YLT(i, j);
+            R zcor = b->corl[llat] + (YLT(i, j) - llat) * (b->corl[llat + 1] - b->corl[llat]);
+            arj = TAB(ar2, iik, id) * s->x1b4 * zcor;
+         }
+         aiu = aii * UB(i, j);
+         aiv = aii * VB(i, j);
+         fct = s->x1b6 * dph;
+         if (jump != 2) {
+            R xr, yr, fxbp, fybp;
+            if (ncatg == 2) {
+               cshltr = 1;
+            } else {
+               R hfx[5];
+               hfx[1] = HB(i - 1, j - 1) - ZBM(i, j);
+               hfx[2] = HB(i, j - 1) - ZBM(i, j);
+               hfx[3] = HB(i - 1, j) - ZBM(i, j);
+               hfx[4] = HB(i, j) - ZBM(i, j);
+               if (ITREE(i, j) != '1') {
+                  R c1 = AMIN1(hfx[1], RC(1.)), c2 = AMIN1(hfx[2], RC(1.));
+                  R c3 = AMIN1(hfx[3], RC(1.)), c4 = AMIN1(hfx[4], RC(1.));
+                  cshltr = RC(.25) * (c1 + c2 + c3 + c4);
+               } else {
+                  int kx;
+                  cshltr = 0;
+                  for (kx = 1; kx <= 4; kx++) {
+                     R hfxx = 1;
+                     if (hfx[kx] <= 0) hfxx = 0;
+                     else if (hfx[kx] < shgt) {
+                        R q = hfx[kx] / shgt;
+                        hfxx = q * q;
+                     }
+                     cshltr = cshltr + hfxx;
+                  }
+                  cshltr = RC(.25) * cshltr;
+               }
+            }
+            xr = ELPCT(i) * COST(j);
+            yr = ELPDT(i) * SINT(j);
+            npls = 1;
+            frcpnt(b, s, xr, yr, &fxbp, &fybp, ncatg, &cshltr, npls, i, j, dph, id);
+            xr = ELPDT(i) * COST(j);
+            yr = ELPCT(i) * SINT(j);
+            fxb = (xr * fxbp + yr * fybp);
+            fyb = (xr * fybp - yr * fxbp);
+            xttx = s->delt * fxb;
+            xtty = s->delt * fyb;
+         } else {
+            xttx = 0;
+            xtty = 0;
+            fxb = 0;
+            fyb = 0;
+         }
+         ubj = UB(i, j);
+         vbj = VB(i, j);
+         UB(i, j) = aiu - fct * (TAB(br2, iik, id) * dhx - TAB(bi2, iik, id) * dhy) +
+                    arj * VB(i, j) + xttx;
+         VB(i, j) = aiv - fct * (TAB(br2, iik, id) * dhy + TAB(bi2, iik, id) * dhx) -
+                    arj * UB(i, j) + xtty;
+         goto L500b;
+      L500:
+         ubj = UB(i, j);     /* not read on this path: the boundary is zeroed */
+         vbj = VB(i, j);
+      L500b:
+         ijump = 1;
+         if (iskip == 3) {
+            if (i == ME(j)) icor = 4;
+            if (i != MS(j)) goto L510;
+            icor = 3;
+            goto L600;
+         } else if (iskip != 2) {
+            if (i == ME(j)) icor = 2;
+            if (i != MS(j)) goto L510;
+            icor = 1;
+            goto L600;
+         }
+         if (i != MS(j)) {
+            if (i != ME(j)) continue;
+            else iskip = 4;
+         }
+      L510:
+         {
+            int isb = i + ISBS[iskip], jsb = j + JSBS[iskip];
+            if (UB(i, j) == 0) {
+               UB(isb, jsb) = 0;
+               VB(isb, jsb) = 0;
+            } else if (ITREE(isb, jsb) != '6' && ITREE(isb, jsb) != '4') {
+               if (ITREE(isb, jsb) == '8') ijump = 2;
+               mntmbd(b, s, st, jump, ijump, isb, jsb, arj, aii, i, j, ubj, vbj, &fxb, &fyb,
+                      ncatg, &cshltr, &npls, dph, id);
+            }
+         }
+         continue;
+      L600:
+         for (kk = 1; kk <= 3; kk++) {
+            int k = ISHF[icor][kk];
+            int isb = i + IH[icor] + IP[k], jsb = j + JH[icor] + JP[k];
+            if (UB(i, j) != 0) {
+               if (ITREE(isb, jsb) == '6' || ITREE(isb, jsb) == '4') break;
+               if (ITREE(isb, jsb) == '8') ijump = 2;
+               mntmbd(b, s, st, jump, ijump, isb, jsb, arj, aii, i, j, ubj, vbj, &fxb, &fyb,
+                      ncatg, &cshltr, &npls, dph, id);
+            } else {
+               UB(isb, jsb) = 0;
+               VB(isb, jsb) = 0;
+            }
+         }
+      }
+   }
+   if (b->dollar[0] == '$' && b->dollar[1] == ' ')
+      for (i = 1; i <= imxb; i++) {
+         UB(i, 1) = UB(i, jmxb);
+         VB(i, 1) = VB(i, jmxb);
+      }
+   (void) hpd;
+}
+
+/* -------------------------------------------------------------------- step */
+int orc_step(const orc_basin *b, const orc_storm *s, orc_state *st, orc_diag *d)
+{
+   int incsm = 20, ismpt2g = 1;
+   char e1 = b->ebsn1;
+   if (d) memset(d, 0, sizeof *d);
+   if (e1 == '+' || e1 == '=') incsm = 1;
+   if (e1 == '!' || e1 == '#') incsm = 5;
+   if (e1 == '@' || e1 == '%') incsm = 10;
+   if (e1 == '&') incsm = 20;
+   if (e1 == '^' || e1 == '*') incsm = 30;
+   if (e1 == '+' || e1 == '&' || e1 == '#' || e1 == '%' || e1 == '*') ismpt2g = 0;
+   /* TMSTPC1 */
+   orc_contty(b, s, st, d);
+   if (ismpt2g != 0 && s->itime > 5 && incsm != 1)
+      if (orc_smpt2g(b, s, st, d)) return -1;
+   orc_bdryht(b, s, st);
+   /* TMSTPC2 */
+   orc_hmxsv(b, s, st);
+   if ((s->itime + 1) % incsm == 0 || e1 == '+') {
+      int isl = (b->dollar[0] == '$' && b->dollar[1] == ' ');
+      if (d) d->filtered = 1;
+      if (isl) return -2;                       /* FLTER2: island basins, not restated */
+      if (b->ismth == 1) orc_filter(b, s, st);
+      else orc_xfilter(b, s, st, strncmp(b->sta, "CH2 (2008)", 10) == 0 ? 1 : 2);
+   }
+   /* TMSTPM */
+   orc_momntm(b, s, st);
+   return 0;
+}

@@ -1,0 +1,71 @@
+/* orc.h -- CPU restatement of one SLOSH time step (the oracle).
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is the checker the GPU path is compared
+ * against; nothing in the product links or calls it.  It is compiled twice:
+ * with REAL=double it must reproduce the reference's default FP64 build bit
+ * for bit (that is how it is pinned), with REAL=float it is the FP32 oracle
+ * the HIP kernels must match.
+ *
+ * Layout: 2-D fields are packed [J-1][I-1] (I fastest, imxb x jmxb); 1-D
+ * vectors by I-1 or J-1; bottom-stress tables [N-1][II-1] with N=1..600,
+ * II=1..245 (the reference's AR2(245,600) column-major order).
+ */
+#ifndef ORC_H
+#define ORC_H
+#ifndef REAL
+#define REAL double
+#endif
+
+typedef struct {
+   int imxb, jmxb;
+   char ebsn, dollar[2], ebsn1;
+   int ismth, iver;
+   REAL hter;
+   char sta[16];
+   const REAL *zb, *zbm, *ylt;
+   const char *itree, *kskp;
+   const int *imann;
+   const int *is, *ie, *ms, *me;              /* by J-1 */
+   const REAL *elpct, *elpdt, *elpcl, *elpdl, *elpcl2, *elpdl2;   /* by I-1 */
+   const REAL *sint, *cost, *sinl, *cosl, *sinl2;                  /* by J-1 */
+   const REAL *ar2, *ai2, *br2, *bi2, *cr2, *ci2;                  /* [600][245] */
+   REAL corl[90], gamfp[4];
+   int nbcpt;
+   const int *ish, *jsh;
+} orc_basin;
+
+/* Storm values the field routines read during one step: everything as it
+ * stands after the previous step's STMVAL, plus this step's TTIMER ramp. */
+typedef struct {
+   int itime;                 /* after TTIMER */
+   REAL bt;
+   REAL delt, x1b4, x1b5, x1b6;
+   REAL c1, c2, ax, ay;
+   REAL x12[51];              /* 1-based */
+   REAL w1218[2];
+   REAL seadtm;
+   const REAL *S, *C, *P, *DELP, *SW, *CW;     /* 1-based, >= 792 entries */
+} orc_storm;
+
+typedef struct {
+   REAL *hb, *ub, *vb, *hmx;  /* imxb x jmxb */
+   REAL *hsub;                /* scratch, imxb x jmxb, persists across steps */
+} orc_state;
+
+typedef struct {
+   int drying_events;         /* CONTTY squares that took the back-step branch */
+   int smooth_pts, smooth_special, smooth_overflow;
+   int filtered;
+} orc_diag;
+
+int orc_step(const orc_basin *b, const orc_storm *s, orc_state *st, orc_diag *d);
+
+/* individual routines, for step-injection tests */
+void orc_contty(const orc_basin *b, const orc_storm *s, orc_state *st, orc_diag *d);
+int orc_smpt2g(const orc_basin *b, const orc_storm *s, orc_state *st, orc_diag *d);
+void orc_bdryht(const orc_basin *b, const orc_storm *s, orc_state *st);
+void orc_hmxsv(const orc_basin *b, const orc_storm *s, orc_state *st);
+void orc_filter(const orc_basin *b, const orc_storm *s, orc_state *st);
+void orc_xfilter(const orc_basin *b, const orc_storm *s, orc_state *st, int iversion);
+void orc_momntm(const orc_basin *b, const orc_storm *s, orc_state *st);
+#endif

@@ -1,0 +1,114 @@
+/* sl_host.h -- host side of the SLOSH time-stepping core: basin and track
+ * ingest, grid metrics, bottom-stress tables, initial water and the per-storm
+ * state machine that advances the parametric storm each step.
+ *
+ * Everything here runs once per basin / once per storm / once per step on
+ * scalars; it is computed in double so that it reproduces the reference's
+ * default (FP64) build bit for bit.  The field arithmetic of the step (the hot
+ * path) runs on the GPU in FP32 from the float-rounded results of this code.
+ *
+ * Index conventions: the reference is Fortran, (I,J) 1-based with I fastest.
+ * Per-point arrays here are packed [J-1][I-1] (imxb x jmxb); per-I vectors are
+ * indexed I-1 and per-J vectors J-1.
+ */
+#ifndef SL_HOST_H
+#define SL_HOST_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SL_NT 999          /* track hours, inputn2.f:10 NT_ */
+#define SL_NBCPTS 12000    /* static-height boundary squares, bdryht.f:59 */
+#define SL_NTAB 800        /* radial tables at 1-mile spacing, force1.f */
+
+typedef struct sl_basin {
+   int imxb, jmxb, imxb1, jmxb1, imxb2, jmxb2;
+   char sta[17];
+   char ebsn, dollar[3], ebsn1, ebsn2, isouth, nofld, nof1d, iterrain, ismooth;
+   int iver, iprj, ismth;
+   double hter, fsouth;
+   double azmth, alto, alno, degree, dela, rmouth, aaxis, baxis, abqab;
+   double xmouth, ymouth, dstnt, asmpt, gsize;
+   double phi, phi1, alt1, alt1c, aln1, cor;
+   int irb, ire, ndb, nde, ndb1;
+   double dltin[3];
+   /* per point, [J-1][I-1] */
+   double *zb, *zbm, *ylt;
+   char *itree, *kskp;
+   int *imann;
+   /* per J */
+   int *is, *ie, *ms, *me, *mf;
+   double *sint, *cost, *sinl, *cosl, *sinl2, *sint2;
+   /* per I */
+   double *elpct, *elpdt, *elpcl, *elpdl, *elpcl2, *elpdl2;
+   /* open-boundary squares (DEPSFC) */
+   int nbcpt;
+   int *ish, *jsh;
+   double *tidesh;
+   /* constants (CNSTNT) */
+   double delt, x1b[10], corl[90], gama, gama1, gamfp[4];
+   /* bottom-stress tables (BTMSTR), [N-1][II-1], N=1..600, II=1..245 */
+   double *ar2, *ai2, *br2, *bi2, *cr2, *ci2;
+   /* sub-grid features: counted, must be zero for this build */
+   int nsqrs, nsqrw, nsqrwc, npss, ncut, nodry;
+   int ipn[10], jpn[10];
+   int have_ylt;
+} sl_basin;
+
+typedef struct sl_storm {
+   /* hourly track after interpolation (1-based, index 0 unused) */
+   int trklen;
+   double X[SL_NT + 1], Y[SL_NT + 1], PT[SL_NT + 1], R[SL_NT + 1];
+   double DIR[SL_NT + 1], SP[SL_NT + 1], XLAT[SL_NT + 1], YLONG[SL_NT + 1];
+   double rlngth, seadtm, dtmlak;
+   int ibgnt, itend, jhr;
+   int date_yr, date_mo, date_dy, date_hr, date_mn;
+   /* time stepping */
+   double delt, dltb, dlthr, dlth2, bt, treal;
+   int mhalt, ndlthr, ndlth2, ic12, ic19, itime, itmadv, nhrad, jumpr, iprt, key1;
+   int iprtsv, iprtad;
+   double x1b[10];
+   /* storm state (STRMSB, WTEMP, FRST, DUMY44, DUMMY4) */
+   double c1, c2, c21, c22, ax, ay, ptency, rtency;
+   double ydelp, zdelp, pnn, yc24, zc24, zlato;
+   double x12[51];                     /* 1-based */
+   double w1218[2], wmax[2];
+   double S[SL_NTAB + 1], C[SL_NTAB + 1], P[SL_NTAB + 1], DELP[SL_NTAB + 1];
+   double SW[SL_NTAB + 1], CW[SL_NTAB + 1];
+   int table_gen;                      /* bumps whenever FORCE1 refills the tables */
+} sl_storm;
+
+/* Basin: parse the dta file and derive everything the step needs.
+ * Returns 0, or -1 with a message in err (e.g. an unsupported feature). */
+int sl_basin_load(const char *dta_path, sl_basin *b, char err[256]);
+/* Cell latitudes for the Coriolis term from the ?basins.dta line
+ * (restates memSetLLx, savellx.c:704-765).  0 on success. */
+int sl_basin_llx(sl_basin *b, const char *bnt_line, char err[256]);
+void sl_basin_free(sl_basin *b);
+
+/* Track: parse the trk file and initialise the storm (INPUTN2, STERON, INTDLT,
+ * INTVAL, SETCMP).  Needs the loaded basin. */
+int sl_storm_load(const char *trk_path, const sl_basin *b, sl_storm *s, char err[256]);
+/* Initial water HB (INTLHT, intlht.f:53-185) into hb[jmxb*imxb]; also zeroes
+ * nothing else -- UB=VB=0 and HMX=HB are the caller's. */
+void sl_initial_water(const sl_basin *b, const sl_storm *s, double *hb);
+
+/* Per-step storm machinery.  Order inside one step, as TMSTPC1/TMSTPM run it:
+ * sl_storm_ttimer() at the start (ITIME+1, ramp BT), then the field kernels,
+ * then sl_storm_stmval() after the momentum update. */
+void sl_storm_ttimer(sl_storm *s);
+int sl_storm_stmval(sl_storm *s, const sl_basin *b);
+/* storm centre lat/lon (XY2LLX, xy2llx.f) for the rex track record */
+void sl_storm_latlon(const sl_storm *s, const sl_basin *b, double *lat, double *lon);
+
+/* FORCE1 and INTWND, exposed for tests. */
+int sl_force1(double pdrop, double rmax, double *vmax, double alto, int ncatg,
+              double S[SL_NTAB + 1], double C[SL_NTAB + 1], double P[SL_NTAB + 1],
+              double DELP[SL_NTAB + 1]);
+double sl_intwnd(double pdrop, double rmax, double alto);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,426 @@
+"""Seeded synthetic SLOSH inputs: basin (dta + bnt line) and track (trk 2013).
+
+No real basin or track ships with the reference (they are downloaded assets), so
+every workload here is generated.  The writers follow the reference readers
+column for column:
+
+* dta (Ver2014-08 layout): header read by ``CRDRD1`` (sorc/slosh.fd/crdrd1.f:102-213),
+  body by ``CRDRD2`` (crdrd2.f:65-470): barriers ``(7(I4,I4,F3.0))``, flow/weir/
+  channel/cut counts, history stations, the ITREE map ``(60A1)`` per J, print
+  sheets and depths in 15-column strips ``(15F5.0)``.
+* bnt line for ``memSetLLx``: ``A3,A1,A18,A9,1X,A10,2X,A9,1X,A10,1X,F5.1,1X,F5.1,
+  1X,F6.4,4F5.1,20X,F6.5`` (savellx.c:392-431); latitudes as DDD.MMSSs (a2lat,
+  savellx.c:332-366).
+* trk, FileVersion=2013 (inputn2.f:110-605).
+
+Depth comes from a seeded smooth Gaussian random field plus a shelf trend, and is
+capped by the explicit scheme's gravity-wave limit on each cell: the largest
+total depth allowed in a cell of size h at step dt is (C*h/dt)^2/g.  Without the
+cap a random 60 m depth under a 0.5 km cell at 60 s breaks the CFL limit and the
+reference itself blows up.
+"""
+from __future__ import annotations
+
+import math
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+from scipy.ndimage import gaussian_filter
+
+FT_PER_M = 3.280839895
+G_FT = 32.2
+MI_PER_KT = 1.150779
+
+
+@dataclass
+class Basin:
+    """A generated basin: grid geometry plus per-point fields (1-based in files)."""
+    name: str            # 3-letter code (the dta/bnt key)
+    gtype: str           # ' ' polar, '$' elliptic, '+' hyperbolic (dta line-1 col 12)
+    imxb: int            # momentum points in I (cells = imxb-1)
+    jmxb: int
+    irb: int
+    ndb: int
+    dela: float          # radians
+    rmouth: float        # miles
+    abqab: float
+    azmth: float
+    alto: float          # tangent point latitude (deg N)
+    alno: float          # tangent point longitude (deg W, positive)
+    dltin: float         # seconds
+    zb: np.ndarray       # (jmxb, imxb) depth ft (positive water), int
+    itree: np.ndarray    # (jmxb, imxb) single chars
+    barriers: list = field(default_factory=list)   # (i, j, h) 1-based, h ft
+    hist: list = field(default_factory=list)       # 10 (i, j) history points
+    extra: dict = field(default_factory=dict)      # dsntn/asmpt/aaxis/baxis/gsize
+
+    @property
+    def abbrev(self) -> str:
+        """Basin argument of the reference CLI: ' tst' polar, 'etst' e, 'htst' h."""
+        return {" ": "", "$": "e", "+": "h"}[self.gtype] + self.name
+
+    @property
+    def cells(self) -> int:
+        return (self.imxb - 1) * (self.jmxb - 1)
+
+
+# --------------------------------------------------------------------------------------
+# grid geometry (the metric SETDST builds, sorc/slosh.fd/setdst.f:47-87)
+
+def _cell_size_ft(b_imxb, b_jmxb, irb, ndb, dela, rmouth, abqab):
+    """Cell size (ft) at height points: DELA*|dz/dw| of z = R(e^w + A e^-w)."""
+    i = np.arange(1, b_imxb + 1) - 1 - irb + 0.5
+    j = np.arange(1, b_jmxb + 1) - 1 - ndb + 0.5
+    s = i[None, :] * dela
+    t = j[:, None] * dela
+    w = s + 1j * t
+    dz = rmouth * np.abs(np.exp(w) - abqab * np.exp(-w))
+    return dz * dela * 5280.0
+
+
+def _grf(rng, shape, corr):
+    f = gaussian_filter(rng.standard_normal(shape), sigma=corr / 2.0, mode="reflect")
+    return (f - f.mean()) / (f.std() + 1e-12)
+
+
+def make_basin(name="tst", gtype=" ", ni=100, nj=90, seed=1, land_frac=0.20,
+               n_barriers=50, dt=60.0, size_km=(0.5, 3.0), alto=29.5, alno=89.5,
+               depth_m=60.0, land_m=5.0, corr=10.0, cfl=0.45) -> Basin:
+    """Generate a basin of ni x nj cells (height points), as in the workloads:
+    polar with radial cell size growing geometrically from size_km[0] to size_km[1],
+    elliptic ('$') or hyperbolic ('+') with the same depth/land generator."""
+    rng = np.random.default_rng(seed)
+    imxb, jmxb = ni + 1, nj + 1
+    extra = {}
+    if gtype == " ":
+        dela = math.log(size_km[1] / size_km[0]) / ni
+        irb = ni // 2
+        ndb = nj // 2
+        r_inner_mi = size_km[0] / 1.609344 / dela          # radius of ring I=1
+        rmouth = r_inner_mi * math.exp(irb * dela)          # radius of ring IRB+1
+        abqab = 0.0
+    elif gtype == "$":
+        # elliptic I: BAXIS = DSTNT/cos(ASMPT), AAXIS = 0 (crdrd1.f:175-187)
+        ndb = 0
+        asmpt = 90.0 - 90.0 * 1.0                            # rays span 0..90 deg
+        asmpt = 5.0
+        dela = math.radians((90.0 - asmpt) / nj)
+        irb = -int(round(0.25 / dela))                      # keep off the focal line
+        dstnt = 40.0
+        baxis = dstnt / math.cos(math.radians(asmpt))
+        rmouth = 0.5 * baxis
+        abqab = -1.0
+        extra = dict(dstnt=dstnt, asmpt=asmpt)
+    elif gtype == "+":
+        # hyperbolic (elliptic II): DELA = 2 GSIZE/(AAXIS+BAXIS) (crdrd1.f:188-195)
+        ndb = nj // 2
+        aaxis, baxis = 60.0, 40.0
+        gsize = 0.6
+        dela = 2.0 * gsize / (aaxis + baxis)
+        irb = -int(round(0.1 / dela))
+        rmouth = 0.5 * (aaxis + baxis)
+        abqab = (aaxis - baxis) / (aaxis + baxis)
+        extra = dict(aaxis=aaxis, baxis=baxis, gsize=gsize)
+    else:
+        raise ValueError(gtype)
+
+    # ---- depth / land field on height points (jmxb-1 x imxb-1) ----
+    hsize = _cell_size_ft(imxb - 1, jmxb - 1, irb, ndb, dela, rmouth, abqab)
+    f = _grf(rng, (nj, ni), corr)
+    trend = np.linspace(-1.6, 1.6, ni)[None, :]             # land toward small I
+    z = 0.55 * f + trend
+    thr = np.quantile(z, land_frac)
+    water = z > thr
+    zw = (z - thr) / (z.max() - thr + 1e-9)                  # 0..1 in water
+    zl = (thr - z) / (thr - z.min() + 1e-9)                  # 0..1 on land
+    depth_ft = np.where(water, 1.0 + zw * (depth_m * FT_PER_M - 1.0), -zl * land_m * FT_PER_M)
+    hmax = (cfl * hsize / dt) ** 2 / G_FT                    # gravity-wave limit, total depth
+    depth_ft = np.where(water, np.minimum(depth_ft, np.maximum(hmax - 20.0, 1.0)), depth_ft)
+    zb = np.zeros((jmxb, imxb))
+    zb[:nj, :ni] = np.round(depth_ft)
+    zb[nj, :] = zb[nj - 1, :]
+    zb[:, ni] = zb[:, ni - 1]
+    zb = zb.astype(int)
+
+    # ---- ITREE map on momentum points ----
+    itree = np.full((jmxb, imxb), "3", dtype="<U1")
+    tree_f = _grf(rng, (jmxb, imxb), corr)
+    zpad = np.pad(zb[:nj, :ni], 1, mode="edge")              # squares around momentum pts
+    sq = np.stack([zpad[:-1, :-1], zpad[:-1, 1:], zpad[1:, :-1], zpad[1:, 1:]])
+    mind, maxd = sq.min(0), sq.max(0)
+    itree[(mind > 10)] = "2"                                 # open water: ocean winds
+    itree[(maxd <= 0) & (tree_f > 0.9)] = "1"                # wooded land
+    # boundaries: open water -> static height ('6'), land -> closed ('4'); very
+    # shallow water next to land gets the shallow-water condition ('9') that
+    # MNTMBD evaluates (mntmbd.f:69-101).
+    def _bnd(jj, ii):
+        if maxd[jj, ii] <= 0:
+            return "4"
+        if mind[jj, ii] <= 0 and maxd[jj, ii] < 8:
+            return "9"
+        return "6"
+    for jj in range(jmxb):
+        itree[jj, imxb - 1] = _bnd(jj, imxb - 1)
+        itree[jj, 0] = "4"
+    for jj in (0, jmxb - 1):
+        for ii in range(1, imxb - 1):
+            itree[jj, ii] = _bnd(jj, ii)
+
+    # ---- barriers: coastal momentum points, crest 2..4 m ----
+    coast = [(i, j) for j in range(2, jmxb - 1) for i in range(2, imxb - 1)
+             if mind[j - 1, i - 1] <= 0 < maxd[j - 1, i - 1] and itree[j - 1, i - 1] not in "46"]
+    rng.shuffle(coast)
+    barriers = []
+    for (i, j) in coast[:n_barriers]:
+        h = int(round(rng.uniform(2.0, 4.0) * FT_PER_M))
+        barriers.append((i, j, h))
+
+    hist = []
+    wet = np.argwhere(zb[:nj, :ni] > 0)
+    pick = rng.choice(len(wet), size=10, replace=False)
+    for k in pick:
+        hist.append((int(wet[k][1]) + 1, int(wet[k][0]) + 1))
+
+    return Basin(name=name, gtype=gtype, imxb=imxb, jmxb=jmxb, irb=irb, ndb=ndb,
+                 dela=dela, rmouth=rmouth, abqab=abqab, azmth=180.0, alto=alto,
+                 alno=alno, dltin=dt, zb=zb, itree=itree, barriers=barriers,
+                 hist=hist, extra=extra)
+
+
+# --------------------------------------------------------------------------------------
+# writers
+
+def _f10(*vals):
+    return "".join("%10.6f" % v for v in vals)
+
+
+def dta_text(b: Basin) -> str:
+    """The basin as a Ver2014-08 dta file (crdrd1.f:102-213, crdrd2.f:65-470)."""
+    L = []
+    L.append("Ver2014-08")
+    L.append("Proj=Clarke")
+    nm = ("SYN " + b.name.upper())[:10].ljust(10)
+    # col 12 grid type, 13-14 island flag, 15 smoothing, 16 order, 17 hemi, 18-21 opts
+    L.append(nm + " " + b.gtype + "  " + " " * 7)
+    L.append(_f10(b.azmth))
+    if b.gtype in "$+":
+        L.append(_f10(b.alto, b.alno))                       # TLAT TLONG (unused)
+    L.append(_f10(b.alto, b.alno))
+    if b.gtype == " ":
+        L.append(_f10(math.degrees(b.dela)))
+    if b.gtype == "$":
+        L.append(_f10(b.extra["dstnt"], b.extra["asmpt"]))
+    if b.gtype == "+":
+        L.append(_f10(b.extra["aaxis"], b.extra["baxis"], b.extra["gsize"]))
+    ire = b.imxb - 1 - b.irb
+    nde = b.jmxb - 1 - b.ndb
+    L.append("%5d%5d%5d%5d%5d" % (b.irb, ire, b.ndb, nde, 0))
+    L.append(_f10(b.dltin, b.dltin, b.dltin))
+    if b.gtype != "+":
+        L.append(_f10(b.rmouth, 0.0))                        # XMOUTH YMOUTH
+    # barriers (crdrd2.f:78-100): count with the letter code for thousands
+    n = len(b.barriers)
+    L.append("%3d  " % n if n < 1000 else "%3d%s " % (n % 1000, "XYZWVUT"[n // 1000 - 1]))
+    for k in range(0, n, 7):
+        L.append("".join("%4d%4d%3d" % bb for bb in b.barriers[k:k + 7]))
+    L.append("  0    0")                                     # 1-D flow points (none)
+    L.append("  0  ")                                        # weirs
+    L.append("".join(("STA%02d" % (k + 1)).ljust(10) for k in range(6)))
+    L.append("".join(("STA%02d" % (k + 7)).ljust(10) for k in range(4)))
+    L.append("".join("%3d" % h[0] for h in b.hist))
+    L.append("".join("%3d" % h[1] for h in b.hist))
+    L.append("  0")                                          # channels
+    L.append("  0  ")                                        # cuts
+    L.append("  1")                                          # MGPT: tree map follows
+    for j in range(b.jmxb):
+        row = "".join(b.itree[j, :])
+        for k in range(0, len(row), 60):
+            L.append(row[k:k + 60])
+    L.append("  1 X")                                        # print sheets
+    L.append("%3d" % 1)
+    L.append("%3d" % min(b.jmxb - 1, 999))
+    for k in range(0, b.imxb, 15):
+        for j in range(b.jmxb):
+            vals = b.zb[j, k:min(k + 15, b.imxb)]
+            L.append("".join("%5d" % v for v in vals))
+    return "\n".join(L) + "\n"
+
+
+def _dms(x):
+    """Decimal degrees -> the DDD.MMSSs text a2lat parses (savellx.c:332-366)."""
+    s = 1 if x >= 0 else -1
+    x = abs(x)
+    d = int(x)
+    m = int((x - d) * 60)
+    sec10 = int(round(((x - d) * 60 - m) * 600))
+    if sec10 >= 600:
+        sec10 -= 600
+        m += 1
+    if m >= 60:
+        m -= 60
+        d += 1
+    return "%.5f" % (s * (d + m / 100.0 + sec10 / 100000.0))
+
+
+def bnt_line(b: Basin) -> str:
+    """One ?basins.dta line for memSetLLx (savellx.c:396-431, 704-765).
+
+    G is the tangent point; P is placed so the projection matches the dta grid:
+    polar P is the pole, XMOUTH miles toward local north (AZMTH=180 puts +x south)."""
+    lat0 = b.alto
+    if b.gtype == " ":
+        dist = b.rmouth
+        typ = " "
+        xig = b.irb + 1.0
+        yjg = b.ndb + 1.0
+        gsize = 2.0 * dist * math.sinh(b.dela / 2.0)
+        elip = 0.0
+    elif b.gtype == "$":
+        dist = b.extra["dstnt"]
+        typ = "e"
+        xig = b.irb + 1.0
+        yjg = b.ndb + 1.0 + (90.0 - b.extra["asmpt"]) / math.degrees(b.dela)
+        gsize = b.extra["asmpt"]
+        elip = 0.0
+    else:
+        a, bb = b.extra["aaxis"], b.extra["baxis"]
+        elip = bb / a
+        dist = b.rmouth * (1.0 + b.abqab)
+        typ = "h"
+        xig = b.irb + 1.0
+        yjg = b.ndb + 1.0
+        gsize = b.dela * b.rmouth
+    plat = lat0 + dist / 69.05
+    plon = b.alno
+    s = b.name.upper()[:3].ljust(3) + typ + ("SYNTHETIC " + b.name.upper()).ljust(18)[:18]
+    s += _dms(lat0).rjust(9)[:9] + " " + _dms(b.alno).rjust(10)[:10] + "  "
+    s += _dms(plat).rjust(9)[:9] + " " + _dms(plon).rjust(10)[:10] + " "
+    s += "%5.1f %5.1f %6.4f" % (xig, yjg, gsize)
+    s += "%5.1f%5.1f%5.1f%5.1f" % (1.0, float(b.imxb), 1.0, float(b.jmxb))
+    s += " " * 20 + "%6.5f" % elip
+    return s
+
+
+@dataclass
+class Track:
+    hours: list
+    lat: list
+    lon: list          # degrees east (negative west), as the trk file holds it
+    dp: list
+    rmw: list
+    begin: int
+    end: int
+    landfall: int
+    interp: str = "Linear"
+
+
+def straight_track(b: Basin, speed_kt=12.0, heading=330.0, dp=60.0, rmw=25.0,
+                   hours=48, landfall=24, offset_mi=0.0) -> Track:
+    """Straight track crossing the tangent point at `landfall` (+ lateral offset)."""
+    v = speed_kt * MI_PER_KT
+    th = math.radians(heading)
+    lat0 = b.alto + offset_mi * math.cos(th + math.pi / 2) / 69.05
+    lon0 = -b.alno + offset_mi * math.sin(th + math.pi / 2) / (69.05 * math.cos(math.radians(b.alto)))
+    H, la, lo = [], [], []
+    for h in range(0, hours + 3):
+        d = v * (h - landfall)
+        la.append(lat0 + d * math.cos(th) / 69.05)
+        lo.append(lon0 + d * math.sin(th) / (69.05 * math.cos(math.radians(b.alto))))
+        H.append(h)
+    n = len(H)
+    return Track(H, la, lo, [dp] * n, [rmw] * n, 0, hours, landfall)
+
+
+def spline_track(b: Basin, seed=2, hours=72, dp=(40.0, 80.0), rmw=(15.0, 30.0),
+                 speeds=(8.0, 20.0)) -> Track:
+    """Curved track: 4 seeded waypoints, hourly values by the reference's cubic
+    spline (trk InterpolationType=Spline, inputn2.f:513-566)."""
+    rng = np.random.default_rng(seed)
+    step_h = 26
+    H = [0, step_h, 2 * step_h, 3 * step_h]
+    hd = math.radians(330.0 + rng.uniform(-25, 25))
+    la, lo = [], []
+    spd = np.linspace(speeds[0], speeds[1], 3) * MI_PER_KT
+    # landfall near hour 36: waypoint 1 before the coast, waypoint 2 after
+    pts = [(0.0, 0.0)]
+    for k in range(3):
+        turn = rng.uniform(-0.35, 0.35)
+        hd += turn
+        d = spd[k] * step_h
+        x, y = pts[-1]
+        pts.append((x + d * math.sin(hd), y + d * math.cos(hd)))
+    xs = np.array([p[0] for p in pts])
+    ys = np.array([p[1] for p in pts])
+    # shift so the track passes the tangent point near hour 36
+    f = 36.0 / step_h
+    k = int(f)
+    cx = xs[k] + (f - k) * (xs[k + 1] - xs[k])
+    cy = ys[k] + (f - k) * (ys[k + 1] - ys[k])
+    xs -= cx
+    ys -= cy
+    for x, y in zip(xs, ys):
+        la.append(b.alto + y / 69.05)
+        lo.append(-b.alno + x / (69.05 * math.cos(math.radians(b.alto))))
+    dps = list(np.linspace(dp[0], dp[1], 4))
+    rms = list(np.linspace(rmw[0], rmw[1], 4))
+    return Track(H, la, lo, dps, rms, 0, hours, 36, interp="Spline")
+
+
+def trk_text(t: Track) -> str:
+    L = ["FileVersion=2013",
+         "# synthetic track (seeded)",
+         "DateTime=2025-09-01T00:00:00",
+         "DateHour=%d" % t.hours[0],
+         "LandfallHour=%d" % t.landfall,
+         "BeginHour=%d" % t.begin,
+         "EndHour=%d" % t.end,
+         "OceanDatum=0.0",
+         "LakeDatum=0.0",
+         "InterpolationType=%s" % t.interp,
+         "PressureType=0",
+         "Hour,Lat,Lon,DeltaP,RMW"]
+    for h, la, lo, d, r in zip(t.hours, t.lat, t.lon, t.dp, t.rmw):
+        L.append("%d,%.4f,%.4f,%.2f,%.2f" % (h, la, lo, d, r))
+    L.append("999,0,0,0,0")
+    return "\n".join(L) + "\n"
+
+
+def write_case(root: str, b: Basin, t: Track, trk_name="storm.trk"):
+    """Write rootDir layout the reference CLI expects (usrparse.c:421-436):
+    <root>/dta/<bsn>dta, <root>/bnt/[e|h]basins.dta, and the trk file."""
+    os.makedirs(os.path.join(root, "dta"), exist_ok=True)
+    os.makedirs(os.path.join(root, "bnt"), exist_ok=True)
+    dta = os.path.join(root, "dta", b.name.lower() + "dta")
+    with open(dta, "w") as fh:
+        fh.write(dta_text(b))
+    pre = {" ": "", "$": "e", "+": "h"}[b.gtype]
+    bnt = os.path.join(root, "bnt", pre + "basins.dta")
+    with open(bnt, "w") as fh:
+        fh.write(bnt_line(b) + "\n")
+    trk = os.path.join(root, trk_name)
+    with open(trk, "w") as fh:
+        fh.write(trk_text(t))
+    return dict(dta=dta, bnt=bnt, trk=trk, bntline=bnt_line(b))
+
+
+# --------------------------------------------------------------------------------------
+# the named workloads
+
+def config1(seed=1):
+    """Polar 100 x 90, 0.5 -> 3 km, ~20 % land, 50 barriers, 48 h at 60 s."""
+    b = make_basin("tst", " ", 100, 90, seed=seed, land_frac=0.20, n_barriers=50, dt=60.0)
+    return b, straight_track(b, 12.0, 330.0, 60.0, 25.0, hours=48, landfall=24)
+
+
+def config2(seed=11):
+    """Elliptic 250 x 300, ~35 % land, 400 barriers, spline track, 72 h at 30 s."""
+    b = make_basin("ell", "$", 250, 300, seed=seed, land_frac=0.35, n_barriers=400, dt=30.0)
+    return b, spline_track(b, seed=seed + 1)
+
+
+def tiny(seed=5, gtype=" ", ni=24, nj=20, dt=60.0, hours=6):
+    """Small basin for per-step parity tests (oracle finishes in milliseconds)."""
+    b = make_basin("tny", gtype, ni, nj, seed=seed, land_frac=0.25,
+                   n_barriers=max(4, ni * nj // 60), dt=dt, size_km=(1.0, 3.0), corr=4.0)
+    return b, straight_track(b, 12.0, 330.0, 60.0, 15.0, hours=hours, landfall=hours // 2)
