@@ -1,0 +1,73 @@
+/* slosh_hip.h -- C ABI of the MI355X (gfx950) SLOSH time-stepping core.
+ *
+ * The reference (NOAA-MDL/slosh-test v4.23) runs one storm per process: its C
+ * driver calls the Fortran step routines TMSTPC1 / TMSTPC2 / TMSTPM once per
+ * time step (sorc/slosh.fd/slosh2.c:1400, 1540, 1542) on state held in Fortran
+ * COMMON blocks.  This library replaces that path.  Its handle API owns the
+ * basin statics and any number of storm runs on one basin, resident in HBM and
+ * stepped together in FP32 by hand-written HIP kernels.
+ *
+ * Plain pointers and sizes only; no torch or HIP types cross this boundary.
+ * Functions return 0 on success or a negative SLG_E* code, with a message in
+ * err[errlen] where an err argument is present.
+ */
+#ifndef SLOSH_HIP_H
+#define SLOSH_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SLG_EINPUT (-1)  /* bad or unreadable input file / argument */
+#define SLG_EUNSUP (-2)  /* basin or track feature this build does not implement */
+#define SLG_EHIP (-3)    /* HIP runtime error (device missing, out of memory, ...) */
+#define SLG_ELIMIT (-4)  /* a reference hard limit was reached (the reference STOPs) */
+
+typedef struct slg_basin slg_basin;
+typedef struct slg_batch slg_batch;
+
+/* Load a basin: the dta file (read as CRDRD1/CRDRD2 do, crdrd1.f:102-213,
+ * crdrd2.f:65-489) and its ?basins.dta grid line (memSetLLx, savellx.c:704-765),
+ * then derive masks, loop bounds, metrics, bottom-stress tables and constants
+ * (INITLZ, initlz.f:61-84).  Replaces the basin half of INITAL (inital.f:1-277).
+ * bnt_line may be NULL if slg_basin_set_ylt supplies the cell latitudes. */
+int slg_basin_open(const char *dta_path, const char *bnt_line, slg_basin **out, char *err,
+                   int errlen);
+/* Cell latitudes YLT (double, [jmxb][imxb], I fastest) from the caller, as the
+ * reference's C side fills COMMON /LLXFLE/ before INITAL (savellx.c:757-762). */
+int slg_basin_set_ylt(slg_basin *b, const double *ylt);
+int slg_basin_dims(const slg_basin *b, int *imxb, int *jmxb, int *nbcpt);
+void slg_basin_close(slg_basin *b);
+
+/* Open nstorm runs on one basin, one trk file each (INPUTN2, STERON, INTVAL,
+ * SETCMP, INTLHT: inputn2.f, steron.f, intval.f, setcmp.f, intlht.f).  The
+ * storm schedule of every run is computed on the host here. */
+int slg_batch_open(slg_basin *b, int nstorm, const char *const *trk_paths, slg_batch **out,
+                   char *err, int errlen);
+/* Steps of one run (MHALT after any time-step change, intval.f:68). */
+int slg_batch_nsteps(const slg_batch *x, int storm);
+/* Allocate device state, upload statics, schedules and initial water. */
+int slg_batch_upload(slg_batch *x, char *err, int errlen);
+/* Advance every run by n time steps (asynchronous; one step = the reference's
+ * TMSTPC1 + TMSTPC2 + TMSTPM, slosh2.c:1400-1545). */
+int slg_batch_step(slg_batch *x, int n, char *err, int errlen);
+int slg_batch_sync(slg_batch *x, char *err, int errlen);
+int slg_batch_current_step(const slg_batch *x);
+/* Copy one run's field to the host as double [jmxb][imxb]: "HB", "UB", "VB" or
+ * "HMX" (COMMON /DUMB7/ and /DUMB10/, parm.for:39-45). */
+int slg_batch_get(slg_batch *x, int storm, const char *field, double *dst, char *err, int errlen);
+/* Per-run counters since upload: [0] corner-smoothing momentum points, [1] dry
+ * corners, [2] smoothing list 1, [3] smoothing list 2, [4] steps that replayed
+ * a drying cascade. */
+int slg_batch_stats(slg_batch *x, int storm, int *out8, char *err, int errlen);
+void slg_batch_close(slg_batch *x);
+
+/* Write the host initialisation of one run, and its storm schedule, in the
+ * tagged-record layout used for the reference's state dumps (for checking). */
+int slg_export_init(const slg_batch *x, int storm, const char *path);
+int slg_export_storm(const slg_batch *x, int storm, const char *rows_path, const char *tables_path);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -19,6 +19,9 @@
 
 typedef REAL R;
 #define RC(x) ((R) (x))
+/* square root rounded to REAL before it enters further arithmetic: sqrt of a
+ * float computed in double and rounded once is the correctly rounded float */
+#define SQRT(x) ((R) sqrt((double) (x)))
 #define F2(a, i, j) (a)[(size_t) ((j) - 1) * imxb + ((i) - 1)]
 #define HB(i, j) F2(st->hb, i, j)
 #define UB(i, j) F2(st->ub, i, j)
@@ -341,7 +344,7 @@ void orc_bdryht(const orc_basin *b, const orc_storm *s, orc_state *st)
       x = xr - s->c1 - s->ax;
       y = yr - s->c2 - s->ay;
       rsq = x * x + y * y;
-      r1 = sqrt(rsq) / RC(5280.) + RC(1.);
+      r1 = SQRT(rsq) / RC(5280.) + RC(1.);
       k = (int) r1;
       r2 = k;
       dr = r1 - r2;
@@ -615,7 +618,7 @@ static void frcpnt(const orc_basin *b, const orc_storm *s, R x, R y, R *fxb, R *
                    R *cshltr, int npls, int i, int j, R dph, int id)
 {
    R xp = x - s->c1 - s->ax, yp = y - s->c2 - s->ay;
-   R rsq = xp * xp + yp * yp, rs = sqrt(rsq), r1 = rs / RC(5280.) + RC(1.), r2, dr;
+   R rsq = xp * xp + yp * yp, rs = SQRT(rsq), r1 = rs / RC(5280.) + RC(1.), r2, dr;
    R ccn, x1218, ck1, sk1, a, bb, fctt, fxx, fyy, aa, pxx, pyy;
    int k = (int) r1, iik = iik_of(b, i, j);
    r2 = k;
@@ -641,7 +644,7 @@ static void frcpnt(const orc_basin *b, const orc_storm *s, R x, R y, R *fxb, R *
    }
    a = a * s->x12[4] / ccn;
    bb = bb * s->x12[4] / ccn;
-   fctt = RC(3.E-6) * sqrt(a * a + bb * bb);
+   fctt = RC(3.E-6) * SQRT(a * a + bb * bb);
    fctt = fctt * *cshltr * s->bt;
    fxx = fctt * a;
    fyy = fctt * bb;

@@ -336,6 +336,12 @@ static int read_body(frec *r, sl_basin *b, char err[256])
    /* trees (60A1 per J, format reversion every 60 characters) */
    fr_next(r);
    mgpt = fr_int(r, 3);
+   if (mgpt == 0) {
+      /* without the tree map the I=1 / IMXB barrier ring is not raised and
+       * MOMNTM would start at I=1 (momntm.f:132-140 reads HB(I-1,.)) */
+      snprintf(err, 256, "basins without a tree map (MGPT=0) are not supported by this build");
+      return -1;
+   }
    if (mgpt != 0) {
       for (j = 1; j <= jmxb; j++) {
          fr_next(r);

@@ -1,0 +1,762 @@
+// sl_capi.cpp -- C ABI of the SLOSH time-stepping core on MI355X.
+//
+// A basin is loaded once (host restatement of the reference's basin init, then
+// float static fields uploaded to HBM); a batch holds one or more storm runs on
+// that basin with their fields resident on the device.  Each storm's forcing
+// depends only on its track, so the whole per-step storm schedule (the values
+// STMVAL leaves for the next step, and every radial table FORCE1 produces) is
+// computed on the host up front and uploaded once; stepping then needs no host
+// round trip.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <vector>
+#include <string>
+#include "sl_host.h"
+#include "sl_dev.h"
+#include "../../include/slosh_hip.h"
+
+extern "C" int sl_dev_launch_step(const SlBasinDev *B, const SlRunDev *R, int step, int smpt,
+                                  int filt, int parity, hipStream_t st);
+
+#define PIX(b, i, j) ((size_t) ((j) - 1) * (size_t) (b)->imxb + (size_t) ((i) - 1))
+
+static void seterr(char *err, int len, const char *fmt, const char *a = "", int v = 0)
+{
+   if (err && len > 0) snprintf(err, (size_t) len, fmt, a, v);
+}
+
+#define HIPOK(x)                                                                           \
+   do {                                                                                    \
+      hipError_t e_ = (x);                                                                 \
+      if (e_ != hipSuccess) {                                                              \
+         seterr(err, errlen, "HIP error %s (%d)", hipGetErrorString(e_), (int) e_);         \
+         return SLG_EHIP;                                                                  \
+      }                                                                                    \
+   } while (0)
+
+struct slg_basin {
+   sl_basin h;
+   char bnt[512];
+   int pitch, jend;
+   // host float images of the device static data
+   std::vector<float> zb, zbm, zcor, vi[6], vj[5], tab, slpai;
+   std::vector<uint32_t> flags;
+   std::vector<int> is, ie, ms, me;
+   int incsm, ismpt2g;
+   // device
+   bool on_dev;
+   void *dmem;
+   SlBasinDev dev;
+};
+
+struct storm_sched {
+   sl_storm s0;                        // storm state after initialisation
+   std::vector<double> rows;           // [nsteps+1][90] ref-format rows
+   std::vector<float> tables;          // [ntab][6][SL_NTABV]
+   std::vector<int> tab_step;          // step after which each table set took effect
+   int nsteps;
+};
+
+struct slg_batch {
+   slg_basin *b;
+   int nstorm, maxsteps, step, parity;
+   std::vector<storm_sched> sc;
+   std::vector<double> hb0;            // initial HB per storm (double, imxb*jmxb)
+   void *dmem;
+   SlRunDev dev;
+   hipStream_t st;
+   size_t bytes;
+};
+
+// --------------------------------------------------------------------------
+// basin
+
+static uint32_t flag_bits(const slg_basin *B, int i, int j)
+{
+   const sl_basin *b = &B->h;
+   uint32_t f = 0;
+   char it = b->itree[PIX(b, i, j)];
+   if (i <= b->imxb1 && j <= b->jmxb1 && b->kskp[PIX(b, i, j)] == '1') f |= F_ACT;
+   if (it == '1') f |= F_TREE1;
+   if (it == '2' || it == '5') f |= F_OCEAN;
+   if (it == '2') f |= F_IT2;
+   if (it == '6' || it == '4') f |= F_BSKIP;
+   if (it == '8') f |= F_B8;
+   if (i >= 2 && i <= b->imxb1 && j >= 2 && j <= b->jmxb1) f |= F_ZR;
+   // IIK of FRCPNT/MOMNTM (frcpnt.f:75-118)
+   {
+      int iik;
+      if (b->zb[PIX(b, i, j)] >= 0) iik = (b->iver == 201903) ? 4 : 1;
+      else if (b->iver == 201903) {
+         int m = b->imann[PIX(b, i, j)];
+         iik = (m >= 100) ? ((m - 5 > 245) ? 245 : m - 5) : 95;
+      } else
+         iik = !strncmp(b->sta, "SOUTH FL", 8) ? 245 : 1;
+      f |= (uint32_t) iik << 8;
+   }
+   // MOMNTM interior points and their boundary duty (momntm.f:96-130, 384-458)
+   if (j >= 2 && j <= B->jend && i >= b->ms[j - 1] && i <= b->me[j - 1] && it != '4' && it != '6') {
+      int iskip = 2, bop = 0;
+      f |= F_MOM;
+      if (j == 2) iskip = 1;
+      if (j == B->jend) iskip = 3;
+      if (iskip == 3) bop = (i != b->ms[j - 1]) ? 3 : 6;
+      else if (iskip == 1) bop = (i != b->ms[j - 1]) ? 1 : 5;
+      else if (i == b->ms[j - 1]) bop = 2;
+      else if (i == b->me[j - 1]) bop = 4;
+      f |= (uint32_t) bop << 16;
+   }
+   return f;
+}
+
+static void basin_images(slg_basin *B)
+{
+   sl_basin *b = &B->h;
+   const int P = B->pitch, NJ = b->jmxb;
+   size_t np = (size_t) P * NJ;
+   B->zb.assign(np, 0.f);
+   B->zbm.assign(np, 0.f);
+   B->zcor.assign(np, 0.f);
+   B->flags.assign(np, 0);
+   float corl[90];
+   for (int l = 0; l < 90; l++) corl[l] = (float) b->corl[l];
+   for (int j = 1; j <= NJ; j++)
+      for (int i = 1; i <= b->imxb; i++) {
+         size_t q = (size_t) (j - 1) * P + (i - 1);
+         B->zb[q] = (float) b->zb[PIX(b, i, j)];
+         B->zbm[q] = (float) b->zbm[PIX(b, i, j)];
+         B->flags[q] = flag_bits(B, i, j);
+         // Coriolis factor at the point, FP32 as momntm.f:294-295 evaluates it
+         float ylt = (float) b->ylt[PIX(b, i, j)];
+         int llat = (int) ylt;
+         if (llat >= 0 && llat <= 88)
+            B->zcor[q] = corl[llat] + (ylt - (float) llat) * (corl[llat + 1] - corl[llat]);
+      }
+   // BDRYHT squares (the ISH/JSH list DEPSFC builds, depsfc.f:376-390)
+   for (int n = 0; n < b->nbcpt; n++)
+      B->flags[(size_t) (b->jsh[n] - 1) * P + (b->ish[n] - 1)] |= F_BDRY;
+   double *vsrc[6] = {b->elpct, b->elpdt, b->elpcl, b->elpdl, b->elpcl2, b->elpdl2};
+   for (int k = 0; k < 6; k++) {
+      B->vi[k].assign(P, 0.f);
+      for (int i = 0; i < b->imxb; i++) B->vi[k][i] = (float) vsrc[k][i];
+   }
+   double *jsrc[5] = {b->sint, b->cost, b->sinl, b->cosl, b->sinl2};
+   for (int k = 0; k < 5; k++) {
+      B->vj[k].assign(NJ, 0.f);
+      for (int j = 0; j < NJ; j++) B->vj[k][j] = (float) jsrc[k][j];
+   }
+   B->is.assign(b->is, b->is + NJ);
+   B->ie.assign(b->ie, b->ie + NJ);
+   B->ms.assign(b->ms, b->ms + NJ);
+   B->me.assign(b->me, b->me + NJ);
+   double *tsrc[6] = {b->ar2, b->ai2, b->br2, b->bi2, b->cr2, b->ci2};
+   B->tab.assign((size_t) 6 * 245 * 600, 0.f);
+   for (int t = 0; t < 6; t++)
+      for (int ii = 1; ii <= 245; ii++)
+         for (int n = 1; n <= 600; n++)
+            B->tab[(size_t) t * 147000 + (size_t) (ii - 1) * 600 + (n - 1)] =
+                (float) tsrc[t][(size_t) (n - 1) * 245 + (ii - 1)];
+   // SLPAI of momntm.f:81-89, in FP32 from the FP32 AI2 column
+   B->slpai.assign(601, 0.f);
+   int col = (b->iver == 201903) ? 4 : 1;
+   for (int i = 1; i <= 599; i++)
+      B->slpai[i] = B->tab[(size_t) T_AI2_OFF + (size_t) (col - 1) * 600 + i] -
+                    B->tab[(size_t) T_AI2_OFF + (size_t) (col - 1) * 600 + (i - 1)];
+   B->slpai[600] = B->slpai[599];
+}
+
+static int basin_upload(slg_basin *B, char *err, int errlen)
+{
+   sl_basin *b = &B->h;
+   const int P = B->pitch, NJ = b->jmxb;
+   size_t np = (size_t) P * NJ;
+   size_t off[32], tot = 0;
+   int n = 0;
+   auto add = [&](size_t bytes) {
+      off[n++] = tot;
+      tot += (bytes + 255) & ~(size_t) 255;
+   };
+   add(np * 4);   // zb
+   add(np * 4);   // zbm
+   add(np * 4);   // zcor
+   add(np * 4);   // flags
+   for (int k = 0; k < 4; k++) add((size_t) NJ * 4);   // is ie ms me
+   for (int k = 0; k < 6; k++) add((size_t) P * 4);
+   for (int k = 0; k < 5; k++) add((size_t) NJ * 4);
+   add(B->tab.size() * 4);
+   add(601 * 4);
+   char *d = nullptr;
+   HIPOK(hipMalloc((void **) &d, tot));
+   B->dmem = d;
+   int m = 0;
+   auto up = [&](const void *src, size_t bytes) -> char * {
+      char *dst = d + off[m++];
+      hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
+      return dst;
+   };
+   SlBasinDev &D = B->dev;
+   memset(&D, 0, sizeof D);
+   D.imxb = b->imxb;
+   D.jmxb = b->jmxb;
+   D.pitch = P;
+   D.jend = B->jend;
+   D.ebsn_ell = (b->ebsn == '$' || b->ebsn == '+');
+   D.ebsn1_amp = (b->ebsn1 == '&');
+   D.ismth = b->ismth;
+   D.xver = strncmp(b->sta, "CH2 (2008)", 10) == 0 ? 1 : 2;
+   D.hter = (float) b->hter;
+   {
+      static const float hc[12] = {0, 0, .5f, .1f, .2f, .3f, .4f, 1.E-5f, .2f, .3f, .4f, .5f};
+      D.hcrt = (b->ismth >= 2 && b->ismth <= 11) ? hc[b->ismth] : 0.f;
+      D.exclude = (b->ismth >= 7) ? 1.E-5f : D.hcrt;
+   }
+   for (int k = 0; k < 4; k++) D.gamfp[k] = (float) b->gamfp[k];
+   D.nbcpt = b->nbcpt;
+   D.zb = (const float *) up(B->zb.data(), np * 4);
+   D.zbm = (const float *) up(B->zbm.data(), np * 4);
+   D.zcor = (const float *) up(B->zcor.data(), np * 4);
+   D.flags = (const uint32_t *) up(B->flags.data(), np * 4);
+   D.is = (const int *) up(B->is.data(), (size_t) NJ * 4);
+   D.ie = (const int *) up(B->ie.data(), (size_t) NJ * 4);
+   D.ms = (const int *) up(B->ms.data(), (size_t) NJ * 4);
+   D.me = (const int *) up(B->me.data(), (size_t) NJ * 4);
+   const float **vi[6] = {&D.elpct, &D.elpdt, &D.elpcl, &D.elpdl, &D.elpcl2, &D.elpdl2};
+   for (int k = 0; k < 6; k++) *vi[k] = (const float *) up(B->vi[k].data(), (size_t) P * 4);
+   const float **vj[5] = {&D.sint, &D.cost, &D.sinl, &D.cosl, &D.sinl2};
+   for (int k = 0; k < 5; k++) *vj[k] = (const float *) up(B->vj[k].data(), (size_t) NJ * 4);
+   D.tab = (const float *) up(B->tab.data(), B->tab.size() * 4);
+   D.slpai = (const float *) up(B->slpai.data(), 601 * 4);
+   HIPOK(hipDeviceSynchronize());
+   B->on_dev = true;
+   return 0;
+}
+
+extern "C" int slg_basin_open(const char *dta, const char *bnt_line, slg_basin **out, char *err,
+                              int errlen)
+{
+   char e[256];
+   *out = nullptr;
+   slg_basin *B = new slg_basin();
+   if (sl_basin_load(dta, &B->h, e)) {
+      seterr(err, errlen, "%s", e);
+      delete B;
+      return SLG_EINPUT;
+   }
+   sl_basin *b = &B->h;
+   if (!strcmp(b->dollar, "$ ")) {
+      seterr(err, errlen, "%s", "island (periodic) basins are not supported by this build");
+      sl_basin_free(b);
+      delete B;
+      return SLG_EUNSUP;
+   }
+   if (bnt_line && bnt_line[0]) {
+      if (sl_basin_llx(b, bnt_line, e)) {
+         seterr(err, errlen, "%s", e);
+         sl_basin_free(b);
+         delete B;
+         return SLG_EINPUT;
+      }
+      strncpy(B->bnt, bnt_line, sizeof B->bnt - 1);
+   }
+   B->pitch = (b->imxb + 63) & ~63;
+   B->jend = b->jmxb1;
+   char e1 = b->ebsn1;
+   B->incsm = 20;
+   if (e1 == '+' || e1 == '=') B->incsm = 1;
+   if (e1 == '!' || e1 == '#') B->incsm = 5;
+   if (e1 == '@' || e1 == '%') B->incsm = 10;
+   if (e1 == '^' || e1 == '*') B->incsm = 30;
+   B->ismpt2g = !(e1 == '+' || e1 == '&' || e1 == '#' || e1 == '%' || e1 == '*');
+   B->on_dev = false;
+   B->dmem = nullptr;
+   basin_images(B);
+   *out = B;
+   return 0;
+}
+
+extern "C" int slg_basin_set_ylt(slg_basin *B, const double *ylt)
+{
+   sl_basin *b = &B->h;
+   memcpy(b->ylt, ylt, sizeof(double) * (size_t) b->imxb * b->jmxb);
+   b->have_ylt = 1;
+   basin_images(B);
+   return 0;
+}
+
+extern "C" int slg_basin_dims(const slg_basin *B, int *imxb, int *jmxb, int *nbcpt)
+{
+   *imxb = B->h.imxb;
+   *jmxb = B->h.jmxb;
+   if (nbcpt) *nbcpt = B->h.nbcpt;
+   return 0;
+}
+
+extern "C" void slg_basin_close(slg_basin *B)
+{
+   if (!B) return;
+   if (B->dmem) hipFree(B->dmem);
+   sl_basin_free(&B->h);
+   delete B;
+}
+
+// --------------------------------------------------------------------------
+// storm schedule: ref-format rows (oracle/ref_driver.c dump_storm layout)
+
+static void storm_row(const sl_storm *s, const sl_basin *b, double *v)
+{
+   memset(v, 0, 90 * sizeof(double));
+   v[0] = s->itime;   v[1] = s->bt;       v[2] = s->delt;   v[3] = s->c1;     v[4] = s->c2;
+   v[5] = s->ax;      v[6] = s->ay;       v[7] = s->c21;    v[8] = s->c22;    v[9] = s->ydelp;
+   v[10] = s->zdelp;  v[11] = s->pnn;     v[12] = s->yc24;  v[13] = s->zc24;  v[14] = s->treal;
+   v[15] = s->w1218[0]; v[16] = s->w1218[1]; v[17] = s->wmax[0]; v[18] = s->wmax[1];
+   v[19] = s->itmadv; v[20] = s->nhrad;   v[21] = s->ptency; v[22] = s->rtency; v[23] = s->mhalt;
+   v[24] = s->x1b[3]; v[25] = s->x1b[4];  v[26] = s->x1b[5]; v[27] = s->ic19;  v[28] = s->seadtm;
+   v[29] = s->jhr;    v[30] = s->zlato;
+   for (int k = 1; k <= 50; k++) v[40 + k - 1] = s->x12[k];
+   (void) b;
+}
+
+static void push_tables(storm_sched &S, const sl_storm *s, int step)
+{
+   const double *src[6] = {s->S, s->C, s->P, s->DELP, s->SW, s->CW};
+   size_t base = S.tables.size();
+   S.tables.resize(base + 6 * SL_NTABV, 0.f);
+   for (int t = 0; t < 6; t++)
+      for (int k = 1; k <= 800; k++) S.tables[base + t * SL_NTABV + k] = (float) src[t][k];
+   S.tab_step.push_back(step);
+}
+
+static int make_schedule(const slg_basin *B, const char *trk, storm_sched &S, char *err, int errlen)
+{
+   char e[256];
+   if (sl_storm_load(trk, &B->h, &S.s0, e)) {
+      seterr(err, errlen, "%s", e);
+      return SLG_EINPUT;
+   }
+   sl_storm s = S.s0;
+   S.rows.clear();
+   S.tables.clear();
+   S.tab_step.clear();
+   S.rows.resize(90);
+   storm_row(&s, &B->h, S.rows.data());
+   push_tables(S, &s, 0);
+   int gen = s.table_gen, n = 0;
+   while (s.itime < s.mhalt) {
+      sl_storm_ttimer(&s);
+      if (sl_storm_stmval(&s, &B->h)) {
+         seterr(err, errlen, "%s", "FORCE1 did not converge");
+         return SLG_EINPUT;
+      }
+      n++;
+      size_t o = S.rows.size();
+      S.rows.resize(o + 90);
+      storm_row(&s, &B->h, S.rows.data() + o);
+      if (s.table_gen != gen) {
+         push_tables(S, &s, n);
+         gen = s.table_gen;
+      }
+   }
+   S.nsteps = n;
+   return 0;
+}
+
+// per-step device parameters for step k (1-based) from the row after step k-1
+static void step_params(const slg_basin *B, const storm_sched &S, int k, float *P)
+{
+   const double *r = S.rows.data() + (size_t) (k - 1) * 90;
+   int itime = (int) r[0] + 1, ic19 = (int) r[27];
+   double bt = 1.;
+   if (itime < ic19) {   // TTIMER, ttimer.f:36-41
+      double time = itime, ang = time / ic19;
+      bt = .5 * (1. - cos(ang * 3.14159265358979323846));
+   }
+   memset(P, 0, SL_NPAR * sizeof(float));
+   P[P_BT] = (float) bt;
+   P[P_DELT] = (float) r[2];
+   P[P_X1B4] = (float) r[24];
+   P[P_X1B5] = (float) r[25];
+   P[P_X1B6] = (float) r[26];
+   P[P_C1] = (float) r[3];
+   P[P_C2] = (float) r[4];
+   P[P_AX] = (float) r[5];
+   P[P_AY] = (float) r[6];
+   P[P_X4] = (float) r[40 + 3];
+   P[P_X9] = (float) r[40 + 8];
+   P[P_X10] = (float) r[40 + 9];
+   P[P_X15] = (float) r[40 + 14];
+   P[P_X20] = (float) r[40 + 19];
+   P[P_X21] = (float) r[40 + 20];
+   P[P_X23] = (float) r[40 + 22];
+   P[P_X24] = (float) r[40 + 23];
+   P[P_W0] = (float) r[15];
+   P[P_W1] = (float) r[16];
+   P[P_SEA] = (float) r[28];
+   int tv = 0;
+   for (size_t t = 0; t < S.tab_step.size(); t++)
+      if (S.tab_step[t] <= k - 1) tv = (int) t;
+   int fl = 0;
+   if (B->ismpt2g && itime > 5 && B->incsm != 1) fl |= SF_SMPT;
+   if ((itime + 1) % B->incsm == 0 || B->h.ebsn1 == '+') fl |= SF_FILT;
+   int ib;
+   memcpy(&ib, &itime, 4);
+   memcpy(&P[P_ITIME], &ib, 4);
+   memcpy(&P[P_TVER], &tv, 4);
+   memcpy(&P[P_FLAGS], &fl, 4);
+}
+
+// --------------------------------------------------------------------------
+// batch
+
+extern "C" int slg_batch_open(slg_basin *B, int nstorm, const char *const *trk, slg_batch **out,
+                              char *err, int errlen)
+{
+   *out = nullptr;
+   if (nstorm < 1) {
+      seterr(err, errlen, "%s", "nstorm must be >= 1");
+      return SLG_EINPUT;
+   }
+   slg_batch *X = new slg_batch();
+   X->b = B;
+   X->nstorm = nstorm;
+   X->sc.resize(nstorm);
+   X->maxsteps = 0;
+   X->dmem = nullptr;
+   for (int s = 0; s < nstorm; s++) {
+      int rc = make_schedule(B, trk[s], X->sc[s], err, errlen);
+      if (rc) {
+         delete X;
+         return rc;
+      }
+      if (X->sc[s].nsteps > X->maxsteps) X->maxsteps = X->sc[s].nsteps;
+   }
+   // initial water (INTLHT) of every run, kept in double on the host
+   const size_t np = (size_t) B->h.imxb * B->h.jmxb;
+   X->hb0.resize((size_t) nstorm * np);
+   for (int s = 0; s < nstorm; s++) sl_initial_water(&B->h, &X->sc[s].s0, &X->hb0[(size_t) s * np]);
+   X->step = 0;
+   X->parity = 0;
+   *out = X;
+   return 0;
+}
+
+extern "C" int slg_batch_nsteps(const slg_batch *X, int storm)
+{
+   return X->sc[storm].nsteps;
+}
+
+extern "C" int slg_batch_upload(slg_batch *X, char *err, int errlen)
+{
+   slg_basin *B = X->b;
+   if (!B->on_dev) {
+      int rc = basin_upload(B, err, errlen);
+      if (rc) return rc;
+   }
+   const sl_basin *b = &B->h;
+   const int P = B->pitch, NJ = b->jmxb, S = X->nstorm;
+   const size_t fstride = (size_t) P * NJ, np = fstride * S;
+   size_t ntab = 0;
+   for (auto &sc : X->sc) ntab += sc.tab_step.size();
+   size_t off[16], tot = 0;
+   int n = 0;
+   auto add = [&](size_t bytes) {
+      off[n++] = tot;
+      tot += (bytes + 255) & ~(size_t) 255;
+   };
+   add(np * 4); add(np * 4); add(np * 4); add(np * 4); add(np * 4);   // hb0 hb1 ub vb hmx
+   add((size_t) S * P * 4);                                            // hsubtop
+   add(np); add(np);                                                   // pflag hclass
+   add((size_t) S * SL_EVCAP * 4);                                     // evkey
+   add((size_t) S * 4);                                                // evcnt
+   add((size_t) S * 8 * 4);                                            // stat
+   add((size_t) S * X->maxsteps * SL_NPAR * 4);                        // params
+   add(ntab * 6 * SL_NTABV * 4);                                       // tables
+   add((size_t) S * 4);                                                // tbase
+   add((size_t) S * 4);                                                // nsteps
+   char *d = nullptr;
+   HIPOK(hipMalloc((void **) &d, tot));
+   X->dmem = d;
+   X->bytes = tot;
+   SlRunDev &R = X->dev;
+   memset(&R, 0, sizeof R);
+   R.nstorm = S;
+   R.fstride = fstride;
+   R.hb0 = (float *) (d + off[0]);
+   R.hb1 = (float *) (d + off[1]);
+   R.ub = (float *) (d + off[2]);
+   R.vb = (float *) (d + off[3]);
+   R.hmx = (float *) (d + off[4]);
+   R.hsubtop = (float *) (d + off[5]);
+   R.pflag = (uint8_t *) (d + off[6]);
+   R.hclass = (uint8_t *) (d + off[7]);
+   R.evkey = (uint32_t *) (d + off[8]);
+   R.evcnt = (int *) (d + off[9]);
+   R.stat = (int *) (d + off[10]);
+   R.maxsteps = X->maxsteps;
+   HIPOK(hipMemset(d, 0, tot));
+   // initial water (INTLHT) per storm: HB, HMX=HB, UB=VB=0
+   std::vector<float> f(fstride, 0.f);
+   std::vector<float> params((size_t) S * X->maxsteps * SL_NPAR, 0.f);
+   std::vector<float> tables;
+   std::vector<int> tbase(S), nst(S);
+   for (int s = 0; s < S; s++) {
+      const double *hb = &X->hb0[(size_t) s * b->imxb * NJ];
+      for (int j = 1; j <= NJ; j++)
+         for (int i = 1; i <= b->imxb; i++) f[(size_t) (j - 1) * P + (i - 1)] = (float) hb[PIX(b, i, j)];
+      HIPOK(hipMemcpy(R.hb0 + (size_t) s * fstride, f.data(), fstride * 4, hipMemcpyHostToDevice));
+      HIPOK(hipMemcpy(R.hb1 + (size_t) s * fstride, f.data(), fstride * 4, hipMemcpyHostToDevice));
+      for (int j = 1; j <= NJ; j++)       // HMX = HB on squares only (intlht.f:172-177)
+         for (int i = 1; i <= b->imxb; i++)
+            if (i == b->imxb || j == NJ) f[(size_t) (j - 1) * P + (i - 1)] = 0.f;
+      HIPOK(hipMemcpy(R.hmx + (size_t) s * fstride, f.data(), fstride * 4, hipMemcpyHostToDevice));
+      for (int k = 1; k <= X->sc[s].nsteps; k++)
+         step_params(B, X->sc[s], k, &params[((size_t) s * X->maxsteps + (k - 1)) * SL_NPAR]);
+      tbase[s] = (int) (tables.size() / (6 * SL_NTABV));
+      tables.insert(tables.end(), X->sc[s].tables.begin(), X->sc[s].tables.end());
+      nst[s] = X->sc[s].nsteps;
+   }
+   HIPOK(hipMemcpy(d + off[11], params.data(), params.size() * 4, hipMemcpyHostToDevice));
+   HIPOK(hipMemcpy(d + off[12], tables.data(), tables.size() * 4, hipMemcpyHostToDevice));
+   HIPOK(hipMemcpy(d + off[13], tbase.data(), (size_t) S * 4, hipMemcpyHostToDevice));
+   HIPOK(hipMemcpy(d + off[14], nst.data(), (size_t) S * 4, hipMemcpyHostToDevice));
+   R.params = (const float *) (d + off[11]);
+   R.tables = (const float *) (d + off[12]);
+   R.tbase = (const int *) (d + off[13]);
+   R.nsteps = (const int *) (d + off[14]);
+   HIPOK(hipStreamCreate(&X->st));
+   X->step = 0;
+   X->parity = 0;
+   return 0;
+}
+
+// advance every storm of the batch by n steps (stops at the shortest run)
+extern "C" int slg_batch_step(slg_batch *X, int n, char *err, int errlen)
+{
+   const slg_basin *B = X->b;
+   int lim = X->maxsteps;
+   for (auto &sc : X->sc)
+      if (sc.nsteps < lim) lim = sc.nsteps;
+   for (int k = 0; k < n && X->step < lim; k++) {
+      int step = X->step + 1;
+      int itime = step;
+      int smpt = (B->ismpt2g && itime > 5 && B->incsm != 1);
+      int filt = ((itime + 1) % B->incsm == 0 || B->h.ebsn1 == '+');
+      int rc = sl_dev_launch_step(&B->dev, &X->dev, step, smpt, filt, X->parity, X->st);
+      if (rc < 0) {
+         seterr(err, errlen, "kernel launch failed (%s%d)", "", -rc);
+         return SLG_EHIP;
+      }
+      X->parity = rc;
+      X->step = step;
+   }
+   return 0;
+}
+
+extern "C" int slg_batch_sync(slg_batch *X, char *err, int errlen)
+{
+   HIPOK(hipStreamSynchronize(X->st));
+   return 0;
+}
+
+extern "C" int slg_batch_current_step(const slg_batch *X)
+{
+   return X->step;
+}
+
+// download a field of one storm as double [jmxb][imxb]
+extern "C" int slg_batch_get(slg_batch *X, int storm, const char *name, double *dst, char *err,
+                             int errlen)
+{
+   const sl_basin *b = &X->b->h;
+   const SlRunDev &R = X->dev;
+   const float *src;
+   if (!strcmp(name, "HB")) src = X->parity ? R.hb1 : R.hb0;
+   else if (!strcmp(name, "UB")) src = R.ub;
+   else if (!strcmp(name, "VB")) src = R.vb;
+   else if (!strcmp(name, "HMX")) src = R.hmx;
+   else {
+      seterr(err, errlen, "unknown field %s", name);
+      return SLG_EINPUT;
+   }
+   std::vector<float> f(R.fstride);
+   HIPOK(hipStreamSynchronize(X->st));
+   HIPOK(hipMemcpy(f.data(), src + (size_t) storm * R.fstride, R.fstride * 4, hipMemcpyDeviceToHost));
+   const int P = X->b->pitch;
+   for (int j = 1; j <= b->jmxb; j++)
+      for (int i = 1; i <= b->imxb; i++) dst[PIX(b, i, j)] = f[(size_t) (j - 1) * P + (i - 1)];
+   return 0;
+}
+
+extern "C" int slg_batch_stats(slg_batch *X, int storm, int *out8, char *err, int errlen)
+{
+   HIPOK(hipStreamSynchronize(X->st));
+   HIPOK(hipMemcpy(out8, X->dev.stat + storm * 8, 32, hipMemcpyDeviceToHost));
+   return 0;
+}
+
+extern "C" void slg_batch_close(slg_batch *X)
+{
+   if (!X) return;
+   if (X->dmem) {
+      hipStreamSynchronize(X->st);
+      hipStreamDestroy(X->st);
+      hipFree(X->dmem);
+   }
+   delete X;
+}
+
+// --------------------------------------------------------------------------
+// exports in the tagged-record format of the reference state dumps, so the
+// host initialisation can be compared with the reference's COMMON blocks and
+// the oracle can start from exactly the product's inputs.
+
+static void rec(FILE *fp, const char *name, char type, long n, const void *data)
+{
+   char hdr[20];
+   int esz = (type == 'd') ? 8 : (type == 'i') ? 4 : (type == 'h') ? 2 : 1;
+   memset(hdr, 0, sizeof hdr);
+   strncpy(hdr, name, 16);
+   hdr[16] = type;
+   fwrite(hdr, 1, 20, fp);
+   fwrite(&n, 8, 1, fp);
+   fwrite(data, (size_t) esz, (size_t) n, fp);
+}
+
+extern "C" int slg_export_init(const slg_batch *X, int storm, const char *path)
+{
+   const sl_basin *b = &X->b->h;
+   const sl_storm *s = &X->sc[storm].s0;
+   FILE *fp = fopen(path, "wb");
+   if (!fp) return SLG_EINPUT;
+   size_t np = (size_t) b->imxb * b->jmxb;
+   int d3[6] = {b->imxb, b->jmxb, b->imxb1, b->jmxb1, b->imxb2, b->jmxb2};
+   rec(fp, "DUMB3", 'i', 6, d3);
+   rec(fp, "ZB", 'd', (long) np, b->zb);
+   rec(fp, "ZBM", 'd', (long) np, b->zbm);
+   rec(fp, "ITREE", 'c', (long) np, b->itree);
+   rec(fp, "IMANN", 'i', (long) np, b->imann);
+   rec(fp, "KSKP", 'c', (long) np, b->kskp);
+   rec(fp, "YLT", 'd', (long) np, b->ylt);
+   const double *hb = &X->hb0[(size_t) storm * np];
+   std::vector<double> z(np, 0.);
+   rec(fp, "HB", 'd', (long) np, hb);
+   rec(fp, "UB", 'd', (long) np, z.data());
+   rec(fp, "VB", 'd', (long) np, z.data());
+   std::vector<double> hmx(np, 0.);   // HMX = HB on squares only (intlht.f:172-177)
+   for (int j = 1; j <= b->jmxb1; j++)
+      for (int i = 1; i <= b->imxb1; i++) hmx[PIX(b, i, j)] = hb[PIX(b, i, j)];
+   rec(fp, "HMX", 'd', (long) np, hmx.data());
+   rec(fp, "ELPCT", 'd', b->imxb, b->elpct);
+   rec(fp, "ELPDT", 'd', b->imxb, b->elpdt);
+   rec(fp, "ELPCL", 'd', b->imxb, b->elpcl);
+   rec(fp, "ELPDL", 'd', b->imxb, b->elpdl);
+   rec(fp, "ELPCL2", 'd', b->imxb, b->elpcl2);
+   rec(fp, "ELPDL2", 'd', b->imxb, b->elpdl2);
+   rec(fp, "SINL2", 'd', b->jmxb, b->sinl2);
+   rec(fp, "SINT2", 'd', b->jmxb, b->sint2);
+   rec(fp, "SINT", 'd', b->jmxb, b->sint);
+   rec(fp, "COST", 'd', b->jmxb, b->cost);
+   rec(fp, "SINL", 'd', b->jmxb, b->sinl);
+   rec(fp, "COSL", 'd', b->jmxb, b->cosl);
+   rec(fp, "IS", 'i', b->jmxb, b->is);
+   rec(fp, "MS", 'i', b->jmxb, b->ms);
+   rec(fp, "MF", 'i', b->jmxb, b->mf);
+   rec(fp, "IE", 'i', b->jmxb, b->ie);
+   rec(fp, "ME", 'i', b->jmxb, b->me);
+   rec(fp, "X1B", 'd', 10, b->x1b);
+   rec(fp, "CORL", 'd', 90, b->corl);
+   rec(fp, "AR2", 'd', 600 * 245, b->ar2);
+   rec(fp, "AI2", 'd', 600 * 245, b->ai2);
+   rec(fp, "BR2", 'd', 600 * 245, b->br2);
+   rec(fp, "BI2", 'd', 600 * 245, b->bi2);
+   rec(fp, "CR2", 'd', 600 * 245, b->cr2);
+   rec(fp, "CI2", 'd', 600 * 245, b->ci2);
+   double gam[6] = {b->gama, b->gama1, b->gamfp[0], b->gamfp[1], b->gamfp[2], b->gamfp[3]};
+   rec(fp, "GAMAF", 'd', 6, gam);
+   double egth[4] = {5280., b->delt, 32.2, b->cor};
+   rec(fp, "EGTH", 'd', 4, egth);
+   double bsn[7] = {b->phi, b->alto, b->alno, b->phi1, b->alt1, b->aln1, b->alt1c};
+   rec(fp, "BSN", 'd', 7, bsn);
+   double pol[4] = {b->degree, b->rmouth, b->azmth, b->dela};
+   rec(fp, "POLAR", 'd', 4, pol);
+   double ell[3] = {b->aaxis, b->baxis, b->abqab};
+   rec(fp, "ELLIP", 'd', 3, ell);
+   double org[2] = {b->xmouth, b->ymouth};
+   rec(fp, "ORIGN", 'd', 2, org);
+   double dat[2] = {s->seadtm, s->dtmlak};
+   rec(fp, "DATUM", 'd', 2, dat);
+   double dlt[4] = {b->dltin[0], b->dltin[1], b->dltin[2], s->dltb};
+   rec(fp, "DLTDTA", 'd', 4, dlt);
+   rec(fp, "HTER", 'd', 1, &b->hter);
+   rec(fp, "NBCPT", 'i', 1, &b->nbcpt);
+   rec(fp, "ISH", 'i', b->nbcpt, b->ish);
+   rec(fp, "JSH", 'i', b->nbcpt, b->jsh);
+   rec(fp, "TIDESH", 'd', b->nbcpt, b->tidesh);
+   int ints[21] = {s->ic12, s->ic19, s->itime, s->mhalt, b->iver, b->iprj, b->ismth, s->key1,
+                   0, s->jhr, s->itmadv, s->nhrad, s->ibgnt, s->itend, s->ndlthr, s->ndlth2,
+                   b->nsqrs, b->nsqrw, b->nsqrwc, b->npss, b->ncut};
+   rec(fp, "INTS", 'i', 21, ints);
+   double reals[5] = {s->bt, s->dlthr, s->dlth2, s->treal, s->zlato};
+   rec(fp, "REALS", 'd', 5, reals);
+   char flags[8] = {b->dollar[0], b->dollar[1], b->ebsn, b->ebsn1, b->ebsn2, ' ', ' ', ' '};
+   rec(fp, "FLAGS", 'c', 8, flags);
+   rec(fp, "STA", 'c', 16, b->sta);
+   double strmsb[8] = {s->c1, s->c2, s->c21, s->c22, s->ax, s->ay, s->ptency, s->rtency};
+   rec(fp, "STRMSB", 'd', 8, strmsb);
+   double wtemp[5] = {s->ydelp, s->zdelp, s->pnn, s->yc24, s->zc24};
+   rec(fp, "WTEMP", 'd', 5, wtemp);
+   rec(fp, "TRK_X", 'd', 999, s->X + 1);
+   rec(fp, "TRK_Y", 'd', 999, s->Y + 1);
+   rec(fp, "TRK_PT", 'd', 999, s->PT + 1);
+   rec(fp, "TRK_R", 'd', 999, s->R + 1);
+   rec(fp, "TRK_DIR", 'd', 999, s->DIR + 1);
+   rec(fp, "TRK_SP", 'd', 999, s->SP + 1);
+   rec(fp, "S", 'd', 800, s->S + 1);
+   rec(fp, "C", 'd', 800, s->C + 1);
+   rec(fp, "P", 'd', 800, s->P + 1);
+   rec(fp, "DELP", 'd', 800, s->DELP + 1);
+   rec(fp, "SW", 'd', 800, s->SW + 1);
+   rec(fp, "CW", 'd', 800, s->CW + 1);
+   rec(fp, "W1218", 'd', 2, s->w1218);
+   rec(fp, "WMAX", 'd', 2, s->wmax);
+   rec(fp, "X12", 'd', 50, s->x12 + 1);
+   fclose(fp);
+   return 0;
+}
+
+extern "C" int slg_export_storm(const slg_batch *X, int storm, const char *rows_path,
+                                const char *tables_path)
+{
+   const storm_sched &S = X->sc[storm];
+   FILE *fp = fopen(rows_path, "wb");
+   if (!fp) return SLG_EINPUT;
+   fwrite(S.rows.data(), sizeof(double), S.rows.size(), fp);
+   fclose(fp);
+   fp = fopen(tables_path, "wb");
+   if (!fp) return SLG_EINPUT;
+   // the table sets in double, replayed from the schedule
+   sl_storm s = S.s0;
+   int gen = s.table_gen;
+   for (int n = 1; n <= S.nsteps; n++) {
+      sl_storm_ttimer(&s);
+      sl_storm_stmval(&s, &X->b->h);
+      if (s.table_gen != gen) {
+         gen = s.table_gen;
+         rec(fp, "STEP", 'i', 1, &n);
+         rec(fp, "S", 'd', 800, s.S + 1);
+         rec(fp, "C", 'd', 800, s.C + 1);
+         rec(fp, "P", 'd', 800, s.P + 1);
+         rec(fp, "DELP", 'd', 800, s.DELP + 1);
+         rec(fp, "SW", 'd', 800, s.SW + 1);
+         rec(fp, "CW", 'd', 800, s.CW + 1);
+         rec(fp, "W1218", 'd', 2, s.w1218);
+         rec(fp, "WMAX", 'd', 2, s.wmax);
+         rec(fp, "X12", 'd', 50, s.x12 + 1);
+      }
+   }
+   fclose(fp);
+   return 0;
+}

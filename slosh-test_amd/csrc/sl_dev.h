@@ -1,0 +1,72 @@
+/* sl_dev.h -- device-side data of the SLOSH step (shared by the HIP kernels and
+ * the host code that fills them).  All field arithmetic is FP32.
+ *
+ * Fields are [storm][J-1][pitch] with I fastest; pitch is imxb rounded up to 64
+ * floats so every row starts on a 256-byte boundary.
+ */
+#ifndef SL_DEV_H
+#define SL_DEV_H
+#include <stdint.h>
+
+#define SL_NTABV 801          /* radial tables, index 1..800 used (Fortran order) */
+#define SL_NPAR 24
+#define SL_EVCAP 4096         /* per-storm CONTTY event / dirty-square capacity */
+#define T_AI2_OFF 147000      /* offset of AI2 in the [6][245][600] table block */
+#define SL_M2G 30000          /* corner-smoothing list limits, smpt2g.f:17 / parm.for:26 */
+#define SL_MCT 8000
+
+/* per-point static flags */
+#define F_ACT (1u << 0)       /* CONTTY square: KSKP == '1' */
+#define F_BDRY (1u << 1)      /* BDRYHT static-height square */
+#define F_MOM (1u << 2)       /* MOMNTM interior point (J 2..JEND, I MS..ME, ITREE not 4/6) */
+#define F_TREE1 (1u << 3)     /* ITREE == '1' (trees: strong sheltering) */
+#define F_OCEAN (1u << 4)     /* ITREE '2' or '5': ocean winds */
+#define F_IT2 (1u << 5)       /* ITREE == '2' */
+#define F_BSKIP (1u << 6)     /* ITREE '6' or '4' (as a boundary point: no computation) */
+#define F_B8 (1u << 7)        /* ITREE == '8' (intermediate-depth boundary) */
+#define F_ZR (1u << 19)       /* 2 <= I <= IMXB1 and 2 <= J <= JMXB1 (UB underflow flush) */
+#define F_IIK(f) (((f) >> 8) & 0xffu)
+#define F_BOP(f) (((f) >> 16) & 7u)   /* 1..4: one boundary point, ISKIP; 5/6: corner 1/3 */
+
+typedef struct {
+   int imxb, jmxb, pitch, jend;
+   int ebsn_ell;              /* grid type '$' or '+': metric GAMF in FILTER/FLT2G */
+   int ebsn1_amp;             /* EBSN1 == '&' (HMXSV: no smoothing) */
+   int ismth, xver;           /* FILTER (1) or XFILTER hcrt code; XFILTER version */
+   float hter, hcrt, exclude;
+   float gamfp[4];
+   int nbcpt;
+   const float *zb, *zbm, *zcor;
+   const uint32_t *flags;
+   const int *is, *ie, *ms, *me;                                      /* [jmxb] */
+   const float *elpct, *elpdt, *elpcl, *elpdl, *elpcl2, *elpdl2;      /* [pitch] */
+   const float *sint, *cost, *sinl, *cosl, *sinl2;                    /* [jmxb] */
+   const float *tab;          /* [6][245][600]: AR2 AI2 BR2 BI2 CR2 CI2, (iik-1)*600+(id-1) */
+   const float *slpai;        /* [601] */
+} SlBasinDev;
+
+/* per-step storm values (what the field routines read during that step) */
+enum {
+   P_BT, P_DELT, P_X1B4, P_X1B5, P_X1B6, P_C1, P_C2, P_AX, P_AY, P_X4, P_X9, P_X10,
+   P_X15, P_X20, P_X21, P_X23, P_X24, P_W0, P_W1, P_SEA, P_ITIME, P_TVER, P_FLAGS, P_PAD
+};
+#define SF_SMPT (1 << 0)      /* SMPT2G/FLT2G this step */
+#define SF_FILT (1 << 1)      /* FILTER/XFILTER this step */
+
+typedef struct {
+   int nstorm;
+   size_t fstride;            /* floats per storm field */
+   float *hb0, *hb1, *ub, *vb, *hmx;   /* [nstorm][fstride] */
+   float *hsubtop;            /* [nstorm][pitch]: HSUB row JMXB for XFILTER */
+   uint8_t *pflag, *hclass;   /* [nstorm][fstride] */
+   uint32_t *evkey;           /* [nstorm][SL_EVCAP] */
+   int *evcnt;                /* [nstorm] */
+   int *stat;                 /* [nstorm][8]: counters / error flags */
+   const float *params;       /* [nstorm][maxsteps][SL_NPAR] */
+   int maxsteps;
+   const float *tables;       /* table pool [*][6][SL_NTABV] */
+   const int *tbase;          /* [nstorm] first table of each storm in the pool */
+   const int *nsteps;         /* [nstorm] */
+} SlRunDev;
+
+#endif
