@@ -52,6 +52,19 @@ int slg_batch_upload(slg_batch *x, char *err, int errlen);
  * TMSTPC1 + TMSTPC2 + TMSTPM, slosh2.c:1400-1545). */
 int slg_batch_step(slg_batch *x, int n, char *err, int errlen);
 int slg_batch_sync(slg_batch *x, char *err, int errlen);
+/* Kernel slots of one step, for slg_batch_step_timed. */
+#define SLG_K_CONTTY 0   /* CONTTY Jacobi pass */
+#define SLG_K_REPLAY 1   /* CONTTY drying-cascade replay */
+#define SLG_K_PFLAG 2    /* SMPT2G momentum points */
+#define SLG_K_HCLASS 3   /* SMPT2G height-point classes */
+#define SLG_K_SMOOTH 4   /* FLT2G + BDRYHT */
+#define SLG_K_HMXSV 5    /* HMXSV envelope */
+#define SLG_K_FILTER 6   /* FILTER / XFILTER */
+#define SLG_K_MOMNTM 7   /* MOMNTM + FRCPNT + MNTMBD */
+#define SLG_NKERN 8
+/* slg_batch_step with HIP events around every kernel on the batch's stream
+ * (synchronises after each step): ms[k] += kernel k time, launches[k] += 1. */
+int slg_batch_step_timed(slg_batch *x, int n, double *ms, int *launches, char *err, int errlen);
 int slg_batch_current_step(const slg_batch *x);
 /* Copy one run's field to the host as double [jmxb][imxb]: "HB", "UB", "VB" or
  * "HMX" (COMMON /DUMB7/ and /DUMB10/, parm.for:39-45). */

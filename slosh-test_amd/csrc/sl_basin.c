@@ -12,6 +12,7 @@
  * identity).  Sub-grid 1-D flow, weirs, channels, cuts, Manning friction and
  * the MSY / Okeechobee / dry-cell basin variants are detected and rejected.
  */
+#define _GNU_SOURCE 1   /* sincos */
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -606,6 +607,17 @@ static cq mkq(fq re, fq im)
    return z;
 }
 
+/* Complex quotient by the textbook formula (a+bi)/(c+di) =
+ * ((ac+bd) + (bc-ad)i) / (c^2+d^2), which is how the reference's Fortran
+ * compiler (LLVM flang) divides COMPLEX(16) values; libgcc's Smith-style
+ * division rounds differently in the last bit. */
+static cq cdivq(cq x, cq y)
+{
+   fq a = __real__ x, bb = __imag__ x, c = __real__ y, d = __imag__ y;
+   fq den = c * c + d * d;
+   return mkq((a * c + bb * d) / den, (bb * c - a * d) / den);
+}
+
 static void btmstr(sl_basin *b)
 {
    double zlato = b->alto;
@@ -624,7 +636,7 @@ static void btmstr(sl_basin *b)
       if (b->iver == 201903) c7 = (6 + (ii - 1)) / 1000.;
       else c7 = 0.006;
       for (n = 1; n <= 600; n++) {
-         double a, e, cor1, sx, sy, co1x, co1y, co2x, co2y, den;
+         double a, e, cor1, sx, sy, co1x, co1y, co2x, co2y, den, ssx, csx;
          cq co1, co2, co3, co4, co5, cp, co7, co17, co16, co10;
          size_t k = (size_t) (n - 1) * 245 + (ii - 1);
          if (ii == 1) {
@@ -640,22 +652,26 @@ static void btmstr(sl_basin *b)
          cor1 = cor * a / c7;
          sx = -e;
          sy = e;
-         co1x = cos(sx) * cosh(sy);
-         co1y = -1 * sin(sx) * sinh(sy);
+         /* The reference build evaluates SIN and COS of one argument with a
+          * single sincos() call, whose sine differs from sin() in the last
+          * bit for some arguments; the tables follow the reference. */
+         sincos(sx, &ssx, &csx);
+         co1x = csx * cosh(sy);
+         co1y = -1 * ssx * sinh(sy);
          co1 = mkq(co1x, co1y);
-         co2x = sin(sx) * cosh(sy);
-         co2y = cos(sx) * sinh(sy);
+         co2x = ssx * cosh(sy);
+         co2y = csx * sinh(sy);
          den = (co2x * co2x + co2y * co2y);
          co2x = (sx * co2x + sy * co2y) / den;
          co2y = (-1 * sx * co2x + sy * co2y) / den;
          co2 = mkq(co2x, co2y);
          co1 = co1 * co2;
-         co3 = 1 / (co1 + mkq(-1., cor1));
+         co3 = cdivq(mkq(1, 0), co1 + mkq(-1., cor1));
          co4 = co3 * co3;
          co5 = co2 * co2 + co1;
          cp = 1 - co2;
          co7 = ((fq) .5 * co5 - 1) * co4;
-         co17 = 1 / (1 + co7);
+         co17 = cdivq(mkq(1, 0), 1 + co7);
          co16 = (1 + co3) * co17;
          b->ar2[k] = (double) __real__ co16;
          b->ai2[k] = (double) __imag__ co16;

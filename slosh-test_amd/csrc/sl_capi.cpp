@@ -20,7 +20,7 @@
 #include "../../include/slosh_hip.h"
 
 extern "C" int sl_dev_launch_step(const SlBasinDev *B, const SlRunDev *R, int step, int smpt,
-                                  int filt, int parity, hipStream_t st);
+                                  int filt, int parity, hipStream_t st, hipEvent_t *ev);
 
 #define PIX(b, i, j) ((size_t) ((j) - 1) * (size_t) (b)->imxb + (size_t) ((i) - 1))
 
@@ -55,7 +55,8 @@ struct slg_basin {
 
 struct storm_sched {
    sl_storm s0;                        // storm state after initialisation
-   std::vector<double> rows;           // [nsteps+1][90] ref-format rows
+   std::vector<double> rows;           // [nsteps+1][90] ref-format rows (kept for export)
+   std::vector<float> params;          // [nsteps][SL_NPAR] device values of each step
    std::vector<float> tables;          // [ntab][6][SL_NTABV]
    std::vector<int> tab_step;          // step after which each table set took effect
    int nsteps;
@@ -330,31 +331,40 @@ static void push_tables(storm_sched &S, const sl_storm *s, int step)
    S.tab_step.push_back(step);
 }
 
-static int make_schedule(const slg_basin *B, const char *trk, storm_sched &S, char *err, int errlen)
+static void step_params(const slg_basin *B, const double *r, int tv, float *P);
+
+static int make_schedule(const slg_basin *B, const char *trk, storm_sched &S, bool keep_rows,
+                         char *err, int errlen)
 {
    char e[256];
+   double row[90];
    if (sl_storm_load(trk, &B->h, &S.s0, e)) {
       seterr(err, errlen, "%s", e);
       return SLG_EINPUT;
    }
    sl_storm s = S.s0;
    S.rows.clear();
+   S.params.clear();
    S.tables.clear();
    S.tab_step.clear();
-   S.rows.resize(90);
-   storm_row(&s, &B->h, S.rows.data());
+   storm_row(&s, &B->h, row);
+   if (keep_rows) S.rows.insert(S.rows.end(), row, row + 90);
    push_tables(S, &s, 0);
    int gen = s.table_gen, n = 0;
+   S.params.reserve((size_t) (s.mhalt + 1) * SL_NPAR);
    while (s.itime < s.mhalt) {
+      // values step n+1 reads: the row after step n, and the table set in force
+      size_t o = S.params.size();
+      S.params.resize(o + SL_NPAR);
+      step_params(B, row, (int) S.tab_step.size() - 1, &S.params[o]);
       sl_storm_ttimer(&s);
       if (sl_storm_stmval(&s, &B->h)) {
          seterr(err, errlen, "%s", "FORCE1 did not converge");
          return SLG_EINPUT;
       }
       n++;
-      size_t o = S.rows.size();
-      S.rows.resize(o + 90);
-      storm_row(&s, &B->h, S.rows.data() + o);
+      storm_row(&s, &B->h, row);
+      if (keep_rows) S.rows.insert(S.rows.end(), row, row + 90);
       if (s.table_gen != gen) {
          push_tables(S, &s, n);
          gen = s.table_gen;
@@ -364,10 +374,9 @@ static int make_schedule(const slg_basin *B, const char *trk, storm_sched &S, ch
    return 0;
 }
 
-// per-step device parameters for step k (1-based) from the row after step k-1
-static void step_params(const slg_basin *B, const storm_sched &S, int k, float *P)
+// device parameters of a step from the ref-format row left by the step before
+static void step_params(const slg_basin *B, const double *r, int tv, float *P)
 {
-   const double *r = S.rows.data() + (size_t) (k - 1) * 90;
    int itime = (int) r[0] + 1, ic19 = (int) r[27];
    double bt = 1.;
    if (itime < ic19) {   // TTIMER, ttimer.f:36-41
@@ -395,9 +404,6 @@ static void step_params(const slg_basin *B, const storm_sched &S, int k, float *
    P[P_W0] = (float) r[15];
    P[P_W1] = (float) r[16];
    P[P_SEA] = (float) r[28];
-   int tv = 0;
-   for (size_t t = 0; t < S.tab_step.size(); t++)
-      if (S.tab_step[t] <= k - 1) tv = (int) t;
    int fl = 0;
    if (B->ismpt2g && itime > 5 && B->incsm != 1) fl |= SF_SMPT;
    if ((itime + 1) % B->incsm == 0 || B->h.ebsn1 == '+') fl |= SF_FILT;
@@ -426,7 +432,7 @@ extern "C" int slg_batch_open(slg_basin *B, int nstorm, const char *const *trk, 
    X->maxsteps = 0;
    X->dmem = nullptr;
    for (int s = 0; s < nstorm; s++) {
-      int rc = make_schedule(B, trk[s], X->sc[s], err, errlen);
+      int rc = make_schedule(B, trk[s], X->sc[s], nstorm <= 8, err, errlen);
       if (rc) {
          delete X;
          return rc;
@@ -512,8 +518,8 @@ extern "C" int slg_batch_upload(slg_batch *X, char *err, int errlen)
          for (int i = 1; i <= b->imxb; i++)
             if (i == b->imxb || j == NJ) f[(size_t) (j - 1) * P + (i - 1)] = 0.f;
       HIPOK(hipMemcpy(R.hmx + (size_t) s * fstride, f.data(), fstride * 4, hipMemcpyHostToDevice));
-      for (int k = 1; k <= X->sc[s].nsteps; k++)
-         step_params(B, X->sc[s], k, &params[((size_t) s * X->maxsteps + (k - 1)) * SL_NPAR]);
+      memcpy(&params[(size_t) s * X->maxsteps * SL_NPAR], X->sc[s].params.data(),
+             X->sc[s].params.size() * sizeof(float));
       tbase[s] = (int) (tables.size() / (6 * SL_NTABV));
       tables.insert(tables.end(), X->sc[s].tables.begin(), X->sc[s].tables.end());
       nst[s] = X->sc[s].nsteps;
@@ -544,7 +550,7 @@ extern "C" int slg_batch_step(slg_batch *X, int n, char *err, int errlen)
       int itime = step;
       int smpt = (B->ismpt2g && itime > 5 && B->incsm != 1);
       int filt = ((itime + 1) % B->incsm == 0 || B->h.ebsn1 == '+');
-      int rc = sl_dev_launch_step(&B->dev, &X->dev, step, smpt, filt, X->parity, X->st);
+      int rc = sl_dev_launch_step(&B->dev, &X->dev, step, smpt, filt, X->parity, X->st, nullptr);
       if (rc < 0) {
          seterr(err, errlen, "kernel launch failed (%s%d)", "", -rc);
          return SLG_EHIP;
@@ -552,6 +558,43 @@ extern "C" int slg_batch_step(slg_batch *X, int n, char *err, int errlen)
       X->parity = rc;
       X->step = step;
    }
+   return 0;
+}
+
+// As slg_batch_step, with HIP events recorded on the batch's stream around every
+// kernel; ms[k] accumulates the time of kernel slot k (SLG_K_* order) and
+// launches[k] the number of launches timed.
+extern "C" int slg_batch_step_timed(slg_batch *X, int n, double *ms, int *launches, char *err,
+                                    int errlen)
+{
+   const slg_basin *B = X->b;
+   hipEvent_t ev[SLG_NKERN + 1];
+   for (int k = 0; k <= SLG_NKERN; k++) HIPOK(hipEventCreate(&ev[k]));
+   int lim = X->maxsteps;
+   for (auto &sc : X->sc)
+      if (sc.nsteps < lim) lim = sc.nsteps;
+   for (int k = 0; k < n && X->step < lim; k++) {
+      int step = X->step + 1, itime = step;
+      int smpt = (B->ismpt2g && itime > 5 && B->incsm != 1);
+      int filt = ((itime + 1) % B->incsm == 0 || B->h.ebsn1 == '+');
+      int rc = sl_dev_launch_step(&B->dev, &X->dev, step, smpt, filt, X->parity, X->st, ev);
+      if (rc < 0) {
+         seterr(err, errlen, "kernel launch failed (%s%d)", "", -rc);
+         return SLG_EHIP;
+      }
+      X->parity = rc;
+      X->step = step;
+      HIPOK(hipEventSynchronize(ev[SLG_NKERN]));
+      const int ran[SLG_NKERN] = {1, 1, smpt, smpt, 1, 1, filt, 1};
+      for (int q = 0; q < SLG_NKERN; q++) {
+         float t = 0.f;
+         if (!ran[q]) continue;
+         HIPOK(hipEventElapsedTime(&t, ev[q], ev[q + 1]));
+         ms[q] += t;
+         launches[q]++;
+      }
+   }
+   for (int k = 0; k <= SLG_NKERN; k++) (void) hipEventDestroy(ev[k]);
    return 0;
 }
 

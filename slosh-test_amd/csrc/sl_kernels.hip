@@ -341,15 +341,26 @@ L350:
 }
 
 // ------------------------------------------------------------------ SMPT2G
+// Sum of a per-lane count over the 64 lanes of a wave (every lane must call it);
+// returns the total in every lane.
+__device__ __forceinline__ int wave_sum(int v)
+{
+   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+   return v;
+}
+
+// The counters feed the checks against the reference's list limits (the
+// 30000 / 8000 entries of smpt2g.f:31-38, 57-62, 78-96); one atomic per wave.
 __global__ void __launch_bounds__(TX * TY) k_pflag(SlBasinDev B, SlRunDev R, StepArgs A)
 {
    const int i = blockIdx.x * TX + threadIdx.x + 1, j = blockIdx.y * TY + threadIdx.y + 1;
    const int s = blockIdx.z;
-   if (i > B.imxb || j > B.jmxb) return;
-   const size_t so = (size_t) s * R.fstride, p = IDX(B, i, j);
+   const bool in = (i <= B.imxb && j <= B.jmxb);
+   const size_t so = (size_t) s * R.fstride, p = in ? IDX(B, i, j) : 0;
    const float *ub = R.ub + so, *hn = A.hc + so;
    uint8_t f = 0;
-   if (i >= 2 && i <= B.imxb - 2 && j >= 2 && j <= B.jmxb - 2) {
+   int np = 0, nd = 0;
+   if (in && i >= 2 && i <= B.imxb - 2 && j >= 2 && j <= B.jmxb - 2) {
       if (ub[p] != 0.f && B.zbm[p] > -30.f) {
          if (ub[p - 1] == 0.f || ub[p - B.pitch] == 0.f || ub[p + B.pitch] == 0.f || ub[p + 1] == 0.f) {
             int ndry = 0;
@@ -359,33 +370,42 @@ __global__ void __launch_bounds__(TX * TY) k_pflag(SlBasinDev B, SlRunDev R, Ste
                if (hn[q] + B.zb[q] == 0.f) ndry++;
             }
             if (ndry) f |= 2;
-            atomicAdd(&R.stat[s * 8 + 0], 1);
-            if (ndry) atomicAdd(&R.stat[s * 8 + 1], ndry);
+            np = 1;
+            nd = ndry;
          }
       }
    }
-   R.pflag[so + p] = f;
+   np = wave_sum(np);
+   nd = wave_sum(nd);
+   if (threadIdx.x == 0) {
+      if (np) atomicAdd(&R.stat[s * 8 + 0], np);
+      if (nd) atomicAdd(&R.stat[s * 8 + 1], nd);
+   }
+   if (in) R.pflag[so + p] = f;
 }
 
 __global__ void __launch_bounds__(TX * TY) k_hclass(SlBasinDev B, SlRunDev R, StepArgs A)
 {
    const int i = blockIdx.x * TX + threadIdx.x + 1, j = blockIdx.y * TY + threadIdx.y + 1;
    const int s = blockIdx.z;
-   if (i > B.imxb || j > B.jmxb) return;
-   const size_t so = (size_t) s * R.fstride, p = IDX(B, i, j);
+   const bool in = (i <= B.imxb && j <= B.jmxb);
+   const size_t so = (size_t) s * R.fstride, p = in ? IDX(B, i, j) : 0;
    uint8_t c = 0;
-   if (i <= B.imxb - 1 && j <= B.jmxb - 1) {
+   if (in && i <= B.imxb - 1 && j <= B.jmxb - 1) {
       const float *hn = A.hc + so;
       if (hn[p] + B.zb[p] != 0.f) {
          const uint8_t *pf = R.pflag + so;
          // momentum points having (i,j) as a corner: (i+1,j+1) (i,j+1) (i+1,j) (i,j)
          uint8_t a = pf[p + B.pitch + 1] | pf[p + B.pitch] | pf[p + 1] | pf[p];
          c = (a & 2) ? 2 : (a & 1) ? 1 : 0;
-         if (c == 1) atomicAdd(&R.stat[s * 8 + 2], 1);
-         if (c == 2) atomicAdd(&R.stat[s * 8 + 3], 1);
       }
    }
-   R.hclass[so + p] = c;
+   const int n1 = wave_sum(c == 1), n2 = wave_sum(c == 2);
+   if (threadIdx.x == 0) {
+      if (n1) atomicAdd(&R.stat[s * 8 + 2], n1);
+      if (n2) atomicAdd(&R.stat[s * 8 + 3], n2);
+   }
+   if (in) R.hclass[so + p] = c;
 }
 
 // FLT2G (both passes) + BDRYHT + the CONTTY row copy.  hc = HB after CONTTY,
@@ -905,30 +925,43 @@ __global__ void __launch_bounds__(TX * TY) k_momntm(SlBasinDev B, SlRunDev R, St
 }
 
 // ------------------------------------------------------------------ launch
+// Kernel slots of one step, for per-kernel timing (SL_NKERN in sl_dev.h).
 extern "C" int sl_dev_launch_step(const SlBasinDev *B, const SlRunDev *R, int step, int smpt,
-                                  int filt, int parity, hipStream_t st)
+                                  int filt, int parity, hipStream_t st, hipEvent_t *ev)
 {
    dim3 blk(TX, TY), grd((B->imxb + TX - 1) / TX, (B->jmxb + TY - 1) / TY, R->nstorm);
    float *h0 = parity ? R->hb1 : R->hb0, *h1 = parity ? R->hb0 : R->hb1;
    StepArgs a;
    a.step = step;
+#define MARK(k) \
+   if (ev) (void) hipEventRecord(ev[k], st)
    // CONTTY: h0 (HSUB) -> h1
    a.hc = h0;
    a.hn = h1;
+   MARK(0);
    hipLaunchKernelGGL(k_contty, grd, blk, 0, st, *B, *R, a);
+   MARK(1);
    hipLaunchKernelGGL(k_replay, dim3((R->nstorm + 63) / 64), dim3(64), 0, st, *B, *R, a);
+   MARK(2);
    // SMPT2G classification on h1
    a.hc = h1;
    a.hn = h0;
    if (smpt) {
       hipLaunchKernelGGL(k_pflag, grd, blk, 0, st, *B, *R, a);
+      MARK(3);
       hipLaunchKernelGGL(k_hclass, grd, blk, 0, st, *B, *R, a);
+      MARK(4);
+   } else {
+      MARK(3);
+      MARK(4);
    }
    // FLT2G + BDRYHT + row copy: h1 -> h0
    hipLaunchKernelGGL(k_smooth, grd, blk, 0, st, *B, *R, a);
+   MARK(5);
    // HMXSV on h0
    a.hc = h0;
    hipLaunchKernelGGL(k_hmxsv, grd, blk, 0, st, *B, *R, a);
+   MARK(6);
    float *fin = h0;
    if (filt) {
       a.hc = h0;
@@ -936,8 +969,11 @@ extern "C" int sl_dev_launch_step(const SlBasinDev *B, const SlRunDev *R, int st
       hipLaunchKernelGGL(k_filter, grd, blk, 0, st, *B, *R, a);
       fin = h1;
    }
+   MARK(7);
    a.hc = fin;
    hipLaunchKernelGGL(k_momntm, grd, blk, 0, st, *B, *R, a);
+   MARK(8);
+#undef MARK
    hipError_t e = hipGetLastError();
    if (e != hipSuccess) return -(int) e;
    return (fin == R->hb0) ? 0 : 1;   // which buffer holds HB after the step

@@ -16,6 +16,9 @@ LIB_PATH = os.path.join(HERE, "libslosh_hip.so")
 
 _lib = None
 
+# kernel slots of one step (SLG_K_* in include/slosh_hip.h) and their symbols
+KERNELS = ["contty", "replay", "pflag", "hclass", "smooth", "hmxsv", "filter", "momntm"]
+
 
 class SloshError(RuntimeError):
     pass
@@ -38,6 +41,7 @@ def lib():
         L.slg_batch_upload.argtypes = [vp, cp, i]
         L.slg_batch_step.argtypes = [vp, i, cp, i]
         L.slg_batch_sync.argtypes = [vp, cp, i]
+        L.slg_batch_step_timed.argtypes = [vp, i, vp, vp, cp, i]
         L.slg_batch_current_step.argtypes = [vp]
         L.slg_batch_get.argtypes = [vp, i, cp, vp, cp, i]
         L.slg_batch_stats.argtypes = [vp, i, vp, cp, i]
@@ -100,6 +104,15 @@ class Batch:
     def step(self, n: int):
         err = ctypes.create_string_buffer(512)
         _chk(lib().slg_batch_step(self.h, n, err, 512), err)
+
+    def step_timed(self, n: int):
+        """n steps with HIP events around each kernel; returns (ms[8], launches[8])."""
+        ms = np.zeros(KERNELS.__len__(), dtype=np.float64)
+        cnt = np.zeros(len(KERNELS), dtype=np.int32)
+        err = ctypes.create_string_buffer(512)
+        _chk(lib().slg_batch_step_timed(self.h, n, ms.ctypes.data_as(ctypes.c_void_p),
+                                        cnt.ctypes.data_as(ctypes.c_void_p), err, 512), err)
+        return ms, cnt
 
     def sync(self):
         err = ctypes.create_string_buffer(512)

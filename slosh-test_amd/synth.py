@@ -101,17 +101,20 @@ def make_basin(name="tst", gtype=" ", ni=100, nj=90, seed=1, land_frac=0.20,
         rmouth = r_inner_mi * math.exp(irb * dela)          # radius of ring IRB+1
         abqab = 0.0
     elif gtype == "$":
-        # elliptic I: BAXIS = DSTNT/cos(ASMPT), AAXIS = 0 (crdrd1.f:175-187)
+        # elliptic I: DEGREE = (90-ASMPT)/NDE, BAXIS = DSTNT/cos(ASMPT), AAXIS = 0,
+        # so z = 2 RMOUTH sinh(w) (crdrd1.f:175-187, setdst.f:47-87)
         ndb = 0
-        asmpt = 90.0 - 90.0 * 1.0                            # rays span 0..90 deg
         asmpt = 5.0
         dela = math.radians((90.0 - asmpt) / nj)
         irb = -int(round(0.25 / dela))                      # keep off the focal line
-        dstnt = 40.0
-        baxis = dstnt / math.cos(math.radians(asmpt))
-        rmouth = 0.5 * baxis
+        w = (np.arange(ni) - irb + 0.5)[None, :] * dela + 1j * ((np.arange(nj) + 0.5) * dela)[:, None]
+        dzmin = np.abs(np.cosh(w)).min() * 2.0 * dela       # cell size per mile of RMOUTH
+        rmouth = size_km[0] / 1.609344 / dzmin
+        dstnt = 2.0 * rmouth * math.cos(math.radians(asmpt))
         abqab = -1.0
-        extra = dict(dstnt=dstnt, asmpt=asmpt)
+        wm = ((ni // 2) - irb + 0.5) * dela + 1j * ((nj // 2) + 0.5) * dela
+        zm = 2.0 * rmouth * np.sinh(wm)                      # grid midpoint, miles
+        extra = dict(dstnt=dstnt, asmpt=asmpt, xmouth=float(zm.real), ymouth=float(zm.imag))
     elif gtype == "+":
         # hyperbolic (elliptic II): DELA = 2 GSIZE/(AAXIS+BAXIS) (crdrd1.f:188-195)
         ndb = nj // 2
@@ -134,7 +137,10 @@ def make_basin(name="tst", gtype=" ", ni=100, nj=90, seed=1, land_frac=0.20,
     water = z > thr
     zw = (z - thr) / (z.max() - thr + 1e-9)                  # 0..1 in water
     zl = (thr - z) / (thr - z.min() + 1e-9)                  # 0..1 on land
-    depth_ft = np.where(water, 1.0 + zw * (depth_m * FT_PER_M - 1.0), -zl * land_m * FT_PER_M)
+    # land starts 1 ft above datum: cells exactly at datum would sit at the
+    # still-water level and flood as zero-thickness sheets
+    depth_ft = np.where(water, 1.0 + zw * (depth_m * FT_PER_M - 1.0),
+                        -(1.0 + zl * (land_m * FT_PER_M - 1.0)))
     hmax = (cfl * hsize / dt) ** 2 / G_FT                    # gravity-wave limit, total depth
     depth_ft = np.where(water, np.minimum(depth_ft, np.maximum(hmax - 20.0, 1.0)), depth_ft)
     zb = np.zeros((jmxb, imxb))
@@ -202,7 +208,9 @@ def dta_text(b: Basin) -> str:
     L.append("Proj=Clarke")
     nm = ("SYN " + b.name.upper())[:10].ljust(10)
     # col 12 grid type, 13-14 island flag, 15 smoothing, 16 order, 17 hemi, 18-21 opts
-    L.append(nm + " " + b.gtype + "  " + " " * 7)
+    # col 15 EBSN1 (smoothing cadence), col 21 X-filter code (crdrd1.f:72-142)
+    L.append(nm + " " + b.gtype + "  " + b.extra.get("ebsn1", " ") + " " * 5 +
+             b.extra.get("ismooth", " "))
     L.append(_f10(b.azmth))
     if b.gtype in "$+":
         L.append(_f10(b.alto, b.alno))                       # TLAT TLONG (unused)
@@ -217,8 +225,8 @@ def dta_text(b: Basin) -> str:
     nde = b.jmxb - 1 - b.ndb
     L.append("%5d%5d%5d%5d%5d" % (b.irb, ire, b.ndb, nde, 0))
     L.append(_f10(b.dltin, b.dltin, b.dltin))
-    if b.gtype != "+":
-        L.append(_f10(b.rmouth, 0.0))                        # XMOUTH YMOUTH
+    if b.gtype != "+":                                       # XMOUTH YMOUTH
+        L.append(_f10(b.extra.get("xmouth", b.rmouth), b.extra.get("ymouth", 0.0)))
     # barriers (crdrd2.f:78-100): count with the letter code for thousands
     n = len(b.barriers)
     L.append("%3d  " % n if n < 1000 else "%3d%s " % (n % 1000, "XYZWVUT"[n // 1000 - 1]))
@@ -417,6 +425,26 @@ def config2(seed=11):
     """Elliptic 250 x 300, ~35 % land, 400 barriers, spline track, 72 h at 30 s."""
     b = make_basin("ell", "$", 250, 300, seed=seed, land_frac=0.35, n_barriers=400, dt=30.0)
     return b, spline_track(b, seed=seed + 1)
+
+
+def config3_basin(seed=21):
+    """The ensemble basin: config-1 (polar) generator at 250 x 300 cells, 30 s step."""
+    return make_basin("mow", " ", 250, 300, seed=seed, land_frac=0.20, n_barriers=400,
+                      dt=30.0)
+
+
+def meow_tracks(b: Basin, n=1024, hours=48, landfall=24):
+    """MEOW-style ensemble: 8 headings x {5,10,15,25} kt x {30,50,70,90} mb x
+    {15,30} mi x 4 parallel offsets (the first n of the 1024 combinations)."""
+    out = []
+    for off in (-24.0, -8.0, 8.0, 24.0):
+        for rmw in (15.0, 30.0):
+            for dp in (30.0, 50.0, 70.0, 90.0):
+                for spd in (5.0, 10.0, 15.0, 25.0):
+                    for hd in range(0, 360, 45):
+                        out.append(straight_track(b, spd, float(hd), dp, rmw, hours=hours,
+                                                  landfall=landfall, offset_mi=off))
+    return out[:n]
 
 
 def tiny(seed=5, gtype=" ", ni=24, nj=20, dt=60.0, hours=6):

@@ -1,0 +1,58 @@
+"""The C-ABI library loads without a GPU and exports every entry point its
+public header declares; host-only entry points behave as documented."""
+import ctypes
+import glob
+import os
+import re
+
+import pytest
+
+import helpers
+import sloshgpu
+
+HEADERS = glob.glob(os.path.join(helpers.REPO, "include", "*.h"))
+
+
+def declared():
+    names = set()
+    for h in HEADERS:
+        txt = re.sub(r"/\*.*?\*/", "", open(h).read(), flags=re.S)
+        names |= set(re.findall(r"^\s*(?:int|void|double|const char \*)\s+\**(sl\w+)\s*\(", txt, re.M))
+    return sorted(names)
+
+
+def test_header_declares_entry_points():
+    assert len(declared()) >= 16
+
+
+@pytest.mark.parametrize("sym", declared())
+def test_library_exports(sym):
+    L = sloshgpu.lib()
+    assert hasattr(L, sym), "%s declared but not exported" % sym
+
+
+def test_basin_open_reports_errors(tmp_path):
+    p = tmp_path / "bad.dta"
+    p.write_text("not a basin\n")
+    with pytest.raises(sloshgpu.SloshError):
+        sloshgpu.Basin(str(p), "x")
+
+
+def test_unsupported_track_format_rejected(tmp_path):
+    b, t = helpers.synth.tiny(seed=3)
+    case = helpers.make_case(str(tmp_path), b, t)
+    bad = tmp_path / "old.trk"
+    bad.write_text("this is not a 2013-format track\n")
+    B = sloshgpu.Basin(case["dta"], case["bntline"])
+    with pytest.raises(sloshgpu.SloshError):
+        sloshgpu.Batch(B, [str(bad)])
+    B.close()
+
+
+def test_host_state_without_gpu(tmp_path):
+    """Opening a basin and a batch, and exporting its host-side state, needs no
+    GPU; the step itself does and says so."""
+    b, t = helpers.synth.tiny(seed=4)
+    case = helpers.make_case(str(tmp_path), b, t)
+    rows, tabs, nsteps = helpers.product_export(case, str(tmp_path))
+    assert nsteps == rows.shape[0] - 1 and len(tabs) >= 1
