@@ -35,18 +35,26 @@ import sloshgpu  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 # Algorithmic HBM bytes per storm.cell of one launch of each kernel (FP32 fields,
-# each element read or written once; static basin fields counted per storm):
+# each element read or written once by the launch; the basin's static fields are
+# shared by every storm of the batch and are not counted per storm):
 ALG_BYTES = {
-    "contty": 24,   # HB in/out, UB, VB, ZB, flags
-    "replay": 0,    # serial cascade: touches only a few squares
-    "pflag": 17,    # UB, ZBM, HB, ZB, flag byte out
-    "hclass": 10,   # HB, ZB, flag byte in, class byte out
-    "smooth": 21,   # HB in/out, class byte, ZB, ZBM, flags
-    "hmxsv": 20,    # HB, ZB, ZBM, HMX in/out
-    "filter": 16,   # HB in/out, ZB, ZBM
-    "momntm": 36,   # HB, ZB, ZBM, Coriolis factor, flags, UB/VB in/out
+    # fused path
+    "contty": 17,   # HB in, UB, VB in; HB out, U==0 mask byte out
+    "replay": 0,    # serial cascade: touches only the drying squares
+    "stepB": 33,    # HB in, mask byte, HMX in/out, UB/VB in/out, HB out
+    # staged path (one kernel per routine)
+    "pflag": 9,     # UB, HB in, flag byte out
+    "hclass": 6,    # HB, flag byte in, class byte out
+    "smooth": 9,    # HB in/out, class byte
+    "hmxsv": 12,    # HB, HMX in/out
+    "filter": 8,    # HB in/out
+    "momntm": 20,   # HB, UB/VB in/out
 }
-B_ALG_STEP = 48.0   # whole-step algorithmic bytes per storm.cell.step (single storm)
+# Whole-step algorithmic bytes per storm.cell.step: HB, UB, VB, HMX each read
+# and written once (32 B) plus the static ZB, ZBM, Coriolis factor and flags
+# (16 B) once per basin cell, shared by the S storms of the batch.
+def b_alg_step(storms):
+    return 32.0 + 16.0 / storms
 
 
 def build_ensemble(root, nstorm):
@@ -62,11 +70,12 @@ def build_ensemble(root, nstorm):
     return b, paths, trks
 
 
-def gpu_run(dta, bnt, trks, steps, warmup, timed_steps):
+def gpu_run(dta, bnt, trks, steps, warmup, timed_steps, path=0, rows=16):
     B = sloshgpu.Basin(dta, bnt)
     t0 = time.time()
     X = sloshgpu.Batch(B, trks)
     t_sched = time.time() - t0
+    X.set_path(path, rows)
     X.upload()
     X.step(warmup)
     X.sync()
@@ -115,6 +124,8 @@ def main():
     ap.add_argument("--timed-steps", type=int, default=20)
     ap.add_argument("--cpu-steps", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--path", choices=["fused", "staged"], default="fused")
+    ap.add_argument("--tile-rows", type=int, default=16)
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported "
@@ -129,7 +140,9 @@ def main():
             fh.write(paths["bntline"] + "\n")
         print("bench: batched %d storms on %dx%d polar basin" % (a.storms, b.imxb - 1, b.jmxb - 1),
               file=sys.stderr, flush=True)
-        r = gpu_run(paths["dta"], paths["bntline"], trks, a.steps, a.warmup, a.timed_steps)
+        path = sloshgpu.PATH_FUSED if a.path == "fused" else sloshgpu.PATH_STAGED
+        r = gpu_run(paths["dta"], paths["bntline"], trks, a.steps, a.warmup, a.timed_steps, path,
+                    a.tile_rows)
         rate = a.storms * r["cells"] * a.steps / r["dt"]
         print("bench: batched %.3e storm.cell.steps/s" % rate, file=sys.stderr, flush=True)
 
@@ -137,13 +150,14 @@ def main():
         root2 = os.path.join(root, "plain")
         b2, t2 = synth.config2()
         p2 = synth.write_case(root2, b2, t2)
-        rp = gpu_run(p2["dta"], p2["bntline"], [p2["trk"]], a.plain_steps, a.warmup, a.timed_steps)
+        rp = gpu_run(p2["dta"], p2["bntline"], [p2["trk"]], a.plain_steps, a.warmup, a.timed_steps,
+                     path, a.tile_rows)
         plain_rate = rp["cells"] * a.plain_steps / rp["dt"]
         print("bench: plain %.3e storm.cell.steps/s" % plain_rate, file=sys.stderr, flush=True)
 
         # dominant kernel of the batched run, timed with HIP events on its stream
         avg = {k: (r["ms"][i] / r["cnt"][i] if r["cnt"][i] else 0.0)
-               for i, k in enumerate(sloshgpu.KERNELS)}
+               for i, k in enumerate(sloshgpu.KERNELS) if r["cnt"][i]}
         dom = max(avg, key=lambda k: avg[k] * (r["cnt"][sloshgpu.KERNELS.index(k)] or 0))
         dom_bytes = ALG_BYTES[dom] * a.storms * r["cells"]
         achieved = dom_bytes / (avg[dom] * 1e-3) / 1e9
@@ -183,7 +197,8 @@ def main():
                          "traffic": None,
                          "alg_bytes_per_launch": dom_bytes,
                          "avg_launch_ms": avg[dom]},
-            "step_roofline_frac": rate * B_ALG_STEP / (HBM_PEAK_GBS * 1e9),
+            "step_roofline_frac": rate * b_alg_step(a.storms) / (HBM_PEAK_GBS * 1e9),
+            "path": a.path, "tile_rows": a.tile_rows,
             "kernel_avg_ms": avg,
             "cpu_baseline": cpu,
         }

@@ -61,7 +61,15 @@ int slg_batch_sync(slg_batch *x, char *err, int errlen);
 #define SLG_K_HMXSV 5    /* HMXSV envelope */
 #define SLG_K_FILTER 6   /* FILTER / XFILTER */
 #define SLG_K_MOMNTM 7   /* MOMNTM + FRCPNT + MNTMBD */
-#define SLG_NKERN 8
+#define SLG_K_STEPB 8    /* fused path: SMPT2G..MOMNTM in one tiled kernel */
+#define SLG_NKERN 9
+/* Launch paths.  FUSED (default): CONTTY, the cascade replay, then one kernel
+ * for SMPT2G/FLT2G/BDRYHT/HMXSV/FILTER/MOMNTM on LDS tiles of 64 x tile_rows
+ * points.  STAGED: one kernel per routine (slots 0-7).  Results are identical. */
+#define SLG_PATH_FUSED 0
+#define SLG_PATH_STAGED 1
+/* Choose the launch path and the fused tile height (8, 16 or 32 rows). */
+int slg_batch_set_path(slg_batch *x, int path, int tile_rows);
 /* slg_batch_step with HIP events around every kernel on the batch's stream
  * (synchronises after each step): ms[k] += kernel k time, launches[k] += 1. */
 int slg_batch_step_timed(slg_batch *x, int n, double *ms, int *launches, char *err, int errlen);

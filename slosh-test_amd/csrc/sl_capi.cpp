@@ -20,7 +20,8 @@
 #include "../../include/slosh_hip.h"
 
 extern "C" int sl_dev_launch_step(const SlBasinDev *B, const SlRunDev *R, int step, int smpt,
-                                  int filt, int parity, hipStream_t st, hipEvent_t *ev);
+                                  int filt, int parity, int path, int bth, hipStream_t st,
+                                  hipEvent_t *ev);
 
 #define PIX(b, i, j) ((size_t) ((j) - 1) * (size_t) (b)->imxb + (size_t) ((i) - 1))
 
@@ -43,7 +44,7 @@ struct slg_basin {
    char bnt[512];
    int pitch, jend;
    // host float images of the device static data
-   std::vector<float> zb, zbm, zcor, vi[6], vj[5], tab, slpai;
+   std::vector<float> zb, zbm, zcor, vi[6], vj[5], tab, slpai, tab8, g4;
    std::vector<uint32_t> flags;
    std::vector<int> is, ie, ms, me;
    int incsm, ismpt2g;
@@ -57,7 +58,7 @@ struct storm_sched {
    sl_storm s0;                        // storm state after initialisation
    std::vector<double> rows;           // [nsteps+1][90] ref-format rows (kept for export)
    std::vector<float> params;          // [nsteps][SL_NPAR] device values of each step
-   std::vector<float> tables;          // [ntab][6][SL_NTABV]
+   std::vector<float> tables;          // [ntab][SL_NTABV][8] (sl_dev.h)
    std::vector<int> tab_step;          // step after which each table set took effect
    int nsteps;
 };
@@ -65,6 +66,7 @@ struct storm_sched {
 struct slg_batch {
    slg_basin *b;
    int nstorm, maxsteps, step, parity;
+   int path, tile_rows;                // launch path (SLG_PATH_*) and stage-B tile height
    std::vector<storm_sched> sc;
    std::vector<double> hb0;            // initial HB per storm (double, imxb*jmxb)
    void *dmem;
@@ -88,6 +90,9 @@ static uint32_t flag_bits(const slg_basin *B, int i, int j)
    if (it == '6' || it == '4') f |= F_BSKIP;
    if (it == '8') f |= F_B8;
    if (i >= 2 && i <= b->imxb1 && j >= 2 && j <= b->jmxb1) f |= F_ZR;
+   // SMPT2G candidate momentum point (smpt2g.f:23-30): interior and ZBM > -30
+   if (i >= 2 && i <= b->imxb - 2 && j >= 2 && j <= b->jmxb - 2 && (float) b->zbm[PIX(b, i, j)] > -30.f)
+      f |= F_PFC;
    // IIK of FRCPNT/MOMNTM (frcpnt.f:75-118)
    {
       int iik;
@@ -168,6 +173,40 @@ static void basin_images(slg_basin *B)
       B->slpai[i] = B->tab[(size_t) T_AI2_OFF + (size_t) (col - 1) * 600 + i] -
                     B->tab[(size_t) T_AI2_OFF + (size_t) (col - 1) * 600 + (i - 1)];
    B->slpai[600] = B->slpai[599];
+   // Device layout: one 32-byte record per (IIK, depth): AI2 SLPAI AR2 BR2 BI2 CR2 CI2 0
+   B->tab8.assign((size_t) 245 * 600 * 8, 0.f);
+   for (int ii = 1; ii <= 245; ii++)
+      for (int n = 1; n <= 600; n++) {
+         float *e = &B->tab8[((size_t) (ii - 1) * 600 + (n - 1)) * 8];
+         const size_t o = (size_t) (ii - 1) * 600 + (n - 1);
+         e[0] = B->tab[147000 * 1 + o];   // AI2
+         e[1] = B->slpai[n];
+         e[2] = B->tab[147000 * 0 + o];   // AR2
+         e[3] = B->tab[147000 * 2 + o];   // BR2
+         e[4] = B->tab[147000 * 3 + o];   // BI2
+         e[5] = B->tab[147000 * 4 + o];   // CR2
+         e[6] = B->tab[147000 * 5 + o];   // CI2
+      }
+   // Metric weights GAMF of the four neighbours of every square, as HMXSV
+   // (hmxsv.f:84-92) and, on elliptic/hyperbolic grids, FLT2G and FILTER
+   // evaluate them: z = GAMFK/GAM0, GAMF = 2z/(1+z) - 1, in FP32 from the FP32
+   // metric images (the same operations the kernels would do every step).
+   B->g4.assign((size_t) 4 * np, 0.f);
+   {
+      const std::vector<float> &cl2 = B->vi[4], &dl2 = B->vi[5], &sl2 = B->vj[4];
+      static const int iih[4] = {0, 1, 0, -1}, jjh[4] = {-1, 0, 1, 0};
+      for (int j = 1; j <= NJ - 1; j++)
+         for (int i = 1; i <= b->imxb - 1; i++) {
+            const float gam0 = dl2[i - 1] + (cl2[i - 1] - dl2[i - 1]) * sl2[j - 1];
+            for (int k = 0; k < 4; k++) {
+               const int ii = i + iih[k], jj = j + jjh[k];
+               if (ii == 0 || jj == 0) continue;
+               const float gamfk = dl2[ii - 1] + (cl2[ii - 1] - dl2[ii - 1]) * sl2[jj - 1];
+               const float z = gamfk / gam0;
+               B->g4[(size_t) k * np + (size_t) (j - 1) * P + (i - 1)] = 2.f * z / (1.f + z) - 1.f;
+            }
+         }
+   }
 }
 
 static int basin_upload(slg_basin *B, char *err, int errlen)
@@ -188,8 +227,8 @@ static int basin_upload(slg_basin *B, char *err, int errlen)
    for (int k = 0; k < 4; k++) add((size_t) NJ * 4);   // is ie ms me
    for (int k = 0; k < 6; k++) add((size_t) P * 4);
    for (int k = 0; k < 5; k++) add((size_t) NJ * 4);
-   add(B->tab.size() * 4);
-   add(601 * 4);
+   add(B->tab8.size() * 4);
+   add(4 * np * 4);   // g4
    char *d = nullptr;
    HIPOK(hipMalloc((void **) &d, tot));
    B->dmem = d;
@@ -204,6 +243,7 @@ static int basin_upload(slg_basin *B, char *err, int errlen)
    D.imxb = b->imxb;
    D.jmxb = b->jmxb;
    D.pitch = P;
+   D.npts = (int) np;
    D.jend = B->jend;
    D.ebsn_ell = (b->ebsn == '$' || b->ebsn == '+');
    D.ebsn1_amp = (b->ebsn1 == '&');
@@ -229,8 +269,8 @@ static int basin_upload(slg_basin *B, char *err, int errlen)
    for (int k = 0; k < 6; k++) *vi[k] = (const float *) up(B->vi[k].data(), (size_t) P * 4);
    const float **vj[5] = {&D.sint, &D.cost, &D.sinl, &D.cosl, &D.sinl2};
    for (int k = 0; k < 5; k++) *vj[k] = (const float *) up(B->vj[k].data(), (size_t) NJ * 4);
-   D.tab = (const float *) up(B->tab.data(), B->tab.size() * 4);
-   D.slpai = (const float *) up(B->slpai.data(), 601 * 4);
+   D.tab8 = (const float *) up(B->tab8.data(), B->tab8.size() * 4);
+   D.g4 = (const float *) up(B->g4.data(), 4 * np * 4);
    HIPOK(hipDeviceSynchronize());
    B->on_dev = true;
    return 0;
@@ -265,6 +305,15 @@ extern "C" int slg_basin_open(const char *dta, const char *bnt_line, slg_basin *
    }
    B->pitch = (b->imxb + 63) & ~63;
    B->jend = b->jmxb1;
+   // MOMNTM reads the four squares around a point and writes the boundary point
+   // beside it; the momentum rows must leave column 1 and column IMXB for those.
+   for (int j = 2; j <= B->jend; j++)
+      if (b->ms[j - 1] <= b->me[j - 1] && (b->ms[j - 1] < 2 || b->me[j - 1] > b->imxb - 1)) {
+         seterr(err, errlen, "momentum row %d spans columns outside 2..IMXB-1", "", j);
+         sl_basin_free(b);
+         delete B;
+         return SLG_EUNSUP;
+      }
    char e1 = b->ebsn1;
    B->incsm = 20;
    if (e1 == '+' || e1 == '=') B->incsm = 1;
@@ -323,11 +372,12 @@ static void storm_row(const sl_storm *s, const sl_basin *b, double *v)
 
 static void push_tables(storm_sched &S, const sl_storm *s, int step)
 {
-   const double *src[6] = {s->S, s->C, s->P, s->DELP, s->SW, s->CW};
+   // device record k: C S CW SW P DELP 0 0
+   const double *src[6] = {s->C, s->S, s->CW, s->SW, s->P, s->DELP};
    size_t base = S.tables.size();
-   S.tables.resize(base + 6 * SL_NTABV, 0.f);
-   for (int t = 0; t < 6; t++)
-      for (int k = 1; k <= 800; k++) S.tables[base + t * SL_NTABV + k] = (float) src[t][k];
+   S.tables.resize(base + (size_t) SL_NTABV * 8, 0.f);
+   for (int k = 1; k <= 800; k++)
+      for (int t = 0; t < 6; t++) S.tables[base + (size_t) k * 8 + t] = (float) src[t][k];
    S.tab_step.push_back(step);
 }
 
@@ -445,7 +495,19 @@ extern "C" int slg_batch_open(slg_basin *B, int nstorm, const char *const *trk, 
    for (int s = 0; s < nstorm; s++) sl_initial_water(&B->h, &X->sc[s].s0, &X->hb0[(size_t) s * np]);
    X->step = 0;
    X->parity = 0;
+   X->path = SLG_PATH_FUSED;
+   X->tile_rows = 16;
    *out = X;
+   return 0;
+}
+
+extern "C" int slg_batch_set_path(slg_batch *X, int path, int tile_rows)
+{
+   if ((path != SLG_PATH_FUSED && path != SLG_PATH_STAGED) ||
+       (tile_rows != 8 && tile_rows != 16 && tile_rows != 32))
+      return SLG_EINPUT;
+   X->path = path;
+   X->tile_rows = tile_rows;
    return 0;
 }
 
@@ -474,12 +536,12 @@ extern "C" int slg_batch_upload(slg_batch *X, char *err, int errlen)
    };
    add(np * 4); add(np * 4); add(np * 4); add(np * 4); add(np * 4);   // hb0 hb1 ub vb hmx
    add((size_t) S * P * 4);                                            // hsubtop
-   add(np); add(np);                                                   // pflag hclass
+   add(np); add(np); add(np);                                          // pflag hclass uz
    add((size_t) S * SL_EVCAP * 4);                                     // evkey
    add((size_t) S * 4);                                                // evcnt
    add((size_t) S * 8 * 4);                                            // stat
    add((size_t) S * X->maxsteps * SL_NPAR * 4);                        // params
-   add(ntab * 6 * SL_NTABV * 4);                                       // tables
+   add(ntab * SL_NTABV * 8 * 4);                                       // tables
    add((size_t) S * 4);                                                // tbase
    add((size_t) S * 4);                                                // nsteps
    char *d = nullptr;
@@ -498,9 +560,10 @@ extern "C" int slg_batch_upload(slg_batch *X, char *err, int errlen)
    R.hsubtop = (float *) (d + off[5]);
    R.pflag = (uint8_t *) (d + off[6]);
    R.hclass = (uint8_t *) (d + off[7]);
-   R.evkey = (uint32_t *) (d + off[8]);
-   R.evcnt = (int *) (d + off[9]);
-   R.stat = (int *) (d + off[10]);
+   R.uz = (uint8_t *) (d + off[8]);
+   R.evkey = (uint32_t *) (d + off[9]);
+   R.evcnt = (int *) (d + off[10]);
+   R.stat = (int *) (d + off[11]);
    R.maxsteps = X->maxsteps;
    HIPOK(hipMemset(d, 0, tot));
    // initial water (INTLHT) per storm: HB, HMX=HB, UB=VB=0
@@ -520,18 +583,18 @@ extern "C" int slg_batch_upload(slg_batch *X, char *err, int errlen)
       HIPOK(hipMemcpy(R.hmx + (size_t) s * fstride, f.data(), fstride * 4, hipMemcpyHostToDevice));
       memcpy(&params[(size_t) s * X->maxsteps * SL_NPAR], X->sc[s].params.data(),
              X->sc[s].params.size() * sizeof(float));
-      tbase[s] = (int) (tables.size() / (6 * SL_NTABV));
+      tbase[s] = (int) (tables.size() / ((size_t) SL_NTABV * 8));
       tables.insert(tables.end(), X->sc[s].tables.begin(), X->sc[s].tables.end());
       nst[s] = X->sc[s].nsteps;
    }
-   HIPOK(hipMemcpy(d + off[11], params.data(), params.size() * 4, hipMemcpyHostToDevice));
-   HIPOK(hipMemcpy(d + off[12], tables.data(), tables.size() * 4, hipMemcpyHostToDevice));
-   HIPOK(hipMemcpy(d + off[13], tbase.data(), (size_t) S * 4, hipMemcpyHostToDevice));
-   HIPOK(hipMemcpy(d + off[14], nst.data(), (size_t) S * 4, hipMemcpyHostToDevice));
-   R.params = (const float *) (d + off[11]);
-   R.tables = (const float *) (d + off[12]);
-   R.tbase = (const int *) (d + off[13]);
-   R.nsteps = (const int *) (d + off[14]);
+   HIPOK(hipMemcpy(d + off[12], params.data(), params.size() * 4, hipMemcpyHostToDevice));
+   HIPOK(hipMemcpy(d + off[13], tables.data(), tables.size() * 4, hipMemcpyHostToDevice));
+   HIPOK(hipMemcpy(d + off[14], tbase.data(), (size_t) S * 4, hipMemcpyHostToDevice));
+   HIPOK(hipMemcpy(d + off[15], nst.data(), (size_t) S * 4, hipMemcpyHostToDevice));
+   R.params = (const float *) (d + off[12]);
+   R.tables = (const float *) (d + off[13]);
+   R.tbase = (const int *) (d + off[14]);
+   R.nsteps = (const int *) (d + off[15]);
    HIPOK(hipStreamCreate(&X->st));
    X->step = 0;
    X->parity = 0;
@@ -550,7 +613,8 @@ extern "C" int slg_batch_step(slg_batch *X, int n, char *err, int errlen)
       int itime = step;
       int smpt = (B->ismpt2g && itime > 5 && B->incsm != 1);
       int filt = ((itime + 1) % B->incsm == 0 || B->h.ebsn1 == '+');
-      int rc = sl_dev_launch_step(&B->dev, &X->dev, step, smpt, filt, X->parity, X->st, nullptr);
+      int rc = sl_dev_launch_step(&B->dev, &X->dev, step, smpt, filt, X->parity, X->path,
+                                  X->tile_rows, X->st, nullptr);
       if (rc < 0) {
          seterr(err, errlen, "kernel launch failed (%s%d)", "", -rc);
          return SLG_EHIP;
@@ -577,16 +641,28 @@ extern "C" int slg_batch_step_timed(slg_batch *X, int n, double *ms, int *launch
       int step = X->step + 1, itime = step;
       int smpt = (B->ismpt2g && itime > 5 && B->incsm != 1);
       int filt = ((itime + 1) % B->incsm == 0 || B->h.ebsn1 == '+');
-      int rc = sl_dev_launch_step(&B->dev, &X->dev, step, smpt, filt, X->parity, X->st, ev);
+      int rc = sl_dev_launch_step(&B->dev, &X->dev, step, smpt, filt, X->parity, X->path,
+                                  X->tile_rows, X->st, ev);
       if (rc < 0) {
          seterr(err, errlen, "kernel launch failed (%s%d)", "", -rc);
          return SLG_EHIP;
       }
       X->parity = rc;
       X->step = step;
-      HIPOK(hipEventSynchronize(ev[SLG_NKERN]));
-      const int ran[SLG_NKERN] = {1, 1, smpt, smpt, 1, 1, filt, 1};
-      for (int q = 0; q < SLG_NKERN; q++) {
+      if (X->path == SLG_PATH_FUSED) {
+         HIPOK(hipEventSynchronize(ev[3]));
+         const int slot[3] = {SLG_K_CONTTY, SLG_K_REPLAY, SLG_K_STEPB};
+         for (int q = 0; q < 3; q++) {
+            float t = 0.f;
+            HIPOK(hipEventElapsedTime(&t, ev[q], ev[q + 1]));
+            ms[slot[q]] += t;
+            launches[slot[q]]++;
+         }
+         continue;
+      }
+      HIPOK(hipEventSynchronize(ev[8]));
+      const int ran[8] = {1, 1, smpt, smpt, 1, 1, filt, 1};
+      for (int q = 0; q < 8; q++) {
          float t = 0.f;
          if (!ran[q]) continue;
          HIPOK(hipEventElapsedTime(&t, ev[q], ev[q + 1]));

@@ -25,11 +25,13 @@
 #define F_BSKIP (1u << 6)     /* ITREE '6' or '4' (as a boundary point: no computation) */
 #define F_B8 (1u << 7)        /* ITREE == '8' (intermediate-depth boundary) */
 #define F_ZR (1u << 19)       /* 2 <= I <= IMXB1 and 2 <= J <= JMXB1 (UB underflow flush) */
+#define F_PFC (1u << 20)      /* SMPT2G candidate: 2 <= I <= IMXB-2, 2 <= J <= JMXB-2, ZBM > -30 */
 #define F_IIK(f) (((f) >> 8) & 0xffu)
 #define F_BOP(f) (((f) >> 16) & 7u)   /* 1..4: one boundary point, ISKIP; 5/6: corner 1/3 */
 
 typedef struct {
    int imxb, jmxb, pitch, jend;
+   int npts;                  /* pitch * jmxb: elements of one field */
    int ebsn_ell;              /* grid type '$' or '+': metric GAMF in FILTER/FLT2G */
    int ebsn1_amp;             /* EBSN1 == '&' (HMXSV: no smoothing) */
    int ismth, xver;           /* FILTER (1) or XFILTER hcrt code; XFILTER version */
@@ -41,8 +43,8 @@ typedef struct {
    const int *is, *ie, *ms, *me;                                      /* [jmxb] */
    const float *elpct, *elpdt, *elpcl, *elpdl, *elpcl2, *elpdl2;      /* [pitch] */
    const float *sint, *cost, *sinl, *cosl, *sinl2;                    /* [jmxb] */
-   const float *tab;          /* [6][245][600]: AR2 AI2 BR2 BI2 CR2 CI2, (iik-1)*600+(id-1) */
-   const float *slpai;        /* [601] */
+   const float *tab8;         /* [245][600][8]: AI2 SLPAI AR2 BR2 BI2 CR2 CI2 0 per (iik, id) */
+   const float *g4;           /* [4][pitch*jmxb]: metric weight GAMF of neighbour k (IIH/JJH order) */
 } SlBasinDev;
 
 /* per-step storm values (what the field routines read during that step) */
@@ -58,13 +60,14 @@ typedef struct {
    size_t fstride;            /* floats per storm field */
    float *hb0, *hb1, *ub, *vb, *hmx;   /* [nstorm][fstride] */
    float *hsubtop;            /* [nstorm][pitch]: HSUB row JMXB for XFILTER */
-   uint8_t *pflag, *hclass;   /* [nstorm][fstride] */
+   uint8_t *pflag, *hclass;   /* [nstorm][fstride] (staged path) */
+   uint8_t *uz;               /* [nstorm][fstride]: UB == 0 after CONTTY (fused path) */
    uint32_t *evkey;           /* [nstorm][SL_EVCAP] */
    int *evcnt;                /* [nstorm] */
    int *stat;                 /* [nstorm][8]: counters / error flags */
    const float *params;       /* [nstorm][maxsteps][SL_NPAR] */
    int maxsteps;
-   const float *tables;       /* table pool [*][6][SL_NTABV] */
+   const float *tables;       /* table pool [*][SL_NTABV][8]: C S CW SW P DELP 0 0 per radius */
    const int *tbase;          /* [nstorm] first table of each storm in the pool */
    const int *nsteps;         /* [nstorm] */
 } SlRunDev;

@@ -1,22 +1,35 @@
 // sl_kernels.hip -- the SLOSH time step on CDNA4 (gfx950), FP32.
 //
-// One launch of each kernel advances every storm of a batch through one stage
-// of the step: blockIdx.z (or the thread index of the per-storm kernels) is the
-// storm, so a single storm is simply a batch of one.  Each stage restates a
-// reference routine (file:line under sorc/slosh.fd):
+// Every stage of the step is written once, as a per-point device function that
+// reads its neighbours through an accessor (global memory or an LDS tile), and
+// restates a reference routine (file:line under sorc/slosh.fd):
 //
-//   k_contty     CONTTY  contty.f:85-221 (Jacobi pass; drying squares listed)
+//   contty_sq    CONTTY  contty.f:85-221 (Jacobi pass; drying squares listed)
 //   k_replay     CONTTY  contty.f:146-213 serial back-step cascade, replayed
 //                exactly from the first drying square, visiting only squares
 //                whose inputs the cascade changed
-//   k_pflag      SMPT2G  smpt2g.f:23-45  (momentum points next to U=0 points)
-//   k_hclass     SMPT2G  smpt2g.f:51-120 (height points: copy / 888 / 999)
-//   k_smooth     FLT2G   flt2g.f:38-132 both passes + BDRYHT bdryht.f:72-86
-//                + the HB(I,JMXB)=HB(I,1) copy contty.f:218-219
-//   k_hmxsv      HMXSV   hmxsv.f:44-136
-//   k_filter     FILTER  filter.f:121-211, XFILTER xfilter.f:176-713
-//   k_momntm     MOMNTM  momntm.f:78-470 with FRCPNT frcpnt.f:63-200 and
+//   pflag_pt     SMPT2G  smpt2g.f:23-45  (momentum points next to U=0 points)
+//   hclass_pt    SMPT2G  smpt2g.f:51-120 (height points: copy / 888 / 999)
+//   smooth_pt    FLT2G   flt2g.f:38-132 both passes + BDRYHT bdryht.f:72-86
+//   hmxsv_pt     HMXSV   hmxsv.f:44-136
+//   filter_pt    FILTER  filter.f:121-211, XFILTER xfilter.f:176-713
+//   momntm_pt    MOMNTM  momntm.f:78-470 with FRCPNT frcpnt.f:63-200 and
 //                MNTMBD mntmbd.f:58-117
+//
+// Two launch paths share those functions:
+//
+//   fused (default): k_conttyA (CONTTY over a 64x16 tile per workgroup, plus
+//     the U==0 byte mask SMPT2G needs), k_replay, and k_stepB, which runs
+//     SMPT2G/FLT2G/BDRYHT/HMXSV/FILTER/MOMNTM for one 64xTH tile of one storm
+//     out of LDS: the tile and a halo wide enough for every stencil in the
+//     chain are loaded once, each stage leaves its result in LDS for the next,
+//     and only HB, HMX, UB and VB of the tile's own points go back to HBM.
+//   staged: one kernel per stage over global memory (per-stage timing and an
+//     independent check of the fused path).
+//
+// Workgroups are numbered so that each XCD (workgroup id mod 8) walks a
+// contiguous run of tiles and storms: a tile's halo and the basin's static
+// fields are then found in that XCD's own L2.
 //
 // Bit-exactness with the FP32 restatement of the reference depends on the
 // operation order being the one written here: the file is compiled with
@@ -27,15 +40,24 @@
 
 #define TX 64
 #define TY 4
-#define IDX(B, i, j) ((size_t) ((j) - 1) * (size_t) (B).pitch + (size_t) ((i) - 1))
-#define TABI(B, t, iik, id) (B).tab[(size_t) (t) * 147000 + (size_t) ((iik) - 1) * 600 + ((id) - 1)]
-enum { T_AR2, T_AI2, T_BR2, T_BI2, T_CR2, T_CI2 };
-enum { T_S, T_C, T_P, T_DELP, T_SW, T_CW };
+// 32-bit element index of grid point (i,j): with a uniform base pointer the
+// compiler addresses global memory as SGPR base + 32-bit VGPR offset.
+#define IDX(B, i, j) ((uint32_t) (((j) - 1) * (B).pitch + ((i) - 1)))
+// Bottom-stress record of slip class IIK and depth ID (one 32-byte line):
+// AI2, SLPAI(ID), AR2, BR2, BI2, CR2, CI2 (btmstr.f:185-191, momntm.f:81-89).
+enum { E_AI2, E_SLP, E_AR2, E_BR2, E_BI2, E_CR2, E_CI2 };
+#define EREC(B, iik, id) ((B).tab8 + (uint32_t) ((((iik) - 1) * 600 + ((id) - 1)) * 8))
+// Radial record K of a storm table set (one 32-byte line): C, S (ocean inflow),
+// CW, SW (lake inflow), P (pressure / r), DELP (static height) (force1.f:171-181).
+enum { R_C, R_S, R_CW, R_SW, R_P, R_DELP };
 
 // BLOCK DATA shift tables (flw1dm_block_c.f:37-38), 1-based
 __constant__ int cIP[5] = {0, 0, 1, 1, 0}, cJP[5] = {0, 0, 0, 1, 1};
 __constant__ int cIH[5] = {0, -1, 0, -1, 0}, cJH[5] = {0, -1, -1, 0, 0};
 __constant__ int cIIH[5] = {0, 0, 1, 0, -1}, cJJH[5] = {0, -1, 0, 1, 0};
+// MOMNTM boundary offsets: one boundary point per ISKIP code, three per corner
+__constant__ int cISBS[5] = {0, 0, -1, 0, 1}, cJSBS[5] = {0, -1, 0, 1, 0};
+__constant__ int cISHF[4][3] = {{0, 0, 0}, {1, 2, 4}, {0, 0, 0}, {1, 3, 4}};
 
 struct StepArgs {
    const float *hc;     // HB entering the stage (per storm base = hc + s*fstride)
@@ -51,7 +73,7 @@ __device__ __forceinline__ const float *par(const SlRunDev &R, int s, int step)
 __device__ __forceinline__ const float *tabv(const SlRunDev &R, int s, const float *P)
 {
    int v = __float_as_int(P[P_TVER]);
-   return R.tables + (size_t) (R.tbase[s] + v) * 6 * SL_NTABV;
+   return R.tables + (size_t) (R.tbase[s] + v) * SL_NTABV * 8;
 }
 
 __device__ __forceinline__ bool zr(const SlBasinDev &B, int i, int j)
@@ -59,37 +81,99 @@ __device__ __forceinline__ bool zr(const SlBasinDev &B, int i, int j)
    return i >= 2 && i <= B.imxb - 1 && j >= 2 && j <= B.jmxb - 1;
 }
 
-// ---------------------------------------------------------------------- CONTTY
-// Jacobi pass over all squares.  HSUB is the entering HB (hc).  UB values with
-// |UB| <= 1e-10 inside 2..IMXB1 x 2..JMXB1 count as 0 (and are flushed by their
-// owner thread), as contty.f:90-96 does before the sweep.
-__global__ void __launch_bounds__(TX * TY) k_contty(SlBasinDev B, SlRunDev R, StepArgs A)
+// ------------------------------------------------------------------ accessors
+// (i, j) are 1-based grid indices; every caller stays inside the grid.
+struct GF {   // a field of one storm in global memory
+   const float *p;
+   int pitch;
+   __device__ float operator()(int i, int j) const { return p[(uint32_t) ((j - 1) * pitch + (i - 1))]; }
+};
+struct LF {   // an LDS tile whose element 0 is grid point (i0, j0)
+   const float *p;
+   int i0, j0, w;
+   __device__ float operator()(int i, int j) const { return p[(j - j0) * w + (i - i0)]; }
+};
+struct GB {
+   const uint8_t *p;
+   int pitch;
+   __device__ uint8_t operator()(int i, int j) const { return p[(uint32_t) ((j - 1) * pitch + (i - 1))]; }
+};
+struct LB {
+   const uint8_t *p;
+   int i0, j0, w;
+   __device__ uint8_t operator()(int i, int j) const { return p[(j - j0) * w + (i - i0)]; }
+};
+struct GUZ {  // "UB(i,j) == 0" straight from the transport field
+   const float *ub;
+   int pitch;
+   __device__ bool operator()(int i, int j) const { return ub[(uint32_t) ((j - 1) * pitch + (i - 1))] == 0.f; }
+};
+
+// Static fields ZB (squares) and ZBM (momentum points): global or an LDS tile.
+struct ZG {
+   const float *zb, *zbm;
+   int pitch;
+   __device__ float b(int i, int j) const { return zb[(uint32_t) ((j - 1) * pitch + (i - 1))]; }
+   __device__ float m(int i, int j) const { return zbm[(uint32_t) ((j - 1) * pitch + (i - 1))]; }
+};
+struct ZL {
+   const float *zb, *zbm;
+   int i0, j0, w;
+   __device__ float b(int i, int j) const { return zb[(j - j0) * w + (i - i0)]; }
+   __device__ float m(int i, int j) const { return zbm[(j - j0) * w + (i - i0)]; }
+};
+
+// Workgroup number -> (tile x, tile y, storm).  Hardware hands workgroup g to
+// XCD g mod 8; the k-th workgroup of an XCD takes item x*q + min(x, r) + k of
+// the natural (storm, ty, tx) order, a bijection onto 0..G-1.
+__device__ __forceinline__ void tile_of(int g, int G, int ntx, int nty, int &tx, int &ty, int &s)
 {
-   const int i = blockIdx.x * TX + threadIdx.x + 1, j = blockIdx.y * TY + threadIdx.y + 1;
-   const int s = blockIdx.z;
-   if (i > B.imxb || j > B.jmxb) return;
-   const size_t so = (size_t) s * R.fstride, p = IDX(B, i, j);
-   const float *hc = A.hc + so;
-   float *hn = A.hn + so, *ub = R.ub + so, *vb = R.vb + so;
+   const int x = g & 7, k = g >> 3, q = G >> 3, r = G & 7;
+   const int lin = x * q + (x < r ? x : r) + k;
+   tx = lin % ntx;
+   const int t = lin / ntx;
+   ty = t % nty;
+   s = t / nty;
+}
+
+__device__ __forceinline__ int wave_sum(int v)
+{
+   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+   return v;
+}
+
+// ---------------------------------------------------------------------- CONTTY
+// Jacobi pass over one square.  HSUB is the entering HB (hc).  UB values with
+// |UB| <= 1e-10 inside 2..IMXB1 x 2..JMXB1 count as 0 (and are flushed by their
+// owner thread), as contty.f:90-96 does before the sweep.  Returns the new HB;
+// squares that dry with a transport rescale are listed for the replay.
+__device__ __forceinline__ float contty_sq(const SlBasinDev &B, const SlRunDev &R, int s, int step,
+                                           int i, int j, const float *hc, float *ub, float *vb,
+                                           uint8_t *uz)
+{
+   const uint32_t p = IDX(B, i, j);
    if (j == B.jmxb) {
       if (i <= B.imxb - 1) R.hsubtop[(size_t) s * B.pitch + (i - 1)] = hc[p];
-      hn[p] = hc[p];
-      return;
+      if (uz) uz[p] = ub[p] == 0.f;
+      return hc[p];
    }
    if (i == B.imxb) {
-      hn[p] = hc[p];
-      return;
+      if (uz) uz[p] = ub[p] == 0.f;
+      return hc[p];
    }
    const uint32_t fl = B.flags[p];
+   float u0 = ub[p];
    if (fl & F_ZR) {
-      if (fabsf(ub[p]) <= 1.E-10f) {
+      if (fabsf(u0) <= 1.E-10f) {
          ub[p] = 0.f;
          vb[p] = 0.f;
+         u0 = 0.f;
       }
    }
+   if (uz) uz[p] = u0 == 0.f;
    float out = hc[p];
    if (fl & F_ACT) {
-      const size_t p2 = p + 1, p3 = p + B.pitch + 1, p4 = p + B.pitch;
+      const uint32_t p2 = p + 1, p3 = p + B.pitch + 1, p4 = p + B.pitch;
       float u1 = ub[p], u2 = ub[p2], u3 = ub[p3], u4 = ub[p4];
       float v1 = vb[p], v2 = vb[p2], v3 = vb[p3], v4 = vb[p4];
       if (zr(B, i, j) && fabsf(u1) <= 1.E-10f) { u1 = 0.f; v1 = 0.f; }
@@ -99,7 +183,7 @@ __global__ void __launch_bounds__(TX * TY) k_contty(SlBasinDev B, SlRunDev R, St
       const float dux = -u1 + u2 + u3 - u4;
       const float zb = B.zb[p];
       if (!(zb <= 30.f && dux == 0.f)) {
-         const float *P = par(R, s, A.step);
+         const float *P = par(R, s, step);
          const float hpst = hc[p];
          const float z = B.elpdl2[i - 1] + (B.elpcl2[i - 1] - B.elpdl2[i - 1]) * B.sinl2[j - 1];
          const float fct = P[P_X1B5] / z;
@@ -119,7 +203,38 @@ __global__ void __launch_bounds__(TX * TY) k_contty(SlBasinDev B, SlRunDev R, St
          }
       }
    }
-   hn[p] = out;
+   return out;
+}
+
+// Staged path: one thread per square.
+__global__ void __launch_bounds__(TX * TY) k_contty(SlBasinDev B, SlRunDev R, StepArgs A)
+{
+   const int i = blockIdx.x * TX + threadIdx.x + 1, j = blockIdx.y * TY + threadIdx.y + 1;
+   const int s = blockIdx.z;
+   if (i > B.imxb || j > B.jmxb) return;
+   const size_t so = (size_t) s * R.fstride;
+   A.hn[so + IDX(B, i, j)] = contty_sq(B, R, s, A.step, i, j, A.hc + so, R.ub + so, R.vb + so, nullptr);
+}
+
+// Fused path: a 64 x 16 tile per workgroup (each thread four rows), numbered
+// XCD by XCD; also writes the U==0 mask of every point for k_stepB.
+#define ATH 16
+__global__ void __launch_bounds__(TX * TY) k_conttyA(SlBasinDev B, SlRunDev R, StepArgs A, int ntx,
+                                                     int nty)
+{
+   int tx, ty, s;
+   tile_of(blockIdx.x, gridDim.x, ntx, nty, tx, ty, s);
+   const int i = tx * TX + threadIdx.x + 1;
+   if (i > B.imxb) return;
+   const size_t so = (size_t) s * R.fstride;
+   const float *hc = A.hc + so;
+   float *hn = A.hn + so, *ub = R.ub + so, *vb = R.vb + so;
+   uint8_t *uz = R.uz + so;
+   for (int r = threadIdx.y; r < ATH; r += TY) {
+      const int j = ty * ATH + r + 1;
+      if (j > B.jmxb) break;
+      hn[IDX(B, i, j)] = contty_sq(B, R, s, A.step, i, j, hc, ub, vb, uz);
+   }
 }
 
 // ------------------------------------------------------------- CONTTY replay
@@ -136,6 +251,7 @@ struct Replay {
    const SlBasinDev *B;
    const float *hc;
    float *hn, *ub, *vb;
+   uint8_t *uz;
    uint32_t *key;
    int n, full;
    uint32_t frontier;
@@ -160,7 +276,7 @@ __device__ static void go_full(Replay &X)
    for (int j = 1; j <= B.jmxb - 1; j++)
       for (int i = 1; i <= B.imxb - 1; i++) {
          uint32_t k = ((uint32_t) j << 16) | (uint32_t) i;
-         size_t p = IDX(B, i, j);
+         uint32_t p = IDX(B, i, j);
          if (k > X.frontier && (B.flags[p] & F_ACT)) X.hn[p] = X.hc[p];
       }
 }
@@ -169,7 +285,7 @@ __device__ static void dirty(Replay &X, int i, int j)
 {
    const SlBasinDev &B = *X.B;
    if (i < 1 || j < 1 || i > B.imxb - 1 || j > B.jmxb - 1) return;
-   const size_t p = IDX(B, i, j);
+   const uint32_t p = IDX(B, i, j);
    if (!(B.flags[p] & F_ACT)) return;
    const uint32_t k = ((uint32_t) j << 16) | (uint32_t) i;
    if (X.full) return;
@@ -190,7 +306,7 @@ __device__ static void dirty(Replay &X, int i, int j)
 __device__ static int visit(Replay &X, int i, int j, int &leap, int ipass)
 {
    const SlBasinDev &B = *X.B;
-   const size_t p = IDX(B, i, j);
+   const uint32_t p = IDX(B, i, j);
    if (!(B.flags[p] & F_ACT)) return 0;
    const uint32_t k = ((uint32_t) j << 16) | (uint32_t) i;
    if (k > X.frontier) X.frontier = k;
@@ -201,7 +317,7 @@ __device__ static int visit(Replay &X, int i, int j, int &leap, int ipass)
       return 0;
    }
    float *ub = X.ub, *vb = X.vb;
-   const size_t p2 = p + 1, p3 = p + B.pitch + 1, p4 = p + B.pitch;
+   const uint32_t p2 = p + 1, p3 = p + B.pitch + 1, p4 = p + B.pitch;
    const float dux = -ub[p] + ub[p2] + ub[p3] - ub[p4];
    if (zb <= 30.f && dux == 0.f) return 0;
    const float hpst = X.hc[p];
@@ -225,17 +341,18 @@ __device__ static int visit(Replay &X, int i, int j, int &leap, int ipass)
    const float anum = -zb - hpst;
    float rfct = anum / denm;
    if (fabsf(rfct) < 1.E-5f) rfct = 0.f;
-   const size_t c[4] = {p, p2, p3, p4};
+   const uint32_t c[4] = {p, p2, p3, p4};
    for (int q = 0; q < 4; q++) {
       ub[c[q]] = rfct * ub[c[q]];
       vb[c[q]] = rfct * vb[c[q]];
+      if (X.uz) X.uz[c[q]] = ub[c[q]] == 0.f;
    }
    for (int jj = j - 1; jj <= j + 1; jj++)
       for (int ii = i - 1; ii <= i + 1; ii++) dirty(X, ii, jj);
    return 1;
 }
 
-__global__ void k_replay(SlBasinDev B, SlRunDev R, StepArgs A)
+__global__ void k_replay(SlBasinDev B, SlRunDev R, StepArgs A, int use_uz)
 {
    const int s = blockIdx.x * blockDim.x + threadIdx.x;
    if (s >= R.nstorm) return;
@@ -248,6 +365,7 @@ __global__ void k_replay(SlBasinDev B, SlRunDev R, StepArgs A)
    X.hn = A.hn + so;
    X.ub = R.ub + so;
    X.vb = R.vb + so;
+   X.uz = use_uz ? R.uz + so : nullptr;
    X.key = R.evkey + (size_t) s * SL_EVCAP;
    X.full = 0;
    X.x1b5 = par(R, s, A.step)[P_X1B5];
@@ -341,184 +459,128 @@ L350:
 }
 
 // ------------------------------------------------------------------ SMPT2G
-// Sum of a per-lane count over the 64 lanes of a wave (every lane must call it);
-// returns the total in every lane.
-__device__ __forceinline__ int wave_sum(int v)
+// Momentum point flag: bit 0 = smoothing candidate, bit 1 = next to a dry
+// height point; *ndry = number of dry height points around it.
+template <class UZ, class HS, class Z>
+__device__ __forceinline__ uint8_t pflag_pt(const SlBasinDev &B, int i, int j, const UZ &uz,
+                                            const HS &hs, const Z &z, int *ndry)
 {
-   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-   return v;
-}
-
-// The counters feed the checks against the reference's list limits (the
-// 30000 / 8000 entries of smpt2g.f:31-38, 57-62, 78-96); one atomic per wave.
-__global__ void __launch_bounds__(TX * TY) k_pflag(SlBasinDev B, SlRunDev R, StepArgs A)
-{
-   const int i = blockIdx.x * TX + threadIdx.x + 1, j = blockIdx.y * TY + threadIdx.y + 1;
-   const int s = blockIdx.z;
-   const bool in = (i <= B.imxb && j <= B.jmxb);
-   const size_t so = (size_t) s * R.fstride, p = in ? IDX(B, i, j) : 0;
-   const float *ub = R.ub + so, *hn = A.hc + so;
    uint8_t f = 0;
-   int np = 0, nd = 0;
-   if (in && i >= 2 && i <= B.imxb - 2 && j >= 2 && j <= B.jmxb - 2) {
-      if (ub[p] != 0.f && B.zbm[p] > -30.f) {
-         if (ub[p - 1] == 0.f || ub[p - B.pitch] == 0.f || ub[p + B.pitch] == 0.f || ub[p + 1] == 0.f) {
-            int ndry = 0;
+   *ndry = 0;
+   if (B.flags[IDX(B, i, j)] & F_PFC) {
+      if (!uz(i, j)) {
+         if (uz(i - 1, j) || uz(i, j - 1) || uz(i, j + 1) || uz(i + 1, j)) {
+            int nd = 0;
             f = 1;
             for (int k = 1; k <= 4; k++) {
-               size_t q = IDX(B, i + cIH[k], j + cJH[k]);
-               if (hn[q] + B.zb[q] == 0.f) ndry++;
+               const int a = i + cIH[k], b = j + cJH[k];
+               if (hs(a, b) + z.b(a, b) == 0.f) nd++;
             }
-            if (ndry) f |= 2;
-            np = 1;
-            nd = ndry;
+            if (nd) f |= 2;
+            *ndry = nd;
          }
       }
    }
-   np = wave_sum(np);
-   nd = wave_sum(nd);
-   if (threadIdx.x == 0) {
-      if (np) atomicAdd(&R.stat[s * 8 + 0], np);
-      if (nd) atomicAdd(&R.stat[s * 8 + 1], nd);
-   }
-   if (in) R.pflag[so + p] = f;
+   return f;
 }
 
-__global__ void __launch_bounds__(TX * TY) k_hclass(SlBasinDev B, SlRunDev R, StepArgs A)
+// Height point class: 0 none, 1 copy (HCT 1 ft), 2 next to dry (HCT 0 ft).
+template <class HS, class PF, class Z>
+__device__ __forceinline__ uint8_t hclass_pt(const SlBasinDev &B, int i, int j, const HS &hs, const PF &pf,
+                                             const Z &z)
 {
-   const int i = blockIdx.x * TX + threadIdx.x + 1, j = blockIdx.y * TY + threadIdx.y + 1;
-   const int s = blockIdx.z;
-   const bool in = (i <= B.imxb && j <= B.jmxb);
-   const size_t so = (size_t) s * R.fstride, p = in ? IDX(B, i, j) : 0;
-   uint8_t c = 0;
-   if (in && i <= B.imxb - 1 && j <= B.jmxb - 1) {
-      const float *hn = A.hc + so;
-      if (hn[p] + B.zb[p] != 0.f) {
-         const uint8_t *pf = R.pflag + so;
-         // momentum points having (i,j) as a corner: (i+1,j+1) (i,j+1) (i+1,j) (i,j)
-         uint8_t a = pf[p + B.pitch + 1] | pf[p + B.pitch] | pf[p + 1] | pf[p];
-         c = (a & 2) ? 2 : (a & 1) ? 1 : 0;
-      }
+   if (i <= B.imxb - 1 && j <= B.jmxb - 1 && hs(i, j) + z.b(i, j) != 0.f) {
+      // momentum points having (i,j) as a corner: (i+1,j+1) (i,j+1) (i+1,j) (i,j)
+      const uint8_t a = pf(i + 1, j + 1) | pf(i, j + 1) | pf(i + 1, j) | pf(i, j);
+      return (a & 2) ? 2 : (a & 1) ? 1 : 0;
    }
-   const int n1 = wave_sum(c == 1), n2 = wave_sum(c == 2);
-   if (threadIdx.x == 0) {
-      if (n1) atomicAdd(&R.stat[s * 8 + 2], n1);
-      if (n2) atomicAdd(&R.stat[s * 8 + 3], n2);
-   }
-   if (in) R.hclass[so + p] = c;
+   return 0;
 }
 
-// FLT2G (both passes) + BDRYHT + the CONTTY row copy.  hc = HB after CONTTY,
-// hn = HB after TMSTPC1.
-__device__ __forceinline__ float gamf_of(const SlBasinDev &B, int k, float gam0, int ii, int jj)
+// Metric weight of neighbour k of square p: on elliptic/hyperbolic grids
+// 2z/(1+z) - 1 with z = GAMFK/GAM0 (precomputed per square, g4), on polar grids
+// the constants GAMFP (flt2g.f:60-75, filter.f:140-160).
+__device__ __forceinline__ float gamf_of(const SlBasinDev &B, int k, uint32_t p)
 {
-   if (B.ebsn_ell) {
-      const float gamfk = B.elpdl2[ii - 1] + (B.elpcl2[ii - 1] - B.elpdl2[ii - 1]) * B.sinl2[jj - 1];
-      const float z = gamfk / gam0;
-      return 2.f * z / (1.f + z) - 1.f;
-   }
-   return B.gamfp[k - 1];
+   return B.ebsn_ell ? B.g4[(uint32_t) (k - 1) * B.npts + p] : B.gamfp[k - 1];
 }
 
-__global__ void __launch_bounds__(TX * TY) k_smooth(SlBasinDev B, SlRunDev R, StepArgs A)
+// FLT2G (both passes, when smpt) + BDRYHT for square (i,j), j <= JMXB-1.
+template <class HS, class CL, class Z>
+__device__ __forceinline__ float smooth_pt(const SlBasinDev &B, const float *P, const float *T, bool smpt,
+                                           int i, int j, const HS &hs, const CL &cl, const Z &z)
 {
-   const int i = blockIdx.x * TX + threadIdx.x + 1, j = blockIdx.y * TY + threadIdx.y + 1;
-   const int s = blockIdx.z;
-   if (i > B.imxb || j > B.jmxb) return;
-   const size_t so = (size_t) s * R.fstride, p = IDX(B, i, j);
-   const float *hs = A.hc + so;
-   float *hb = A.hn + so;
-   const float *P = par(R, s, A.step);
-   const int flags = __float_as_int(P[P_FLAGS]);
-   if (j == B.jmxb) {
-      hb[p] = (i <= B.imxb - 1) ? hs[IDX(B, i, 1)] : hs[p];
-      return;
-   }
-   float out = hs[p];
-   if (i <= B.imxb - 1) {
-      if (flags & SF_SMPT) {
-         const uint8_t *cl = R.hclass + so;
-         const uint8_t c = cl[p];
-         if (c) {
-            const float hct = (c == 1) ? 1.f : 0.f;
-            const float hsub = hs[p], zb = B.zb[p];
-            if (!(hsub + zb <= hct)) {
-               float hsm = 0.f, hdif = 0.f, hzz, gamf;
-               const float gam0 = B.elpdl2[i - 1] + (B.elpcl2[i - 1] - B.elpdl2[i - 1]) * B.sinl2[j - 1];
-               for (int k = 1; k <= 4; k++) {
-                  const int ii = i + cIIH[k], jj = j + cJJH[k];
-                  bool self = true;
-                  if (ii == 0 || jj == 0) {
-                     gamf = 0.f;
-                  } else {
-                     gamf = gamf_of(B, k, gam0, ii, jj);
-                     if (!(ii == B.imxb || jj == B.jmxb)) {
-                        const size_t q = IDX(B, ii, jj);
-                        if (cl[q] == c) {                     // HSUB(n) != 999
-                           const float hq = hs[q];
-                           if (!(hq + B.zb[q] <= hct)) {
-                              const int kk = k % 4 + 1;
-                              const float z1 = B.zbm[IDX(B, i + cIP[k], j + cJP[k])];
-                              const float z2 = B.zbm[IDX(B, i + cIP[kk], j + cJP[kk])];
-                              const float zzz = fminf(z1, z2) + hct;
-                              if (c == 1) self = (hsub < zzz || hq < zzz);
-                              else self = (hsub < zzz && hq < zzz);
-                              if (!self) {
-                                 hsm = hsm + hq;
-                                 hzz = hq;
-                              }
+   const uint32_t p = IDX(B, i, j);
+   float out = hs(i, j);
+   if (i > B.imxb - 1) return out;
+   if (smpt) {
+      const uint8_t c = cl(i, j);
+      if (c) {
+         const float hct = (c == 1) ? 1.f : 0.f;
+         const float hsub = out, zb = z.b(i, j);
+         if (!(hsub + zb <= hct)) {
+            float hsm = 0.f, hdif = 0.f, hzz, gamf;
+            for (int k = 1; k <= 4; k++) {
+               const int ii = i + cIIH[k], jj = j + cJJH[k];
+               bool self = true;
+               if (ii == 0 || jj == 0) {
+                  gamf = 0.f;
+               } else {
+                  gamf = gamf_of(B, k, p);
+                  if (!(ii == B.imxb || jj == B.jmxb)) {
+                     if (cl(ii, jj) == c) {                     // HSUB(n) != 999
+                        const float hq = hs(ii, jj);
+                        if (!(hq + z.b(ii, jj) <= hct)) {
+                           const int kk = k % 4 + 1;
+                           const float z1 = z.m(i + cIP[k], j + cJP[k]);
+                           const float z2 = z.m(i + cIP[kk], j + cJP[kk]);
+                           const float zzz = fminf(z1, z2) + hct;
+                           if (c == 1) self = (hsub < zzz || hq < zzz);
+                           else self = (hsub < zzz && hq < zzz);
+                           if (!self) {
+                              hsm = hsm + hq;
+                              hzz = hq;
                            }
                         }
                      }
                   }
-                  if (self) {
-                     hsm = hsm + hsub;
-                     hzz = hsub;
-                  }
-                  hdif = hdif + hzz * gamf;
                }
-               const float hbtemp = (1.f / 3.f) * hsub + (hsm + hdif) / 6.f;
-               out = fmaxf(hbtemp, -zb);
+               if (self) {
+                  hsm = hsm + hsub;
+                  hzz = hsub;
+               }
+               hdif = hdif + hzz * gamf;
             }
+            const float hbtemp = (1.f / 3.f) * hsub + (hsm + hdif) / 6.f;
+            out = fmaxf(hbtemp, -zb);
          }
       }
-      if (B.flags[p] & F_BDRY) {
-         const float *T = tabv(R, s, P);
-         const float *DELP = T + T_DELP * SL_NTABV;
-         const float xr = B.elpcl[i - 1] * B.cosl[j - 1], yr = B.elpdl[i - 1] * B.sinl[j - 1];
-         const float x = xr - P[P_C1] - P[P_AX], y = yr - P[P_C2] - P[P_AY];
-         const float rsq = x * x + y * y, r1 = sqrtf(rsq) / 5280.f + 1.f;
-         int k = (int) r1;
-         const float r2 = (float) k, dr = r1 - r2;
-         if (k > 790) k = 790;
-         out = DELP[k] + dr * (DELP[k + 1] - DELP[k]) + P[P_SEA] + 0.f;
-      }
    }
-   hb[p] = out;
+   if (B.flags[p] & F_BDRY) {
+      const float xr = B.elpcl[i - 1] * B.cosl[j - 1], yr = B.elpdl[i - 1] * B.sinl[j - 1];
+      const float x = xr - P[P_C1] - P[P_AX], y = yr - P[P_C2] - P[P_AY];
+      const float rsq = x * x + y * y, r1 = sqrtf(rsq) / 5280.f + 1.f;
+      int k = (int) r1;
+      const float r2 = (float) k, dr = r1 - r2;
+      if (k > 790) k = 790;
+      const float *Rk = T + (uint32_t) k * 8, *Rk1 = Rk + 8;
+      out = Rk[R_DELP] + dr * (Rk1[R_DELP] - Rk[R_DELP]) + P[P_SEA] + 0.f;
+   }
+   return out;
 }
 
-// ------------------------------------------------------------------- HMXSV
-__global__ void __launch_bounds__(TX * TY) k_hmxsv(SlBasinDev B, SlRunDev R, StepArgs A)
+// HMXSV at square (i,j) (i <= IMXB-1, j <= JMXB-1): the new envelope value.
+template <class HB, class Z>
+__device__ __forceinline__ float hmxsv_pt(const SlBasinDev &B, int i, int j, const HB &hb, float hx,
+                                          const Z &z)
 {
-   const int i = blockIdx.x * TX + threadIdx.x + 1, j = blockIdx.y * TY + threadIdx.y + 1;
-   const int s = blockIdx.z;
-   if (i > B.imxb - 1 || j > B.jmxb - 1) return;
-   const size_t so = (size_t) s * R.fstride, p = IDX(B, i, j);
-   const float *hb = A.hc + so;
-   float *hmx = R.hmx + so;
-   const float hx = hmx[p], h = hb[p];
-   if (!(hx < h)) return;
-   const float zb = B.zb[p];
-   if (zb > 40.f || B.ebsn1_amp) {
-      hmx[p] = h;
-      return;
-   }
-   if (h + zb < 1.f) {
-      hmx[p] = h;
-      return;
-   }
+   const float h = hb(i, j);
+   if (!(hx < h)) return hx;
+   const float zb = z.b(i, j);
+   if (zb > 40.f || B.ebsn1_amp) return h;
+   if (h + zb < 1.f) return h;
    float hsm = 0.f, hdif = 0.f, hzz, gamf;
-   const float gam0 = B.elpdl2[i - 1] + (B.elpcl2[i - 1] - B.elpdl2[i - 1]) * B.sinl2[j - 1];
+   const uint32_t p = IDX(B, i, j);
    for (int k = 1; k <= 4; k++) {
       const int ii = i + cIIH[k], jj = j + cJJH[k];
       if (ii == 0 || jj == 0) {
@@ -526,21 +588,19 @@ __global__ void __launch_bounds__(TX * TY) k_hmxsv(SlBasinDev B, SlRunDev R, Ste
          hsm = hsm + h;
          hzz = h;
       } else {
-         const float gamfk = B.elpdl2[ii - 1] + (B.elpcl2[ii - 1] - B.elpdl2[ii - 1]) * B.sinl2[jj - 1];
-         const float z = gamfk / gam0;
-         gamf = 2.f * z / (1.f + z) - 1.f;
-         const size_t q = IDX(B, ii, jj);
-         if ((jj != B.jmxb && ii != B.imxb) && hb[q] + B.zb[q] >= 1.f) {
+         gamf = B.g4[(uint32_t) (k - 1) * B.npts + p];   // hmxsv.f:84-92
+         if (jj != B.jmxb && ii != B.imxb && hb(ii, jj) + z.b(ii, jj) >= 1.f) {
+            const float hq = hb(ii, jj);
             const int kk = k % 4 + 1;
-            const float z1 = B.zbm[IDX(B, i + cIP[k], j + cJP[k])];
-            const float z2 = B.zbm[IDX(B, i + cIP[kk], j + cJP[kk])];
+            const float z1 = z.m(i + cIP[k], j + cJP[k]);
+            const float z2 = z.m(i + cIP[kk], j + cJP[kk]);
             const float zzz = fminf(z1, z2) + 1.f;
-            if (h < zzz || hb[q] < zzz) {
+            if (h < zzz || hq < zzz) {
                hsm = hsm + h;
                hzz = h;
             } else {
-               hsm = hsm + hb[q];
-               hzz = hb[q];
+               hsm = hsm + hq;
+               hzz = hq;
             }
          } else {
             hsm = hsm + h;
@@ -550,115 +610,107 @@ __global__ void __launch_bounds__(TX * TY) k_hmxsv(SlBasinDev B, SlRunDev R, Ste
       hdif = hdif + hzz * gamf;
    }
    const float htemp = (4.f * h + hsm + hdif) / 8.f;
-   hmx[p] = fmaxf(hx, htemp);
+   return fmaxf(hx, htemp);
 }
 
 // ------------------------------------------------------------ FILTER/XFILTER
-__global__ void __launch_bounds__(TX * TY) k_filter(SlBasinDev B, SlRunDev R, StepArgs A)
+// Square (i,j) with i <= IMXB-1, j <= JMXB-1; hs is the HSUB snapshot, top
+// the HSUB row JMXB CONTTY saved (XFILTER reads it; IMXB column reads 0).
+template <class HS, class Z>
+__device__ __forceinline__ float filter_pt(const SlBasinDev &B, int i, int j, const HS &hs, const float *top,
+                                           const Z &z)
 {
-   const int i = blockIdx.x * TX + threadIdx.x + 1, j = blockIdx.y * TY + threadIdx.y + 1;
-   const int s = blockIdx.z;
-   if (i > B.imxb || j > B.jmxb) return;
-   const size_t so = (size_t) s * R.fstride, p = IDX(B, i, j);
-   const float *hs = A.hc + so;
-   float *hb = A.hn + so;
-   float out = hs[p];
-   if (i <= B.imxb - 1 && j <= B.jmxb - 1) {
-      const float zb = B.zb[p], hsub = hs[p];
-      if (!(zb <= -B.hter)) {
-         if (B.ismth == 1) {
-            const float hcrt = 0.5f;
-            if (!(hsub + zb < hcrt)) {
-               float hsm = 0.f, hdif = 0.f, hzz, gamf;
-               const float gam0 = B.elpdl2[i - 1] + (B.elpcl2[i - 1] - B.elpdl2[i - 1]) * B.sinl2[j - 1];
-               for (int k = 1; k <= 4; k++) {
-                  const int ii = i + cIIH[k], jj = j + cJJH[k];
-                  if (ii == 0 || jj == 0) {
-                     gamf = 0.f;
+   const float zb = z.b(i, j), hsub = hs(i, j);
+   float out = hsub;
+   if (zb <= -B.hter) return out;
+   if (B.ismth == 1) {
+      const float hcrt = 0.5f;
+      if (!(hsub + zb < hcrt)) {
+         float hsm = 0.f, hdif = 0.f, hzz, gamf;
+         const uint32_t p = IDX(B, i, j);
+         for (int k = 1; k <= 4; k++) {
+            const int ii = i + cIIH[k], jj = j + cJJH[k];
+            if (ii == 0 || jj == 0) {
+               gamf = 0.f;
+               hsm = hsm + hsub;
+               hzz = hsub;
+            } else {
+               gamf = gamf_of(B, k, p);
+               if (ii == B.imxb || jj == B.jmxb || hs(ii, jj) + z.b(ii, jj) < hcrt) {
+                  hsm = hsm + hsub;
+                  hzz = hsub;
+               } else {
+                  const float hq = hs(ii, jj);
+                  const int kk = k % 4 + 1;
+                  const float z1 = z.m(i + cIP[k], j + cJP[k]);
+                  const float z2 = z.m(i + cIP[kk], j + cJP[kk]);
+                  const float zzz = fminf(z1, z2) + hcrt;
+                  if (hsub < zzz || hq < zzz) {
                      hsm = hsm + hsub;
                      hzz = hsub;
                   } else {
-                     gamf = gamf_of(B, k, gam0, ii, jj);
-                     const size_t q = IDX(B, ii, jj);
-                     if (ii == B.imxb || jj == B.jmxb || hs[q] + B.zb[q] < hcrt) {
-                        hsm = hsm + hsub;
-                        hzz = hsub;
-                     } else {
-                        const int kk = k % 4 + 1;
-                        const float z1 = B.zbm[IDX(B, i + cIP[k], j + cJP[k])];
-                        const float z2 = B.zbm[IDX(B, i + cIP[kk], j + cJP[kk])];
-                        const float zzz = fminf(z1, z2) + hcrt;
-                        if (hsub < zzz || hs[q] < zzz) {
-                           hsm = hsm + hsub;
-                           hzz = hsub;
-                        } else {
-                           hsm = hsm + hs[q];
-                           hzz = hs[q];
-                        }
-                     }
+                     hsm = hsm + hq;
+                     hzz = hq;
                   }
-                  hdif = hdif + hzz * gamf;
-               }
-               const float htemp = 0.5f * hsub + (hsm + hdif) / 8.f;
-               out = fmaxf(-zb, htemp);
-            }
-         } else if (!(hsub + zb < B.exclude)) {
-            // XFILTER: HSUB(IMXB,.) is never written (0); HSUB(.,JMXB) is the
-            // entering row JMXB saved by k_contty.
-            const float hcrt = B.hcrt;
-            const float *top = R.hsubtop + (size_t) s * B.pitch;
-#define HX(a, b) ((a) == B.imxb ? 0.f : ((b) == B.jmxb ? top[(a) - 1] : hs[IDX(B, a, b)]))
-#define ZX(a, b) (B.zbm[IDX(B, a, b)])
-            const int IIX[5] = {0, -1, 1, 1, -1}, JJX[5] = {0, -1, -1, 1, 1};
-            const int IP1[5] = {0, -1, 1, 2, 0}, JP1[5] = {0, 0, -1, 1, 2};
-            const int IP2[5] = {0, 0, 2, 1, -1}, JP2[5] = {0, -1, 0, 2, 1};
-            int maskii[5] = {0, 0, 0, 0, 0}, maskix[5] = {0, 0, 0, 0, 0};
-            for (int k = 1; k <= 4; k++) {
-               const int ii = i + cIIH[k], jj = j + cJJH[k], ix = i + IIX[k], jx = j + JJX[k];
-               if (ii == 0 || jj == 0 || ix == 0 || jx == 0) continue;
-               if (!(ii == B.imxb || jj == B.jmxb || HX(ii, jj) + B.zb[IDX(B, ii, jj)] < hcrt)) {
-                  const int kk2 = k % 4 + 1;
-                  const float zzz = fminf(ZX(i + cIP[k], j + cJP[k]), ZX(i + cIP[kk2], j + cJP[kk2])) + hcrt;
-                  if (!(hsub < zzz && HX(ii, jj) < zzz)) maskii[k] = 1;
-               }
-               if (!(ix == B.imxb || jx == B.jmxb || HX(ix, jx) + B.zb[IDX(B, ix, jx)] < hcrt)) {
-                  const int kk = k % 4 + 1, km1 = (k + 2) % 4 + 1;
-                  const float z = ZX(i + cIP[k], j + cJP[k]), zz = ZX(i + cIP[kk], j + cJP[kk]);
-                  const float zzz = fminf(z, zz) + hcrt;
-                  const float yyy = fminf(z, ZX(i + IP2[k], j + JP2[k])) + hcrt;
-                  const float xxx = fminf(z, ZX(i + IP1[k], j + JP1[k])) + hcrt;
-                  const float www = fminf(z, ZX(i + cIP[km1], j + cJP[km1])) + hcrt;
-                  const float hii = HX(ii, jj), hix = HX(ix, jx), ha1 = HX(i + IP1[k], j + JP1[k]);
-                  if (((hsub >= zzz && hii >= yyy) || (hix >= yyy && hii >= zzz)) ||
-                      ((hsub >= www && ha1 >= xxx) || (hix >= xxx && ha1 >= www)))
-                     maskix[k] = 1;
                }
             }
-            float hsm2 = 0.f, xsm2 = 0.f;
-            for (int km = 1; km <= 4; km++) {
-               const int kkm = km % 4 + 1, km1m = (km + 2) % 4 + 1;
-               const int ix = i + IIX[km], jx = j + JJX[km], ii = i + cIIH[km], jj = j + cJJH[km];
-               if (ii == 0 || jj == 0 || ix == 0 || jx == 0 || i + IIX[kkm] == 0 || j + JJX[kkm] == 0 ||
-                   i + IIX[km1m] == 0 || j + JJX[km1m] == 0 || i + cIIH[kkm] == 0 ||
-                   j + cJJH[kkm] == 0 || i + cIIH[km1m] == 0 || j + cJJH[km1m] == 0)
-                  continue;
-               hsm2 = hsm2 + (float) maskii[km] * HX(ii, jj) - (float) maskii[km] * hsub;
-               if (B.xver == 1) {
-                  const float w = (float) (maskix[km] * maskii[km] * maskix[kkm]);
-                  xsm2 = xsm2 + 0.5f * (w * HX(ix, jx) - w * hsub);
-               } else {
-                  xsm2 = xsm2 + 0.5f * 1.f * (float) maskix[km] * (float) maskii[km] *
-                                (float) maskii[km1m] * (HX(ix, jx) - hsub);
-               }
-            }
-            const float htemp = hsub + 1.f / 8.f * (hsm2 - 1.f * (xsm2));
-            out = fmaxf(-zb, htemp);
-#undef HX
-#undef ZX
+            hdif = hdif + hzz * gamf;
+         }
+         const float htemp = 0.5f * hsub + (hsm + hdif) / 8.f;
+         out = fmaxf(-zb, htemp);
+      }
+   } else if (!(hsub + zb < B.exclude)) {
+      const float hcrt = B.hcrt;
+#define HX(a, b) ((a) == B.imxb ? 0.f : ((b) == B.jmxb ? top[(a) - 1] : hs(a, b)))
+#define ZX(a, b) (z.m(a, b))
+      const int IIX[5] = {0, -1, 1, 1, -1}, JJX[5] = {0, -1, -1, 1, 1};
+      const int IP1[5] = {0, -1, 1, 2, 0}, JP1[5] = {0, 0, -1, 1, 2};
+      const int IP2[5] = {0, 0, 2, 1, -1}, JP2[5] = {0, -1, 0, 2, 1};
+      int maskii[5] = {0, 0, 0, 0, 0}, maskix[5] = {0, 0, 0, 0, 0};
+      for (int k = 1; k <= 4; k++) {
+         const int ii = i + cIIH[k], jj = j + cJJH[k], ix = i + IIX[k], jx = j + JJX[k];
+         if (ii == 0 || jj == 0 || ix == 0 || jx == 0) continue;
+         if (!(ii == B.imxb || jj == B.jmxb || HX(ii, jj) + z.b(ii, jj) < hcrt)) {
+            const int kk2 = k % 4 + 1;
+            const float zzz = fminf(ZX(i + cIP[k], j + cJP[k]), ZX(i + cIP[kk2], j + cJP[kk2])) + hcrt;
+            if (!(hsub < zzz && HX(ii, jj) < zzz)) maskii[k] = 1;
+         }
+         if (!(ix == B.imxb || jx == B.jmxb || HX(ix, jx) + z.b(ix, jx) < hcrt)) {
+            const int kk = k % 4 + 1, km1 = (k + 2) % 4 + 1;
+            const float z0 = ZX(i + cIP[k], j + cJP[k]), zz = ZX(i + cIP[kk], j + cJP[kk]);
+            const float zzz = fminf(z0, zz) + hcrt;
+            const float yyy = fminf(z0, ZX(i + IP2[k], j + JP2[k])) + hcrt;
+            const float xxx = fminf(z0, ZX(i + IP1[k], j + JP1[k])) + hcrt;
+            const float www = fminf(z0, ZX(i + cIP[km1], j + cJP[km1])) + hcrt;
+            const float hii = HX(ii, jj), hix = HX(ix, jx), ha1 = HX(i + IP1[k], j + JP1[k]);
+            if (((hsub >= zzz && hii >= yyy) || (hix >= yyy && hii >= zzz)) ||
+                ((hsub >= www && ha1 >= xxx) || (hix >= xxx && ha1 >= www)))
+               maskix[k] = 1;
          }
       }
+      float hsm2 = 0.f, xsm2 = 0.f;
+      for (int km = 1; km <= 4; km++) {
+         const int kkm = km % 4 + 1, km1m = (km + 2) % 4 + 1;
+         const int ix = i + IIX[km], jx = j + JJX[km], ii = i + cIIH[km], jj = j + cJJH[km];
+         if (ii == 0 || jj == 0 || ix == 0 || jx == 0 || i + IIX[kkm] == 0 || j + JJX[kkm] == 0 ||
+             i + IIX[km1m] == 0 || j + JJX[km1m] == 0 || i + cIIH[kkm] == 0 ||
+             j + cJJH[kkm] == 0 || i + cIIH[km1m] == 0 || j + cJJH[km1m] == 0)
+            continue;
+         hsm2 = hsm2 + (float) maskii[km] * HX(ii, jj) - (float) maskii[km] * hsub;
+         if (B.xver == 1) {
+            const float w = (float) (maskix[km] * maskii[km] * maskix[kkm]);
+            xsm2 = xsm2 + 0.5f * (w * HX(ix, jx) - w * hsub);
+         } else {
+            xsm2 = xsm2 + 0.5f * 1.f * (float) maskix[km] * (float) maskii[km] *
+                          (float) maskii[km1m] * (HX(ix, jx) - hsub);
+         }
+      }
+      const float htemp = hsub + 1.f / 8.f * (hsm2 - 1.f * (xsm2));
+      out = fmaxf(-zb, htemp);
+#undef HX
+#undef ZX
    }
-   hb[p] = out;
+   return out;
 }
 
 // ------------------------------------------------------------------ MOMNTM
@@ -679,15 +731,14 @@ __device__ static void frcpnt(const SlBasinDev &B, const float *P, const float *
    const float ccn = P[P_X15] + rsq;
    const float x1218 = (ncatg == 1) ? P[P_W0] : P[P_W1];
    float ck1, sk1;
+   const float *Rk = T + (uint32_t) k * 8, *Rk1 = Rk + 8;
    if (ncatg == 1) {
-      const float *CW = T + T_CW * SL_NTABV, *SW = T + T_SW * SL_NTABV;
-      ck1 = CW[k] + dr * (CW[k + 1] - CW[k]);
-      sk1 = SW[k] + dr * (SW[k + 1] - SW[k]);
+      ck1 = Rk[R_CW] + dr * (Rk1[R_CW] - Rk[R_CW]);
+      sk1 = Rk[R_SW] + dr * (Rk1[R_SW] - Rk[R_SW]);
    } else {
-      const float *C = T + T_C * SL_NTABV, *S = T + T_S * SL_NTABV;
       cshltr = 1.f;
-      ck1 = C[k] + dr * (C[k + 1] - C[k]);
-      sk1 = S[k] + dr * (S[k + 1] - S[k]);
+      ck1 = Rk[R_C] + dr * (Rk1[R_C] - Rk[R_C]);
+      sk1 = Rk[R_S] + dr * (Rk1[R_S] - Rk[R_S]);
    }
    float a = rs * P[P_X9] - x1218 * (yp * ck1 + xp * sk1);
    float b = rs * P[P_X10] + x1218 * (xp * ck1 - yp * sk1);
@@ -702,25 +753,25 @@ __device__ static void frcpnt(const SlBasinDev &B, const float *P, const float *
    float fctt = 3.E-6f * sqrtf(a * a + b * b);
    fctt = fctt * cshltr * P[P_BT];
    const float fxx = fctt * a, fyy = fctt * b;
-   const float cr = TABI(B, T_CR2, F.iik, F.id), ci = TABI(B, T_CI2, F.iik, F.id);
+   const float *E = EREC(B, F.iik, F.id);
+   const float cr = E[E_CR2], ci = E[E_CI2];
    fxb = cr * fxx - ci * fyy;
    fyb = cr * fyy + ci * fxx;
    if (npls != 0) {
-      const float *Pt = T + T_P * SL_NTABV;
-      const float aa = -F.dph * (Pt[k] + dr * (Pt[k + 1] - Pt[k]));
+      const float aa = -F.dph * (Rk[R_P] + dr * (Rk1[R_P] - Rk[R_P]));
       const float pxx = aa * xp, pyy = aa * yp;
-      const float br = TABI(B, T_BR2, F.iik, F.id), bi = TABI(B, T_BI2, F.iik, F.id);
+      const float br = E[E_BR2], bi = E[E_BI2];
       fxb = fxb + br * pxx - bi * pyy;
       fyb = fyb + br * pyy + bi * pxx;
    }
 }
 
 __device__ static void mntmbd(const SlBasinDev &B, const float *P, const float *T, float *ub, float *vb,
-                              int jump, int ijump, int isb, int jsb, float arj, float aii, size_t pij,
+                              int jump, int ijump, int isb, int jsb, float arj, float aii, uint32_t pij,
                               float ubj, float vbj, float &fxb, float &fyb, int ncatg, float &cshltr,
                               const Forcing &F)
 {
-   const size_t q = IDX(B, isb, jsb);
+   const uint32_t q = IDX(B, isb, jsb);
    const float aiul = aii * (ub[q] - ubj), aivb = aii * (vb[q] - vbj);
    if (jump == 2) {
       const float xttx = 0.f, xtty = 0.f;
@@ -753,22 +804,17 @@ __device__ static void mntmbd(const SlBasinDev &B, const float *P, const float *
    vb[q] = aii * vb[q] - arj * ub[q] + P[P_DELT] * fyb;
 }
 
-__global__ void __launch_bounds__(TX * TY) k_momntm(SlBasinDev B, SlRunDev R, StepArgs A)
+// Momentum point (i,j) with F_MOM set; hb gives the final HB of the step on the
+// four squares around it.  Writes UB/VB at (i,j) and at the boundary points
+// this point owns.
+template <class HB, class Z>
+__device__ __forceinline__ void momntm_pt(const SlBasinDev &B, const float *P, const float *T, uint32_t fl,
+                                          int i, int j, const HB &hb, const Z &z, float *ub, float *vb)
 {
-   const int i = blockIdx.x * TX + threadIdx.x + 1, j = blockIdx.y * TY + threadIdx.y + 1;
-   const int s = blockIdx.z;
-   if (i > B.imxb || j > B.jmxb) return;
-   const size_t p = IDX(B, i, j);
-   const uint32_t fl = B.flags[p];
-   if (!(fl & F_MOM)) return;
-   const size_t so = (size_t) s * R.fstride;
-   const float *hb = A.hc + so;
-   float *ub = R.ub + so, *vb = R.vb + so;
-   const float *P = par(R, s, A.step);
-   const float *T = tabv(R, s, P);
-   const size_t pa = p - B.pitch - 1, pb = p - B.pitch, pc = p - 1;   // (i-1,j-1) (i,j-1) (i-1,j)
-   const float h1 = hb[pa], h2 = hb[pb], h3 = hb[pc], h4 = hb[p];
-   const float zbm = B.zbm[p];
+   const uint32_t p = IDX(B, i, j);
+   const float h1 = hb(i - 1, j - 1), h2 = hb(i, j - 1), h3 = hb(i - 1, j), h4 = hb(i, j);
+   const float za = z.b(i - 1, j - 1), zb2 = z.b(i, j - 1), zc = z.b(i - 1, j), zp = z.b(i, j);
+   const float zbm = z.m(i, j);
    Forcing F;
    F.iik = (int) F_IIK(fl);
    const int ncatg = (fl & F_OCEAN) ? 2 : 1;
@@ -778,12 +824,13 @@ __global__ void __launch_bounds__(TX * TY) k_momntm(SlBasinDev B, SlRunDev R, St
    if (ncatg != 1 && (fl & F_IT2) && zbm < -50.f) {
       dhx = -h1 + h2 - h3 + h4;
       dhy = -h1 - h2 + h3 + h4;
-      F.dph = .25f * (h1 + h2 + h3 + h4 + B.zb[pa] + B.zb[pb] + B.zb[pc] + B.zb[p]);
+      F.dph = .25f * (h1 + h2 + h3 + h4 + za + zb2 + zc + zp);
       const float dphz = fminf(599.f, F.dph);
       F.id = (int) fminf(599.f, dphz);
       if (F.id <= 0) F.id = 1;
       const float dphi = (float) F.id;
-      aintrp = TABI(B, T_AI2, F.iik, F.id) + B.slpai[F.id] * (dphz - dphi);
+      const float *E = EREC(B, F.iik, F.id);
+      aintrp = E[E_AI2] + E[E_SLP] * (dphz - dphi);
    } else {
       const float hmn = fminf(fminf(fminf(h4, h3), h2), h1);
       if (hmn <= zbm) {
@@ -802,10 +849,10 @@ __global__ void __launch_bounds__(TX * TY) k_momntm(SlBasinDev B, SlRunDev R, St
       }
       if (!blocked) {
          float hpd[4];
-         hpd[0] = h1 + B.zb[pa];
-         hpd[1] = h2 + B.zb[pb];
-         hpd[2] = h3 + B.zb[pc];
-         hpd[3] = h4 + B.zb[p];
+         hpd[0] = h1 + za;
+         hpd[1] = h2 + zb2;
+         hpd[2] = h3 + zc;
+         hpd[3] = h4 + zp;
          F.dph = .25f * (hpd[0] + hpd[1] + hpd[2] + hpd[3]);
          F.id = (int) fminf(599.f, F.dph);
          if (F.dph <= 5.f) {
@@ -825,7 +872,8 @@ __global__ void __launch_bounds__(TX * TY) k_momntm(SlBasinDev B, SlRunDev R, St
             int idd = (int) fminf(599.f, dz);
             if (idd < 1) idd = 1;
             const float dphi = (float) idd;
-            const float aitr = TABI(B, T_AI2, F.iik, idd) + B.slpai[idd] * (dz - dphi);
+            const float *E = EREC(B, F.iik, idd);
+            const float aitr = E[E_AI2] + E[E_SLP] * (dz - dphi);
             aintrp = aintrp + aitr;
          }
          aintrp = .25f * aintrp;
@@ -838,7 +886,8 @@ __global__ void __launch_bounds__(TX * TY) k_momntm(SlBasinDev B, SlRunDev R, St
       vbj = 0.f;
    } else {
       aii = 1.f + P[P_X1B4] * aintrp;
-      arj = TABI(B, T_AR2, F.iik, F.id) * P[P_X1B4] * B.zcor[p];
+      const float *E = EREC(B, F.iik, F.id);
+      arj = E[E_AR2] * P[P_X1B4] * B.zcor[p];
       const float aiu = aii * ub[p], aiv = aii * vb[p];
       const float fct = P[P_X1B6] * F.dph;
       float xttx, xtty;
@@ -879,7 +928,7 @@ __global__ void __launch_bounds__(TX * TY) k_momntm(SlBasinDev B, SlRunDev R, St
       }
       ubj = ub[p];
       vbj = vb[p];
-      const float br = TABI(B, T_BR2, F.iik, F.id), bi = TABI(B, T_BI2, F.iik, F.id);
+      const float br = E[E_BR2], bi = E[E_BI2];
       ub[p] = aiu - fct * (br * dhx - bi * dhy) + arj * vb[p] + xttx;
       vb[p] = aiv - fct * (br * dhy + bi * dhx) - arj * ub[p] + xtty;
    }
@@ -889,9 +938,8 @@ __global__ void __launch_bounds__(TX * TY) k_momntm(SlBasinDev B, SlRunDev R, St
    const float uij = ub[p];
    int ijump = 1;
    if (bop <= 4) {
-      const int ISBS[5] = {0, 0, -1, 0, 1}, JSBS[5] = {0, -1, 0, 1, 0};
-      const int isb = i + ISBS[bop], jsb = j + JSBS[bop];
-      const size_t q = IDX(B, isb, jsb);
+      const int isb = i + cISBS[bop], jsb = j + cJSBS[bop];
+      const uint32_t q = IDX(B, isb, jsb);
       if (uij == 0.f) {
          ub[q] = 0.f;
          vb[q] = 0.f;
@@ -905,11 +953,10 @@ __global__ void __launch_bounds__(TX * TY) k_momntm(SlBasinDev B, SlRunDev R, St
       }
    } else {
       const int icor = (bop == 5) ? 1 : 3;
-      const int ishf[4][3] = {{0, 0, 0}, {1, 2, 4}, {0, 0, 0}, {1, 3, 4}};
       for (int kk = 0; kk < 3; kk++) {
-         const int k = ishf[icor][kk];
+         const int k = cISHF[icor][kk];
          const int isb = i + cIH[icor] + cIP[k], jsb = j + cJH[icor] + cJP[k];
-         const size_t q = IDX(B, isb, jsb);
+         const uint32_t q = IDX(B, isb, jsb);
          if (uij != 0.f) {
             const uint32_t fq = B.flags[q];
             if (fq & F_BSKIP) break;
@@ -924,24 +971,319 @@ __global__ void __launch_bounds__(TX * TY) k_momntm(SlBasinDev B, SlRunDev R, St
    }
 }
 
-// ------------------------------------------------------------------ launch
-// Kernel slots of one step, for per-kernel timing (SL_NKERN in sl_dev.h).
-extern "C" int sl_dev_launch_step(const SlBasinDev *B, const SlRunDev *R, int step, int smpt,
-                                  int filt, int parity, hipStream_t st, hipEvent_t *ev)
+// ----------------------------------------------------------- staged kernels
+// The counters feed the checks against the reference's list limits (the
+// 30000 / 8000 entries of smpt2g.f:31-38, 57-62, 78-96); one atomic per wave.
+__global__ void __launch_bounds__(TX * TY) k_pflag(SlBasinDev B, SlRunDev R, StepArgs A)
 {
-   dim3 blk(TX, TY), grd((B->imxb + TX - 1) / TX, (B->jmxb + TY - 1) / TY, R->nstorm);
+   const int i = blockIdx.x * TX + threadIdx.x + 1, j = blockIdx.y * TY + threadIdx.y + 1;
+   const int s = blockIdx.z;
+   const bool in = (i <= B.imxb && j <= B.jmxb);
+   const size_t so = (size_t) s * R.fstride;
+   uint8_t f = 0;
+   int nd = 0;
+   if (in) f = pflag_pt(B, i, j, GUZ{R.ub + so, B.pitch}, GF{A.hc + so, B.pitch}, ZG{B.zb, B.zbm, B.pitch}, &nd);
+   const int np = wave_sum(f != 0);
+   nd = wave_sum(nd);
+   if (threadIdx.x == 0) {
+      if (np) atomicAdd(&R.stat[s * 8 + 0], np);
+      if (nd) atomicAdd(&R.stat[s * 8 + 1], nd);
+   }
+   if (in) R.pflag[so + IDX(B, i, j)] = f;
+}
+
+__global__ void __launch_bounds__(TX * TY) k_hclass(SlBasinDev B, SlRunDev R, StepArgs A)
+{
+   const int i = blockIdx.x * TX + threadIdx.x + 1, j = blockIdx.y * TY + threadIdx.y + 1;
+   const int s = blockIdx.z;
+   const bool in = (i <= B.imxb && j <= B.jmxb);
+   const size_t so = (size_t) s * R.fstride;
+   uint8_t c = 0;
+   if (in) c = hclass_pt(B, i, j, GF{A.hc + so, B.pitch}, GB{R.pflag + so, B.pitch}, ZG{B.zb, B.zbm, B.pitch});
+   const int n1 = wave_sum(c == 1), n2 = wave_sum(c == 2);
+   if (threadIdx.x == 0) {
+      if (n1) atomicAdd(&R.stat[s * 8 + 2], n1);
+      if (n2) atomicAdd(&R.stat[s * 8 + 3], n2);
+   }
+   if (in) R.hclass[so + IDX(B, i, j)] = c;
+}
+
+// FLT2G + BDRYHT + the CONTTY row copy HB(I,JMXB)=HB(I,1) (contty.f:218-219).
+// hc = HB after CONTTY, hn = HB after TMSTPC1.
+__global__ void __launch_bounds__(TX * TY) k_smooth(SlBasinDev B, SlRunDev R, StepArgs A)
+{
+   const int i = blockIdx.x * TX + threadIdx.x + 1, j = blockIdx.y * TY + threadIdx.y + 1;
+   const int s = blockIdx.z;
+   if (i > B.imxb || j > B.jmxb) return;
+   const size_t so = (size_t) s * R.fstride, p = IDX(B, i, j);
+   const float *hs = A.hc + so;
+   const float *P = par(R, s, A.step);
+   if (j == B.jmxb) {
+      A.hn[so + p] = (i <= B.imxb - 1) ? hs[IDX(B, i, 1)] : hs[p];
+      return;
+   }
+   const bool smpt = __float_as_int(P[P_FLAGS]) & SF_SMPT;
+   A.hn[so + p] = smooth_pt(B, P, tabv(R, s, P), smpt, i, j, GF{hs, B.pitch}, GB{R.hclass + so, B.pitch},
+                            ZG{B.zb, B.zbm, B.pitch});
+}
+
+__global__ void __launch_bounds__(TX * TY) k_hmxsv(SlBasinDev B, SlRunDev R, StepArgs A)
+{
+   const int i = blockIdx.x * TX + threadIdx.x + 1, j = blockIdx.y * TY + threadIdx.y + 1;
+   const int s = blockIdx.z;
+   if (i > B.imxb - 1 || j > B.jmxb - 1) return;
+   const size_t so = (size_t) s * R.fstride, p = IDX(B, i, j);
+   const float hx = R.hmx[so + p];
+   const float v = hmxsv_pt(B, i, j, GF{A.hc + so, B.pitch}, hx, ZG{B.zb, B.zbm, B.pitch});
+   if (__float_as_uint(v) != __float_as_uint(hx)) R.hmx[so + p] = v;
+}
+
+__global__ void __launch_bounds__(TX * TY) k_filter(SlBasinDev B, SlRunDev R, StepArgs A)
+{
+   const int i = blockIdx.x * TX + threadIdx.x + 1, j = blockIdx.y * TY + threadIdx.y + 1;
+   const int s = blockIdx.z;
+   if (i > B.imxb || j > B.jmxb) return;
+   const size_t so = (size_t) s * R.fstride, p = IDX(B, i, j);
+   const float *hs = A.hc + so;
+   float out = hs[p];
+   if (i <= B.imxb - 1 && j <= B.jmxb - 1)
+      out = filter_pt(B, i, j, GF{hs, B.pitch}, R.hsubtop + (size_t) s * B.pitch, ZG{B.zb, B.zbm, B.pitch});
+   A.hn[so + p] = out;
+}
+
+__global__ void __launch_bounds__(TX * TY) k_momntm(SlBasinDev B, SlRunDev R, StepArgs A)
+{
+   const int i = blockIdx.x * TX + threadIdx.x + 1, j = blockIdx.y * TY + threadIdx.y + 1;
+   const int s = blockIdx.z;
+   if (i > B.imxb || j > B.jmxb) return;
+   const uint32_t fl = B.flags[IDX(B, i, j)];
+   if (!(fl & F_MOM)) return;
+   const size_t so = (size_t) s * R.fstride;
+   const float *P = par(R, s, A.step);
+   momntm_pt(B, P, tabv(R, s, P), fl, i, j, GF{A.hc + so, B.pitch}, ZG{B.zb, B.zbm, B.pitch}, R.ub + so, R.vb + so);
+}
+
+// ------------------------------------------------------------ fused stage B
+// One workgroup = one 64 x TH tile of one storm.  Regions (halo below / above
+// the tile, in points, each direction):
+//   F  final HB MOMNTM reads                 1 / 0
+//   S  HB after FLT2G+BDRYHT (HMXSV, filter) SLO / SHI  (2/2 with a filter, else 1/1)
+//   C  height classes FLT2G reads            S + 1
+//   PF momentum flags the classes read       C below, C + 1 above
+//   UZ U==0 mask the flags read              PF + 1
+//   HS HB after CONTTY                       PF + 1 below, PF above (S without SMPT2G)
+//   Z  ZB and ZBM                            S + 2 (the XFILTER masks reach two
+//                                            points up; the SMPT2G flags one more)
+// Every stage computes its whole region (points inside the grid), so each
+// point's inputs are in LDS; only the tile's own points are written back.
+template <int LO, int HI, int TH>
+struct Reg {
+   static constexpr int lo = LO, hi = HI, w = TX + LO + HI, h = TH + LO + HI, n = w * h;
+};
+
+template <int TH, bool SMPT, bool FILT>
+__global__ void __launch_bounds__(256) k_stepB(SlBasinDev B, SlRunDev R, StepArgs A, int ntx, int nty)
+{
+   constexpr int SLO = FILT ? 2 : 1, SHI = FILT ? 2 : 1;
+   using RS = Reg<SLO, SHI, TH>;
+   using RC = Reg<SLO + 1, SHI + 1, TH>;
+   using RP = Reg<SLO + 1, SHI + 2, TH>;
+   using RU = Reg<SLO + 2, SHI + 3, TH>;
+   using RH = Reg<SMPT ? SLO + 2 : SLO, SMPT ? SHI + 2 : SHI, TH>;
+   using RF = Reg<1, 0, TH>;
+   using RZ = Reg<SLO + 2, SHI + 2, TH>;
+   __shared__ float sh_hs[RH::n];
+   __shared__ float sh_s[RS::n];
+   __shared__ float sh_f[FILT ? RF::n : 1];
+   __shared__ uint8_t sh_uz[SMPT ? RU::n : 1];
+   __shared__ uint8_t sh_pf[SMPT ? RP::n : 1];
+   __shared__ uint8_t sh_cl[SMPT ? RC::n : 1];
+   __shared__ float sh_zb[RZ::n];
+   __shared__ float sh_zbm[RZ::n];
+   __shared__ int sh_cnt[4];
+
+   int tx, ty, s;
+   tile_of(blockIdx.x, gridDim.x, ntx, nty, tx, ty, s);
+   const int tid = threadIdx.x;
+   const int I0 = tx * TX + 1, J0 = ty * TH + 1;
+   const int IM = B.imxb, JM = B.jmxb;
+   const size_t so = (size_t) s * R.fstride;
+   const float *hs_g = A.hc + so;
+   const float *P = par(R, s, A.step);
+   const float *T = tabv(R, s, P);
+   if (tid < 4) sh_cnt[tid] = 0;
+
+   // HB after CONTTY (and the U==0 mask) with halo; the static ZB and ZBM
+   for (int e = tid; e < RH::n; e += 256) {
+      const int gi = I0 - RH::lo + e % RH::w, gj = J0 - RH::lo + e / RH::w;
+      sh_hs[e] = (gi >= 1 && gi <= IM && gj >= 1 && gj <= JM) ? hs_g[IDX(B, gi, gj)] : 0.f;
+   }
+   for (int e = tid; e < RZ::n; e += 256) {
+      const int gi = I0 - RZ::lo + e % RZ::w, gj = J0 - RZ::lo + e / RZ::w;
+      const bool in = gi >= 1 && gi <= IM && gj >= 1 && gj <= JM;
+      sh_zb[e] = in ? B.zb[IDX(B, gi, gj)] : 0.f;
+      sh_zbm[e] = in ? B.zbm[IDX(B, gi, gj)] : 0.f;
+   }
+   if (SMPT) {
+      const uint8_t *uz_g = R.uz + so;
+      for (int e = tid; e < RU::n; e += 256) {
+         const int gi = I0 - RU::lo + e % RU::w, gj = J0 - RU::lo + e / RU::w;
+         sh_uz[e] = (gi >= 1 && gi <= IM && gj >= 1 && gj <= JM) ? uz_g[IDX(B, gi, gj)] : 0;
+      }
+   }
+   __syncthreads();
+   const LF hsL{sh_hs, I0 - RH::lo, J0 - RH::lo, RH::w};
+   const LB clL{sh_cl, I0 - RC::lo, J0 - RC::lo, RC::w};
+   const ZL zL{sh_zb, sh_zbm, I0 - RZ::lo, J0 - RZ::lo, RZ::w};
+   if (SMPT) {
+      const LB uzL{sh_uz, I0 - RU::lo, J0 - RU::lo, RU::w};
+      int np = 0, ndt = 0;
+      for (int e = tid; e < RP::n; e += 256) {
+         const int gi = I0 - RP::lo + e % RP::w, gj = J0 - RP::lo + e / RP::w;
+         uint8_t f = 0;
+         if (gi >= 1 && gi <= IM && gj >= 1 && gj <= JM) {
+            int nd;
+            f = pflag_pt(B, gi, gj, uzL, hsL, zL, &nd);
+            if (gi >= I0 && gi < I0 + TX && gj >= J0 && gj < J0 + TH) {
+               np += f != 0;
+               ndt += nd;
+            }
+         }
+         sh_pf[e] = f;
+      }
+      np = wave_sum(np);
+      ndt = wave_sum(ndt);
+      if ((tid & 63) == 0) {
+         if (np) atomicAdd(&sh_cnt[0], np);
+         if (ndt) atomicAdd(&sh_cnt[1], ndt);
+      }
+      __syncthreads();
+      const LB pfL{sh_pf, I0 - RP::lo, J0 - RP::lo, RP::w};
+      int n1 = 0, n2 = 0;
+      for (int e = tid; e < RC::n; e += 256) {
+         const int gi = I0 - RC::lo + e % RC::w, gj = J0 - RC::lo + e / RC::w;
+         uint8_t c = 0;
+         if (gi >= 1 && gi <= IM && gj >= 1 && gj <= JM) {
+            c = hclass_pt(B, gi, gj, hsL, pfL, zL);
+            if (gi >= I0 && gi < I0 + TX && gj >= J0 && gj < J0 + TH) {
+               n1 += c == 1;
+               n2 += c == 2;
+            }
+         }
+         sh_cl[e] = c;
+      }
+      n1 = wave_sum(n1);
+      n2 = wave_sum(n2);
+      if ((tid & 63) == 0) {
+         if (n1) atomicAdd(&sh_cnt[2], n1);
+         if (n2) atomicAdd(&sh_cnt[3], n2);
+      }
+      __syncthreads();
+      if (tid < 4 && sh_cnt[tid]) atomicAdd(&R.stat[s * 8 + tid], sh_cnt[tid]);
+   }
+   // FLT2G + BDRYHT + row copy
+   for (int e = tid; e < RS::n; e += 256) {
+      const int gi = I0 - RS::lo + e % RS::w, gj = J0 - RS::lo + e / RS::w;
+      float v = 0.f;
+      if (gi >= 1 && gi <= IM && gj >= 1 && gj <= JM) {
+         if (gj == JM) v = (gi <= IM - 1) ? hs_g[IDX(B, gi, 1)] : hsL(gi, gj);
+         else v = smooth_pt(B, P, T, SMPT, gi, gj, hsL, clL, zL);
+      }
+      sh_s[e] = v;
+   }
+   __syncthreads();
+   const LF sL{sh_s, I0 - RS::lo, J0 - RS::lo, RS::w};
+   // HMXSV on the tile's own squares
+   float *hmx = R.hmx + so;
+   for (int e = tid; e < TX * TH; e += 256) {
+      const int gi = I0 + e % TX, gj = J0 + e / TX;
+      if (gi <= IM - 1 && gj <= JM - 1) {
+         const uint32_t p = IDX(B, gi, gj);
+         const float hx = hmx[p];
+         const float v = hmxsv_pt(B, gi, gj, sL, hx, zL);
+         if (__float_as_uint(v) != __float_as_uint(hx)) hmx[p] = v;
+      }
+   }
+   float *hn_g = A.hn + so;
+   if (FILT) {
+      const float *top = R.hsubtop + (size_t) s * B.pitch;
+      for (int e = tid; e < RF::n; e += 256) {
+         const int gi = I0 - RF::lo + e % RF::w, gj = J0 - RF::lo + e / RF::w;
+         float v = 0.f;
+         if (gi >= 1 && gi <= IM && gj >= 1 && gj <= JM)
+            v = (gi <= IM - 1 && gj <= JM - 1) ? filter_pt(B, gi, gj, sL, top, zL) : sL(gi, gj);
+         sh_f[e] = v;
+      }
+      __syncthreads();
+   }
+   const LF fL{FILT ? sh_f : sh_s, I0 - (FILT ? RF::lo : RS::lo), J0 - (FILT ? RF::lo : RS::lo),
+               FILT ? RF::w : RS::w};
+   float *ub = R.ub + so, *vb = R.vb + so;
+   for (int e = tid; e < TX * TH; e += 256) {
+      const int gi = I0 + e % TX, gj = J0 + e / TX;
+      if (gi <= IM && gj <= JM) {
+         hn_g[IDX(B, gi, gj)] = fL(gi, gj);
+         const uint32_t fl = B.flags[IDX(B, gi, gj)];
+         if (fl & F_MOM) momntm_pt(B, P, T, fl, gi, gj, fL, zL, ub, vb);
+      }
+   }
+}
+
+// ------------------------------------------------------------------ launch
+template <int TH>
+static void launch_B(int smpt, int filt, dim3 g, hipStream_t st, const SlBasinDev &B, const SlRunDev &R,
+                     StepArgs a, int ntx, int nty)
+{
+   if (smpt && filt) hipLaunchKernelGGL((k_stepB<TH, true, true>), g, dim3(256), 0, st, B, R, a, ntx, nty);
+   else if (smpt) hipLaunchKernelGGL((k_stepB<TH, true, false>), g, dim3(256), 0, st, B, R, a, ntx, nty);
+   else if (filt) hipLaunchKernelGGL((k_stepB<TH, false, true>), g, dim3(256), 0, st, B, R, a, ntx, nty);
+   else hipLaunchKernelGGL((k_stepB<TH, false, false>), g, dim3(256), 0, st, B, R, a, ntx, nty);
+}
+
+// One time step of every storm.  path 0 = fused (events: 0 before CONTTY,
+// 1 after it, 2 after the replay, 3 after stage B); path 1 = staged (events
+// around each of the SL_NKERN stage kernels).  Returns which HB buffer holds
+// the step's result.
+extern "C" int sl_dev_launch_step(const SlBasinDev *B, const SlRunDev *R, int step, int smpt,
+                                  int filt, int parity, int path, int bth, hipStream_t st,
+                                  hipEvent_t *ev)
+{
    float *h0 = parity ? R->hb1 : R->hb0, *h1 = parity ? R->hb0 : R->hb1;
    StepArgs a;
    a.step = step;
 #define MARK(k) \
    if (ev) (void) hipEventRecord(ev[k], st)
+   if (path == 0) {
+      const int ntx = (B->imxb + TX - 1) / TX;
+      int nty = (B->jmxb + ATH - 1) / ATH;
+      a.hc = h0;
+      a.hn = h1;
+      MARK(0);
+      hipLaunchKernelGGL(k_conttyA, dim3(ntx * nty * R->nstorm), dim3(TX, TY), 0, st, *B, *R, a, ntx, nty);
+      MARK(1);
+      hipLaunchKernelGGL(k_replay, dim3((R->nstorm + 63) / 64), dim3(64), 0, st, *B, *R, a, 1);
+      MARK(2);
+      a.hc = h1;
+      a.hn = h0;
+      const int th = (bth == 8 || bth == 32) ? bth : 16;
+      nty = (B->jmxb + th - 1) / th;
+      const dim3 g(ntx * nty * R->nstorm);
+      if (th == 8) launch_B<8>(smpt, filt, g, st, *B, *R, a, ntx, nty);
+      else if (th == 32) launch_B<32>(smpt, filt, g, st, *B, *R, a, ntx, nty);
+      else launch_B<16>(smpt, filt, g, st, *B, *R, a, ntx, nty);
+      MARK(3);
+      hipError_t e = hipGetLastError();
+      if (e != hipSuccess) return -(int) e;
+      return parity;
+   }
+   dim3 blk(TX, TY), grd((B->imxb + TX - 1) / TX, (B->jmxb + TY - 1) / TY, R->nstorm);
    // CONTTY: h0 (HSUB) -> h1
    a.hc = h0;
    a.hn = h1;
    MARK(0);
    hipLaunchKernelGGL(k_contty, grd, blk, 0, st, *B, *R, a);
    MARK(1);
-   hipLaunchKernelGGL(k_replay, dim3((R->nstorm + 63) / 64), dim3(64), 0, st, *B, *R, a);
+   hipLaunchKernelGGL(k_replay, dim3((R->nstorm + 63) / 64), dim3(64), 0, st, *B, *R, a, 0);
    MARK(2);
    // SMPT2G classification on h1
    a.hc = h1;

@@ -17,7 +17,8 @@ LIB_PATH = os.path.join(HERE, "libslosh_hip.so")
 _lib = None
 
 # kernel slots of one step (SLG_K_* in include/slosh_hip.h) and their symbols
-KERNELS = ["contty", "replay", "pflag", "hclass", "smooth", "hmxsv", "filter", "momntm"]
+KERNELS = ["contty", "replay", "pflag", "hclass", "smooth", "hmxsv", "filter", "momntm", "stepB"]
+PATH_FUSED, PATH_STAGED = 0, 1
 
 
 class SloshError(RuntimeError):
@@ -42,6 +43,7 @@ def lib():
         L.slg_batch_step.argtypes = [vp, i, cp, i]
         L.slg_batch_sync.argtypes = [vp, cp, i]
         L.slg_batch_step_timed.argtypes = [vp, i, vp, vp, cp, i]
+        L.slg_batch_set_path.argtypes = [vp, i, i]
         L.slg_batch_current_step.argtypes = [vp]
         L.slg_batch_get.argtypes = [vp, i, cp, vp, cp, i]
         L.slg_batch_stats.argtypes = [vp, i, vp, cp, i]
@@ -97,6 +99,10 @@ class Batch:
         if L.slg_export_storm(self.h, storm, rows_path.encode(), tables_path.encode()):
             raise SloshError("export_storm failed")
 
+    def set_path(self, path: int = PATH_FUSED, tile_rows: int = 16):
+        if lib().slg_batch_set_path(self.h, path, tile_rows) != 0:
+            raise SloshError("bad launch path %r / tile rows %r" % (path, tile_rows))
+
     def upload(self):
         err = ctypes.create_string_buffer(512)
         _chk(lib().slg_batch_upload(self.h, err, 512), err)
@@ -106,8 +112,9 @@ class Batch:
         _chk(lib().slg_batch_step(self.h, n, err, 512), err)
 
     def step_timed(self, n: int):
-        """n steps with HIP events around each kernel; returns (ms[8], launches[8])."""
-        ms = np.zeros(KERNELS.__len__(), dtype=np.float64)
+        """n steps with HIP events around each kernel; returns (ms, launches) per
+        kernel slot (KERNELS order)."""
+        ms = np.zeros(len(KERNELS), dtype=np.float64)
         cnt = np.zeros(len(KERNELS), dtype=np.int32)
         err = ctypes.create_string_buffer(512)
         _chk(lib().slg_batch_step_timed(self.h, n, ms.ctypes.data_as(ctypes.c_void_p),

@@ -182,17 +182,19 @@ def tmpdir(prefix="slg_"):
     return tempfile.mkdtemp(prefix=prefix)
 
 
-def gpu_vs_oracle(case, nsteps, checks, storms=None):
+def gpu_vs_oracle(case, nsteps, checks, storms=None, path=0, tile_rows=16):
     """Run one storm of a case on the GPU and the FP32 oracle side by side.
 
     Returns {step: {field: (gpu, oracle)}} at each checkpoint (plus the last
     step) and the oracle's drying-cascade event count.  ``storms`` lists extra
-    track files batched with the case's track (the case's own storm is 0)."""
+    track files batched with the case's track (the case's own storm is 0);
+    ``path``/``tile_rows`` select the launch path (sloshgpu.PATH_*)."""
     root = tmpdir("gvo_")
     rows, tabs, total = product_export(case, root)
     n = min(nsteps, total)
     B = sloshgpu.Basin(case["dta"], case["bntline"])
     X = sloshgpu.Batch(B, [case["trk"]] + list(storms or []))
+    X.set_path(path, tile_rows)
     X.upload()
     o = Oracle("32", root + "/init.bin")
     out, events = {}, 0

@@ -1,15 +1,17 @@
 """HIP time step (libslosh_hip.so, through its C ABI) against the FP32 oracle:
 bit-identical HB/UB/VB/HMX after every checkpoint, on the golden cases, on
 every smoothing/filter variant of the basin flags, on the three grid types and
-on the full polar 100x90 two-day run; and, over the horizon on which the FP32
-and FP64 trajectories stay on one branch, against the reference FP64 build's
-golden vectors to 0.01 ft."""
+on the full polar 100x90 two-day run, for both launch paths (fused LDS tiles
+of 8/16/32 rows, and one kernel per routine); and, over the horizon on which
+the FP32 and FP64 trajectories stay on one branch, against the reference FP64
+build's golden vectors to 0.01 ft."""
 import numpy as np
 import pytest
 
 import helpers
 from conftest import golden_case_files, load_golden, GOLDEN_CASES
 from helpers import synth
+import sloshgpu
 
 pytestmark = pytest.mark.gpu
 
@@ -23,13 +25,19 @@ def _assert_identical(out, label):
                 o.ravel()[bad[0]])
 
 
+PATHS = [(sloshgpu.PATH_FUSED, 16), (sloshgpu.PATH_FUSED, 8), (sloshgpu.PATH_FUSED, 32),
+         (sloshgpu.PATH_STAGED, 16)]
+PATH_IDS = ["fused16", "fused8", "fused32", "staged"]
+
+
+@pytest.mark.parametrize("path,rows", PATHS, ids=PATH_IDS)
 @pytest.mark.parametrize("name", GOLDEN_CASES)
-def test_golden_cases_bitwise(name, tmp_path):
+def test_golden_cases_bitwise(name, path, rows, tmp_path):
     g = load_golden(name)
     case = golden_case_files(g, str(tmp_path))
     checks = [int(c) for c in g["checks"]]
-    print("case", name, "steps", int(g["nsteps"][0]), flush=True)
-    out, _ = helpers.gpu_vs_oracle(case, int(g["nsteps"][0]), checks)
+    print("case", name, "steps", int(g["nsteps"][0]), "path", path, rows, flush=True)
+    out, _ = helpers.gpu_vs_oracle(case, int(g["nsteps"][0]), checks, path=path, tile_rows=rows)
     _assert_identical(out, name)
 
 
@@ -57,30 +65,33 @@ def test_golden_cases_vs_reference_fp64(name, tmp_path):
     B.close()
 
 
+@pytest.mark.parametrize("path", [sloshgpu.PATH_FUSED, sloshgpu.PATH_STAGED], ids=["fused", "staged"])
 @pytest.mark.parametrize("idx", range(len(helpers.VARIANTS)), ids=helpers.variant_id)
-def test_variants_bitwise(idx, tmp_path):
+def test_variants_bitwise(idx, path, tmp_path):
     case = helpers.variant_case(str(tmp_path), idx)
-    print("variant", helpers.variant_id(idx), flush=True)
-    out, _ = helpers.gpu_vs_oracle(case, 300, [1, 5, 31, 100, 200])
+    print("variant", helpers.variant_id(idx), "path", path, flush=True)
+    out, _ = helpers.gpu_vs_oracle(case, 300, [1, 5, 31, 100, 200], path=path)
     _assert_identical(out, helpers.variant_id(idx))
 
 
-def test_polar_100x90_two_days_bitwise(tmp_path):
+@pytest.mark.parametrize("path", [sloshgpu.PATH_FUSED, sloshgpu.PATH_STAGED], ids=["fused", "staged"])
+def test_polar_100x90_two_days_bitwise(path, tmp_path):
     """The reference's own CPU-runnable workload size: polar 100x90 basin,
     60 s step, 48 h (2880 steps) with barriers and drying."""
     b, t = synth.config1()
     case = helpers.make_case(str(tmp_path), b, t)
-    print("polar 100x90 2880 steps", flush=True)
-    out, events = helpers.gpu_vs_oracle(case, 2880, [1, 100, 720, 1440, 2160])
+    print("polar 100x90 2880 steps path", path, flush=True)
+    out, events = helpers.gpu_vs_oracle(case, 2880, [1, 100, 720, 1440, 2160], path=path)
     _assert_identical(out, "polar100x90")
     assert events > 1000          # the drying cascade is exercised
 
 
-def test_elliptic_250x300_prefix_bitwise(tmp_path):
+@pytest.mark.parametrize("rows", [16, 32])
+def test_elliptic_250x300_prefix_bitwise(rows, tmp_path):
     b, t = synth.config2()
     case = helpers.make_case(str(tmp_path), b, t)
-    print("elliptic 250x300 300 steps", flush=True)
-    out, _ = helpers.gpu_vs_oracle(case, 300, [1, 50, 150])
+    print("elliptic 250x300 300 steps rows", rows, flush=True)
+    out, _ = helpers.gpu_vs_oracle(case, 300, [1, 50, 150], tile_rows=rows)
     _assert_identical(out, "elliptic250x300")
 
 
@@ -113,3 +124,35 @@ def test_batch_member_equals_alone(tmp_path):
             assert np.array_equal(X.get(s, f), alone[s][f]), (s, f)
     X.close()
     B.close()
+
+
+def test_fused_equals_staged_ensemble(tmp_path):
+    """Sixteen MEOW storms on the 250x300 polar basin for 400 steps: the fused
+    path (LDS tiles, XCD-ordered workgroups) and the one-kernel-per-routine
+    path give identical fields for every storm."""
+    b = synth.config3_basin()
+    tracks = synth.meow_tracks(b, n=1024)[::64]
+    case = helpers.make_case(str(tmp_path), b, tracks[0])
+    paths = [case["trk"]]
+    for k, tr in enumerate(tracks[1:]):
+        p = str(tmp_path / ("e%d.trk" % k))
+        open(p, "w").write(synth.trk_text(tr))
+        paths.append(p)
+    B = sloshgpu.Basin(case["dta"], case["bntline"])
+    res = []
+    for path, rows in [(sloshgpu.PATH_STAGED, 16), (sloshgpu.PATH_FUSED, 16), (sloshgpu.PATH_FUSED, 32)]:
+        print("ensemble path", path, rows, flush=True)
+        X = sloshgpu.Batch(B, paths)
+        X.set_path(path, rows)
+        X.upload()
+        X.step(400)
+        X.sync()
+        res.append(([{f: X.get(s, f) for f in helpers.FIELDS} for s in range(len(paths))],
+                    [X.stats(s) for s in range(len(paths))]))
+        X.close()
+    B.close()
+    for r in res[1:]:
+        for s in range(len(paths)):
+            for f in helpers.FIELDS:
+                assert np.array_equal(r[0][s][f], res[0][0][s][f]), (s, f)
+            assert np.array_equal(r[1][s], res[0][1][s]), ("stats", s)
