@@ -38,6 +38,9 @@ int slg_basin_open(const char *dta_path, const char *bnt_line, slg_basin **out, 
 int slg_basin_set_ylt(slg_basin *b, const double *ylt);
 int slg_basin_dims(const slg_basin *b, int *imxb, int *jmxb, int *nbcpt);
 void slg_basin_close(slg_basin *b);
+/* Static field of the basin as double [jmxb][imxb]: "ZB", "ZBM" (COMMON /DUMB5/,
+ * depsfc.f:83-138) or "YLT" (/LLXFLE/). */
+int slg_basin_field(const slg_basin *b, const char *field, double *dst);
 
 /* Open nstorm runs on one basin, one trk file each (INPUTN2, STERON, INTVAL,
  * SETCMP, INTLHT: inputn2.f, steron.f, intval.f, setcmp.f, intlht.f).  The
@@ -75,8 +78,17 @@ int slg_batch_set_path(slg_batch *x, int path, int tile_rows);
 int slg_batch_step_timed(slg_batch *x, int n, double *ms, int *launches, char *err, int errlen);
 int slg_batch_current_step(const slg_batch *x);
 /* Copy one run's field to the host as double [jmxb][imxb]: "HB", "UB", "VB" or
- * "HMX" (COMMON /DUMB7/ and /DUMB10/, parm.for:39-45). */
+ * "HMX" (COMMON /DUMB7/ and /DUMB10/, parm.for:39-45); "HB0" the initial water
+ * (INTLHT, intlht.f:53-185); "HBS" HB of the last step after HMXSV and before
+ * FILTER, the state TMSTPC2 copies to the C driver's HHB (tmstpc2.f:70-85;
+ * kept on the staged path, and on either path when the step did not filter). */
 int slg_batch_get(slg_batch *x, int storm, const char *field, double *dst, char *err, int errlen);
+/* Values TMSTPM returns to the C driver after step n (tmstpm.f:59-70; 0 =
+ * after INITAL): ITIME, MHALT, DELT, storm lat, storm lon (XY2LLX), DELP,
+ * size, speed, direction.  Available for batches of up to 8 runs. */
+int slg_batch_track(const slg_batch *x, int storm, int n, double *out9);
+/* Run start date INITAL returns (inital.f:204-249): year, month, day, hour, min. */
+int slg_batch_start_date(const slg_batch *x, int storm, int *out5);
 /* Per-run counters since upload: [0] corner-smoothing momentum points, [1] dry
  * corners, [2] smoothing list 1, [3] smoothing list 2, [4] steps that replayed
  * a drying cascade. */

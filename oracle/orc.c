@@ -929,7 +929,7 @@ void orc_momntm(const orc_basin *b, const orc_storm *s, orc_state *st)
 }
 
 /* -------------------------------------------------------------------- step */
-int orc_step(const orc_basin *b, const orc_storm *s, orc_state *st, orc_diag *d)
+int orc_step(const orc_basin *b, const orc_storm *s, orc_state *st, orc_diag *d, REAL *hbs)
 {
    int incsm = 20, ismpt2g = 1;
    char e1 = b->ebsn1;
@@ -947,6 +947,8 @@ int orc_step(const orc_basin *b, const orc_storm *s, orc_state *st, orc_diag *d)
    orc_bdryht(b, s, st);
    /* TMSTPC2 */
    orc_hmxsv(b, s, st);
+   /* HB after HMXSV: what TMSTPC2 copies to the C driver's HHB (tmstpc2.f:70-85) */
+   if (hbs) memcpy(hbs, st->hb, sizeof(REAL) * (size_t) b->imxb * b->jmxb);
    if ((s->itime + 1) % incsm == 0 || e1 == '+') {
       int isl = (b->dollar[0] == '$' && b->dollar[1] == ' ');
       if (d) d->filtered = 1;

@@ -58,7 +58,7 @@ typedef struct {
    int filtered;
 } orc_diag;
 
-int orc_step(const orc_basin *b, const orc_storm *s, orc_state *st, orc_diag *d);
+int orc_step(const orc_basin *b, const orc_storm *s, orc_state *st, orc_diag *d, REAL *hbs);
 
 /* individual routines, for step-injection tests */
 void orc_contty(const orc_basin *b, const orc_storm *s, orc_state *st, orc_diag *d);

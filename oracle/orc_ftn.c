@@ -1,0 +1,259 @@
+/* orc_ftn.c -- TEST INFRASTRUCTURE.  The reference's Fortran entry points
+ * (INITAL, TMSTPC1, TMSTPC2, TMSTPM, CLNUP) and COMMON blocks served by the
+ * CPU oracle (liborc32 / liborc64), so that the reference's own C driver and
+ * CLI can be linked over the oracle exactly as they are linked over the GPU
+ * (slosh-test_amd/csrc/sl_ftn.cpp).  Two checks follow:
+ *   - over liborc64 the CLI must write the same env and rex bytes as the
+ *     reference FP64 build itself (pins this driver glue and the host
+ *     initialisation against the reference end to end);
+ *   - the GPU drop-in must write the same bytes as this file over liborc32.
+ * Initial state and the storm schedule come from the product's host
+ * restatement (exported in the state-dump record format); the time step is the
+ * oracle's.  Never part of the product.
+ *
+ * Entry-point semantics follow slosh2.c:24-150 / inital.f:204-272 /
+ * tmstpc2.f:70-85 / tmstpm.f:59-70 / clnup.f:13-24, as in sl_ftn.cpp. */
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <unistd.h>
+#include "../include/slosh_hip.h"
+
+#define BAS_X 999
+#define BAS_Y 1655
+#define NBCPTS 12000
+
+void *orc_open(const char *path);
+int orc_info(void *c, int *imxb, int *jmxb);
+int orc_set_tables(void *c, const double *tab);
+int orc_advance_hbs(void *c, const double *row, int *diag, double *hbs);
+int orc_get(void *c, const char *name, double *out);
+void orc_close(void *c);
+
+#pragma pack(push, 2)
+struct {
+   double ZB[BAS_Y][BAS_X], ZBM[BAS_Y][BAS_X], D_WAV[BAS_Y][BAS_X];
+} dumb5_;
+struct {
+   double UB[BAS_Y][BAS_X], VB[BAS_Y][BAS_X], HB[BAS_Y][BAS_X];
+} dumb7_;
+struct {
+   double HMX[BAS_Y][BAS_X];
+} dumb10_;
+struct {
+   double DELS, DELT, G, COR;
+} egth_;
+struct {
+   int IRANK, ISTAR, ISTOP, JSTAR, JSTOP, IHALO;
+} frank_;
+struct {
+   double TIDESH[NBCPTS];
+   int NBCPT, ISH[NBCPTS], JSH[NBCPTS];
+} bcpts_;
+struct {
+   double YLT[BAS_Y][BAS_X], YLG[BAS_Y][BAS_X];
+} llxfle_;
+#pragma pack(pop)
+
+static struct {
+   slg_basin *basin;
+   slg_batch *batch;
+   void *orc;
+   int imxb, jmxb, nsteps, step;
+   double *rows;            /* [nsteps+1][90] */
+   int ntab, *tab_step;     /* table sets and the step after which each applies */
+   double *tabs;            /* [ntab][6*800] in S C P DELP SW CW order */
+   double *buf, *hbs;
+} G;
+
+static void fatal(const char *what, const char *detail)
+{
+   fprintf(stderr, "orc_ftn: %s: %s\n", what, detail ? detail : "");
+   exit(1);
+}
+
+static void fstr(char *dst, const char *s, int len)
+{
+   int n = len < 1023 ? len : 1023;
+   memcpy(dst, s, (size_t) n);
+   dst[n] = 0;
+   while (n > 0 && (dst[n - 1] == ' ' || dst[n - 1] == 0)) dst[--n] = 0;
+}
+
+/* table sets from the exported record file: STEP (i), then S C P DELP SW CW (d, 800) */
+static void load_tables(const char *path)
+{
+   FILE *fp = fopen(path, "rb");
+   char hdr[20];
+   long n;
+   int cur = -1, slot = 0;
+   static const char *names[6] = {"S", "C", "P", "DELP", "SW", "CW"};
+   if (!fp) fatal("tables", path);
+   while (fread(hdr, 1, 20, fp) == 20 && fread(&n, 8, 1, fp) == 1) {
+      char name[17];
+      int esz = hdr[16] == 'd' ? 8 : 4, k;
+      void *data = malloc((size_t) n * esz);
+      memcpy(name, hdr, 16);
+      name[16] = 0;
+      if (fread(data, esz, (size_t) n, fp) != (size_t) n) fatal("tables", "short read");
+      if (!strcmp(name, "STEP")) {
+         G.tab_step = realloc(G.tab_step, sizeof(int) * (G.ntab + 1));
+         G.tabs = realloc(G.tabs, sizeof(double) * 4800 * (G.ntab + 1));
+         G.tab_step[G.ntab] = *(int *) data;
+         cur = G.ntab++;
+         slot = 0;
+      } else if (cur >= 0) {
+         for (k = 0; k < 6; k++)
+            if (!strcmp(name, names[k]) && n == 800) {
+               memcpy(G.tabs + (size_t) cur * 4800 + k * 800, data, 800 * sizeof(double));
+               slot++;
+            }
+      }
+      free(data);
+   }
+   fclose(fp);
+}
+
+int initalcommon_(void)
+{
+   return 0;
+}
+
+int inital_(int *mhalt, int *imxb, int *jmxb, double zb[BAS_Y][BAS_X], int *hr, int *mn, int *dy,
+            int *mo, int *yr, char *trk, int *len_trk, char *dta, int *len_dta, char *env, int *len_env,
+            char *ft40, int *len_ft40, double *delt, char *bsn, int *len_bsn, double *clock, short *wave,
+            short *full_depth)
+{
+   char err[512], dpath[1024], tpath[1024], dir[] = "/tmp/orcftnXXXXXX", f1[1100], f2[1100], f3[1100];
+   const char *trks[1];
+   int ni, nj, i, j, d[5];
+   double t[9];
+   FILE *fp;
+   long nb;
+   (void) env; (void) len_env; (void) ft40; (void) len_ft40; (void) bsn; (void) len_bsn;
+   if ((wave && *wave) || (full_depth && *full_depth)) fatal("unsupported option", "wave/fullDepth");
+   fstr(dpath, dta, *len_dta);
+   fstr(tpath, trk, *len_trk);
+   if (slg_basin_open(dpath, NULL, &G.basin, err, sizeof err)) fatal("basin", err);
+   slg_basin_dims(G.basin, &ni, &nj, NULL);
+   G.imxb = ni;
+   G.jmxb = nj;
+   G.buf = calloc((size_t) ni * nj, sizeof(double));
+   G.hbs = calloc((size_t) ni * nj, sizeof(double));
+   for (j = 0; j < nj; j++)
+      for (i = 0; i < ni; i++) G.buf[(size_t) j * ni + i] = llxfle_.YLT[j][i];
+   slg_basin_set_ylt(G.basin, G.buf);
+   trks[0] = tpath;
+   if (slg_batch_open(G.basin, 1, trks, &G.batch, err, sizeof err)) fatal("track", err);
+   G.nsteps = slg_batch_nsteps(G.batch, 0);
+   if (!mkdtemp(dir)) fatal("mkdtemp", dir);
+   snprintf(f1, sizeof f1, "%s/init.bin", dir);
+   snprintf(f2, sizeof f2, "%s/storm.bin", dir);
+   snprintf(f3, sizeof f3, "%s/tables.bin", dir);
+   if (slg_export_init(G.batch, 0, f1) || slg_export_storm(G.batch, 0, f2, f3)) fatal("export", dir);
+   G.orc = orc_open(f1);
+   if (!G.orc) fatal("oracle", f1);
+   fp = fopen(f2, "rb");
+   fseek(fp, 0, SEEK_END);
+   nb = ftell(fp);
+   fseek(fp, 0, SEEK_SET);
+   G.rows = malloc((size_t) nb);
+   if (fread(G.rows, 1, (size_t) nb, fp) != (size_t) nb) fatal("rows", f2);
+   fclose(fp);
+   load_tables(f3);
+   unlink(f1);
+   unlink(f2);
+   unlink(f3);
+   rmdir(dir);
+   G.step = 0;
+   *mhalt = G.nsteps;
+   *imxb = ni;
+   *jmxb = nj;
+   slg_basin_field(G.basin, "ZB", G.buf);
+   for (j = 0; j < nj; j++)
+      for (i = 0; i < ni; i++) zb[j][i] = dumb5_.ZB[j][i] = G.buf[(size_t) j * ni + i];
+   slg_basin_field(G.basin, "ZBM", G.buf);
+   for (j = 0; j < nj; j++)
+      for (i = 0; i < ni; i++) dumb5_.ZBM[j][i] = G.buf[(size_t) j * ni + i];
+   orc_get(G.orc, "HB", G.buf);
+   for (j = 0; j < nj; j++)
+      for (i = 0; i < ni; i++) dumb7_.HB[j][i] = G.buf[(size_t) j * ni + i];
+   slg_batch_start_date(G.batch, 0, d);
+   *yr = d[0];
+   *mo = d[1];
+   *dy = d[2];
+   *hr = d[3];
+   *mn = d[4];
+   slg_batch_track(G.batch, 0, 0, t);
+   *delt = t[2];
+   *clock = 0.;
+   egth_.DELS = 5280.;
+   egth_.DELT = t[2];
+   egth_.G = 32.2;
+   return 0;
+}
+
+int tmstpc1_(double hb[BAS_Y][BAS_X], short *fpass, short *restart)
+{
+   (void) hb; (void) fpass;
+   if (restart && *restart) fatal("unsupported option", "restart");
+   return 0;
+}
+
+int tmstpc2_(double hb[BAS_Y][BAS_X], short *fpass, short *restart)
+{
+   int k = G.step, n, i, j, diag[8];
+   (void) restart;
+   for (n = 0; n < G.ntab; n++)
+      if (G.tab_step[n] == k) orc_set_tables(G.orc, G.tabs + (size_t) n * 4800);
+   if (orc_advance_hbs(G.orc, G.rows + (size_t) k * 90, diag, G.hbs)) fatal("oracle step", "");
+   G.step++;
+   if (fpass && *fpass) {
+      orc_get(G.orc, "HMX", G.buf);
+      for (j = 0; j < G.jmxb; j++)
+         for (i = 0; i < G.imxb; i++) {
+            const double h = G.hbs[(size_t) j * G.imxb + i], m = G.buf[(size_t) j * G.imxb + i];
+            hb[j][i] = (m < h) ? m : h;
+         }
+   }
+   return 0;
+}
+
+int tmstpm_(int *itime, int *mhalt, double *delt, double *lat, double *lon, double *delp, double *size,
+            double *wspd, double *wdir, double *clock, short *restart)
+{
+   double t[9];
+   (void) clock; (void) restart;
+   if (slg_batch_track(G.batch, 0, G.step, t)) fatal("storm schedule", "");
+   *itime = (int) t[0];
+   *mhalt = (int) t[1];
+   *delt = t[2];
+   *lat = t[3];
+   *lon = t[4];
+   *delp = t[5];
+   *size = t[6];
+   *wspd = t[7];
+   *wdir = t[8];
+   egth_.DELT = t[2];
+   return 0;
+}
+
+int clnup_(double hb[BAS_Y][BAS_X], int *f_save_env)
+{
+   int i, j;
+   if (f_save_env && *f_save_env) fatal("unsupported option", "ARCHIV");
+   orc_get(G.orc, "HMX", G.buf);
+   for (j = 0; j < G.jmxb; j++)
+      for (i = 0; i < G.imxb; i++) hb[j][i] = dumb10_.HMX[j][i] = G.buf[(size_t) j * G.imxb + i];
+   orc_close(G.orc);
+   slg_batch_close(G.batch);
+   slg_basin_close(G.basin);
+   return 0;
+}
+
+int staticht_(int *i, int *j, double *a) { (void) i; (void) j; (void) a; fatal("unsupported", "STATICHT"); return 0; }
+int issubgrid_(int *i, int *j, int *a) { (void) i; (void) j; (void) a; fatal("unsupported", "ISSUBGRID"); return 0; }
+int tmstp2_(void) { fatal("unsupported", "TMSTP2"); return 0; }
+int intlhtsub_(void) { fatal("unsupported", "INTLHTSUB"); return 0; }
+int intlhtadd_(void) { fatal("unsupported", "INTLHTADD"); return 0; }
+int cln_hs_(void) { return 0; }   /* deletes hot-start files; none are written */

@@ -250,14 +250,22 @@ static void make_storm(ctx_t *c, const double *row, orc_storm *s)
    s->CW = c->CW;
 }
 
-/* one full step; diag[0..4] = drying events, smoothing pts, special, overflow, filtered */
-int orc_advance(void *vc, const double *row, int *diag)
+/* one full step; diag[0..4] = drying events, smoothing pts, special, overflow,
+ * filtered.  If hbs is not NULL it receives HB after HMXSV (before FILTER) as
+ * double [jmxb][imxb]. */
+int orc_advance_hbs(void *vc, const double *row, int *diag, double *hbs)
 {
    ctx_t *c = vc;
    orc_storm s;
    int rc;
+   size_t k, n = (size_t) c->b.imxb * c->b.jmxb;
+   REAL *tmp = hbs ? malloc(sizeof(REAL) * n) : NULL;
    make_storm(c, row, &s);
-   rc = orc_step(&c->b, &s, &c->st, &c->diag);
+   rc = orc_step(&c->b, &s, &c->st, &c->diag, tmp);
+   if (hbs) {
+      for (k = 0; k < n; k++) hbs[k] = (double) tmp[k];
+      free(tmp);
+   }
    if (diag) {
       diag[0] = c->diag.drying_events;
       diag[1] = c->diag.smooth_pts;
@@ -266,6 +274,11 @@ int orc_advance(void *vc, const double *row, int *diag)
       diag[4] = c->diag.filtered;
    }
    return rc;
+}
+
+int orc_advance(void *vc, const double *row, int *diag)
+{
+   return orc_advance_hbs(vc, row, diag, NULL);
 }
 
 /* single routine, for step-injection tests */

@@ -67,6 +67,7 @@ struct slg_batch {
    slg_basin *b;
    int nstorm, maxsteps, step, parity;
    int path, tile_rows;                // launch path (SLG_PATH_*) and stage-B tile height
+   int hbs_buf;                        // buffer holding HB before FILTER of the last step (-1: none)
    std::vector<storm_sched> sc;
    std::vector<double> hb0;            // initial HB per storm (double, imxb*jmxb)
    void *dmem;
@@ -497,6 +498,7 @@ extern "C" int slg_batch_open(slg_basin *B, int nstorm, const char *const *trk, 
    X->parity = 0;
    X->path = SLG_PATH_FUSED;
    X->tile_rows = 16;
+   X->hbs_buf = -1;
    *out = X;
    return 0;
 }
@@ -619,6 +621,8 @@ extern "C" int slg_batch_step(slg_batch *X, int n, char *err, int errlen)
          seterr(err, errlen, "kernel launch failed (%s%d)", "", -rc);
          return SLG_EHIP;
       }
+      // staged: FLT2G/HMXSV leave HB in the entering buffer, FILTER writes the other
+      X->hbs_buf = !filt ? rc : (X->path == SLG_PATH_STAGED ? X->parity : -1);
       X->parity = rc;
       X->step = step;
    }
@@ -647,6 +651,7 @@ extern "C" int slg_batch_step_timed(slg_batch *X, int n, double *ms, int *launch
          seterr(err, errlen, "kernel launch failed (%s%d)", "", -rc);
          return SLG_EHIP;
       }
+      X->hbs_buf = !filt ? rc : (X->path == SLG_PATH_STAGED ? X->parity : -1);
       X->parity = rc;
       X->step = step;
       if (X->path == SLG_PATH_FUSED) {
@@ -685,6 +690,90 @@ extern "C" int slg_batch_current_step(const slg_batch *X)
    return X->step;
 }
 
+// Static basin field as double [jmxb][imxb] ("ZB", "ZBM", "YLT").
+extern "C" int slg_basin_field(const slg_basin *B, const char *name, double *dst)
+{
+   const sl_basin *b = &B->h;
+   const double *src = !strcmp(name, "ZB") ? b->zb : !strcmp(name, "ZBM") ? b->zbm
+                     : !strcmp(name, "YLT") ? b->ylt : nullptr;
+   if (!src) return SLG_EINPUT;
+   memcpy(dst, src, sizeof(double) * (size_t) b->imxb * b->jmxb);
+   return 0;
+}
+
+// Values TMSTPM hands back to the C driver after step n (tmstpm.f:59-70):
+// ITIME, MHALT, DELT, storm latitude and longitude (XY2LLX of (C1+AX, C2+AY)),
+// DELP (YDELP), size (YC24), speed (C21), direction (AZMTH - C22 in degrees).
+// Step 0 gives the state after initialisation.  Kept for batches of <= 8 runs.
+extern "C" int slg_batch_track(const slg_batch *X, int storm, int n, double *out9)
+{
+   if (storm < 0 || storm >= X->nstorm) return SLG_EINPUT;
+   const storm_sched &S = X->sc[storm];
+   if (n < 0 || n > S.nsteps || S.rows.size() < (size_t) (n + 1) * 90) return SLG_EINPUT;
+   const double *r = &S.rows[(size_t) n * 90];
+   sl_storm t = S.s0;
+   t.c1 = r[3];
+   t.c2 = r[4];
+   t.ax = r[5];
+   t.ay = r[6];
+   double lat, lon;
+   sl_storm_latlon(&t, &X->b->h, &lat, &lon);
+   out9[0] = r[0];
+   out9[1] = r[23];
+   out9[2] = r[2];
+   out9[3] = lat;
+   out9[4] = lon;
+   out9[5] = r[9];
+   out9[6] = r[12];
+   out9[7] = r[7];
+   out9[8] = X->b->h.azmth - r[8] * 180. / 3.14159265358979323846;
+   return 0;
+}
+
+// Start of the run as INITAL hands it to the C driver (inital.f:204-249): the
+// trk DateTime moved back by JHR - IBGNT hours.  out5 = year, month, day, hour,
+// minute.
+extern "C" int slg_batch_start_date(const slg_batch *X, int storm, int *out5)
+{
+   if (storm < 0 || storm >= X->nstorm) return SLG_EINPUT;
+   const sl_storm &s = X->sc[storm].s0;
+   int ndy[13] = {0, 31, 28, 31, 30, 31, 30, 31, 31, 30, 31, 30, 31};
+   int myr = s.date_yr, mbnt = s.date_mo, mday = s.date_dy, mhr = s.date_hr;
+   if (myr % 4 == 0) ndy[2] = 29;
+   if (myr % 100 == 0) ndy[2] = 28;
+   if (myr % 400 == 0) ndy[2] = 29;
+   const int ntmlng = s.jhr - s.ibgnt;
+   int mbhr = mhr - ntmlng;
+   int mbdy = mday + mbhr / 24;   // Fortran integer division truncates toward zero
+   mbhr = mbhr % 24;
+   if (mbhr < 0) {
+      mbdy = mbdy - 1;
+      mbhr = 24 + mbhr;
+   }
+   if (mbdy < 1) {
+      mbnt = mbnt - 1;
+      if (mbnt < 1) {
+         myr = myr - 1;
+         mbnt = mbnt + 12;
+      }
+      mbdy = ndy[mbnt] + mbdy;
+   }
+   if (mbdy > ndy[mbnt]) {
+      mbdy = mbdy - ndy[mbnt];
+      mbnt = mbnt + 1;
+      if (mbnt > 12) {
+         myr = myr + 1;
+         mbnt = mbnt - 12;
+      }
+   }
+   out5[0] = myr;
+   out5[1] = mbnt;
+   out5[2] = mbdy;
+   out5[3] = mbhr;
+   out5[4] = s.date_mn;
+   return 0;
+}
+
 // download a field of one storm as double [jmxb][imxb]
 extern "C" int slg_batch_get(slg_batch *X, int storm, const char *name, double *dst, char *err,
                              int errlen)
@@ -692,7 +781,23 @@ extern "C" int slg_batch_get(slg_batch *X, int storm, const char *name, double *
    const sl_basin *b = &X->b->h;
    const SlRunDev &R = X->dev;
    const float *src;
+   if (storm < 0 || storm >= X->nstorm) {
+      seterr(err, errlen, "storm %s%d out of range", "", storm);
+      return SLG_EINPUT;
+   }
+   if (!strcmp(name, "HB0")) {   // initial water (INTLHT), double, kept on the host
+      const size_t np = (size_t) b->imxb * b->jmxb;
+      memcpy(dst, &X->hb0[(size_t) storm * np], np * sizeof(double));
+      return 0;
+   }
    if (!strcmp(name, "HB")) src = X->parity ? R.hb1 : R.hb0;
+   else if (!strcmp(name, "HBS")) {
+      if (X->hbs_buf < 0) {
+         seterr(err, errlen, "%s", "HB before FILTER is kept only by the staged path");
+         return SLG_EINPUT;
+      }
+      src = X->hbs_buf ? R.hb1 : R.hb0;
+   }
    else if (!strcmp(name, "UB")) src = R.ub;
    else if (!strcmp(name, "VB")) src = R.vb;
    else if (!strcmp(name, "HMX")) src = R.hmx;

@@ -1,0 +1,276 @@
+// sl_ftn.cpp -- the reference's Fortran entry points and COMMON blocks, served
+// by the HIP time step (libslosh_ftn.so).
+//
+// The reference's C driver (slosh2.c: RunInit / RunLoopStep / RunCleanup, and
+// the CLI in cstart.c) calls the Fortran model through INITAL, TMSTPC1, TMSTPC2,
+// TMSTPM and CLNUP and reads a few COMMON blocks directly (slosh2.c:24-303,
+// savellx.c:686-700).  Linking that driver against this library instead of
+// the Fortran objects runs the same CLI -- dta/bnt/trk in, env and rex files
+// out, written by the driver's own EnvSave / RexSaveStep -- with the time step
+// on the GPU.  The ABI is the reference's default build (-DDOUBLE_FORTRAN):
+// double arrays [BAS_Y][BAS_X] = [1655][999], #pragma pack(2) structs.
+//
+// Coherence with the C driver (what it reads and when):
+//   DUMB5.ZB   after INITAL                      -> filled by inital_
+//   DUMB7.HB   on the first step (mask init)     -> initial water, by inital_
+//   hb (HHB)   after TMSTPC2 when FPASS          -> min(HB, HMX) after HMXSV
+//   hb         after CLNUP                       -> HMX (the envelope)
+//   EGTH.DELT  on tide-only steps                -> kept current by tmstpm_
+//   LLXFLE     written by memSetLLx before INITAL -> read by inital_ (Coriolis)
+// Paths this build does not model (tides, hot start, waves, full depth) stop
+// the run with a message, as the reference's Fortran STOPs on fatal input.
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <string>
+#include <vector>
+#include "../../include/slosh_hip.h"
+#include "../../include/slosh_ftn.h"
+
+#define BAS_X 999
+#define BAS_Y 1655
+#define NBCPTS 12000
+
+// COMMON blocks (parm.for; C views slosh2.c:175-303, savellx.c:686-700)
+#pragma pack(push, 2)
+extern "C" {
+struct {
+   double ZB[BAS_Y][BAS_X], ZBM[BAS_Y][BAS_X], D_WAV[BAS_Y][BAS_X];
+} dumb5_;
+struct {
+   double UB[BAS_Y][BAS_X], VB[BAS_Y][BAS_X], HB[BAS_Y][BAS_X];
+} dumb7_;
+struct {
+   double HMX[BAS_Y][BAS_X];
+} dumb10_;
+struct {
+   double DELS, DELT, G, COR;
+} egth_;
+struct {
+   int IRANK, ISTAR, ISTOP, JSTAR, JSTOP, IHALO;
+} frank_;
+struct {
+   double TIDESH[NBCPTS];
+   int NBCPT, ISH[NBCPTS], JSH[NBCPTS];
+} bcpts_;
+struct {
+   double YLT[BAS_Y][BAS_X], YLG[BAS_Y][BAS_X];
+} llxfle_;
+}
+#pragma pack(pop)
+
+namespace {
+
+struct Run {
+   slg_basin *basin = nullptr;
+   slg_batch *batch = nullptr;
+   int imxb = 0, jmxb = 0, nsteps = 0, step = 0;
+   std::vector<double> buf;   // [jmxb][imxb] staging
+};
+Run g_run;
+
+[[noreturn]] void fatal(const char *what, const char *detail)
+{
+   fprintf(stderr, "slosh_ftn: %s%s%s\n", what, detail && *detail ? ": " : "", detail ? detail : "");
+   fflush(stderr);
+   exit(1);
+}
+
+std::string fstr(const char *s, int len)
+{
+   std::string r(s, s + (len > 0 ? len : 0));
+   while (!r.empty() && (r.back() == ' ' || r.back() == '\0')) r.pop_back();
+   return r;
+}
+
+void get(const char *field)
+{
+   char err[256];
+   if (slg_batch_get(g_run.batch, 0, field, g_run.buf.data(), err, sizeof err)) fatal("field copy failed", err);
+}
+
+}   // namespace
+
+extern "C" {
+
+int initalcommon_(void)
+{
+   return 0;   // the mirrors start zeroed
+}
+
+int inital_(int *mhalt, int *imxb, int *jmxb, double zb[BAS_Y][BAS_X], int *hr, int *mn, int *dy,
+            int *mo, int *yr, char *trk, int *len_trk, char *dta, int *len_dta, char *env, int *len_env,
+            char *ft40, int *len_ft40, double *delt, char *bsn, int *len_bsn, double *clock, short *wave,
+            short *full_depth)
+{
+   (void) env;
+   (void) len_env;
+   (void) ft40;
+   (void) len_ft40;
+   (void) bsn;
+   (void) len_bsn;
+   char err[512];
+   if (wave && *wave) fatal("the wave model is not part of this build", "");
+   if (full_depth && *full_depth) fatal("-fullDepth is not supported by this build", "");
+   const std::string dta_path = fstr(dta, *len_dta), trk_path = fstr(trk, *len_trk);
+   if (slg_basin_open(dta_path.c_str(), nullptr, &g_run.basin, err, sizeof err)) fatal("basin", err);
+   int ni, nj;
+   slg_basin_dims(g_run.basin, &ni, &nj, nullptr);
+   if (ni > BAS_X || nj > BAS_Y) fatal("basin larger than the compiled dimensions", dta_path.c_str());
+   g_run.imxb = ni;
+   g_run.jmxb = nj;
+   g_run.buf.assign((size_t) ni * nj, 0.);
+   // cell latitudes memSetLLx left in LLXFLE (savellx.c:752-762)
+   for (int j = 0; j < nj; j++)
+      for (int i = 0; i < ni; i++) g_run.buf[(size_t) j * ni + i] = llxfle_.YLT[j][i];
+   slg_basin_set_ylt(g_run.basin, g_run.buf.data());
+   const char *trks[1] = {trk_path.c_str()};
+   if (slg_batch_open(g_run.basin, 1, trks, &g_run.batch, err, sizeof err)) fatal("track", err);
+   if (slg_batch_upload(g_run.batch, err, sizeof err)) fatal("device", err);
+   g_run.nsteps = slg_batch_nsteps(g_run.batch, 0);
+   g_run.step = 0;
+   // outputs (inital.f:204-272)
+   *mhalt = g_run.nsteps;
+   *imxb = ni;
+   *jmxb = nj;
+   slg_basin_field(g_run.basin, "ZB", g_run.buf.data());
+   for (int j = 0; j < nj; j++)
+      for (int i = 0; i < ni; i++) {
+         zb[j][i] = g_run.buf[(size_t) j * ni + i];
+         dumb5_.ZB[j][i] = zb[j][i];
+      }
+   slg_basin_field(g_run.basin, "ZBM", g_run.buf.data());
+   for (int j = 0; j < nj; j++)
+      for (int i = 0; i < ni; i++) dumb5_.ZBM[j][i] = g_run.buf[(size_t) j * ni + i];
+   get("HB0");
+   for (int j = 0; j < nj; j++)
+      for (int i = 0; i < ni; i++) dumb7_.HB[j][i] = g_run.buf[(size_t) j * ni + i];
+   int d[5];
+   slg_batch_start_date(g_run.batch, 0, d);
+   *yr = d[0];
+   *mo = d[1];
+   *dy = d[2];
+   *hr = d[3];
+   *mn = d[4];
+   double t[9];
+   slg_batch_track(g_run.batch, 0, 0, t);
+   *delt = t[2];
+   *clock = 0.;   // DMCLOCK: no hot start
+   egth_.DELS = 5280.;
+   egth_.DELT = t[2];
+   egth_.G = 32.2;
+   return 0;
+}
+
+// TMSTPC1: the continuity stage runs with the rest of the step in tmstpc2_
+// (nothing the C driver reads changes in between).
+int tmstpc1_(double hb[BAS_Y][BAS_X], short *fpass, short *restart)
+{
+   (void) hb;
+   (void) fpass;
+   if (restart && *restart) fatal("restart files are not supported by this build", "");
+   if (!g_run.batch) fatal("TMSTPC1 before INITAL", "");
+   return 0;
+}
+
+// TMSTPC2: advance the GPU state through the whole step (CONTTY .. MOMNTM);
+// when FPASS, hand back HHB = min(HB, HMX) of HB after HMXSV (tmstpc2.f:70-85).
+int tmstpc2_(double hb[BAS_Y][BAS_X], short *fpass, short *restart)
+{
+   (void) restart;
+   char err[512];
+   const bool pass = fpass && *fpass;
+   // the staged path keeps HB before FILTER, which HHB needs on filter steps
+   slg_batch_set_path(g_run.batch, pass ? SLG_PATH_STAGED : SLG_PATH_FUSED, 16);
+   if (slg_batch_step(g_run.batch, 1, err, sizeof err)) fatal("step", err);
+   g_run.step++;
+   if (pass) {
+      const int ni = g_run.imxb, nj = g_run.jmxb;
+      std::vector<double> hmx((size_t) ni * nj);
+      if (slg_batch_get(g_run.batch, 0, "HMX", hmx.data(), err, sizeof err)) fatal("HMX", err);
+      get("HBS");
+      for (int j = 0; j < nj; j++)
+         for (int i = 0; i < ni; i++) {
+            const double h = g_run.buf[(size_t) j * ni + i], m = hmx[(size_t) j * ni + i];
+            hb[j][i] = (m < h) ? m : h;
+         }
+   }
+   return 0;
+}
+
+// TMSTPM: the momentum stage ran in tmstpc2_; return the storm values
+// (tmstpm.f:59-70) from the precomputed schedule.
+int tmstpm_(int *itime, int *mhalt, double *delt, double *lat, double *lon, double *delp, double *size,
+            double *wspd, double *wdir, double *clock, short *restart)
+{
+   (void) clock;
+   (void) restart;
+   double t[9];
+   if (slg_batch_track(g_run.batch, 0, g_run.step, t)) fatal("storm schedule", "");
+   *itime = (int) t[0];
+   *mhalt = (int) t[1];
+   *delt = t[2];
+   *lat = t[3];
+   *lon = t[4];
+   *delp = t[5];
+   *size = t[6];
+   *wspd = t[7];
+   *wdir = t[8];
+   egth_.DELT = t[2];
+   return 0;
+}
+
+// CLNUP: the envelope HMX to the driver (clnup.f:13-24); ARCHIV (iENV == 1) is
+// the legacy Fortran envelope file, which the C driver never asks for.
+int clnup_(double hb[BAS_Y][BAS_X], int *f_save_env)
+{
+   if (f_save_env && *f_save_env) fatal("the Fortran envelope archive (ARCHIV) is not supported", "");
+   get("HMX");
+   const int ni = g_run.imxb, nj = g_run.jmxb;
+   for (int j = 0; j < nj; j++)
+      for (int i = 0; i < ni; i++) {
+         hb[j][i] = g_run.buf[(size_t) j * ni + i];
+         dumb10_.HMX[j][i] = hb[j][i];
+      }
+   slg_batch_close(g_run.batch);
+   slg_basin_close(g_run.basin);
+   g_run.batch = nullptr;
+   g_run.basin = nullptr;
+   return 0;
+}
+
+// Tide and sub-grid entry points: reached only with -f_tide or a tide spin-up.
+int staticht_(int *i, int *j, double *ans)
+{
+   (void) i;
+   (void) j;
+   (void) ans;
+   fatal("tides (STATICHT) are not supported by this build", "");
+}
+int issubgrid_(int *i, int *j, int *ans)
+{
+   (void) i;
+   (void) j;
+   (void) ans;
+   fatal("tides (ISSUBGRID) are not supported by this build", "");
+}
+int tmstp2_(void)
+{
+   fatal("tide spin-up (TMSTP2) is not supported by this build", "");
+}
+int intlhtsub_(void)
+{
+   fatal("tides (INTLHTSUB) are not supported by this build", "");
+}
+int intlhtadd_(void)
+{
+   fatal("tides (INTLHTADD) are not supported by this build", "");
+}
+// CLN_HS (cln_hs.f) deletes the hot-start files; this build writes none.  The
+// driver calls it at the end of every run (slosh2.c:2771).
+int cln_hs_(void)
+{
+   return 0;
+}
+
+}   // extern "C"

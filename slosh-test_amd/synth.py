@@ -399,17 +399,18 @@ def write_case(root: str, b: Basin, t: Track, trk_name="storm.trk"):
     <root>/dta/<bsn>dta, <root>/bnt/[e|h]basins.dta, and the trk file."""
     os.makedirs(os.path.join(root, "dta"), exist_ok=True)
     os.makedirs(os.path.join(root, "bnt"), exist_ok=True)
-    dta = os.path.join(root, "dta", b.name.lower() + "dta")
+    # elliptic / hyperbolic basins go by 'e' / 'h' + name (setup.c:41-64)
+    pre = {" ": "", "$": "e", "+": "h"}[b.gtype]
+    dta = os.path.join(root, "dta", pre + b.name.lower() + "dta")
     with open(dta, "w") as fh:
         fh.write(dta_text(b))
-    pre = {" ": "", "$": "e", "+": "h"}[b.gtype]
     bnt = os.path.join(root, "bnt", pre + "basins.dta")
     with open(bnt, "w") as fh:
         fh.write(bnt_line(b) + "\n")
     trk = os.path.join(root, trk_name)
     with open(trk, "w") as fh:
         fh.write(trk_text(t))
-    return dict(dta=dta, bnt=bnt, trk=trk, bntline=bnt_line(b))
+    return dict(dta=dta, bnt=bnt, trk=trk, bntline=bnt_line(b), basin=pre + b.name.lower())
 
 
 # --------------------------------------------------------------------------------------

@@ -240,3 +240,72 @@ def variant_case(root, idx, ni=28, nj=24, hours=6):
     b.extra["ebsn1"] = e
     b.extra["ismooth"] = m
     return make_case(root, b, t)
+
+
+# ------------------------------------------------------------------ drop-in CLI
+REF_CLI = os.path.join(REPO, "oracle", "_ref", "slosh64")          # reference FP64 build
+ORC_CLI = {"64": os.path.join(REPO, "oracle", "_ref", "slosh_orc64"),  # its C driver over the oracle
+           "32": os.path.join(REPO, "oracle", "_ref", "slosh_orc32")}
+GPU_CLI = os.path.join(REPO, "oracle", "_ref", "slosh_gpu")         # its C driver over libslosh_ftn
+
+
+def _big_stack():
+    import resource
+    soft, hard = resource.getrlimit(resource.RLIMIT_STACK)
+    resource.setrlimit(resource.RLIMIT_STACK, (hard, hard))
+
+
+def run_cli(exe, case_root, basin, tag, timeout=900):
+    """Run a reference-style CLI (slosh -basin -rootDir -trk -rex -env) on a case
+    written by synth.write_case; returns (env_path, rex_path)."""
+    env = os.path.join(case_root, tag + ".env")
+    rex = os.path.join(case_root, tag + ".rex")
+    r = subprocess.run([exe, "-basin", basin, "-rootDir", case_root, "-trk",
+                        os.path.join(case_root, "storm.trk"), "-rex", rex, "-env", env, "-verbose", "0"],
+                       cwd=case_root, capture_output=True, text=True, timeout=timeout,
+                       preexec_fn=_big_stack)
+    if r.returncode != 0 or not os.path.exists(env):
+        raise RuntimeError("%s failed (%d): %s %s" % (exe, r.returncode, r.stdout[-800:], r.stderr[-800:]))
+    return env, rex
+
+
+def read_env(path):
+    """env file (EnvSave, slosh2.c:1611-1671): imxb, jmxb, 160-char comment,
+    imxb*jmxb int16 tenths of a foot (999 dry), then 2 floats."""
+    b = open(path, "rb").read()
+    imxb, jmxb = struct.unpack("<ii", b[:8])
+    comment = b[8:168].split(b"\0")[0]
+    vals = np.frombuffer(b, dtype="<i2", count=imxb * jmxb, offset=168).reshape(jmxb, imxb)
+    tail = b[168 + 2 * imxb * jmxb:]
+    return imxb, jmxb, comment, vals, tail
+
+
+def same_cli_outputs(env_a, rex_a, env_b, rex_b):
+    """Byte comparison of two runs' env and rex files, except bytes the
+    reference writes from uninitialised memory: the env comment after its
+    terminating NUL (slosh2.c:1643 writes 160 bytes of a shorter string) and the
+    100 track records RexSaveEnv copies after 'envl' (rex.c:57-120 parses a
+    2013-format trk with a space-separated pattern that never matches, leaving
+    the floats it writes unset).  Returns a list of differences (empty = same)."""
+    out = []
+    a, b = read_env(env_a), read_env(env_b)
+    if a[:3] != b[:3]:
+        out.append("env header")
+    if not np.array_equal(a[3], b[3]):
+        d = np.flatnonzero(a[3].ravel() != b[3].ravel())
+        out.append("env: %d cells differ, first %d: %d vs %d" % (d.size, d[0], a[3].ravel()[d[0]],
+                                                                  b[3].ravel()[d[0]]))
+    if a[4] != b[4]:
+        out.append("env datums")
+    ra, rb = open(rex_a, "rb").read(), open(rex_b, "rb").read()
+    ia, ib = ra.rfind(b"envl"), rb.rfind(b"envl")
+    if len(ra) != len(rb) or ia != ib:
+        out.append("rex size/layout %d/%d %d/%d" % (len(ra), len(rb), ia, ib))
+    else:
+        if ia >= 0:
+            ra = ra[:ia + 4] + ra[ia + 4 + 2400:]
+            rb = rb[:ib + 4] + rb[ib + 4 + 2400:]
+        if ra != rb:
+            d = next(k for k in range(len(ra)) if ra[k] != rb[k])
+            out.append("rex differs from byte %d" % d)
+    return out

@@ -1,0 +1,57 @@
+"""The reference's own C driver and CLI (cstart.c, slosh2.c, rex.c ...) linked
+over the FP64 oracle through the Fortran-ABI glue (oracle/orc_ftn.c) must
+write the same env and rex files as the reference FP64 build itself.  This
+pins, end to end through the reference's writers, the glue semantics the GPU
+drop-in (libslosh_ftn.so) shares: what INITAL returns (date, DELT, ZB, initial
+water), HHB = min(HB, HMX) after HMXSV on rex steps, the storm values TMSTPM
+returns, and the envelope CLNUP hands over.  Skipped where the reference
+build (oracle/_ref) is absent."""
+import os
+
+import pytest
+
+import helpers
+from helpers import synth
+
+pytestmark = pytest.mark.skipif(not (os.path.exists(helpers.REF_CLI) and os.path.exists(helpers.ORC_CLI["64"])),
+                                reason="oracle/_ref not built")
+
+
+def cli_cases():
+    """(id, basin, track) -- grid types, smoothing variants, a deepening storm."""
+    b, t = synth.tiny(seed=5)
+    yield "polar", b, t
+    for idx in (9, 18, 19, 20, 21):   # xfilter, ellip, ellip+filter, hyper, hyper+filter
+        g, e, m = helpers.VARIANTS[idx]
+        b, t = synth.tiny(seed=100 + idx, gtype=g, ni=28, nj=24, hours=6)
+        t.dp = [85.0] * len(t.dp)
+        b.extra["ebsn1"] = e
+        b.extra["ismooth"] = m
+        yield helpers.variant_id(idx), b, t
+    b, t = synth.tiny(seed=7, gtype="$", ni=30, nj=24, hours=8)
+    import numpy as np
+    t.dp = list(np.linspace(40.0, 80.0, len(t.dp)))
+    t.rmw = list(np.linspace(12.0, 20.0, len(t.dp)))
+    yield "ellip-deepening", b, t
+
+
+CASES = list(cli_cases())
+
+
+@pytest.mark.parametrize("cid,b,t", CASES, ids=[c[0] for c in CASES])
+def test_reference_cli_over_oracle64_matches_reference(cid, b, t, tmp_path):
+    root = str(tmp_path)
+    bsn = synth.write_case(root, b, t)["basin"]
+    env_r, rex_r = helpers.run_cli(helpers.REF_CLI, root, bsn, "ref")
+    env_o, rex_o = helpers.run_cli(helpers.ORC_CLI["64"], root, bsn, "orc")
+    assert helpers.same_cli_outputs(env_r, rex_r, env_o, rex_o) == []
+
+
+def test_polar_100x90_cli(tmp_path):
+    """The reference's CPU-runnable workload (polar 100x90, 48 h) through the CLI."""
+    b, t = synth.config1()
+    root = str(tmp_path)
+    bsn = synth.write_case(root, b, t)["basin"]
+    env_r, rex_r = helpers.run_cli(helpers.REF_CLI, root, bsn, "ref")
+    env_o, rex_o = helpers.run_cli(helpers.ORC_CLI["64"], root, bsn, "orc")
+    assert helpers.same_cli_outputs(env_r, rex_r, env_o, rex_o) == []
