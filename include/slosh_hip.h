@@ -47,6 +47,12 @@ int slg_basin_field(const slg_basin *b, const char *field, double *dst);
  * storm schedule of every run is computed on the host here. */
 int slg_batch_open(slg_basin *b, int nstorm, const char *const *trk_paths, slg_batch **out,
                    char *err, int errlen);
+/* As slg_batch_open, choosing the arithmetic of the time step: 32 (FP32) or 64
+ * (FP64, the precision of the reference's default build, -fdefault-real-8:
+ * Makefile:15-21; its results reproduce that build bit for bit). */
+int slg_batch_open_prec(slg_basin *b, int nstorm, const char *const *trk_paths, int precision,
+                        slg_batch **out, char *err, int errlen);
+int slg_batch_precision(const slg_batch *x);
 /* Steps of one run (MHALT after any time-step change, intval.f:68). */
 int slg_batch_nsteps(const slg_batch *x, int storm);
 /* Allocate device state, upload statics, schedules and initial water. */

@@ -19,9 +19,22 @@
 #include "sl_dev.h"
 #include "../../include/slosh_hip.h"
 
-extern "C" int sl_dev_launch_step(const SlBasinDev *B, const SlRunDev *R, int step, int smpt,
-                                  int filt, int parity, int path, int bth, hipStream_t st,
-                                  hipEvent_t *ev);
+extern "C" int sl_dev_launch_step_f32(const SlBasinDevT<float> *B, const SlRunDevT<float> *R, int step,
+                                      int smpt, int filt, int parity, int path, int bth, hipStream_t st,
+                                      hipEvent_t *ev);
+extern "C" int sl_dev_launch_step_f64(const SlBasinDevT<double> *B, const SlRunDevT<double> *R, int step,
+                                      int smpt, int filt, int parity, int path, int bth, hipStream_t st,
+                                      hipEvent_t *ev);
+static int dev_launch(const SlBasinDevT<float> *B, const SlRunDevT<float> *R, int step, int smpt, int filt,
+                      int parity, int path, int bth, hipStream_t st, hipEvent_t *ev)
+{
+   return sl_dev_launch_step_f32(B, R, step, smpt, filt, parity, path, bth, st, ev);
+}
+static int dev_launch(const SlBasinDevT<double> *B, const SlRunDevT<double> *R, int step, int smpt, int filt,
+                      int parity, int path, int bth, hipStream_t st, hipEvent_t *ev)
+{
+   return sl_dev_launch_step_f64(B, R, step, smpt, filt, parity, path, bth, st, ev);
+}
 
 #define PIX(b, i, j) ((size_t) ((j) - 1) * (size_t) (b)->imxb + (size_t) ((i) - 1))
 
@@ -39,46 +52,60 @@ static void seterr(char *err, int len, const char *fmt, const char *a = "", int 
       }                                                                                    \
    } while (0)
 
+// Host images of the device static data in one precision, and their device copy.
+template <class R>
+struct BasinImg {
+   std::vector<R> zb, zbm, zcor, vi[6], vj[5], tab, slpai, tab8, g4;
+   std::vector<uint32_t> flags;
+   bool built = false, on_dev = false;
+   void *dmem = nullptr;
+   SlBasinDevT<R> dev;
+};
+
 struct slg_basin {
    sl_basin h;
    char bnt[512];
    int pitch, jend;
-   // host float images of the device static data
-   std::vector<float> zb, zbm, zcor, vi[6], vj[5], tab, slpai, tab8, g4;
-   std::vector<uint32_t> flags;
    std::vector<int> is, ie, ms, me;
    int incsm, ismpt2g;
-   // device
-   bool on_dev;
-   void *dmem;
-   SlBasinDev dev;
+   BasinImg<float> i32;
+   BasinImg<double> i64;
 };
+static BasinImg<float> &img(slg_basin *B, float) { return B->i32; }
+static BasinImg<double> &img(slg_basin *B, double) { return B->i64; }
 
 struct storm_sched {
    sl_storm s0;                        // storm state after initialisation
    std::vector<double> rows;           // [nsteps+1][90] ref-format rows (kept for export)
-   std::vector<float> params;          // [nsteps][SL_NPAR] device values of each step
-   std::vector<float> tables;          // [ntab][SL_NTABV][8] (sl_dev.h)
+   // device values of each step ([nsteps][SL_NPAR]) and the table sets
+   // ([ntab][SL_NTABV][8], sl_dev.h), in the batch's precision
+   std::vector<float> p32, t32;
+   std::vector<double> p64, t64;
    std::vector<int> tab_step;          // step after which each table set took effect
    int nsteps;
 };
 
 struct slg_batch {
    slg_basin *b;
+   int prec;                           // 32 or 64
    int nstorm, maxsteps, step, parity;
    int path, tile_rows;                // launch path (SLG_PATH_*) and stage-B tile height
    int hbs_buf;                        // buffer holding HB before FILTER of the last step (-1: none)
    std::vector<storm_sched> sc;
    std::vector<double> hb0;            // initial HB per storm (double, imxb*jmxb)
    void *dmem;
-   SlRunDev dev;
+   SlRunDevT<float> d32;
+   SlRunDevT<double> d64;
    hipStream_t st;
    size_t bytes;
 };
+static SlRunDevT<float> &run(slg_batch *X, float) { return X->d32; }
+static SlRunDevT<double> &run(slg_batch *X, double) { return X->d64; }
 
 // --------------------------------------------------------------------------
 // basin
 
+template <class R>
 static uint32_t flag_bits(const slg_basin *B, int i, int j)
 {
    const sl_basin *b = &B->h;
@@ -92,7 +119,7 @@ static uint32_t flag_bits(const slg_basin *B, int i, int j)
    if (it == '8') f |= F_B8;
    if (i >= 2 && i <= b->imxb1 && j >= 2 && j <= b->jmxb1) f |= F_ZR;
    // SMPT2G candidate momentum point (smpt2g.f:23-30): interior and ZBM > -30
-   if (i >= 2 && i <= b->imxb - 2 && j >= 2 && j <= b->jmxb - 2 && (float) b->zbm[PIX(b, i, j)] > -30.f)
+   if (i >= 2 && i <= b->imxb - 2 && j >= 2 && j <= b->jmxb - 2 && (R) b->zbm[PIX(b, i, j)] > (R) -30.)
       f |= F_PFC;
    // IIK of FRCPNT/MOMNTM (frcpnt.f:75-118)
    {
@@ -120,126 +147,142 @@ static uint32_t flag_bits(const slg_basin *B, int i, int j)
    return f;
 }
 
-static void basin_images(slg_basin *B)
+// Bounds of the grid loops (shared by both precisions).
+static void basin_bounds(slg_basin *B)
 {
    sl_basin *b = &B->h;
-   const int P = B->pitch, NJ = b->jmxb;
-   size_t np = (size_t) P * NJ;
-   B->zb.assign(np, 0.f);
-   B->zbm.assign(np, 0.f);
-   B->zcor.assign(np, 0.f);
-   B->flags.assign(np, 0);
-   float corl[90];
-   for (int l = 0; l < 90; l++) corl[l] = (float) b->corl[l];
-   for (int j = 1; j <= NJ; j++)
-      for (int i = 1; i <= b->imxb; i++) {
-         size_t q = (size_t) (j - 1) * P + (i - 1);
-         B->zb[q] = (float) b->zb[PIX(b, i, j)];
-         B->zbm[q] = (float) b->zbm[PIX(b, i, j)];
-         B->flags[q] = flag_bits(B, i, j);
-         // Coriolis factor at the point, FP32 as momntm.f:294-295 evaluates it
-         float ylt = (float) b->ylt[PIX(b, i, j)];
-         int llat = (int) ylt;
-         if (llat >= 0 && llat <= 88)
-            B->zcor[q] = corl[llat] + (ylt - (float) llat) * (corl[llat + 1] - corl[llat]);
-      }
-   // BDRYHT squares (the ISH/JSH list DEPSFC builds, depsfc.f:376-390)
-   for (int n = 0; n < b->nbcpt; n++)
-      B->flags[(size_t) (b->jsh[n] - 1) * P + (b->ish[n] - 1)] |= F_BDRY;
-   double *vsrc[6] = {b->elpct, b->elpdt, b->elpcl, b->elpdl, b->elpcl2, b->elpdl2};
-   for (int k = 0; k < 6; k++) {
-      B->vi[k].assign(P, 0.f);
-      for (int i = 0; i < b->imxb; i++) B->vi[k][i] = (float) vsrc[k][i];
-   }
-   double *jsrc[5] = {b->sint, b->cost, b->sinl, b->cosl, b->sinl2};
-   for (int k = 0; k < 5; k++) {
-      B->vj[k].assign(NJ, 0.f);
-      for (int j = 0; j < NJ; j++) B->vj[k][j] = (float) jsrc[k][j];
-   }
+   const int NJ = b->jmxb;
    B->is.assign(b->is, b->is + NJ);
    B->ie.assign(b->ie, b->ie + NJ);
    B->ms.assign(b->ms, b->ms + NJ);
    B->me.assign(b->me, b->me + NJ);
+}
+
+// Static fields in precision R: every derived value is evaluated in R, as the
+// reference does in its REAL (double in the default build; float for the FP32
+// step, as the FP32 restatement evaluates them).
+template <class R>
+static void basin_images(slg_basin *B)
+{
+   sl_basin *b = &B->h;
+   BasinImg<R> &I = img(B, R());
+   const int P = B->pitch, NJ = b->jmxb;
+   size_t np = (size_t) P * NJ;
+   I.zb.assign(np, R(0));
+   I.zbm.assign(np, R(0));
+   I.zcor.assign(np, R(0));
+   I.flags.assign(np, 0);
+   R corl[90];
+   for (int l = 0; l < 90; l++) corl[l] = (R) b->corl[l];
+   for (int j = 1; j <= NJ; j++)
+      for (int i = 1; i <= b->imxb; i++) {
+         size_t q = (size_t) (j - 1) * P + (i - 1);
+         I.zb[q] = (R) b->zb[PIX(b, i, j)];
+         I.zbm[q] = (R) b->zbm[PIX(b, i, j)];
+         I.flags[q] = flag_bits<R>(B, i, j);
+         // Coriolis factor at the point as momntm.f:294-295 evaluates it
+         R ylt = (R) b->ylt[PIX(b, i, j)];
+         int llat = (int) ylt;
+         if (llat >= 0 && llat <= 88)
+            I.zcor[q] = corl[llat] + (ylt - (R) llat) * (corl[llat + 1] - corl[llat]);
+      }
+   // BDRYHT squares (the ISH/JSH list DEPSFC builds, depsfc.f:376-390)
+   for (int n = 0; n < b->nbcpt; n++)
+      I.flags[(size_t) (b->jsh[n] - 1) * P + (b->ish[n] - 1)] |= F_BDRY;
+   double *vsrc[6] = {b->elpct, b->elpdt, b->elpcl, b->elpdl, b->elpcl2, b->elpdl2};
+   for (int k = 0; k < 6; k++) {
+      I.vi[k].assign(P, R(0));
+      for (int i = 0; i < b->imxb; i++) I.vi[k][i] = (R) vsrc[k][i];
+   }
+   double *jsrc[5] = {b->sint, b->cost, b->sinl, b->cosl, b->sinl2};
+   for (int k = 0; k < 5; k++) {
+      I.vj[k].assign(NJ, R(0));
+      for (int j = 0; j < NJ; j++) I.vj[k][j] = (R) jsrc[k][j];
+   }
    double *tsrc[6] = {b->ar2, b->ai2, b->br2, b->bi2, b->cr2, b->ci2};
-   B->tab.assign((size_t) 6 * 245 * 600, 0.f);
+   I.tab.assign((size_t) 6 * 245 * 600, R(0));
    for (int t = 0; t < 6; t++)
       for (int ii = 1; ii <= 245; ii++)
          for (int n = 1; n <= 600; n++)
-            B->tab[(size_t) t * 147000 + (size_t) (ii - 1) * 600 + (n - 1)] =
-                (float) tsrc[t][(size_t) (n - 1) * 245 + (ii - 1)];
-   // SLPAI of momntm.f:81-89, in FP32 from the FP32 AI2 column
-   B->slpai.assign(601, 0.f);
+            I.tab[(size_t) t * 147000 + (size_t) (ii - 1) * 600 + (n - 1)] =
+                (R) tsrc[t][(size_t) (n - 1) * 245 + (ii - 1)];
+   // SLPAI of momntm.f:81-89, from the AI2 column in R
+   I.slpai.assign(601, R(0));
    int col = (b->iver == 201903) ? 4 : 1;
    for (int i = 1; i <= 599; i++)
-      B->slpai[i] = B->tab[(size_t) T_AI2_OFF + (size_t) (col - 1) * 600 + i] -
-                    B->tab[(size_t) T_AI2_OFF + (size_t) (col - 1) * 600 + (i - 1)];
-   B->slpai[600] = B->slpai[599];
-   // Device layout: one 32-byte record per (IIK, depth): AI2 SLPAI AR2 BR2 BI2 CR2 CI2 0
-   B->tab8.assign((size_t) 245 * 600 * 8, 0.f);
+      I.slpai[i] = I.tab[(size_t) T_AI2_OFF + (size_t) (col - 1) * 600 + i] -
+                   I.tab[(size_t) T_AI2_OFF + (size_t) (col - 1) * 600 + (i - 1)];
+   I.slpai[600] = I.slpai[599];
+   // Device layout: one record per (IIK, depth): AI2 SLPAI AR2 BR2 BI2 CR2 CI2 0
+   I.tab8.assign((size_t) 245 * 600 * 8, R(0));
    for (int ii = 1; ii <= 245; ii++)
       for (int n = 1; n <= 600; n++) {
-         float *e = &B->tab8[((size_t) (ii - 1) * 600 + (n - 1)) * 8];
+         R *e = &I.tab8[((size_t) (ii - 1) * 600 + (n - 1)) * 8];
          const size_t o = (size_t) (ii - 1) * 600 + (n - 1);
-         e[0] = B->tab[147000 * 1 + o];   // AI2
-         e[1] = B->slpai[n];
-         e[2] = B->tab[147000 * 0 + o];   // AR2
-         e[3] = B->tab[147000 * 2 + o];   // BR2
-         e[4] = B->tab[147000 * 3 + o];   // BI2
-         e[5] = B->tab[147000 * 4 + o];   // CR2
-         e[6] = B->tab[147000 * 5 + o];   // CI2
+         e[0] = I.tab[147000 * 1 + o];   // AI2
+         e[1] = I.slpai[n];
+         e[2] = I.tab[147000 * 0 + o];   // AR2
+         e[3] = I.tab[147000 * 2 + o];   // BR2
+         e[4] = I.tab[147000 * 3 + o];   // BI2
+         e[5] = I.tab[147000 * 4 + o];   // CR2
+         e[6] = I.tab[147000 * 5 + o];   // CI2
       }
    // Metric weights GAMF of the four neighbours of every square, as HMXSV
    // (hmxsv.f:84-92) and, on elliptic/hyperbolic grids, FLT2G and FILTER
-   // evaluate them: z = GAMFK/GAM0, GAMF = 2z/(1+z) - 1, in FP32 from the FP32
-   // metric images (the same operations the kernels would do every step).
-   B->g4.assign((size_t) 4 * np, 0.f);
+   // evaluate them: z = GAMFK/GAM0, GAMF = 2z/(1+z) - 1, in R from the metric
+   // images (the same operations the kernels would do every step).
+   I.g4.assign((size_t) 4 * np, R(0));
    {
-      const std::vector<float> &cl2 = B->vi[4], &dl2 = B->vi[5], &sl2 = B->vj[4];
+      const std::vector<R> &cl2 = I.vi[4], &dl2 = I.vi[5], &sl2 = I.vj[4];
       static const int iih[4] = {0, 1, 0, -1}, jjh[4] = {-1, 0, 1, 0};
       for (int j = 1; j <= NJ - 1; j++)
          for (int i = 1; i <= b->imxb - 1; i++) {
-            const float gam0 = dl2[i - 1] + (cl2[i - 1] - dl2[i - 1]) * sl2[j - 1];
+            const R gam0 = dl2[i - 1] + (cl2[i - 1] - dl2[i - 1]) * sl2[j - 1];
             for (int k = 0; k < 4; k++) {
                const int ii = i + iih[k], jj = j + jjh[k];
                if (ii == 0 || jj == 0) continue;
-               const float gamfk = dl2[ii - 1] + (cl2[ii - 1] - dl2[ii - 1]) * sl2[jj - 1];
-               const float z = gamfk / gam0;
-               B->g4[(size_t) k * np + (size_t) (j - 1) * P + (i - 1)] = 2.f * z / (1.f + z) - 1.f;
+               const R gamfk = dl2[ii - 1] + (cl2[ii - 1] - dl2[ii - 1]) * sl2[jj - 1];
+               const R z = gamfk / gam0;
+               I.g4[(size_t) k * np + (size_t) (j - 1) * P + (i - 1)] = R(2) * z / (R(1) + z) - R(1);
             }
          }
    }
+   I.built = true;
 }
 
+template <class R>
 static int basin_upload(slg_basin *B, char *err, int errlen)
 {
+   BasinImg<R> &I = img(B, R());
+   if (!I.built) basin_images<R>(B);
    sl_basin *b = &B->h;
    const int P = B->pitch, NJ = b->jmxb;
-   size_t np = (size_t) P * NJ;
+   const size_t np = (size_t) P * NJ, rs = sizeof(R);
    size_t off[32], tot = 0;
    int n = 0;
    auto add = [&](size_t bytes) {
       off[n++] = tot;
       tot += (bytes + 255) & ~(size_t) 255;
    };
-   add(np * 4);   // zb
-   add(np * 4);   // zbm
-   add(np * 4);   // zcor
-   add(np * 4);   // flags
+   add(np * rs);   // zb
+   add(np * rs);   // zbm
+   add(np * rs);   // zcor
+   add(np * 4);    // flags
    for (int k = 0; k < 4; k++) add((size_t) NJ * 4);   // is ie ms me
-   for (int k = 0; k < 6; k++) add((size_t) P * 4);
-   for (int k = 0; k < 5; k++) add((size_t) NJ * 4);
-   add(B->tab8.size() * 4);
-   add(4 * np * 4);   // g4
+   for (int k = 0; k < 6; k++) add((size_t) P * rs);
+   for (int k = 0; k < 5; k++) add((size_t) NJ * rs);
+   add(I.tab8.size() * rs);
+   add(4 * np * rs);   // g4
    char *d = nullptr;
    HIPOK(hipMalloc((void **) &d, tot));
-   B->dmem = d;
+   I.dmem = d;
    int m = 0;
    auto up = [&](const void *src, size_t bytes) -> char * {
       char *dst = d + off[m++];
       hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
       return dst;
    };
-   SlBasinDev &D = B->dev;
+   SlBasinDevT<R> &D = I.dev;
    memset(&D, 0, sizeof D);
    D.imxb = b->imxb;
    D.jmxb = b->jmxb;
@@ -250,30 +293,31 @@ static int basin_upload(slg_basin *B, char *err, int errlen)
    D.ebsn1_amp = (b->ebsn1 == '&');
    D.ismth = b->ismth;
    D.xver = strncmp(b->sta, "CH2 (2008)", 10) == 0 ? 1 : 2;
-   D.hter = (float) b->hter;
+   D.hter = (R) b->hter;
    {
-      static const float hc[12] = {0, 0, .5f, .1f, .2f, .3f, .4f, 1.E-5f, .2f, .3f, .4f, .5f};
-      D.hcrt = (b->ismth >= 2 && b->ismth <= 11) ? hc[b->ismth] : 0.f;
-      D.exclude = (b->ismth >= 7) ? 1.E-5f : D.hcrt;
+      // HCRT and EXCLUDE of XFILTER by smoothing code (xfilter.f:176-207)
+      static const double hc[12] = {0, 0, .5, .1, .2, .3, .4, 1.E-5, .2, .3, .4, .5};
+      D.hcrt = (b->ismth >= 2 && b->ismth <= 11) ? (R) hc[b->ismth] : R(0);
+      D.exclude = (b->ismth >= 7) ? (R) 1.E-5 : D.hcrt;
    }
-   for (int k = 0; k < 4; k++) D.gamfp[k] = (float) b->gamfp[k];
+   for (int k = 0; k < 4; k++) D.gamfp[k] = (R) b->gamfp[k];
    D.nbcpt = b->nbcpt;
-   D.zb = (const float *) up(B->zb.data(), np * 4);
-   D.zbm = (const float *) up(B->zbm.data(), np * 4);
-   D.zcor = (const float *) up(B->zcor.data(), np * 4);
-   D.flags = (const uint32_t *) up(B->flags.data(), np * 4);
+   D.zb = (const R *) up(I.zb.data(), np * rs);
+   D.zbm = (const R *) up(I.zbm.data(), np * rs);
+   D.zcor = (const R *) up(I.zcor.data(), np * rs);
+   D.flags = (const uint32_t *) up(I.flags.data(), np * 4);
    D.is = (const int *) up(B->is.data(), (size_t) NJ * 4);
    D.ie = (const int *) up(B->ie.data(), (size_t) NJ * 4);
    D.ms = (const int *) up(B->ms.data(), (size_t) NJ * 4);
    D.me = (const int *) up(B->me.data(), (size_t) NJ * 4);
-   const float **vi[6] = {&D.elpct, &D.elpdt, &D.elpcl, &D.elpdl, &D.elpcl2, &D.elpdl2};
-   for (int k = 0; k < 6; k++) *vi[k] = (const float *) up(B->vi[k].data(), (size_t) P * 4);
-   const float **vj[5] = {&D.sint, &D.cost, &D.sinl, &D.cosl, &D.sinl2};
-   for (int k = 0; k < 5; k++) *vj[k] = (const float *) up(B->vj[k].data(), (size_t) NJ * 4);
-   D.tab8 = (const float *) up(B->tab8.data(), B->tab8.size() * 4);
-   D.g4 = (const float *) up(B->g4.data(), 4 * np * 4);
+   const R **vi[6] = {&D.elpct, &D.elpdt, &D.elpcl, &D.elpdl, &D.elpcl2, &D.elpdl2};
+   for (int k = 0; k < 6; k++) *vi[k] = (const R *) up(I.vi[k].data(), (size_t) P * rs);
+   const R **vj[5] = {&D.sint, &D.cost, &D.sinl, &D.cosl, &D.sinl2};
+   for (int k = 0; k < 5; k++) *vj[k] = (const R *) up(I.vj[k].data(), (size_t) NJ * rs);
+   D.tab8 = (const R *) up(I.tab8.data(), I.tab8.size() * rs);
+   D.g4 = (const R *) up(I.g4.data(), 4 * np * rs);
    HIPOK(hipDeviceSynchronize());
-   B->on_dev = true;
+   I.on_dev = true;
    return 0;
 }
 
@@ -322,9 +366,7 @@ extern "C" int slg_basin_open(const char *dta, const char *bnt_line, slg_basin *
    if (e1 == '@' || e1 == '%') B->incsm = 10;
    if (e1 == '^' || e1 == '*') B->incsm = 30;
    B->ismpt2g = !(e1 == '+' || e1 == '&' || e1 == '#' || e1 == '%' || e1 == '*');
-   B->on_dev = false;
-   B->dmem = nullptr;
-   basin_images(B);
+   basin_bounds(B);
    *out = B;
    return 0;
 }
@@ -332,9 +374,10 @@ extern "C" int slg_basin_open(const char *dta, const char *bnt_line, slg_basin *
 extern "C" int slg_basin_set_ylt(slg_basin *B, const double *ylt)
 {
    sl_basin *b = &B->h;
+   if (B->i32.on_dev || B->i64.on_dev) return SLG_EINPUT;   // before the first upload only
    memcpy(b->ylt, ylt, sizeof(double) * (size_t) b->imxb * b->jmxb);
    b->have_ylt = 1;
-   basin_images(B);
+   B->i32.built = B->i64.built = false;
    return 0;
 }
 
@@ -349,7 +392,8 @@ extern "C" int slg_basin_dims(const slg_basin *B, int *imxb, int *jmxb, int *nbc
 extern "C" void slg_basin_close(slg_basin *B)
 {
    if (!B) return;
-   if (B->dmem) hipFree(B->dmem);
+   if (B->i32.dmem) (void) hipFree(B->i32.dmem);
+   if (B->i64.dmem) (void) hipFree(B->i64.dmem);
    sl_basin_free(&B->h);
    delete B;
 }
@@ -371,20 +415,28 @@ static void storm_row(const sl_storm *s, const sl_basin *b, double *v)
    (void) b;
 }
 
-static void push_tables(storm_sched &S, const sl_storm *s, int step)
+template <class R>
+static void push_tables(std::vector<R> &T, const sl_storm *s)
 {
    // device record k: C S CW SW P DELP 0 0
    const double *src[6] = {s->C, s->S, s->CW, s->SW, s->P, s->DELP};
-   size_t base = S.tables.size();
-   S.tables.resize(base + (size_t) SL_NTABV * 8, 0.f);
+   size_t base = T.size();
+   T.resize(base + (size_t) SL_NTABV * 8, R(0));
    for (int k = 1; k <= 800; k++)
-      for (int t = 0; t < 6; t++) S.tables[base + (size_t) k * 8 + t] = (float) src[t][k];
+      for (int t = 0; t < 6; t++) T[base + (size_t) k * 8 + t] = (R) src[t][k];
+}
+
+static void push_tables(storm_sched &S, const sl_storm *s, int step, int prec)
+{
+   if (prec == 64) push_tables(S.t64, s);
+   else push_tables(S.t32, s);
    S.tab_step.push_back(step);
 }
 
-static void step_params(const slg_basin *B, const double *r, int tv, float *P);
+template <class R>
+static void step_params(const slg_basin *B, const double *r, int tv, R *P);
 
-static int make_schedule(const slg_basin *B, const char *trk, storm_sched &S, bool keep_rows,
+static int make_schedule(const slg_basin *B, const char *trk, storm_sched &S, bool keep_rows, int prec,
                          char *err, int errlen)
 {
    char e[256];
@@ -395,19 +447,29 @@ static int make_schedule(const slg_basin *B, const char *trk, storm_sched &S, bo
    }
    sl_storm s = S.s0;
    S.rows.clear();
-   S.params.clear();
-   S.tables.clear();
+   S.p32.clear();
+   S.p64.clear();
+   S.t32.clear();
+   S.t64.clear();
    S.tab_step.clear();
    storm_row(&s, &B->h, row);
    if (keep_rows) S.rows.insert(S.rows.end(), row, row + 90);
-   push_tables(S, &s, 0);
+   push_tables(S, &s, 0, prec);
    int gen = s.table_gen, n = 0;
-   S.params.reserve((size_t) (s.mhalt + 1) * SL_NPAR);
+   if (prec == 64) S.p64.reserve((size_t) (s.mhalt + 1) * SL_NPAR);
+   else S.p32.reserve((size_t) (s.mhalt + 1) * SL_NPAR);
    while (s.itime < s.mhalt) {
       // values step n+1 reads: the row after step n, and the table set in force
-      size_t o = S.params.size();
-      S.params.resize(o + SL_NPAR);
-      step_params(B, row, (int) S.tab_step.size() - 1, &S.params[o]);
+      const int tv = (int) S.tab_step.size() - 1;
+      if (prec == 64) {
+         size_t o = S.p64.size();
+         S.p64.resize(o + SL_NPAR);
+         step_params(B, row, tv, &S.p64[o]);
+      } else {
+         size_t o = S.p32.size();
+         S.p32.resize(o + SL_NPAR);
+         step_params(B, row, tv, &S.p32[o]);
+      }
       sl_storm_ttimer(&s);
       if (sl_storm_stmval(&s, &B->h)) {
          seterr(err, errlen, "%s", "FORCE1 did not converge");
@@ -417,7 +479,7 @@ static int make_schedule(const slg_basin *B, const char *trk, storm_sched &S, bo
       storm_row(&s, &B->h, row);
       if (keep_rows) S.rows.insert(S.rows.end(), row, row + 90);
       if (s.table_gen != gen) {
-         push_tables(S, &s, n);
+         push_tables(S, &s, n, prec);
          gen = s.table_gen;
       }
    }
@@ -426,7 +488,9 @@ static int make_schedule(const slg_basin *B, const char *trk, storm_sched &S, bo
 }
 
 // device parameters of a step from the ref-format row left by the step before
-static void step_params(const slg_basin *B, const double *r, int tv, float *P)
+// (the storm schedule is computed in double and rounded to R where it is used)
+template <class R>
+static void step_params(const slg_basin *B, const double *r, int tv, R *P)
 {
    int itime = (int) r[0] + 1, ic19 = (int) r[27];
    double bt = 1.;
@@ -434,56 +498,60 @@ static void step_params(const slg_basin *B, const double *r, int tv, float *P)
       double time = itime, ang = time / ic19;
       bt = .5 * (1. - cos(ang * 3.14159265358979323846));
    }
-   memset(P, 0, SL_NPAR * sizeof(float));
-   P[P_BT] = (float) bt;
-   P[P_DELT] = (float) r[2];
-   P[P_X1B4] = (float) r[24];
-   P[P_X1B5] = (float) r[25];
-   P[P_X1B6] = (float) r[26];
-   P[P_C1] = (float) r[3];
-   P[P_C2] = (float) r[4];
-   P[P_AX] = (float) r[5];
-   P[P_AY] = (float) r[6];
-   P[P_X4] = (float) r[40 + 3];
-   P[P_X9] = (float) r[40 + 8];
-   P[P_X10] = (float) r[40 + 9];
-   P[P_X15] = (float) r[40 + 14];
-   P[P_X20] = (float) r[40 + 19];
-   P[P_X21] = (float) r[40 + 20];
-   P[P_X23] = (float) r[40 + 22];
-   P[P_X24] = (float) r[40 + 23];
-   P[P_W0] = (float) r[15];
-   P[P_W1] = (float) r[16];
-   P[P_SEA] = (float) r[28];
+   for (int k = 0; k < SL_NPAR; k++) P[k] = R(0);
+   P[P_BT] = (R) bt;
+   P[P_DELT] = (R) r[2];
+   P[P_X1B4] = (R) r[24];
+   P[P_X1B5] = (R) r[25];
+   P[P_X1B6] = (R) r[26];
+   P[P_C1] = (R) r[3];
+   P[P_C2] = (R) r[4];
+   P[P_AX] = (R) r[5];
+   P[P_AY] = (R) r[6];
+   P[P_X4] = (R) r[40 + 3];
+   P[P_X9] = (R) r[40 + 8];
+   P[P_X10] = (R) r[40 + 9];
+   P[P_X15] = (R) r[40 + 14];
+   P[P_X20] = (R) r[40 + 19];
+   P[P_X21] = (R) r[40 + 20];
+   P[P_X23] = (R) r[40 + 22];
+   P[P_X24] = (R) r[40 + 23];
+   P[P_W0] = (R) r[15];
+   P[P_W1] = (R) r[16];
+   P[P_SEA] = (R) r[28];
    int fl = 0;
    if (B->ismpt2g && itime > 5 && B->incsm != 1) fl |= SF_SMPT;
    if ((itime + 1) % B->incsm == 0 || B->h.ebsn1 == '+') fl |= SF_FILT;
-   int ib;
-   memcpy(&ib, &itime, 4);
-   memcpy(&P[P_ITIME], &ib, 4);
-   memcpy(&P[P_TVER], &tv, 4);
-   memcpy(&P[P_FLAGS], &fl, 4);
+   // small integers, exact in either precision
+   P[P_ITIME] = (R) itime;
+   P[P_TVER] = (R) tv;
+   P[P_FLAGS] = (R) fl;
 }
 
 // --------------------------------------------------------------------------
 // batch
 
-extern "C" int slg_batch_open(slg_basin *B, int nstorm, const char *const *trk, slg_batch **out,
-                              char *err, int errlen)
+extern "C" int slg_batch_open_prec(slg_basin *B, int nstorm, const char *const *trk, int precision,
+                                   slg_batch **out, char *err, int errlen)
 {
    *out = nullptr;
    if (nstorm < 1) {
       seterr(err, errlen, "%s", "nstorm must be >= 1");
       return SLG_EINPUT;
    }
+   if (precision != 32 && precision != 64) {
+      seterr(err, errlen, "%s", "precision must be 32 or 64");
+      return SLG_EINPUT;
+   }
    slg_batch *X = new slg_batch();
    X->b = B;
+   X->prec = precision;
    X->nstorm = nstorm;
    X->sc.resize(nstorm);
    X->maxsteps = 0;
    X->dmem = nullptr;
    for (int s = 0; s < nstorm; s++) {
-      int rc = make_schedule(B, trk[s], X->sc[s], nstorm <= 8, err, errlen);
+      int rc = make_schedule(B, trk[s], X->sc[s], nstorm <= 8, precision, err, errlen);
       if (rc) {
          delete X;
          return rc;
@@ -503,6 +571,17 @@ extern "C" int slg_batch_open(slg_basin *B, int nstorm, const char *const *trk, 
    return 0;
 }
 
+extern "C" int slg_batch_open(slg_basin *B, int nstorm, const char *const *trk, slg_batch **out,
+                              char *err, int errlen)
+{
+   return slg_batch_open_prec(B, nstorm, trk, 32, out, err, errlen);
+}
+
+extern "C" int slg_batch_precision(const slg_batch *X)
+{
+   return X->prec;
+}
+
 extern "C" int slg_batch_set_path(slg_batch *X, int path, int tile_rows)
 {
    if ((path != SLG_PATH_FUSED && path != SLG_PATH_STAGED) ||
@@ -518,13 +597,20 @@ extern "C" int slg_batch_nsteps(const slg_batch *X, int storm)
    return X->sc[storm].nsteps;
 }
 
-extern "C" int slg_batch_upload(slg_batch *X, char *err, int errlen)
+static const std::vector<float> &sched_params(const storm_sched &S, float) { return S.p32; }
+static const std::vector<double> &sched_params(const storm_sched &S, double) { return S.p64; }
+static const std::vector<float> &sched_tables(const storm_sched &S, float) { return S.t32; }
+static const std::vector<double> &sched_tables(const storm_sched &S, double) { return S.t64; }
+
+template <class R>
+static int batch_upload(slg_batch *X, char *err, int errlen)
 {
    slg_basin *B = X->b;
-   if (!B->on_dev) {
-      int rc = basin_upload(B, err, errlen);
+   if (!img(B, R()).on_dev) {
+      int rc = basin_upload<R>(B, err, errlen);
       if (rc) return rc;
    }
+   const size_t rs = sizeof(R);
    const sl_basin *b = &B->h;
    const int P = B->pitch, NJ = b->jmxb, S = X->nstorm;
    const size_t fstride = (size_t) P * NJ, np = fstride * S;
@@ -536,71 +622,83 @@ extern "C" int slg_batch_upload(slg_batch *X, char *err, int errlen)
       off[n++] = tot;
       tot += (bytes + 255) & ~(size_t) 255;
    };
-   add(np * 4); add(np * 4); add(np * 4); add(np * 4); add(np * 4);   // hb0 hb1 ub vb hmx
-   add((size_t) S * P * 4);                                            // hsubtop
+   add(np * rs); add(np * rs); add(np * rs); add(np * rs); add(np * rs);   // hb0 hb1 ub vb hmx
+   add((size_t) S * P * rs);                                           // hsubtop
    add(np); add(np); add(np);                                          // pflag hclass uz
    add((size_t) S * SL_EVCAP * 4);                                     // evkey
    add((size_t) S * 4);                                                // evcnt
    add((size_t) S * 8 * 4);                                            // stat
-   add((size_t) S * X->maxsteps * SL_NPAR * 4);                        // params
-   add(ntab * SL_NTABV * 8 * 4);                                       // tables
+   add((size_t) S * X->maxsteps * SL_NPAR * rs);                       // params
+   add(ntab * SL_NTABV * 8 * rs);                                      // tables
    add((size_t) S * 4);                                                // tbase
    add((size_t) S * 4);                                                // nsteps
    char *d = nullptr;
    HIPOK(hipMalloc((void **) &d, tot));
    X->dmem = d;
    X->bytes = tot;
-   SlRunDev &R = X->dev;
-   memset(&R, 0, sizeof R);
-   R.nstorm = S;
-   R.fstride = fstride;
-   R.hb0 = (float *) (d + off[0]);
-   R.hb1 = (float *) (d + off[1]);
-   R.ub = (float *) (d + off[2]);
-   R.vb = (float *) (d + off[3]);
-   R.hmx = (float *) (d + off[4]);
-   R.hsubtop = (float *) (d + off[5]);
-   R.pflag = (uint8_t *) (d + off[6]);
-   R.hclass = (uint8_t *) (d + off[7]);
-   R.uz = (uint8_t *) (d + off[8]);
-   R.evkey = (uint32_t *) (d + off[9]);
-   R.evcnt = (int *) (d + off[10]);
-   R.stat = (int *) (d + off[11]);
-   R.maxsteps = X->maxsteps;
+   SlRunDevT<R> &D = run(X, R());
+   memset(&D, 0, sizeof D);
+   D.nstorm = S;
+   D.fstride = fstride;
+   D.hb0 = (R *) (d + off[0]);
+   D.hb1 = (R *) (d + off[1]);
+   D.ub = (R *) (d + off[2]);
+   D.vb = (R *) (d + off[3]);
+   D.hmx = (R *) (d + off[4]);
+   D.hsubtop = (R *) (d + off[5]);
+   D.pflag = (uint8_t *) (d + off[6]);
+   D.hclass = (uint8_t *) (d + off[7]);
+   D.uz = (uint8_t *) (d + off[8]);
+   D.evkey = (uint32_t *) (d + off[9]);
+   D.evcnt = (int *) (d + off[10]);
+   D.stat = (int *) (d + off[11]);
+   D.maxsteps = X->maxsteps;
    HIPOK(hipMemset(d, 0, tot));
    // initial water (INTLHT) per storm: HB, HMX=HB, UB=VB=0
-   std::vector<float> f(fstride, 0.f);
-   std::vector<float> params((size_t) S * X->maxsteps * SL_NPAR, 0.f);
-   std::vector<float> tables;
+   std::vector<R> f(fstride, R(0));
+   std::vector<R> params((size_t) S * X->maxsteps * SL_NPAR, R(0));
+   std::vector<R> tables;
    std::vector<int> tbase(S), nst(S);
    for (int s = 0; s < S; s++) {
       const double *hb = &X->hb0[(size_t) s * b->imxb * NJ];
       for (int j = 1; j <= NJ; j++)
-         for (int i = 1; i <= b->imxb; i++) f[(size_t) (j - 1) * P + (i - 1)] = (float) hb[PIX(b, i, j)];
-      HIPOK(hipMemcpy(R.hb0 + (size_t) s * fstride, f.data(), fstride * 4, hipMemcpyHostToDevice));
-      HIPOK(hipMemcpy(R.hb1 + (size_t) s * fstride, f.data(), fstride * 4, hipMemcpyHostToDevice));
+         for (int i = 1; i <= b->imxb; i++) f[(size_t) (j - 1) * P + (i - 1)] = (R) hb[PIX(b, i, j)];
+      HIPOK(hipMemcpy(D.hb0 + (size_t) s * fstride, f.data(), fstride * rs, hipMemcpyHostToDevice));
+      HIPOK(hipMemcpy(D.hb1 + (size_t) s * fstride, f.data(), fstride * rs, hipMemcpyHostToDevice));
       for (int j = 1; j <= NJ; j++)       // HMX = HB on squares only (intlht.f:172-177)
          for (int i = 1; i <= b->imxb; i++)
-            if (i == b->imxb || j == NJ) f[(size_t) (j - 1) * P + (i - 1)] = 0.f;
-      HIPOK(hipMemcpy(R.hmx + (size_t) s * fstride, f.data(), fstride * 4, hipMemcpyHostToDevice));
-      memcpy(&params[(size_t) s * X->maxsteps * SL_NPAR], X->sc[s].params.data(),
-             X->sc[s].params.size() * sizeof(float));
+            if (i == b->imxb || j == NJ) f[(size_t) (j - 1) * P + (i - 1)] = R(0);
+      HIPOK(hipMemcpy(D.hmx + (size_t) s * fstride, f.data(), fstride * rs, hipMemcpyHostToDevice));
+      const std::vector<R> &sp = sched_params(X->sc[s], R()), &st = sched_tables(X->sc[s], R());
+      memcpy(&params[(size_t) s * X->maxsteps * SL_NPAR], sp.data(), sp.size() * sizeof(R));
       tbase[s] = (int) (tables.size() / ((size_t) SL_NTABV * 8));
-      tables.insert(tables.end(), X->sc[s].tables.begin(), X->sc[s].tables.end());
+      tables.insert(tables.end(), st.begin(), st.end());
       nst[s] = X->sc[s].nsteps;
    }
-   HIPOK(hipMemcpy(d + off[12], params.data(), params.size() * 4, hipMemcpyHostToDevice));
-   HIPOK(hipMemcpy(d + off[13], tables.data(), tables.size() * 4, hipMemcpyHostToDevice));
+   HIPOK(hipMemcpy(d + off[12], params.data(), params.size() * rs, hipMemcpyHostToDevice));
+   HIPOK(hipMemcpy(d + off[13], tables.data(), tables.size() * rs, hipMemcpyHostToDevice));
    HIPOK(hipMemcpy(d + off[14], tbase.data(), (size_t) S * 4, hipMemcpyHostToDevice));
    HIPOK(hipMemcpy(d + off[15], nst.data(), (size_t) S * 4, hipMemcpyHostToDevice));
-   R.params = (const float *) (d + off[12]);
-   R.tables = (const float *) (d + off[13]);
-   R.tbase = (const int *) (d + off[14]);
-   R.nsteps = (const int *) (d + off[15]);
+   D.params = (const R *) (d + off[12]);
+   D.tables = (const R *) (d + off[13]);
+   D.tbase = (const int *) (d + off[14]);
+   D.nsteps = (const int *) (d + off[15]);
    HIPOK(hipStreamCreate(&X->st));
    X->step = 0;
    X->parity = 0;
    return 0;
+}
+
+extern "C" int slg_batch_upload(slg_batch *X, char *err, int errlen)
+{
+   return X->prec == 64 ? batch_upload<double>(X, err, errlen) : batch_upload<float>(X, err, errlen);
+}
+
+static int launch_one(slg_batch *X, int step, int smpt, int filt, hipEvent_t *ev)
+{
+   if (X->prec == 64)
+      return dev_launch(&X->b->i64.dev, &X->d64, step, smpt, filt, X->parity, X->path, X->tile_rows, X->st, ev);
+   return dev_launch(&X->b->i32.dev, &X->d32, step, smpt, filt, X->parity, X->path, X->tile_rows, X->st, ev);
 }
 
 // advance every storm of the batch by n steps (stops at the shortest run)
@@ -615,8 +713,7 @@ extern "C" int slg_batch_step(slg_batch *X, int n, char *err, int errlen)
       int itime = step;
       int smpt = (B->ismpt2g && itime > 5 && B->incsm != 1);
       int filt = ((itime + 1) % B->incsm == 0 || B->h.ebsn1 == '+');
-      int rc = sl_dev_launch_step(&B->dev, &X->dev, step, smpt, filt, X->parity, X->path,
-                                  X->tile_rows, X->st, nullptr);
+      int rc = launch_one(X, step, smpt, filt, nullptr);
       if (rc < 0) {
          seterr(err, errlen, "kernel launch failed (%s%d)", "", -rc);
          return SLG_EHIP;
@@ -645,8 +742,7 @@ extern "C" int slg_batch_step_timed(slg_batch *X, int n, double *ms, int *launch
       int step = X->step + 1, itime = step;
       int smpt = (B->ismpt2g && itime > 5 && B->incsm != 1);
       int filt = ((itime + 1) % B->incsm == 0 || B->h.ebsn1 == '+');
-      int rc = sl_dev_launch_step(&B->dev, &X->dev, step, smpt, filt, X->parity, X->path,
-                                  X->tile_rows, X->st, ev);
+      int rc = launch_one(X, step, smpt, filt, ev);
       if (rc < 0) {
          seterr(err, errlen, "kernel launch failed (%s%d)", "", -rc);
          return SLG_EHIP;
@@ -774,13 +870,45 @@ extern "C" int slg_batch_start_date(const slg_batch *X, int storm, int *out5)
    return 0;
 }
 
+template <class R>
+static int batch_get(slg_batch *X, int storm, const char *name, double *dst, char *err, int errlen)
+{
+   const sl_basin *b = &X->b->h;
+   const SlRunDevT<R> &D = run(X, R());
+   const R *src;
+   if (!strcmp(name, "HB")) src = X->parity ? D.hb1 : D.hb0;
+   else if (!strcmp(name, "HBS")) {
+      if (X->hbs_buf < 0) {
+         seterr(err, errlen, "%s", "HB before FILTER is kept only by the staged path");
+         return SLG_EINPUT;
+      }
+      src = X->hbs_buf ? D.hb1 : D.hb0;
+   }
+   else if (!strcmp(name, "UB")) src = D.ub;
+   else if (!strcmp(name, "VB")) src = D.vb;
+   else if (!strcmp(name, "HMX")) src = D.hmx;
+   else {
+      seterr(err, errlen, "unknown field %s", name);
+      return SLG_EINPUT;
+   }
+   if (!X->dmem) {
+      seterr(err, errlen, "%s", "batch not uploaded");
+      return SLG_EINPUT;
+   }
+   std::vector<R> f(D.fstride);
+   HIPOK(hipStreamSynchronize(X->st));
+   HIPOK(hipMemcpy(f.data(), src + (size_t) storm * D.fstride, D.fstride * sizeof(R), hipMemcpyDeviceToHost));
+   const int P = X->b->pitch;
+   for (int j = 1; j <= b->jmxb; j++)
+      for (int i = 1; i <= b->imxb; i++) dst[PIX(b, i, j)] = (double) f[(size_t) (j - 1) * P + (i - 1)];
+   return 0;
+}
+
 // download a field of one storm as double [jmxb][imxb]
 extern "C" int slg_batch_get(slg_batch *X, int storm, const char *name, double *dst, char *err,
                              int errlen)
 {
    const sl_basin *b = &X->b->h;
-   const SlRunDev &R = X->dev;
-   const float *src;
    if (storm < 0 || storm >= X->nstorm) {
       seterr(err, errlen, "storm %s%d out of range", "", storm);
       return SLG_EINPUT;
@@ -790,34 +918,19 @@ extern "C" int slg_batch_get(slg_batch *X, int storm, const char *name, double *
       memcpy(dst, &X->hb0[(size_t) storm * np], np * sizeof(double));
       return 0;
    }
-   if (!strcmp(name, "HB")) src = X->parity ? R.hb1 : R.hb0;
-   else if (!strcmp(name, "HBS")) {
-      if (X->hbs_buf < 0) {
-         seterr(err, errlen, "%s", "HB before FILTER is kept only by the staged path");
-         return SLG_EINPUT;
-      }
-      src = X->hbs_buf ? R.hb1 : R.hb0;
-   }
-   else if (!strcmp(name, "UB")) src = R.ub;
-   else if (!strcmp(name, "VB")) src = R.vb;
-   else if (!strcmp(name, "HMX")) src = R.hmx;
-   else {
-      seterr(err, errlen, "unknown field %s", name);
-      return SLG_EINPUT;
-   }
-   std::vector<float> f(R.fstride);
-   HIPOK(hipStreamSynchronize(X->st));
-   HIPOK(hipMemcpy(f.data(), src + (size_t) storm * R.fstride, R.fstride * 4, hipMemcpyDeviceToHost));
-   const int P = X->b->pitch;
-   for (int j = 1; j <= b->jmxb; j++)
-      for (int i = 1; i <= b->imxb; i++) dst[PIX(b, i, j)] = f[(size_t) (j - 1) * P + (i - 1)];
-   return 0;
+   return X->prec == 64 ? batch_get<double>(X, storm, name, dst, err, errlen)
+                        : batch_get<float>(X, storm, name, dst, err, errlen);
 }
 
 extern "C" int slg_batch_stats(slg_batch *X, int storm, int *out8, char *err, int errlen)
 {
+   if (!X->dmem || storm < 0 || storm >= X->nstorm) {
+      seterr(err, errlen, "%s", "no such storm on the device");
+      return SLG_EINPUT;
+   }
+   const int *stat = X->prec == 64 ? X->d64.stat : X->d32.stat;
    HIPOK(hipStreamSynchronize(X->st));
-   HIPOK(hipMemcpy(out8, X->dev.stat + storm * 8, 32, hipMemcpyDeviceToHost));
+   HIPOK(hipMemcpy(out8, stat + storm * 8, 32, hipMemcpyDeviceToHost));
    return 0;
 }
 
@@ -825,9 +938,9 @@ extern "C" void slg_batch_close(slg_batch *X)
 {
    if (!X) return;
    if (X->dmem) {
-      hipStreamSynchronize(X->st);
-      hipStreamDestroy(X->st);
-      hipFree(X->dmem);
+      (void) hipStreamSynchronize(X->st);
+      (void) hipStreamDestroy(X->st);
+      (void) hipFree(X->dmem);
    }
    delete X;
 }

@@ -1,5 +1,6 @@
 /* sl_dev.h -- device-side data of the SLOSH step (shared by the HIP kernels and
- * the host code that fills them).  All field arithmetic is FP32.
+ * the host code that fills them), for R = float (the FP32 step) or double (the
+ * FP64 step, the reference's default precision).
  *
  * Fields are [storm][J-1][pitch] with I fastest; pitch is imxb rounded up to 64
  * floats so every row starts on a 256-byte boundary.
@@ -29,25 +30,27 @@
 #define F_IIK(f) (((f) >> 8) & 0xffu)
 #define F_BOP(f) (((f) >> 16) & 7u)   /* 1..4: one boundary point, ISKIP; 5/6: corner 1/3 */
 
-typedef struct {
+template <class R>
+struct SlBasinDevT {
    int imxb, jmxb, pitch, jend;
    int npts;                  /* pitch * jmxb: elements of one field */
    int ebsn_ell;              /* grid type '$' or '+': metric GAMF in FILTER/FLT2G */
    int ebsn1_amp;             /* EBSN1 == '&' (HMXSV: no smoothing) */
    int ismth, xver;           /* FILTER (1) or XFILTER hcrt code; XFILTER version */
-   float hter, hcrt, exclude;
-   float gamfp[4];
+   R hter, hcrt, exclude;
+   R gamfp[4];
    int nbcpt;
-   const float *zb, *zbm, *zcor;
+   const R *zb, *zbm, *zcor;
    const uint32_t *flags;
    const int *is, *ie, *ms, *me;                                      /* [jmxb] */
-   const float *elpct, *elpdt, *elpcl, *elpdl, *elpcl2, *elpdl2;      /* [pitch] */
-   const float *sint, *cost, *sinl, *cosl, *sinl2;                    /* [jmxb] */
-   const float *tab8;         /* [245][600][8]: AI2 SLPAI AR2 BR2 BI2 CR2 CI2 0 per (iik, id) */
-   const float *g4;           /* [4][pitch*jmxb]: metric weight GAMF of neighbour k (IIH/JJH order) */
-} SlBasinDev;
+   const R *elpct, *elpdt, *elpcl, *elpdl, *elpcl2, *elpdl2;          /* [pitch] */
+   const R *sint, *cost, *sinl, *cosl, *sinl2;                        /* [jmxb] */
+   const R *tab8;             /* [245][600][8]: AI2 SLPAI AR2 BR2 BI2 CR2 CI2 0 per (iik, id) */
+   const R *g4;               /* [4][pitch*jmxb]: metric weight GAMF of neighbour k (IIH/JJH order) */
+};
 
-/* per-step storm values (what the field routines read during that step) */
+/* per-step storm values (what the field routines read during that step);
+ * P_ITIME, P_TVER and P_FLAGS hold small integers as values */
 enum {
    P_BT, P_DELT, P_X1B4, P_X1B5, P_X1B6, P_C1, P_C2, P_AX, P_AY, P_X4, P_X9, P_X10,
    P_X15, P_X20, P_X21, P_X23, P_X24, P_W0, P_W1, P_SEA, P_ITIME, P_TVER, P_FLAGS, P_PAD
@@ -55,21 +58,22 @@ enum {
 #define SF_SMPT (1 << 0)      /* SMPT2G/FLT2G this step */
 #define SF_FILT (1 << 1)      /* FILTER/XFILTER this step */
 
-typedef struct {
+template <class R>
+struct SlRunDevT {
    int nstorm;
-   size_t fstride;            /* floats per storm field */
-   float *hb0, *hb1, *ub, *vb, *hmx;   /* [nstorm][fstride] */
-   float *hsubtop;            /* [nstorm][pitch]: HSUB row JMXB for XFILTER */
+   size_t fstride;            /* elements per storm field */
+   R *hb0, *hb1, *ub, *vb, *hmx;   /* [nstorm][fstride] */
+   R *hsubtop;                /* [nstorm][pitch]: HSUB row JMXB for XFILTER */
    uint8_t *pflag, *hclass;   /* [nstorm][fstride] (staged path) */
    uint8_t *uz;               /* [nstorm][fstride]: UB == 0 after CONTTY (fused path) */
    uint32_t *evkey;           /* [nstorm][SL_EVCAP] */
    int *evcnt;                /* [nstorm] */
    int *stat;                 /* [nstorm][8]: counters / error flags */
-   const float *params;       /* [nstorm][maxsteps][SL_NPAR] */
+   const R *params;           /* [nstorm][maxsteps][SL_NPAR] */
    int maxsteps;
-   const float *tables;       /* table pool [*][SL_NTABV][8]: C S CW SW P DELP 0 0 per radius */
+   const R *tables;           /* table pool [*][SL_NTABV][8]: C S CW SW P DELP 0 0 per radius */
    const int *tbase;          /* [nstorm] first table of each storm in the pool */
    const int *nsteps;         /* [nstorm] */
-} SlRunDev;
+};
 
 #endif

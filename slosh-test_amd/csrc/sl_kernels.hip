@@ -38,6 +38,29 @@
 #include <stdint.h>
 #include "sl_dev.h"
 
+// One source, two builds: with SL_F64 every kernel computes in double (the
+// reference's default build, and its results bit for bit), otherwise in float.
+#ifdef SL_F64
+typedef double real;
+#define SL_NS sl_f64
+#define SL_LAUNCH sl_dev_launch_step_f64
+#else
+typedef float real;
+#define SL_NS sl_f32
+#define SL_LAUNCH sl_dev_launch_step_f32
+#endif
+
+namespace SL_NS {
+typedef SlBasinDevT<real> SlBasinDev;
+typedef SlRunDevT<real> SlRunDev;
+
+// bit equality (a store is skipped only when it would not change memory)
+__device__ __forceinline__ bool same_bits(float a, float b) { return __float_as_uint(a) == __float_as_uint(b); }
+__device__ __forceinline__ bool same_bits(double a, double b)
+{
+   return __double_as_longlong(a) == __double_as_longlong(b);
+}
+
 #define TX 64
 #define TY 4
 // 32-bit element index of grid point (i,j): with a uniform base pointer the
@@ -60,19 +83,19 @@ __constant__ int cISBS[5] = {0, 0, -1, 0, 1}, cJSBS[5] = {0, -1, 0, 1, 0};
 __constant__ int cISHF[4][3] = {{0, 0, 0}, {1, 2, 4}, {0, 0, 0}, {1, 3, 4}};
 
 struct StepArgs {
-   const float *hc;     // HB entering the stage (per storm base = hc + s*fstride)
-   float *hn;           // HB leaving the stage
+   const real *hc;     // HB entering the stage (per storm base = hc + s*fstride)
+   real *hn;           // HB leaving the stage
    int step;            // 1-based step number
 };
 
-__device__ __forceinline__ const float *par(const SlRunDev &R, int s, int step)
+__device__ __forceinline__ const real *par(const SlRunDev &R, int s, int step)
 {
    return R.params + ((size_t) s * R.maxsteps + (step - 1)) * SL_NPAR;
 }
 
-__device__ __forceinline__ const float *tabv(const SlRunDev &R, int s, const float *P)
+__device__ __forceinline__ const real *tabv(const SlRunDev &R, int s, const real *P)
 {
-   int v = __float_as_int(P[P_TVER]);
+   int v = (int) P[P_TVER];
    return R.tables + (size_t) (R.tbase[s] + v) * SL_NTABV * 8;
 }
 
@@ -84,14 +107,14 @@ __device__ __forceinline__ bool zr(const SlBasinDev &B, int i, int j)
 // ------------------------------------------------------------------ accessors
 // (i, j) are 1-based grid indices; every caller stays inside the grid.
 struct GF {   // a field of one storm in global memory
-   const float *p;
+   const real *p;
    int pitch;
-   __device__ float operator()(int i, int j) const { return p[(uint32_t) ((j - 1) * pitch + (i - 1))]; }
+   __device__ real operator()(int i, int j) const { return p[(uint32_t) ((j - 1) * pitch + (i - 1))]; }
 };
 struct LF {   // an LDS tile whose element 0 is grid point (i0, j0)
-   const float *p;
+   const real *p;
    int i0, j0, w;
-   __device__ float operator()(int i, int j) const { return p[(j - j0) * w + (i - i0)]; }
+   __device__ real operator()(int i, int j) const { return p[(j - j0) * w + (i - i0)]; }
 };
 struct GB {
    const uint8_t *p;
@@ -104,23 +127,23 @@ struct LB {
    __device__ uint8_t operator()(int i, int j) const { return p[(j - j0) * w + (i - i0)]; }
 };
 struct GUZ {  // "UB(i,j) == 0" straight from the transport field
-   const float *ub;
+   const real *ub;
    int pitch;
-   __device__ bool operator()(int i, int j) const { return ub[(uint32_t) ((j - 1) * pitch + (i - 1))] == 0.f; }
+   __device__ bool operator()(int i, int j) const { return ub[(uint32_t) ((j - 1) * pitch + (i - 1))] == real(0.); }
 };
 
 // Static fields ZB (squares) and ZBM (momentum points): global or an LDS tile.
 struct ZG {
-   const float *zb, *zbm;
+   const real *zb, *zbm;
    int pitch;
-   __device__ float b(int i, int j) const { return zb[(uint32_t) ((j - 1) * pitch + (i - 1))]; }
-   __device__ float m(int i, int j) const { return zbm[(uint32_t) ((j - 1) * pitch + (i - 1))]; }
+   __device__ real b(int i, int j) const { return zb[(uint32_t) ((j - 1) * pitch + (i - 1))]; }
+   __device__ real m(int i, int j) const { return zbm[(uint32_t) ((j - 1) * pitch + (i - 1))]; }
 };
 struct ZL {
-   const float *zb, *zbm;
+   const real *zb, *zbm;
    int i0, j0, w;
-   __device__ float b(int i, int j) const { return zb[(j - j0) * w + (i - i0)]; }
-   __device__ float m(int i, int j) const { return zbm[(j - j0) * w + (i - i0)]; }
+   __device__ real b(int i, int j) const { return zb[(j - j0) * w + (i - i0)]; }
+   __device__ real m(int i, int j) const { return zbm[(j - j0) * w + (i - i0)]; }
 };
 
 // Workgroup number -> (tile x, tile y, storm).  Hardware hands workgroup g to
@@ -147,51 +170,51 @@ __device__ __forceinline__ int wave_sum(int v)
 // |UB| <= 1e-10 inside 2..IMXB1 x 2..JMXB1 count as 0 (and are flushed by their
 // owner thread), as contty.f:90-96 does before the sweep.  Returns the new HB;
 // squares that dry with a transport rescale are listed for the replay.
-__device__ __forceinline__ float contty_sq(const SlBasinDev &B, const SlRunDev &R, int s, int step,
-                                           int i, int j, const float *hc, float *ub, float *vb,
+__device__ __forceinline__ real contty_sq(const SlBasinDev &B, const SlRunDev &R, int s, int step,
+                                           int i, int j, const real *hc, real *ub, real *vb,
                                            uint8_t *uz)
 {
    const uint32_t p = IDX(B, i, j);
    if (j == B.jmxb) {
       if (i <= B.imxb - 1) R.hsubtop[(size_t) s * B.pitch + (i - 1)] = hc[p];
-      if (uz) uz[p] = ub[p] == 0.f;
+      if (uz) uz[p] = ub[p] == real(0.);
       return hc[p];
    }
    if (i == B.imxb) {
-      if (uz) uz[p] = ub[p] == 0.f;
+      if (uz) uz[p] = ub[p] == real(0.);
       return hc[p];
    }
    const uint32_t fl = B.flags[p];
-   float u0 = ub[p];
+   real u0 = ub[p];
    if (fl & F_ZR) {
-      if (fabsf(u0) <= 1.E-10f) {
-         ub[p] = 0.f;
-         vb[p] = 0.f;
-         u0 = 0.f;
+      if (fabs(u0) <= real(1.E-10)) {
+         ub[p] = real(0.);
+         vb[p] = real(0.);
+         u0 = real(0.);
       }
    }
-   if (uz) uz[p] = u0 == 0.f;
-   float out = hc[p];
+   if (uz) uz[p] = u0 == real(0.);
+   real out = hc[p];
    if (fl & F_ACT) {
       const uint32_t p2 = p + 1, p3 = p + B.pitch + 1, p4 = p + B.pitch;
-      float u1 = ub[p], u2 = ub[p2], u3 = ub[p3], u4 = ub[p4];
-      float v1 = vb[p], v2 = vb[p2], v3 = vb[p3], v4 = vb[p4];
-      if (zr(B, i, j) && fabsf(u1) <= 1.E-10f) { u1 = 0.f; v1 = 0.f; }
-      if (zr(B, i + 1, j) && fabsf(u2) <= 1.E-10f) { u2 = 0.f; v2 = 0.f; }
-      if (zr(B, i + 1, j + 1) && fabsf(u3) <= 1.E-10f) { u3 = 0.f; v3 = 0.f; }
-      if (zr(B, i, j + 1) && fabsf(u4) <= 1.E-10f) { u4 = 0.f; v4 = 0.f; }
-      const float dux = -u1 + u2 + u3 - u4;
-      const float zb = B.zb[p];
-      if (!(zb <= 30.f && dux == 0.f)) {
-         const float *P = par(R, s, step);
-         const float hpst = hc[p];
-         const float z = B.elpdl2[i - 1] + (B.elpcl2[i - 1] - B.elpdl2[i - 1]) * B.sinl2[j - 1];
-         const float fct = P[P_X1B5] / z;
-         const float dvy = -v1 - v2 + v3 + v4;
-         const float hbj = hpst - fct * (dux + dvy);
-         const float chng = hbj + zb;
-         if (chng < 0.f) {
-            if (-chng < 1E-3f) out = -zb;
+      real u1 = ub[p], u2 = ub[p2], u3 = ub[p3], u4 = ub[p4];
+      real v1 = vb[p], v2 = vb[p2], v3 = vb[p3], v4 = vb[p4];
+      if (zr(B, i, j) && fabs(u1) <= real(1.E-10)) { u1 = real(0.); v1 = real(0.); }
+      if (zr(B, i + 1, j) && fabs(u2) <= real(1.E-10)) { u2 = real(0.); v2 = real(0.); }
+      if (zr(B, i + 1, j + 1) && fabs(u3) <= real(1.E-10)) { u3 = real(0.); v3 = real(0.); }
+      if (zr(B, i, j + 1) && fabs(u4) <= real(1.E-10)) { u4 = real(0.); v4 = real(0.); }
+      const real dux = -u1 + u2 + u3 - u4;
+      const real zb = B.zb[p];
+      if (!(zb <= real(30.) && dux == real(0.))) {
+         const real *P = par(R, s, step);
+         const real hpst = hc[p];
+         const real z = B.elpdl2[i - 1] + (B.elpcl2[i - 1] - B.elpdl2[i - 1]) * B.sinl2[j - 1];
+         const real fct = P[P_X1B5] / z;
+         const real dvy = -v1 - v2 + v3 + v4;
+         const real hbj = hpst - fct * (dux + dvy);
+         const real chng = hbj + zb;
+         if (chng < real(0.)) {
+            if (-chng < real(1E-3)) out = -zb;
             else if (hc[p] == -zb) out = hc[p];
             else {
                out = -zb;   // final value comes from the replay
@@ -227,8 +250,8 @@ __global__ void __launch_bounds__(TX * TY) k_conttyA(SlBasinDev B, SlRunDev R, S
    const int i = tx * TX + threadIdx.x + 1;
    if (i > B.imxb) return;
    const size_t so = (size_t) s * R.fstride;
-   const float *hc = A.hc + so;
-   float *hn = A.hn + so, *ub = R.ub + so, *vb = R.vb + so;
+   const real *hc = A.hc + so;
+   real *hn = A.hn + so, *ub = R.ub + so, *vb = R.vb + so;
    uint8_t *uz = R.uz + so;
    for (int r = threadIdx.y; r < ATH; r += TY) {
       const int j = ty * ATH + r + 1;
@@ -249,13 +272,13 @@ __global__ void __launch_bounds__(TX * TY) k_conttyA(SlBasinDev B, SlRunDev R, S
 // overflows, the replay falls back to visiting every square (still exact).
 struct Replay {
    const SlBasinDev *B;
-   const float *hc;
-   float *hn, *ub, *vb;
+   const real *hc;
+   real *hn, *ub, *vb;
    uint8_t *uz;
    uint32_t *key;
    int n, full;
    uint32_t frontier;
-   float x1b5;
+   real x1b5;
 };
 
 __device__ static int lbound(const uint32_t *key, int n, uint32_t k)
@@ -310,24 +333,24 @@ __device__ static int visit(Replay &X, int i, int j, int &leap, int ipass)
    if (!(B.flags[p] & F_ACT)) return 0;
    const uint32_t k = ((uint32_t) j << 16) | (uint32_t) i;
    if (k > X.frontier) X.frontier = k;
-   const float zb = B.zb[p];
+   const real zb = B.zb[p];
    if (leap == 2 && i == ipass) {
       X.hn[p] = -zb;
       leap = 1;
       return 0;
    }
-   float *ub = X.ub, *vb = X.vb;
+   real *ub = X.ub, *vb = X.vb;
    const uint32_t p2 = p + 1, p3 = p + B.pitch + 1, p4 = p + B.pitch;
-   const float dux = -ub[p] + ub[p2] + ub[p3] - ub[p4];
-   if (zb <= 30.f && dux == 0.f) return 0;
-   const float hpst = X.hc[p];
-   const float z = B.elpdl2[i - 1] + (B.elpcl2[i - 1] - B.elpdl2[i - 1]) * B.sinl2[j - 1];
-   const float fct = X.x1b5 / z;
-   const float dvy = -vb[p] - vb[p2] + vb[p3] + vb[p4];
-   const float hbj = hpst - fct * (dux + dvy);
-   const float chng = hbj + zb;
-   if (chng < 0.f) {
-      if (-chng < 1E-3f) {
+   const real dux = -ub[p] + ub[p2] + ub[p3] - ub[p4];
+   if (zb <= real(30.) && dux == real(0.)) return 0;
+   const real hpst = X.hc[p];
+   const real z = B.elpdl2[i - 1] + (B.elpcl2[i - 1] - B.elpdl2[i - 1]) * B.sinl2[j - 1];
+   const real fct = X.x1b5 / z;
+   const real dvy = -vb[p] - vb[p2] + vb[p3] + vb[p4];
+   const real hbj = hpst - fct * (dux + dvy);
+   const real chng = hbj + zb;
+   if (chng < real(0.)) {
+      if (-chng < real(1E-3)) {
          X.hn[p] = -zb;
          return 0;
       }
@@ -336,16 +359,16 @@ __device__ static int visit(Replay &X, int i, int j, int &leap, int ipass)
       return 0;
    }
    if (X.hc[p] == -zb) return 0;
-   float denm = hbj - hpst;
-   denm = copysignf(fmaxf(1.E-5f, fabsf(denm)), denm);
-   const float anum = -zb - hpst;
-   float rfct = anum / denm;
-   if (fabsf(rfct) < 1.E-5f) rfct = 0.f;
+   real denm = hbj - hpst;
+   denm = copysign(fmax(real(1.E-5), fabs(denm)), denm);
+   const real anum = -zb - hpst;
+   real rfct = anum / denm;
+   if (fabs(rfct) < real(1.E-5)) rfct = real(0.);
    const uint32_t c[4] = {p, p2, p3, p4};
    for (int q = 0; q < 4; q++) {
       ub[c[q]] = rfct * ub[c[q]];
       vb[c[q]] = rfct * vb[c[q]];
-      if (X.uz) X.uz[c[q]] = ub[c[q]] == 0.f;
+      if (X.uz) X.uz[c[q]] = ub[c[q]] == real(0.);
    }
    for (int jj = j - 1; jj <= j + 1; jj++)
       for (int ii = i - 1; ii <= i + 1; ii++) dirty(X, ii, jj);
@@ -474,7 +497,7 @@ __device__ __forceinline__ uint8_t pflag_pt(const SlBasinDev &B, int i, int j, c
             f = 1;
             for (int k = 1; k <= 4; k++) {
                const int a = i + cIH[k], b = j + cJH[k];
-               if (hs(a, b) + z.b(a, b) == 0.f) nd++;
+               if (hs(a, b) + z.b(a, b) == real(0.)) nd++;
             }
             if (nd) f |= 2;
             *ndry = nd;
@@ -489,7 +512,7 @@ template <class HS, class PF, class Z>
 __device__ __forceinline__ uint8_t hclass_pt(const SlBasinDev &B, int i, int j, const HS &hs, const PF &pf,
                                              const Z &z)
 {
-   if (i <= B.imxb - 1 && j <= B.jmxb - 1 && hs(i, j) + z.b(i, j) != 0.f) {
+   if (i <= B.imxb - 1 && j <= B.jmxb - 1 && hs(i, j) + z.b(i, j) != real(0.)) {
       // momentum points having (i,j) as a corner: (i+1,j+1) (i,j+1) (i+1,j) (i,j)
       const uint8_t a = pf(i + 1, j + 1) | pf(i, j + 1) | pf(i + 1, j) | pf(i, j);
       return (a & 2) ? 2 : (a & 1) ? 1 : 0;
@@ -500,41 +523,41 @@ __device__ __forceinline__ uint8_t hclass_pt(const SlBasinDev &B, int i, int j, 
 // Metric weight of neighbour k of square p: on elliptic/hyperbolic grids
 // 2z/(1+z) - 1 with z = GAMFK/GAM0 (precomputed per square, g4), on polar grids
 // the constants GAMFP (flt2g.f:60-75, filter.f:140-160).
-__device__ __forceinline__ float gamf_of(const SlBasinDev &B, int k, uint32_t p)
+__device__ __forceinline__ real gamf_of(const SlBasinDev &B, int k, uint32_t p)
 {
    return B.ebsn_ell ? B.g4[(uint32_t) (k - 1) * B.npts + p] : B.gamfp[k - 1];
 }
 
 // FLT2G (both passes, when smpt) + BDRYHT for square (i,j), j <= JMXB-1.
 template <class HS, class CL, class Z>
-__device__ __forceinline__ float smooth_pt(const SlBasinDev &B, const float *P, const float *T, bool smpt,
+__device__ __forceinline__ real smooth_pt(const SlBasinDev &B, const real *P, const real *T, bool smpt,
                                            int i, int j, const HS &hs, const CL &cl, const Z &z)
 {
    const uint32_t p = IDX(B, i, j);
-   float out = hs(i, j);
+   real out = hs(i, j);
    if (i > B.imxb - 1) return out;
    if (smpt) {
       const uint8_t c = cl(i, j);
       if (c) {
-         const float hct = (c == 1) ? 1.f : 0.f;
-         const float hsub = out, zb = z.b(i, j);
+         const real hct = (c == 1) ? real(1.) : real(0.);
+         const real hsub = out, zb = z.b(i, j);
          if (!(hsub + zb <= hct)) {
-            float hsm = 0.f, hdif = 0.f, hzz, gamf;
+            real hsm = real(0.), hdif = real(0.), hzz, gamf;
             for (int k = 1; k <= 4; k++) {
                const int ii = i + cIIH[k], jj = j + cJJH[k];
                bool self = true;
                if (ii == 0 || jj == 0) {
-                  gamf = 0.f;
+                  gamf = real(0.);
                } else {
                   gamf = gamf_of(B, k, p);
                   if (!(ii == B.imxb || jj == B.jmxb)) {
                      if (cl(ii, jj) == c) {                     // HSUB(n) != 999
-                        const float hq = hs(ii, jj);
+                        const real hq = hs(ii, jj);
                         if (!(hq + z.b(ii, jj) <= hct)) {
                            const int kk = k % 4 + 1;
-                           const float z1 = z.m(i + cIP[k], j + cJP[k]);
-                           const float z2 = z.m(i + cIP[kk], j + cJP[kk]);
-                           const float zzz = fminf(z1, z2) + hct;
+                           const real z1 = z.m(i + cIP[k], j + cJP[k]);
+                           const real z2 = z.m(i + cIP[kk], j + cJP[kk]);
+                           const real zzz = fmin(z1, z2) + hct;
                            if (c == 1) self = (hsub < zzz || hq < zzz);
                            else self = (hsub < zzz && hq < zzz);
                            if (!self) {
@@ -551,50 +574,50 @@ __device__ __forceinline__ float smooth_pt(const SlBasinDev &B, const float *P, 
                }
                hdif = hdif + hzz * gamf;
             }
-            const float hbtemp = (1.f / 3.f) * hsub + (hsm + hdif) / 6.f;
-            out = fmaxf(hbtemp, -zb);
+            const real hbtemp = (real(1.) / real(3.)) * hsub + (hsm + hdif) / real(6.);
+            out = fmax(hbtemp, -zb);
          }
       }
    }
    if (B.flags[p] & F_BDRY) {
-      const float xr = B.elpcl[i - 1] * B.cosl[j - 1], yr = B.elpdl[i - 1] * B.sinl[j - 1];
-      const float x = xr - P[P_C1] - P[P_AX], y = yr - P[P_C2] - P[P_AY];
-      const float rsq = x * x + y * y, r1 = sqrtf(rsq) / 5280.f + 1.f;
+      const real xr = B.elpcl[i - 1] * B.cosl[j - 1], yr = B.elpdl[i - 1] * B.sinl[j - 1];
+      const real x = xr - P[P_C1] - P[P_AX], y = yr - P[P_C2] - P[P_AY];
+      const real rsq = x * x + y * y, r1 = sqrt(rsq) / real(5280.) + real(1.);
       int k = (int) r1;
-      const float r2 = (float) k, dr = r1 - r2;
+      const real r2 = (real) k, dr = r1 - r2;
       if (k > 790) k = 790;
-      const float *Rk = T + (uint32_t) k * 8, *Rk1 = Rk + 8;
-      out = Rk[R_DELP] + dr * (Rk1[R_DELP] - Rk[R_DELP]) + P[P_SEA] + 0.f;
+      const real *Rk = T + (uint32_t) k * 8, *Rk1 = Rk + 8;
+      out = Rk[R_DELP] + dr * (Rk1[R_DELP] - Rk[R_DELP]) + P[P_SEA] + real(0.);
    }
    return out;
 }
 
 // HMXSV at square (i,j) (i <= IMXB-1, j <= JMXB-1): the new envelope value.
 template <class HB, class Z>
-__device__ __forceinline__ float hmxsv_pt(const SlBasinDev &B, int i, int j, const HB &hb, float hx,
+__device__ __forceinline__ real hmxsv_pt(const SlBasinDev &B, int i, int j, const HB &hb, real hx,
                                           const Z &z)
 {
-   const float h = hb(i, j);
+   const real h = hb(i, j);
    if (!(hx < h)) return hx;
-   const float zb = z.b(i, j);
-   if (zb > 40.f || B.ebsn1_amp) return h;
-   if (h + zb < 1.f) return h;
-   float hsm = 0.f, hdif = 0.f, hzz, gamf;
+   const real zb = z.b(i, j);
+   if (zb > real(40.) || B.ebsn1_amp) return h;
+   if (h + zb < real(1.)) return h;
+   real hsm = real(0.), hdif = real(0.), hzz, gamf;
    const uint32_t p = IDX(B, i, j);
    for (int k = 1; k <= 4; k++) {
       const int ii = i + cIIH[k], jj = j + cJJH[k];
       if (ii == 0 || jj == 0) {
-         gamf = 0.f;
+         gamf = real(0.);
          hsm = hsm + h;
          hzz = h;
       } else {
          gamf = B.g4[(uint32_t) (k - 1) * B.npts + p];   // hmxsv.f:84-92
-         if (jj != B.jmxb && ii != B.imxb && hb(ii, jj) + z.b(ii, jj) >= 1.f) {
-            const float hq = hb(ii, jj);
+         if (jj != B.jmxb && ii != B.imxb && hb(ii, jj) + z.b(ii, jj) >= real(1.)) {
+            const real hq = hb(ii, jj);
             const int kk = k % 4 + 1;
-            const float z1 = z.m(i + cIP[k], j + cJP[k]);
-            const float z2 = z.m(i + cIP[kk], j + cJP[kk]);
-            const float zzz = fminf(z1, z2) + 1.f;
+            const real z1 = z.m(i + cIP[k], j + cJP[k]);
+            const real z2 = z.m(i + cIP[kk], j + cJP[kk]);
+            const real zzz = fmin(z1, z2) + real(1.);
             if (h < zzz || hq < zzz) {
                hsm = hsm + h;
                hzz = h;
@@ -609,29 +632,29 @@ __device__ __forceinline__ float hmxsv_pt(const SlBasinDev &B, int i, int j, con
       }
       hdif = hdif + hzz * gamf;
    }
-   const float htemp = (4.f * h + hsm + hdif) / 8.f;
-   return fmaxf(hx, htemp);
+   const real htemp = (real(4.) * h + hsm + hdif) / real(8.);
+   return fmax(hx, htemp);
 }
 
 // ------------------------------------------------------------ FILTER/XFILTER
 // Square (i,j) with i <= IMXB-1, j <= JMXB-1; hs is the HSUB snapshot, top
 // the HSUB row JMXB CONTTY saved (XFILTER reads it; IMXB column reads 0).
 template <class HS, class Z>
-__device__ __forceinline__ float filter_pt(const SlBasinDev &B, int i, int j, const HS &hs, const float *top,
+__device__ __forceinline__ real filter_pt(const SlBasinDev &B, int i, int j, const HS &hs, const real *top,
                                            const Z &z)
 {
-   const float zb = z.b(i, j), hsub = hs(i, j);
-   float out = hsub;
+   const real zb = z.b(i, j), hsub = hs(i, j);
+   real out = hsub;
    if (zb <= -B.hter) return out;
    if (B.ismth == 1) {
-      const float hcrt = 0.5f;
+      const real hcrt = real(0.5);
       if (!(hsub + zb < hcrt)) {
-         float hsm = 0.f, hdif = 0.f, hzz, gamf;
+         real hsm = real(0.), hdif = real(0.), hzz, gamf;
          const uint32_t p = IDX(B, i, j);
          for (int k = 1; k <= 4; k++) {
             const int ii = i + cIIH[k], jj = j + cJJH[k];
             if (ii == 0 || jj == 0) {
-               gamf = 0.f;
+               gamf = real(0.);
                hsm = hsm + hsub;
                hzz = hsub;
             } else {
@@ -640,11 +663,11 @@ __device__ __forceinline__ float filter_pt(const SlBasinDev &B, int i, int j, co
                   hsm = hsm + hsub;
                   hzz = hsub;
                } else {
-                  const float hq = hs(ii, jj);
+                  const real hq = hs(ii, jj);
                   const int kk = k % 4 + 1;
-                  const float z1 = z.m(i + cIP[k], j + cJP[k]);
-                  const float z2 = z.m(i + cIP[kk], j + cJP[kk]);
-                  const float zzz = fminf(z1, z2) + hcrt;
+                  const real z1 = z.m(i + cIP[k], j + cJP[k]);
+                  const real z2 = z.m(i + cIP[kk], j + cJP[kk]);
+                  const real zzz = fmin(z1, z2) + hcrt;
                   if (hsub < zzz || hq < zzz) {
                      hsm = hsm + hsub;
                      hzz = hsub;
@@ -656,12 +679,12 @@ __device__ __forceinline__ float filter_pt(const SlBasinDev &B, int i, int j, co
             }
             hdif = hdif + hzz * gamf;
          }
-         const float htemp = 0.5f * hsub + (hsm + hdif) / 8.f;
-         out = fmaxf(-zb, htemp);
+         const real htemp = real(0.5) * hsub + (hsm + hdif) / real(8.);
+         out = fmax(-zb, htemp);
       }
    } else if (!(hsub + zb < B.exclude)) {
-      const float hcrt = B.hcrt;
-#define HX(a, b) ((a) == B.imxb ? 0.f : ((b) == B.jmxb ? top[(a) - 1] : hs(a, b)))
+      const real hcrt = B.hcrt;
+#define HX(a, b) ((a) == B.imxb ? real(0.) : ((b) == B.jmxb ? top[(a) - 1] : hs(a, b)))
 #define ZX(a, b) (z.m(a, b))
       const int IIX[5] = {0, -1, 1, 1, -1}, JJX[5] = {0, -1, -1, 1, 1};
       const int IP1[5] = {0, -1, 1, 2, 0}, JP1[5] = {0, 0, -1, 1, 2};
@@ -672,23 +695,23 @@ __device__ __forceinline__ float filter_pt(const SlBasinDev &B, int i, int j, co
          if (ii == 0 || jj == 0 || ix == 0 || jx == 0) continue;
          if (!(ii == B.imxb || jj == B.jmxb || HX(ii, jj) + z.b(ii, jj) < hcrt)) {
             const int kk2 = k % 4 + 1;
-            const float zzz = fminf(ZX(i + cIP[k], j + cJP[k]), ZX(i + cIP[kk2], j + cJP[kk2])) + hcrt;
+            const real zzz = fmin(ZX(i + cIP[k], j + cJP[k]), ZX(i + cIP[kk2], j + cJP[kk2])) + hcrt;
             if (!(hsub < zzz && HX(ii, jj) < zzz)) maskii[k] = 1;
          }
          if (!(ix == B.imxb || jx == B.jmxb || HX(ix, jx) + z.b(ix, jx) < hcrt)) {
             const int kk = k % 4 + 1, km1 = (k + 2) % 4 + 1;
-            const float z0 = ZX(i + cIP[k], j + cJP[k]), zz = ZX(i + cIP[kk], j + cJP[kk]);
-            const float zzz = fminf(z0, zz) + hcrt;
-            const float yyy = fminf(z0, ZX(i + IP2[k], j + JP2[k])) + hcrt;
-            const float xxx = fminf(z0, ZX(i + IP1[k], j + JP1[k])) + hcrt;
-            const float www = fminf(z0, ZX(i + cIP[km1], j + cJP[km1])) + hcrt;
-            const float hii = HX(ii, jj), hix = HX(ix, jx), ha1 = HX(i + IP1[k], j + JP1[k]);
+            const real z0 = ZX(i + cIP[k], j + cJP[k]), zz = ZX(i + cIP[kk], j + cJP[kk]);
+            const real zzz = fmin(z0, zz) + hcrt;
+            const real yyy = fmin(z0, ZX(i + IP2[k], j + JP2[k])) + hcrt;
+            const real xxx = fmin(z0, ZX(i + IP1[k], j + JP1[k])) + hcrt;
+            const real www = fmin(z0, ZX(i + cIP[km1], j + cJP[km1])) + hcrt;
+            const real hii = HX(ii, jj), hix = HX(ix, jx), ha1 = HX(i + IP1[k], j + JP1[k]);
             if (((hsub >= zzz && hii >= yyy) || (hix >= yyy && hii >= zzz)) ||
                 ((hsub >= www && ha1 >= xxx) || (hix >= xxx && ha1 >= www)))
                maskix[k] = 1;
          }
       }
-      float hsm2 = 0.f, xsm2 = 0.f;
+      real hsm2 = real(0.), xsm2 = real(0.);
       for (int km = 1; km <= 4; km++) {
          const int kkm = km % 4 + 1, km1m = (km + 2) % 4 + 1;
          const int ix = i + IIX[km], jx = j + JJX[km], ii = i + cIIH[km], jj = j + cJJH[km];
@@ -696,17 +719,17 @@ __device__ __forceinline__ float filter_pt(const SlBasinDev &B, int i, int j, co
              i + IIX[km1m] == 0 || j + JJX[km1m] == 0 || i + cIIH[kkm] == 0 ||
              j + cJJH[kkm] == 0 || i + cIIH[km1m] == 0 || j + cJJH[km1m] == 0)
             continue;
-         hsm2 = hsm2 + (float) maskii[km] * HX(ii, jj) - (float) maskii[km] * hsub;
+         hsm2 = hsm2 + (real) maskii[km] * HX(ii, jj) - (real) maskii[km] * hsub;
          if (B.xver == 1) {
-            const float w = (float) (maskix[km] * maskii[km] * maskix[kkm]);
-            xsm2 = xsm2 + 0.5f * (w * HX(ix, jx) - w * hsub);
+            const real w = (real) (maskix[km] * maskii[km] * maskix[kkm]);
+            xsm2 = xsm2 + real(0.5) * (w * HX(ix, jx) - w * hsub);
          } else {
-            xsm2 = xsm2 + 0.5f * 1.f * (float) maskix[km] * (float) maskii[km] *
-                          (float) maskii[km1m] * (HX(ix, jx) - hsub);
+            xsm2 = xsm2 + real(0.5) * real(1.) * (real) maskix[km] * (real) maskii[km] *
+                          (real) maskii[km1m] * (HX(ix, jx) - hsub);
          }
       }
-      const float htemp = hsub + 1.f / 8.f * (hsm2 - 1.f * (xsm2));
-      out = fmaxf(-zb, htemp);
+      const real htemp = hsub + real(1.) / real(8.) * (hsm2 - real(1.) * (xsm2));
+      out = fmax(-zb, htemp);
 #undef HX
 #undef ZX
    }
@@ -715,86 +738,86 @@ __device__ __forceinline__ float filter_pt(const SlBasinDev &B, int i, int j, co
 
 // ------------------------------------------------------------------ MOMNTM
 struct Forcing {
-   float dph;
+   real dph;
    int id, iik;
 };
 
-__device__ static void frcpnt(const SlBasinDev &B, const float *P, const float *T, float x, float y,
-                              float &fxb, float &fyb, int ncatg, float &cshltr, int npls,
+__device__ static void frcpnt(const SlBasinDev &B, const real *P, const real *T, real x, real y,
+                              real &fxb, real &fyb, int ncatg, real &cshltr, int npls,
                               const Forcing &F)
 {
-   const float xp = x - P[P_C1] - P[P_AX], yp = y - P[P_C2] - P[P_AY];
-   const float rsq = xp * xp + yp * yp, rs = sqrtf(rsq), r1 = rs / 5280.f + 1.f;
+   const real xp = x - P[P_C1] - P[P_AX], yp = y - P[P_C2] - P[P_AY];
+   const real rsq = xp * xp + yp * yp, rs = sqrt(rsq), r1 = rs / real(5280.) + real(1.);
    int k = (int) r1;
-   const float r2 = (float) k, dr = r1 - r2;
+   const real r2 = (real) k, dr = r1 - r2;
    if (k > 790) k = 790;
-   const float ccn = P[P_X15] + rsq;
-   const float x1218 = (ncatg == 1) ? P[P_W0] : P[P_W1];
-   float ck1, sk1;
-   const float *Rk = T + (uint32_t) k * 8, *Rk1 = Rk + 8;
+   const real ccn = P[P_X15] + rsq;
+   const real x1218 = (ncatg == 1) ? P[P_W0] : P[P_W1];
+   real ck1, sk1;
+   const real *Rk = T + (uint32_t) k * 8, *Rk1 = Rk + 8;
    if (ncatg == 1) {
       ck1 = Rk[R_CW] + dr * (Rk1[R_CW] - Rk[R_CW]);
       sk1 = Rk[R_SW] + dr * (Rk1[R_SW] - Rk[R_SW]);
    } else {
-      cshltr = 1.f;
+      cshltr = real(1.);
       ck1 = Rk[R_C] + dr * (Rk1[R_C] - Rk[R_C]);
       sk1 = Rk[R_S] + dr * (Rk1[R_S] - Rk[R_S]);
    }
-   float a = rs * P[P_X9] - x1218 * (yp * ck1 + xp * sk1);
-   float b = rs * P[P_X10] + x1218 * (xp * ck1 - yp * sk1);
+   real a = rs * P[P_X9] - x1218 * (yp * ck1 + xp * sk1);
+   real b = rs * P[P_X10] + x1218 * (xp * ck1 - yp * sk1);
    if (ncatg == 1) {
-      const float rhol = fabsf(xp * P[P_X20] + yp * P[P_X21]);
-      const float ccc = ccn * rhol / (P[P_X15] + rhol * rhol);
+      const real rhol = fabs(xp * P[P_X20] + yp * P[P_X21]);
+      const real ccc = ccn * rhol / (P[P_X15] + rhol * rhol);
       a = a + ccc * P[P_X23];
       b = b + ccc * P[P_X24];
    }
    a = a * P[P_X4] / ccn;
    b = b * P[P_X4] / ccn;
-   float fctt = 3.E-6f * sqrtf(a * a + b * b);
+   real fctt = real(3.E-6) * sqrt(a * a + b * b);
    fctt = fctt * cshltr * P[P_BT];
-   const float fxx = fctt * a, fyy = fctt * b;
-   const float *E = EREC(B, F.iik, F.id);
-   const float cr = E[E_CR2], ci = E[E_CI2];
+   const real fxx = fctt * a, fyy = fctt * b;
+   const real *E = EREC(B, F.iik, F.id);
+   const real cr = E[E_CR2], ci = E[E_CI2];
    fxb = cr * fxx - ci * fyy;
    fyb = cr * fyy + ci * fxx;
    if (npls != 0) {
-      const float aa = -F.dph * (Rk[R_P] + dr * (Rk1[R_P] - Rk[R_P]));
-      const float pxx = aa * xp, pyy = aa * yp;
-      const float br = E[E_BR2], bi = E[E_BI2];
+      const real aa = -F.dph * (Rk[R_P] + dr * (Rk1[R_P] - Rk[R_P]));
+      const real pxx = aa * xp, pyy = aa * yp;
+      const real br = E[E_BR2], bi = E[E_BI2];
       fxb = fxb + br * pxx - bi * pyy;
       fyb = fyb + br * pyy + bi * pxx;
    }
 }
 
-__device__ static void mntmbd(const SlBasinDev &B, const float *P, const float *T, float *ub, float *vb,
-                              int jump, int ijump, int isb, int jsb, float arj, float aii, uint32_t pij,
-                              float ubj, float vbj, float &fxb, float &fyb, int ncatg, float &cshltr,
+__device__ static void mntmbd(const SlBasinDev &B, const real *P, const real *T, real *ub, real *vb,
+                              int jump, int ijump, int isb, int jsb, real arj, real aii, uint32_t pij,
+                              real ubj, real vbj, real &fxb, real &fyb, int ncatg, real &cshltr,
                               const Forcing &F)
 {
    const uint32_t q = IDX(B, isb, jsb);
-   const float aiul = aii * (ub[q] - ubj), aivb = aii * (vb[q] - vbj);
+   const real aiul = aii * (ub[q] - ubj), aivb = aii * (vb[q] - vbj);
    if (jump == 2) {
-      const float xttx = 0.f, xtty = 0.f;
+      const real xttx = real(0.), xtty = real(0.);
       ub[q] = ub[pij] + aiul + arj * (vb[q] - vbj) + xttx;
       vb[q] = vb[pij] + aivb - arj * (ub[q] - ub[pij]) + xtty;
       return;
    }
-   float fxbp, fybp;
+   real fxbp, fybp;
    if (ijump != 2) {
-      const float fxd = fxb, fyd = fyb;
-      float xr = B.elpct[isb - 1] * B.cost[jsb - 1], yr = B.elpdt[isb - 1] * B.sint[jsb - 1];
+      const real fxd = fxb, fyd = fyb;
+      real xr = B.elpct[isb - 1] * B.cost[jsb - 1], yr = B.elpdt[isb - 1] * B.sint[jsb - 1];
       frcpnt(B, P, T, xr, yr, fxbp, fybp, ncatg, cshltr, 1, F);
       xr = B.elpdt[isb - 1] * B.cost[jsb - 1];
       yr = B.elpct[isb - 1] * B.sint[jsb - 1];
       fxb = xr * fxbp + yr * fybp;
       fyb = xr * fybp - yr * fxbp;
-      const float xttx = P[P_DELT] * (fxb - fxd), xtty = P[P_DELT] * (fyb - fyd);
+      const real xttx = P[P_DELT] * (fxb - fxd), xtty = P[P_DELT] * (fyb - fyd);
       ub[q] = ub[pij] + aiul + arj * (vb[q] - vbj) + xttx;
       vb[q] = vb[pij] + aivb - arj * (ub[q] - ub[pij]) + xtty;
       return;
    }
-   cshltr = 1.f;
-   float xr = B.elpct[isb - 1] * B.cost[jsb - 1], yr = B.elpdt[isb - 1] * B.sint[jsb - 1];
+   cshltr = real(1.);
+   real xr = B.elpct[isb - 1] * B.cost[jsb - 1], yr = B.elpdt[isb - 1] * B.sint[jsb - 1];
    frcpnt(B, P, T, xr, yr, fxbp, fybp, ncatg, cshltr, 0, F);
    xr = B.elpdt[isb - 1] * B.cost[jsb - 1];
    yr = B.elpct[isb - 1] * B.sint[jsb - 1];
@@ -808,37 +831,37 @@ __device__ static void mntmbd(const SlBasinDev &B, const float *P, const float *
 // four squares around it.  Writes UB/VB at (i,j) and at the boundary points
 // this point owns.
 template <class HB, class Z>
-__device__ __forceinline__ void momntm_pt(const SlBasinDev &B, const float *P, const float *T, uint32_t fl,
-                                          int i, int j, const HB &hb, const Z &z, float *ub, float *vb)
+__device__ __forceinline__ void momntm_pt(const SlBasinDev &B, const real *P, const real *T, uint32_t fl,
+                                          int i, int j, const HB &hb, const Z &z, real *ub, real *vb)
 {
    const uint32_t p = IDX(B, i, j);
-   const float h1 = hb(i - 1, j - 1), h2 = hb(i, j - 1), h3 = hb(i - 1, j), h4 = hb(i, j);
-   const float za = z.b(i - 1, j - 1), zb2 = z.b(i, j - 1), zc = z.b(i - 1, j), zp = z.b(i, j);
-   const float zbm = z.m(i, j);
+   const real h1 = hb(i - 1, j - 1), h2 = hb(i, j - 1), h3 = hb(i - 1, j), h4 = hb(i, j);
+   const real za = z.b(i - 1, j - 1), zb2 = z.b(i, j - 1), zc = z.b(i - 1, j), zp = z.b(i, j);
+   const real zbm = z.m(i, j);
    Forcing F;
    F.iik = (int) F_IIK(fl);
    const int ncatg = (fl & F_OCEAN) ? 2 : 1;
    int jump = 1;
-   float dhx, dhy, aintrp, arj, aii, fxb = 0.f, fyb = 0.f, cshltr = 0.f, ubj, vbj;
+   real dhx, dhy, aintrp, arj, aii, fxb = real(0.), fyb = real(0.), cshltr = real(0.), ubj, vbj;
    bool blocked = false;
-   if (ncatg != 1 && (fl & F_IT2) && zbm < -50.f) {
+   if (ncatg != 1 && (fl & F_IT2) && zbm < -real(50.)) {
       dhx = -h1 + h2 - h3 + h4;
       dhy = -h1 - h2 + h3 + h4;
-      F.dph = .25f * (h1 + h2 + h3 + h4 + za + zb2 + zc + zp);
-      const float dphz = fminf(599.f, F.dph);
-      F.id = (int) fminf(599.f, dphz);
+      F.dph = real(.25) * (h1 + h2 + h3 + h4 + za + zb2 + zc + zp);
+      const real dphz = fmin(real(599.), F.dph);
+      F.id = (int) fmin(real(599.), dphz);
       if (F.id <= 0) F.id = 1;
-      const float dphi = (float) F.id;
-      const float *E = EREC(B, F.iik, F.id);
+      const real dphi = (real) F.id;
+      const real *E = EREC(B, F.iik, F.id);
       aintrp = E[E_AI2] + E[E_SLP] * (dphz - dphi);
    } else {
-      const float hmn = fminf(fminf(fminf(h4, h3), h2), h1);
+      const real hmn = fmin(fmin(fmin(h4, h3), h2), h1);
       if (hmn <= zbm) {
-         const float hmxx = fmaxf(fmaxf(fmaxf(h4, h3), h2), h1);
+         const real hmxx = fmax(fmax(fmax(h4, h3), h2), h1);
          if (hmxx <= zbm) {
             blocked = true;
          } else {
-            const float hf1 = fmaxf(h1, zbm), hf2 = fmaxf(h2, zbm), hf3 = fmaxf(h3, zbm), hf4 = fmaxf(h4, zbm);
+            const real hf1 = fmax(h1, zbm), hf2 = fmax(h2, zbm), hf3 = fmax(h3, zbm), hf4 = fmax(h4, zbm);
             jump = 2;
             dhx = -hf1 + hf2 - hf3 + hf4;
             dhy = -hf1 - hf2 + hf3 + hf4;
@@ -848,73 +871,73 @@ __device__ __forceinline__ void momntm_pt(const SlBasinDev &B, const float *P, c
          dhy = -h1 - h2 + h3 + h4;
       }
       if (!blocked) {
-         float hpd[4];
+         real hpd[4];
          hpd[0] = h1 + za;
          hpd[1] = h2 + zb2;
          hpd[2] = h3 + zc;
          hpd[3] = h4 + zp;
-         F.dph = .25f * (hpd[0] + hpd[1] + hpd[2] + hpd[3]);
-         F.id = (int) fminf(599.f, F.dph);
-         if (F.dph <= 5.f) {
+         F.dph = real(.25) * (hpd[0] + hpd[1] + hpd[2] + hpd[3]);
+         F.id = (int) fmin(real(599.), F.dph);
+         if (F.dph <= real(5.)) {
             if (jump != 2) {
                for (int k = 0; k < 4; k++) {
-                  if (hpd[k] < 5.f) hpd[k] = hpd[k] * (.4f + .12f * hpd[k] - .008f * hpd[k] * hpd[k]);
-                  else hpd[k] = hpd[k] - 1.f;
+                  if (hpd[k] < real(5.)) hpd[k] = hpd[k] * (real(.4) + real(.12) * hpd[k] - real(.008) * hpd[k] * hpd[k]);
+                  else hpd[k] = hpd[k] - real(1.);
                }
-               F.dph = .25f * (hpd[0] + hpd[1] + hpd[2] + hpd[3]);
-               F.id = (int) fminf(599.f, F.dph);
+               F.dph = real(.25) * (hpd[0] + hpd[1] + hpd[2] + hpd[3]);
+               F.id = (int) fmin(real(599.), F.dph);
             }
             if (F.id < 1) F.id = 1;
          }
-         aintrp = 0.f;
+         aintrp = real(0.);
          for (int k = 0; k < 4; k++) {
-            const float dz = hpd[k];
-            int idd = (int) fminf(599.f, dz);
+            const real dz = hpd[k];
+            int idd = (int) fmin(real(599.), dz);
             if (idd < 1) idd = 1;
-            const float dphi = (float) idd;
-            const float *E = EREC(B, F.iik, idd);
-            const float aitr = E[E_AI2] + E[E_SLP] * (dz - dphi);
+            const real dphi = (real) idd;
+            const real *E = EREC(B, F.iik, idd);
+            const real aitr = E[E_AI2] + E[E_SLP] * (dz - dphi);
             aintrp = aintrp + aitr;
          }
-         aintrp = .25f * aintrp;
+         aintrp = real(.25) * aintrp;
       }
    }
    if (blocked) {
-      ub[p] = 0.f;
-      vb[p] = 0.f;
-      ubj = 0.f;
-      vbj = 0.f;
+      ub[p] = real(0.);
+      vb[p] = real(0.);
+      ubj = real(0.);
+      vbj = real(0.);
    } else {
-      aii = 1.f + P[P_X1B4] * aintrp;
-      const float *E = EREC(B, F.iik, F.id);
+      aii = real(1.) + P[P_X1B4] * aintrp;
+      const real *E = EREC(B, F.iik, F.id);
       arj = E[E_AR2] * P[P_X1B4] * B.zcor[p];
-      const float aiu = aii * ub[p], aiv = aii * vb[p];
-      const float fct = P[P_X1B6] * F.dph;
-      float xttx, xtty;
+      const real aiu = aii * ub[p], aiv = aii * vb[p];
+      const real fct = P[P_X1B6] * F.dph;
+      real xttx, xtty;
       if (jump != 2) {
          if (ncatg == 2) {
-            cshltr = 1.f;
+            cshltr = real(1.);
          } else {
-            const float hf1 = h1 - zbm, hf2 = h2 - zbm, hf3 = h3 - zbm, hf4 = h4 - zbm;
+            const real hf1 = h1 - zbm, hf2 = h2 - zbm, hf3 = h3 - zbm, hf4 = h4 - zbm;
             if (!(fl & F_TREE1)) {
-               cshltr = .25f * (fminf(hf1, 1.f) + fminf(hf2, 1.f) + fminf(hf3, 1.f) + fminf(hf4, 1.f));
+               cshltr = real(.25) * (fmin(hf1, real(1.)) + fmin(hf2, real(1.)) + fmin(hf3, real(1.)) + fmin(hf4, real(1.)));
             } else {
-               const float hfx[4] = {hf1, hf2, hf3, hf4};
-               cshltr = 0.f;
+               const real hfx[4] = {hf1, hf2, hf3, hf4};
+               cshltr = real(0.);
                for (int kx = 0; kx < 4; kx++) {
-                  float hfxx = 1.f;
-                  if (hfx[kx] <= 0.f) hfxx = 0.f;
-                  else if (hfx[kx] < 50.f) {
-                     const float q = hfx[kx] / 50.f;
+                  real hfxx = real(1.);
+                  if (hfx[kx] <= real(0.)) hfxx = real(0.);
+                  else if (hfx[kx] < real(50.)) {
+                     const real q = hfx[kx] / real(50.);
                      hfxx = q * q;
                   }
                   cshltr = cshltr + hfxx;
                }
-               cshltr = .25f * cshltr;
+               cshltr = real(.25) * cshltr;
             }
          }
-         float fxbp, fybp;
-         float xr = B.elpct[i - 1] * B.cost[j - 1], yr = B.elpdt[i - 1] * B.sint[j - 1];
+         real fxbp, fybp;
+         real xr = B.elpct[i - 1] * B.cost[j - 1], yr = B.elpdt[i - 1] * B.sint[j - 1];
          frcpnt(B, P, T, xr, yr, fxbp, fybp, ncatg, cshltr, 1, F);
          xr = B.elpdt[i - 1] * B.cost[j - 1];
          yr = B.elpct[i - 1] * B.sint[j - 1];
@@ -923,26 +946,26 @@ __device__ __forceinline__ void momntm_pt(const SlBasinDev &B, const float *P, c
          xttx = P[P_DELT] * fxb;
          xtty = P[P_DELT] * fyb;
       } else {
-         xttx = 0.f;
-         xtty = 0.f;
+         xttx = real(0.);
+         xtty = real(0.);
       }
       ubj = ub[p];
       vbj = vb[p];
-      const float br = E[E_BR2], bi = E[E_BI2];
+      const real br = E[E_BR2], bi = E[E_BI2];
       ub[p] = aiu - fct * (br * dhx - bi * dhy) + arj * vb[p] + xttx;
       vb[p] = aiv - fct * (br * dhy + bi * dhx) - arj * ub[p] + xtty;
    }
    // boundary transports (momntm.f:384-458)
    const uint32_t bop = F_BOP(fl);
    if (bop == 0) return;
-   const float uij = ub[p];
+   const real uij = ub[p];
    int ijump = 1;
    if (bop <= 4) {
       const int isb = i + cISBS[bop], jsb = j + cJSBS[bop];
       const uint32_t q = IDX(B, isb, jsb);
-      if (uij == 0.f) {
-         ub[q] = 0.f;
-         vb[q] = 0.f;
+      if (uij == real(0.)) {
+         ub[q] = real(0.);
+         vb[q] = real(0.);
       } else {
          const uint32_t fq = B.flags[q];
          if (!(fq & F_BSKIP)) {
@@ -957,15 +980,15 @@ __device__ __forceinline__ void momntm_pt(const SlBasinDev &B, const float *P, c
          const int k = cISHF[icor][kk];
          const int isb = i + cIH[icor] + cIP[k], jsb = j + cJH[icor] + cJP[k];
          const uint32_t q = IDX(B, isb, jsb);
-         if (uij != 0.f) {
+         if (uij != real(0.)) {
             const uint32_t fq = B.flags[q];
             if (fq & F_BSKIP) break;
             if (fq & F_B8) ijump = 2;
             mntmbd(B, P, T, ub, vb, jump, ijump, isb, jsb, arj, aii, p, ubj, vbj, fxb, fyb, ncatg,
                    cshltr, F);
          } else {
-            ub[q] = 0.f;
-            vb[q] = 0.f;
+            ub[q] = real(0.);
+            vb[q] = real(0.);
          }
       }
    }
@@ -1016,13 +1039,13 @@ __global__ void __launch_bounds__(TX * TY) k_smooth(SlBasinDev B, SlRunDev R, St
    const int s = blockIdx.z;
    if (i > B.imxb || j > B.jmxb) return;
    const size_t so = (size_t) s * R.fstride, p = IDX(B, i, j);
-   const float *hs = A.hc + so;
-   const float *P = par(R, s, A.step);
+   const real *hs = A.hc + so;
+   const real *P = par(R, s, A.step);
    if (j == B.jmxb) {
       A.hn[so + p] = (i <= B.imxb - 1) ? hs[IDX(B, i, 1)] : hs[p];
       return;
    }
-   const bool smpt = __float_as_int(P[P_FLAGS]) & SF_SMPT;
+   const bool smpt = (int) P[P_FLAGS] & SF_SMPT;
    A.hn[so + p] = smooth_pt(B, P, tabv(R, s, P), smpt, i, j, GF{hs, B.pitch}, GB{R.hclass + so, B.pitch},
                             ZG{B.zb, B.zbm, B.pitch});
 }
@@ -1033,9 +1056,9 @@ __global__ void __launch_bounds__(TX * TY) k_hmxsv(SlBasinDev B, SlRunDev R, Ste
    const int s = blockIdx.z;
    if (i > B.imxb - 1 || j > B.jmxb - 1) return;
    const size_t so = (size_t) s * R.fstride, p = IDX(B, i, j);
-   const float hx = R.hmx[so + p];
-   const float v = hmxsv_pt(B, i, j, GF{A.hc + so, B.pitch}, hx, ZG{B.zb, B.zbm, B.pitch});
-   if (__float_as_uint(v) != __float_as_uint(hx)) R.hmx[so + p] = v;
+   const real hx = R.hmx[so + p];
+   const real v = hmxsv_pt(B, i, j, GF{A.hc + so, B.pitch}, hx, ZG{B.zb, B.zbm, B.pitch});
+   if (!same_bits(v, hx)) R.hmx[so + p] = v;
 }
 
 __global__ void __launch_bounds__(TX * TY) k_filter(SlBasinDev B, SlRunDev R, StepArgs A)
@@ -1044,8 +1067,8 @@ __global__ void __launch_bounds__(TX * TY) k_filter(SlBasinDev B, SlRunDev R, St
    const int s = blockIdx.z;
    if (i > B.imxb || j > B.jmxb) return;
    const size_t so = (size_t) s * R.fstride, p = IDX(B, i, j);
-   const float *hs = A.hc + so;
-   float out = hs[p];
+   const real *hs = A.hc + so;
+   real out = hs[p];
    if (i <= B.imxb - 1 && j <= B.jmxb - 1)
       out = filter_pt(B, i, j, GF{hs, B.pitch}, R.hsubtop + (size_t) s * B.pitch, ZG{B.zb, B.zbm, B.pitch});
    A.hn[so + p] = out;
@@ -1059,7 +1082,7 @@ __global__ void __launch_bounds__(TX * TY) k_momntm(SlBasinDev B, SlRunDev R, St
    const uint32_t fl = B.flags[IDX(B, i, j)];
    if (!(fl & F_MOM)) return;
    const size_t so = (size_t) s * R.fstride;
-   const float *P = par(R, s, A.step);
+   const real *P = par(R, s, A.step);
    momntm_pt(B, P, tabv(R, s, P), fl, i, j, GF{A.hc + so, B.pitch}, ZG{B.zb, B.zbm, B.pitch}, R.ub + so, R.vb + so);
 }
 
@@ -1092,14 +1115,14 @@ __global__ void __launch_bounds__(256) k_stepB(SlBasinDev B, SlRunDev R, StepArg
    using RH = Reg<SMPT ? SLO + 2 : SLO, SMPT ? SHI + 2 : SHI, TH>;
    using RF = Reg<1, 0, TH>;
    using RZ = Reg<SLO + 2, SHI + 2, TH>;
-   __shared__ float sh_hs[RH::n];
-   __shared__ float sh_s[RS::n];
-   __shared__ float sh_f[FILT ? RF::n : 1];
+   __shared__ real sh_hs[RH::n];
+   __shared__ real sh_s[RS::n];
+   __shared__ real sh_f[FILT ? RF::n : 1];
    __shared__ uint8_t sh_uz[SMPT ? RU::n : 1];
    __shared__ uint8_t sh_pf[SMPT ? RP::n : 1];
    __shared__ uint8_t sh_cl[SMPT ? RC::n : 1];
-   __shared__ float sh_zb[RZ::n];
-   __shared__ float sh_zbm[RZ::n];
+   __shared__ real sh_zb[RZ::n];
+   __shared__ real sh_zbm[RZ::n];
    __shared__ int sh_cnt[4];
 
    int tx, ty, s;
@@ -1108,21 +1131,21 @@ __global__ void __launch_bounds__(256) k_stepB(SlBasinDev B, SlRunDev R, StepArg
    const int I0 = tx * TX + 1, J0 = ty * TH + 1;
    const int IM = B.imxb, JM = B.jmxb;
    const size_t so = (size_t) s * R.fstride;
-   const float *hs_g = A.hc + so;
-   const float *P = par(R, s, A.step);
-   const float *T = tabv(R, s, P);
+   const real *hs_g = A.hc + so;
+   const real *P = par(R, s, A.step);
+   const real *T = tabv(R, s, P);
    if (tid < 4) sh_cnt[tid] = 0;
 
    // HB after CONTTY (and the U==0 mask) with halo; the static ZB and ZBM
    for (int e = tid; e < RH::n; e += 256) {
       const int gi = I0 - RH::lo + e % RH::w, gj = J0 - RH::lo + e / RH::w;
-      sh_hs[e] = (gi >= 1 && gi <= IM && gj >= 1 && gj <= JM) ? hs_g[IDX(B, gi, gj)] : 0.f;
+      sh_hs[e] = (gi >= 1 && gi <= IM && gj >= 1 && gj <= JM) ? hs_g[IDX(B, gi, gj)] : real(0.);
    }
    for (int e = tid; e < RZ::n; e += 256) {
       const int gi = I0 - RZ::lo + e % RZ::w, gj = J0 - RZ::lo + e / RZ::w;
       const bool in = gi >= 1 && gi <= IM && gj >= 1 && gj <= JM;
-      sh_zb[e] = in ? B.zb[IDX(B, gi, gj)] : 0.f;
-      sh_zbm[e] = in ? B.zbm[IDX(B, gi, gj)] : 0.f;
+      sh_zb[e] = in ? B.zb[IDX(B, gi, gj)] : real(0.);
+      sh_zbm[e] = in ? B.zbm[IDX(B, gi, gj)] : real(0.);
    }
    if (SMPT) {
       const uint8_t *uz_g = R.uz + so;
@@ -1184,7 +1207,7 @@ __global__ void __launch_bounds__(256) k_stepB(SlBasinDev B, SlRunDev R, StepArg
    // FLT2G + BDRYHT + row copy
    for (int e = tid; e < RS::n; e += 256) {
       const int gi = I0 - RS::lo + e % RS::w, gj = J0 - RS::lo + e / RS::w;
-      float v = 0.f;
+      real v = real(0.);
       if (gi >= 1 && gi <= IM && gj >= 1 && gj <= JM) {
          if (gj == JM) v = (gi <= IM - 1) ? hs_g[IDX(B, gi, 1)] : hsL(gi, gj);
          else v = smooth_pt(B, P, T, SMPT, gi, gj, hsL, clL, zL);
@@ -1194,22 +1217,22 @@ __global__ void __launch_bounds__(256) k_stepB(SlBasinDev B, SlRunDev R, StepArg
    __syncthreads();
    const LF sL{sh_s, I0 - RS::lo, J0 - RS::lo, RS::w};
    // HMXSV on the tile's own squares
-   float *hmx = R.hmx + so;
+   real *hmx = R.hmx + so;
    for (int e = tid; e < TX * TH; e += 256) {
       const int gi = I0 + e % TX, gj = J0 + e / TX;
       if (gi <= IM - 1 && gj <= JM - 1) {
          const uint32_t p = IDX(B, gi, gj);
-         const float hx = hmx[p];
-         const float v = hmxsv_pt(B, gi, gj, sL, hx, zL);
-         if (__float_as_uint(v) != __float_as_uint(hx)) hmx[p] = v;
+         const real hx = hmx[p];
+         const real v = hmxsv_pt(B, gi, gj, sL, hx, zL);
+         if (!same_bits(v, hx)) hmx[p] = v;
       }
    }
-   float *hn_g = A.hn + so;
+   real *hn_g = A.hn + so;
    if (FILT) {
-      const float *top = R.hsubtop + (size_t) s * B.pitch;
+      const real *top = R.hsubtop + (size_t) s * B.pitch;
       for (int e = tid; e < RF::n; e += 256) {
          const int gi = I0 - RF::lo + e % RF::w, gj = J0 - RF::lo + e / RF::w;
-         float v = 0.f;
+         real v = real(0.);
          if (gi >= 1 && gi <= IM && gj >= 1 && gj <= JM)
             v = (gi <= IM - 1 && gj <= JM - 1) ? filter_pt(B, gi, gj, sL, top, zL) : sL(gi, gj);
          sh_f[e] = v;
@@ -1218,7 +1241,7 @@ __global__ void __launch_bounds__(256) k_stepB(SlBasinDev B, SlRunDev R, StepArg
    }
    const LF fL{FILT ? sh_f : sh_s, I0 - (FILT ? RF::lo : RS::lo), J0 - (FILT ? RF::lo : RS::lo),
                FILT ? RF::w : RS::w};
-   float *ub = R.ub + so, *vb = R.vb + so;
+   real *ub = R.ub + so, *vb = R.vb + so;
    for (int e = tid; e < TX * TH; e += 256) {
       const int gi = I0 + e % TX, gj = J0 + e / TX;
       if (gi <= IM && gj <= JM) {
@@ -1244,11 +1267,14 @@ static void launch_B(int smpt, int filt, dim3 g, hipStream_t st, const SlBasinDe
 // 1 after it, 2 after the replay, 3 after stage B); path 1 = staged (events
 // around each of the SL_NKERN stage kernels).  Returns which HB buffer holds
 // the step's result.
-extern "C" int sl_dev_launch_step(const SlBasinDev *B, const SlRunDev *R, int step, int smpt,
-                                  int filt, int parity, int path, int bth, hipStream_t st,
-                                  hipEvent_t *ev)
+}   // namespace SL_NS
+
+using namespace SL_NS;
+
+extern "C" int SL_LAUNCH(const SlBasinDev *B, const SlRunDev *R, int step, int smpt, int filt, int parity,
+                         int path, int bth, hipStream_t st, hipEvent_t *ev)
 {
-   float *h0 = parity ? R->hb1 : R->hb0, *h1 = parity ? R->hb0 : R->hb1;
+   real *h0 = parity ? R->hb1 : R->hb0, *h1 = parity ? R->hb0 : R->hb1;
    StepArgs a;
    a.step = step;
 #define MARK(k) \
@@ -1304,7 +1330,7 @@ extern "C" int sl_dev_launch_step(const SlBasinDev *B, const SlRunDev *R, int st
    a.hc = h0;
    hipLaunchKernelGGL(k_hmxsv, grd, blk, 0, st, *B, *R, a);
    MARK(6);
-   float *fin = h0;
+   real *fin = h0;
    if (filt) {
       a.hc = h0;
       a.hn = h1;

@@ -38,6 +38,8 @@ def lib():
         L.slg_basin_close.argtypes = [vp]
         L.slg_basin_close.restype = None
         L.slg_batch_open.argtypes = [vp, i, ctypes.POINTER(cp), ctypes.POINTER(vp), cp, i]
+        L.slg_batch_open_prec.argtypes = [vp, i, ctypes.POINTER(cp), i, ctypes.POINTER(vp), cp, i]
+        L.slg_batch_precision.argtypes = [vp]
         L.slg_batch_nsteps.argtypes = [vp, i]
         L.slg_batch_upload.argtypes = [vp, cp, i]
         L.slg_batch_step.argtypes = [vp, i, cp, i]
@@ -82,14 +84,15 @@ class Basin:
 
 
 class Batch:
-    def __init__(self, basin: Basin, trks):
+    def __init__(self, basin: Basin, trks, precision: int = 32):
         L = lib()
         self.basin = basin
         self.n = len(trks)
         arr = (ctypes.c_char_p * self.n)(*[t.encode() for t in trks])
         self.h = ctypes.c_void_p()
         err = ctypes.create_string_buffer(512)
-        _chk(L.slg_batch_open(basin.h, self.n, arr, ctypes.byref(self.h), err, 512), err)
+        _chk(L.slg_batch_open_prec(basin.h, self.n, arr, precision, ctypes.byref(self.h), err, 512), err)
+        self.precision = precision
         self.nsteps = [L.slg_batch_nsteps(self.h, s) for s in range(self.n)]
 
     def export(self, storm: int, init_path: str, rows_path: str, tables_path: str):

@@ -182,21 +182,22 @@ def tmpdir(prefix="slg_"):
     return tempfile.mkdtemp(prefix=prefix)
 
 
-def gpu_vs_oracle(case, nsteps, checks, storms=None, path=0, tile_rows=16):
+def gpu_vs_oracle(case, nsteps, checks, storms=None, path=0, tile_rows=16, precision=32):
     """Run one storm of a case on the GPU and the FP32 oracle side by side.
 
     Returns {step: {field: (gpu, oracle)}} at each checkpoint (plus the last
     step) and the oracle's drying-cascade event count.  ``storms`` lists extra
     track files batched with the case's track (the case's own storm is 0);
-    ``path``/``tile_rows`` select the launch path (sloshgpu.PATH_*)."""
+    ``path``/``tile_rows`` select the launch path (sloshgpu.PATH_*); ``precision``
+    the arithmetic (32 against the FP32 oracle, 64 against the FP64 one)."""
     root = tmpdir("gvo_")
     rows, tabs, total = product_export(case, root)
     n = min(nsteps, total)
     B = sloshgpu.Basin(case["dta"], case["bntline"])
-    X = sloshgpu.Batch(B, [case["trk"]] + list(storms or []))
+    X = sloshgpu.Batch(B, [case["trk"]] + list(storms or []), precision=precision)
     X.set_path(path, tile_rows)
     X.upload()
-    o = Oracle("32", root + "/init.bin")
+    o = Oracle(str(precision), root + "/init.bin")
     out, events = {}, 0
     want = set(c for c in checks if c <= n) | {n}
     for k in range(n):

@@ -48,3 +48,35 @@ def test_gpu_cli_polar_100x90_vs_reference(tmp_path):
     print("fp32 vs reference fp64: wet/dry agreement %.4f, within 0.1 ft %.4f, max %.1f ft"
           % ((wet_g == wet_r).mean(), (d <= 1).mean(), d.max() / 10.0))
     assert (wet_g == wet_r).mean() >= 0.99
+
+
+def _run_gpu64(root, bsn):
+    old = os.environ.get("SLG_PRECISION")
+    os.environ["SLG_PRECISION"] = "64"
+    try:
+        return helpers.run_cli(helpers.GPU_CLI, root, bsn, "gpu64")
+    finally:
+        if old is None:
+            del os.environ["SLG_PRECISION"]
+        else:
+            os.environ["SLG_PRECISION"] = old
+
+
+@pytest.mark.parametrize("cid,b,t", CASES, ids=[c[0] for c in CASES])
+def test_gpu_cli_fp64_matches_reference_cli(cid, b, t, tmp_path):
+    """With the FP64 step the drop-in writes the reference CLI's own env and rex
+    bytes."""
+    root = str(tmp_path)
+    bsn = synth.write_case(root, b, t)["basin"]
+    env_g, rex_g = _run_gpu64(root, bsn)
+    env_r, rex_r = helpers.run_cli(helpers.REF_CLI, root, bsn, "ref")
+    assert helpers.same_cli_outputs(env_g, rex_g, env_r, rex_r) == []
+
+
+def test_gpu_cli_fp64_polar_100x90_matches_reference_cli(tmp_path):
+    b, t = synth.config1()
+    root = str(tmp_path)
+    bsn = synth.write_case(root, b, t)["basin"]
+    env_g, rex_g = _run_gpu64(root, bsn)
+    env_r, rex_r = helpers.run_cli(helpers.REF_CLI, root, bsn, "ref")
+    assert helpers.same_cli_outputs(env_g, rex_g, env_r, rex_r) == []

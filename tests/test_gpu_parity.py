@@ -156,3 +156,47 @@ def test_fused_equals_staged_ensemble(tmp_path):
             for f in helpers.FIELDS:
                 assert np.array_equal(r[0][s][f], res[0][0][s][f]), (s, f)
             assert np.array_equal(r[1][s], res[0][1][s]), ("stats", s)
+
+
+# ---------------------------------------------------------------- FP64 step
+# The same kernels built in double reproduce the reference's default FP64
+# build bit for bit: against the reference's own golden state dumps directly,
+# and against the FP64 oracle on every variant.
+
+@pytest.mark.parametrize("path", [sloshgpu.PATH_FUSED, sloshgpu.PATH_STAGED], ids=["fused", "staged"])
+@pytest.mark.parametrize("name", GOLDEN_CASES)
+def test_fp64_golden_equals_reference_dumps(name, path, tmp_path):
+    g = load_golden(name)
+    case = golden_case_files(g, str(tmp_path))
+    B = sloshgpu.Basin(case["dta"], case["bntline"])
+    X = sloshgpu.Batch(B, [case["trk"]], precision=64)
+    X.set_path(path, 16)
+    X.upload()
+    print("fp64", name, "path", path, flush=True)
+    for c in [int(c) for c in g["checks"]] + [int(g["nsteps"][0])]:
+        X.step(c - X.current_step)
+        X.sync()
+        pre = "final_" if c == int(g["nsteps"][0]) else "s%d_" % c
+        for f in helpers.FIELDS:
+            ref = g[pre + f].reshape(B.jmxb, B.imxb)
+            got = X.get(0, f)
+            bad = np.flatnonzero(got.ravel() != ref.ravel())
+            assert bad.size == 0, "%s step %d %s: %d cells differ" % (name, c, f, bad.size)
+    X.close()
+    B.close()
+
+
+@pytest.mark.parametrize("idx", range(len(helpers.VARIANTS)), ids=helpers.variant_id)
+def test_fp64_variants_bitwise(idx, tmp_path):
+    case = helpers.variant_case(str(tmp_path), idx)
+    print("fp64 variant", helpers.variant_id(idx), flush=True)
+    out, _ = helpers.gpu_vs_oracle(case, 300, [1, 5, 31, 100, 200], precision=64)
+    _assert_identical(out, helpers.variant_id(idx))
+
+
+def test_fp64_polar_100x90_two_days_bitwise(tmp_path):
+    b, t = synth.config1()
+    case = helpers.make_case(str(tmp_path), b, t)
+    print("fp64 polar 100x90 2880 steps", flush=True)
+    out, events = helpers.gpu_vs_oracle(case, 2880, [1, 720, 1440, 2160], precision=64)
+    _assert_identical(out, "polar100x90-fp64")
