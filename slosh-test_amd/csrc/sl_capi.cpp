@@ -19,21 +19,21 @@
 #include "sl_dev.h"
 #include "../../include/slosh_hip.h"
 
-extern "C" int sl_dev_launch_step_f32(const SlBasinDevT<float> *B, const SlRunDevT<float> *R, int step,
-                                      int smpt, int filt, int parity, int path, int bth, hipStream_t st,
+extern "C" int sl_dev_launch_step_f32(const SlBasinDevT<float> *B, const SlRunDevT<float> *R, int smpt,
+                                      int filt, int dyn, int parity, int path, int bth, hipStream_t st,
                                       hipEvent_t *ev);
-extern "C" int sl_dev_launch_step_f64(const SlBasinDevT<double> *B, const SlRunDevT<double> *R, int step,
-                                      int smpt, int filt, int parity, int path, int bth, hipStream_t st,
+extern "C" int sl_dev_launch_step_f64(const SlBasinDevT<double> *B, const SlRunDevT<double> *R, int smpt,
+                                      int filt, int dyn, int parity, int path, int bth, hipStream_t st,
                                       hipEvent_t *ev);
-static int dev_launch(const SlBasinDevT<float> *B, const SlRunDevT<float> *R, int step, int smpt, int filt,
+static int dev_launch(const SlBasinDevT<float> *B, const SlRunDevT<float> *R, int smpt, int filt, int dyn,
                       int parity, int path, int bth, hipStream_t st, hipEvent_t *ev)
 {
-   return sl_dev_launch_step_f32(B, R, step, smpt, filt, parity, path, bth, st, ev);
+   return sl_dev_launch_step_f32(B, R, smpt, filt, dyn, parity, path, bth, st, ev);
 }
-static int dev_launch(const SlBasinDevT<double> *B, const SlRunDevT<double> *R, int step, int smpt, int filt,
+static int dev_launch(const SlBasinDevT<double> *B, const SlRunDevT<double> *R, int smpt, int filt, int dyn,
                       int parity, int path, int bth, hipStream_t st, hipEvent_t *ev)
 {
-   return sl_dev_launch_step_f64(B, R, step, smpt, filt, parity, path, bth, st, ev);
+   return sl_dev_launch_step_f64(B, R, smpt, filt, dyn, parity, path, bth, st, ev);
 }
 
 #define PIX(b, i, j) ((size_t) ((j) - 1) * (size_t) (b)->imxb + (size_t) ((i) - 1))
@@ -91,6 +91,7 @@ struct slg_batch {
    int nstorm, maxsteps, step, parity;
    int path, tile_rows;                // launch path (SLG_PATH_*) and stage-B tile height
    int hbs_buf;                        // buffer holding HB before FILTER of the last step (-1: none)
+   std::vector<int> last_par;          // per storm: buffer holding its HB after its last step
    std::vector<storm_sched> sc;
    std::vector<double> hb0;            // initial HB per storm (double, imxb*jmxb)
    void *dmem;
@@ -567,6 +568,7 @@ extern "C" int slg_batch_open_prec(slg_basin *B, int nstorm, const char *const *
    X->path = SLG_PATH_FUSED;
    X->tile_rows = 16;
    X->hbs_buf = -1;
+   X->last_par.assign(nstorm, 0);
    *out = X;
    return 0;
 }
@@ -616,7 +618,7 @@ static int batch_upload(slg_batch *X, char *err, int errlen)
    const size_t fstride = (size_t) P * NJ, np = fstride * S;
    size_t ntab = 0;
    for (auto &sc : X->sc) ntab += sc.tab_step.size();
-   size_t off[16], tot = 0;
+   size_t off[20], tot = 0;
    int n = 0;
    auto add = [&](size_t bytes) {
       off[n++] = tot;
@@ -632,6 +634,7 @@ static int batch_upload(slg_batch *X, char *err, int errlen)
    add(ntab * SL_NTABV * 8 * rs);                                      // tables
    add((size_t) S * 4);                                                // tbase
    add((size_t) S * 4);                                                // nsteps
+   add((size_t) S * 4);                                                // cur
    char *d = nullptr;
    HIPOK(hipMalloc((void **) &d, tot));
    X->dmem = d;
@@ -683,6 +686,7 @@ static int batch_upload(slg_batch *X, char *err, int errlen)
    D.tables = (const R *) (d + off[13]);
    D.tbase = (const int *) (d + off[14]);
    D.nsteps = (const int *) (d + off[15]);
+   D.cur = (int *) (d + off[16]);   // zeroed with the rest
    HIPOK(hipStreamCreate(&X->st));
    X->step = 0;
    X->parity = 0;
@@ -694,26 +698,31 @@ extern "C" int slg_batch_upload(slg_batch *X, char *err, int errlen)
    return X->prec == 64 ? batch_upload<double>(X, err, errlen) : batch_upload<float>(X, err, errlen);
 }
 
-static int launch_one(slg_batch *X, int step, int smpt, int filt, hipEvent_t *ev)
+static int launch_one(slg_batch *X, int smpt, int filt, hipEvent_t *ev)
 {
    if (X->prec == 64)
-      return dev_launch(&X->b->i64.dev, &X->d64, step, smpt, filt, X->parity, X->path, X->tile_rows, X->st, ev);
-   return dev_launch(&X->b->i32.dev, &X->d32, step, smpt, filt, X->parity, X->path, X->tile_rows, X->st, ev);
+      return dev_launch(&X->b->i64.dev, &X->d64, smpt, filt, 0, X->parity, X->path, X->tile_rows, X->st, ev);
+   return dev_launch(&X->b->i32.dev, &X->d32, smpt, filt, 0, X->parity, X->path, X->tile_rows, X->st, ev);
+}
+
+// after a lockstep step: where each storm still running left its HB
+static void note_step(slg_batch *X, int step, int rc)
+{
+   for (int s = 0; s < X->nstorm; s++)
+      if (X->sc[s].nsteps >= step) X->last_par[s] = rc;
 }
 
 // advance every storm of the batch by n steps (stops at the shortest run)
 extern "C" int slg_batch_step(slg_batch *X, int n, char *err, int errlen)
 {
    const slg_basin *B = X->b;
-   int lim = X->maxsteps;
-   for (auto &sc : X->sc)
-      if (sc.nsteps < lim) lim = sc.nsteps;
+   const int lim = X->maxsteps;   // storms with shorter runs stop at their own end
    for (int k = 0; k < n && X->step < lim; k++) {
       int step = X->step + 1;
       int itime = step;
       int smpt = (B->ismpt2g && itime > 5 && B->incsm != 1);
       int filt = ((itime + 1) % B->incsm == 0 || B->h.ebsn1 == '+');
-      int rc = launch_one(X, step, smpt, filt, nullptr);
+      int rc = launch_one(X, smpt, filt, nullptr);
       if (rc < 0) {
          seterr(err, errlen, "kernel launch failed (%s%d)", "", -rc);
          return SLG_EHIP;
@@ -722,6 +731,7 @@ extern "C" int slg_batch_step(slg_batch *X, int n, char *err, int errlen)
       X->hbs_buf = !filt ? rc : (X->path == SLG_PATH_STAGED ? X->parity : -1);
       X->parity = rc;
       X->step = step;
+      note_step(X, step, rc);
    }
    return 0;
 }
@@ -735,14 +745,12 @@ extern "C" int slg_batch_step_timed(slg_batch *X, int n, double *ms, int *launch
    const slg_basin *B = X->b;
    hipEvent_t ev[SLG_NKERN + 1];
    for (int k = 0; k <= SLG_NKERN; k++) HIPOK(hipEventCreate(&ev[k]));
-   int lim = X->maxsteps;
-   for (auto &sc : X->sc)
-      if (sc.nsteps < lim) lim = sc.nsteps;
+   const int lim = X->maxsteps;
    for (int k = 0; k < n && X->step < lim; k++) {
       int step = X->step + 1, itime = step;
       int smpt = (B->ismpt2g && itime > 5 && B->incsm != 1);
       int filt = ((itime + 1) % B->incsm == 0 || B->h.ebsn1 == '+');
-      int rc = launch_one(X, step, smpt, filt, ev);
+      int rc = launch_one(X, smpt, filt, ev);
       if (rc < 0) {
          seterr(err, errlen, "kernel launch failed (%s%d)", "", -rc);
          return SLG_EHIP;
@@ -750,6 +758,7 @@ extern "C" int slg_batch_step_timed(slg_batch *X, int n, double *ms, int *launch
       X->hbs_buf = !filt ? rc : (X->path == SLG_PATH_STAGED ? X->parity : -1);
       X->parity = rc;
       X->step = step;
+      note_step(X, step, rc);
       if (X->path == SLG_PATH_FUSED) {
          HIPOK(hipEventSynchronize(ev[3]));
          const int slot[3] = {SLG_K_CONTTY, SLG_K_REPLAY, SLG_K_STEPB};
@@ -876,9 +885,9 @@ static int batch_get(slg_batch *X, int storm, const char *name, double *dst, cha
    const sl_basin *b = &X->b->h;
    const SlRunDevT<R> &D = run(X, R());
    const R *src;
-   if (!strcmp(name, "HB")) src = X->parity ? D.hb1 : D.hb0;
+   if (!strcmp(name, "HB")) src = X->last_par[storm] ? D.hb1 : D.hb0;
    else if (!strcmp(name, "HBS")) {
-      if (X->hbs_buf < 0) {
+      if (X->hbs_buf < 0 || X->sc[storm].nsteps < X->step) {
          seterr(err, errlen, "%s", "HB before FILTER is kept only by the staged path");
          return SLG_EINPUT;
       }

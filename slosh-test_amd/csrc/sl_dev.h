@@ -73,7 +73,9 @@ struct SlRunDevT {
    int maxsteps;
    const R *tables;           /* table pool [*][SL_NTABV][8]: C S CW SW P DELP 0 0 per radius */
    const int *tbase;          /* [nstorm] first table of each storm in the pool */
-   const int *nsteps;         /* [nstorm] */
+   const int *nsteps;         /* [nstorm] steps of each run (0: empty slot) */
+   int *cur;                  /* [nstorm] steps completed; the kernels of a step work on
+                                 step cur+1 of each storm and skip finished storms */
 };
 
 #endif

@@ -85,8 +85,14 @@ __constant__ int cISHF[4][3] = {{0, 0, 0}, {1, 2, 4}, {0, 0, 0}, {1, 3, 4}};
 struct StepArgs {
    const real *hc;     // HB entering the stage (per storm base = hc + s*fstride)
    real *hn;           // HB leaving the stage
-   int step;            // 1-based step number
 };
+
+// The step storm s is on (1-based), or 0 when its run is complete.
+__device__ __forceinline__ int storm_step(const SlRunDev &R, int s)
+{
+   const int c = R.cur[s];
+   return c < R.nsteps[s] ? c + 1 : 0;
+}
 
 __device__ __forceinline__ const real *par(const SlRunDev &R, int s, int step)
 {
@@ -233,10 +239,10 @@ __device__ __forceinline__ real contty_sq(const SlBasinDev &B, const SlRunDev &R
 __global__ void __launch_bounds__(TX * TY) k_contty(SlBasinDev B, SlRunDev R, StepArgs A)
 {
    const int i = blockIdx.x * TX + threadIdx.x + 1, j = blockIdx.y * TY + threadIdx.y + 1;
-   const int s = blockIdx.z;
-   if (i > B.imxb || j > B.jmxb) return;
+   const int s = blockIdx.z, step = storm_step(R, s);
+   if (i > B.imxb || j > B.jmxb || !step) return;
    const size_t so = (size_t) s * R.fstride;
-   A.hn[so + IDX(B, i, j)] = contty_sq(B, R, s, A.step, i, j, A.hc + so, R.ub + so, R.vb + so, nullptr);
+   A.hn[so + IDX(B, i, j)] = contty_sq(B, R, s, step, i, j, A.hc + so, R.ub + so, R.vb + so, nullptr);
 }
 
 // Fused path: a 64 x 16 tile per workgroup (each thread four rows), numbered
@@ -247,8 +253,8 @@ __global__ void __launch_bounds__(TX * TY) k_conttyA(SlBasinDev B, SlRunDev R, S
 {
    int tx, ty, s;
    tile_of(blockIdx.x, gridDim.x, ntx, nty, tx, ty, s);
-   const int i = tx * TX + threadIdx.x + 1;
-   if (i > B.imxb) return;
+   const int i = tx * TX + threadIdx.x + 1, step = storm_step(R, s);
+   if (i > B.imxb || !step) return;
    const size_t so = (size_t) s * R.fstride;
    const real *hc = A.hc + so;
    real *hn = A.hn + so, *ub = R.ub + so, *vb = R.vb + so;
@@ -256,7 +262,7 @@ __global__ void __launch_bounds__(TX * TY) k_conttyA(SlBasinDev B, SlRunDev R, S
    for (int r = threadIdx.y; r < ATH; r += TY) {
       const int j = ty * ATH + r + 1;
       if (j > B.jmxb) break;
-      hn[IDX(B, i, j)] = contty_sq(B, R, s, A.step, i, j, hc, ub, vb, uz);
+      hn[IDX(B, i, j)] = contty_sq(B, R, s, step, i, j, hc, ub, vb, uz);
    }
 }
 
@@ -379,8 +385,9 @@ __global__ void k_replay(SlBasinDev B, SlRunDev R, StepArgs A, int use_uz)
 {
    const int s = blockIdx.x * blockDim.x + threadIdx.x;
    if (s >= R.nstorm) return;
+   const int step = storm_step(R, s);
    int n = R.evcnt[s];
-   if (n == 0) return;
+   if (n == 0 || !step) return;
    const size_t so = (size_t) s * R.fstride;
    Replay X;
    X.B = &B;
@@ -391,7 +398,7 @@ __global__ void k_replay(SlBasinDev B, SlRunDev R, StepArgs A, int use_uz)
    X.uz = use_uz ? R.uz + so : nullptr;
    X.key = R.evkey + (size_t) s * SL_EVCAP;
    X.full = 0;
-   X.x1b5 = par(R, s, A.step)[P_X1B5];
+   X.x1b5 = par(R, s, step)[P_X1B5];
    if (n > SL_EVCAP) {
       n = 0;
       X.full = 1;
@@ -1001,6 +1008,7 @@ __global__ void __launch_bounds__(TX * TY) k_pflag(SlBasinDev B, SlRunDev R, Ste
 {
    const int i = blockIdx.x * TX + threadIdx.x + 1, j = blockIdx.y * TY + threadIdx.y + 1;
    const int s = blockIdx.z;
+   if (!storm_step(R, s)) return;
    const bool in = (i <= B.imxb && j <= B.jmxb);
    const size_t so = (size_t) s * R.fstride;
    uint8_t f = 0;
@@ -1019,6 +1027,7 @@ __global__ void __launch_bounds__(TX * TY) k_hclass(SlBasinDev B, SlRunDev R, St
 {
    const int i = blockIdx.x * TX + threadIdx.x + 1, j = blockIdx.y * TY + threadIdx.y + 1;
    const int s = blockIdx.z;
+   if (!storm_step(R, s)) return;
    const bool in = (i <= B.imxb && j <= B.jmxb);
    const size_t so = (size_t) s * R.fstride;
    uint8_t c = 0;
@@ -1036,11 +1045,11 @@ __global__ void __launch_bounds__(TX * TY) k_hclass(SlBasinDev B, SlRunDev R, St
 __global__ void __launch_bounds__(TX * TY) k_smooth(SlBasinDev B, SlRunDev R, StepArgs A)
 {
    const int i = blockIdx.x * TX + threadIdx.x + 1, j = blockIdx.y * TY + threadIdx.y + 1;
-   const int s = blockIdx.z;
-   if (i > B.imxb || j > B.jmxb) return;
+   const int s = blockIdx.z, step = storm_step(R, s);
+   if (i > B.imxb || j > B.jmxb || !step) return;
    const size_t so = (size_t) s * R.fstride, p = IDX(B, i, j);
    const real *hs = A.hc + so;
-   const real *P = par(R, s, A.step);
+   const real *P = par(R, s, step);
    if (j == B.jmxb) {
       A.hn[so + p] = (i <= B.imxb - 1) ? hs[IDX(B, i, 1)] : hs[p];
       return;
@@ -1054,7 +1063,7 @@ __global__ void __launch_bounds__(TX * TY) k_hmxsv(SlBasinDev B, SlRunDev R, Ste
 {
    const int i = blockIdx.x * TX + threadIdx.x + 1, j = blockIdx.y * TY + threadIdx.y + 1;
    const int s = blockIdx.z;
-   if (i > B.imxb - 1 || j > B.jmxb - 1) return;
+   if (i > B.imxb - 1 || j > B.jmxb - 1 || !storm_step(R, s)) return;
    const size_t so = (size_t) s * R.fstride, p = IDX(B, i, j);
    const real hx = R.hmx[so + p];
    const real v = hmxsv_pt(B, i, j, GF{A.hc + so, B.pitch}, hx, ZG{B.zb, B.zbm, B.pitch});
@@ -1065,7 +1074,7 @@ __global__ void __launch_bounds__(TX * TY) k_filter(SlBasinDev B, SlRunDev R, St
 {
    const int i = blockIdx.x * TX + threadIdx.x + 1, j = blockIdx.y * TY + threadIdx.y + 1;
    const int s = blockIdx.z;
-   if (i > B.imxb || j > B.jmxb) return;
+   if (i > B.imxb || j > B.jmxb || !storm_step(R, s)) return;
    const size_t so = (size_t) s * R.fstride, p = IDX(B, i, j);
    const real *hs = A.hc + so;
    real out = hs[p];
@@ -1077,12 +1086,12 @@ __global__ void __launch_bounds__(TX * TY) k_filter(SlBasinDev B, SlRunDev R, St
 __global__ void __launch_bounds__(TX * TY) k_momntm(SlBasinDev B, SlRunDev R, StepArgs A)
 {
    const int i = blockIdx.x * TX + threadIdx.x + 1, j = blockIdx.y * TY + threadIdx.y + 1;
-   const int s = blockIdx.z;
-   if (i > B.imxb || j > B.jmxb) return;
+   const int s = blockIdx.z, step = storm_step(R, s);
+   if (i > B.imxb || j > B.jmxb || !step) return;
    const uint32_t fl = B.flags[IDX(B, i, j)];
    if (!(fl & F_MOM)) return;
    const size_t so = (size_t) s * R.fstride;
-   const real *P = par(R, s, A.step);
+   const real *P = par(R, s, step);
    momntm_pt(B, P, tabv(R, s, P), fl, i, j, GF{A.hc + so, B.pitch}, ZG{B.zb, B.zbm, B.pitch}, R.ub + so, R.vb + so);
 }
 
@@ -1104,8 +1113,11 @@ struct Reg {
    static constexpr int lo = LO, hi = HI, w = TX + LO + HI, h = TH + LO + HI, n = w * h;
 };
 
+// SMPT / FILT size the regions and, with dyn == 0, decide the stages; with
+// dyn != 0 (storms at different steps in one launch) each storm's own step
+// flags decide, inside the regions of the <TH, true, true> instance.
 template <int TH, bool SMPT, bool FILT>
-__global__ void __launch_bounds__(256) k_stepB(SlBasinDev B, SlRunDev R, StepArgs A, int ntx, int nty)
+__global__ void __launch_bounds__(256) k_stepB(SlBasinDev B, SlRunDev R, StepArgs A, int ntx, int nty, int dyn)
 {
    constexpr int SLO = FILT ? 2 : 1, SHI = FILT ? 2 : 1;
    using RS = Reg<SLO, SHI, TH>;
@@ -1127,13 +1139,17 @@ __global__ void __launch_bounds__(256) k_stepB(SlBasinDev B, SlRunDev R, StepArg
 
    int tx, ty, s;
    tile_of(blockIdx.x, gridDim.x, ntx, nty, tx, ty, s);
+   const int step = storm_step(R, s);
+   if (!step) return;   // the whole workgroup: its storm's run is complete
    const int tid = threadIdx.x;
    const int I0 = tx * TX + 1, J0 = ty * TH + 1;
    const int IM = B.imxb, JM = B.jmxb;
    const size_t so = (size_t) s * R.fstride;
    const real *hs_g = A.hc + so;
-   const real *P = par(R, s, A.step);
+   const real *P = par(R, s, step);
    const real *T = tabv(R, s, P);
+   const int sf = (int) P[P_FLAGS];
+   const bool smpt = SMPT && (!dyn || (sf & SF_SMPT)), filt = FILT && (!dyn || (sf & SF_FILT));
    if (tid < 4) sh_cnt[tid] = 0;
 
    // HB after CONTTY (and the U==0 mask) with halo; the static ZB and ZBM
@@ -1147,7 +1163,7 @@ __global__ void __launch_bounds__(256) k_stepB(SlBasinDev B, SlRunDev R, StepArg
       sh_zb[e] = in ? B.zb[IDX(B, gi, gj)] : real(0.);
       sh_zbm[e] = in ? B.zbm[IDX(B, gi, gj)] : real(0.);
    }
-   if (SMPT) {
+   if (smpt) {
       const uint8_t *uz_g = R.uz + so;
       for (int e = tid; e < RU::n; e += 256) {
          const int gi = I0 - RU::lo + e % RU::w, gj = J0 - RU::lo + e / RU::w;
@@ -1158,7 +1174,7 @@ __global__ void __launch_bounds__(256) k_stepB(SlBasinDev B, SlRunDev R, StepArg
    const LF hsL{sh_hs, I0 - RH::lo, J0 - RH::lo, RH::w};
    const LB clL{sh_cl, I0 - RC::lo, J0 - RC::lo, RC::w};
    const ZL zL{sh_zb, sh_zbm, I0 - RZ::lo, J0 - RZ::lo, RZ::w};
-   if (SMPT) {
+   if (smpt) {
       const LB uzL{sh_uz, I0 - RU::lo, J0 - RU::lo, RU::w};
       int np = 0, ndt = 0;
       for (int e = tid; e < RP::n; e += 256) {
@@ -1210,7 +1226,7 @@ __global__ void __launch_bounds__(256) k_stepB(SlBasinDev B, SlRunDev R, StepArg
       real v = real(0.);
       if (gi >= 1 && gi <= IM && gj >= 1 && gj <= JM) {
          if (gj == JM) v = (gi <= IM - 1) ? hs_g[IDX(B, gi, 1)] : hsL(gi, gj);
-         else v = smooth_pt(B, P, T, SMPT, gi, gj, hsL, clL, zL);
+         else v = smooth_pt(B, P, T, smpt, gi, gj, hsL, clL, zL);
       }
       sh_s[e] = v;
    }
@@ -1228,7 +1244,7 @@ __global__ void __launch_bounds__(256) k_stepB(SlBasinDev B, SlRunDev R, StepArg
       }
    }
    real *hn_g = A.hn + so;
-   if (FILT) {
+   if (filt) {
       const real *top = R.hsubtop + (size_t) s * B.pitch;
       for (int e = tid; e < RF::n; e += 256) {
          const int gi = I0 - RF::lo + e % RF::w, gj = J0 - RF::lo + e / RF::w;
@@ -1239,8 +1255,8 @@ __global__ void __launch_bounds__(256) k_stepB(SlBasinDev B, SlRunDev R, StepArg
       }
       __syncthreads();
    }
-   const LF fL{FILT ? sh_f : sh_s, I0 - (FILT ? RF::lo : RS::lo), J0 - (FILT ? RF::lo : RS::lo),
-               FILT ? RF::w : RS::w};
+   const LF fL{filt ? sh_f : sh_s, I0 - (filt ? RF::lo : RS::lo), J0 - (filt ? RF::lo : RS::lo),
+               filt ? RF::w : RS::w};
    real *ub = R.ub + so, *vb = R.vb + so;
    for (int e = tid; e < TX * TH; e += 256) {
       const int gi = I0 + e % TX, gj = J0 + e / TX;
@@ -1254,29 +1270,39 @@ __global__ void __launch_bounds__(256) k_stepB(SlBasinDev B, SlRunDev R, StepArg
 
 // ------------------------------------------------------------------ launch
 template <int TH>
-static void launch_B(int smpt, int filt, dim3 g, hipStream_t st, const SlBasinDev &B, const SlRunDev &R,
-                     StepArgs a, int ntx, int nty)
+static void launch_B(int smpt, int filt, int dyn, dim3 g, hipStream_t st, const SlBasinDev &B,
+                     const SlRunDev &R, StepArgs a, int ntx, int nty)
 {
-   if (smpt && filt) hipLaunchKernelGGL((k_stepB<TH, true, true>), g, dim3(256), 0, st, B, R, a, ntx, nty);
-   else if (smpt) hipLaunchKernelGGL((k_stepB<TH, true, false>), g, dim3(256), 0, st, B, R, a, ntx, nty);
-   else if (filt) hipLaunchKernelGGL((k_stepB<TH, false, true>), g, dim3(256), 0, st, B, R, a, ntx, nty);
-   else hipLaunchKernelGGL((k_stepB<TH, false, false>), g, dim3(256), 0, st, B, R, a, ntx, nty);
+   if (dyn || (smpt && filt))
+      hipLaunchKernelGGL((k_stepB<TH, true, true>), g, dim3(256), 0, st, B, R, a, ntx, nty, dyn);
+   else if (smpt) hipLaunchKernelGGL((k_stepB<TH, true, false>), g, dim3(256), 0, st, B, R, a, ntx, nty, 0);
+   else if (filt) hipLaunchKernelGGL((k_stepB<TH, false, true>), g, dim3(256), 0, st, B, R, a, ntx, nty, 0);
+   else hipLaunchKernelGGL((k_stepB<TH, false, false>), g, dim3(256), 0, st, B, R, a, ntx, nty, 0);
 }
 
-// One time step of every storm.  path 0 = fused (events: 0 before CONTTY,
-// 1 after it, 2 after the replay, 3 after stage B); path 1 = staged (events
-// around each of the SL_NKERN stage kernels).  Returns which HB buffer holds
-// the step's result.
+// End of a step: every storm still running has completed one more step.
+__global__ void k_advance(SlRunDev R)
+{
+   const int s = blockIdx.x * blockDim.x + threadIdx.x;
+   if (s < R.nstorm && R.cur[s] < R.nsteps[s]) R.cur[s]++;
+}
+
 }   // namespace SL_NS
 
 using namespace SL_NS;
 
-extern "C" int SL_LAUNCH(const SlBasinDev *B, const SlRunDev *R, int step, int smpt, int filt, int parity,
+// One time step of every storm still running (each on its own step cur+1).
+// smpt / filt say whether SMPT2G and FILTER run this step when the storms are
+// in lockstep; dyn != 0 makes each storm's own step flags decide (fused path).
+// path 0 = fused (events: 0 before CONTTY, 1 after it, 2 after the replay, 3
+// after stage B); path 1 = staged (events around each of the 8 stage kernels).
+// Returns which HB buffer holds the step's result.
+extern "C" int SL_LAUNCH(const SlBasinDev *B, const SlRunDev *R, int smpt, int filt, int dyn, int parity,
                          int path, int bth, hipStream_t st, hipEvent_t *ev)
 {
    real *h0 = parity ? R->hb1 : R->hb0, *h1 = parity ? R->hb0 : R->hb1;
+   const dim3 gadv((R->nstorm + 255) / 256), badv(256);
    StepArgs a;
-   a.step = step;
 #define MARK(k) \
    if (ev) (void) hipEventRecord(ev[k], st)
    if (path == 0) {
@@ -1294,10 +1320,11 @@ extern "C" int SL_LAUNCH(const SlBasinDev *B, const SlRunDev *R, int step, int s
       const int th = (bth == 8 || bth == 32) ? bth : 16;
       nty = (B->jmxb + th - 1) / th;
       const dim3 g(ntx * nty * R->nstorm);
-      if (th == 8) launch_B<8>(smpt, filt, g, st, *B, *R, a, ntx, nty);
-      else if (th == 32) launch_B<32>(smpt, filt, g, st, *B, *R, a, ntx, nty);
-      else launch_B<16>(smpt, filt, g, st, *B, *R, a, ntx, nty);
+      if (th == 8) launch_B<8>(smpt, filt, dyn, g, st, *B, *R, a, ntx, nty);
+      else if (th == 32) launch_B<32>(smpt, filt, dyn, g, st, *B, *R, a, ntx, nty);
+      else launch_B<16>(smpt, filt, dyn, g, st, *B, *R, a, ntx, nty);
       MARK(3);
+      hipLaunchKernelGGL(k_advance, gadv, badv, 0, st, *R);
       hipError_t e = hipGetLastError();
       if (e != hipSuccess) return -(int) e;
       return parity;
@@ -1341,6 +1368,7 @@ extern "C" int SL_LAUNCH(const SlBasinDev *B, const SlRunDev *R, int step, int s
    a.hc = fin;
    hipLaunchKernelGGL(k_momntm, grd, blk, 0, st, *B, *R, a);
    MARK(8);
+   hipLaunchKernelGGL(k_advance, gadv, badv, 0, st, *R);
 #undef MARK
    hipError_t e = hipGetLastError();
    if (e != hipSuccess) return -(int) e;

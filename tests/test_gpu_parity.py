@@ -200,3 +200,37 @@ def test_fp64_polar_100x90_two_days_bitwise(tmp_path):
     print("fp64 polar 100x90 2880 steps", flush=True)
     out, events = helpers.gpu_vs_oracle(case, 2880, [1, 720, 1440, 2160], precision=64)
     _assert_identical(out, "polar100x90-fp64")
+
+
+def test_uneven_run_lengths_equal_alone(tmp_path):
+    """Storms of different run lengths in one batch: each stops at its own
+    MHALT, and its fields equal those of the same storm run alone."""
+    b, t = synth.tiny(seed=41, ni=30, nj=26, hours=8)
+    case = helpers.make_case(str(tmp_path), b, t)
+    paths = [case["trk"]]
+    for k, hours in enumerate((6, 7, 10)):
+        tr = synth.straight_track(b, 10.0 + 3 * k, 300.0 + 20 * k, 50.0 + 10 * k, 20.0, hours=hours,
+                                  landfall=hours // 2)
+        p = str(tmp_path / ("u%d.trk" % k))
+        open(p, "w").write(synth.trk_text(tr))
+        paths.append(p)
+    B = sloshgpu.Basin(case["dta"], case["bntline"])
+    for path in (sloshgpu.PATH_FUSED, sloshgpu.PATH_STAGED):
+        X = sloshgpu.Batch(B, paths)
+        X.set_path(path, 16)
+        X.upload()
+        assert len(set(X.nsteps)) == len(paths)
+        X.step(max(X.nsteps))
+        X.sync()
+        mix = [{f: X.get(s, f) for f in helpers.FIELDS} for s in range(len(paths))]
+        X.close()
+        for s, p in enumerate(paths):
+            A = sloshgpu.Batch(B, [p])
+            A.set_path(path, 16)
+            A.upload()
+            A.step(A.nsteps[0])
+            A.sync()
+            for f in helpers.FIELDS:
+                assert np.array_equal(A.get(0, f), mix[s][f]), (path, s, f)
+            A.close()
+    B.close()
