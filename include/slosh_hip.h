@@ -101,6 +101,42 @@ int slg_batch_start_date(const slg_batch *x, int storm, int *out5);
 int slg_batch_stats(slg_batch *x, int storm, int *out8, char *err, int errlen);
 void slg_batch_close(slg_batch *x);
 
+/* Envelopes of every run as EnvSave quantises them (slosh2.c:1611-1671: int16
+ * tenths of a foot, 999 dry, 0 on the last row and column), env[nstorm][jmxb]
+ * [imxb], and the MOM of the batch, the maximum of MEOWs (gui/tclsrc/meowgen.tcl:
+ * 34-110: per cell the largest envelope below 50 ft, 999 where none),
+ * mom[jmxb][imxb]; computed on the device, either output may be NULL. */
+int slg_batch_envelopes(slg_batch *x, short *env, short *mom, char *err, int errlen);
+/* Datums a track carries for the env trailer (ReadTrkFile, slosh2.c:1885-1975). */
+int slg_track_datums(const char *trk_path, float *ht1, float *ht2);
+/* Write an env file in EnvSave's layout (slosh2.c:1611-1671). */
+int slg_write_env(const char *path, int imxb, int jmxb, const char *comment, const short *vals, float ht1,
+                  float ht2);
+
+/* ---- Serving: storm runs on several basins admitted as they arrive ----------
+ * No counterpart in the reference, which runs one storm per process (slosh2.c
+ * PerformRun) and queues whole runs in shell scripts (dev/a1.multiRun.sh).
+ * A server holds one pool of run slots per registered basin, sized from free
+ * HBM; runs wait in a FIFO until a slot of their basin is free, then step
+ * together with the runs already there (continuous batching); every pump
+ * retires finished runs (their envelope is computed on the device), admits
+ * waiting runs and launches up to k further steps on every pool. */
+typedef struct slg_server slg_server;
+int slg_server_open(int precision, double hbm_fraction, slg_server **out, char *err, int errlen);
+/* Register a basin with up to max_slots slots of max_steps steps and an arena
+ * of tables_per_slot radial-table sets per slot; *basin_id identifies it. */
+int slg_server_add_basin(slg_server *sv, const char *dta_path, const char *bnt_line, int max_slots,
+                         int max_steps, int tables_per_slot, int *basin_id, char *err, int errlen);
+int slg_server_slots(const slg_server *sv, int basin_id);
+int slg_server_submit(slg_server *sv, int basin_id, const char *trk_path, int *run_id, char *err, int errlen);
+/* Returns the number of runs retired this round (>= 0) or an error code. */
+int slg_server_pump(slg_server *sv, int k, char *err, int errlen);
+/* 0 queued, 1 running, 2 done (then env [jmxb][imxb] and latency are filled). */
+int slg_server_result(const slg_server *sv, int run_id, short *env, double *latency_s);
+/* [0] queued [1] running [2] done [3] slots [4] pool bytes [5] storm.cell.steps */
+int slg_server_stats(const slg_server *sv, long long *out6);
+void slg_server_close(slg_server *sv);
+
 /* Write the host initialisation of one run, and its storm schedule, in the
  * tagged-record layout used for the reference's state dumps (for checking). */
 int slg_export_init(const slg_batch *x, int storm, const char *path);

@@ -7,101 +7,7 @@
 // STMVAL leaves for the next step, and every radial table FORCE1 produces) is
 // computed on the host up front and uploaded once; stepping then needs no host
 // round trip.
-#include <hip/hip_runtime.h>
-#include <math.h>
-#include <stdint.h>
-#include <stdio.h>
-#include <stdlib.h>
-#include <string.h>
-#include <vector>
-#include <string>
-#include "sl_host.h"
-#include "sl_dev.h"
-#include "../../include/slosh_hip.h"
-
-extern "C" int sl_dev_launch_step_f32(const SlBasinDevT<float> *B, const SlRunDevT<float> *R, int smpt,
-                                      int filt, int dyn, int parity, int path, int bth, hipStream_t st,
-                                      hipEvent_t *ev);
-extern "C" int sl_dev_launch_step_f64(const SlBasinDevT<double> *B, const SlRunDevT<double> *R, int smpt,
-                                      int filt, int dyn, int parity, int path, int bth, hipStream_t st,
-                                      hipEvent_t *ev);
-static int dev_launch(const SlBasinDevT<float> *B, const SlRunDevT<float> *R, int smpt, int filt, int dyn,
-                      int parity, int path, int bth, hipStream_t st, hipEvent_t *ev)
-{
-   return sl_dev_launch_step_f32(B, R, smpt, filt, dyn, parity, path, bth, st, ev);
-}
-static int dev_launch(const SlBasinDevT<double> *B, const SlRunDevT<double> *R, int smpt, int filt, int dyn,
-                      int parity, int path, int bth, hipStream_t st, hipEvent_t *ev)
-{
-   return sl_dev_launch_step_f64(B, R, smpt, filt, dyn, parity, path, bth, st, ev);
-}
-
-#define PIX(b, i, j) ((size_t) ((j) - 1) * (size_t) (b)->imxb + (size_t) ((i) - 1))
-
-static void seterr(char *err, int len, const char *fmt, const char *a = "", int v = 0)
-{
-   if (err && len > 0) snprintf(err, (size_t) len, fmt, a, v);
-}
-
-#define HIPOK(x)                                                                           \
-   do {                                                                                    \
-      hipError_t e_ = (x);                                                                 \
-      if (e_ != hipSuccess) {                                                              \
-         seterr(err, errlen, "HIP error %s (%d)", hipGetErrorString(e_), (int) e_);         \
-         return SLG_EHIP;                                                                  \
-      }                                                                                    \
-   } while (0)
-
-// Host images of the device static data in one precision, and their device copy.
-template <class R>
-struct BasinImg {
-   std::vector<R> zb, zbm, zcor, vi[6], vj[5], tab, slpai, tab8, g4;
-   std::vector<uint32_t> flags;
-   bool built = false, on_dev = false;
-   void *dmem = nullptr;
-   SlBasinDevT<R> dev;
-};
-
-struct slg_basin {
-   sl_basin h;
-   char bnt[512];
-   int pitch, jend;
-   std::vector<int> is, ie, ms, me;
-   int incsm, ismpt2g;
-   BasinImg<float> i32;
-   BasinImg<double> i64;
-};
-static BasinImg<float> &img(slg_basin *B, float) { return B->i32; }
-static BasinImg<double> &img(slg_basin *B, double) { return B->i64; }
-
-struct storm_sched {
-   sl_storm s0;                        // storm state after initialisation
-   std::vector<double> rows;           // [nsteps+1][90] ref-format rows (kept for export)
-   // device values of each step ([nsteps][SL_NPAR]) and the table sets
-   // ([ntab][SL_NTABV][8], sl_dev.h), in the batch's precision
-   std::vector<float> p32, t32;
-   std::vector<double> p64, t64;
-   std::vector<int> tab_step;          // step after which each table set took effect
-   int nsteps;
-};
-
-struct slg_batch {
-   slg_basin *b;
-   int prec;                           // 32 or 64
-   int nstorm, maxsteps, step, parity;
-   int path, tile_rows;                // launch path (SLG_PATH_*) and stage-B tile height
-   int hbs_buf;                        // buffer holding HB before FILTER of the last step (-1: none)
-   std::vector<int> last_par;          // per storm: buffer holding its HB after its last step
-   std::vector<storm_sched> sc;
-   std::vector<double> hb0;            // initial HB per storm (double, imxb*jmxb)
-   void *dmem;
-   SlRunDevT<float> d32;
-   SlRunDevT<double> d64;
-   hipStream_t st;
-   size_t bytes;
-};
-static SlRunDevT<float> &run(slg_batch *X, float) { return X->d32; }
-static SlRunDevT<double> &run(slg_batch *X, double) { return X->d64; }
+#include "sl_int.h"
 
 // --------------------------------------------------------------------------
 // basin
@@ -274,6 +180,7 @@ static int basin_upload(slg_basin *B, char *err, int errlen)
    for (int k = 0; k < 5; k++) add((size_t) NJ * rs);
    add(I.tab8.size() * rs);
    add(4 * np * rs);   // g4
+   add(np * 8);        // ZB in double
    char *d = nullptr;
    HIPOK(hipMalloc((void **) &d, tot));
    I.dmem = d;
@@ -317,9 +224,21 @@ static int basin_upload(slg_basin *B, char *err, int errlen)
    for (int k = 0; k < 5; k++) *vj[k] = (const R *) up(I.vj[k].data(), (size_t) NJ * rs);
    D.tab8 = (const R *) up(I.tab8.data(), I.tab8.size() * rs);
    D.g4 = (const R *) up(I.g4.data(), 4 * np * rs);
+   {
+      std::vector<double> zd(np, 0.);
+      for (int j = 1; j <= NJ; j++)
+         for (int i = 1; i <= b->imxb; i++) zd[(size_t) (j - 1) * P + (i - 1)] = b->zb[PIX(b, i, j)];
+      I.zbd = (const double *) up(zd.data(), np * 8);
+   }
    HIPOK(hipDeviceSynchronize());
    I.on_dev = true;
    return 0;
+}
+
+int sl_basin_ensure(slg_basin *B, int prec, char *err, int errlen)
+{
+   if (prec == 64) return B->i64.on_dev ? 0 : basin_upload<double>(B, err, errlen);
+   return B->i32.on_dev ? 0 : basin_upload<float>(B, err, errlen);
 }
 
 extern "C" int slg_basin_open(const char *dta, const char *bnt_line, slg_basin **out, char *err,
@@ -437,8 +356,8 @@ static void push_tables(storm_sched &S, const sl_storm *s, int step, int prec)
 template <class R>
 static void step_params(const slg_basin *B, const double *r, int tv, R *P);
 
-static int make_schedule(const slg_basin *B, const char *trk, storm_sched &S, bool keep_rows, int prec,
-                         char *err, int errlen)
+int sl_make_schedule(const slg_basin *B, const char *trk, storm_sched &S, bool keep_rows, int prec,
+                     char *err, int errlen)
 {
    char e[256];
    double row[90];
@@ -552,7 +471,7 @@ extern "C" int slg_batch_open_prec(slg_basin *B, int nstorm, const char *const *
    X->maxsteps = 0;
    X->dmem = nullptr;
    for (int s = 0; s < nstorm; s++) {
-      int rc = make_schedule(B, trk[s], X->sc[s], nstorm <= 8, precision, err, errlen);
+      int rc = sl_make_schedule(B, trk[s], X->sc[s], nstorm <= 8, precision, err, errlen);
       if (rc) {
          delete X;
          return rc;
@@ -599,10 +518,6 @@ extern "C" int slg_batch_nsteps(const slg_batch *X, int storm)
    return X->sc[storm].nsteps;
 }
 
-static const std::vector<float> &sched_params(const storm_sched &S, float) { return S.p32; }
-static const std::vector<double> &sched_params(const storm_sched &S, double) { return S.p64; }
-static const std::vector<float> &sched_tables(const storm_sched &S, float) { return S.t32; }
-static const std::vector<double> &sched_tables(const storm_sched &S, double) { return S.t64; }
 
 template <class R>
 static int batch_upload(slg_batch *X, char *err, int errlen)
@@ -941,6 +856,83 @@ extern "C" int slg_batch_stats(slg_batch *X, int storm, int *out8, char *err, in
    HIPOK(hipStreamSynchronize(X->st));
    HIPOK(hipMemcpy(out8, stat + storm * 8, 32, hipMemcpyDeviceToHost));
    return 0;
+}
+
+template <class R>
+static int batch_envelopes(slg_batch *X, short *env, short *mom, char *err, int errlen)
+{
+   slg_basin *B = X->b;
+   const sl_basin *b = &B->h;
+   const size_t n = (size_t) b->imxb * b->jmxb;
+   short *denv = nullptr, *dmom = nullptr;
+   if (env) HIPOK(hipMalloc((void **) &denv, n * X->nstorm * 2));
+   if (mom) HIPOK(hipMalloc((void **) &dmom, n * 2));
+   int rc = dev_envelopes(&img(B, R()).dev, &run(X, R()), img(B, R()).zbd, 0, X->nstorm, denv, dmom, X->st);
+   if (rc == 0) {
+      if (env) HIPOK(hipMemcpyAsync(env, denv, n * X->nstorm * 2, hipMemcpyDeviceToHost, X->st));
+      if (mom) HIPOK(hipMemcpyAsync(mom, dmom, n * 2, hipMemcpyDeviceToHost, X->st));
+      HIPOK(hipStreamSynchronize(X->st));
+   }
+   if (denv) (void) hipFree(denv);
+   if (dmom) (void) hipFree(dmom);
+   if (rc) {
+      seterr(err, errlen, "envelope kernel failed (%s%d)", "", -rc);
+      return SLG_EHIP;
+   }
+   return 0;
+}
+
+// Envelopes of every run ([nstorm][jmxb][imxb]) and their MOM ([jmxb][imxb]),
+// computed on the device; either output may be NULL.
+extern "C" int slg_batch_envelopes(slg_batch *X, short *env, short *mom, char *err, int errlen)
+{
+   if (!X->dmem) {
+      seterr(err, errlen, "%s", "batch not uploaded");
+      return SLG_EINPUT;
+   }
+   return X->prec == 64 ? batch_envelopes<double>(X, env, mom, err, errlen)
+                        : batch_envelopes<float>(X, env, mom, err, errlen);
+}
+
+// Datums of a run's track for the env trailer (ReadTrkFile, slosh2.c:1885-1975):
+// OceanDatum and LakeDatum; with a CanalDatum line, LakeDatum and CanalDatum.
+extern "C" int slg_track_datums(const char *trk_path, float *ht1, float *ht2)
+{
+   FILE *fp = fopen(trk_path, "r");
+   char line[512];
+   float oc = 0.f, lk = 0.f, cn = -999.f;
+   if (!fp) return SLG_EINPUT;
+   while (fgets(line, sizeof line, fp)) {
+      if (!strncmp(line, "Hour,", 5)) break;
+      if (line[0] == '#') continue;
+      if (!strncmp(line, "OceanDatum=", 11)) oc = (float) atof(line + 11);
+      else if (!strncmp(line, "LakeDatum=", 10)) lk = (float) atof(line + 10);
+      else if (!strncmp(line, "CanalDatum=", 11)) cn = (float) atof(line + 11);
+   }
+   fclose(fp);
+   const bool canal = cn > -100;   // Okeechobee-style datums: lake and canal
+   *ht1 = canal ? lk : oc;
+   *ht2 = canal ? cn : lk;
+   return 0;
+}
+
+// env file in the layout EnvSave writes (slosh2.c:1611-1671): int32 imxb,
+// jmxb, a 160-byte comment (NUL padded), imxb*jmxb int16 (I fastest), two
+// float datums; all little-endian.
+extern "C" int slg_write_env(const char *path, int imxb, int jmxb, const char *comment, const short *vals,
+                             float ht1, float ht2)
+{
+   FILE *fp = fopen(path, "wb");
+   if (!fp) return SLG_EINPUT;
+   char c[160];
+   memset(c, 0, sizeof c);
+   if (comment) strncpy(c, comment, sizeof c);
+   const int32_t d[2] = {imxb, jmxb};
+   int ok = fwrite(d, 4, 2, fp) == 2 && fwrite(c, 1, 160, fp) == 160 &&
+            fwrite(vals, 2, (size_t) imxb * jmxb, fp) == (size_t) imxb * jmxb && fwrite(&ht1, 4, 1, fp) == 1 &&
+            fwrite(&ht2, 4, 1, fp) == 1;
+   fclose(fp);
+   return ok ? 0 : SLG_EINPUT;
 }
 
 extern "C" void slg_batch_close(slg_batch *X)

@@ -1,0 +1,128 @@
+// sl_int.h -- internal data of the C ABI (basins, storm schedules, batches),
+// shared by sl_capi.cpp (the batch API) and sl_serve.cpp (the serving loop).
+#ifndef SL_INT_H
+#define SL_INT_H
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <vector>
+#include <string>
+#include "sl_host.h"
+#include "sl_dev.h"
+#include "../../include/slosh_hip.h"
+
+extern "C" int sl_dev_launch_step_f32(const SlBasinDevT<float> *B, const SlRunDevT<float> *R, int smpt,
+                                      int filt, int dyn, int parity, int path, int bth, hipStream_t st,
+                                      hipEvent_t *ev);
+extern "C" int sl_dev_launch_step_f64(const SlBasinDevT<double> *B, const SlRunDevT<double> *R, int smpt,
+                                      int filt, int dyn, int parity, int path, int bth, hipStream_t st,
+                                      hipEvent_t *ev);
+static inline int dev_launch(const SlBasinDevT<float> *B, const SlRunDevT<float> *R, int smpt, int filt, int dyn,
+                      int parity, int path, int bth, hipStream_t st, hipEvent_t *ev)
+{
+   return sl_dev_launch_step_f32(B, R, smpt, filt, dyn, parity, path, bth, st, ev);
+}
+static inline int dev_launch(const SlBasinDevT<double> *B, const SlRunDevT<double> *R, int smpt, int filt, int dyn,
+                      int parity, int path, int bth, hipStream_t st, hipEvent_t *ev)
+{
+   return sl_dev_launch_step_f64(B, R, smpt, filt, dyn, parity, path, bth, st, ev);
+}
+
+#define PIX(b, i, j) ((size_t) ((j) - 1) * (size_t) (b)->imxb + (size_t) ((i) - 1))
+
+static inline void seterr(char *err, int len, const char *fmt, const char *a = "", int v = 0)
+{
+   if (err && len > 0) snprintf(err, (size_t) len, fmt, a, v);
+}
+
+#define HIPOK(x)                                                                           \
+   do {                                                                                    \
+      hipError_t e_ = (x);                                                                 \
+      if (e_ != hipSuccess) {                                                              \
+         seterr(err, errlen, "HIP error %s (%d)", hipGetErrorString(e_), (int) e_);         \
+         return SLG_EHIP;                                                                  \
+      }                                                                                    \
+   } while (0)
+
+// Host images of the device static data in one precision, and their device copy.
+template <class R>
+struct BasinImg {
+   std::vector<R> zb, zbm, zcor, vi[6], vj[5], tab, slpai, tab8, g4;
+   std::vector<uint32_t> flags;
+   bool built = false, on_dev = false;
+   void *dmem = nullptr;
+   SlBasinDevT<R> dev;
+   const double *zbd = nullptr;   // device ZB in double (envelope dry test, as EnvSave)
+};
+
+struct slg_basin {
+   sl_basin h;
+   char bnt[512];
+   int pitch, jend;
+   std::vector<int> is, ie, ms, me;
+   int incsm, ismpt2g;
+   BasinImg<float> i32;
+   BasinImg<double> i64;
+};
+static inline BasinImg<float> &img(slg_basin *B, float) { return B->i32; }
+static inline BasinImg<double> &img(slg_basin *B, double) { return B->i64; }
+
+struct storm_sched {
+   sl_storm s0;                        // storm state after initialisation
+   std::vector<double> rows;           // [nsteps+1][90] ref-format rows (kept for export)
+   // device values of each step ([nsteps][SL_NPAR]) and the table sets
+   // ([ntab][SL_NTABV][8], sl_dev.h), in the batch's precision
+   std::vector<float> p32, t32;
+   std::vector<double> p64, t64;
+   std::vector<int> tab_step;          // step after which each table set took effect
+   int nsteps;
+};
+
+struct slg_batch {
+   slg_basin *b;
+   int prec;                           // 32 or 64
+   int nstorm, maxsteps, step, parity;
+   int path, tile_rows;                // launch path (SLG_PATH_*) and stage-B tile height
+   int hbs_buf;                        // buffer holding HB before FILTER of the last step (-1: none)
+   std::vector<int> last_par;          // per storm: buffer holding its HB after its last step
+   std::vector<storm_sched> sc;
+   std::vector<double> hb0;            // initial HB per storm (double, imxb*jmxb)
+   void *dmem;
+   SlRunDevT<float> d32;
+   SlRunDevT<double> d64;
+   hipStream_t st;
+   size_t bytes;
+};
+static inline SlRunDevT<float> &run(slg_batch *X, float) { return X->d32; }
+static inline SlRunDevT<double> &run(slg_batch *X, double) { return X->d64; }
+
+
+extern "C" int sl_dev_envelopes_f32(const SlBasinDevT<float> *B, const SlRunDevT<float> *R, const double *zbd,
+                                    int s0, int ns, short *env, short *mom, hipStream_t st);
+extern "C" int sl_dev_envelopes_f64(const SlBasinDevT<double> *B, const SlRunDevT<double> *R, const double *zbd,
+                                    int s0, int ns, short *env, short *mom, hipStream_t st);
+static inline int dev_envelopes(const SlBasinDevT<float> *B, const SlRunDevT<float> *R, const double *zbd,
+                                int s0, int ns, short *env, short *mom, hipStream_t st)
+{
+   return sl_dev_envelopes_f32(B, R, zbd, s0, ns, env, mom, st);
+}
+static inline int dev_envelopes(const SlBasinDevT<double> *B, const SlRunDevT<double> *R, const double *zbd,
+                                int s0, int ns, short *env, short *mom, hipStream_t st)
+{
+   return sl_dev_envelopes_f64(B, R, zbd, s0, ns, env, mom, st);
+}
+
+// storm schedule of one run (host, double): rows kept when keep_rows
+int sl_make_schedule(const slg_basin *B, const char *trk, storm_sched &S, bool keep_rows, int prec, char *err,
+                     int errlen);
+// the basin's static fields on the device in precision prec (uploads once)
+int sl_basin_ensure(slg_basin *B, int prec, char *err, int errlen);
+static inline const std::vector<float> &sched_params(const storm_sched &S, float) { return S.p32; }
+static inline const std::vector<double> &sched_params(const storm_sched &S, double) { return S.p64; }
+static inline const std::vector<float> &sched_tables(const storm_sched &S, float) { return S.t32; }
+static inline const std::vector<double> &sched_tables(const storm_sched &S, double) { return S.t64; }
+
+#endif

@@ -1280,6 +1280,41 @@ static void launch_B(int smpt, int filt, int dyn, dim3 g, hipStream_t st, const 
    else hipLaunchKernelGGL((k_stepB<TH, false, false>), g, dim3(256), 0, st, B, R, a, ntx, nty, 0);
 }
 
+// ------------------------------------------------------------------ products
+// Envelope of storm s at grid point (i,j) as EnvSave writes it (slosh2.c:
+// 1611-1671): int16 tenths of a foot, (short)(HMX*10+.5) in double, 999 where
+// HMX+ZB == 0 (dry), 0 on the last row and column.  zbd is ZB in double, as
+// the reference's C driver holds it.
+__device__ __forceinline__ short env_val(const SlBasinDev &B, const real *hmx, const double *zbd, int i, int j)
+{
+   if (i == B.imxb || j == B.jmxb) return 0;
+   const uint32_t p = IDX(B, i, j);
+   const double h = (double) hmx[p];
+   if (h + zbd[p] == 0.0) return 999;
+   return (short) (int) (h * 10 + .5);
+}
+
+// Envelopes of storms s0..s0+ns-1 (env, [ns][jmxb][imxb], may be null) and
+// their MOM, the maximum of MEOWs (gui/tclsrc/meowgen.tcl:34-110): per point
+// the largest envelope value below 50 ft over the storms, 999 where none.
+__global__ void k_envelopes(SlBasinDev B, SlRunDev R, const double *zbd, int s0, int ns, short *env, short *mom)
+{
+   const int cell = blockIdx.x * blockDim.x + threadIdx.x, n = B.imxb * B.jmxb;
+   if (cell >= n) return;
+   const int i = cell % B.imxb + 1, j = cell / B.imxb + 1;
+   int m = 999;
+   bool any = false;
+   for (int s = s0; s < s0 + ns; s++) {
+      const short v = env_val(B, R.hmx + (size_t) s * R.fstride, zbd, i, j);
+      if (env) env[(size_t) (s - s0) * n + cell] = v;
+      if (v < 500 && (!any || v > m)) {
+         m = v;
+         any = true;
+      }
+   }
+   if (mom) mom[cell] = (short) (any ? m : 999);
+}
+
 // End of a step: every storm still running has completed one more step.
 __global__ void k_advance(SlRunDev R)
 {
@@ -1297,6 +1332,20 @@ using namespace SL_NS;
 // path 0 = fused (events: 0 before CONTTY, 1 after it, 2 after the replay, 3
 // after stage B); path 1 = staged (events around each of the 8 stage kernels).
 // Returns which HB buffer holds the step's result.
+#ifdef SL_F64
+#define SL_ENVS sl_dev_envelopes_f64
+#else
+#define SL_ENVS sl_dev_envelopes_f32
+#endif
+extern "C" int SL_ENVS(const SlBasinDev *B, const SlRunDev *R, const double *zbd, int s0, int ns, short *env,
+                       short *mom, hipStream_t st)
+{
+   const int n = B->imxb * B->jmxb;
+   hipLaunchKernelGGL(k_envelopes, dim3((n + 255) / 256), dim3(256), 0, st, *B, *R, zbd, s0, ns, env, mom);
+   hipError_t e = hipGetLastError();
+   return e == hipSuccess ? 0 : -(int) e;
+}
+
 extern "C" int SL_LAUNCH(const SlBasinDev *B, const SlRunDev *R, int smpt, int filt, int dyn, int parity,
                          int path, int bth, hipStream_t st, hipEvent_t *ev)
 {

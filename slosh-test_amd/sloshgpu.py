@@ -46,6 +46,18 @@ def lib():
         L.slg_batch_sync.argtypes = [vp, cp, i]
         L.slg_batch_step_timed.argtypes = [vp, i, vp, vp, cp, i]
         L.slg_batch_set_path.argtypes = [vp, i, i]
+        L.slg_batch_envelopes.argtypes = [vp, vp, vp, cp, i]
+        L.slg_track_datums.argtypes = [cp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]
+        L.slg_write_env.argtypes = [cp, i, i, cp, vp, ctypes.c_float, ctypes.c_float]
+        L.slg_server_open.argtypes = [i, ctypes.c_double, ctypes.POINTER(vp), cp, i]
+        L.slg_server_add_basin.argtypes = [vp, cp, cp, i, i, i, ctypes.POINTER(i), cp, i]
+        L.slg_server_slots.argtypes = [vp, i]
+        L.slg_server_submit.argtypes = [vp, i, cp, ctypes.POINTER(i), cp, i]
+        L.slg_server_pump.argtypes = [vp, i, cp, i]
+        L.slg_server_result.argtypes = [vp, i, vp, ctypes.POINTER(ctypes.c_double)]
+        L.slg_server_stats.argtypes = [vp, vp]
+        L.slg_server_close.argtypes = [vp]
+        L.slg_server_close.restype = None
         L.slg_batch_current_step.argtypes = [vp]
         L.slg_batch_get.argtypes = [vp, i, cp, vp, cp, i]
         L.slg_batch_stats.argtypes = [vp, i, vp, cp, i]
@@ -146,7 +158,89 @@ class Batch:
              err)
         return out
 
+    def envelopes(self, want_env=True, want_mom=True):
+        """Per-run envelopes (EnvSave int16 tenths, [nstorm, jmxb, imxb]) and
+        their MOM ([jmxb, imxb]), computed on the device."""
+        nj, ni = self.basin.jmxb, self.basin.imxb
+        env = np.zeros((self.n, nj, ni), dtype=np.int16) if want_env else None
+        mom = np.zeros((nj, ni), dtype=np.int16) if want_mom else None
+        err = ctypes.create_string_buffer(512)
+        _chk(lib().slg_batch_envelopes(self.h, env.ctypes.data_as(ctypes.c_void_p) if want_env else None,
+                                       mom.ctypes.data_as(ctypes.c_void_p) if want_mom else None,
+                                       err, 512), err)
+        return env, mom
+
     def close(self):
         if self.h:
             lib().slg_batch_close(self.h)
+            self.h = ctypes.c_void_p()
+
+
+def track_datums(trk: str):
+    a, b = ctypes.c_float(), ctypes.c_float()
+    if lib().slg_track_datums(trk.encode(), ctypes.byref(a), ctypes.byref(b)):
+        raise SloshError("cannot read %s" % trk)
+    return a.value, b.value
+
+
+def write_env(path: str, vals: np.ndarray, ht1: float, ht2: float, comment: str = "FileVersion=2013"):
+    """env file in the reference's EnvSave layout; vals int16 [jmxb, imxb]."""
+    v = np.ascontiguousarray(vals, dtype=np.int16)
+    if lib().slg_write_env(path.encode(), v.shape[1], v.shape[0], comment.encode(),
+                           v.ctypes.data_as(ctypes.c_void_p), ht1, ht2):
+        raise SloshError("cannot write %s" % path)
+
+
+class Server:
+    """Storm runs on several basins, admitted as they arrive (continuous batching)."""
+
+    def __init__(self, precision: int = 32, hbm_fraction: float = 0.5):
+        self.h = ctypes.c_void_p()
+        err = ctypes.create_string_buffer(512)
+        _chk(lib().slg_server_open(precision, hbm_fraction, ctypes.byref(self.h), err, 512), err)
+        self.dims = {}
+
+    def add_basin(self, dta: str, bnt_line: str, max_slots: int, max_steps: int, tables_per_slot: int = 64):
+        bid = ctypes.c_int()
+        err = ctypes.create_string_buffer(512)
+        _chk(lib().slg_server_add_basin(self.h, dta.encode(), bnt_line.encode(), max_slots, max_steps,
+                                        tables_per_slot, ctypes.byref(bid), err, 512), err)
+        B = Basin(dta, bnt_line)
+        self.dims[bid.value] = (B.jmxb, B.imxb, B.cells)
+        B.close()
+        return bid.value
+
+    def slots(self, basin_id: int) -> int:
+        return lib().slg_server_slots(self.h, basin_id)
+
+    def submit(self, basin_id: int, trk: str) -> int:
+        rid = ctypes.c_int()
+        err = ctypes.create_string_buffer(512)
+        _chk(lib().slg_server_submit(self.h, basin_id, trk.encode(), ctypes.byref(rid), err, 512), err)
+        return rid.value
+
+    def pump(self, k: int) -> int:
+        err = ctypes.create_string_buffer(512)
+        rc = lib().slg_server_pump(self.h, k, err, 512)
+        if rc < 0:
+            _chk(rc, err)
+        return rc
+
+    def result(self, run_id: int, basin_id: int):
+        """(state, env or None, latency or None); state 0 queued, 1 running, 2 done."""
+        nj, ni, _ = self.dims[basin_id]
+        env = np.zeros((nj, ni), dtype=np.int16)
+        lat = ctypes.c_double()
+        st = lib().slg_server_result(self.h, run_id, env.ctypes.data_as(ctypes.c_void_p), ctypes.byref(lat))
+        return (st, env, lat.value) if st == 2 else (st, None, None)
+
+    def stats(self):
+        out = np.zeros(6, dtype=np.int64)
+        lib().slg_server_stats(self.h, out.ctypes.data_as(ctypes.c_void_p))
+        return dict(queued=int(out[0]), running=int(out[1]), done=int(out[2]), slots=int(out[3]),
+                    bytes=int(out[4]), cell_steps=int(out[5]))
+
+    def close(self):
+        if self.h:
+            lib().slg_server_close(self.h)
             self.h = ctypes.c_void_p()

@@ -1,0 +1,128 @@
+"""Batched products and the serving loop on the GPU.
+
+- The device envelope of a run is the env file the reference's own writer
+  produces for it: the product's env file equals (values, datums, header) the
+  one the drop-in CLI writes through EnvSave for the same storm.
+- The device MOM is the maximum of the MEOWs by the rule of
+  gui/tclsrc/meowgen.tcl:34-110, recomputed here from the per-run envelopes.
+- Serving: a run's envelope is the same when it is served alone, in a mix of
+  runs on two basins admitted at different times, and through the batch API."""
+import os
+
+import numpy as np
+import pytest
+
+import helpers
+import sloshgpu
+from helpers import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _tracks(b, root, n, hours=6, seed=0):
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n):
+        tr = synth.straight_track(b, float(rng.uniform(6, 20)), float(rng.uniform(0, 360)),
+                                  float(rng.uniform(30, 90)), float(rng.uniform(12, 30)), hours=hours,
+                                  landfall=hours // 2, offset_mi=float(rng.uniform(-10, 10)))
+        p = os.path.join(root, "r%03d_%d.trk" % (k, seed))
+        open(p, "w").write(synth.trk_text(tr))
+        out.append(p)
+    return out
+
+
+@pytest.mark.skipif(not os.path.exists(helpers.GPU_CLI), reason="drop-in CLI not built")
+def test_env_file_equals_dropin_cli(tmp_path):
+    b, t = synth.tiny(seed=5)
+    root = str(tmp_path)
+    case = synth.write_case(root, b, t)
+    env_cli, _ = helpers.run_cli(helpers.GPU_CLI, root, case["basin"], "cli")
+    B = sloshgpu.Basin(case["dta"], case["bntline"])
+    X = sloshgpu.Batch(B, [case["trk"]])
+    X.upload()
+    X.step(X.nsteps[0])
+    env, mom = X.envelopes()
+    ht1, ht2 = sloshgpu.track_datums(case["trk"])
+    mine = os.path.join(root, "mine.env")
+    sloshgpu.write_env(mine, env[0], ht1, ht2)
+    X.close()
+    B.close()
+    a, c = helpers.read_env(mine), helpers.read_env(env_cli)
+    assert a[:3] == c[:3]
+    assert np.array_equal(a[3], c[3])
+    assert a[4] == c[4]
+
+
+def test_mom_is_max_of_meows(tmp_path):
+    b = synth.tiny(seed=11, ni=32, nj=28, hours=6)[0]
+    root = str(tmp_path)
+    trks = _tracks(b, root, 12, seed=1)
+    case = synth.write_case(root, b, synth.tiny(seed=11, ni=32, nj=28, hours=6)[1])
+    B = sloshgpu.Basin(case["dta"], case["bntline"])
+    X = sloshgpu.Batch(B, trks)
+    X.upload()
+    X.step(max(X.nsteps))
+    env, mom = X.envelopes()
+    X.close()
+    B.close()
+    v = env.astype(np.int32)
+    ok = v < 500                       # meowgen.tcl: value/10 < 50 ft counts
+    best = np.where(ok, v, -10 ** 6).max(axis=0)
+    want = np.where(ok.any(axis=0), best, 999)
+    assert np.array_equal(mom.astype(np.int32), want)
+    assert (env == 999).any() and (env != 999).any()
+
+
+@pytest.mark.parametrize("precision", [32, 64])
+def test_served_run_same_alone_and_in_mix(precision, tmp_path):
+    root = str(tmp_path)
+    b1, t1 = synth.tiny(seed=21, ni=30, nj=26, hours=6)
+    b2, t2 = synth.tiny(seed=22, gtype="$", ni=26, nj=22, hours=6)
+    b2.name = "tnz"
+    c1 = synth.write_case(os.path.join(root, "b1"), b1, t1)
+    c2 = synth.write_case(os.path.join(root, "b2"), b2, t2)
+    trk1 = _tracks(b1, root, 9, seed=3)
+    trk2 = _tracks(b2, root, 5, seed=4)
+    probe = trk1[4]
+    print("serve precision", precision, flush=True)
+
+    # alone
+    S = sloshgpu.Server(precision=precision)
+    bid = S.add_basin(c1["dta"], c1["bntline"], max_slots=4, max_steps=800)
+    rid = S.submit(bid, probe)
+    while S.result(rid, bid)[0] != 2:
+        S.pump(50)
+    alone = S.result(rid, bid)[1]
+    S.close()
+
+    # in a mix: two basins, 4 slots each (fewer than runs), arrivals spread over
+    # the pumps so runs join pools that are already stepping
+    S = sloshgpu.Server(precision=precision)
+    b_1 = S.add_basin(c1["dta"], c1["bntline"], max_slots=4, max_steps=800)
+    b_2 = S.add_basin(c2["dta"], c2["bntline"], max_slots=4, max_steps=800)
+    ids = []
+    arrivals = [(b_1, p) for p in trk1] + [(b_2, p) for p in trk2]
+    rng = np.random.default_rng(7)
+    rng.shuffle(arrivals)
+    pending = list(arrivals)
+    while pending or any(S.result(r, bb)[0] != 2 for r, bb, _ in ids):
+        for _ in range(min(len(pending), 2)):
+            bb, p = pending.pop()
+            ids.append((S.submit(bb, p), bb, p))
+        S.pump(37)
+    st = S.stats()
+    assert st["done"] == len(arrivals) and st["queued"] == 0
+    mix = {p: S.result(r, bb)[1] for r, bb, p in ids}
+    S.close()
+    assert np.array_equal(mix[probe], alone)
+
+    # and the batch API on the same run
+    B = sloshgpu.Basin(c1["dta"], c1["bntline"])
+    X = sloshgpu.Batch(B, [probe], precision=precision)
+    X.upload()
+    X.step(X.nsteps[0])
+    env, _ = X.envelopes(want_mom=False)
+    X.close()
+    B.close()
+    assert np.array_equal(env[0], alone)
