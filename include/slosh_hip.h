@@ -79,6 +79,16 @@ int slg_batch_sync(slg_batch *x, char *err, int errlen);
 #define SLG_PATH_STAGED 1
 /* Choose the launch path and the fused tile height (8, 16 or 32 rows). */
 int slg_batch_set_path(slg_batch *x, int path, int tile_rows);
+/* Replay the fused step loop as hipGraphs of steps_per_graph steps (0: off).
+ * The captured steps let every storm's own step flags decide, so one graph
+ * serves the whole run. */
+int slg_batch_set_graph(slg_batch *x, int steps_per_graph);
+/* Online autotuner: times fused tiles of 8/16/32 rows and the staged path on
+ * consecutive chunks of steps_per_trial real steps (all give identical fields)
+ * and keeps the fastest; ms_out[4] (may be NULL) gets each candidate's time per
+ * step. */
+int slg_batch_autotune(slg_batch *x, int steps_per_trial, int *rows_out, int *path_out, double *ms_out,
+                       char *err, int errlen);
 /* slg_batch_step with HIP events around every kernel on the batch's stream
  * (synchronises after each step): ms[k] += kernel k time, launches[k] += 1. */
 int slg_batch_step_timed(slg_batch *x, int n, double *ms, int *launches, char *err, int errlen);

@@ -470,13 +470,30 @@ extern "C" int slg_batch_open_prec(slg_basin *B, int nstorm, const char *const *
    X->sc.resize(nstorm);
    X->maxsteps = 0;
    X->dmem = nullptr;
-   for (int s = 0; s < nstorm; s++) {
-      int rc = sl_make_schedule(B, trk[s], X->sc[s], nstorm <= 8, precision, err, errlen);
-      if (rc) {
-         delete X;
-         return rc;
+   // storm schedules are independent: compute them on up to 16 host threads
+   {
+      const int nt = std::max(1, std::min<int>({16, nstorm, (int) std::thread::hardware_concurrency()}));
+      std::vector<int> rcs(nstorm, 0);
+      std::vector<std::string> msgs(nstorm);
+      std::vector<std::thread> pool;
+      for (int t = 0; t < nt; t++)
+         pool.emplace_back([&, t]() {
+            char e[512];
+            for (int s = t; s < nstorm; s += nt) {
+               e[0] = 0;
+               rcs[s] = sl_make_schedule(B, trk[s], X->sc[s], nstorm <= 8, precision, e, sizeof e);
+               if (rcs[s]) msgs[s] = e;
+            }
+         });
+      for (auto &th : pool) th.join();
+      for (int s = 0; s < nstorm; s++) {
+         if (rcs[s]) {
+            seterr(err, errlen, "%s", msgs[s].c_str());
+            delete X;
+            return rcs[s];
+         }
+         if (X->sc[s].nsteps > X->maxsteps) X->maxsteps = X->sc[s].nsteps;
       }
-      if (X->sc[s].nsteps > X->maxsteps) X->maxsteps = X->sc[s].nsteps;
    }
    // initial water (INTLHT) of every run, kept in double on the host
    const size_t np = (size_t) B->h.imxb * B->h.jmxb;
@@ -503,6 +520,8 @@ extern "C" int slg_batch_precision(const slg_batch *X)
    return X->prec;
 }
 
+static void drop_graph(slg_batch *X);
+
 extern "C" int slg_batch_set_path(slg_batch *X, int path, int tile_rows)
 {
    if ((path != SLG_PATH_FUSED && path != SLG_PATH_STAGED) ||
@@ -510,6 +529,7 @@ extern "C" int slg_batch_set_path(slg_batch *X, int path, int tile_rows)
       return SLG_EINPUT;
    X->path = path;
    X->tile_rows = tile_rows;
+   drop_graph(X);
    return 0;
 }
 
@@ -613,11 +633,48 @@ extern "C" int slg_batch_upload(slg_batch *X, char *err, int errlen)
    return X->prec == 64 ? batch_upload<double>(X, err, errlen) : batch_upload<float>(X, err, errlen);
 }
 
-static int launch_one(slg_batch *X, int smpt, int filt, hipEvent_t *ev)
+static int launch_one(slg_batch *X, int smpt, int filt, hipEvent_t *ev, int dyn = 0)
 {
    if (X->prec == 64)
-      return dev_launch(&X->b->i64.dev, &X->d64, smpt, filt, 0, X->parity, X->path, X->tile_rows, X->st, ev);
-   return dev_launch(&X->b->i32.dev, &X->d32, smpt, filt, 0, X->parity, X->path, X->tile_rows, X->st, ev);
+      return dev_launch(&X->b->i64.dev, &X->d64, smpt, filt, dyn, X->parity, X->path, X->tile_rows, X->st, ev);
+   return dev_launch(&X->b->i32.dev, &X->d32, smpt, filt, dyn, X->parity, X->path, X->tile_rows, X->st, ev);
+}
+
+static void drop_graph(slg_batch *X)
+{
+   if (X->gexec) (void) hipGraphExecDestroy(X->gexec);
+   if (X->graph) (void) hipGraphDestroy(X->graph);
+   X->gexec = nullptr;
+   X->graph = nullptr;
+}
+
+// Capture graph_steps fused steps in which every storm's own step flags decide
+// (so one graph serves every step of the run) and instantiate it.
+static int build_graph(slg_batch *X, char *err, int errlen)
+{
+   drop_graph(X);
+   HIPOK(hipStreamBeginCapture(X->st, hipStreamCaptureModeThreadLocal));
+   int rc = 0;
+   for (int k = 0; k < X->graph_steps && rc >= 0; k++) rc = launch_one(X, 1, 1, nullptr, 1);
+   hipGraph_t g = nullptr;
+   HIPOK(hipStreamEndCapture(X->st, &g));
+   if (rc < 0) {
+      (void) hipGraphDestroy(g);
+      seterr(err, errlen, "kernel launch failed in capture (%s%d)", "", -rc);
+      return SLG_EHIP;
+   }
+   X->graph = g;
+   HIPOK(hipGraphInstantiate(&X->gexec, g, nullptr, nullptr, 0));
+   return 0;
+}
+
+// Replay the step loop as hipGraphs of n steps each (fused path; 0 = off).
+extern "C" int slg_batch_set_graph(slg_batch *X, int steps_per_graph)
+{
+   if (steps_per_graph < 0) return SLG_EINPUT;
+   drop_graph(X);
+   X->graph_steps = steps_per_graph;
+   return 0;
 }
 
 // after a lockstep step: where each storm still running left its HB
@@ -632,6 +689,29 @@ extern "C" int slg_batch_step(slg_batch *X, int n, char *err, int errlen)
 {
    const slg_basin *B = X->b;
    const int lim = X->maxsteps;   // storms with shorter runs stop at their own end
+   if (X->graph_steps > 0 && X->path == SLG_PATH_FUSED) {
+      // whole graphs while they fit, then single steps with the same kernels
+      if (!X->gexec) {
+         int rc = build_graph(X, err, errlen);
+         if (rc) return rc;
+      }
+      while (n >= X->graph_steps && X->step + X->graph_steps <= lim) {
+         HIPOK(hipGraphLaunch(X->gexec, X->st));
+         X->step += X->graph_steps;
+         n -= X->graph_steps;
+      }
+      for (; n > 0 && X->step < lim; n--) {
+         int rc = launch_one(X, 1, 1, nullptr, 1);
+         if (rc < 0) {
+            seterr(err, errlen, "kernel launch failed (%s%d)", "", -rc);
+            return SLG_EHIP;
+         }
+         X->step++;
+      }
+      X->hbs_buf = -1;
+      note_step(X, X->step, 0);
+      return 0;
+   }
    for (int k = 0; k < n && X->step < lim; k++) {
       int step = X->step + 1;
       int itime = step;
@@ -935,9 +1015,55 @@ extern "C" int slg_write_env(const char *path, int imxb, int jmxb, const char *c
    return ok ? 0 : SLG_EINPUT;
 }
 
+// Online autotuner: time each launch configuration (fused with 8, 16 and 32
+// row tiles, and staged) on consecutive chunks of the run's own steps -- every
+// configuration computes identical fields, so the trial steps are real
+// progress -- and keep the fastest.  ms_out (4 entries, may be NULL) receives
+// the time per step of each candidate.
+extern "C" int slg_batch_autotune(slg_batch *X, int steps_per_trial, int *rows_out, int *path_out,
+                                  double *ms_out, char *err, int errlen)
+{
+   const int cand[4][2] = {{SLG_PATH_FUSED, 8}, {SLG_PATH_FUSED, 16}, {SLG_PATH_FUSED, 32}, {SLG_PATH_STAGED, 16}};
+   double best = 1e300;
+   int bi = 1;
+   const int keep_graph = X->graph_steps;
+   X->graph_steps = 0;
+   drop_graph(X);
+   hipEvent_t e0, e1;
+   HIPOK(hipEventCreate(&e0));
+   HIPOK(hipEventCreate(&e1));
+   for (int c = 0; c < 4; c++) {
+      if (X->step + steps_per_trial > X->maxsteps) break;
+      X->path = cand[c][0];
+      X->tile_rows = cand[c][1];
+      HIPOK(hipEventRecord(e0, X->st));
+      int rc = slg_batch_step(X, steps_per_trial, err, errlen);
+      if (rc) return rc;
+      HIPOK(hipEventRecord(e1, X->st));
+      HIPOK(hipEventSynchronize(e1));
+      float ms = 0.f;
+      HIPOK(hipEventElapsedTime(&ms, e0, e1));
+      const double per = ms / steps_per_trial;
+      if (ms_out) ms_out[c] = per;
+      if (per < best) {
+         best = per;
+         bi = c;
+      }
+   }
+   (void) hipEventDestroy(e0);
+   (void) hipEventDestroy(e1);
+   X->path = cand[bi][0];
+   X->tile_rows = cand[bi][1];
+   X->graph_steps = keep_graph;
+   if (rows_out) *rows_out = X->tile_rows;
+   if (path_out) *path_out = X->path;
+   return 0;
+}
+
 extern "C" void slg_batch_close(slg_batch *X)
 {
    if (!X) return;
+   drop_graph(X);
    if (X->dmem) {
       (void) hipStreamSynchronize(X->st);
       (void) hipStreamDestroy(X->st);

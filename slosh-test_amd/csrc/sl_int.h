@@ -10,6 +10,8 @@
 #include <string.h>
 #include <vector>
 #include <string>
+#include <thread>
+#include <algorithm>
 #include "sl_host.h"
 #include "sl_dev.h"
 #include "../../include/slosh_hip.h"
@@ -88,6 +90,9 @@ struct slg_batch {
    int path, tile_rows;                // launch path (SLG_PATH_*) and stage-B tile height
    int hbs_buf;                        // buffer holding HB before FILTER of the last step (-1: none)
    std::vector<int> last_par;          // per storm: buffer holding its HB after its last step
+   int graph_steps = 0;                // > 0: replay the step loop as hipGraphs of this many steps
+   hipGraph_t graph = nullptr;
+   hipGraphExec_t gexec = nullptr;
    std::vector<storm_sched> sc;
    std::vector<double> hb0;            // initial HB per storm (double, imxb*jmxb)
    void *dmem;

@@ -10,11 +10,18 @@
 // slot's own step counter (runs in one pool are at different steps) and skip
 // empty or finished slots.  A finished run's envelope (EnvSave quantisation)
 // is computed on the device and copied out; its slot and tables are freed.
+// A run's storm schedule (the host-side TTIMER/STMVAL/FORCE1 sequence) and
+// initial water are prepared on worker threads between submission and
+// admission, so arrivals do not stall the stepping.
 // A run's fields depend only on its own track and basin, so its result is the
 // same alone or in any mix.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
+#include <condition_variable>
 #include <deque>
+#include <memory>
+#include <mutex>
 #include "sl_int.h"
 
 namespace {
@@ -40,7 +47,9 @@ struct Pool {
 };
 
 struct Run {
-   int basin = 0, state = 0, slot = -1;   // state: 0 queued, 1 running, 2 done, 3 failed
+   int basin = 0, slot = -1;
+   std::atomic<int> state{-1};   // -1 being prepared, 0 queued, 1 running, 2 done, 3 failed
+   std::string trk, err;
    storm_sched sched;
    std::vector<double> hb0;
    std::vector<short> env;
@@ -81,10 +90,53 @@ struct slg_server {
    double frac = 0.5;
    int tile_rows = 16;
    std::vector<Pool> pools;
-   std::vector<Run> runs;
-   std::deque<int> queue;
+   std::vector<std::unique_ptr<Run>> runs;
+   std::deque<int> queue;   // submitted, not yet admitted (submission order)
    long long cell_steps = 0;
+   // schedule preparation workers
+   std::mutex mu;
+   std::condition_variable cv;
+   std::deque<Run *> todo;
+   bool stop = false;
+   std::vector<std::thread> workers;
 };
+
+static void prepare(slg_server *S, Run *r)
+{
+   Pool &P = S->pools[r->basin];
+   char e[512];
+   e[0] = 0;
+   int rc = sl_make_schedule(P.basin, r->trk.c_str(), r->sched, false, S->prec, e, sizeof e);
+   if (!rc && (r->sched.nsteps > P.maxsteps || (int) r->sched.tab_step.size() > P.tsets)) {
+      snprintf(e, sizeof e, "%s", "run longer (or with more table sets) than the pool's slots hold");
+      rc = SLG_ELIMIT;
+   }
+   if (rc) {
+      r->err = e;
+      r->state = 3;
+      return;
+   }
+   const size_t npt = (size_t) P.basin->h.imxb * P.basin->h.jmxb;
+   r->hb0.resize(npt);
+   sl_initial_water(&P.basin->h, &r->sched.s0, r->hb0.data());
+   r->ntab = (int) r->sched.tab_step.size();
+   r->state = 0;
+}
+
+static void worker(slg_server *S)
+{
+   for (;;) {
+      Run *r;
+      {
+         std::unique_lock<std::mutex> lk(S->mu);
+         S->cv.wait(lk, [S] { return S->stop || !S->todo.empty(); });
+         if (S->stop && S->todo.empty()) return;
+         r = S->todo.front();
+         S->todo.pop_front();
+      }
+      prepare(S, r);
+   }
+}
 
 static size_t slot_bytes(const slg_basin *B, int maxsteps, int tsets_per_slot, int prec)
 {
@@ -160,6 +212,8 @@ extern "C" int slg_server_open(int precision, double hbm_fraction, slg_server **
    slg_server *S = new slg_server();
    S->prec = precision;
    S->frac = hbm_fraction;
+   const int nw = std::max(1, std::min<int>(8, (int) std::thread::hardware_concurrency()));
+   for (int k = 0; k < nw; k++) S->workers.emplace_back(worker, S);
    *out = S;
    return 0;
 }
@@ -217,7 +271,8 @@ extern "C" int slg_server_slots(const slg_server *S, int basin_id)
    return (basin_id >= 0 && basin_id < (int) S->pools.size()) ? S->pools[basin_id].slots : -1;
 }
 
-// Submit a run (trk file) on a registered basin; its schedule is computed now.
+// Submit a run (trk file) on a registered basin; its schedule is prepared on a
+// worker thread and the run waits for a slot.
 extern "C" int slg_server_submit(slg_server *S, int basin_id, const char *trk, int *run_id, char *err,
                                  int errlen)
 {
@@ -225,30 +280,26 @@ extern "C" int slg_server_submit(slg_server *S, int basin_id, const char *trk, i
       seterr(err, errlen, "%s", "unknown basin");
       return SLG_EINPUT;
    }
-   Pool &P = S->pools[basin_id];
-   Run r;
-   r.basin = basin_id;
-   r.t_submit = now_s();
-   int rc = sl_make_schedule(P.basin, trk, r.sched, false, S->prec, err, errlen);
-   if (rc) return rc;
-   if (r.sched.nsteps > P.maxsteps || (int) r.sched.tab_step.size() > P.tsets) {
-      seterr(err, errlen, "%s", "run longer (or with more table sets) than the pool's slots hold");
-      return SLG_ELIMIT;
-   }
-   const size_t npt = (size_t) P.basin->h.imxb * P.basin->h.jmxb;
-   r.hb0.resize(npt);
-   sl_initial_water(&P.basin->h, &r.sched.s0, r.hb0.data());
-   r.ntab = (int) r.sched.tab_step.size();
+   std::unique_ptr<Run> r(new Run());
+   r->basin = basin_id;
+   r->trk = trk;
+   r->t_submit = now_s();
+   Run *rp = r.get();
    S->runs.push_back(std::move(r));
    *run_id = (int) S->runs.size() - 1;
    S->queue.push_back(*run_id);
+   {
+      std::lock_guard<std::mutex> lk(S->mu);
+      S->todo.push_back(rp);
+   }
+   S->cv.notify_one();
    return 0;
 }
 
 template <class R>
 static int admit(slg_server *S, Pool &P, int slot, int id, char *err, int errlen)
 {
-   Run &r = S->runs[id];
+   Run &r = *S->runs[id];
    const slg_basin *B = P.basin;
    const sl_basin *b = &B->h;
    const int PI = B->pitch, NJ = b->jmxb;
@@ -288,7 +339,7 @@ template <class R>
 static int retire(slg_server *S, Pool &P, int slot, char *err, int errlen)
 {
    const int id = P.run[slot];
-   Run &r = S->runs[id];
+   Run &r = *S->runs[id];
    BasinImg<R> &I = img(P.basin, R());
    SlRunDevT<R> &D = *(SlRunDevT<R> *) (sizeof(R) == 8 ? (void *) &P.d64 : (void *) &P.d32);
    const size_t n = (size_t) P.basin->h.imxb * P.basin->h.jmxb;
@@ -338,7 +389,12 @@ extern "C" int slg_server_pump(slg_server *S, int k, char *err, int errlen)
    while (!S->queue.empty()) {
       const int id = S->queue.front();
       S->queue.pop_front();
-      Run &r = S->runs[id];
+      Run &r = *S->runs[id];
+      if (r.state == 3) continue;   // failed in preparation
+      if (r.state != 0) {           // still being prepared
+         wait.push_back(id);
+         continue;
+      }
       Pool &P = S->pools[r.basin];
       int slot = -1;
       for (int q = 0; q < P.slots && slot < 0; q++)
@@ -386,7 +442,7 @@ extern "C" int slg_server_pump(slg_server *S, int k, char *err, int errlen)
 extern "C" int slg_server_result(const slg_server *S, int run_id, short *env, double *latency_s)
 {
    if (run_id < 0 || run_id >= (int) S->runs.size()) return SLG_EINPUT;
-   const Run &r = S->runs[run_id];
+   const Run &r = *S->runs[run_id];
    if (r.state == 2) {
       if (env) memcpy(env, r.env.data(), r.env.size() * 2);
       if (latency_s) *latency_s = r.t_done - r.t_submit;
@@ -400,8 +456,8 @@ extern "C" int slg_server_stats(const slg_server *S, long long *out6)
 {
    long long q = (long long) S->queue.size(), run = 0, done = 0, slots = 0, bytes = 0;
    for (auto &r : S->runs) {
-      run += r.state == 1;
-      done += r.state == 2;
+      run += r->state == 1;
+      done += r->state == 2;
    }
    for (auto &P : S->pools) {
       slots += P.slots;
@@ -419,6 +475,12 @@ extern "C" int slg_server_stats(const slg_server *S, long long *out6)
 extern "C" void slg_server_close(slg_server *S)
 {
    if (!S) return;
+   {
+      std::lock_guard<std::mutex> lk(S->mu);
+      S->stop = true;
+   }
+   S->cv.notify_all();
+   for (auto &w : S->workers) w.join();
    for (auto &P : S->pools) {
       if (P.st) {
          (void) hipStreamSynchronize(P.st);

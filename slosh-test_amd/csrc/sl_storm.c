@@ -85,9 +85,10 @@ static int read_trk(const char *path, sl_storm *s, char err[256])
    int beg = -999, end = -999, land = -999, dhr = -999, press = 0, ntrk = 0, adj = 0;
    double ocean = -999.9, lake = -999.9, windf = 1.0, rmwf = 1.0, amb = -999.9;
    int spline_mode = 0, in_data = 0, i, jj;
-   static int hrs[SL_NT + 2];
-   static double fla[SL_NT + 2], flo[SL_NT + 2], fdp[SL_NT + 2], frm[SL_NT + 2];
-   static double hla[SL_NT + 2], hlo[SL_NT + 2], hdp[SL_NT + 2], hrm[SL_NT + 2];
+   /* per call (tracks may be read on several threads at once) */
+   int hrs[SL_NT + 2];
+   double fla[SL_NT + 2], flo[SL_NT + 2], fdp[SL_NT + 2], frm[SL_NT + 2];
+   double hla[SL_NT + 2], hlo[SL_NT + 2], hdp[SL_NT + 2], hrm[SL_NT + 2];
    int ihour = 0;
 
    if (!fp) {
@@ -210,7 +211,7 @@ static int read_trk(const char *path, sl_storm *s, char err[256])
          }
       }
    } else {
-      static double xh[SL_NT + 2], y2a[SL_NT + 2], y2o[SL_NT + 2], y2d[SL_NT + 2], y2r[SL_NT + 2];
+      double xh[SL_NT + 2], y2a[SL_NT + 2], y2o[SL_NT + 2], y2d[SL_NT + 2], y2r[SL_NT + 2];
       for (i = 1; i <= ntrk; i++) xh[i] = hrs[i];
       spline(xh, fla, ntrk, 1.0E+30, 1.0E+30, y2a);
       spline(xh, flo, ntrk, 1.0E+30, 1.0E+30, y2o);

@@ -46,6 +46,8 @@ def lib():
         L.slg_batch_sync.argtypes = [vp, cp, i]
         L.slg_batch_step_timed.argtypes = [vp, i, vp, vp, cp, i]
         L.slg_batch_set_path.argtypes = [vp, i, i]
+        L.slg_batch_set_graph.argtypes = [vp, i]
+        L.slg_batch_autotune.argtypes = [vp, i, ctypes.POINTER(i), ctypes.POINTER(i), vp, cp, i]
         L.slg_batch_envelopes.argtypes = [vp, vp, vp, cp, i]
         L.slg_track_datums.argtypes = [cp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]
         L.slg_write_env.argtypes = [cp, i, i, cp, vp, ctypes.c_float, ctypes.c_float]
@@ -117,6 +119,20 @@ class Batch:
     def set_path(self, path: int = PATH_FUSED, tile_rows: int = 16):
         if lib().slg_batch_set_path(self.h, path, tile_rows) != 0:
             raise SloshError("bad launch path %r / tile rows %r" % (path, tile_rows))
+
+    def set_graph(self, steps_per_graph: int):
+        if lib().slg_batch_set_graph(self.h, steps_per_graph) != 0:
+            raise SloshError("bad graph length %r" % steps_per_graph)
+
+    def autotune(self, steps_per_trial: int = 20):
+        """Times fused 8/16/32-row tiles and the staged path on real steps and
+        keeps the fastest; returns (tile_rows, path, ms per step of each)."""
+        rows, path = ctypes.c_int(), ctypes.c_int()
+        ms = np.zeros(4)
+        err = ctypes.create_string_buffer(512)
+        _chk(lib().slg_batch_autotune(self.h, steps_per_trial, ctypes.byref(rows), ctypes.byref(path),
+                                      ms.ctypes.data_as(ctypes.c_void_p), err, 512), err)
+        return rows.value, path.value, ms
 
     def upload(self):
         err = ctypes.create_string_buffer(512)
@@ -232,6 +248,8 @@ class Server:
         env = np.zeros((nj, ni), dtype=np.int16)
         lat = ctypes.c_double()
         st = lib().slg_server_result(self.h, run_id, env.ctypes.data_as(ctypes.c_void_p), ctypes.byref(lat))
+        if st == 3:
+            raise SloshError("run %d failed in preparation" % run_id)
         return (st, env, lat.value) if st == 2 else (st, None, None)
 
     def stats(self):

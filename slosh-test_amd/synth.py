@@ -453,3 +453,57 @@ def tiny(seed=5, gtype=" ", ni=24, nj=20, dt=60.0, hours=6):
     b = make_basin("tny", gtype, ni, nj, seed=seed, land_frac=0.25,
                    n_barriers=max(4, ni * nj // 60), dt=dt, size_km=(1.0, 3.0), corr=4.0)
     return b, straight_track(b, 12.0, 330.0, 60.0, 15.0, hours=hours, landfall=hours // 2)
+
+
+# uneven ensemble: six basins of uneven size (name, grid type, ni, nj, barrier edges)
+C4_BASINS = [("pla", " ", 80, 60, 100), ("plb", " ", 150, 120, 250), ("ela", "$", 200, 250, 400),
+             ("elb", "$", 300, 300, 600), ("hya", "+", 350, 200, 500), ("elc", "$", 400, 350, 800)]
+
+
+def config4_basins(seed=31, dt=30.0):
+    """The six basins of the uneven ensemble: polar 80x60 and 150x120, elliptic
+    200x250, 300x300 and 400x350, hyperbolic 350x200 (seeded depth fields)."""
+    out = []
+    for k, (nm, g, ni, nj, nbar) in enumerate(C4_BASINS):
+        out.append(make_basin(nm, g, ni, nj, seed=seed + k, land_frac=0.25, n_barriers=nbar, dt=dt))
+    return out
+
+
+def random_track(b: Basin, rng, hours_range=(24, 72)):
+    """Uniformly drawn storm: heading 0-360, 5-25 kt, 20-100 mb, RMW 10-40 mi,
+    run length in hours_range, landfall at mid-run, offset +-20 mi."""
+    hours = int(rng.integers(hours_range[0], hours_range[1] + 1))
+    return straight_track(b, float(rng.uniform(5, 25)), float(rng.uniform(0, 360)),
+                          float(rng.uniform(20, 100)), float(rng.uniform(10, 40)), hours=hours,
+                          landfall=hours // 2, offset_mi=float(rng.uniform(-20, 20)))
+
+
+def forecast_track(b: Basin, rng, hours=72):
+    """Forecast-style track: 6-hourly points (5-13 of them over the horizon) with
+    deficit, RMW and speed varying point to point, interpolated hourly by the
+    reference's linear interpolation (trk InterpolationType=Linear)."""
+    npts = int(rng.integers(5, 14))
+    step = max(6, int(math.ceil(hours / (npts - 1) / 6.0)) * 6)
+    H = [k * step for k in range(npts)]
+    while H[-1] < hours + 2:
+        H.append(H[-1] + step)
+    hd = math.radians(rng.uniform(0, 360))
+    la, lo, dps, rms = [], [], [], []
+    land = hours // 2
+    x = y = 0.0
+    pts = []
+    for h in H:
+        pts.append((x, y))
+        spd = rng.uniform(5, 25) * MI_PER_KT
+        hd += rng.uniform(-0.2, 0.2)
+        x += spd * step * math.sin(hd)
+        y += spd * step * math.cos(hd)
+        dps.append(float(rng.uniform(20, 100)))
+        rms.append(float(rng.uniform(10, 40)))
+    # pass the basin's tangent point at the landfall hour
+    kl = min(range(len(H)), key=lambda k: abs(H[k] - land))
+    cx, cy = pts[kl]
+    for (px, py) in pts:
+        la.append(b.alto + (py - cy) / 69.05)
+        lo.append(-b.alno + (px - cx) / (69.05 * math.cos(math.radians(b.alto))))
+    return Track(H, la, lo, dps, rms, 0, hours, H[kl], interp="Linear")

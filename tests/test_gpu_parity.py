@@ -234,3 +234,37 @@ def test_uneven_run_lengths_equal_alone(tmp_path):
                 assert np.array_equal(A.get(0, f), mix[s][f]), (path, s, f)
             A.close()
     B.close()
+
+
+@pytest.mark.parametrize("precision", [32, 64])
+def test_graph_replay_and_autotune_bitwise(precision, tmp_path):
+    """The step loop replayed as hipGraphs (storm-own step flags), and a run whose
+    first steps went to the autotuner's trials, both give the fields of the
+    plain fused loop."""
+    b, t = synth.tiny(seed=51, ni=30, nj=26, hours=6)
+    case = helpers.make_case(str(tmp_path), b, t)
+    tracks = synth.meow_tracks(b, n=1024, hours=6, landfall=3)[::200]
+    paths = [case["trk"]]
+    for k, tr in enumerate(tracks):
+        p = str(tmp_path / ("g%d.trk" % k))
+        open(p, "w").write(synth.trk_text(tr))
+        paths.append(p)
+    B = sloshgpu.Basin(case["dta"], case["bntline"])
+    res = {}
+    for mode in ("plain", "graph", "tuned"):
+        X = sloshgpu.Batch(B, paths, precision=precision)
+        X.upload()
+        if mode == "graph":
+            X.set_graph(16)
+        if mode == "tuned":
+            rows, path, ms = X.autotune(20)
+            print("autotune", rows, path, ms, flush=True)
+        X.step(max(X.nsteps) - X.current_step)
+        X.sync()
+        res[mode] = [{f: X.get(s, f) for f in helpers.FIELDS} for s in range(len(paths))]
+        X.close()
+    B.close()
+    for mode in ("graph", "tuned"):
+        for s in range(len(paths)):
+            for f in helpers.FIELDS:
+                assert np.array_equal(res[mode][s][f], res["plain"][s][f]), (mode, s, f)
