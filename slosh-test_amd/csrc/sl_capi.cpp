@@ -185,9 +185,11 @@ static int basin_upload(slg_basin *B, char *err, int errlen)
    HIPOK(hipMalloc((void **) &d, tot));
    I.dmem = d;
    int m = 0;
+   hipError_t upe = hipSuccess;
    auto up = [&](const void *src, size_t bytes) -> char * {
       char *dst = d + off[m++];
-      hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
+      const hipError_t e = hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
+      if (upe == hipSuccess) upe = e;
       return dst;
    };
    SlBasinDevT<R> &D = I.dev;
@@ -230,6 +232,7 @@ static int basin_upload(slg_basin *B, char *err, int errlen)
          for (int i = 1; i <= b->imxb; i++) zd[(size_t) (j - 1) * P + (i - 1)] = b->zb[PIX(b, i, j)];
       I.zbd = (const double *) up(zd.data(), np * 8);
    }
+   HIPOK(upe);
    HIPOK(hipDeviceSynchronize());
    I.on_dev = true;
    return 0;
@@ -737,46 +740,74 @@ extern "C" int slg_batch_step(slg_batch *X, int n, char *err, int errlen)
 extern "C" int slg_batch_step_timed(slg_batch *X, int n, double *ms, int *launches, char *err,
                                     int errlen)
 {
+   // events around every kernel of a ring of RING steps; the host waits only
+   // when the ring is full, so timing does not serialise the step loop
+   const int RING = 64, NE = SLG_NKERN + 1;
    const slg_basin *B = X->b;
-   hipEvent_t ev[SLG_NKERN + 1];
-   for (int k = 0; k <= SLG_NKERN; k++) HIPOK(hipEventCreate(&ev[k]));
+   std::vector<hipEvent_t> ev((size_t) RING * NE);
+   struct Slot {
+      int smpt, filt;
+   };
+   std::vector<Slot> ran(RING);
+   for (auto &e : ev) HIPOK(hipEventCreate(&e));
+   int pending = 0, rc_all = 0;
+   auto drain = [&]() -> int {
+      if (!pending) return 0;
+      if (hipEventSynchronize(ev[(size_t) (pending - 1) * NE + (X->path == SLG_PATH_FUSED ? 3 : 8)]) != hipSuccess)
+         return SLG_EHIP;
+      for (int r = 0; r < pending; r++) {
+         hipEvent_t *E = &ev[(size_t) r * NE];
+         float t = 0.f;
+         if (X->path == SLG_PATH_FUSED) {
+            const int slot[3] = {SLG_K_CONTTY, SLG_K_REPLAY, SLG_K_STEPB};
+            for (int q = 0; q < 3; q++) {
+               if (hipEventElapsedTime(&t, E[q], E[q + 1]) != hipSuccess) return SLG_EHIP;
+               ms[slot[q]] += t;
+               launches[slot[q]]++;
+            }
+            continue;
+         }
+         const int on[8] = {1, 1, ran[r].smpt, ran[r].smpt, 1, 1, ran[r].filt, 1};
+         for (int q = 0; q < 8; q++) {
+            if (!on[q]) continue;
+            if (hipEventElapsedTime(&t, E[q], E[q + 1]) != hipSuccess) return SLG_EHIP;
+            ms[q] += t;
+            launches[q]++;
+         }
+      }
+      pending = 0;
+      return 0;
+   };
    const int lim = X->maxsteps;
    for (int k = 0; k < n && X->step < lim; k++) {
       int step = X->step + 1, itime = step;
       int smpt = (B->ismpt2g && itime > 5 && B->incsm != 1);
       int filt = ((itime + 1) % B->incsm == 0 || B->h.ebsn1 == '+');
-      int rc = launch_one(X, smpt, filt, ev);
+      int rc = launch_one(X, smpt, filt, &ev[(size_t) pending * NE]);
       if (rc < 0) {
          seterr(err, errlen, "kernel launch failed (%s%d)", "", -rc);
-         return SLG_EHIP;
+         rc_all = SLG_EHIP;
+         break;
       }
+      ran[pending] = {smpt, filt};
+      pending++;
       X->hbs_buf = !filt ? rc : (X->path == SLG_PATH_STAGED ? X->parity : -1);
       X->parity = rc;
       X->step = step;
       note_step(X, step, rc);
-      if (X->path == SLG_PATH_FUSED) {
-         HIPOK(hipEventSynchronize(ev[3]));
-         const int slot[3] = {SLG_K_CONTTY, SLG_K_REPLAY, SLG_K_STEPB};
-         for (int q = 0; q < 3; q++) {
-            float t = 0.f;
-            HIPOK(hipEventElapsedTime(&t, ev[q], ev[q + 1]));
-            ms[slot[q]] += t;
-            launches[slot[q]]++;
-         }
-         continue;
-      }
-      HIPOK(hipEventSynchronize(ev[8]));
-      const int ran[8] = {1, 1, smpt, smpt, 1, 1, filt, 1};
-      for (int q = 0; q < 8; q++) {
-         float t = 0.f;
-         if (!ran[q]) continue;
-         HIPOK(hipEventElapsedTime(&t, ev[q], ev[q + 1]));
-         ms[q] += t;
-         launches[q]++;
+      if (pending == RING && drain()) {
+         seterr(err, errlen, "%s", "event timing failed");
+         rc_all = SLG_EHIP;
+         break;
       }
    }
-   for (int k = 0; k <= SLG_NKERN; k++) (void) hipEventDestroy(ev[k]);
-   return 0;
+   if (!rc_all && drain()) {
+      seterr(err, errlen, "%s", "event timing failed");
+      rc_all = SLG_EHIP;
+   }
+   (void) hipStreamSynchronize(X->st);
+   for (auto &e : ev) (void) hipEventDestroy(e);
+   return rc_all;
 }
 
 extern "C" int slg_batch_sync(slg_batch *X, char *err, int errlen)

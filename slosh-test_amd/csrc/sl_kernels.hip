@@ -310,49 +310,82 @@ __device__ static void go_full(Replay &X)
       }
 }
 
-__device__ static void dirty(Replay &X, int i, int j)
+// The nine squares around (i,j) become candidates for the rest of the sweep
+// (their transports changed).  A square not yet reached by the sweep gets
+// HB = HSUB back, as the serial sweep would still find it.  The nine squares'
+// operands are loaded in one round trip, each row's position in the sorted
+// list is found once, and the new keys are merged into the list in one pass.
+__device__ static void dirty9(Replay &X, int i, int j)
 {
    const SlBasinDev &B = *X.B;
-   if (i < 1 || j < 1 || i > B.imxb - 1 || j > B.jmxb - 1) return;
-   const uint32_t p = IDX(B, i, j);
-   if (!(B.flags[p] & F_ACT)) return;
-   const uint32_t k = ((uint32_t) j << 16) | (uint32_t) i;
    if (X.full) return;
-   int pos = lbound(X.key, X.n, k);
-   if (pos < X.n && X.key[pos] == k) return;
-   if (k > X.frontier) X.hn[p] = X.hc[p];   // not visited yet: HB is still HSUB
-   if (X.n >= SL_EVCAP) {
+   uint32_t pp[9], fl[9];
+   real hv[9];
+#pragma unroll
+   for (int q = 0; q < 9; q++) {
+      const int ii = i - 1 + q % 3, jj = j - 1 + q / 3;
+      const bool in = ii >= 1 && jj >= 1 && ii <= B.imxb - 1 && jj <= B.jmxb - 1;
+      pp[q] = in ? IDX(B, ii, jj) : 0u;
+      fl[q] = in ? B.flags[pp[q]] : 0u;
+      hv[q] = in ? X.hc[pp[q]] : real(0.);
+   }
+   uint32_t nk[9];
+   int m = 0;
+   for (int r = 0; r < 3; r++) {
+      const uint32_t row = (uint32_t) (j - 1 + r) << 16;
+      int pos = -1;
+      for (int c = 0; c < 3; c++) {
+         const int q = r * 3 + c;
+         if (!(fl[q] & F_ACT)) continue;
+         const uint32_t k = row | (uint32_t) (i - 1 + c);
+         if (pos < 0) pos = lbound(X.key, X.n, k);
+         while (pos < X.n && X.key[pos] < k) pos++;
+         if (pos < X.n && X.key[pos] == k) continue;   // listed already
+         if (k > X.frontier) X.hn[pp[q]] = hv[q];       // not visited yet: HB is still HSUB
+         nk[m++] = k;
+      }
+   }
+   if (!m) return;
+   if (X.n + m > SL_EVCAP) {   // list full: the rest of the sweep visits every square
       go_full(X);
       return;
    }
-   for (int q = X.n; q > pos; q--) X.key[q] = X.key[q - 1];
-   X.key[pos] = k;
-   X.n++;
+   int a = X.n - 1, b = m - 1, w = X.n + m - 1;
+   while (b >= 0) {
+      if (a >= 0 && X.key[a] > nk[b]) X.key[w--] = X.key[a--];
+      else X.key[w--] = nk[b--];
+   }
+   X.n += m;
 }
 
 // One visit of square (i,j) (the DO 320 body).  Returns 1 if it dried with a
-// transport rescale (the caller then back-steps), else 0.
+// transport rescale (the caller then back-steps), else 0.  Every operand is
+// loaded before the first test (all addresses lie inside the field), so a
+// visit costs one memory round trip.
 __device__ static int visit(Replay &X, int i, int j, int &leap, int ipass)
 {
    const SlBasinDev &B = *X.B;
    const uint32_t p = IDX(B, i, j);
-   if (!(B.flags[p] & F_ACT)) return 0;
+   const uint32_t p2 = p + 1, p3 = p + B.pitch + 1, p4 = p + B.pitch;
+   real *ub = X.ub, *vb = X.vb;
+   const uint32_t fl = B.flags[p];
+   const real zb = B.zb[p], hpst = X.hc[p];
+   const real u1 = ub[p], u2 = ub[p2], u3 = ub[p3], u4 = ub[p4];
+   const real v1 = vb[p], v2 = vb[p2], v3 = vb[p3], v4 = vb[p4];
+   const real ed = B.elpdl2[i - 1], ec = B.elpcl2[i - 1], sl = B.sinl2[j - 1];
+   if (!(fl & F_ACT)) return 0;
    const uint32_t k = ((uint32_t) j << 16) | (uint32_t) i;
    if (k > X.frontier) X.frontier = k;
-   const real zb = B.zb[p];
    if (leap == 2 && i == ipass) {
       X.hn[p] = -zb;
       leap = 1;
       return 0;
    }
-   real *ub = X.ub, *vb = X.vb;
-   const uint32_t p2 = p + 1, p3 = p + B.pitch + 1, p4 = p + B.pitch;
-   const real dux = -ub[p] + ub[p2] + ub[p3] - ub[p4];
+   const real dux = -u1 + u2 + u3 - u4;
    if (zb <= real(30.) && dux == real(0.)) return 0;
-   const real hpst = X.hc[p];
-   const real z = B.elpdl2[i - 1] + (B.elpcl2[i - 1] - B.elpdl2[i - 1]) * B.sinl2[j - 1];
+   const real z = ed + (ec - ed) * sl;
    const real fct = X.x1b5 / z;
-   const real dvy = -vb[p] - vb[p2] + vb[p3] + vb[p4];
+   const real dvy = -v1 - v2 + v3 + v4;
    const real hbj = hpst - fct * (dux + dvy);
    const real chng = hbj + zb;
    if (chng < real(0.)) {
@@ -364,30 +397,58 @@ __device__ static int visit(Replay &X, int i, int j, int &leap, int ipass)
       X.hn[p] = hbj;
       return 0;
    }
-   if (X.hc[p] == -zb) return 0;
+   if (hpst == -zb) return 0;
    real denm = hbj - hpst;
    denm = copysign(fmax(real(1.E-5), fabs(denm)), denm);
    const real anum = -zb - hpst;
    real rfct = anum / denm;
    if (fabs(rfct) < real(1.E-5)) rfct = real(0.);
    const uint32_t c[4] = {p, p2, p3, p4};
+   const real uo[4] = {u1, u2, u3, u4}, vo[4] = {v1, v2, v3, v4};
    for (int q = 0; q < 4; q++) {
-      ub[c[q]] = rfct * ub[c[q]];
-      vb[c[q]] = rfct * vb[c[q]];
-      if (X.uz) X.uz[c[q]] = ub[c[q]] == real(0.);
+      const real un = rfct * uo[q];
+      ub[c[q]] = un;
+      vb[c[q]] = rfct * vo[q];
+      if (X.uz) X.uz[c[q]] = un == real(0.);
    }
-   for (int jj = j - 1; jj <= j + 1; jj++)
-      for (int ii = i - 1; ii <= i + 1; ii++) dirty(X, ii, jj);
+   dirty9(X, i, j);
    return 1;
 }
 
-__global__ void k_replay(SlBasinDev B, SlRunDev R, StepArgs A, int use_uz)
+// One wavefront per storm.  The wave sorts the storm's drying squares in LDS
+// (bitonic, row-major key order), then lane 0 walks the serial cascade with
+// the sorted list in LDS: the list lookups and insertions stay on chip and
+// only the visited squares touch HBM.
+__global__ __launch_bounds__(64) void k_replay(SlBasinDev B, SlRunDev R, StepArgs A, int use_uz)
 {
-   const int s = blockIdx.x * blockDim.x + threadIdx.x;
+   __shared__ uint32_t sk[SL_EVCAP];
+   const int s = blockIdx.x, lane = threadIdx.x;
    if (s >= R.nstorm) return;
    const int step = storm_step(R, s);
    int n = R.evcnt[s];
    if (n == 0 || !step) return;
+   if (n <= SL_EVCAP) {
+      const uint32_t *gk = R.evkey + (size_t) s * SL_EVCAP;
+      int m = 1;
+      while (m < n) m <<= 1;
+      for (int e = lane; e < m; e += 64) sk[e] = e < n ? gk[e] : 0xffffffffu;
+      __syncthreads();
+      for (int k = 2; k <= m; k <<= 1)
+         for (int h = k >> 1; h > 0; h >>= 1) {
+            for (int e = lane; e < m; e += 64) {
+               const int x = e ^ h;
+               if (x > e) {
+                  const uint32_t a = sk[e], b = sk[x];
+                  if ((a > b) == ((e & k) == 0)) {
+                     sk[e] = b;
+                     sk[x] = a;
+                  }
+               }
+            }
+            __syncthreads();
+         }
+   }
+   if (lane != 0) return;
    const size_t so = (size_t) s * R.fstride;
    Replay X;
    X.B = &B;
@@ -396,22 +457,12 @@ __global__ void k_replay(SlBasinDev B, SlRunDev R, StepArgs A, int use_uz)
    X.ub = R.ub + so;
    X.vb = R.vb + so;
    X.uz = use_uz ? R.uz + so : nullptr;
-   X.key = R.evkey + (size_t) s * SL_EVCAP;
+   X.key = sk;
    X.full = 0;
    X.x1b5 = par(R, s, step)[P_X1B5];
    if (n > SL_EVCAP) {
       n = 0;
       X.full = 1;
-   } else {   // insertion sort of the drying squares, row-major
-      for (int a = 1; a < n; a++) {
-         uint32_t v = X.key[a];
-         int b = a - 1;
-         while (b >= 0 && X.key[b] > v) {
-            X.key[b + 1] = X.key[b];
-            b--;
-         }
-         X.key[b + 1] = v;
-      }
    }
    X.n = n;
    const int jmxb1 = B.jmxb - 1;
@@ -1362,7 +1413,7 @@ extern "C" int SL_LAUNCH(const SlBasinDev *B, const SlRunDev *R, int smpt, int f
       MARK(0);
       hipLaunchKernelGGL(k_conttyA, dim3(ntx * nty * R->nstorm), dim3(TX, TY), 0, st, *B, *R, a, ntx, nty);
       MARK(1);
-      hipLaunchKernelGGL(k_replay, dim3((R->nstorm + 63) / 64), dim3(64), 0, st, *B, *R, a, 1);
+      hipLaunchKernelGGL(k_replay, dim3(R->nstorm), dim3(64), 0, st, *B, *R, a, 1);
       MARK(2);
       a.hc = h1;
       a.hn = h0;
@@ -1385,7 +1436,7 @@ extern "C" int SL_LAUNCH(const SlBasinDev *B, const SlRunDev *R, int smpt, int f
    MARK(0);
    hipLaunchKernelGGL(k_contty, grd, blk, 0, st, *B, *R, a);
    MARK(1);
-   hipLaunchKernelGGL(k_replay, dim3((R->nstorm + 63) / 64), dim3(64), 0, st, *B, *R, a, 0);
+   hipLaunchKernelGGL(k_replay, dim3(R->nstorm), dim3(64), 0, st, *B, *R, a, 0);
    MARK(2);
    // SMPT2G classification on h1
    a.hc = h1;

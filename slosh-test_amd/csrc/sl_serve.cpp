@@ -415,13 +415,22 @@ extern "C" int slg_server_pump(slg_server *S, int k, char *err, int errlen)
    }
    S->queue.swap(wait);
    for (auto &P : S->pools) {
-      int most = 0;
+      // launch over slots [0, hi): admission fills the lowest free slot, so the
+      // grid shrinks with the load and empty slots beyond hi cost nothing
+      int most = 0, hi = 0;
       for (int slot = 0; slot < P.slots; slot++)
-         if (P.run[slot] >= 0 && P.left[slot] > most) most = P.left[slot];
+         if (P.run[slot] >= 0) {
+            most = std::max(most, P.left[slot]);
+            hi = slot + 1;
+         }
       const int n = std::min(k, most);
+      SlRunDevT<float> d32 = P.d32;
+      SlRunDevT<double> d64 = P.d64;
+      d32.nstorm = hi;
+      d64.nstorm = hi;
       for (int t = 0; t < n; t++) {
-         int rc = S->prec == 64 ? dev_launch(&P.basin->i64.dev, &P.d64, 0, 0, 1, 0, SLG_PATH_FUSED, S->tile_rows, P.st, nullptr)
-                                : dev_launch(&P.basin->i32.dev, &P.d32, 0, 0, 1, 0, SLG_PATH_FUSED, S->tile_rows, P.st, nullptr);
+         int rc = S->prec == 64 ? dev_launch(&P.basin->i64.dev, &d64, 0, 0, 1, 0, SLG_PATH_FUSED, S->tile_rows, P.st, nullptr)
+                                : dev_launch(&P.basin->i32.dev, &d32, 0, 0, 1, 0, SLG_PATH_FUSED, S->tile_rows, P.st, nullptr);
          if (rc < 0) {
             seterr(err, errlen, "kernel launch failed (%s%d)", "", -rc);
             return SLG_EHIP;
