@@ -67,6 +67,20 @@ def b_alg_step(storms):
     return 32.0 + 16.0 / storms
 
 
+def measured_traffic(kernel, storms):
+    """HBM bytes per launch of `kernel` at this workload, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-calibrated); None if there
+    is no measurement for it."""
+    path = os.path.join(HERE, "profiles", "stepB_meow1024_hbm_traffic.md")
+    if kernel != "stepB" or storms != 1024 or not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        for line in fh:
+            if line.startswith("traffic_bytes_per_launch stepB"):
+                return float(line.split()[-1])
+    return None
+
+
 def log(msg):
     print("bench: " + msg, file=sys.stderr, flush=True)
 
@@ -388,7 +402,8 @@ def main():
             achieved = dom_bytes / (avg[dom] * 1e-3) / 1e9
             out.update(value=rate, ms_per_step=r["dt"] * 1e3 / a.steps)
             out["roofline"] = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                               "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                               "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                               "traffic": measured_traffic(dom, a.storms),
                                "alg_bytes_per_launch": dom_bytes, "avg_launch_ms": avg[dom]}
             out["autotune"] = r["tune"]
             out["kernel_avg_ms"] = avg
