@@ -37,6 +37,10 @@ int slg_basin_open(const char *dta_path, const char *bnt_line, slg_basin **out, 
  * reference's C side fills COMMON /LLXFLE/ before INITAL (savellx.c:757-762). */
 int slg_basin_set_ylt(slg_basin *b, const double *ylt);
 int slg_basin_dims(const slg_basin *b, int *imxb, int *jmxb, int *nbcpt);
+/* Counts of the basin's sub-grid 1-D flow sides as CRDRD2 reads them (COMMON
+ * /FLWCPT/ NSQRS, NSQRW, NSQRWC; crdrd2.f:104-357): 1-D flow sides, plus
+ * raised weirs, plus cuts.  Any pointer may be NULL. */
+int slg_basin_sides(const slg_basin *b, int *nsqrs, int *nsqrw, int *nsqrwc);
 void slg_basin_close(slg_basin *b);
 /* Static field of the basin as double [jmxb][imxb]: "ZB", "ZBM" (COMMON /DUMB5/,
  * depsfc.f:83-138) or "YLT" (/LLXFLE/). */
@@ -74,7 +78,11 @@ int slg_batch_sync(slg_batch *x, char *err, int errlen);
 #define SLG_NKERN 9
 /* Launch paths.  FUSED (default): CONTTY, the cascade replay, then one kernel
  * for SMPT2G/FLT2G/BDRYHT/HMXSV/FILTER/MOMNTM on LDS tiles of 64 x tile_rows
- * points.  STAGED: one kernel per routine (slots 0-7).  Results are identical. */
+ * points.  STAGED: one kernel per routine (slots 0-7).  Results are identical.
+ * Basins with sub-grid 1-D flow sides (1-D points, weirs, cuts: FLW1DM_C,
+ * flw1dm_block_c.f:83-435, and FLW1DM_M, flw1dm_block_m.f:1-403, plus
+ * SMPT2G's side pass, smpt2g.f:101-123) step on the staged path only: FUSED
+ * and hipGraphs return SLG_EUNSUP for them. */
 #define SLG_PATH_FUSED 0
 #define SLG_PATH_STAGED 1
 /* Choose the launch path and the fused tile height (8, 16 or 32 rows). */
@@ -90,14 +98,16 @@ int slg_batch_set_graph(slg_batch *x, int steps_per_graph);
 int slg_batch_autotune(slg_batch *x, int steps_per_trial, int *rows_out, int *path_out, double *ms_out,
                        char *err, int errlen);
 /* slg_batch_step with HIP events around every kernel on the batch's stream
- * (synchronises after each step): ms[k] += kernel k time, launches[k] += 1. */
+ * (the host waits once every 64 steps): ms[k] += kernel k time, launches[k] += 1. */
 int slg_batch_step_timed(slg_batch *x, int n, double *ms, int *launches, char *err, int errlen);
 int slg_batch_current_step(const slg_batch *x);
 /* Copy one run's field to the host as double [jmxb][imxb]: "HB", "UB", "VB" or
  * "HMX" (COMMON /DUMB7/ and /DUMB10/, parm.for:39-45); "HB0" the initial water
  * (INTLHT, intlht.f:53-185); "HBS" HB of the last step after HMXSV and before
  * FILTER, the state TMSTPC2 copies to the C driver's HHB (tmstpc2.f:70-85;
- * kept on the staged path, and on either path when the step did not filter). */
+ * kept on the staged path, and on either path when the step did not filter);
+ * "FLWSQR" the transport across each sub-grid 1-D flow side (COMMON /ACT1D/,
+ * parm.for:58; nsqrwc values into dst). */
 int slg_batch_get(slg_batch *x, int storm, const char *field, double *dst, char *err, int errlen);
 /* Values TMSTPM returns to the C driver after step n (tmstpm.f:59-70; 0 =
  * after INITAL): ITIME, MHALT, DELT, storm lat, storm lon (XY2LLX), DELP,

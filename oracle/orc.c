@@ -22,6 +22,7 @@ typedef REAL R;
 /* square root rounded to REAL before it enters further arithmetic: sqrt of a
  * float computed in double and rounded once is the correctly rounded float */
 #define SQRT(x) ((R) sqrt((double) (x)))
+#define FABS(x) ((R) fabs((double) (x)))
 #define F2(a, i, j) (a)[(size_t) ((j) - 1) * imxb + ((i) - 1)]
 #define HB(i, j) F2(st->hb, i, j)
 #define UB(i, j) F2(st->ub, i, j)
@@ -57,6 +58,25 @@ typedef REAL R;
 static const int IP[5] = {0, 0, 1, 1, 0}, JP[5] = {0, 0, 0, 1, 1};
 static const int IH[5] = {0, -1, 0, -1, 0}, JH[5] = {0, -1, -1, 0, 0};
 static const int IIH[5] = {0, 0, 1, 0, -1}, JJH[5] = {0, -1, 0, 1, 0};
+static const int IHH[5] = {0, -1, 1, 0, 0}, JHH[5] = {0, 0, 0, -1, 1};
+/* 1-D flow: sign of the flow across each side (flw1dm_block_c.f:239 SIGA) */
+static const REAL SIGA[5] = {0, -1, 1, -1, 1};
+#define ISQR(l) b->isqr[(l) - 1]
+#define JSQR(l) b->jsqr[(l) - 1]
+#define ISIDE(l) b->iside[(l) - 1]
+#define KTREE(l) b->ktree[(l) - 1]
+#define HWEIR(l) b->hweir[(l) - 1]
+#define ZBMIN(l) b->zbmin[(l) - 1]
+#define BANK1(l) b->bank1[(l) - 1]
+#define BANK2(l) b->bank2[(l) - 1]
+#define DELCUT(l) b->delcut[(l) - 1]
+#define COS1D(l) b->cos1d[(l) - 1]
+#define SIN1D(l) b->sin1d[(l) - 1]
+#define CUTL(l) b->cutl[(l) - 1]
+#define CUTLI(l) b->cutli[(l) - 1]
+#define CUTLE(l) b->cutle[(l) - 1]
+#define FLWSQR(l) st->flwsqr[(l) - 1]
+#define F1DACT(l) st->f1dact[(l) - 1]
 
 #define M2G_ 30000
 #define MCT_ 8000
@@ -294,6 +314,29 @@ int orc_smpt2g(const orc_basin *b, const orc_storm *s, orc_state *st, orc_diag *
             if (HB(i, j) + ZB(i, j) != 0) HSUB(i, j) = 888;
          }
       }
+   /* sides of 1-D flow not carrying water both ways at the last FLW1DM_C, with
+    * either square at most 1 ft above the side's lower barrier: both squares
+    * become special corner points (smpt2g.f:101-123) */
+   if (b->nsqrw != 0) {
+      int l;
+      for (l = 1; l <= b->nsqrw; l++)
+         if (F1DACT(l) != 'T') {
+            i = ISQR(l);
+            if (i > 1 && i < imxb1) {
+               j = JSQR(l);
+               if (j > 1 && j < jmxb1 && HSUB(i, j) != 999) {
+                  k = ISIDE(l);
+                  ii = i + IHH[k];
+                  jj = j + JHH[k];
+                  if (HSUB(ii, jj) != 999)
+                     if (HSUB(i, j) <= (ZBMIN(l) + RC(1.)) || HSUB(ii, jj) <= (ZBMIN(l) + RC(1.))) {
+                        HSUB(i, j) = 888;
+                        HSUB(ii, jj) = 888;
+                     }
+               }
+            }
+         }
+   }
    m = 0;
    mm = 0;
    for (j = 1; j <= jmxb1; j++)
@@ -928,6 +971,294 @@ void orc_momntm(const orc_basin *b, const orc_storm *s, orc_state *st)
    (void) hpd;
 }
 
+/* ---------------------------------------------------------------- FLW1DM_C */
+/* Continuity across the open sides of 1-D flow squares, one side at a time in
+ * list order (flw1dm_block_c.f:229-430): the side's transport FLWSQR fills one
+ * square and drains the other, shaped by banks (DELCUT) or cut widths (CUTL,
+ * CUTLI, CUTLE); a square that would go below its bottom is left dry and the
+ * transport is cut back.  HSUB of the two squares accumulates the water the
+ * bank/cut shaping removed (read by FLW1DM_M as the friction head). */
+void orc_flw1dm_c(const orc_basin *b, const orc_storm *s, orc_state *st)
+{
+   const int imxb = b->imxb, jmxb = b->jmxb;
+   int l;
+   for (l = 1; l <= b->nsqrwc; l++) {
+      const int i = ISQR(l), j = JSQR(l);
+      if (i > 0 && j > 0 && i < imxb && j < jmxb) {
+         const int k = ISIDE(l), ii = i + IHH[k], jj = j + JHH[k];
+         HSUB(i, j) = 0;
+         if (ii > 0 && jj > 0 && ii < imxb && jj < jmxb) HSUB(ii, jj) = 0;
+         F1DACT(l) = 'F';
+      }
+   }
+   for (l = 1; l <= b->nsqrwc; l++) {
+      int i, j, k, ii, jj, kcut, ll = 0, leap;
+      R z, z1, chnghi, chnghe, chng, cstrt, ht, bk, cutw = 0, delb, hbj, hbjj, depli, deple;
+      if (FLWSQR(l) == 0) continue;
+      i = ISQR(l);
+      j = JSQR(l);
+      if (!(i > 0 && j > 0 && i < imxb && j < jmxb)) continue;
+      k = ISIDE(l);
+      ii = i + IHH[k];
+      jj = j + JHH[k];
+      z = ELPDL2(i) + (ELPCL2(i) - ELPDL2(i)) * SINL2(j);
+      z1 = ELPDL2(ii) + (ELPCL2(ii) - ELPDL2(ii)) * SINL2(jj);
+      chnghi = -(SIGA[k] * s->x1b5 * FLWSQR(l) / z);
+      chnghe = -(chnghi * z / z1);
+      if (l > b->nsqrw) {
+         kcut = 2;
+         ll = l - b->nsqrw;
+         if (CUTLI(ll) >= RC(1.) && CUTLE(ll) >= RC(1.)) {
+            chnghi = chnghi * CUTL(ll) / CUTLI(ll);
+            chnghe = chnghe * CUTL(ll) / CUTLE(ll);
+            goto L470;
+         }
+      } else {
+         kcut = 1;
+         if (DELCUT(l) >= RC(1.)) goto L470;
+      }
+      /* outflow from the interior square (leap 1) or from the exterior (leap 2) */
+      if (chnghi > 0) {
+         leap = 2;
+         goto L194;
+      }
+      leap = 1;
+   L192:
+      chng = chnghi;
+      cstrt = chnghi;
+      ht = HB(i, j);
+      bk = BANK1(l);
+      if (kcut == 2) cutw = CUTLI(ll);
+      if (leap != 2) goto L200;
+      goto L300;
+   L194:
+      chng = chnghe;
+      cstrt = chnghe;
+      ht = HB(ii, jj);
+      bk = BANK2(l);
+      if (kcut == 2) cutw = CUTLE(ll);
+      if (leap != 2) goto L300;
+   L200: /* water leaving a square */
+      if (ht <= bk) {
+         if (kcut == 2) chng = chng * CUTL(ll) / cutw;
+      } else {
+         delb = ht - bk;
+         if (kcut != 2) {
+            if (-chng * DELCUT(l) <= delb) chng = chng * DELCUT(l);
+            else chng = chng - delb + delb / DELCUT(l);
+         } else {
+            if (-chng * CUTL(ll) <= delb) chng = chng * CUTL(ll);
+            else chng = (chng * CUTL(ll) + delb) / cutw - delb;
+         }
+      }
+      if (leap != 2) {
+         chnghi = chng;
+         HSUB(i, j) = HSUB(i, j) + FABS(cstrt - chng);
+         goto L194;
+      }
+      chnghe = chng;
+      HSUB(ii, jj) = HSUB(ii, jj) + FABS(cstrt - chng);
+      goto L192;
+   L300: /* water entering a square */
+      if (ht >= bk) {
+         if (kcut != 2) chng = chng * DELCUT(l);
+         else chng = chng * CUTL(ll);
+      } else {
+         delb = bk - ht;
+         if (kcut != 2) {
+            if (!(chng <= delb)) chng = delb + (chng - delb) * DELCUT(l);
+         } else {
+            if (chng * CUTL(ll) <= delb * cutw) chng = chng * CUTL(ll) / cutw;
+            else chng = chng * CUTL(ll) - delb * cutw + delb;
+         }
+      }
+      if (leap != 2) {
+         chnghe = chng;
+         HSUB(ii, jj) = HSUB(ii, jj) + FABS(cstrt - chng);
+      } else {
+         chnghi = chng;
+         HSUB(i, j) = HSUB(i, j) + FABS(cstrt - chng);
+      }
+   L470:
+      hbj = HB(i, j) + chnghi;
+      hbjj = HB(ii, jj) + chnghe;
+      depli = hbj + ZB(i, j);
+      deple = hbjj + ZB(ii, jj);
+      if (depli < 0) {
+         const R cii = chnghi - depli, sube = -(cii * z / z1);
+         HB(i, j) = -ZB(i, j);
+         HB(ii, jj) = AMAX1(HB(ii, jj) - chnghe + sube, -ZB(ii, jj));
+         FLWSQR(l) = FLWSQR(l) * cii / chnghi;
+      } else if (deple < 0) {
+         const R cee = chnghe - deple, subi = -(cee * z1 / z);
+         HB(i, j) = AMAX1(HB(i, j) - chnghi + subi, -ZB(i, j));
+         HB(ii, jj) = -ZB(ii, jj);
+         FLWSQR(l) = FLWSQR(l) * cee / chnghe;
+      } else {
+         HB(i, j) = hbj;
+         HB(ii, jj) = hbjj;
+         F1DACT(l) = 'T';
+      }
+   }
+}
+
+/* ---------------------------------------------------------------- FLW1DM_M */
+static R dim_(R a, R c)
+{
+   return a > c ? a - c : RC(0.);
+}
+
+/* Momentum across the open sides (flw1dm_block_m.f:153-397): the head between
+ * the two squares, clamped between the weir height HWEIR and the lower barrier
+ * ZBMIN, drives FLWSQR against bottom friction; below the barrier the lake
+ * wind stress at the side's force point (FRCPNT, rotated into the side's flow
+ * direction) adds on; the friction head HSUB left by FLW1DM_C damps the
+ * result by a 20-step fixed point. */
+void orc_flw1dm_m(const orc_basin *b, const orc_storm *s, orc_state *st)
+{
+   const int imxb = b->imxb, jmxb = b->jmxb;
+   const R slpa = TAB(ai2, 1, 2) - TAB(ai2, 1, 1);
+   const int ncatg = 1, npls = 1, sfl = !strncmp(b->sta, "SOUTH FL", 8);
+   int l;
+   for (l = 1; l <= b->nsqrwc; l++) {
+      const int i = ISQR(l), j = JSQR(l);
+      int k, ii, jj, iik, jump, id, kk;
+      R hint, hext, zl, hovt, hint1, hext1, head, dph1d[3], dph, biobr, hcoff, flwt, frc12t;
+      R aintrp, flwcof, flw, fwd;
+      if (!(i > 0 && j > 0 && i < imxb && j < jmxb)) continue;
+      iik = iik_of(b, i, j);
+      hint = HB(i, j);
+      k = ISIDE(l);
+      ii = i + IHH[k];
+      jj = j + JHH[k];
+      if (ii <= 0 || jj <= 0 || ii >= imxb + 1 || jj >= jmxb + 1) {
+         FLWSQR(l) = 0;
+         HSUB(i, j) = 0;
+         continue;
+      }
+      hext = HB(ii, jj);
+      zl = ZBMIN(l);
+      if (AMIN1(hint, hext) >= zl || (hint == -ZB(i, j) && hext == -ZB(ii, jj))) {
+         FLWSQR(l) = 0;
+         HSUB(i, j) = 0;
+         HSUB(ii, jj) = 0;
+         continue;
+      }
+      hovt = HWEIR(l);
+      if (AMAX1(hint, hext) <= hovt) {
+         FLWSQR(l) = 0;
+         HSUB(i, j) = 0;
+         HSUB(ii, jj) = 0;
+         continue;
+      }
+      hint1 = HB(i, j);
+      hext1 = HB(ii, jj);
+      hint = AMIN1(AMAX1(hint1, hovt), zl);
+      hext = AMIN1(AMAX1(hext1, hovt), zl);
+      jump = 2;
+      if (AMAX1(hint, hext) >= zl) jump = 1;
+      head = hext - hint;
+      head = SIGA[k] * head;
+      dph1d[1] = hint + ZB(i, j);
+      dph1d[2] = hext + ZB(ii, jj);
+      dph = RC(.5) * (dph1d[1] + dph1d[2]);
+      if (dph < RC(0.00000001)) {
+         FLWSQR(l) = 0;
+         HSUB(i, j) = 0;
+         HSUB(ii, jj) = 0;
+         continue;
+      }
+      id = (int) AMIN1(RC(599.), dph);
+      if (id < 1) id = 1;
+      biobr = TAB(bi2, iik, id) / TAB(br2, iik, id);
+      if (b->nsqrw != b->nsqrs && l > b->nsqrs && l <= b->nsqrw) biobr = 0;
+      hcoff = TAB(br2, iik, id) + TAB(bi2, iik, id) * biobr;
+      flwt = -(s->x1b1 * hcoff * dph * head);
+      if (jump != 1) {
+         R cshltr, xr0, yr0, elpcz, elpdz, xr, yr, fxb, fyb, fxbp, fybp, frc1, frc2;
+         int iii = i, jjj = j;
+         if (KTREE(l) == 'T') {
+            const R q = dph / RC(50.);
+            cshltr = q * q;
+         } else {
+            const R cshint = RC(1.) - dim_(RC(1.), dim_(hint1, hovt));
+            const R cshext = RC(1.) - dim_(RC(1.), dim_(hext1, hovt));
+            const R csh2i = RC(1.) - dim_(RC(1.), dim_(zl, hint1));
+            const R csh2e = RC(1.) - dim_(RC(1.), dim_(zl, hext1));
+            cshltr = RC(.5) * AMIN1(cshint + cshext, csh2i + csh2e);
+         }
+         if (k != 3 && k != 4) {   /* force point on the side (polar grid (x,y)) */
+            if (k == 2) iii = i + 1;
+            xr0 = COSL(jjj);
+            yr0 = SINL(jjj);
+            elpcz = ELPCT(iii);
+            elpdz = ELPDT(iii);
+         } else {
+            if (k == 4) jjj = j + 1;
+            xr0 = COST(jjj);
+            yr0 = SINT(jjj);
+            elpcz = ELPCL(iii);
+            elpdz = ELPDL(iii);
+         }
+         xr = xr0 * elpcz;
+         yr = yr0 * elpdz;
+         frcpnt(b, s, xr, yr, &fxb, &fyb, ncatg, &cshltr, npls, i, j, dph, id);
+         xr = xr0 * elpdz;
+         yr = yr0 * elpcz;
+         fxbp = xr * fxb + yr * fyb;
+         fybp = xr * fyb - yr * fxb;
+         if (COS1D(l) != RC(1.)) {
+            fxb = COS1D(l) * fxbp - SIN1D(l) * fybp;
+            fyb = SIN1D(l) * fxbp + COS1D(l) * fybp;
+         } else {
+            fxb = fxbp;
+            fyb = fybp;
+         }
+         frc1 = RC(.75) * fxb;
+         frc2 = RC(.75) * fyb;
+         if (k == 3 || k == 4) {
+            const R t = frc2;
+            frc2 = -frc1;
+            frc1 = t;
+         }
+         frc12t = s->delt * (frc1 + biobr * frc2);
+      } else {
+         frc12t = 0;
+      }
+      aintrp = 0;
+      for (kk = 1; kk <= 2; kk++) {
+         const int idd = (int) AMIN1(RC(299.), dph1d[kk]);
+         R aitr;
+         if (idd <= 1) {
+            if (b->iver < 201903 && sfl) aitr = TAB(ai2, 1, 1) + slpa * (dph1d[kk] - RC(1.));
+            else aitr = TAB(ai2, iik, 1) + slpa * (dph1d[kk] - RC(1.));
+         } else {
+            const R slpai = TAB(ai2, iik, idd + 1) - TAB(ai2, iik, idd), dphi = (R) idd;
+            aitr = TAB(ai2, iik, idd) + slpai * (dph1d[kk] - dphi);
+         }
+         aintrp = aintrp + aitr;
+      }
+      aintrp = RC(.5) * aintrp;
+      flwcof = s->x1b4 * (aintrp - TAB(ar2, iik, id) * biobr) + RC(1.);
+      flw = flwt + flwcof * FLWSQR(l) + frc12t;
+      fwd = HSUB(ii, jj) - HSUB(i, j);
+      if (fwd != 0) {
+         R fd2 = -(SIGA[k] * fwd * s->x1b2 / (dph * dph)), z, z1, xy, flw1;
+         int itr;
+         fd2 = FABS(fd2);
+         z = ELPDL2(i) + (ELPCL2(i) - ELPDL2(i)) * SINL2(j);
+         z1 = ELPDL2(ii) + (ELPCL2(ii) - ELPDL2(ii)) * SINL2(jj);
+         xy = RC(.5) * (RC(1.) / z + RC(1.) / z1);
+         fd2 = fd2 * xy;
+         flw1 = flw;
+         for (itr = 1; itr <= 20; itr++) flw1 = flw / (RC(1.) + fd2 * FABS(flw1));
+         FLWSQR(l) = flw1;
+      } else {
+         FLWSQR(l) = flw;
+      }
+   }
+}
+
 /* -------------------------------------------------------------------- step */
 int orc_step(const orc_basin *b, const orc_storm *s, orc_state *st, orc_diag *d, REAL *hbs)
 {
@@ -945,7 +1276,9 @@ int orc_step(const orc_basin *b, const orc_storm *s, orc_state *st, orc_diag *d,
    if (ismpt2g != 0 && s->itime > 5 && incsm != 1)
       if (orc_smpt2g(b, s, st, d)) return -1;
    orc_bdryht(b, s, st);
+   if (b->nsqrwc != 0 && b->nof1d != '+') orc_flw1dm_c(b, s, st);
    /* TMSTPC2 */
+   if (b->nsqrwc != 0 && b->nof1d != '+') orc_flw1dm_m(b, s, st);
    orc_hmxsv(b, s, st);
    /* HB after HMXSV: what TMSTPC2 copies to the C driver's HHB (tmstpc2.f:70-85) */
    if (hbs) memcpy(hbs, st->hb, sizeof(REAL) * (size_t) b->imxb * b->jmxb);

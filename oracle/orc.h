@@ -32,6 +32,13 @@ typedef struct {
    REAL corl[90], gamfp[4];
    int nbcpt;
    const int *ish, *jsh;
+   /* sub-grid 1-D flow edges, L = 1..nsqrwc at index L-1 (1-D sides, then
+    * weirs from nsqrs+1, then cuts from nsqrw+1; cut data at L-nsqrw-1) */
+   int nsqrs, nsqrw, nsqrwc, ncut;
+   char nof1d;
+   const int *isqr, *jsqr, *iside;
+   const char *ktree;
+   const REAL *hweir, *zbmin, *bank1, *bank2, *delcut, *cos1d, *sin1d, *cutl, *cutli, *cutle;
 } orc_basin;
 
 /* Storm values the field routines read during one step: everything as it
@@ -39,7 +46,7 @@ typedef struct {
 typedef struct {
    int itime;                 /* after TTIMER */
    REAL bt;
-   REAL delt, x1b4, x1b5, x1b6;
+   REAL delt, x1b1, x1b2, x1b4, x1b5, x1b6;
    REAL c1, c2, ax, ay;
    REAL x12[51];              /* 1-based */
    REAL w1218[2];
@@ -50,6 +57,8 @@ typedef struct {
 typedef struct {
    REAL *hb, *ub, *vb, *hmx;  /* imxb x jmxb */
    REAL *hsub;                /* scratch, imxb x jmxb, persists across steps */
+   REAL *flwsqr;              /* transport across each 1-D flow side (nsqrwc) */
+   char *f1dact;              /* 'T' where both squares of a side held water after FLW1DM_C */
 } orc_state;
 
 typedef struct {
@@ -68,4 +77,6 @@ void orc_hmxsv(const orc_basin *b, const orc_storm *s, orc_state *st);
 void orc_filter(const orc_basin *b, const orc_storm *s, orc_state *st);
 void orc_xfilter(const orc_basin *b, const orc_storm *s, orc_state *st, int iversion);
 void orc_momntm(const orc_basin *b, const orc_storm *s, orc_state *st);
+void orc_flw1dm_c(const orc_basin *b, const orc_storm *s, orc_state *st);
+void orc_flw1dm_m(const orc_basin *b, const orc_storm *s, orc_state *st);
 #endif

@@ -22,6 +22,9 @@ typedef struct {
    char *itree, *kskp;
    int *imann, *is, *ie, *ms, *me, *ish, *jsh;
    R *elp[6], *trig[5], *tab[6];
+   int *isqr, *jsqr, *iside;
+   char *ktree, *f1dact;
+   R *edge[10], *flwsqr;
    R S[801], C[801], P[801], DELP[801], SW[801], CW[801];
    R seadtm;
    int ic19;
@@ -87,6 +90,23 @@ static int *toI(recd *x)
    return o;
 }
 
+/* an optional record as REAL (zero-filled to at least n entries) */
+static R *toR_opt(recd *x, long n)
+{
+   R *o = calloc((size_t) (n > 0 ? n : 1), sizeof(R));
+   long k;
+   if (x)
+      for (k = 0; k < x->n && k < n; k++) o[k] = (R) ((double *) x->data)[k];
+   return o;
+}
+
+static int *toI_opt(recd *x, long n)
+{
+   int *o = calloc((size_t) (n > 0 ? n : 1), sizeof(int));
+   if (x) memcpy(o, x->data, sizeof(int) * (size_t) (x->n < n ? x->n : n));
+   return o;
+}
+
 static void load_tables(ctx_t *c, recd *r, int n)
 {
    const char *nm[6] = {"S", "C", "P", "DELP", "SW", "CW"};
@@ -147,6 +167,43 @@ void *orc_open(const char *path)
       c->b.dollar[1] = fl[1];
       c->b.ebsn = fl[2];
       c->b.ebsn1 = fl[3];
+      c->b.nof1d = fl[5];
+   }
+   {
+      /* sub-grid 1-D flow edges (absent from files written before they existed) */
+      const char *en[10] = {"HWEIR", "ZBMIN", "BANK1", "BANK2", "DELCUT", "COS1D", "SIN1D",
+                            "CUTL", "CUTLI", "CUTLE"};
+      long nl = ints[18], nc = ints[20];
+      recd *kt = find(r, n, "KTREE");
+      c->b.nsqrs = ints[16];
+      c->b.nsqrw = ints[17];
+      c->b.nsqrwc = ints[18];
+      c->b.ncut = ints[20];
+      c->isqr = toI_opt(find(r, n, "EDGE_I"), nl);
+      c->jsqr = toI_opt(find(r, n, "EDGE_J"), nl);
+      c->iside = toI_opt(find(r, n, "EDGE_SIDE"), nl);
+      c->ktree = calloc((size_t) nl + 1, 1);
+      if (kt) memcpy(c->ktree, kt->data, (size_t) (kt->n < nl ? kt->n : nl));
+      for (k = 0; k < 10; k++) c->edge[k] = toR_opt(find(r, n, en[k]), k < 7 ? nl : nc);
+      c->flwsqr = calloc((size_t) nl + 1, sizeof(R));
+      c->f1dact = malloc((size_t) nl + 1);
+      memset(c->f1dact, '0', (size_t) nl + 1);
+      c->b.isqr = c->isqr;
+      c->b.jsqr = c->jsqr;
+      c->b.iside = c->iside;
+      c->b.ktree = c->ktree;
+      c->b.hweir = c->edge[0];
+      c->b.zbmin = c->edge[1];
+      c->b.bank1 = c->edge[2];
+      c->b.bank2 = c->edge[3];
+      c->b.delcut = c->edge[4];
+      c->b.cos1d = c->edge[5];
+      c->b.sin1d = c->edge[6];
+      c->b.cutl = c->edge[7];
+      c->b.cutli = c->edge[8];
+      c->b.cutle = c->edge[9];
+      c->st.flwsqr = c->flwsqr;
+      c->st.f1dact = c->f1dact;
    }
    memcpy(c->b.sta, find(r, n, "STA")->data, 16);
    c->b.hter = (R) ((double *) find(r, n, "HTER")->data)[0];
@@ -237,6 +294,8 @@ static void make_storm(ctx_t *c, const double *row, orc_storm *s)
    s->ay = (R) row[6];
    s->w1218[0] = (R) row[15];
    s->w1218[1] = (R) row[16];
+   s->x1b1 = (R) row[31];
+   s->x1b2 = (R) row[32];
    s->x1b4 = (R) row[24];
    s->x1b5 = (R) row[25];
    s->x1b6 = (R) row[26];
@@ -322,6 +381,10 @@ int orc_get(void *vc, const char *name, double *out)
    ctx_t *c = vc;
    R *f = field(c, name);
    size_t k, np = (size_t) c->b.imxb * c->b.jmxb;
+   if (!strcmp(name, "FLWSQR")) {   /* one value per 1-D flow side */
+      for (k = 0; k < (size_t) c->b.nsqrwc; k++) out[k] = (double) c->flwsqr[k];
+      return c->b.nsqrwc;
+   }
    if (!f) return -1;
    for (k = 0; k < np; k++) out[k] = (double) f[k];
    return 0;
@@ -348,5 +411,7 @@ void orc_close(void *vc)
    for (k = 0; k < 6; k++) free(c->elp[k]);
    for (k = 0; k < 5; k++) free(c->trig[k]);
    for (k = 0; k < 6; k++) free(c->tab[k]);
+   for (k = 0; k < 10; k++) free(c->edge[k]);
+   free(c->isqr); free(c->jsqr); free(c->iside); free(c->ktree); free(c->f1dact); free(c->flwsqr);
    free(c);
 }

@@ -73,6 +73,15 @@ extern struct { R XMOUTH, YMOUTH; } orign_;
 extern struct { R ZLATO; } zlato_;
 extern struct { int IRANK, ISTAR, ISTOP, JSTAR, JSTOP, IHALO; } frank_;
 extern struct { short MM2G, MCMX, I2G[30000], J2G[30000]; } sm2g_;
+#define L_ 41000
+#define NCT_ 3000
+extern struct { R HWEIR[L_], ZBMIN[L_]; int ISQR[L_], JSQR[L_], ISIDE[L_]; } dumb18_;
+extern struct { char KTREE[L_]; } ktree_;
+extern struct { R FLWSQR[L_]; char F1DACT[L_]; } act1d_;
+extern struct { R COS1D[L_], SIN1D[L_]; } cs1da_;
+extern struct { R CUTL[NCT_], CUTLI[NCT_], CUTLE[NCT_]; } dumcut_;
+extern struct { R DELCUT[L_], BANK[2][L_]; } bank2_;
+extern struct { char NOFLD, NOF1D; } opts_;
 
 extern void initalcommon_(void);
 extern void inital_(int *mhalt, int *imxb, int *jmxb, R zb[N_][M_], int *hr,
@@ -125,6 +134,7 @@ static void dump_state(const char *path)
    rec2(fp, "UB", 'd', dumb7_.UB);
    rec2(fp, "VB", 'd', dumb7_.VB);
    rec2(fp, "HMX", 'd', dumb10_.HMX);
+   if (flwcpt_.NSQRWC > 0) rec(fp, "FLWSQR", 'd', flwcpt_.NSQRWC, act1d_.FLWSQR);
    fclose(fp);
 }
 
@@ -143,8 +153,8 @@ static void dump_storm(FILE *fp)
    v[n++] = strmsb_.PTENCY;     v[n++] = strmsb_.RTENCY;     v[n++] = ffth_.MHALT;
    v[n++] = dumb4_.X1B[3];      v[n++] = dumb4_.X1B[4];      v[n++] = dumb4_.X1B[5];
    v[n++] = frth_.IC19;         v[n++] = datum_.SEADTM;      v[n++] = stime2_.JHR;
-   v[n++] = zlato_.ZLATO;
-   for (k = 0; k < 9; k++) v[n++] = 0;
+   v[n++] = zlato_.ZLATO;       v[n++] = dumb4_.X1B[0];      v[n++] = dumb4_.X1B[1];
+   for (k = 0; k < 7; k++) v[n++] = 0;
    fwrite(v, 8, 40, fp);
    fwrite(frst_.X12, 8, 50, fp);
 }
@@ -239,8 +249,27 @@ static void dump_init(const char *path)
       char flags[8];
       flags[0] = gprt1_.DOLLAR[0]; flags[1] = gprt1_.DOLLAR[1];
       flags[2] = gprt1_.EBSN; flags[3] = gprt2_.EBSN1; flags[4] = gprt2_.EBSN2;
-      flags[5] = flags[6] = flags[7] = ' ';
+      flags[5] = opts_.NOF1D;
+      flags[6] = flags[7] = ' ';
       rec(fp, "FLAGS", 'c', 8, flags);
+   }
+   {
+      /* sub-grid 1-D flow edges (COMMON /DUMB18/ /KTREE/ /CS1DA/ /BANK2/ /DUMCUT/) */
+      int nl = flwcpt_.NSQRWC, nc = flwcpt_.NCUT;
+      rec(fp, "EDGE_I", 'i', nl, dumb18_.ISQR);
+      rec(fp, "EDGE_J", 'i', nl, dumb18_.JSQR);
+      rec(fp, "EDGE_SIDE", 'i', nl, dumb18_.ISIDE);
+      rec(fp, "KTREE", 'c', nl, ktree_.KTREE);
+      rec(fp, "HWEIR", 'd', nl, dumb18_.HWEIR);
+      rec(fp, "ZBMIN", 'd', nl, dumb18_.ZBMIN);
+      rec(fp, "BANK1", 'd', nl, bank2_.BANK[0]);
+      rec(fp, "BANK2", 'd', nl, bank2_.BANK[1]);
+      rec(fp, "DELCUT", 'd', nl, bank2_.DELCUT);
+      rec(fp, "COS1D", 'd', nl, cs1da_.COS1D);
+      rec(fp, "SIN1D", 'd', nl, cs1da_.SIN1D);
+      rec(fp, "CUTL", 'd', nc, dumcut_.CUTL);
+      rec(fp, "CUTLI", 'd', nc, dumcut_.CUTLI);
+      rec(fp, "CUTLE", 'd', nc, dumcut_.CUTLE);
    }
    rec(fp, "STA", 'c', 16, gprt_.STA);
    rec(fp, "STRMSB", 'd', 8, &strmsb_);

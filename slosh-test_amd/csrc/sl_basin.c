@@ -247,10 +247,35 @@ static int read_header(frec *r, sl_basin *b, char err[256])
    return 0;
 }
 
-/* CRDRD2, crdrd2.f:65-489 (features beyond barriers/trees/depths rejected) */
+/* 1-D flow edge arrays at the reference's capacity (parm.for:16 L_, NCT_),
+ * with CRDRD2's defaults (crdrd2.f:105-110: no banks, no weir, DELCUT 1) */
+static void edges_alloc(sl_basin *b)
+{
+   int l;
+   b->isqr = xcalloc(SL_L1D, sizeof(int));
+   b->jsqr = xcalloc(SL_L1D, sizeof(int));
+   b->iside = xcalloc(SL_L1D, sizeof(int));
+   b->iang = xcalloc(SL_L1D, sizeof(int));
+   b->ktree = xcalloc(SL_L1D, 1);
+   memset(b->ktree, '0', SL_L1D);
+   b->hweir = xcalloc(SL_L1D, sizeof(double));
+   b->zbmin = xcalloc(SL_L1D, sizeof(double));
+   b->bank1 = xcalloc(SL_L1D, sizeof(double));
+   b->bank2 = xcalloc(SL_L1D, sizeof(double));
+   b->delcut = xcalloc(SL_L1D, sizeof(double));
+   b->cos1d = xcalloc(SL_L1D, sizeof(double));
+   b->sin1d = xcalloc(SL_L1D, sizeof(double));
+   b->cutl = xcalloc(SL_NCUT, sizeof(double));
+   b->cutli = xcalloc(SL_NCUT, sizeof(double));
+   b->cutle = xcalloc(SL_NCUT, sizeof(double));
+   for (l = 0; l < SL_L1D; l++) b->delcut[l] = 1.;
+}
+
+/* CRDRD2, crdrd2.f:65-489 (channels and Manning-n data rejected) */
 static int read_body(frec *r, sl_basin *b, char err[256])
 {
-   int imxb = b->imxb, jmxb = b->jmxb, i, j, k, n;
+   int imxb = b->imxb, jmxb = b->jmxb, i, j, k;
+   long n;
    long inwp, nsqrs, nnsqr, nweirs, ncut, mgpt, nsheet;
    char ac, bc, ix60;
    size_t np = (size_t) imxb * jmxb;
@@ -287,28 +312,81 @@ static int read_body(frec *r, sl_basin *b, char err[256])
          }
       }
    }
-   /* 1-D flow points */
+   /* 1-D flow points (crdrd2.f:104-182): I, J, KTREE, side, angle; six per
+    * record, I3 fields in 1992 basins and I4 after */
    fr_next(r);
    nsqrs = fr_int(r, 3);
    ac = fr_char(r);
    bc = fr_char(r);
    nnsqr = fr_int(r, 3);
    nsqrs = nnsqr * 1000 + nsqrs + thousands(ac, "XYZWVUTSRQPONMLKJIHGFEDCBAzyxwvutsrqpon");
-   if (nsqrs != 0) {
-      snprintf(err, 256, "1-D flow points (%ld) are not supported by this build", nsqrs);
+   if (nsqrs > SL_L1D) {
+      snprintf(err, 256, "number of 1-D flow points (%ld) exceeds %d", nsqrs, SL_L1D);
       return -1;
    }
-   (void) bc;
+   edges_alloc(b);
+   if (nsqrs != 0) {
+      int w = (b->iver == 199201) ? 3 : 4;
+      for (n = 0; n < nsqrs; n++) {
+         if (n % 6 == 0) fr_next(r);
+         b->isqr[n] = (int) fr_int(r, w);
+         b->jsqr[n] = (int) fr_int(r, w);
+         b->ktree[n] = fr_char(r);
+         b->iside[n] = (int) fr_int(r, 1);
+         b->iang[n] = (int) fr_int(r, 3);
+      }
+      /* banks (crdrd2.f:185-255): matched to the 1-D points in list order */
+      if (bc == 'X') {
+         long nbnks;
+         int lll = 0;
+         fr_next(r);
+         nbnks = fr_int(r, 3);
+         ac = fr_char(r);
+         nbnks += thousands(ac, "XYZWVUTSRQP");
+         for (k = 0; k < nbnks; k++) {
+            int ib, jb, kb, l;
+            double bki, bke, wchn;
+            if (k % 4 == 0) fr_next(r);
+            ib = (int) fr_int(r, w);
+            jb = (int) fr_int(r, w);
+            (void) fr_char(r);
+            kb = (int) fr_int(r, 1);
+            bki = fr_real(r, 3, 0);
+            bke = fr_real(r, 3, 0);
+            wchn = fr_real(r, 4, 2);
+            for (l = lll; l < nsqrs; l++)
+               if (b->isqr[l] == ib && b->jsqr[l] == jb && b->iside[l] == kb) {
+                  lll = l + 1;
+                  b->bank1[l] = bki;
+                  b->bank2[l] = bke;
+                  b->delcut[l] = wchn;
+                  break;
+               }
+            /* a bank with no matching 1-D point is reported and skipped by the reference */
+         }
+      }
+   }
+   /* raised weirs (crdrd2.f:263-276): FORMAT (5(2I3,A1,I1,F5.1)) */
    fr_next(r);
    nweirs = fr_int(r, 3);
    ac = fr_char(r);
    nweirs += thousands(ac, "XYZ");
-   if (nweirs != 0) {
-      snprintf(err, 256, "raised weirs (%ld) are not supported by this build", nweirs);
+   if (nsqrs + nweirs > SL_L1D) {
+      snprintf(err, 256, "number of 1-D flow points and weirs (%ld) exceeds %d", nsqrs + nweirs, SL_L1D);
       return -1;
    }
-   b->nsqrs = 0;
-   b->nsqrw = 0;
+   for (n = 0; n < nweirs; n++) {
+      long m = nsqrs + n;
+      if (n % 5 == 0) fr_next(r);
+      b->isqr[m] = (int) fr_int(r, 3);
+      b->jsqr[m] = (int) fr_int(r, 3);
+      b->ktree[m] = fr_char(r);
+      b->iside[m] = (int) fr_int(r, 1);
+      b->hweir[m] = fr_real(r, 5, 1);
+      b->iang[m] = 0;
+   }
+   b->nsqrs = (int) nsqrs;
+   b->nsqrw = (int) (nsqrs + nweirs);
    /* history stations: names (6A10 -> 2 records), IPN, JPN (15I3) */
    fr_next(r);
    fr_next(r);
@@ -323,17 +401,74 @@ static int read_body(frec *r, sl_basin *b, char err[256])
       snprintf(err, 256, "channels (%d) are not supported by this build", b->npss);
       return -1;
    }
-   /* cuts */
+   /* cuts (crdrd2.f:301-357), one per record */
    fr_next(r);
    ncut = fr_int(r, 3);
    ac = fr_char(r);
    ncut += thousands(ac, "XY");
-   if (ncut != 0) {
-      snprintf(err, 256, "cuts (%ld) are not supported by this build", ncut);
+   if (b->nsqrw + ncut > SL_L1D || ncut > SL_NCUT) {
+      snprintf(err, 256, "number of 1-D flow points and cuts (%ld) exceeds the reference's limits",
+               b->nsqrw + ncut);
       return -1;
    }
-   b->ncut = 0;
-   b->nsqrwc = 0;
+   for (n = 0; n < ncut; n++) {
+      long m = b->nsqrw + n;
+      int w = (b->iver == 199201) ? 3 : 4;
+      fr_next(r);
+      b->isqr[m] = (int) fr_int(r, w);
+      b->jsqr[m] = (int) fr_int(r, w);
+      b->ktree[m] = fr_char(r);
+      b->iside[m] = (int) fr_int(r, 1);
+      b->bank1[m] = fr_real(r, 4, 0);
+      b->bank2[m] = fr_real(r, 4, 0);
+      b->cutl[n] = fr_real(r, 5, 2);
+      b->cutli[n] = fr_real(r, 5, 2);
+      b->cutle[n] = fr_real(r, 5, 2);
+      b->iang[m] = (int) fr_int(r, 3);
+      b->hweir[m] = fr_real(r, 5, 1);
+      if (b->cutl[n] == 0.) b->cutl[n] = 1.;
+      if (b->cutli[n] == 0.) b->cutli[n] = 1.;
+      if (b->cutle[n] == 0.) b->cutle[n] = 1.;
+   }
+   b->ncut = (int) ncut;
+   b->nsqrwc = (int) (b->nsqrw + ncut);
+   /* rotation of the wind stress into each edge's flow direction (crdrd2.f:362-371) */
+   for (n = 0; n < b->nsqrwc; n++) {
+      if (b->iang[n] != 0) {
+         double ang = b->iang[n] * RADF;
+         b->cos1d[n] = cos(ang);
+         b->sin1d[n] = sin(ang);
+      } else {
+         b->cos1d[n] = 1.;
+         b->sin1d[n] = 0.;
+      }
+   }
+   /* the sides must join two squares inside the basin (the reference does not
+    * check the exterior square and then reads outside its arrays) */
+   for (n = 0; n < b->nsqrwc; n++) {
+      const int i = b->isqr[n], j = b->jsqr[n], s = b->iside[n];
+      int ii, jj;
+      if (s < 1 || s > 4) {
+         snprintf(err, 256, "1-D flow edge %ld has side %d (must be 1-4)", (long) n + 1, s);
+         return -1;
+      }
+      ii = i + (s == 1 ? -1 : s == 2 ? 1 : 0);
+      jj = j + (s == 3 ? -1 : s == 4 ? 1 : 0);
+      if (i < 1 || j < 1 || i > imxb - 1 || j > jmxb - 1 || ii < 1 || jj < 1 || ii > imxb - 1 ||
+          jj > jmxb - 1) {
+         snprintf(err, 256, "1-D flow edge %ld (%d,%d side %d) touches the basin's outer row or column, "
+                  "which this build does not support", (long) n + 1, i, j, s);
+         return -1;
+      }
+      if (n >= b->nsqrs && n < b->nsqrw) {   /* a weir's two end points become momentum barriers */
+         const int i1 = i + (s == 2), i2 = i + (s != 1), j1 = j + (s != 3), j2 = j + (s == 4);
+         if (i1 < 2 || i2 > imxb - 1 || j1 < 2 || j2 < 2 || j1 > jmxb - 1 || j2 > jmxb - 1) {
+            snprintf(err, 256, "weir %ld (%d,%d side %d) ends on the basin's outer row or column, "
+                     "which this build does not support", (long) n + 1, i, j, s);
+            return -1;
+         }
+      }
+   }
    /* trees (60A1 per J, format reversion every 60 characters) */
    fr_next(r);
    mgpt = fr_int(r, 3);
@@ -404,7 +539,78 @@ static int read_body(frec *r, sl_basin *b, char err[256])
    return 0;
 }
 
-/* DEPSFC, depsfc.f:62-451, without the 1-D flow / weir branch */
+/* DEPSFC's 1-D flow branch, depsfc.f:239-372: bank heights from the barrier
+ * heights where the dta gives none, weir heights, weir sides made momentum
+ * barriers (ITREE '4', ZBM = HTER+1), the weir/sill height HWEIR of every side
+ * and ZBMIN, the lower barrier at the ends of the side.  Every edge joins two
+ * interior squares (checked when the edges are read), so the reference's
+ * out-of-range branches do not arise. */
+static void depsfc_edges(sl_basin *b)
+{
+   static const int IHH[5] = {0, -1, 1, 0, 0}, JHH[5] = {0, 0, 0, -1, 1};
+   static const int IZ1[5] = {0, 0, 1, 0, 0}, IZ2[5] = {0, 0, 1, 1, 1};
+   static const int JZ1[5] = {0, 1, 1, 0, 1}, JZ2[5] = {0, 0, 0, 0, 1};
+   int l, i, j;
+#define ZB(i, j) b->zb[PIX(b, i, j)]
+#define ZBM(i, j) b->zbm[PIX(b, i, j)]
+#define IT(i, j) b->itree[PIX(b, i, j)]
+#define MIN4(a, c, d, e) fmin(fmin(fmin(a, c), d), e)
+   for (l = 0; l < b->nsqrwc; l++) {
+      if (b->bank1[l] == 0.) {
+         i = b->isqr[l];
+         j = b->jsqr[l];
+         b->bank1[l] = MIN4(ZBM(i, j), ZBM(i, j + 1), ZBM(i + 1, j), ZBM(i + 1, j + 1));
+      }
+      if (b->bank2[l] != 0.) continue;
+      i = b->isqr[l] + IHH[b->iside[l]];
+      j = b->jsqr[l] + JHH[b->iside[l]];
+      b->bank2[l] = MIN4(ZBM(i, j), ZBM(i, j + 1), ZBM(i + 1, j), ZBM(i + 1, j + 1));
+   }
+   if (b->nsqrw != b->nsqrs) {
+      for (l = b->nsqrs; l < b->nsqrw; l++) {
+         const int k = b->iside[l];
+         i = b->isqr[l];
+         j = b->jsqr[l];
+         if (b->hweir[l] == 0.) b->hweir[l] = fmin(ZBM(i + IZ1[k], j + JZ1[k]), ZBM(i + IZ2[k], j + JZ2[k]));
+      }
+      for (l = b->nsqrs; l < b->nsqrw; l++) {
+         const int k = b->iside[l];
+         i = b->isqr[l];
+         j = b->jsqr[l];
+         IT(i + IZ1[k], j + JZ1[k]) = 'Q';
+         IT(i + IZ2[k], j + JZ2[k]) = 'Q';
+      }
+   }
+   for (j = 2; j <= b->jmxb1; j++)
+      for (i = 2; i <= b->imxb1; i++)
+         if (IT(i, j) == 'Q') {
+            ZBM(i, j) = b->hter + 1;
+            IT(i, j) = '4';
+         }
+   for (l = 0; l < b->nsqrwc; l++) {
+      const int k = b->iside[l];
+      i = b->isqr[l];
+      j = b->jsqr[l];
+      if (b->hweir[l] == 0.) b->hweir[l] = fmax(-ZB(i, j), -ZB(i + IHH[k], j + JHH[k]));
+      {
+         /* IZBX is an INTEGER (implicit typing): the barrier height is truncated */
+         const int i1 = i + IZ1[k], j1 = j + JZ1[k], i2 = i + IZ2[k], j2 = j + JZ2[k];
+         const int in1 = i1 > 0 && j1 > 0 && i1 < b->imxb && j1 < b->jmxb;
+         const int in2 = i2 > 0 && j2 > 0 && i2 < b->imxb && j2 < b->jmxb;
+         double z = 0.;
+         if (in1 && in2) z = fmin(ZBM(i1, j1), ZBM(i2, j2));
+         else if (in1) z = ZBM(i1, j1);
+         else if (in2) z = ZBM(i2, j2);
+         b->zbmin[l] = (double) (int) z;
+      }
+   }
+#undef ZB
+#undef ZBM
+#undef IT
+#undef MIN4
+}
+
+/* DEPSFC, depsfc.f:62-451 */
 static int depsfc(sl_basin *b, char err[256])
 {
    int imxb = b->imxb, jmxb = b->jmxb, imxb1 = b->imxb1, jmxb1 = b->jmxb1;
@@ -511,6 +717,7 @@ static int depsfc(sl_basin *b, char err[256])
       b->mf[j - 1] = i;                      /* exhausted: I keeps its last value */
    }
    b->mf[0] = isl ? b->mf[jmxb - 1] : b->mf[1];
+   if (b->nsqrwc != 0) depsfc_edges(b);
    /* static-height boundary squares (depsfc.f:376-390), I outer, J inner */
    n = 0;
    b->ish = xcalloc(SL_NBCPTS, sizeof(int));
@@ -741,7 +948,9 @@ void sl_basin_free(sl_basin *b)
    void *p[] = {b->zb, b->zbm, b->ylt, b->itree, b->kskp, b->imann, b->is, b->ie, b->ms,
                 b->me, b->mf, b->sint, b->cost, b->sinl, b->cosl, b->sinl2, b->sint2,
                 b->elpct, b->elpdt, b->elpcl, b->elpdl, b->elpcl2, b->elpdl2, b->ish,
-                b->jsh, b->tidesh, b->ar2, b->ai2, b->br2, b->bi2, b->cr2, b->ci2};
+                b->jsh, b->tidesh, b->ar2, b->ai2, b->br2, b->bi2, b->cr2, b->ci2,
+                b->isqr, b->jsqr, b->iside, b->iang, b->ktree, b->hweir, b->zbmin, b->bank1,
+                b->bank2, b->delcut, b->cos1d, b->sin1d, b->cutl, b->cutli, b->cutle};
    size_t k;
    for (k = 0; k < sizeof p / sizeof p[0]; k++) free(p[k]);
    memset(b, 0, sizeof *b);

@@ -7,6 +7,7 @@
 // STMVAL leaves for the next step, and every radial table FORCE1 produces) is
 // computed on the host up front and uploaded once; stepping then needs no host
 // round trip.
+#include <map>
 #include "sl_int.h"
 
 // --------------------------------------------------------------------------
@@ -154,6 +155,49 @@ static void basin_images(slg_basin *B)
             }
          }
    }
+   // sub-grid 1-D flow sides: indices, friction class, KTREE 'T', and the
+   // compact index of each joined square (HSUB of those squares lives in a
+   // per-storm array of nsq values); every side joins two interior squares
+   {
+      static const int IHH[5] = {0, -1, 1, 0, 0}, JHH[5] = {0, 0, 0, -1, 1};
+      const int nl = b->nsqrwc, nc = b->ncut;
+      const bool sfl = !strncmp(b->sta, "SOUTH FL", 8);
+      std::map<long, int> sq;
+      auto qidx = [&](int i, int j) {
+         auto it = sq.emplace((long) j * 100000 + i, (int) sq.size()).first;
+         return it->second;
+      };
+      I.eint.assign((size_t) std::max(nl, 1) * SL_EI, 0);
+      I.ereal.assign((size_t) std::max(nl, 1) * 8, R(0));
+      I.ecut.assign((size_t) std::max(nc, 1) * 4, R(0));
+      for (int l = 0; l < nl; l++) {
+         int *e = &I.eint[(size_t) l * SL_EI];
+         const int i = b->isqr[l], j = b->jsqr[l], k = b->iside[l];
+         e[E1_I] = i;
+         e[E1_J] = j;
+         e[E1_K] = k;
+         e[E1_II] = i + IHH[k];
+         e[E1_JJ] = j + JHH[k];
+         e[E1_IIK] = (b->zb[PIX(b, i, j)] >= 0 || !sfl) ? 1 : 245;   // flw1dm_block_m.f:173-222
+         e[E1_T] = b->ktree[l] == 'T';
+         e[E1_QI] = qidx(i, j);
+         e[E1_QE] = qidx(e[E1_II], e[E1_JJ]);
+         R *r = &I.ereal[(size_t) l * 8];
+         r[E2_HWEIR] = (R) b->hweir[l];
+         r[E2_ZBMIN] = (R) b->zbmin[l];
+         r[E2_BANK1] = (R) b->bank1[l];
+         r[E2_BANK2] = (R) b->bank2[l];
+         r[E2_DELCUT] = (R) b->delcut[l];
+         r[E2_COS] = (R) b->cos1d[l];
+         r[E2_SIN] = (R) b->sin1d[l];
+      }
+      for (int n = 0; n < nc; n++) {
+         I.ecut[(size_t) n * 4 + 0] = (R) b->cutl[n];
+         I.ecut[(size_t) n * 4 + 1] = (R) b->cutli[n];
+         I.ecut[(size_t) n * 4 + 2] = (R) b->cutle[n];
+      }
+      I.nsq = (int) sq.size();
+   }
    I.built = true;
 }
 
@@ -180,6 +224,9 @@ static int basin_upload(slg_basin *B, char *err, int errlen)
    for (int k = 0; k < 5; k++) add((size_t) NJ * rs);
    add(I.tab8.size() * rs);
    add(4 * np * rs);   // g4
+   add(I.eint.size() * 4);
+   add(I.ereal.size() * rs);
+   add(I.ecut.size() * rs);
    add(np * 8);        // ZB in double
    char *d = nullptr;
    HIPOK(hipMalloc((void **) &d, tot));
@@ -226,6 +273,15 @@ static int basin_upload(slg_basin *B, char *err, int errlen)
    for (int k = 0; k < 5; k++) *vj[k] = (const R *) up(I.vj[k].data(), (size_t) NJ * rs);
    D.tab8 = (const R *) up(I.tab8.data(), I.tab8.size() * rs);
    D.g4 = (const R *) up(I.g4.data(), 4 * np * rs);
+   D.e_int = (const int *) up(I.eint.data(), I.eint.size() * 4);
+   D.e_real = (const R *) up(I.ereal.data(), I.ereal.size() * rs);
+   D.e_cut = (const R *) up(I.ecut.data(), I.ecut.size() * rs);
+   D.nsqrs = b->nsqrs;
+   D.nsqrw = b->nsqrw;
+   D.nsqrwc = b->nsqrwc;
+   D.nsq = I.nsq;
+   D.f1d_on = b->nsqrwc != 0 && b->nof1d != '+';
+   D.sfl = !strncmp(b->sta, "SOUTH FL", 8);
    {
       std::vector<double> zd(np, 0.);
       for (int j = 1; j <= NJ; j++)
@@ -304,6 +360,14 @@ extern "C" int slg_basin_set_ylt(slg_basin *B, const double *ylt)
    return 0;
 }
 
+extern "C" int slg_basin_sides(const slg_basin *B, int *nsqrs, int *nsqrw, int *nsqrwc)
+{
+   if (nsqrs) *nsqrs = B->h.nsqrs;
+   if (nsqrw) *nsqrw = B->h.nsqrw;
+   if (nsqrwc) *nsqrwc = B->h.nsqrwc;
+   return 0;
+}
+
 extern "C" int slg_basin_dims(const slg_basin *B, int *imxb, int *jmxb, int *nbcpt)
 {
    *imxb = B->h.imxb;
@@ -321,6 +385,12 @@ extern "C" void slg_basin_close(slg_basin *B)
    delete B;
 }
 
+// basins with sub-grid 1-D flow sides (their passes need the staged path)
+static bool has_sides(const slg_basin *B)
+{
+   return B->h.nsqrw > 0 || (B->h.nsqrwc > 0 && B->h.nof1d != '+');
+}
+
 // --------------------------------------------------------------------------
 // storm schedule: ref-format rows (oracle/ref_driver.c dump_storm layout)
 
@@ -333,7 +403,7 @@ static void storm_row(const sl_storm *s, const sl_basin *b, double *v)
    v[15] = s->w1218[0]; v[16] = s->w1218[1]; v[17] = s->wmax[0]; v[18] = s->wmax[1];
    v[19] = s->itmadv; v[20] = s->nhrad;   v[21] = s->ptency; v[22] = s->rtency; v[23] = s->mhalt;
    v[24] = s->x1b[3]; v[25] = s->x1b[4];  v[26] = s->x1b[5]; v[27] = s->ic19;  v[28] = s->seadtm;
-   v[29] = s->jhr;    v[30] = s->zlato;
+   v[29] = s->jhr;    v[30] = s->zlato;  v[31] = s->x1b[0]; v[32] = s->x1b[1];
    for (int k = 1; k <= 50; k++) v[40 + k - 1] = s->x12[k];
    (void) b;
 }
@@ -442,6 +512,8 @@ static void step_params(const slg_basin *B, const double *r, int tv, R *P)
    P[P_W0] = (R) r[15];
    P[P_W1] = (R) r[16];
    P[P_SEA] = (R) r[28];
+   P[P_X1B1] = (R) r[31];
+   P[P_X1B2] = (R) r[32];
    int fl = 0;
    if (B->ismpt2g && itime > 5 && B->incsm != 1) fl |= SF_SMPT;
    if ((itime + 1) % B->incsm == 0 || B->h.ebsn1 == '+') fl |= SF_FILT;
@@ -504,7 +576,9 @@ extern "C" int slg_batch_open_prec(slg_basin *B, int nstorm, const char *const *
    for (int s = 0; s < nstorm; s++) sl_initial_water(&B->h, &X->sc[s].s0, &X->hb0[(size_t) s * np]);
    X->step = 0;
    X->parity = 0;
-   X->path = SLG_PATH_FUSED;
+   // basins with 1-D flow sides step on the staged path (their side passes sit
+   // between routines the fused kernel runs in one pass)
+   X->path = has_sides(B) ? SLG_PATH_STAGED : SLG_PATH_FUSED;
    X->tile_rows = 16;
    X->hbs_buf = -1;
    X->last_par.assign(nstorm, 0);
@@ -530,6 +604,7 @@ extern "C" int slg_batch_set_path(slg_batch *X, int path, int tile_rows)
    if ((path != SLG_PATH_FUSED && path != SLG_PATH_STAGED) ||
        (tile_rows != 8 && tile_rows != 16 && tile_rows != 32))
       return SLG_EINPUT;
+   if (path == SLG_PATH_FUSED && has_sides(X->b)) return SLG_EUNSUP;
    X->path = path;
    X->tile_rows = tile_rows;
    drop_graph(X);
@@ -556,7 +631,7 @@ static int batch_upload(slg_batch *X, char *err, int errlen)
    const size_t fstride = (size_t) P * NJ, np = fstride * S;
    size_t ntab = 0;
    for (auto &sc : X->sc) ntab += sc.tab_step.size();
-   size_t off[20], tot = 0;
+   size_t off[24], tot = 0;
    int n = 0;
    auto add = [&](size_t bytes) {
       off[n++] = tot;
@@ -573,6 +648,11 @@ static int batch_upload(slg_batch *X, char *err, int errlen)
    add((size_t) S * 4);                                                // tbase
    add((size_t) S * 4);                                                // nsteps
    add((size_t) S * 4);                                                // cur
+   const BasinImg<R> &I = img(B, R());
+   const size_t nl = (size_t) std::max(b->nsqrwc, 1), nq = (size_t) std::max(I.nsq, 1);
+   add((size_t) S * nl * rs);                                          // flwsqr
+   add((size_t) S * nl);                                               // f1dact
+   add((size_t) S * nq * rs);                                          // hsq
    char *d = nullptr;
    HIPOK(hipMalloc((void **) &d, tot));
    X->dmem = d;
@@ -594,6 +674,9 @@ static int batch_upload(slg_batch *X, char *err, int errlen)
    D.evcnt = (int *) (d + off[10]);
    D.stat = (int *) (d + off[11]);
    D.maxsteps = X->maxsteps;
+   D.flwsqr = (R *) (d + off[17]);    // FLWSQR = 0 (intlht.f:190-191), F1DACT not 'T'
+   D.f1dact = (uint8_t *) (d + off[18]);
+   D.hsq = (R *) (d + off[19]);
    HIPOK(hipMemset(d, 0, tot));
    // initial water (INTLHT) per storm: HB, HMX=HB, UB=VB=0
    std::vector<R> f(fstride, R(0));
@@ -675,6 +758,7 @@ static int build_graph(slg_batch *X, char *err, int errlen)
 extern "C" int slg_batch_set_graph(slg_batch *X, int steps_per_graph)
 {
    if (steps_per_graph < 0) return SLG_EINPUT;
+   if (steps_per_graph > 0 && has_sides(X->b)) return SLG_EUNSUP;   // graphs replay fused steps
    drop_graph(X);
    X->graph_steps = steps_per_graph;
    return 0;
@@ -911,6 +995,18 @@ static int batch_get(slg_batch *X, int storm, const char *name, double *dst, cha
    const sl_basin *b = &X->b->h;
    const SlRunDevT<R> &D = run(X, R());
    const R *src;
+   if (!strcmp(name, "FLWSQR")) {   // transport across each 1-D flow side (nsqrwc values)
+      if (!X->dmem) {
+         seterr(err, errlen, "%s", "batch not uploaded");
+         return SLG_EINPUT;
+      }
+      const int nl = b->nsqrwc;
+      std::vector<R> f((size_t) std::max(nl, 1));
+      HIPOK(hipStreamSynchronize(X->st));
+      if (nl) HIPOK(hipMemcpy(f.data(), D.flwsqr + (size_t) storm * nl, (size_t) nl * sizeof(R), hipMemcpyDeviceToHost));
+      for (int l = 0; l < nl; l++) dst[l] = (double) f[l];
+      return 0;
+   }
    if (!strcmp(name, "HB")) src = X->last_par[storm] ? D.hb1 : D.hb0;
    else if (!strcmp(name, "HBS")) {
       if (X->hbs_buf < 0 || X->sc[storm].nsteps < X->step) {
@@ -1056,7 +1152,7 @@ extern "C" int slg_batch_autotune(slg_batch *X, int steps_per_trial, int *rows_o
 {
    const int cand[4][2] = {{SLG_PATH_FUSED, 8}, {SLG_PATH_FUSED, 16}, {SLG_PATH_FUSED, 32}, {SLG_PATH_STAGED, 16}};
    double best = 1e300;
-   int bi = 1;
+   int bi = has_sides(X->b) ? 3 : 1;
    const int keep_graph = X->graph_steps;
    X->graph_steps = 0;
    drop_graph(X);
@@ -1065,6 +1161,10 @@ extern "C" int slg_batch_autotune(slg_batch *X, int steps_per_trial, int *rows_o
    HIPOK(hipEventCreate(&e1));
    for (int c = 0; c < 4; c++) {
       if (X->step + steps_per_trial > X->maxsteps) break;
+      if (cand[c][0] == SLG_PATH_FUSED && has_sides(X->b)) {
+         if (ms_out) ms_out[c] = 0.;
+         continue;
+      }
       X->path = cand[c][0];
       X->tile_rows = cand[c][1];
       HIPOK(hipEventRecord(e0, X->st));
@@ -1196,8 +1296,23 @@ extern "C" int slg_export_init(const slg_batch *X, int storm, const char *path)
    rec(fp, "INTS", 'i', 21, ints);
    double reals[5] = {s->bt, s->dlthr, s->dlth2, s->treal, s->zlato};
    rec(fp, "REALS", 'd', 5, reals);
-   char flags[8] = {b->dollar[0], b->dollar[1], b->ebsn, b->ebsn1, b->ebsn2, ' ', ' ', ' '};
+   char flags[8] = {b->dollar[0], b->dollar[1], b->ebsn, b->ebsn1, b->ebsn2, b->nof1d, ' ', ' '};
    rec(fp, "FLAGS", 'c', 8, flags);
+   const int nl = b->nsqrwc, nc = b->ncut;
+   rec(fp, "EDGE_I", 'i', nl, b->isqr);
+   rec(fp, "EDGE_J", 'i', nl, b->jsqr);
+   rec(fp, "EDGE_SIDE", 'i', nl, b->iside);
+   rec(fp, "KTREE", 'c', nl, b->ktree);
+   rec(fp, "HWEIR", 'd', nl, b->hweir);
+   rec(fp, "ZBMIN", 'd', nl, b->zbmin);
+   rec(fp, "BANK1", 'd', nl, b->bank1);
+   rec(fp, "BANK2", 'd', nl, b->bank2);
+   rec(fp, "DELCUT", 'd', nl, b->delcut);
+   rec(fp, "COS1D", 'd', nl, b->cos1d);
+   rec(fp, "SIN1D", 'd', nl, b->sin1d);
+   rec(fp, "CUTL", 'd', nc, b->cutl);
+   rec(fp, "CUTLI", 'd', nc, b->cutli);
+   rec(fp, "CUTLE", 'd', nc, b->cutle);
    rec(fp, "STA", 'c', 16, b->sta);
    double strmsb[8] = {s->c1, s->c2, s->c21, s->c22, s->ax, s->ay, s->ptency, s->rtency};
    rec(fp, "STRMSB", 'd', 8, strmsb);

@@ -10,7 +10,7 @@
 #include <stdint.h>
 
 #define SL_NTABV 801          /* radial tables, index 1..800 used (Fortran order) */
-#define SL_NPAR 24
+#define SL_NPAR 26
 #define SL_EVCAP 4096         /* per-storm CONTTY event / dirty-square capacity */
 #define T_AI2_OFF 147000      /* offset of AI2 in the [6][245][600] table block */
 #define SL_M2G 30000          /* corner-smoothing list limits, smpt2g.f:17 / parm.for:26 */
@@ -47,13 +47,24 @@ struct SlBasinDevT {
    const R *sint, *cost, *sinl, *cosl, *sinl2;                        /* [jmxb] */
    const R *tab8;             /* [245][600][8]: AI2 SLPAI AR2 BR2 BI2 CR2 CI2 0 per (iik, id) */
    const R *g4;               /* [4][pitch*jmxb]: metric weight GAMF of neighbour k (IIH/JJH order) */
+   /* sub-grid 1-D flow sides, L = 1..nsqrwc at index L-1 (1-D sides, weirs
+    * from nsqrs+1, cuts from nsqrw+1); f1d_on: FLW1DM runs (sides and NOF1D
+    * not '+'); sfl: STA starts with "SOUTH FL" */
+   int nsqrs, nsqrw, nsqrwc, nsq, f1d_on, sfl;
+   const int *e_int;          /* [nsqrwc][SL_EI]: see E1_* */
+   const R *e_real;           /* [nsqrwc][8]: HWEIR ZBMIN BANK(,1) BANK(,2) DELCUT COS1D SIN1D 0 */
+   const R *e_cut;            /* [ncut][4]: CUTL CUTLI CUTLE 0 */
 };
+#define SL_EI 10
+enum { E1_I, E1_J, E1_K, E1_II, E1_JJ, E1_IIK, E1_T, E1_QI, E1_QE, E1_PAD };   /* E1_T: KTREE 'T' */
+enum { E2_HWEIR, E2_ZBMIN, E2_BANK1, E2_BANK2, E2_DELCUT, E2_COS, E2_SIN };
 
 /* per-step storm values (what the field routines read during that step);
  * P_ITIME, P_TVER and P_FLAGS hold small integers as values */
 enum {
    P_BT, P_DELT, P_X1B4, P_X1B5, P_X1B6, P_C1, P_C2, P_AX, P_AY, P_X4, P_X9, P_X10,
-   P_X15, P_X20, P_X21, P_X23, P_X24, P_W0, P_W1, P_SEA, P_ITIME, P_TVER, P_FLAGS, P_PAD
+   P_X15, P_X20, P_X21, P_X23, P_X24, P_W0, P_W1, P_SEA, P_ITIME, P_TVER, P_FLAGS, P_X1B1, P_X1B2,
+   P_PAD
 };
 #define SF_SMPT (1 << 0)      /* SMPT2G/FLT2G this step */
 #define SF_FILT (1 << 1)      /* FILTER/XFILTER this step */
@@ -76,6 +87,9 @@ struct SlRunDevT {
    const int *nsteps;         /* [nstorm] steps of each run (0: empty slot) */
    int *cur;                  /* [nstorm] steps completed; the kernels of a step work on
                                  step cur+1 of each storm and skip finished storms */
+   R *flwsqr;                 /* [nstorm][nsqrwc]: transport across each 1-D flow side */
+   uint8_t *f1dact;           /* [nstorm][nsqrwc]: 1 where FLW1DM_C left water on both sides */
+   R *hsq;                    /* [nstorm][nsq]: HSUB of the 1-D flow squares (FLW1DM_C -> _M) */
 };
 
 #endif

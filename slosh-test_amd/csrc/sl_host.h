@@ -21,6 +21,8 @@ extern "C" {
 #define SL_NT 999          /* track hours, inputn2.f:10 NT_ */
 #define SL_NBCPTS 12000    /* static-height boundary squares, bdryht.f:59 */
 #define SL_NTAB 800        /* radial tables at 1-mile spacing, force1.f */
+#define SL_L1D 41000       /* 1-D flow points + weirs + cuts, parm.for:16 L_ */
+#define SL_NCUT 3000       /* cuts, parm.for:16 NCT_ */
 
 typedef struct sl_basin {
    int imxb, jmxb, imxb1, jmxb1, imxb2, jmxb2;
@@ -50,8 +52,15 @@ typedef struct sl_basin {
    double delt, x1b[10], corl[90], gama, gama1, gamfp[4];
    /* bottom-stress tables (BTMSTR), [N-1][II-1], N=1..600, II=1..245 */
    double *ar2, *ai2, *br2, *bi2, *cr2, *ci2;
-   /* sub-grid features: counted, must be zero for this build */
+   /* sub-grid 1-D flow edges (CRDRD2 / DEPSFC), L = 1..nsqrwc at index L-1:
+    * 1..nsqrs 1-D flow sides, nsqrs+1..nsqrw raised weirs, nsqrw+1..nsqrwc
+    * cuts (cut data cutl/cutli/cutle at index L-nsqrw-1).  Channels (npss)
+    * must be zero for this build. */
    int nsqrs, nsqrw, nsqrwc, npss, ncut, nodry;
+   int *isqr, *jsqr, *iside, *iang;
+   char *ktree;
+   double *hweir, *zbmin, *bank1, *bank2, *delcut, *cos1d, *sin1d;
+   double *cutl, *cutli, *cutle;
    int ipn[10], jpn[10];
    int have_ylt;
 } sl_basin;

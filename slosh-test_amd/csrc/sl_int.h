@@ -54,6 +54,9 @@ template <class R>
 struct BasinImg {
    std::vector<R> zb, zbm, zcor, vi[6], vj[5], tab, slpai, tab8, g4;
    std::vector<uint32_t> flags;
+   std::vector<int> eint;          // 1-D flow sides [nsqrwc][SL_EI]
+   std::vector<R> ereal, ecut;     // [nsqrwc][8], [ncut][4]
+   int nsq = 0;                    // distinct squares the sides join
    bool built = false, on_dev = false;
    void *dmem = nullptr;
    SlBasinDevT<R> dev;

@@ -1110,6 +1110,291 @@ __global__ void __launch_bounds__(TX * TY) k_smooth(SlBasinDev B, SlRunDev R, St
                             ZG{B.zb, B.zbm, B.pitch});
 }
 
+// ------------------------------------------------------- sub-grid 1-D flow
+// The sides of 1-D flow squares are visited in list order with each storm's
+// state changing from one side to the next (a square may border several
+// sides), so one lane per storm walks them, as the reference loops do.
+__constant__ int cIHH[5] = {0, -1, 1, 0, 0}, cJHH[5] = {0, 0, 0, -1, 1};
+__constant__ real cSIGA[5] = {0, -1, 1, -1, 1};
+
+__device__ __forceinline__ real zmet(const SlBasinDev &B, int i, int j)
+{
+   return B.elpdl2[i - 1] + (B.elpcl2[i - 1] - B.elpdl2[i - 1]) * B.sinl2[j - 1];
+}
+
+// SMPT2G's last pass over the 1-D sides (smpt2g.f:101-123), after the height
+// classes: a side that did not carry water both ways at the last FLW1DM_C,
+// with both squares classed and either at most 1 ft above the side's lower
+// barrier, makes both squares special corner points (class 2, HSUB 888).
+// hc = HB after CONTTY (the HSUB value of class-1 points).
+__global__ void k_f1d_mark(SlBasinDev B, SlRunDev R, StepArgs A)
+{
+   const int s = blockIdx.x * blockDim.x + threadIdx.x;
+   if (s >= R.nstorm) return;
+   const int step = storm_step(R, s);
+   if (!step || !((int) par(R, s, step)[P_FLAGS] & SF_SMPT)) return;
+   const size_t so = (size_t) s * R.fstride;
+   const real *hs = A.hc + so;
+   uint8_t *cl = R.hclass + so;
+   const uint8_t *act = R.f1dact + (size_t) s * B.nsqrwc;
+   int moved = 0;
+   for (int l = 0; l < B.nsqrw; l++) {
+      if (act[l]) continue;
+      const int *e = B.e_int + (size_t) l * SL_EI;
+      const int i = e[E1_I], j = e[E1_J];
+      if (!(i > 1 && i < B.imxb - 1 && j > 1 && j < B.jmxb - 1)) continue;
+      const uint32_t p = IDX(B, i, j), q = IDX(B, e[E1_II], e[E1_JJ]);
+      const uint8_t ci = cl[p];
+      if (!ci) continue;
+      const uint8_t ce = cl[q];
+      if (!ce) continue;
+      const real lim = B.e_real[(size_t) l * 8 + E2_ZBMIN] + real(1.);
+      const real vi = (ci == 2) ? real(888.) : hs[p], ve = (ce == 2) ? real(888.) : hs[q];
+      if (vi <= lim || ve <= lim) {
+         moved += (ci == 1) + (ce == 1);
+         cl[p] = 2;
+         cl[q] = 2;
+      }
+   }
+   if (moved) {   // the points change lists (the corner-point limits count them)
+      R.stat[s * 8 + 2] -= moved;
+      R.stat[s * 8 + 3] += moved;
+   }
+}
+
+__device__ __forceinline__ real dim_(real a, real c)
+{
+   return a > c ? a - c : real(0.);
+}
+
+// FLW1DM_C (flw1dm_block_c.f:229-430) then FLW1DM_M (flw1dm_block_m.f:153-397)
+// on HB after BDRYHT (in place).  HSUB of the joined squares lives in hsq.
+__global__ void k_flw1d(SlBasinDev B, SlRunDev R, StepArgs A)
+{
+   const int s = blockIdx.x * blockDim.x + threadIdx.x;
+   if (s >= R.nstorm) return;
+   const int step = storm_step(R, s);
+   if (!step) return;
+   const size_t so = (size_t) s * R.fstride;
+   real *hb = A.hn + so, *flw = R.flwsqr + (size_t) s * B.nsqrwc, *hq = R.hsq + (size_t) s * B.nsq;
+   uint8_t *act = R.f1dact + (size_t) s * B.nsqrwc;
+   const real *P = par(R, s, step);
+   const real *T = tabv(R, s, P);
+   const int nl = B.nsqrwc;
+   // ---- continuity
+   for (int l = 0; l < nl; l++) {
+      const int *e = B.e_int + (size_t) l * SL_EI;
+      hq[e[E1_QI]] = real(0.);
+      hq[e[E1_QE]] = real(0.);
+      act[l] = 0;
+   }
+   for (int l = 0; l < nl; l++) {
+      const real f0 = flw[l];
+      if (f0 == real(0.)) continue;
+      const int *e = B.e_int + (size_t) l * SL_EI;
+      const real *E = B.e_real + (size_t) l * 8;
+      const int i = e[E1_I], j = e[E1_J], k = e[E1_K], ii = e[E1_II], jj = e[E1_JJ];
+      const uint32_t p = IDX(B, i, j), q = IDX(B, ii, jj);
+      real *hqi = hq + e[E1_QI], *hqe = hq + e[E1_QE];
+      const real z = zmet(B, i, j), z1 = zmet(B, ii, jj);
+      real chnghi = -(cSIGA[k] * P[P_X1B5] * f0 / z);
+      real chnghe = -(chnghi * z / z1);
+      const bool cut = l >= B.nsqrw;
+      const real *C = cut ? B.e_cut + (size_t) (l - B.nsqrw) * 4 : nullptr;
+      bool shaped = true;
+      if (cut) {
+         if (C[1] >= real(1.) && C[2] >= real(1.)) {
+            chnghi = chnghi * C[0] / C[1];
+            chnghe = chnghe * C[0] / C[2];
+            shaped = false;
+         }
+      } else if (E[E2_DELCUT] >= real(1.)) {
+         shaped = false;
+      }
+      if (shaped) {
+         // the draining square first, then the filling one
+         const int leap = chnghi > real(0.) ? 2 : 1;
+         for (int pass = 0; pass < 2; pass++) {
+            const bool interior = (leap == 1) == (pass == 0);   // square being treated
+            const bool out = pass == 0;                           // its water leaves
+            real chng = interior ? chnghi : chnghe;
+            const real cstrt = chng, ht = interior ? hb[p] : hb[q];
+            const real bk = interior ? E[E2_BANK1] : E[E2_BANK2];
+            const real cutw = cut ? (interior ? C[1] : C[2]) : real(0.);
+            if (out) {
+               if (ht <= bk) {
+                  if (cut) chng = chng * C[0] / cutw;
+               } else {
+                  const real delb = ht - bk;
+                  if (!cut) {
+                     if (-chng * E[E2_DELCUT] <= delb) chng = chng * E[E2_DELCUT];
+                     else chng = chng - delb + delb / E[E2_DELCUT];
+                  } else {
+                     if (-chng * C[0] <= delb) chng = chng * C[0];
+                     else chng = (chng * C[0] + delb) / cutw - delb;
+                  }
+               }
+            } else {
+               if (ht >= bk) {
+                  chng = cut ? chng * C[0] : chng * E[E2_DELCUT];
+               } else {
+                  const real delb = bk - ht;
+                  if (!cut) {
+                     if (!(chng <= delb)) chng = delb + (chng - delb) * E[E2_DELCUT];
+                  } else {
+                     if (chng * C[0] <= delb * cutw) chng = chng * C[0] / cutw;
+                     else chng = chng * C[0] - delb * cutw + delb;
+                  }
+               }
+            }
+            if (interior) {
+               chnghi = chng;
+               *hqi = *hqi + fabs(cstrt - chng);
+            } else {
+               chnghe = chng;
+               *hqe = *hqe + fabs(cstrt - chng);
+            }
+         }
+      }
+      const real hbj = hb[p] + chnghi, hbjj = hb[q] + chnghe;
+      const real zbi = B.zb[p], zbe = B.zb[q];
+      const real depli = hbj + zbi, deple = hbjj + zbe;
+      if (depli < real(0.)) {
+         const real cii = chnghi - depli, sube = -(cii * z / z1);
+         hb[p] = -zbi;
+         hb[q] = fmax(hb[q] - chnghe + sube, -zbe);
+         flw[l] = f0 * cii / chnghi;
+      } else if (deple < real(0.)) {
+         const real cee = chnghe - deple, subi = -(cee * z1 / z);
+         hb[p] = fmax(hb[p] - chnghi + subi, -zbi);
+         hb[q] = -zbe;
+         flw[l] = f0 * cee / chnghe;
+      } else {
+         hb[p] = hbj;
+         hb[q] = hbjj;
+         act[l] = 1;
+      }
+   }
+   // ---- momentum
+   const real slpa = EREC(B, 1, 2)[E_AI2] - EREC(B, 1, 1)[E_AI2];
+   for (int l = 0; l < nl; l++) {
+      const int *e = B.e_int + (size_t) l * SL_EI;
+      const real *E = B.e_real + (size_t) l * 8;
+      const int i = e[E1_I], j = e[E1_J], k = e[E1_K], ii = e[E1_II], jj = e[E1_JJ], iik = e[E1_IIK];
+      const uint32_t p = IDX(B, i, j), q = IDX(B, ii, jj);
+      real *hqi = hq + e[E1_QI], *hqe = hq + e[E1_QE];
+      const real hint1 = hb[p], hext1 = hb[q], zl = E[E2_ZBMIN], zbi = B.zb[p], zbe = B.zb[q];
+      if (fmin(hint1, hext1) >= zl || (hint1 == -zbi && hext1 == -zbe)) {
+         flw[l] = real(0.);
+         *hqi = real(0.);
+         *hqe = real(0.);
+         continue;
+      }
+      const real hovt = E[E2_HWEIR];
+      if (fmax(hint1, hext1) <= hovt) {
+         flw[l] = real(0.);
+         *hqi = real(0.);
+         *hqe = real(0.);
+         continue;
+      }
+      const real hint = fmin(fmax(hint1, hovt), zl), hext = fmin(fmax(hext1, hovt), zl);
+      const int jump = (fmax(hint, hext) >= zl) ? 1 : 2;
+      const real head = cSIGA[k] * (hext - hint);
+      const real d1 = hint + zbi, d2 = hext + zbe;
+      const real dph = real(.5) * (d1 + d2);
+      if (dph < real(0.00000001)) {
+         flw[l] = real(0.);
+         *hqi = real(0.);
+         *hqe = real(0.);
+         continue;
+      }
+      int id = (int) fmin(real(599.), dph);
+      if (id < 1) id = 1;
+      const real *Ed = EREC(B, iik, id);
+      real biobr = Ed[E_BI2] / Ed[E_BR2];
+      if (B.nsqrw != B.nsqrs && l >= B.nsqrs && l < B.nsqrw) biobr = real(0.);
+      const real hcoff = Ed[E_BR2] + Ed[E_BI2] * biobr;
+      const real flwt = -(P[P_X1B1] * hcoff * dph * head);
+      real frc12t = real(0.);
+      if (jump != 1) {
+         real cshltr;
+         if (e[E1_T]) {
+            const real r = dph / real(50.);
+            cshltr = r * r;
+         } else {
+            const real cshint = real(1.) - dim_(real(1.), dim_(hint1, hovt));
+            const real cshext = real(1.) - dim_(real(1.), dim_(hext1, hovt));
+            const real csh2i = real(1.) - dim_(real(1.), dim_(zl, hint1));
+            const real csh2e = real(1.) - dim_(real(1.), dim_(zl, hext1));
+            cshltr = real(.5) * fmin(cshint + cshext, csh2i + csh2e);
+         }
+         int iii = i, jjj = j;
+         real xr0, yr0, elpcz, elpdz;
+         if (k != 3 && k != 4) {   // force point on the side
+            if (k == 2) iii = i + 1;
+            xr0 = B.cosl[jjj - 1];
+            yr0 = B.sinl[jjj - 1];
+            elpcz = B.elpct[iii - 1];
+            elpdz = B.elpdt[iii - 1];
+         } else {
+            if (k == 4) jjj = j + 1;
+            xr0 = B.cost[jjj - 1];
+            yr0 = B.sint[jjj - 1];
+            elpcz = B.elpcl[iii - 1];
+            elpdz = B.elpdl[iii - 1];
+         }
+         real fxb, fyb;
+         const Forcing F{dph, id, iik};
+         frcpnt(B, P, T, xr0 * elpcz, yr0 * elpdz, fxb, fyb, 1, cshltr, 1, F);
+         const real xr = xr0 * elpdz, yr = yr0 * elpcz;
+         const real fxbp = xr * fxb + yr * fyb, fybp = xr * fyb - yr * fxb;
+         if (E[E2_COS] != real(1.)) {   // stress along the side's flow direction
+            fxb = E[E2_COS] * fxbp - E[E2_SIN] * fybp;
+            fyb = E[E2_SIN] * fxbp + E[E2_COS] * fybp;
+         } else {
+            fxb = fxbp;
+            fyb = fybp;
+         }
+         real frc1 = real(.75) * fxb, frc2 = real(.75) * fyb;
+         if (k == 3 || k == 4) {
+            const real t = frc2;
+            frc2 = -frc1;
+            frc1 = t;
+         }
+         frc12t = P[P_DELT] * (frc1 + biobr * frc2);
+      }
+      real aintrp = real(0.);
+      const real dd[2] = {d1, d2};
+      for (int kk = 0; kk < 2; kk++) {
+         const int idd = (int) fmin(real(299.), dd[kk]);
+         real aitr;
+         if (idd <= 1) {
+            aitr = EREC(B, B.sfl ? 1 : iik, 1)[E_AI2] + slpa * (dd[kk] - real(1.));
+         } else {
+            const real *a0 = EREC(B, iik, idd), *a1 = EREC(B, iik, idd + 1);
+            const real slpai = a1[E_AI2] - a0[E_AI2], dphi = (real) idd;
+            aitr = a0[E_AI2] + slpai * (dd[kk] - dphi);
+         }
+         aintrp = aintrp + aitr;
+      }
+      aintrp = real(.5) * aintrp;
+      const real flwcof = P[P_X1B4] * (aintrp - Ed[E_AR2] * biobr) + real(1.);
+      const real fl = flwt + flwcof * flw[l] + frc12t;
+      const real fwd = *hqe - *hqi;
+      if (fwd != real(0.)) {
+         real fd2 = fabs(-(cSIGA[k] * fwd * P[P_X1B2] / (dph * dph)));
+         const real z = zmet(B, i, j), z1 = zmet(B, ii, jj);
+         const real xy = real(.5) * (real(1.) / z + real(1.) / z1);
+         fd2 = fd2 * xy;
+         real f1 = fl;
+         for (int itr = 0; itr < 20; itr++) f1 = fl / (real(1.) + fd2 * fabs(f1));
+         flw[l] = f1;
+      } else {
+         flw[l] = fl;
+      }
+   }
+}
+
 __global__ void __launch_bounds__(TX * TY) k_hmxsv(SlBasinDev B, SlRunDev R, StepArgs A)
 {
    const int i = blockIdx.x * TX + threadIdx.x + 1, j = blockIdx.y * TY + threadIdx.y + 1;
@@ -1402,6 +1687,9 @@ extern "C" int SL_LAUNCH(const SlBasinDev *B, const SlRunDev *R, int smpt, int f
 {
    real *h0 = parity ? R->hb1 : R->hb0, *h1 = parity ? R->hb0 : R->hb1;
    const dim3 gadv((R->nstorm + 255) / 256), badv(256);
+   // basins with 1-D flow sides step on the staged path (the side passes sit
+   // between routines the fused kernel runs in one pass)
+   if (path == 0 && (B->nsqrw > 0 || B->f1d_on)) return -(int) hipErrorNotSupported;
    StepArgs a;
 #define MARK(k) \
    if (ev) (void) hipEventRecord(ev[k], st)
@@ -1450,8 +1738,12 @@ extern "C" int SL_LAUNCH(const SlBasinDev *B, const SlRunDev *R, int smpt, int f
       MARK(3);
       MARK(4);
    }
+   // SMPT2G's pass over the 1-D flow sides
+   if (smpt && B->nsqrw > 0) hipLaunchKernelGGL(k_f1d_mark, dim3((R->nstorm + 63) / 64), dim3(64), 0, st, *B, *R, a);
    // FLT2G + BDRYHT + row copy: h1 -> h0
    hipLaunchKernelGGL(k_smooth, grd, blk, 0, st, *B, *R, a);
+   // FLW1DM_C + FLW1DM_M on h0
+   if (B->f1d_on) hipLaunchKernelGGL(k_flw1d, dim3((R->nstorm + 63) / 64), dim3(64), 0, st, *B, *R, a);
    MARK(5);
    // HMXSV on h0
    a.hc = h0;

@@ -35,6 +35,7 @@ def lib():
         L.slg_basin_open.argtypes = [cp, cp, ctypes.POINTER(vp), cp, i]
         L.slg_basin_set_ylt.argtypes = [vp, vp]
         L.slg_basin_dims.argtypes = [vp, ctypes.POINTER(i), ctypes.POINTER(i), ctypes.POINTER(i)]
+        L.slg_basin_sides.argtypes = [vp, ctypes.POINTER(i), ctypes.POINTER(i), ctypes.POINTER(i)]
         L.slg_basin_close.argtypes = [vp]
         L.slg_basin_close.restype = None
         L.slg_batch_open.argtypes = [vp, i, ctypes.POINTER(cp), ctypes.POINTER(vp), cp, i]
@@ -86,6 +87,8 @@ class Basin:
         a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         L.slg_basin_dims(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
         self.imxb, self.jmxb, self.nbcpt = a.value, b.value, c.value
+        L.slg_basin_sides(self.h, None, None, ctypes.byref(c))
+        self.nsides = c.value
 
     @property
     def cells(self):
@@ -166,6 +169,14 @@ class Batch:
         _chk(lib().slg_batch_get(self.h, storm, name.encode(), out.ctypes.data_as(ctypes.c_void_p),
                                  err, 512), err)
         return out.reshape(self.basin.jmxb, self.basin.imxb)
+
+    def flwsqr(self, storm: int) -> np.ndarray:
+        """Transport across each sub-grid 1-D flow side of the basin."""
+        out = np.zeros(max(self.basin.nsides, 1), dtype=np.float64)
+        err = ctypes.create_string_buffer(512)
+        _chk(lib().slg_batch_get(self.h, storm, b"FLWSQR", out.ctypes.data_as(ctypes.c_void_p), err, 512),
+             err)
+        return out[:self.basin.nsides]
 
     def stats(self, storm: int):
         out = np.zeros(8, dtype=np.int32)

@@ -54,6 +54,10 @@ class Basin:
     barriers: list = field(default_factory=list)   # (i, j, h) 1-based, h ft
     hist: list = field(default_factory=list)       # 10 (i, j) history points
     extra: dict = field(default_factory=dict)      # dsntn/asmpt/aaxis/baxis/gsize
+    # sub-grid 1-D flow (crdrd2.f:104-357): flow sides (i, j, ktree, side, iang),
+    # banks (i, j, side, bki, bke, wchn), weirs (i, j, ktree, side, hweir),
+    # cuts (i, j, ktree, side, bank1, bank2, cutl, cutli, cutle, iang, hweir)
+    edges: dict = field(default_factory=dict)
 
     @property
     def abbrev(self) -> str:
@@ -194,6 +198,58 @@ def make_basin(name="tst", gtype=" ", ni=100, nj=90, seed=1, land_frac=0.20,
                  hist=hist, extra=extra)
 
 
+def add_edges(b: Basin, seed=7, n_flow=12, n_weirs=4, n_cuts=4, banks=True, crest_ft=(8, 14)):
+    """Sub-grid 1-D flow on coastal water squares: each side gets barriers at
+    both of its end points (so 2-D flow across it is blocked below the crest
+    and the side carries 1-D flow), some sides with banks, raised weirs and
+    cuts, some with 'T' (canal) sheltering and rotated stress directions.
+    Sides stay one square inside the basin's outer rows and columns."""
+    rng = np.random.default_rng(seed)
+    ni, nj = b.imxb - 1, b.jmxb - 1
+    cand = [(i, j) for j in range(3, nj - 2) for i in range(3, ni - 2)
+            if 1 <= b.zb[j - 1, i - 1] <= 25]
+    rng.shuffle(cand)
+    used, out = set(), {"flow": [], "banks": [], "weirs": [], "cuts": []}
+    ends = {1: ((0, 1), (0, 0)), 2: ((1, 1), (1, 0)), 3: ((0, 0), (1, 0)), 4: ((0, 1), (1, 1))}
+    di = {1: -1, 2: 1, 3: 0, 4: 0}
+    dj = {1: 0, 2: 0, 3: -1, 4: 1}
+    want = [("flow", n_flow), ("weirs", n_weirs), ("cuts", n_cuts)]
+    it = iter(cand)
+    for kind, n in want:
+        made = 0
+        while made < n:
+            try:
+                i, j = next(it)
+            except StopIteration:
+                break
+            side = int(rng.integers(1, 5))
+            ii, jj = i + di[side], j + dj[side]
+            if (i, j) in used or (ii, jj) in used or not (1 <= b.zb[jj - 1, ii - 1] <= 25):
+                continue
+            used.update({(i, j), (ii, jj)})
+            crest = int(rng.integers(crest_ft[0], crest_ft[1] + 1))
+            for (a, c) in ends[side]:
+                b.barriers.append((i + a, j + c, crest))
+            ktree = "T" if rng.random() < 0.25 else " "
+            iang = int(rng.choice([0, 0, 30, 45, 120]))
+            if kind == "flow":
+                out["flow"].append((i, j, ktree, side, iang))
+                if banks and rng.random() < 0.5:
+                    out["banks"].append((i, j, side, int(rng.integers(-3, 4)), int(rng.integers(-3, 4)),
+                                         float(rng.choice([0.25, 0.5, 0.75, 1.0]))))
+            elif kind == "weirs":
+                out["weirs"].append((i, j, ktree, side, float(rng.choice([0.0, -2.5, 1.5, 4.0]))))
+            else:
+                out["cuts"].append((i, j, ktree, side, int(rng.choice([0, 0, 2, -1])),
+                                    int(rng.choice([0, 0, 3])), float(rng.choice([0.5, 1.0, 1.5])),
+                                    float(rng.choice([0.0, 0.5, 1.0, 2.0])),
+                                    float(rng.choice([0.0, 0.6, 1.0, 1.5])), iang,
+                                    float(rng.choice([0.0, 0.0, -3.0]))))
+            made += 1
+    b.edges = out
+    return b
+
+
 # --------------------------------------------------------------------------------------
 # writers
 
@@ -209,8 +265,9 @@ def dta_text(b: Basin) -> str:
     nm = ("SYN " + b.name.upper())[:10].ljust(10)
     # col 12 grid type, 13-14 island flag, 15 smoothing, 16 order, 17 hemi, 18-21 opts
     # col 15 EBSN1 (smoothing cadence), col 21 X-filter code (crdrd1.f:72-142)
-    L.append(nm + " " + b.gtype + "  " + b.extra.get("ebsn1", " ") + " " * 5 +
-             b.extra.get("ismooth", " "))
+    # (A10,1X,A1,A2,8A1): STA, EBSN, DOLLAR, EBSN1, EBSN2, ISOUTH, NOFLD, NOF1D, ITERRAIN, ISMOOTH
+    L.append(nm + " " + b.gtype + "  " + b.extra.get("ebsn1", " ") + " " * 3 + b.extra.get("nof1d", " ") +
+             " " + b.extra.get("ismooth", " "))
     L.append(_f10(b.azmth))
     if b.gtype in "$+":
         L.append(_f10(b.alto, b.alno))                       # TLAT TLONG (unused)
@@ -232,14 +289,26 @@ def dta_text(b: Basin) -> str:
     L.append("%3d  " % n if n < 1000 else "%3d%s " % (n % 1000, "XYZWVUT"[n // 1000 - 1]))
     for k in range(0, n, 7):
         L.append("".join("%4d%4d%3d" % bb for bb in b.barriers[k:k + 7]))
-    L.append("  0    0")                                     # 1-D flow points (none)
-    L.append("  0  ")                                        # weirs
+    E = b.edges
+    flow, banks, weirs, cuts = (E.get(k, []) for k in ("flow", "banks", "weirs", "cuts"))
+    L.append("%3d %s%3d" % (len(flow), "X" if banks else " ", 0))   # (I3,2A1,I3)
+    for k in range(0, len(flow), 6):                                 # (6(2I4,A1,I1,I3))
+        L.append("".join("%4d%4d%s%1d%3d" % e for e in flow[k:k + 6]))
+    if flow and banks:
+        L.append("%3d  " % len(banks))                              # (I3,2A1)
+        for k in range(0, len(banks), 4):                            # (4(2I4,A1,I1,2F3.0,F4.2))
+            L.append("".join("%4d%4d %1d%3d%3d%4.2f" % e for e in banks[k:k + 4]))
+    L.append("%3d  " % len(weirs))
+    for k in range(0, len(weirs), 5):                                # (5(2I3,A1,I1,F5.1))
+        L.append("".join("%3d%3d%s%1d%5.1f" % e for e in weirs[k:k + 5]))
     L.append("".join(("STA%02d" % (k + 1)).ljust(10) for k in range(6)))
     L.append("".join(("STA%02d" % (k + 7)).ljust(10) for k in range(4)))
     L.append("".join("%3d" % h[0] for h in b.hist))
     L.append("".join("%3d" % h[1] for h in b.hist))
     L.append("  0")                                          # channels
-    L.append("  0  ")                                        # cuts
+    L.append("%3d  " % len(cuts))                            # cuts, one per record
+    for e in cuts:                                           # (2I4,A1,I1,2F4.0,3F5.2,I3,F5.1)
+        L.append("%4d%4d%s%1d%4d%4d%5.2f%5.2f%5.2f%3d%5.1f" % e)
     L.append("  1")                                          # MGPT: tree map follows
     for j in range(b.jmxb):
         row = "".join(b.itree[j, :])

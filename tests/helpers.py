@@ -112,6 +112,11 @@ class Oracle:
         self.L.orc_get(self.c, name.encode(), out.ctypes.data_as(ctypes.c_void_p))
         return out.reshape(self.jmxb, self.imxb)
 
+    def flwsqr(self):
+        out = np.zeros(41000)
+        n = self.L.orc_get(self.c, b"FLWSQR", out.ctypes.data_as(ctypes.c_void_p))
+        return out[:n]
+
     def put(self, name, a):
         a = np.ascontiguousarray(a, dtype=np.float64)
         self.L.orc_put(self.c, name.encode(), a.ctypes.data_as(ctypes.c_void_p))
@@ -210,6 +215,8 @@ def gpu_vs_oracle(case, nsteps, checks, storms=None, path=0, tile_rows=16, preci
             X.step(k + 1 - X.current_step)
             X.sync()
             out[k + 1] = {f: (X.get(0, f), o.get(f)) for f in FIELDS}
+            if B.nsides:
+                out[k + 1]["FLWSQR"] = (X.flwsqr(0), o.flwsqr())
     o.close()
     X.close()
     B.close()
@@ -240,6 +247,30 @@ def variant_case(root, idx, ni=28, nj=24, hours=6):
     t.dp = [85.0] * len(t.dp)
     b.extra["ebsn1"] = e
     b.extra["ismooth"] = m
+    return make_case(root, b, t)
+
+
+# Basins with sub-grid 1-D flow sides (1-D sides with and without banks, raised
+# weirs, cuts; 'T' canal sheltering and rotated stress directions): (grid type,
+# seed, ebsn1, ismooth, nof1d).  NOF1D '+' keeps SMPT2G's side pass but turns
+# FLW1DM off (tmstpc1.f:92, tmstpc2.f:69).
+EDGE_VARIANTS = [(" ", 5, " ", " ", " "), ("$", 6, " ", " ", " "), ("+", 7, " ", " ", " "),
+                 (" ", 8, "!", "@", " "), (" ", 9, "+", " ", " "), (" ", 10, " ", " ", "+")]
+
+
+def edge_id(idx):
+    g, seed, e, m, n = EDGE_VARIANTS[idx]
+    return "%s-s%d-e%02x-m%02x%s" % ({" ": "polar", "$": "ellip", "+": "hyper"}[g], seed, ord(e), ord(m),
+                                     "-nof1d" if n == "+" else "")
+
+
+def edge_case(root, idx, hours=8):
+    g, seed, e, m, n = EDGE_VARIANTS[idx]
+    b, t = synth.tiny(seed=seed, gtype=g, ni=40, nj=34, hours=hours)
+    b.extra["ebsn1"] = e
+    b.extra["ismooth"] = m
+    b.extra["nof1d"] = n
+    synth.add_edges(b, seed=seed + 2)
     return make_case(root, b, t)
 
 

@@ -158,6 +158,27 @@ def test_fused_equals_staged_ensemble(tmp_path):
             assert np.array_equal(r[1][s], res[0][1][s]), ("stats", s)
 
 
+# ------------------------------------------------- sub-grid 1-D flow sides
+# Basins with 1-D flow sides, banks, raised weirs and cuts step on the staged
+# path with SMPT2G's side pass and FLW1DM_C / FLW1DM_M between its routines.
+
+@pytest.mark.parametrize("idx", range(len(helpers.EDGE_VARIANTS)), ids=helpers.edge_id)
+def test_sides_bitwise(idx, tmp_path):
+    case = helpers.edge_case(str(tmp_path), idx)
+    print("sides", helpers.edge_id(idx), flush=True)
+    out, _ = helpers.gpu_vs_oracle(case, 480, [1, 5, 31, 100, 200, 400], path=sloshgpu.PATH_STAGED)
+    assert all("FLWSQR" in f for f in out.values())
+    _assert_identical(out, helpers.edge_id(idx))
+
+
+@pytest.mark.parametrize("idx", range(len(helpers.EDGE_VARIANTS)), ids=helpers.edge_id)
+def test_fp64_sides_bitwise(idx, tmp_path):
+    case = helpers.edge_case(str(tmp_path), idx)
+    print("fp64 sides", helpers.edge_id(idx), flush=True)
+    out, _ = helpers.gpu_vs_oracle(case, 480, [1, 31, 200, 400], path=sloshgpu.PATH_STAGED, precision=64)
+    _assert_identical(out, helpers.edge_id(idx))
+
+
 # ---------------------------------------------------------------- FP64 step
 # The same kernels built in double reproduce the reference's default FP64
 # build bit for bit: against the reference's own golden state dumps directly,

@@ -30,3 +30,59 @@ def test_variant_oracle64_and_host_match_reference(idx, tmp_path):
         st = helpers.read_rec(ref + "/s%06d.bin" % c)
         for f in helpers.FIELDS:
             assert np.array_equal(snaps[c][f].ravel(), st[f]), "step %d %s differs" % (c, f)
+
+
+@pytest.mark.parametrize("idx", range(len(helpers.EDGE_VARIANTS)), ids=helpers.edge_id)
+def test_sides_oracle64_and_host_match_reference(idx, tmp_path):
+    """Basins with sub-grid 1-D flow sides: the product's reading of the sides
+    (CRDRD2) and DEPSFC's bank, weir and barrier heights equal the reference's
+    COMMON blocks; the FP64 oracle's FLW1DM_C / FLW1DM_M and SMPT2G side pass
+    reproduce HB/UB/VB/HMX and every side's transport FLWSQR bit for bit."""
+    root = str(tmp_path)
+    case = helpers.edge_case(root, idx)
+    checks = [1, 5, 31, 100, 200, 400]
+    ref = helpers.run_reference(case, root + "/ref", 400, ",".join(map(str, checks)))
+    rows, tabs, _ = helpers.product_export(case, root)
+    ri = helpers.read_rec(ref + "/init.bin")
+    mi = helpers.read_rec(root + "/init.bin")
+    for k in mi:
+        assert np.array_equal(ri[k], mi[k]), "init record %s differs" % k
+    nl = int(mi["INTS"][18])
+    assert nl >= 15 and len(mi["CUTL"]) >= 1 and int(mi["INTS"][17]) > int(mi["INTS"][16])
+    rrows = np.fromfile(ref + "/storm.bin").reshape(-1, 90)
+    assert np.array_equal(rows[:len(rrows)], rrows)
+    o = helpers.Oracle("64", root + "/init.bin")
+    active = 0
+    for k in range(400):
+        if k in tabs:
+            o.set_tables(tabs[k])
+        rc, _ = o.advance(rows[k])
+        assert rc == 0
+        if k + 1 in checks:
+            st = helpers.read_rec(ref + "/s%06d.bin" % (k + 1))
+            for f in helpers.FIELDS:
+                assert np.array_equal(o.get(f).ravel(), st[f]), "step %d %s differs" % (k + 1, f)
+            assert np.array_equal(o.flwsqr(), st["FLWSQR"]), "step %d FLWSQR differs" % (k + 1)
+            active = max(active, int((st["FLWSQR"] != 0).sum()))
+    o.close()
+    if helpers.EDGE_VARIANTS[idx][4] != "+":
+        assert active >= nl // 2      # the sides carry flow
+    else:
+        assert active == 0            # NOF1D '+': FLW1DM off
+
+
+def test_side_planted_mistake_is_caught(tmp_path):
+    """The sides' wind-stress rotation (COS1D, SIN1D) turned by about 0.6 degrees
+    in the oracle's input must change the run: the comparison above sees the
+    side momentum terms."""
+    root = str(tmp_path)
+    case = helpers.edge_case(root, 0)
+    rows, tabs, _ = helpers.product_export(case, root)
+    a = helpers.run_oracle("64", root + "/init.bin", rows, tabs, 200, {200})
+    recs = helpers.read_rec(root + "/init.bin")
+    recs = {k: np.array(v) for k, v in recs.items()}
+    ang = np.arctan2(recs["SIN1D"], recs["COS1D"]) + 0.01
+    recs["COS1D"], recs["SIN1D"] = np.cos(ang), np.sin(ang)
+    helpers.write_rec(root + "/init2.bin", recs)
+    b = helpers.run_oracle("64", root + "/init2.bin", rows, tabs, 200, {200})
+    assert any(not np.array_equal(a[200][f], b[200][f]) for f in helpers.FIELDS)
