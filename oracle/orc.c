@@ -971,6 +971,97 @@ void orc_momntm(const orc_basin *b, const orc_storm *s, orc_state *st)
    (void) hpd;
 }
 
+/* ------------------------------------------------------------------ CHANNL */
+/* Fractional powers: libm pow in double (the reference's FP64 build calls it
+ * for REAL**REAL); in float, the double result rounded. */
+#define POWR(x, y) ((R) pow((double) (x), (double) (y)))
+
+/* Channels between distant squares (channl.f:74-186): last step's flow Q
+ * fills the exit squares and drains the entrance squares; then the new flow
+ * from the head between the two ends: critical depth by Newton (at most 10
+ * corrections), critical flow when the low end is below it, else Manning-type
+ * subcritical flow; heads under 0.1 ft scale the flow down linearly. */
+void orc_channl(const orc_basin *b, const orc_storm *s, orc_state *st)
+{
+   const int imxb = b->imxb, jmxb = b->jmxb;
+   const R g = RC(32.2), cdely = RC(.1);
+   int np, k;
+   for (np = 0; np < b->npss; np++) {
+      const R qwdt = st->chq[np] * s->delt;
+      for (k = 0; k < 2; k++) {
+         const int i = b->ipt0[np][k], j = b->jpt0[np][k], ii = b->iptl[np][k], jj = b->jptl[np][k];
+         if (i > 0 && j > 0 && i < imxb && j < jmxb) {
+            HB(i, j) = HB(i, j) - RC(.5) * qwdt / b->aresq0[np];
+            HB(i, j) = AMAX1(HB(i, j), -ZB(i, j));
+         }
+         if (ii > 0 && jj > 0 && ii < imxb && jj < jmxb) {
+            HB(ii, jj) = HB(ii, jj) + RC(.5) * qwdt / b->aresql[np];
+            HB(ii, jj) = AMAX1(HB(ii, jj), -ZB(ii, jj));
+         }
+      }
+   }
+   for (np = 0; np < b->npss; np++) {
+      const R clngth = b->chlnh[np] * RC(5280.), gl = g * clngth;
+      R y0 = 0, yl = 0, sig = RC(1.), dely, hit, y, af, af3, ae, hcrit, qflw, q;
+      int it = 0;
+      for (k = 0; k < 2; k++) {
+         const int i = b->ipt0[np][k], j = b->jpt0[np][k];
+         if (i > 0 && j > 0 && i < imxb && j < jmxb) y0 = y0 + RC(.5) * HB(i, j);
+      }
+      y0 = y0 + b->chdph[np];
+      for (k = 0; k < 2; k++) {
+         const int ii = b->iptl[np][k], jj = b->jptl[np][k];
+         if (ii > 0 && jj > 0 && ii < imxb && jj < jmxb) yl = yl + RC(.5) * HB(ii, jj);
+      }
+      yl = yl + b->chdph[np];
+      if (y0 <= yl) {
+         const R z = y0;
+         y0 = yl;
+         yl = z;
+         sig = RC(-1.);
+      }
+      dely = y0 - yl;
+      if (dely < cdely) yl = y0 - cdely;
+      hit = yl;
+      y = y0;
+      af = gl + b->gb * POWR(y, RC(4.) / RC(3.));
+      af3 = RC(3.) * af;
+      ae = b->aman * POWR(y, RC(13.) / RC(3.));
+      for (;;) {
+         const R hito3 = POWR(hit, RC(1.) / RC(3.)), hit2 = hit * hit, hit3 = hit * hit2;
+         const R hit10 = hit3 * hito3, hit13 = hit * hit10;
+         const R cnum = af * hit3 + b->agb * hit13 - ae, cdnm = af3 * hit2 + b->agb13 * hit10;
+         hcrit = hit - cnum / cdnm;
+         if (FABS(hcrit - hit) >= RC(.001)) {
+            it = it + 1;
+            if (it <= 10) {
+               hit = hcrit;
+               continue;
+            }
+         }
+         break;
+      }
+      if (yl <= hcrit) {
+         qflw = SQRT(g * hcrit) * hcrit;
+      } else {
+         const R y0trd = POWR(y0, RC(1.) / RC(3.)), yltrd = POWR(yl, RC(1.) / RC(3.));
+         const R y04 = y0 * y0trd, yl4 = yl * yltrd;
+         R z = y0 * y0;
+         const R y013 = z * z * y0trd;
+         z = yl * yl;
+         {
+            const R yl13 = z * z * yltrd;
+            const R qflwsq = b->aman * (y013 - yl13) / (clngth + b->bman * (y04 - yl4));
+            qflw = SQRT(FABS(qflwsq));
+         }
+      }
+      q = qflw * sig;
+      q = q * b->chwth[np];
+      if (dely < cdely) q = q * dely / cdely;
+      st->chq[np] = q;
+   }
+}
+
 /* ---------------------------------------------------------------- FLW1DM_C */
 /* Continuity across the open sides of 1-D flow squares, one side at a time in
  * list order (flw1dm_block_c.f:229-430): the side's transport FLWSQR fills one
@@ -1276,6 +1367,7 @@ int orc_step(const orc_basin *b, const orc_storm *s, orc_state *st, orc_diag *d,
    if (ismpt2g != 0 && s->itime > 5 && incsm != 1)
       if (orc_smpt2g(b, s, st, d)) return -1;
    orc_bdryht(b, s, st);
+   if (b->npss != 0) orc_channl(b, s, st);
    if (b->nsqrwc != 0 && b->nof1d != '+') orc_flw1dm_c(b, s, st);
    /* TMSTPC2 */
    if (b->nsqrwc != 0 && b->nof1d != '+') orc_flw1dm_m(b, s, st);

@@ -39,6 +39,12 @@ typedef struct {
    const int *isqr, *jsqr, *iside;
    const char *ktree;
    const REAL *hweir, *zbmin, *bank1, *bank2, *delcut, *cos1d, *sin1d, *cutl, *cutli, *cutle;
+   /* channels (at most 5): entrance/exit squares, length, width, depth, end
+    * square areas, CNSTNT constants */
+   int npss;
+   int ipt0[5][2], jpt0[5][2], iptl[5][2], jptl[5][2];
+   REAL chlnh[5], chwth[5], chdph[5], aresq0[5], aresql[5];
+   REAL aman, bman, gb, agb, agb13;
 } orc_basin;
 
 /* Storm values the field routines read during one step: everything as it
@@ -59,6 +65,7 @@ typedef struct {
    REAL *hsub;                /* scratch, imxb x jmxb, persists across steps */
    REAL *flwsqr;              /* transport across each 1-D flow side (nsqrwc) */
    char *f1dact;              /* 'T' where both squares of a side held water after FLW1DM_C */
+   REAL chq[5];               /* each channel's flow (CHANNL's saved Q) */
 } orc_state;
 
 typedef struct {
@@ -78,5 +85,6 @@ void orc_filter(const orc_basin *b, const orc_storm *s, orc_state *st);
 void orc_xfilter(const orc_basin *b, const orc_storm *s, orc_state *st, int iversion);
 void orc_momntm(const orc_basin *b, const orc_storm *s, orc_state *st);
 void orc_flw1dm_c(const orc_basin *b, const orc_storm *s, orc_state *st);
+void orc_channl(const orc_basin *b, const orc_storm *s, orc_state *st);
 void orc_flw1dm_m(const orc_basin *b, const orc_storm *s, orc_state *st);
 #endif

@@ -205,6 +205,37 @@ void *orc_open(const char *path)
       c->st.flwsqr = c->flwsqr;
       c->st.f1dact = c->f1dact;
    }
+   {
+      /* channels (absent from files written before they existed) */
+      recd *pt = find(r, n, "CHNL_PTS"), *cd = find(r, n, "CHNLD"), *ca = find(r, n, "CHNARA");
+      recd *cn = find(r, n, "CHNLNO");
+      c->b.npss = ints[19];
+      if (pt && cd && ca && cn) {
+         const int *p = (const int *) pt->data;
+         const double *d = (const double *) cd->data, *a = (const double *) ca->data, *q = (const double *) cn->data;
+         int e;
+         for (k = 0; k < 5; k++) {
+            for (e = 0; e < 2; e++) {
+               c->b.ipt0[k][e] = p[0 + k * 2 + e];
+               c->b.jpt0[k][e] = p[10 + k * 2 + e];
+               c->b.iptl[k][e] = p[20 + k * 2 + e];
+               c->b.jptl[k][e] = p[30 + k * 2 + e];
+            }
+            c->b.chlnh[k] = (R) d[k];
+            c->b.chwth[k] = (R) d[5 + k];
+            c->b.chdph[k] = (R) d[10 + k];
+            c->b.aresq0[k] = (R) a[k];
+            c->b.aresql[k] = (R) a[5 + k];
+         }
+         c->b.aman = (R) q[0];
+         c->b.bman = (R) q[1];
+         c->b.gb = (R) q[2];
+         c->b.agb = (R) q[3];
+         c->b.agb13 = (R) q[4];
+      } else {
+         c->b.npss = 0;
+      }
+   }
    memcpy(c->b.sta, find(r, n, "STA")->data, 16);
    c->b.hter = (R) ((double *) find(r, n, "HTER")->data)[0];
    c->seadtm = (R) ((double *) find(r, n, "DATUM")->data)[0];
@@ -381,6 +412,10 @@ int orc_get(void *vc, const char *name, double *out)
    ctx_t *c = vc;
    R *f = field(c, name);
    size_t k, np = (size_t) c->b.imxb * c->b.jmxb;
+   if (!strcmp(name, "CHQ")) {      /* one value per channel */
+      for (k = 0; k < (size_t) c->b.npss; k++) out[k] = (double) c->st.chq[k];
+      return c->b.npss;
+   }
    if (!strcmp(name, "FLWSQR")) {   /* one value per 1-D flow side */
       for (k = 0; k < (size_t) c->b.nsqrwc; k++) out[k] = (double) c->flwsqr[k];
       return c->b.nsqrwc;

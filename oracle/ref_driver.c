@@ -82,6 +82,10 @@ extern struct { R COS1D[L_], SIN1D[L_]; } cs1da_;
 extern struct { R CUTL[NCT_], CUTLI[NCT_], CUTLE[NCT_]; } dumcut_;
 extern struct { R DELCUT[L_], BANK[2][L_]; } bank2_;
 extern struct { char NOFLD, NOF1D; } opts_;
+extern struct { int IPT0[5][2], JPT0[5][2], IPTL[5][2], JPTL[5][2]; R ENTEXT[5]; } chnl_;
+extern struct { R CHLNH[5], CHWTH[5], CHDPH[5]; } chnld_;
+extern struct { R ARESQ0[5], ARESQL[5]; } chnara_;
+extern struct { R AMAN, BMAN, GB, AGB, AGB13; } chnlno_;
 
 extern void initalcommon_(void);
 extern void inital_(int *mhalt, int *imxb, int *jmxb, R zb[N_][M_], int *hr,
@@ -270,6 +274,10 @@ static void dump_init(const char *path)
       rec(fp, "CUTL", 'd', nc, dumcut_.CUTL);
       rec(fp, "CUTLI", 'd', nc, dumcut_.CUTLI);
       rec(fp, "CUTLE", 'd', nc, dumcut_.CUTLE);
+      rec(fp, "CHNL_PTS", 'i', 40, chnl_.IPT0);
+      rec(fp, "CHNLD", 'd', 15, &chnld_);
+      rec(fp, "CHNARA", 'd', 10, &chnara_);
+      rec(fp, "CHNLNO", 'd', 5, &chnlno_);
    }
    rec(fp, "STA", 'c', 16, gprt_.STA);
    rec(fp, "STRMSB", 'd', 8, &strmsb_);

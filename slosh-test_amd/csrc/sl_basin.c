@@ -397,9 +397,35 @@ static int read_body(frec *r, sl_basin *b, char err[256])
    /* channels */
    fr_next(r);
    b->npss = (int) fr_int(r, 3);
-   if (b->npss != 0) {
-      snprintf(err, 256, "channels (%d) are not supported by this build", b->npss);
+   if (b->npss < 0 || b->npss > 5) {
+      snprintf(err, 256, "number of channels (%d) outside 0-5", b->npss);
       return -1;
+   }
+   if (b->npss != 0) {   /* (5F5.1), (5F6.0), (5F5.0), then four (10I3) records */
+      int (*pt[4])[2] = {b->ipt0, b->jpt0, b->iptl, b->jptl};
+      fr_next(r);
+      for (k = 0; k < b->npss; k++) b->chlnh[k] = fr_real(r, 5, 1);
+      fr_next(r);
+      for (k = 0; k < b->npss; k++) b->chwth[k] = fr_real(r, 6, 0);
+      fr_next(r);
+      for (k = 0; k < b->npss; k++) b->chdph[k] = fr_real(r, 5, 0);
+      for (n = 0; n < 4; n++) {
+         fr_next(r);
+         for (k = 0; k < b->npss; k++) {
+            pt[n][k][0] = (int) fr_int(r, 3);
+            pt[n][k][1] = (int) fr_int(r, 3);
+         }
+      }
+      for (k = 0; k < b->npss; k++)
+         for (n = 0; n < 2; n++) {
+            const int i0 = b->ipt0[k][n], j0 = b->jpt0[k][n], il = b->iptl[k][n], jl = b->jptl[k][n];
+            if (i0 < 1 || j0 < 1 || i0 > imxb - 1 || j0 > jmxb - 1 || il < 1 || jl < 1 || il > imxb - 1 ||
+                jl > jmxb - 1) {
+               snprintf(err, 256, "channel %d end squares (%d,%d) (%d,%d) must lie inside the basin", k + 1,
+                        i0, j0, il, jl);
+               return -1;
+            }
+         }
    }
    /* cuts (crdrd2.f:301-357), one per record */
    fr_next(r);
@@ -797,6 +823,21 @@ static void setdst(sl_basin *b)
       b->elpcl2[i - 1] = b->elpcl[i - 1] * b->elpcl[i - 1];
       b->elpdl2[i - 1] = b->elpdl[i - 1] * b->elpdl[i - 1];
    }
+   /* areas of the channels' entrance and exit squares, setdst.f:88-107 (the
+    * end squares lie inside the basin, checked when they are read) */
+   {
+      int np, k;
+      for (np = 0; np < b->npss; np++) {
+         double z1 = 0., z2 = 0.;
+         for (k = 0; k < 2; k++) {
+            const int i0 = b->ipt0[np][k], j0 = b->jpt0[np][k], il = b->iptl[np][k], jl = b->jptl[np][k];
+            z1 = z1 + b->elpdl2[i0 - 1] + (b->elpcl2[i0 - 1] - b->elpdl2[i0 - 1]) * b->sinl2[j0 - 1];
+            z2 = z2 + b->elpdl2[il - 1] + (b->elpcl2[il - 1] - b->elpdl2[il - 1]) * b->sinl2[jl - 1];
+         }
+         b->aresq0[np] = .5 * z1 * b->dela * b->dela;
+         b->aresql[np] = .5 * z2 * b->dela * b->dela;
+      }
+   }
 }
 
 /* BTMSTR, btmstr.f:79-214.  The reference's default build takes the
@@ -914,6 +955,15 @@ static void cnstnt(sl_basin *b)
    b->gamfp[2] = 0.;
    b->gamfp[1] = b->gama;
    b->gamfp[3] = b->gama1;
+   /* CHANNL's fixed constants, cnstnt.f:62-67 */
+   {
+      const double cman = 1.5 / .03;
+      b->aman = 3. * (cman * cman) / 13.;
+      b->bman = 3. * (cman * cman) / (4. * g);
+      b->gb = g * b->bman;
+      b->agb = b->aman - g * b->bman;
+      b->agb13 = 13. * b->agb / 3.;
+   }
 }
 
 int sl_basin_load(const char *dta_path, sl_basin *b, char err[256])

@@ -282,6 +282,25 @@ static int basin_upload(slg_basin *B, char *err, int errlen)
    D.nsq = I.nsq;
    D.f1d_on = b->nsqrwc != 0 && b->nof1d != '+';
    D.sfl = !strncmp(b->sta, "SOUTH FL", 8);
+   D.npss = b->npss;
+   for (int k = 0; k < 5; k++) {
+      for (int e = 0; e < 2; e++) {
+         D.ch_pt[k][0 + 2 * e] = b->ipt0[k][e];
+         D.ch_pt[k][1 + 2 * e] = b->jpt0[k][e];
+         D.ch_pt[k][4 + 2 * e] = b->iptl[k][e];
+         D.ch_pt[k][5 + 2 * e] = b->jptl[k][e];
+      }
+      D.ch_lnh[k] = (R) b->chlnh[k];
+      D.ch_wth[k] = (R) b->chwth[k];
+      D.ch_dph[k] = (R) b->chdph[k];
+      D.ch_a0[k] = (R) b->aresq0[k];
+      D.ch_al[k] = (R) b->aresql[k];
+   }
+   D.aman = (R) b->aman;
+   D.bman = (R) b->bman;
+   D.gb = (R) b->gb;
+   D.agb = (R) b->agb;
+   D.agb13 = (R) b->agb13;
    {
       std::vector<double> zd(np, 0.);
       for (int j = 1; j <= NJ; j++)
@@ -388,7 +407,7 @@ extern "C" void slg_basin_close(slg_basin *B)
 // basins with sub-grid 1-D flow sides (their passes need the staged path)
 static bool has_sides(const slg_basin *B)
 {
-   return B->h.nsqrw > 0 || (B->h.nsqrwc > 0 && B->h.nof1d != '+');
+   return B->h.nsqrw > 0 || (B->h.nsqrwc > 0 && B->h.nof1d != '+') || B->h.npss > 0;
 }
 
 // --------------------------------------------------------------------------
@@ -653,6 +672,7 @@ static int batch_upload(slg_batch *X, char *err, int errlen)
    add((size_t) S * nl * rs);                                          // flwsqr
    add((size_t) S * nl);                                               // f1dact
    add((size_t) S * nq * rs);                                          // hsq
+   add((size_t) S * 5 * rs);                                           // chq
    char *d = nullptr;
    HIPOK(hipMalloc((void **) &d, tot));
    X->dmem = d;
@@ -677,6 +697,7 @@ static int batch_upload(slg_batch *X, char *err, int errlen)
    D.flwsqr = (R *) (d + off[17]);    // FLWSQR = 0 (intlht.f:190-191), F1DACT not 'T'
    D.f1dact = (uint8_t *) (d + off[18]);
    D.hsq = (R *) (d + off[19]);
+   D.chq = (R *) (d + off[20]);       // Q = 0 (channl.f:73)
    HIPOK(hipMemset(d, 0, tot));
    // initial water (INTLHT) per storm: HB, HMX=HB, UB=VB=0
    std::vector<R> f(fstride, R(0));
@@ -1313,6 +1334,28 @@ extern "C" int slg_export_init(const slg_batch *X, int storm, const char *path)
    rec(fp, "CUTL", 'd', nc, b->cutl);
    rec(fp, "CUTLI", 'd', nc, b->cutli);
    rec(fp, "CUTLE", 'd', nc, b->cutle);
+   {
+      int pts[40];   // IPT0(2,5) JPT0 IPTL JPTL (COMMON /CHNL/)
+      for (int k = 0; k < 5; k++)
+         for (int e = 0; e < 2; e++) {
+            pts[0 + k * 2 + e] = b->ipt0[k][e];
+            pts[10 + k * 2 + e] = b->jpt0[k][e];
+            pts[20 + k * 2 + e] = b->iptl[k][e];
+            pts[30 + k * 2 + e] = b->jptl[k][e];
+         }
+      rec(fp, "CHNL_PTS", 'i', 40, pts);
+      double cd[15], ca[10], cn[5] = {b->aman, b->bman, b->gb, b->agb, b->agb13};
+      for (int k = 0; k < 5; k++) {
+         cd[k] = b->chlnh[k];
+         cd[5 + k] = b->chwth[k];
+         cd[10 + k] = b->chdph[k];
+         ca[k] = b->aresq0[k];
+         ca[5 + k] = b->aresql[k];
+      }
+      rec(fp, "CHNLD", 'd', 15, cd);
+      rec(fp, "CHNARA", 'd', 10, ca);
+      rec(fp, "CHNLNO", 'd', 5, cn);
+   }
    rec(fp, "STA", 'c', 16, b->sta);
    double strmsb[8] = {s->c1, s->c2, s->c21, s->c22, s->ax, s->ay, s->ptency, s->rtency};
    rec(fp, "STRMSB", 'd', 8, strmsb);

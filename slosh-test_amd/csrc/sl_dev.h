@@ -54,6 +54,11 @@ struct SlBasinDevT {
    const int *e_int;          /* [nsqrwc][SL_EI]: see E1_* */
    const R *e_real;           /* [nsqrwc][8]: HWEIR ZBMIN BANK(,1) BANK(,2) DELCUT COS1D SIN1D 0 */
    const R *e_cut;            /* [ncut][4]: CUTL CUTLI CUTLE 0 */
+   /* channels (CHANNL): entrance squares (i,j)x2, exit squares (i,j)x2 */
+   int npss;
+   int ch_pt[5][8];
+   R ch_lnh[5], ch_wth[5], ch_dph[5], ch_a0[5], ch_al[5];
+   R aman, bman, gb, agb, agb13;
 };
 #define SL_EI 10
 enum { E1_I, E1_J, E1_K, E1_II, E1_JJ, E1_IIK, E1_T, E1_QI, E1_QE, E1_PAD };   /* E1_T: KTREE 'T' */
@@ -90,6 +95,7 @@ struct SlRunDevT {
    R *flwsqr;                 /* [nstorm][nsqrwc]: transport across each 1-D flow side */
    uint8_t *f1dact;           /* [nstorm][nsqrwc]: 1 where FLW1DM_C left water on both sides */
    R *hsq;                    /* [nstorm][nsq]: HSUB of the 1-D flow squares (FLW1DM_C -> _M) */
+   R *chq;                    /* [nstorm][5]: each channel's flow, kept from step to step */
 };
 
 #endif

@@ -54,13 +54,18 @@ typedef struct sl_basin {
    double *ar2, *ai2, *br2, *bi2, *cr2, *ci2;
    /* sub-grid 1-D flow edges (CRDRD2 / DEPSFC), L = 1..nsqrwc at index L-1:
     * 1..nsqrs 1-D flow sides, nsqrs+1..nsqrw raised weirs, nsqrw+1..nsqrwc
-    * cuts (cut data cutl/cutli/cutle at index L-nsqrw-1).  Channels (npss)
-    * must be zero for this build. */
+    * cuts (cut data cutl/cutli/cutle at index L-nsqrw-1). */
    int nsqrs, nsqrw, nsqrwc, npss, ncut, nodry;
    int *isqr, *jsqr, *iside, *iang;
    char *ktree;
    double *hweir, *zbmin, *bank1, *bank2, *delcut, *cos1d, *sin1d;
    double *cutl, *cutli, *cutle;
+   /* channels (CRDRD2, crdrd2.f:286-299; at most 5): length (mi), width and
+    * depth (ft), entrance / exit squares (two each), the entrance and exit
+    * areas SETDST approximates (setdst.f:88-107) and CNSTNT's constants */
+   double chlnh[5], chwth[5], chdph[5], aresq0[5], aresql[5];
+   int ipt0[5][2], jpt0[5][2], iptl[5][2], jptl[5][2];
+   double aman, bman, gb, agb, agb13;
    int ipn[10], jpn[10];
    int have_ylt;
 } sl_basin;

@@ -1167,9 +1167,86 @@ __device__ __forceinline__ real dim_(real a, real c)
    return a > c ? a - c : real(0.);
 }
 
-// FLW1DM_C (flw1dm_block_c.f:229-430) then FLW1DM_M (flw1dm_block_m.f:153-397)
-// on HB after BDRYHT (in place).  HSUB of the joined squares lives in hsq.
-__global__ void k_flw1d(SlBasinDev B, SlRunDev R, StepArgs A)
+// Fractional powers of CHANNL: pow in double, rounded to real.  The FP32 step
+// then gets the correctly rounded double result's rounding, as the FP32
+// restatement computes it; the device double pow may differ from the host
+// libm's in the last bit, so channels are the one place the FP64 step is
+// not bit for bit the reference's.
+__device__ __forceinline__ real powr(real x, real y)
+{
+   return (real) pow((double) x, (double) y);
+}
+
+// CHANNL (channl.f:74-186) on HB after BDRYHT, in place; Q of each channel
+// is the storm's chq.
+__device__ static void channl(const SlBasinDev &B, const real *P, real *hb, real *chq)
+{
+   const real g = real(32.2), cdely = real(.1);
+   for (int np = 0; np < B.npss; np++) {
+      const real qwdt = chq[np] * P[P_DELT];
+      for (int k = 0; k < 2; k++) {
+         const int *c = B.ch_pt[np];
+         const uint32_t p0 = IDX(B, c[0 + 2 * k], c[1 + 2 * k]), pl = IDX(B, c[4 + 2 * k], c[5 + 2 * k]);
+         hb[p0] = fmax(hb[p0] - real(.5) * qwdt / B.ch_a0[np], -B.zb[p0]);
+         hb[pl] = fmax(hb[pl] + real(.5) * qwdt / B.ch_al[np], -B.zb[pl]);
+      }
+   }
+   for (int np = 0; np < B.npss; np++) {
+      const int *c = B.ch_pt[np];
+      const real clngth = B.ch_lnh[np] * real(5280.), gl = g * clngth;
+      real y0 = real(0.), yl = real(0.), sig = real(1.);
+      for (int k = 0; k < 2; k++) y0 = y0 + real(.5) * hb[IDX(B, c[0 + 2 * k], c[1 + 2 * k])];
+      y0 = y0 + B.ch_dph[np];
+      for (int k = 0; k < 2; k++) yl = yl + real(.5) * hb[IDX(B, c[4 + 2 * k], c[5 + 2 * k])];
+      yl = yl + B.ch_dph[np];
+      if (y0 <= yl) {
+         const real z = y0;
+         y0 = yl;
+         yl = z;
+         sig = real(-1.);
+      }
+      const real dely = y0 - yl;
+      if (dely < cdely) yl = y0 - cdely;
+      real hit = yl, hcrit;
+      const real y = y0;
+      const real af = gl + B.gb * powr(y, real(4.) / real(3.)), af3 = real(3.) * af;
+      const real ae = B.aman * powr(y, real(13.) / real(3.));
+      for (int it = 0;;) {   // Newton for the critical depth, at most 10 corrections
+         const real hito3 = powr(hit, real(1.) / real(3.)), hit2 = hit * hit, hit3 = hit * hit2;
+         const real hit10 = hit3 * hito3, hit13 = hit * hit10;
+         const real cnum = af * hit3 + B.agb * hit13 - ae, cdnm = af3 * hit2 + B.agb13 * hit10;
+         hcrit = hit - cnum / cdnm;
+         if (fabs(hcrit - hit) >= real(.001)) {
+            it = it + 1;
+            if (it <= 10) {
+               hit = hcrit;
+               continue;
+            }
+         }
+         break;
+      }
+      real qflw;
+      if (yl <= hcrit) {
+         qflw = sqrt(g * hcrit) * hcrit;
+      } else {
+         const real y0trd = powr(y0, real(1.) / real(3.)), yltrd = powr(yl, real(1.) / real(3.));
+         const real y04 = y0 * y0trd, yl4 = yl * yltrd;
+         const real z0 = y0 * y0, zl = yl * yl;
+         const real y013 = z0 * z0 * y0trd, yl13 = zl * zl * yltrd;
+         const real qflwsq = B.aman * (y013 - yl13) / (clngth + B.bman * (y04 - yl4));
+         qflw = sqrt(fabs(qflwsq));
+      }
+      real q = qflw * sig;
+      q = q * B.ch_wth[np];
+      if (dely < cdely) q = q * dely / cdely;
+      chq[np] = q;
+   }
+}
+
+// CHANNL, then FLW1DM_C (flw1dm_block_c.f:229-430) and FLW1DM_M
+// (flw1dm_block_m.f:153-397), on HB after BDRYHT (in place).  HSUB of the
+// squares the sides join lives in hsq.
+__global__ void k_sides(SlBasinDev B, SlRunDev R, StepArgs A)
 {
    const int s = blockIdx.x * blockDim.x + threadIdx.x;
    if (s >= R.nstorm) return;
@@ -1180,6 +1257,8 @@ __global__ void k_flw1d(SlBasinDev B, SlRunDev R, StepArgs A)
    uint8_t *act = R.f1dact + (size_t) s * B.nsqrwc;
    const real *P = par(R, s, step);
    const real *T = tabv(R, s, P);
+   if (B.npss) channl(B, P, hb, R.chq + (size_t) s * 5);
+   if (!B.f1d_on) return;
    const int nl = B.nsqrwc;
    // ---- continuity
    for (int l = 0; l < nl; l++) {
@@ -1687,9 +1766,9 @@ extern "C" int SL_LAUNCH(const SlBasinDev *B, const SlRunDev *R, int smpt, int f
 {
    real *h0 = parity ? R->hb1 : R->hb0, *h1 = parity ? R->hb0 : R->hb1;
    const dim3 gadv((R->nstorm + 255) / 256), badv(256);
-   // basins with 1-D flow sides step on the staged path (the side passes sit
-   // between routines the fused kernel runs in one pass)
-   if (path == 0 && (B->nsqrw > 0 || B->f1d_on)) return -(int) hipErrorNotSupported;
+   // basins with 1-D flow sides or channels step on the staged path (their
+   // passes sit between routines the fused kernel runs in one pass)
+   if (path == 0 && (B->nsqrw > 0 || B->f1d_on || B->npss)) return -(int) hipErrorNotSupported;
    StepArgs a;
 #define MARK(k) \
    if (ev) (void) hipEventRecord(ev[k], st)
@@ -1742,8 +1821,8 @@ extern "C" int SL_LAUNCH(const SlBasinDev *B, const SlRunDev *R, int smpt, int f
    if (smpt && B->nsqrw > 0) hipLaunchKernelGGL(k_f1d_mark, dim3((R->nstorm + 63) / 64), dim3(64), 0, st, *B, *R, a);
    // FLT2G + BDRYHT + row copy: h1 -> h0
    hipLaunchKernelGGL(k_smooth, grd, blk, 0, st, *B, *R, a);
-   // FLW1DM_C + FLW1DM_M on h0
-   if (B->f1d_on) hipLaunchKernelGGL(k_flw1d, dim3((R->nstorm + 63) / 64), dim3(64), 0, st, *B, *R, a);
+   // CHANNL, FLW1DM_C, FLW1DM_M on h0
+   if (B->f1d_on || B->npss) hipLaunchKernelGGL(k_sides, dim3((R->nstorm + 63) / 64), dim3(64), 0, st, *B, *R, a);
    MARK(5);
    // HMXSV on h0
    a.hc = h0;

@@ -232,9 +232,9 @@ extern "C" int slg_server_add_basin(slg_server *S, const char *dta, const char *
    slg_basin *B = nullptr;
    int rc = slg_basin_open(dta, bnt_line, &B, err, errlen);
    if (rc) return rc;
-   if (B->h.nsqrw > 0 || (B->h.nsqrwc > 0 && B->h.nof1d != '+')) {
+   if (B->h.nsqrw > 0 || (B->h.nsqrwc > 0 && B->h.nof1d != '+') || B->h.npss > 0) {
       slg_basin_close(B);
-      seterr(err, errlen, "%s", "basins with 1-D flow sides are stepped through the batch API, not served");
+      seterr(err, errlen, "%s", "basins with 1-D flow sides or channels are stepped through the batch API, not served");
       return SLG_EUNSUP;
    }
    rc = sl_basin_ensure(B, S->prec, err, errlen);

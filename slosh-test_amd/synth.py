@@ -58,6 +58,9 @@ class Basin:
     # banks (i, j, side, bki, bke, wchn), weirs (i, j, ktree, side, hweir),
     # cuts (i, j, ktree, side, bank1, bank2, cutl, cutli, cutle, iang, hweir)
     edges: dict = field(default_factory=dict)
+    # channels (crdrd2.f:286-299): (length mi, width ft, depth ft,
+    # entrance squares ((i, j), (i, j)), exit squares ((i, j), (i, j)))
+    channels: list = field(default_factory=list)
 
     @property
     def abbrev(self) -> str:
@@ -198,6 +201,26 @@ def make_basin(name="tst", gtype=" ", ni=100, nj=90, seed=1, land_frac=0.20,
                  hist=hist, extra=extra)
 
 
+def add_channels(b: Basin, seed=9, n=2):
+    """Channels from pairs of coastal water squares (entrance) to pairs of low
+    land squares (exit), 2-5 miles long, 60-200 ft wide, 6-15 ft deep."""
+    rng = np.random.default_rng(seed)
+    ni, nj = b.imxb - 1, b.jmxb - 1
+    wet = [(i, j) for j in range(3, nj - 2) for i in range(3, ni - 3)
+           if 1 <= b.zb[j - 1, i - 1] <= 20 and 1 <= b.zb[j - 1, i] <= 20]
+    low = [(i, j) for j in range(3, nj - 2) for i in range(3, ni - 3)
+           if -8 <= b.zb[j - 1, i - 1] <= -1 and -8 <= b.zb[j - 1, i] <= -1]
+    rng.shuffle(wet)
+    rng.shuffle(low)
+    out = []
+    for k in range(min(n, len(wet), len(low), 5)):
+        (i0, j0), (il, jl) = wet[k], low[k]
+        out.append((float(rng.choice([2.0, 3.5, 5.0])), float(rng.choice([60.0, 120.0, 200.0])),
+                    float(rng.choice([6.0, 10.0, 15.0])), ((i0, j0), (i0 + 1, j0)), ((il, jl), (il + 1, jl))))
+    b.channels = out
+    return b
+
+
 def add_edges(b: Basin, seed=7, n_flow=12, n_weirs=4, n_cuts=4, banks=True, crest_ft=(8, 14)):
     """Sub-grid 1-D flow on coastal water squares: each side gets barriers at
     both of its end points (so 2-D flow across it is blocked below the crest
@@ -305,7 +328,14 @@ def dta_text(b: Basin) -> str:
     L.append("".join(("STA%02d" % (k + 7)).ljust(10) for k in range(4)))
     L.append("".join("%3d" % h[0] for h in b.hist))
     L.append("".join("%3d" % h[1] for h in b.hist))
-    L.append("  0")                                          # channels
+    ch = b.channels
+    L.append("%3d" % len(ch))                                # channels
+    if ch:
+        L.append("".join("%5.1f" % c[0] for c in ch))         # (5F5.1)
+        L.append("".join("%6.0f" % c[1] for c in ch))         # (5F6.0)
+        L.append("".join("%5.0f" % c[2] for c in ch))         # (5F5.0)
+        for part, axis in ((3, 0), (3, 1), (4, 0), (4, 1)):   # IPT0 JPT0 IPTL JPTL (10I3)
+            L.append("".join("%3d%3d" % (c[part][0][axis], c[part][1][axis]) for c in ch))
     L.append("%3d  " % len(cuts))                            # cuts, one per record
     for e in cuts:                                           # (2I4,A1,I1,2F4.0,3F5.2,I3,F5.1)
         L.append("%4d%4d%s%1d%4d%4d%5.2f%5.2f%5.2f%3d%5.1f" % e)

@@ -251,26 +251,33 @@ def variant_case(root, idx, ni=28, nj=24, hours=6):
 
 
 # Basins with sub-grid 1-D flow sides (1-D sides with and without banks, raised
-# weirs, cuts; 'T' canal sheltering and rotated stress directions): (grid type,
-# seed, ebsn1, ismooth, nof1d).  NOF1D '+' keeps SMPT2G's side pass but turns
-# FLW1DM off (tmstpc1.f:92, tmstpc2.f:69).
-EDGE_VARIANTS = [(" ", 5, " ", " ", " "), ("$", 6, " ", " ", " "), ("+", 7, " ", " ", " "),
-                 (" ", 8, "!", "@", " "), (" ", 9, "+", " ", " "), (" ", 10, " ", " ", "+")]
+# weirs, cuts; 'T' canal sheltering and rotated stress directions) and
+# channels: (grid type, seed, ebsn1, ismooth, nof1d, channels).  NOF1D '+'
+# keeps SMPT2G's side pass but turns FLW1DM off (tmstpc1.f:92, tmstpc2.f:69).
+EDGE_VARIANTS = [(" ", 5, " ", " ", " ", 0), ("$", 6, " ", " ", " ", 2), ("+", 7, " ", " ", " ", 0),
+                 (" ", 8, "!", "@", " ", 3), (" ", 9, "+", " ", " ", 0), (" ", 10, " ", " ", "+", 1)]
 
 
 def edge_id(idx):
-    g, seed, e, m, n = EDGE_VARIANTS[idx]
-    return "%s-s%d-e%02x-m%02x%s" % ({" ": "polar", "$": "ellip", "+": "hyper"}[g], seed, ord(e), ord(m),
-                                     "-nof1d" if n == "+" else "")
+    g, seed, e, m, n, c = EDGE_VARIANTS[idx]
+    return "%s-s%d-e%02x-m%02x%s%s" % ({" ": "polar", "$": "ellip", "+": "hyper"}[g], seed, ord(e), ord(m),
+                                       "-nof1d" if n == "+" else "", "-ch%d" % c if c else "")
 
 
-def edge_case(root, idx, hours=8):
-    g, seed, e, m, n = EDGE_VARIANTS[idx]
+def edge_basin(idx, hours=8):
+    g, seed, e, m, n, c = EDGE_VARIANTS[idx]
     b, t = synth.tiny(seed=seed, gtype=g, ni=40, nj=34, hours=hours)
     b.extra["ebsn1"] = e
     b.extra["ismooth"] = m
     b.extra["nof1d"] = n
     synth.add_edges(b, seed=seed + 2)
+    if c:
+        synth.add_channels(b, seed=seed + 3, n=c)
+    return b, t
+
+
+def edge_case(root, idx, hours=8):
+    b, t = edge_basin(idx, hours)
     return make_case(root, b, t)
 
 

@@ -33,11 +33,8 @@ def cli_cases():
     t.dp = list(np.linspace(40.0, 80.0, len(t.dp)))
     t.rmw = list(np.linspace(12.0, 20.0, len(t.dp)))
     yield "ellip-deepening", b, t
-    for idx in (0, 1):                   # sub-grid 1-D flow sides, banks, weirs, cuts
-        g, seed, e, m, n = helpers.EDGE_VARIANTS[idx]
-        b, t = synth.tiny(seed=seed, gtype=g, ni=40, nj=34, hours=8)
-        b.extra["ebsn1"], b.extra["ismooth"], b.extra["nof1d"] = e, m, n
-        synth.add_edges(b, seed=seed + 2)
+    for idx in (0, 1):                   # sub-grid 1-D flow sides, banks, weirs, cuts, channels
+        b, t = helpers.edge_basin(idx)
         yield "sides-" + helpers.edge_id(idx), b, t
 
 

@@ -34,10 +34,12 @@ def test_variant_oracle64_and_host_match_reference(idx, tmp_path):
 
 @pytest.mark.parametrize("idx", range(len(helpers.EDGE_VARIANTS)), ids=helpers.edge_id)
 def test_sides_oracle64_and_host_match_reference(idx, tmp_path):
-    """Basins with sub-grid 1-D flow sides: the product's reading of the sides
-    (CRDRD2) and DEPSFC's bank, weir and barrier heights equal the reference's
-    COMMON blocks; the FP64 oracle's FLW1DM_C / FLW1DM_M and SMPT2G side pass
-    reproduce HB/UB/VB/HMX and every side's transport FLWSQR bit for bit."""
+    """Basins with sub-grid 1-D flow sides and channels: the product's reading
+    of the sides and channels (CRDRD2), DEPSFC's bank, weir and barrier
+    heights, SETDST's channel end areas and CNSTNT's channel constants equal
+    the reference's COMMON blocks; the FP64 oracle's CHANNL, FLW1DM_C /
+    FLW1DM_M and SMPT2G side pass reproduce HB/UB/VB/HMX and every side's
+    transport FLWSQR bit for bit."""
     root = str(tmp_path)
     case = helpers.edge_case(root, idx)
     checks = [1, 5, 31, 100, 200, 400]
@@ -49,6 +51,7 @@ def test_sides_oracle64_and_host_match_reference(idx, tmp_path):
         assert np.array_equal(ri[k], mi[k]), "init record %s differs" % k
     nl = int(mi["INTS"][18])
     assert nl >= 15 and len(mi["CUTL"]) >= 1 and int(mi["INTS"][17]) > int(mi["INTS"][16])
+    assert int(mi["INTS"][19]) == helpers.EDGE_VARIANTS[idx][5]     # channels
     rrows = np.fromfile(ref + "/storm.bin").reshape(-1, 90)
     assert np.array_equal(rows[:len(rrows)], rrows)
     o = helpers.Oracle("64", root + "/init.bin")
@@ -83,6 +86,20 @@ def test_side_planted_mistake_is_caught(tmp_path):
     recs = {k: np.array(v) for k, v in recs.items()}
     ang = np.arctan2(recs["SIN1D"], recs["COS1D"]) + 0.01
     recs["COS1D"], recs["SIN1D"] = np.cos(ang), np.sin(ang)
+    helpers.write_rec(root + "/init2.bin", recs)
+    b = helpers.run_oracle("64", root + "/init2.bin", rows, tabs, 200, {200})
+    assert any(not np.array_equal(a[200][f], b[200][f]) for f in helpers.FIELDS)
+
+
+def test_channel_planted_mistake_is_caught(tmp_path):
+    """The channels' widths changed by 10 % in the oracle's input must change the
+    run: the channel flow reaches the surge field."""
+    root = str(tmp_path)
+    case = helpers.edge_case(root, 1)
+    rows, tabs, _ = helpers.product_export(case, root)
+    a = helpers.run_oracle("64", root + "/init.bin", rows, tabs, 200, {200})
+    recs = {k: np.array(v) for k, v in helpers.read_rec(root + "/init.bin").items()}
+    recs["CHNLD"][5:10] *= 1.1
     helpers.write_rec(root + "/init2.bin", recs)
     b = helpers.run_oracle("64", root + "/init2.bin", rows, tabs, 200, {200})
     assert any(not np.array_equal(a[200][f], b[200][f]) for f in helpers.FIELDS)
