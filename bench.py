@@ -67,17 +67,18 @@ def b_alg_step(storms):
     return 32.0 + 16.0 / storms
 
 
-def measured_traffic(kernel, storms):
-    """HBM bytes per launch of `kernel` at this workload, from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-calibrated); None if there
-    is no measurement for it."""
+def measured_traffic(kernel, storms, per_launch):
+    """HBM bytes per launch of `kernel` at this workload (per_launch of the
+    ensemble's storms in one launch), from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary of the 1024-storm launch (copy-calibrated, scaled per
+    storm); None if there is no measurement for it."""
     path = os.path.join(HERE, "profiles", "stepB_meow1024_hbm_traffic.md")
     if kernel != "stepB" or storms != 1024 or not os.path.exists(path):
         return None
     with open(path) as fh:
         for line in fh:
             if line.startswith("traffic_bytes_per_launch stepB"):
-                return float(line.split()[-1])
+                return float(line.split()[-1]) * per_launch / 1024.0
     return None
 
 
@@ -104,7 +105,7 @@ def build_ensemble(root, nstorm):
     return b, paths, trks
 
 
-def gpu_run(dta, bnt, trks, steps, warmup, path=None, rows=16, graph=0, events=True):
+def gpu_run(dta, bnt, trks, steps, warmup, path=None, rows=16, graph=0, events=True, streams=1):
     """Open, upload, (autotune,) warm up, then time `steps` steps of the batch.
     Without graphs the timed steps carry HIP events around every kernel (on
     the batch's stream), which give the per-kernel durations of the same
@@ -121,6 +122,9 @@ def gpu_run(dta, bnt, trks, steps, warmup, path=None, rows=16, graph=0, events=T
                 "ms_per_step": {"fused8": ms[0], "fused16": ms[1], "fused32": ms[2], "staged": ms[3]}}
     else:
         X.set_path(path, rows)
+    if streams > 1 and X.n > 1 and (path or sloshgpu.PATH_FUSED) == sloshgpu.PATH_FUSED \
+            and (tune is None or tune["path"] == "fused"):
+        X.set_streams(streams)
     X.step(warmup)
     X.sync()
     left = min(X.nsteps) - X.current_step
@@ -139,10 +143,11 @@ def gpu_run(dta, bnt, trks, steps, warmup, path=None, rows=16, graph=0, events=T
         t0 = time.perf_counter()
         ms, cnt = X.step_timed(steps)
         dt = time.perf_counter() - t0
+    nstreams = X.streams
     X.close()
     cells = B.cells
     B.close()
-    return dict(dt=dt, cells=cells, ms=ms, cnt=cnt, sched_s=t_sched, tune=tune)
+    return dict(dt=dt, cells=cells, ms=ms, cnt=cnt, sched_s=t_sched, tune=tune, streams=nstreams)
 
 
 # -------------------------------------------------------------------- uneven
@@ -332,6 +337,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2000)
     ap.add_argument("--path", choices=["auto", "fused", "staged"], default="auto")
     ap.add_argument("--tile-rows", type=int, default=16)
+    ap.add_argument("--streams", type=int, default=2, choices=[1, 2],
+                    help="step the batched ensemble as two halves on two streams (2) or one (1)")
     ap.add_argument("--uneven-storms", type=int, default=4096)
     ap.add_argument("--uneven-steps", type=int, default=400)
     ap.add_argument("--serve-rates", default="5,20,50")
@@ -392,20 +399,25 @@ def main():
             log("batched %d storms on %dx%d polar basin" % (a.storms, b.imxb - 1, b.jmxb - 1))
             path = None if a.path == "auto" else (sloshgpu.PATH_FUSED if a.path == "fused"
                                                   else sloshgpu.PATH_STAGED)
-            r = gpu_run(paths["dta"], paths["bntline"], trks, a.steps, a.warmup, path, a.tile_rows)
+            r = gpu_run(paths["dta"], paths["bntline"], trks, a.steps, a.warmup, path, a.tile_rows,
+                        streams=a.streams)
             rate = a.storms * r["cells"] * a.steps / r["dt"]
             log("batched %.3e storm.cell.steps/s (%s)" % (rate, r["tune"]))
             # dominant kernel, timed with HIP events on the batch's stream
             avg = {k: r["ms"][i] / r["cnt"][i] for i, k in enumerate(sloshgpu.KERNELS) if r["cnt"][i]}
             dom = max(avg, key=lambda k: r["ms"][sloshgpu.KERNELS.index(k)])
-            dom_bytes = ALG_BYTES[dom] * a.storms * r["cells"]
+            # with two streams the events time the first half's launches
+            per_launch = a.storms // r["streams"]
+            dom_bytes = ALG_BYTES[dom] * per_launch * r["cells"]
             achieved = dom_bytes / (avg[dom] * 1e-3) / 1e9
             out.update(value=rate, ms_per_step=r["dt"] * 1e3 / a.steps)
             out["roofline"] = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                               "traffic": measured_traffic(dom, a.storms),
+                               "traffic": measured_traffic(dom, a.storms, per_launch),
                                "alg_bytes_per_launch": dom_bytes, "avg_launch_ms": avg[dom]}
             out["autotune"] = r["tune"]
+            out["streams"] = r["streams"]
+            out["roofline"]["storms_per_launch"] = per_launch
             out["kernel_avg_ms"] = avg
             out["step_roofline_frac"] = rate * b_alg_step(a.storms) / (HBM_PEAK_GBS * 1e9)
         if "plain" in parts:

@@ -740,11 +740,92 @@ extern "C" int slg_batch_upload(slg_batch *X, char *err, int errlen)
    return X->prec == 64 ? batch_upload<double>(X, err, errlen) : batch_upload<float>(X, err, errlen);
 }
 
-static int launch_one(slg_batch *X, int smpt, int filt, hipEvent_t *ev, int dyn = 0)
+// storms s0 .. s0+ns-1 of a batch as a batch of their own (same buffers)
+template <class R>
+static SlRunDevT<R> run_view(const SlRunDevT<R> &D, const SlBasinDevT<R> &B, int s0, int ns)
 {
+   SlRunDevT<R> V = D;
+   const size_t s = (size_t) s0, fs = D.fstride;
+   V.nstorm = ns;
+   V.hb0 += s * fs;
+   V.hb1 += s * fs;
+   V.ub += s * fs;
+   V.vb += s * fs;
+   V.hmx += s * fs;
+   V.hsubtop += s * (size_t) B.pitch;
+   V.pflag += s * fs;
+   V.hclass += s * fs;
+   V.uz += s * fs;
+   V.evkey += s * SL_EVCAP;
+   V.evcnt += s;
+   V.stat += s * 8;
+   V.params += s * (size_t) D.maxsteps * SL_NPAR;
+   V.tbase += s;
+   V.nsteps += s;
+   V.cur += s;
+   V.flwsqr += s * (size_t) B.nsqrwc;
+   V.f1dact += s * (size_t) B.nsqrwc;
+   V.hsq += s * (size_t) B.nsq;
+   V.chq += s * 5;
+   return V;
+}
+
+// The second half starts its first step when the first half's cascade replay
+// of that step has finished: from then on the halves run half a step apart,
+// each one's replay beside the other's stencil kernels.  (It also starts after
+// everything queued on the batch stream before.)
+template <class R>
+static int launch_split(slg_batch *X, const SlBasinDevT<R> &B, const SlRunDevT<R> &D, int smpt, int filt,
+                        hipEvent_t *ev, int dyn, bool first)
+{
+   const int h = X->nstorm / 2;
+   const SlRunDevT<R> A = run_view(D, B, 0, h), C = run_view(D, B, h, X->nstorm - h);
+   hipEvent_t *evA = ev ? ev : (first ? X->ev_phase : nullptr);
+   const int r1 = dev_launch(&B, &A, smpt, filt, dyn, X->parity, X->path, X->tile_rows, X->st, evA);
+   if (r1 < 0) return r1;
+   if (first && hipStreamWaitEvent(X->st2, evA[2], 0) != hipSuccess) return -(int) hipErrorUnknown;
+   return dev_launch(&B, &C, smpt, filt, dyn, X->parity, X->path, X->tile_rows, X->st2, nullptr);
+}
+
+static int launch_one(slg_batch *X, int smpt, int filt, hipEvent_t *ev, int dyn = 0, bool first = false)
+{
+   if (X->nsplit == 2 && X->path == SLG_PATH_FUSED) {
+      if (X->prec == 64) return launch_split(X, X->b->i64.dev, X->d64, smpt, filt, ev, dyn, first);
+      return launch_split(X, X->b->i32.dev, X->d32, smpt, filt, ev, dyn, first);
+   }
    if (X->prec == 64)
       return dev_launch(&X->b->i64.dev, &X->d64, smpt, filt, dyn, X->parity, X->path, X->tile_rows, X->st, ev);
    return dev_launch(&X->b->i32.dev, &X->d32, smpt, filt, dyn, X->parity, X->path, X->tile_rows, X->st, ev);
+}
+
+// the batch stream waits for the second stream's steps: callers keep seeing
+// one stream
+static int join_streams(slg_batch *X)
+{
+   if (X->nsplit != 2) return 0;
+   if (hipEventRecord(X->ev_join, X->st2) != hipSuccess) return -1;
+   return hipStreamWaitEvent(X->st, X->ev_join, 0) == hipSuccess ? 0 : -1;
+}
+
+extern "C" int slg_batch_set_streams(slg_batch *X, int nstreams)
+{
+   if (nstreams != 1 && nstreams != 2) return SLG_EINPUT;
+   if (nstreams == 2 && X->nstorm < 2) return SLG_EINPUT;
+   if (nstreams == 2 && !X->st2) {
+      if (!X->dmem) return SLG_EINPUT;   // after slg_batch_upload
+      if (hipStreamCreate(&X->st2) != hipSuccess ||
+          hipEventCreateWithFlags(&X->ev_join, hipEventDisableTiming) != hipSuccess)
+         return SLG_EHIP;
+      for (auto &e : X->ev_phase)
+         if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return SLG_EHIP;
+   }
+   X->nsplit = nstreams;
+   return 0;
+}
+
+extern "C" int slg_batch_streams(const slg_batch *X)
+{
+   return X->nsplit;
 }
 
 static void drop_graph(slg_batch *X)
@@ -797,7 +878,7 @@ extern "C" int slg_batch_step(slg_batch *X, int n, char *err, int errlen)
 {
    const slg_basin *B = X->b;
    const int lim = X->maxsteps;   // storms with shorter runs stop at their own end
-   if (X->graph_steps > 0 && X->path == SLG_PATH_FUSED) {
+   if (X->graph_steps > 0 && X->path == SLG_PATH_FUSED && X->nsplit == 1) {
       // whole graphs while they fit, then single steps with the same kernels
       if (!X->gexec) {
          int rc = build_graph(X, err, errlen);
@@ -825,8 +906,9 @@ extern "C" int slg_batch_step(slg_batch *X, int n, char *err, int errlen)
       int itime = step;
       int smpt = (B->ismpt2g && itime > 5 && B->incsm != 1);
       int filt = ((itime + 1) % B->incsm == 0 || B->h.ebsn1 == '+');
-      int rc = launch_one(X, smpt, filt, nullptr);
+      int rc = launch_one(X, smpt, filt, nullptr, 0, k == 0);
       if (rc < 0) {
+         join_streams(X);
          seterr(err, errlen, "kernel launch failed (%s%d)", "", -rc);
          return SLG_EHIP;
       }
@@ -835,6 +917,10 @@ extern "C" int slg_batch_step(slg_batch *X, int n, char *err, int errlen)
       X->parity = rc;
       X->step = step;
       note_step(X, step, rc);
+   }
+   if (join_streams(X)) {
+      seterr(err, errlen, "%s", "stream join failed");
+      return SLG_EHIP;
    }
    return 0;
 }
@@ -888,7 +974,7 @@ extern "C" int slg_batch_step_timed(slg_batch *X, int n, double *ms, int *launch
       int step = X->step + 1, itime = step;
       int smpt = (B->ismpt2g && itime > 5 && B->incsm != 1);
       int filt = ((itime + 1) % B->incsm == 0 || B->h.ebsn1 == '+');
-      int rc = launch_one(X, smpt, filt, &ev[(size_t) pending * NE]);
+      int rc = launch_one(X, smpt, filt, &ev[(size_t) pending * NE], 0, k == 0);
       if (rc < 0) {
          seterr(err, errlen, "kernel launch failed (%s%d)", "", -rc);
          rc_all = SLG_EHIP;
@@ -910,6 +996,7 @@ extern "C" int slg_batch_step_timed(slg_batch *X, int n, double *ms, int *launch
       seterr(err, errlen, "%s", "event timing failed");
       rc_all = SLG_EHIP;
    }
+   join_streams(X);
    (void) hipStreamSynchronize(X->st);
    for (auto &e : ev) (void) hipEventDestroy(e);
    return rc_all;
@@ -1216,6 +1303,13 @@ extern "C" void slg_batch_close(slg_batch *X)
 {
    if (!X) return;
    drop_graph(X);
+   if (X->st2) {
+      (void) hipStreamSynchronize(X->st2);
+      (void) hipStreamDestroy(X->st2);
+      (void) hipEventDestroy(X->ev_join);
+      for (auto &e : X->ev_phase)
+         if (e) (void) hipEventDestroy(e);
+   }
    if (X->dmem) {
       (void) hipStreamSynchronize(X->st);
       (void) hipStreamDestroy(X->st);

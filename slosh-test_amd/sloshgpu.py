@@ -48,6 +48,8 @@ def lib():
         L.slg_batch_step_timed.argtypes = [vp, i, vp, vp, cp, i]
         L.slg_batch_set_path.argtypes = [vp, i, i]
         L.slg_batch_set_graph.argtypes = [vp, i]
+        L.slg_batch_set_streams.argtypes = [vp, i]
+        L.slg_batch_streams.argtypes = [vp]
         L.slg_batch_autotune.argtypes = [vp, i, ctypes.POINTER(i), ctypes.POINTER(i), vp, cp, i]
         L.slg_batch_envelopes.argtypes = [vp, vp, vp, cp, i]
         L.slg_track_datums.argtypes = [cp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]
@@ -126,6 +128,15 @@ class Batch:
     def set_graph(self, steps_per_graph: int):
         if lib().slg_batch_set_graph(self.h, steps_per_graph) != 0:
             raise SloshError("bad graph length %r" % steps_per_graph)
+
+    def set_streams(self, nstreams: int):
+        """Step the batch as two halves on two streams (2) or as one (1)."""
+        if lib().slg_batch_set_streams(self.h, nstreams) != 0:
+            raise SloshError("cannot step on %r streams (upload first; 2 needs 2+ storms)" % nstreams)
+
+    @property
+    def streams(self):
+        return lib().slg_batch_streams(self.h)
 
     def autotune(self, steps_per_trial: int = 20):
         """Times fused 8/16/32-row tiles and the staged path on real steps and

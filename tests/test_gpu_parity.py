@@ -300,3 +300,39 @@ def test_graph_replay_and_autotune_bitwise(precision, tmp_path):
         for s in range(len(paths)):
             for f in helpers.FIELDS:
                 assert np.array_equal(res[mode][s][f], res["plain"][s][f]), (mode, s, f)
+
+
+def test_two_stream_halves_bitwise(tmp_path):
+    """The batch stepped as two halves on two streams gives the fields of the
+    one-stream loop, storm by storm (uneven storm count, runs of different
+    lengths, both launch tile heights)."""
+    b, t = synth.tiny(seed=61, ni=30, nj=26, hours=8)
+    root = str(tmp_path)
+    case = helpers.make_case(root, b, t)
+    rng = np.random.default_rng(61)
+    trks = [case["trk"]]
+    for k in range(6):
+        tr = synth.random_track(b, rng, hours_range=(6, 8))
+        p = "%s/h%d.trk" % (root, k)
+        with open(p, "w") as fh:
+            fh.write(synth.trk_text(tr))
+        trks.append(p)
+    B = sloshgpu.Basin(case["dta"], case["bntline"])
+    res = []
+    for streams, rows in ((1, 16), (2, 16), (2, 8)):
+        X = sloshgpu.Batch(B, trks)
+        X.upload()
+        X.set_path(sloshgpu.PATH_FUSED, rows)
+        X.set_streams(streams)
+        assert X.streams == streams
+        X.step(300)
+        ms, cnt = X.step_timed(100)
+        X.step(max(X.nsteps))
+        X.sync()
+        res.append([[X.get(s, f) for f in helpers.FIELDS] for s in range(len(trks))])
+        X.close()
+    B.close()
+    for r in res[1:]:
+        for s in range(len(trks)):
+            for k, f in enumerate(helpers.FIELDS):
+                assert np.array_equal(res[0][s][k], r[s][k]), (s, f)
