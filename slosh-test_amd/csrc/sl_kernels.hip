@@ -276,14 +276,20 @@ __global__ void __launch_bounds__(TX * TY) k_conttyA(SlBasinDev B, SlRunDev R, S
 // dirty square the serial sweep has not reached yet is reset to HSUB, which is
 // what the reference holds for an unvisited square.  If the candidate list
 // overflows, the replay falls back to visiting every square (still exact).
+// The walk runs on one wavefront per storm with every lane following the same
+// control flow.  Along a row the next up to 64 candidates are visited at once,
+// one per lane, without side effects; the visits before the first one that
+// rescales are committed (their HB stores), and the sweep continues from that
+// rescale exactly as the serial code would.  The sorted candidate list lives
+// in LDS; new candidates are merged into it by the whole wave.
 struct Replay {
    const SlBasinDev *B;
    const real *hc;
    real *hn, *ub, *vb;
    uint8_t *uz;
-   uint32_t *key;
+   uint32_t *key;             // sorted candidate keys (j << 16 | i), LDS
    int n, full;
-   uint32_t frontier;
+   uint32_t frontier;         // largest key the sweep has reached
    real x1b5;
 };
 
@@ -298,127 +304,152 @@ __device__ static int lbound(const uint32_t *key, int n, uint32_t k)
    return lo;
 }
 
-__device__ static void go_full(Replay &X)
+__device__ __forceinline__ uint32_t rd_lane(uint32_t v, int l)
+{
+   return (uint32_t) __shfl((int) v, l);
+}
+
+// The candidate list overflowed: from here on the sweep visits every square,
+// and every active square it has not reached yet holds HSUB again.
+__device__ static void go_full(Replay &X, int lane)
 {
    const SlBasinDev &B = *X.B;
    X.full = 1;
    for (int j = 1; j <= B.jmxb - 1; j++)
-      for (int i = 1; i <= B.imxb - 1; i++) {
-         uint32_t k = ((uint32_t) j << 16) | (uint32_t) i;
-         uint32_t p = IDX(B, i, j);
+      for (int i0 = 1; i0 <= B.imxb - 1; i0 += 64) {
+         const int i = i0 + lane;
+         if (i > B.imxb - 1) continue;
+         const uint32_t k = ((uint32_t) j << 16) | (uint32_t) i;
+         const uint32_t p = IDX(B, i, j);
          if (k > X.frontier && (B.flags[p] & F_ACT)) X.hn[p] = X.hc[p];
       }
+   __syncthreads();
 }
 
 // The nine squares around (i,j) become candidates for the rest of the sweep
 // (their transports changed).  A square not yet reached by the sweep gets
-// HB = HSUB back, as the serial sweep would still find it.  The nine squares'
-// operands are loaded in one round trip, each row's position in the sorted
-// list is found once, and the new keys are merged into the list in one pass.
-__device__ static void dirty9(Replay &X, int i, int j)
+// HB = HSUB back, as the serial sweep would still find it.  Lane q < 9 takes
+// neighbour q; the new keys are merged into the sorted list by the wave.
+__device__ static void dirty9(Replay &X, int i, int j, int lane)
 {
    const SlBasinDev &B = *X.B;
    if (X.full) return;
-   uint32_t pp[9], fl[9];
-   real hv[9];
-#pragma unroll
-   for (int q = 0; q < 9; q++) {
-      const int ii = i - 1 + q % 3, jj = j - 1 + q / 3;
-      const bool in = ii >= 1 && jj >= 1 && ii <= B.imxb - 1 && jj <= B.jmxb - 1;
-      pp[q] = in ? IDX(B, ii, jj) : 0u;
-      fl[q] = in ? B.flags[pp[q]] : 0u;
-      hv[q] = in ? X.hc[pp[q]] : real(0.);
-   }
-   uint32_t nk[9];
-   int m = 0;
-   for (int r = 0; r < 3; r++) {
-      const uint32_t row = (uint32_t) (j - 1 + r) << 16;
-      int pos = -1;
-      for (int c = 0; c < 3; c++) {
-         const int q = r * 3 + c;
-         if (!(fl[q] & F_ACT)) continue;
-         const uint32_t k = row | (uint32_t) (i - 1 + c);
-         if (pos < 0) pos = lbound(X.key, X.n, k);
-         while (pos < X.n && X.key[pos] < k) pos++;
-         if (pos < X.n && X.key[pos] == k) continue;   // listed already
-         if (k > X.frontier) X.hn[pp[q]] = hv[q];       // not visited yet: HB is still HSUB
-         nk[m++] = k;
+   bool isnew = false;
+   uint32_t k = 0;
+   int pos = 0;
+   if (lane < 9) {
+      const int ii = i - 1 + lane % 3, jj = j - 1 + lane / 3;
+      if (ii >= 1 && jj >= 1 && ii <= B.imxb - 1 && jj <= B.jmxb - 1) {
+         const uint32_t p = IDX(B, ii, jj);
+         const uint32_t fl = B.flags[p];
+         const real hv = X.hc[p];
+         if (fl & F_ACT) {
+            k = ((uint32_t) jj << 16) | (uint32_t) ii;
+            pos = lbound(X.key, X.n, k);
+            isnew = !(pos < X.n && X.key[pos] == k);
+            if (isnew && k > X.frontier) X.hn[p] = hv;
+         }
       }
    }
+   const uint64_t nm = __ballot(isnew);
+   const int m = __popcll(nm);
    if (!m) return;
-   if (X.n + m > SL_EVCAP) {   // list full: the rest of the sweep visits every square
-      go_full(X);
+   if (X.n + m > SL_EVCAP) {
+      go_full(X, lane);
       return;
    }
-   int a = X.n - 1, b = m - 1, w = X.n + m - 1;
-   while (b >= 0) {
-      if (a >= 0 && X.key[a] > nk[b]) X.key[w--] = X.key[a--];
-      else X.key[w--] = nk[b--];
+   // the new keys, in lane (= row-major = key) order, in every lane
+   uint32_t nk[9];
+   int cnt = 0;
+   for (int q = 0; q < 9; q++) {
+      const uint32_t v = rd_lane(k, q);
+      if ((nm >> q) & 1) nk[cnt++] = v;
    }
+   // every old key moves up by the number of new keys below it; chunks of 64
+   // from the top, so no key is overwritten before it has been read
+   for (int base = ((X.n - 1) >> 6) << 6; base >= 0; base -= 64) {
+      const int a = base + lane;
+      uint32_t v = 0;
+      int sh = 0;
+      if (a < X.n) {
+         v = X.key[a];
+         for (int r = 0; r < cnt; r++) sh += nk[r] < v;
+      }
+      __syncthreads();
+      if (a < X.n && sh) X.key[a + sh] = v;
+      __syncthreads();
+   }
+   if (isnew) X.key[pos + __popcll(nm & ((1ull << lane) - 1))] = k;
    X.n += m;
+   __syncthreads();
 }
 
-// One visit of square (i,j) (the DO 320 body).  Returns 1 if it dried with a
-// transport rescale (the caller then back-steps), else 0.  Every operand is
-// loaded before the first test (all addresses lie inside the field), so a
-// visit costs one memory round trip.
-__device__ static int visit(Replay &X, int i, int j, int &leap, int ipass)
+// What one visit of square (i,j) (the DO 320 body) does, without doing it:
+// 0 inactive, 1 active and nothing stored, 2 store HB = val, 3 dries with a
+// transport rescale by val.  All operands are loaded before the first test.
+struct Visit {
+   int kind;
+   real val;
+   real u[4], v[4];
+};
+
+__device__ static Visit visit_eval(const Replay &X, int i, int j, int leap, int ipass)
 {
    const SlBasinDev &B = *X.B;
+   Visit o;
+   o.kind = 0;
+   o.val = real(0.);
    const uint32_t p = IDX(B, i, j);
    const uint32_t p2 = p + 1, p3 = p + B.pitch + 1, p4 = p + B.pitch;
-   real *ub = X.ub, *vb = X.vb;
    const uint32_t fl = B.flags[p];
    const real zb = B.zb[p], hpst = X.hc[p];
-   const real u1 = ub[p], u2 = ub[p2], u3 = ub[p3], u4 = ub[p4];
-   const real v1 = vb[p], v2 = vb[p2], v3 = vb[p3], v4 = vb[p4];
+   o.u[0] = X.ub[p];
+   o.u[1] = X.ub[p2];
+   o.u[2] = X.ub[p3];
+   o.u[3] = X.ub[p4];
+   o.v[0] = X.vb[p];
+   o.v[1] = X.vb[p2];
+   o.v[2] = X.vb[p3];
+   o.v[3] = X.vb[p4];
    const real ed = B.elpdl2[i - 1], ec = B.elpcl2[i - 1], sl = B.sinl2[j - 1];
-   if (!(fl & F_ACT)) return 0;
-   const uint32_t k = ((uint32_t) j << 16) | (uint32_t) i;
-   if (k > X.frontier) X.frontier = k;
+   if (!(fl & F_ACT)) return o;
+   o.kind = 1;
    if (leap == 2 && i == ipass) {
-      X.hn[p] = -zb;
-      leap = 1;
-      return 0;
+      o.kind = 2;
+      o.val = -zb;
+      return o;
    }
-   const real dux = -u1 + u2 + u3 - u4;
-   if (zb <= real(30.) && dux == real(0.)) return 0;
+   const real dux = -o.u[0] + o.u[1] + o.u[2] - o.u[3];
+   if (zb <= real(30.) && dux == real(0.)) return o;
    const real z = ed + (ec - ed) * sl;
    const real fct = X.x1b5 / z;
-   const real dvy = -v1 - v2 + v3 + v4;
+   const real dvy = -o.v[0] - o.v[1] + o.v[2] + o.v[3];
    const real hbj = hpst - fct * (dux + dvy);
    const real chng = hbj + zb;
    if (chng < real(0.)) {
       if (-chng < real(1E-3)) {
-         X.hn[p] = -zb;
-         return 0;
+         o.kind = 2;
+         o.val = -zb;
+         return o;
       }
    } else {
-      X.hn[p] = hbj;
-      return 0;
+      o.kind = 2;
+      o.val = hbj;
+      return o;
    }
-   if (hpst == -zb) return 0;
+   if (hpst == -zb) return o;
    real denm = hbj - hpst;
    denm = copysign(fmax(real(1.E-5), fabs(denm)), denm);
    const real anum = -zb - hpst;
    real rfct = anum / denm;
    if (fabs(rfct) < real(1.E-5)) rfct = real(0.);
-   const uint32_t c[4] = {p, p2, p3, p4};
-   const real uo[4] = {u1, u2, u3, u4}, vo[4] = {v1, v2, v3, v4};
-   for (int q = 0; q < 4; q++) {
-      const real un = rfct * uo[q];
-      ub[c[q]] = un;
-      vb[c[q]] = rfct * vo[q];
-      if (X.uz) X.uz[c[q]] = un == real(0.);
-   }
-   dirty9(X, i, j);
-   return 1;
+   o.kind = 3;
+   o.val = rfct;
+   return o;
 }
 
 // One wavefront per storm.  The wave sorts the storm's drying squares in LDS
-// (bitonic, row-major key order), then lane 0 walks the serial cascade with
-// the sorted list in LDS: the list lookups and insertions stay on chip and
-// only the visited squares touch HBM.
+// (bitonic, row-major key order), then walks the serial cascade.
 __global__ __launch_bounds__(64) void k_replay(SlBasinDev B, SlRunDev R, StepArgs A, int use_uz)
 {
    __shared__ uint32_t sk[SL_EVCAP];
@@ -448,7 +479,6 @@ __global__ __launch_bounds__(64) void k_replay(SlBasinDev B, SlRunDev R, StepArg
             __syncthreads();
          }
    }
-   if (lane != 0) return;
    const size_t so = (size_t) s * R.fstride;
    Replay X;
    X.B = &B;
@@ -470,7 +500,7 @@ __global__ __launch_bounds__(64) void k_replay(SlBasinDev B, SlRunDev R, StepArg
    if (X.full) {
       j = 1;
       X.frontier = 0;
-      go_full(X);
+      go_full(X, lane);
    } else {
       j = (int) (X.key[0] >> 16);
       X.frontier = (uint32_t) j << 16;
@@ -482,33 +512,62 @@ __global__ __launch_bounds__(64) void k_replay(SlBasinDev B, SlRunDev R, StepArg
    long budget = 64L * B.imxb * B.jmxb + 4096;
 L130:
    if (--budget < 0) {
-      atomicOr(&R.stat[s * 8 + 7], 1);
+      if (lane == 0) atomicOr(&R.stat[s * 8 + 7], 1);
       goto L350;
    }
    if (ifn >= ib) {
-      int pos = X.full ? 0 : lbound(X.key, X.n, ((uint32_t) j << 16) | (uint32_t) ib);
-      int i = ib;
+      const uint32_t row = (uint32_t) j << 16;
+      int pos = X.full ? 0 : lbound(X.key, X.n, row | (uint32_t) ib);
+      const int pend = X.full ? 0 : lbound(X.key, X.n, (row | (uint32_t) ifn) + 1u);
+      int icur = ib;
       for (;;) {
-         if (X.full) {
-            if (i > ifn) break;
-         } else {
-            if (pos >= X.n || X.key[pos] > (((uint32_t) j << 16) | (uint32_t) ifn)) break;
-            i = (int) (X.key[pos] & 0xffffu);
-            pos++;
+         const int cnt = X.full ? ifn - icur + 1 : pend - pos;
+         if (cnt <= 0) break;
+         const int nb = cnt < 64 ? cnt : 64;
+         const bool have = lane < nb;
+         const int i = have ? (X.full ? icur + lane : (int) (X.key[pos + lane] & 0xffffu)) : 0;
+         Visit o;
+         o.kind = 0;
+         if (have) o = visit_eval(X, i, j, leap, ipass);
+         const uint64_t resc = __ballot(have && o.kind == 3);
+         const int first = resc ? __ffsll((unsigned long long) resc) - 1 : nb;
+         if (have && lane < first && o.kind == 2) X.hn[IDX(B, i, j)] = o.val;
+         const uint64_t act = __ballot(have && o.kind != 0 && lane <= first);
+         if (act) {
+            const int last = 63 - __clzll((long long) act);
+            const uint32_t kl = row | (uint32_t) __shfl(i, last);
+            if (kl > X.frontier) X.frontier = kl;
          }
-         if (visit(X, i, j, leap, ipass)) {
-            j = j - 1;
-            jump = 1;
-            iset = i;
-            leap = 1;
-            if (j == 0) goto L330;
-            if (!(B.flags[IDX(B, i, j)] & F_ACT)) goto L330;
-            if (B.is[j - 1] > i) goto L330;
-            ib = (i - 1 > B.is[j - 1]) ? i - 1 : B.is[j - 1];
-            ifn = (i + 1 < B.ie[j - 1]) ? i + 1 : B.ie[j - 1];
-            goto L130;
+         if (leap == 2 && (__ballot(have && lane < first && o.kind == 2 && i == ipass) != 0)) leap = 1;
+         if (first == nb) {
+            pos += nb;
+            icur += nb;
+            continue;
          }
-         if (X.full) i++;
+         // the first rescale: its corner transports, then the sweep backs up a row
+         const int ir = __shfl(i, first);
+         if (lane == first) {
+            const uint32_t p = IDX(B, i, j);
+            const uint32_t c[4] = {p, p + 1, p + B.pitch + 1, p + B.pitch};
+            for (int q = 0; q < 4; q++) {
+               const real un = o.val * o.u[q];
+               X.ub[c[q]] = un;
+               X.vb[c[q]] = o.val * o.v[q];
+               if (X.uz) X.uz[c[q]] = un == real(0.);
+            }
+         }
+         __syncthreads();
+         dirty9(X, ir, j, lane);
+         j = j - 1;
+         jump = 1;
+         iset = ir;
+         leap = 1;
+         if (j == 0) goto L330;
+         if (!(B.flags[IDX(B, ir, j)] & F_ACT)) goto L330;
+         if (B.is[j - 1] > ir) goto L330;
+         ib = (ir - 1 > B.is[j - 1]) ? ir - 1 : B.is[j - 1];
+         ifn = (ir + 1 < B.ie[j - 1]) ? ir + 1 : B.ie[j - 1];
+         goto L130;
       }
    }
 L330:
@@ -535,8 +594,10 @@ L330:
    }
    goto L130;
 L350:
-   R.evcnt[s] = 0;
-   atomicAdd(&R.stat[s * 8 + 4], 1);
+   if (lane == 0) {
+      R.evcnt[s] = 0;
+      atomicAdd(&R.stat[s * 8 + 4], 1);
+   }
 }
 
 // ------------------------------------------------------------------ SMPT2G
