@@ -304,6 +304,19 @@ __device__ static int lbound(const uint32_t *key, int n, uint32_t k)
    return lo;
 }
 
+// lbound by the whole wave (every lane gets the result): the keys below k are
+// counted among 64 evenly spaced samples, then inside the one gap that holds
+// the boundary (n <= 4096 keeps a gap under 64 keys).
+__device__ static int lbound_w(const uint32_t *key, int n, uint32_t k, int lane)
+{
+   if (n <= 64) return __popcll(__ballot(lane < n && key[lane] < k));
+   const int stp = (n + 63) >> 6, a = lane * stp;
+   const int c = __popcll(__ballot(a < n && key[a] < k));
+   if (c == 0) return 0;
+   const int base = (c - 1) * stp, b = base + 1 + lane;
+   return base + 1 + __popcll(__ballot(lane < stp - 1 && b < n && key[b] < k));
+}
+
 __device__ __forceinline__ uint32_t rd_lane(uint32_t v, int l)
 {
    return (uint32_t) __shfl((int) v, l);
@@ -330,11 +343,11 @@ __device__ static void go_full(Replay &X, int lane)
 // (their transports changed).  A square not yet reached by the sweep gets
 // HB = HSUB back, as the serial sweep would still find it.  Lane q < 9 takes
 // neighbour q; the new keys are merged into the sorted list by the wave.
-__device__ static void dirty9(Replay &X, int i, int j, int lane)
+// Returns whether the square below, (i, j-1), is active (the back-step test).
+__device__ static bool dirty9(Replay &X, int i, int j, int lane)
 {
    const SlBasinDev &B = *X.B;
-   if (X.full) return;
-   bool isnew = false;
+   bool isnew = false, act = false;
    uint32_t k = 0;
    int pos = 0;
    if (lane < 9) {
@@ -343,7 +356,8 @@ __device__ static void dirty9(Replay &X, int i, int j, int lane)
          const uint32_t p = IDX(B, ii, jj);
          const uint32_t fl = B.flags[p];
          const real hv = X.hc[p];
-         if (fl & F_ACT) {
+         act = (fl & F_ACT) != 0;
+         if (act && !X.full) {
             k = ((uint32_t) jj << 16) | (uint32_t) ii;
             pos = lbound(X.key, X.n, k);
             isnew = !(pos < X.n && X.key[pos] == k);
@@ -351,12 +365,13 @@ __device__ static void dirty9(Replay &X, int i, int j, int lane)
          }
       }
    }
+   const bool below = (__ballot(act) >> 1) & 1;
    const uint64_t nm = __ballot(isnew);
    const int m = __popcll(nm);
-   if (!m) return;
+   if (!m) return below;
    if (X.n + m > SL_EVCAP) {
       go_full(X, lane);
-      return;
+      return below;
    }
    // the new keys, in lane (= row-major = key) order, in every lane
    uint32_t nk[9];
@@ -382,6 +397,7 @@ __device__ static void dirty9(Replay &X, int i, int j, int lane)
    if (isnew) X.key[pos + __popcll(nm & ((1ull << lane) - 1))] = k;
    X.n += m;
    __syncthreads();
+   return below;
 }
 
 // What one visit of square (i,j) (the DO 320 body) does, without doing it:
@@ -517,8 +533,8 @@ L130:
    }
    if (ifn >= ib) {
       const uint32_t row = (uint32_t) j << 16;
-      int pos = X.full ? 0 : lbound(X.key, X.n, row | (uint32_t) ib);
-      const int pend = X.full ? 0 : lbound(X.key, X.n, (row | (uint32_t) ifn) + 1u);
+      int pos = X.full ? 0 : lbound_w(X.key, X.n, row | (uint32_t) ib, lane);
+      const int pend = X.full ? 0 : lbound_w(X.key, X.n, (row | (uint32_t) ifn) + 1u, lane);
       int icur = ib;
       for (;;) {
          const int cnt = X.full ? ifn - icur + 1 : pend - pos;
@@ -557,13 +573,13 @@ L130:
             }
          }
          __syncthreads();
-         dirty9(X, ir, j, lane);
+         const bool below = dirty9(X, ir, j, lane);
          j = j - 1;
          jump = 1;
          iset = ir;
          leap = 1;
          if (j == 0) goto L330;
-         if (!(B.flags[IDX(B, ir, j)] & F_ACT)) goto L330;
+         if (!below) goto L330;
          if (B.is[j - 1] > ir) goto L330;
          ib = (ir - 1 > B.is[j - 1]) ? ir - 1 : B.is[j - 1];
          ifn = (ir + 1 < B.ie[j - 1]) ? ir + 1 : B.ie[j - 1];
@@ -582,7 +598,7 @@ L330:
    } else {
       j = j + 1;
       if (!X.full) {   // rows without candidates change nothing: skip ahead
-         int pos = lbound(X.key, X.n, (uint32_t) j << 16);
+         int pos = lbound_w(X.key, X.n, (uint32_t) j << 16, lane);
          if (pos >= X.n) goto L350;
          int jn = (int) (X.key[pos] >> 16);
          if (jn > j) j = jn;
@@ -1625,7 +1641,8 @@ __global__ void __launch_bounds__(256) k_stepB(SlBasinDev B, SlRunDev R, StepArg
    const real *P = par(R, s, step);
    const real *T = tabv(R, s, P);
    const int sf = (int) P[P_FLAGS];
-   const bool smpt = SMPT && (!dyn || (sf & SF_SMPT)), filt = FILT && (!dyn || (sf & SF_FILT));
+   bool smpt = SMPT && (!dyn || (sf & SF_SMPT));
+   const bool filt = FILT && (!dyn || (sf & SF_FILT));
    if (tid < 4) sh_cnt[tid] = 0;
 
    // HB after CONTTY (and the U==0 mask) with halo; the static ZB and ZBM
@@ -1639,14 +1656,20 @@ __global__ void __launch_bounds__(256) k_stepB(SlBasinDev B, SlRunDev R, StepArg
       sh_zb[e] = in ? B.zb[IDX(B, gi, gj)] : real(0.);
       sh_zbm[e] = in ? B.zbm[IDX(B, gi, gj)] : real(0.);
    }
+   int anyz = 0;
    if (smpt) {
       const uint8_t *uz_g = R.uz + so;
       for (int e = tid; e < RU::n; e += 256) {
          const int gi = I0 - RU::lo + e % RU::w, gj = J0 - RU::lo + e / RU::w;
-         sh_uz[e] = (gi >= 1 && gi <= IM && gj >= 1 && gj <= JM) ? uz_g[IDX(B, gi, gj)] : 0;
+         const uint8_t u = (gi >= 1 && gi <= IM && gj >= 1 && gj <= JM) ? uz_g[IDX(B, gi, gj)] : 0;
+         sh_uz[e] = u;
+         anyz |= u;
       }
    }
-   __syncthreads();
+   // Without a U == 0 point in reach no momentum point is flagged, no height
+   // point classed and FLT2G changes nothing: the tile skips SMPT2G/FLT2G
+   // (open water, most of a basin).
+   if (!__syncthreads_or(anyz)) smpt = false;
    const LF hsL{sh_hs, I0 - RH::lo, J0 - RH::lo, RH::w};
    const LB clL{sh_cl, I0 - RC::lo, J0 - RC::lo, RC::w};
    const ZL zL{sh_zb, sh_zbm, I0 - RZ::lo, J0 - RZ::lo, RZ::w};
