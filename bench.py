@@ -337,8 +337,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2000)
     ap.add_argument("--path", choices=["auto", "fused", "staged"], default="auto")
     ap.add_argument("--tile-rows", type=int, default=16)
-    ap.add_argument("--streams", type=int, default=2, choices=[1, 2],
-                    help="step the batched ensemble as two halves on two streams (2) or one (1)")
+    ap.add_argument("--streams", type=int, default=3, choices=[1, 2, 3, 4],
+                    help="step the batched ensemble in this many parts of storms, each on its own stream")
     ap.add_argument("--uneven-storms", type=int, default=4096)
     ap.add_argument("--uneven-steps", type=int, default=400)
     ap.add_argument("--serve-rates", default="5,20,50")

@@ -91,11 +91,12 @@ int slg_batch_set_path(slg_batch *x, int path, int tile_rows);
  * The captured steps let every storm's own step flags decide, so one graph
  * serves the whole run. */
 int slg_batch_set_graph(slg_batch *x, int steps_per_graph);
-/* Step the fused path's batch as two halves on two HIP streams (nstreams 2) or
- * as one (1, the default); call after slg_batch_upload.  The halves' kernels
- * overlap on the device: one half's CONTTY cascade replay (a serial walk per
- * storm, contty.f:146-213) runs beside the other half's stencil kernels.
- * Fields are identical either way.  hipGraph replay uses one stream. */
+/* Step the fused path's batch in nstreams parts of storms (1-4, at most one
+ * per storm; 1 is the default), each on its own HIP stream; call after
+ * slg_batch_upload.  The parts' kernels overlap on the device: one part's
+ * CONTTY cascade replay (a serial walk per storm, contty.f:146-213) runs
+ * beside the other parts' stencil kernels.  Fields are identical either way.
+ * hipGraph replay uses one stream. */
 int slg_batch_set_streams(slg_batch *x, int nstreams);
 int slg_batch_streams(const slg_batch *x);
 /* Online autotuner: times fused tiles of 8/16/32 rows and the staged path on

@@ -770,26 +770,33 @@ static SlRunDevT<R> run_view(const SlRunDevT<R> &D, const SlBasinDevT<R> &B, int
    return V;
 }
 
-// The second half starts its first step when the first half's cascade replay
-// of that step has finished: from then on the halves run half a step apart,
-// each one's replay beside the other's stencil kernels.  (It also starts after
-// everything queued on the batch stream before.)
+// Part p of N (storms p*n/N .. (p+1)*n/N - 1) steps on its own stream.  On the
+// first step of a call, part p > 0 starts when part p-1's cascade replay of that
+// step has finished: from then on the parts run staggered, each one's replay
+// beside the others' stencil kernels.  (Part 1 also starts after everything
+// queued on the batch stream before.)
 template <class R>
 static int launch_split(slg_batch *X, const SlBasinDevT<R> &B, const SlRunDevT<R> &D, int smpt, int filt,
                         hipEvent_t *ev, int dyn, bool first)
 {
-   const int h = X->nstorm / 2;
-   const SlRunDevT<R> A = run_view(D, B, 0, h), C = run_view(D, B, h, X->nstorm - h);
-   hipEvent_t *evA = ev ? ev : (first ? X->ev_phase : nullptr);
-   const int r1 = dev_launch(&B, &A, smpt, filt, dyn, X->parity, X->path, X->tile_rows, X->st, evA);
-   if (r1 < 0) return r1;
-   if (first && hipStreamWaitEvent(X->st2, evA[2], 0) != hipSuccess) return -(int) hipErrorUnknown;
-   return dev_launch(&B, &C, smpt, filt, dyn, X->parity, X->path, X->tile_rows, X->st2, nullptr);
+   const int N = X->nsplit, n = X->nstorm;
+   hipEvent_t *prev = nullptr;
+   for (int p = 0; p < N; p++) {
+      const int s0 = (int) ((long) n * p / N), s1 = (int) ((long) n * (p + 1) / N);
+      const SlRunDevT<R> V = run_view(D, B, s0, s1 - s0);
+      hipStream_t st = p ? X->stx[p - 1] : X->st;
+      hipEvent_t *evp = (p == 0 && ev) ? ev : (first ? X->ev_phase[p] : nullptr);
+      if (p && first && hipStreamWaitEvent(st, prev[2], 0) != hipSuccess) return -(int) hipErrorUnknown;
+      const int r = dev_launch(&B, &V, smpt, filt, dyn, X->parity, X->path, X->tile_rows, st, evp);
+      if (r < 0) return r;
+      prev = evp;
+   }
+   return 0;
 }
 
 static int launch_one(slg_batch *X, int smpt, int filt, hipEvent_t *ev, int dyn = 0, bool first = false)
 {
-   if (X->nsplit == 2 && X->path == SLG_PATH_FUSED) {
+   if (X->nsplit > 1 && X->path == SLG_PATH_FUSED) {
       if (X->prec == 64) return launch_split(X, X->b->i64.dev, X->d64, smpt, filt, ev, dyn, first);
       return launch_split(X, X->b->i32.dev, X->d32, smpt, filt, ev, dyn, first);
    }
@@ -798,27 +805,32 @@ static int launch_one(slg_batch *X, int smpt, int filt, hipEvent_t *ev, int dyn 
    return dev_launch(&X->b->i32.dev, &X->d32, smpt, filt, dyn, X->parity, X->path, X->tile_rows, X->st, ev);
 }
 
-// the batch stream waits for the second stream's steps: callers keep seeing
-// one stream
+// the batch stream waits for the other parts' steps: callers keep seeing one
+// stream
 static int join_streams(slg_batch *X)
 {
-   if (X->nsplit != 2) return 0;
-   if (hipEventRecord(X->ev_join, X->st2) != hipSuccess) return -1;
-   return hipStreamWaitEvent(X->st, X->ev_join, 0) == hipSuccess ? 0 : -1;
+   for (int p = 1; p < X->nsplit; p++) {
+      if (hipEventRecord(X->ev_join[p - 1], X->stx[p - 1]) != hipSuccess) return -1;
+      if (hipStreamWaitEvent(X->st, X->ev_join[p - 1], 0) != hipSuccess) return -1;
+   }
+   return 0;
 }
 
 extern "C" int slg_batch_set_streams(slg_batch *X, int nstreams)
 {
-   if (nstreams != 1 && nstreams != 2) return SLG_EINPUT;
-   if (nstreams == 2 && X->nstorm < 2) return SLG_EINPUT;
-   if (nstreams == 2 && !X->st2) {
-      if (!X->dmem) return SLG_EINPUT;   // after slg_batch_upload
-      if (hipStreamCreate(&X->st2) != hipSuccess ||
-          hipEventCreateWithFlags(&X->ev_join, hipEventDisableTiming) != hipSuccess)
+   if (nstreams < 1 || nstreams > SL_MAXSPLIT || nstreams > X->nstorm) return SLG_EINPUT;
+   if (nstreams > 1 && !X->dmem) return SLG_EINPUT;   // after slg_batch_upload
+   for (int p = 1; p < nstreams; p++) {
+      if (X->stx[p - 1]) continue;
+      if (hipStreamCreate(&X->stx[p - 1]) != hipSuccess ||
+          hipEventCreateWithFlags(&X->ev_join[p - 1], hipEventDisableTiming) != hipSuccess)
          return SLG_EHIP;
-      for (auto &e : X->ev_phase)
-         if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return SLG_EHIP;
    }
+   for (int p = 0; p < nstreams; p++)
+      for (auto &e : X->ev_phase[p])
+         if (!e && hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return SLG_EHIP;
+   // steps already queued on the batch stream come first
+   if (join_streams(X)) return SLG_EHIP;
    X->nsplit = nstreams;
    return 0;
 }
@@ -1303,13 +1315,15 @@ extern "C" void slg_batch_close(slg_batch *X)
 {
    if (!X) return;
    drop_graph(X);
-   if (X->st2) {
-      (void) hipStreamSynchronize(X->st2);
-      (void) hipStreamDestroy(X->st2);
-      (void) hipEventDestroy(X->ev_join);
-      for (auto &e : X->ev_phase)
+   for (int p = 0; p < SL_MAXSPLIT - 1; p++)
+      if (X->stx[p]) {
+         (void) hipStreamSynchronize(X->stx[p]);
+         (void) hipStreamDestroy(X->stx[p]);
+         (void) hipEventDestroy(X->ev_join[p]);
+      }
+   for (auto &a : X->ev_phase)
+      for (auto &e : a)
          if (e) (void) hipEventDestroy(e);
-   }
    if (X->dmem) {
       (void) hipStreamSynchronize(X->st);
       (void) hipStreamDestroy(X->st);

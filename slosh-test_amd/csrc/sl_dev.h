@@ -12,6 +12,7 @@
 #define SL_NTABV 801          /* radial tables, index 1..800 used (Fortran order) */
 #define SL_NPAR 26
 #define SL_EVCAP 4096         /* per-storm CONTTY event / dirty-square capacity */
+#define SL_MAXSPLIT 4         /* parts of a batch on their own streams */
 #define T_AI2_OFF 147000      /* offset of AI2 in the [6][245][600] table block */
 #define SL_M2G 30000          /* corner-smoothing list limits, smpt2g.f:17 / parm.for:26 */
 #define SL_MCT 8000

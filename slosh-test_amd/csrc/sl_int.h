@@ -103,12 +103,13 @@ struct slg_batch {
    SlRunDevT<double> d64;
    hipStream_t st;
    size_t bytes;
-   // two halves of the batch on two streams (fused path): one half's
-   // latency-bound cascade replay overlaps the other half's stencil kernels
+   // the batch in nsplit parts of storms on as many streams (fused path): one
+   // part's latency-bound cascade replay overlaps the others' stencil kernels.
+   // Part 0 runs on st, part p > 0 on stx[p - 1].
    int nsplit = 1;
-   hipStream_t st2 = nullptr;
-   hipEvent_t ev_join = nullptr;
-   hipEvent_t ev_phase[SLG_NKERN + 1] = {};   // marks of the first half's first step
+   hipStream_t stx[SL_MAXSPLIT - 1] = {};
+   hipEvent_t ev_join[SL_MAXSPLIT - 1] = {};
+   hipEvent_t ev_phase[SL_MAXSPLIT][SLG_NKERN + 1] = {};   // marks of each part's first step
 };
 static inline SlRunDevT<float> &run(slg_batch *X, float) { return X->d32; }
 static inline SlRunDevT<double> &run(slg_batch *X, double) { return X->d64; }

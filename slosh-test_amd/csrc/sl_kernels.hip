@@ -194,8 +194,11 @@ __device__ __forceinline__ real contty_sq(const SlBasinDev &B, const SlRunDev &R
    real u0 = ub[p];
    if (fl & F_ZR) {
       if (fabs(u0) <= real(1.E-10)) {
-         ub[p] = real(0.);
-         vb[p] = real(0.);
+         // stored only where the value is not +0 already (dry land, most of
+         // them): the memory ends up the same, without the writes
+         const real v0 = vb[p];
+         if (!same_bits(u0, real(0.))) ub[p] = real(0.);
+         if (!same_bits(v0, real(0.))) vb[p] = real(0.);
          u0 = real(0.);
       }
    }
@@ -1609,7 +1612,8 @@ struct Reg {
 // dyn != 0 (storms at different steps in one launch) each storm's own step
 // flags decide, inside the regions of the <TH, true, true> instance.
 template <int TH, bool SMPT, bool FILT>
-__global__ void __launch_bounds__(256) k_stepB(SlBasinDev B, SlRunDev R, StepArgs A, int ntx, int nty, int dyn)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TH == 8 ? 8 : 1)))
+k_stepB(SlBasinDev B, SlRunDev R, StepArgs A, int ntx, int nty, int dyn)
 {
    constexpr int SLO = FILT ? 2 : 1, SHI = FILT ? 2 : 1;
    using RS = Reg<SLO, SHI, TH>;

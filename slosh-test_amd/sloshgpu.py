@@ -130,9 +130,9 @@ class Batch:
             raise SloshError("bad graph length %r" % steps_per_graph)
 
     def set_streams(self, nstreams: int):
-        """Step the batch as two halves on two streams (2) or as one (1)."""
+        """Step the batch in nstreams parts (1-4), each on its own stream."""
         if lib().slg_batch_set_streams(self.h, nstreams) != 0:
-            raise SloshError("cannot step on %r streams (upload first; 2 needs 2+ storms)" % nstreams)
+            raise SloshError("cannot step on %r streams (upload first; 1-4, at most one per storm)" % nstreams)
 
     @property
     def streams(self):

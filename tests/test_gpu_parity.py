@@ -302,10 +302,11 @@ def test_graph_replay_and_autotune_bitwise(precision, tmp_path):
                 assert np.array_equal(res[mode][s][f], res["plain"][s][f]), (mode, s, f)
 
 
-def test_two_stream_halves_bitwise(tmp_path):
-    """The batch stepped as two halves on two streams gives the fields of the
-    one-stream loop, storm by storm (uneven storm count, runs of different
-    lengths, both launch tile heights)."""
+def test_split_streams_bitwise(tmp_path):
+    """The batch stepped in 2, 3 or 4 parts on as many streams gives the fields
+    of the one-stream loop, storm by storm (storm counts that do not divide
+    evenly, runs of different lengths, two tile heights, a change of the part
+    count between calls)."""
     b, t = synth.tiny(seed=61, ni=30, nj=26, hours=8)
     root = str(tmp_path)
     case = helpers.make_case(root, b, t)
@@ -319,7 +320,7 @@ def test_two_stream_halves_bitwise(tmp_path):
         trks.append(p)
     B = sloshgpu.Basin(case["dta"], case["bntline"])
     res = []
-    for streams, rows in ((1, 16), (2, 16), (2, 8)):
+    for streams, rows in ((1, 16), (2, 16), (2, 8), (3, 8), (4, 16)):
         X = sloshgpu.Batch(B, trks)
         X.upload()
         X.set_path(sloshgpu.PATH_FUSED, rows)
@@ -327,6 +328,7 @@ def test_two_stream_halves_bitwise(tmp_path):
         assert X.streams == streams
         X.step(300)
         ms, cnt = X.step_timed(100)
+        X.set_streams(5 - streams if streams > 1 else 1)
         X.step(max(X.nsteps))
         X.sync()
         res.append([[X.get(s, f) for f in helpers.FIELDS] for s in range(len(trks))])
