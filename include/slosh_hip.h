@@ -135,8 +135,33 @@ void slg_batch_close(slg_batch *x);
  * 34-110: per cell the largest envelope below 50 ft, 999 where none),
  * mom[jmxb][imxb]; computed on the device, either output may be NULL. */
 int slg_batch_envelopes(slg_batch *x, short *env, short *mom, char *err, int errlen);
-/* Datums a track carries for the env trailer (ReadTrkFile, slosh2.c:1885-1975). */
+/* Datums a track carries for the env trailer (ReadTrkFile, slosh2.c:1885-1975),
+ * after the surge-only InitWater_TideModeOverride (slosh2.c:408-482, 2238). */
 int slg_track_datums(const char *trk_path, float *ht1, float *ht2);
+
+/* ---- rex v2 files of batched runs ---------------------------------------
+ * The frames' bit streams are encoded on the device: the basin loop of
+ * RexSaveStep / RexSaveEnv (rex.c:453-499, 554-620) with the run-length
+ * records of Stuff2_xxx (pack.c:166-212), for storms s0 .. s0+ns-1 at once.
+ * field "HHB": HHB = min(HB, HMX) of the last step, the array TMSTPC2 hands
+ * the C driver for a rex frame (tmstpc2.f:74-85; needs HB before FILTER, so
+ * the last step ran on the staged path or did not filter); "HMX": the
+ * envelope, the last frame RexSaveEnv writes.  Frame k goes to buf + k*cap
+ * and its length to nbytes[k]; cap = (18*(imxb-1)*(jmxb-1) + 7) / 8 always
+ * suffices (SLG_ELIMIT if a frame does not fit). */
+int slg_batch_rex_frames(slg_batch *x, const char *field, int s0, int ns, unsigned char *buf, long cap,
+                         int *nbytes, char *err, int errlen);
+/* The file around the frames, laid out as the reference's rex writer does:
+ * open = RexOpen (version 2) + RexSaveHeader (rex.c:150-180, 264-379); frame =
+ * RexSaveStep's record (rex.c:387-447: lat, lon, wind speed, wind direction,
+ * delp, size as the C driver holds them, and the model clock in seconds since
+ * 1970); env = RexSaveEnv (rex.c:505-650, with Data_WriteTrk's copy of the
+ * track file, rex.c:57-230, and RexSetMinMaxHt, rex.c:232-256). */
+typedef struct slg_rex slg_rex;
+int slg_rex_open(const char *path, const char *comment, const char *abbrev, int imxb, int jmxb, slg_rex **out);
+int slg_rex_frame(slg_rex *r, const double track[6], double clock, const unsigned char *bits, int nbytes);
+int slg_rex_env(slg_rex *r, const char *trk_path, const unsigned char *bits, int nbytes);
+void slg_rex_close(slg_rex *r);
 /* Write an env file in EnvSave's layout (slosh2.c:1611-1671). */
 int slg_write_env(const char *path, int imxb, int jmxb, const char *comment, const short *vals, float ht1,
                   float ht2);

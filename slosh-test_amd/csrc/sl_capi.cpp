@@ -1221,6 +1221,85 @@ extern "C" int slg_batch_envelopes(slg_batch *X, short *env, short *mom, char *e
                         : batch_envelopes<float>(X, env, mom, err, errlen);
 }
 
+// rex frames of storms s0 .. s0+ns-1, in chunks whose scratch (2N+1 words a
+// storm for run starts and records) stays near 256 MB
+template <class R>
+static int batch_rex(slg_batch *X, bool hhb, int s0, int ns, unsigned char *buf, long cap, int *nbytes, char *err,
+                     int errlen)
+{
+   slg_basin *B = X->b;
+   const SlRunDevT<R> &D = run(X, R());
+   const long N = (long) (B->h.imxb - 1) * (B->h.jmxb - 1);
+   const R *hbs = nullptr;
+   if (hhb) hbs = X->hbs_buf ? D.hb1 : D.hb0;
+   const size_t per = (size_t) (2 * N + 1) * 4;
+   int chunk = (int) std::max<size_t>(1, ((size_t) 256 << 20) / per);
+   if (chunk > ns) chunk = ns;
+   unsigned *scr = nullptr;
+   unsigned char *dout = nullptr;
+   int *dnb = nullptr;
+   int rc = 0;
+   HIPOK(hipStreamSynchronize(X->st));
+   if (hipMalloc((void **) &scr, per * chunk) != hipSuccess || hipMalloc((void **) &dout, (size_t) cap * chunk) !=
+       hipSuccess || hipMalloc((void **) &dnb, sizeof(int) * chunk) != hipSuccess)
+      rc = SLG_EHIP;
+   for (int c = 0; c < ns && !rc; c += chunk) {
+      const int n = std::min(chunk, ns - c);
+      int r = dev_rex(&img(B, R()).dev, &D, hbs, img(B, R()).zbd, s0 + c, n, dout, cap, dnb, scr, X->st);
+      if (r || hipMemcpyAsync(nbytes + c, dnb, sizeof(int) * n, hipMemcpyDeviceToHost, X->st) != hipSuccess ||
+          hipStreamSynchronize(X->st) != hipSuccess) {
+         rc = SLG_EHIP;
+         break;
+      }
+      for (int k = 0; k < n && !rc; k++) {
+         if (nbytes[c + k] < 0) {
+            rc = SLG_ELIMIT;
+            break;
+         }
+         if (hipMemcpy(buf + (size_t) (c + k) * cap, dout + (size_t) k * cap, (size_t) nbytes[c + k],
+                       hipMemcpyDeviceToHost) != hipSuccess)
+            rc = SLG_EHIP;
+      }
+   }
+   if (scr) (void) hipFree(scr);
+   if (dout) (void) hipFree(dout);
+   if (dnb) (void) hipFree(dnb);
+   if (rc == SLG_ELIMIT) seterr(err, errlen, "a rex frame is longer than cap = %s%d bytes", "", (int) cap);
+   else if (rc) seterr(err, errlen, "%s", "rex frame kernel or copy failed");
+   return rc;
+}
+
+// rex v2 frames of storms s0 .. s0+ns-1 from HHB of the last step or HMX
+extern "C" int slg_batch_rex_frames(slg_batch *X, const char *field, int s0, int ns, unsigned char *buf, long cap,
+                                    int *nbytes, char *err, int errlen)
+{
+   if (!X->dmem) {
+      seterr(err, errlen, "%s", "batch not uploaded");
+      return SLG_EINPUT;
+   }
+   if (!field || !buf || !nbytes || s0 < 0 || ns < 0 || s0 + ns > X->nstorm || cap < 1) {
+      seterr(err, errlen, "%s", "bad storm range or buffer");
+      return SLG_EINPUT;
+   }
+   const bool hhb = !strcmp(field, "HHB");
+   if (!hhb && strcmp(field, "HMX")) {
+      seterr(err, errlen, "unknown rex field %s (HHB or HMX)", field);
+      return SLG_EINPUT;
+   }
+   if (hhb) {
+      bool ok = X->step > 0 && X->hbs_buf >= 0;
+      for (int s = s0; s < s0 + ns && ok; s++) ok = X->sc[s].nsteps >= X->step;
+      if (!ok) {
+         seterr(err, errlen, "%s", "HHB needs HB before FILTER of the last step of every storm: step it on the "
+                                   "staged path, or a step that did not filter");
+         return SLG_EINPUT;
+      }
+   }
+   if (ns == 0) return 0;
+   return X->prec == 64 ? batch_rex<double>(X, hhb, s0, ns, buf, cap, nbytes, err, errlen)
+                        : batch_rex<float>(X, hhb, s0, ns, buf, cap, nbytes, err, errlen);
+}
+
 // Datums of a run's track for the env trailer (ReadTrkFile, slosh2.c:1885-1975):
 // OceanDatum and LakeDatum; with a CanalDatum line, LakeDatum and CanalDatum.
 extern "C" int slg_track_datums(const char *trk_path, float *ht1, float *ht2)
@@ -1240,6 +1319,7 @@ extern "C" int slg_track_datums(const char *trk_path, float *ht1, float *ht2)
    const bool canal = cn > -100;   // Okeechobee-style datums: lake and canal
    *ht1 = canal ? lk : oc;
    *ht2 = canal ? cn : lk;
+   sl_datum_override(ht1, ht2);    // slosh2.c:2238, a surge-only run
    return 0;
 }
 

@@ -122,6 +122,10 @@ int sl_force1(double pdrop, double rmax, double *vmax, double alto, int ncatg,
               double DELP[SL_NTAB + 1]);
 double sl_intwnd(double pdrop, double rmax, double alto);
 
+/* The datums of a surge-only run after InitWater_TideModeOverride
+ * (slosh2.c:345-482, f_tide 0), as the env and rex trailers carry them. */
+void sl_datum_override(float *ht1, float *ht2);
+
 #ifdef __cplusplus
 }
 #endif

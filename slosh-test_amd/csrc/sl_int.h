@@ -130,6 +130,25 @@ static inline int dev_envelopes(const SlBasinDevT<double> *B, const SlRunDevT<do
    return sl_dev_envelopes_f64(B, R, zbd, s0, ns, env, mom, st);
 }
 
+extern "C" int sl_dev_rex_f32(const SlBasinDevT<float> *B, const SlRunDevT<float> *R, const float *hbs,
+                              const double *zbd, int s0, int ns, unsigned char *out, long cap, int *nbytes,
+                              unsigned *scr, hipStream_t st);
+extern "C" int sl_dev_rex_f64(const SlBasinDevT<double> *B, const SlRunDevT<double> *R, const double *hbs,
+                              const double *zbd, int s0, int ns, unsigned char *out, long cap, int *nbytes,
+                              unsigned *scr, hipStream_t st);
+static inline int dev_rex(const SlBasinDevT<float> *B, const SlRunDevT<float> *R, const float *hbs,
+                          const double *zbd, int s0, int ns, unsigned char *out, long cap, int *nbytes, unsigned *scr,
+                          hipStream_t st)
+{
+   return sl_dev_rex_f32(B, R, hbs, zbd, s0, ns, out, cap, nbytes, scr, st);
+}
+static inline int dev_rex(const SlBasinDevT<double> *B, const SlRunDevT<double> *R, const double *hbs,
+                          const double *zbd, int s0, int ns, unsigned char *out, long cap, int *nbytes, unsigned *scr,
+                          hipStream_t st)
+{
+   return sl_dev_rex_f64(B, R, hbs, zbd, s0, ns, out, cap, nbytes, scr, st);
+}
+
 // storm schedule of one run (host, double): rows kept when keep_rows
 int sl_make_schedule(const slg_basin *B, const char *trk, storm_sched &S, bool keep_rows, int prec, char *err,
                      int errlen);

@@ -1828,6 +1828,124 @@ __global__ void k_advance(SlRunDev R)
    if (s < R.nstorm && R.cur[s] < R.nsteps[s]) R.cur[s]++;
 }
 
+// ------------------------------------------------------------------ rex frames
+// One rex v2 frame per storm: the values of RexSaveStep's basin loop
+// (rex.c:453-499, cells I-outer and J-inner over (IMXB-1) x (JMXB-1)) as the
+// 10-bit codes of Stuff2_xxx (pack.c:185-190), grouped into run-length records
+// of at most 256 (8 bits run-1, then the code), packed most significant bit
+// first.  A workgroup of 1024 threads per storm finds the run starts with a
+// block scan, then the records (a run of L values makes ceil(L/256)), then
+// writes each output byte from the one or two records under it.
+#define RX_T 1024
+
+// code of frame cell k: HHB = HMX where HMX < HB, else HB (tmstpc2.f:74-85),
+// or HMX alone (hbs == nullptr); in double, as the C driver's arrays
+__device__ __forceinline__ int rex_code(const SlBasinDev &B, const real *hbs, const real *hmx, const double *zbd,
+                                        int k, int J1)
+{
+   const int i = k / J1, j = k - i * J1;
+   const uint32_t p = IDX(B, i + 1, j + 1);
+   real h = hmx[p];
+   if (hbs) {
+      const real b = hbs[p];
+      if (!(h < b)) h = b;
+   }
+   const double hd = (double) h, zd = zbd[p];
+   int v = (hd + zd == 0.0) ? 999 : (int) (hd * 10 + .5);
+   if (v < -320) v = -320;
+   else if (v > 700 && v != 999) v = 700;
+   return (v == 999 ? 701 : v) + 320;
+}
+
+// exclusive prefix sum over the workgroup's 1024 threads; *total = the sum
+__device__ static int blk_scan(int v, int *sh, int *total)
+{
+   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+   int x = v;
+   for (int o = 1; o < 64; o <<= 1) {
+      const int y = __shfl_up(x, o);
+      if (lane >= o) x += y;
+   }
+   if (lane == 63) sh[w] = x;
+   __syncthreads();
+   if (w == 0) {
+      int t = lane < RX_T / 64 ? sh[lane] : 0;
+      for (int o = 1; o < RX_T / 64; o <<= 1) {
+         const int y = __shfl_up(t, o);
+         if (lane >= o) t += y;
+      }
+      if (lane < RX_T / 64) sh[lane] = t;
+   }
+   __syncthreads();
+   const int base = w ? sh[w - 1] : 0;
+   *total = sh[RX_T / 64 - 1];
+   __syncthreads();
+   return base + x - v;
+}
+
+// storms s0 + blockIdx.x; scr holds 2N+1 words per storm; frame bytes to
+// out + blockIdx.x * cap, their count to nbytes (-1: over cap)
+__global__ __launch_bounds__(RX_T) void k_rex(SlBasinDev B, SlRunDev R, const real *hbs, const double *zbd,
+                                             int s0, uint8_t *out, long cap, int *nbytes, uint32_t *scr)
+{
+   __shared__ int sh[RX_T / 64];
+   const int s = s0 + blockIdx.x, t = threadIdx.x;
+   const int J1 = B.jmxb - 1, N = (B.imxb - 1) * J1;
+   const size_t so = (size_t) s * R.fstride;
+   const real *hx = R.hmx + so, *hb = hbs ? hbs + so : nullptr;
+   uint32_t *starts = scr + (size_t) blockIdx.x * (2 * (size_t) N + 1), *rec = starts + N + 1;
+   uint8_t *o = out + (size_t) blockIdx.x * cap;
+   int nrun = 0, tot;
+   for (int c = 0; c < N; c += RX_T) {
+      const int k = c + t;
+      int f = 0;
+      if (k < N) f = k == 0 || rex_code(B, hb, hx, zbd, k, J1) != rex_code(B, hb, hx, zbd, k - 1, J1);
+      const int pos = blk_scan(f, sh, &tot);
+      if (f) starts[nrun + pos] = (uint32_t) k;
+      nrun += tot;
+   }
+   if (t == 0) starts[nrun] = (uint32_t) N;
+   __syncthreads();
+   int nrec = 0;
+   for (int c = 0; c < nrun; c += RX_T) {
+      const int r = c + t;
+      int len = 0, n = 0;
+      if (r < nrun) {
+         len = (int) (starts[r + 1] - starts[r]);
+         n = (len + 255) >> 8;
+      }
+      const int pos = blk_scan(n, sh, &tot);
+      if (r < nrun) {
+         const uint32_t code = (uint32_t) rex_code(B, hb, hx, zbd, (int) starts[r], J1);
+         for (int q = 0; q < n; q++) {
+            const int l = len - 256 * q < 256 ? len - 256 * q : 256;
+            rec[nrec + pos + q] = ((uint32_t) (l - 1) << 10) | code;
+         }
+      }
+      nrec += tot;
+   }
+   __syncthreads();
+   const long nbits = 18L * nrec, nb = (nbits + 7) / 8;
+   if (nb > cap) {
+      if (t == 0) nbytes[blockIdx.x] = -1;
+      return;
+   }
+   for (long b = t; b < nb; b += RX_T) {
+      unsigned v = 0;
+      for (int m = 0; m < 8; m++) {
+         const long q = 8 * b + m;
+         unsigned bit = 0;
+         if (q < nbits) {
+            const long r = q / 18;
+            bit = (rec[r] >> (17 - (int) (q - 18 * r))) & 1u;
+         }
+         v = (v << 1) | bit;
+      }
+      o[b] = (uint8_t) v;
+   }
+   if (t == 0) nbytes[blockIdx.x] = (int) nb;
+}
+
 }   // namespace SL_NS
 
 using namespace SL_NS;
@@ -1848,6 +1966,21 @@ extern "C" int SL_ENVS(const SlBasinDev *B, const SlRunDev *R, const double *zbd
 {
    const int n = B->imxb * B->jmxb;
    hipLaunchKernelGGL(k_envelopes, dim3((n + 255) / 256), dim3(256), 0, st, *B, *R, zbd, s0, ns, env, mom);
+   hipError_t e = hipGetLastError();
+   return e == hipSuccess ? 0 : -(int) e;
+}
+
+#ifdef SL_F64
+#define SL_REX sl_dev_rex_f64
+#else
+#define SL_REX sl_dev_rex_f32
+#endif
+// rex frames of storms s0 .. s0+ns-1 from HHB (hbs: the HB buffer before
+// FILTER) or from HMX (hbs == nullptr); scr: ns * (2N+1) words
+extern "C" int SL_REX(const SlBasinDev *B, const SlRunDev *R, const real *hbs, const double *zbd, int s0, int ns,
+                      unsigned char *out, long cap, int *nbytes, unsigned *scr, hipStream_t st)
+{
+   hipLaunchKernelGGL(k_rex, dim3(ns), dim3(RX_T), 0, st, *B, *R, hbs, zbd, s0, out, cap, nbytes, scr);
    hipError_t e = hipGetLastError();
    return e == hipSuccess ? 0 : -(int) e;
 }

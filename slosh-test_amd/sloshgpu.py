@@ -53,6 +53,15 @@ def lib():
         L.slg_batch_autotune.argtypes = [vp, i, ctypes.POINTER(i), ctypes.POINTER(i), vp, cp, i]
         L.slg_batch_envelopes.argtypes = [vp, vp, vp, cp, i]
         L.slg_track_datums.argtypes = [cp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]
+        L.slg_batch_rex_frames.argtypes = [vp, cp, i, i, vp, ctypes.c_long, vp, cp, i]
+        L.slg_batch_track.argtypes = [vp, i, i, vp]
+        L.slg_basin_field.argtypes = [vp, cp, vp]
+        L.slg_batch_start_date.argtypes = [vp, i, vp]
+        L.slg_rex_open.argtypes = [cp, cp, cp, i, i, ctypes.POINTER(vp)]
+        L.slg_rex_frame.argtypes = [vp, vp, ctypes.c_double, vp, i]
+        L.slg_rex_env.argtypes = [vp, cp, vp, i]
+        L.slg_rex_close.argtypes = [vp]
+        L.slg_rex_close.restype = None
         L.slg_write_env.argtypes = [cp, i, i, cp, vp, ctypes.c_float, ctypes.c_float]
         L.slg_server_open.argtypes = [i, ctypes.c_double, ctypes.POINTER(vp), cp, i]
         L.slg_server_add_basin.argtypes = [vp, cp, cp, i, i, i, ctypes.POINTER(i), cp, i]
@@ -91,6 +100,13 @@ class Basin:
         self.imxb, self.jmxb, self.nbcpt = a.value, b.value, c.value
         L.slg_basin_sides(self.h, None, None, ctypes.byref(c))
         self.nsides = c.value
+
+    def field(self, name: str) -> np.ndarray:
+        """Static field "ZB", "ZBM" or "YLT" as double [jmxb, imxb]."""
+        out = np.zeros((self.jmxb, self.imxb))
+        if lib().slg_basin_field(self.h, name.encode(), out.ctypes.data_as(ctypes.c_void_p)):
+            raise SloshError("no basin field %s" % name)
+        return out
 
     @property
     def cells(self):
@@ -208,6 +224,35 @@ class Batch:
                                        err, 512), err)
         return env, mom
 
+    def track(self, storm: int, n: int):
+        """What TMSTPM returns after step n (batches of up to 8 runs): ITIME,
+        MHALT, DELT, storm lat, storm lon, DELP, size, speed, direction."""
+        out = np.zeros(9)
+        if lib().slg_batch_track(self.h, storm, n, out.ctypes.data_as(ctypes.c_void_p)):
+            raise SloshError("no track values for storm %d step %d" % (storm, n))
+        return out
+
+    def start_date(self, storm: int):
+        """year, month, day, hour, minute of the run's start (INITAL)."""
+        out = np.zeros(5, dtype=np.int32)
+        if lib().slg_batch_start_date(self.h, storm, out.ctypes.data_as(ctypes.c_void_p)):
+            raise SloshError("no start date for storm %d" % storm)
+        return [int(v) for v in out]
+
+    def rex_frames(self, field="HHB", s0=0, ns=None):
+        """rex v2 frame bit streams of storms s0..s0+ns-1, encoded on the
+        device from HHB = min(HB, HMX) of the last step or from HMX; a list of
+        bytes objects."""
+        ns = self.n - s0 if ns is None else ns
+        nj, ni = self.basin.jmxb, self.basin.imxb
+        cap = (18 * (ni - 1) * (nj - 1) + 7) // 8
+        buf = np.zeros(ns * cap, dtype=np.uint8)
+        nb = np.zeros(ns, dtype=np.int32)
+        err = ctypes.create_string_buffer(512)
+        _chk(lib().slg_batch_rex_frames(self.h, field.encode(), s0, ns, buf.ctypes.data_as(ctypes.c_void_p), cap,
+                                        nb.ctypes.data_as(ctypes.c_void_p), err, 512), err)
+        return [bytes(buf[k * cap:k * cap + nb[k]]) for k in range(ns)]
+
     def close(self):
         if self.h:
             lib().slg_batch_close(self.h)
@@ -227,6 +272,34 @@ def write_env(path: str, vals: np.ndarray, ht1: float, ht2: float, comment: str 
     if lib().slg_write_env(path.encode(), v.shape[1], v.shape[0], comment.encode(),
                            v.ctypes.data_as(ctypes.c_void_p), ht1, ht2):
         raise SloshError("cannot write %s" % path)
+
+
+class Rex:
+    """A rex v2 file written as the reference's rex writer lays it out; the
+    frames' bit streams come from Batch.rex_frames."""
+
+    def __init__(self, path: str, comment: str, abbrev: str, imxb: int, jmxb: int):
+        h = ctypes.c_void_p()
+        if lib().slg_rex_open(path.encode(), comment.encode(), abbrev.encode(), imxb, jmxb, ctypes.byref(h)):
+            raise SloshError("cannot open rex file %s" % path)
+        self.h = h
+
+    def frame(self, track6, clock: float, bits: bytes):
+        t = np.ascontiguousarray(track6, dtype=np.float64)
+        b = np.frombuffer(bits, dtype=np.uint8) if bits else np.zeros(1, np.uint8)
+        if lib().slg_rex_frame(self.h, t.ctypes.data_as(ctypes.c_void_p), clock, b.ctypes.data_as(ctypes.c_void_p),
+                               len(bits)):
+            raise SloshError("rex frame write failed")
+
+    def env(self, trk: str, bits: bytes):
+        b = np.frombuffer(bits, dtype=np.uint8) if bits else np.zeros(1, np.uint8)
+        if lib().slg_rex_env(self.h, trk.encode(), b.ctypes.data_as(ctypes.c_void_p), len(bits)):
+            raise SloshError("rex envelope write failed (track file %s)" % trk)
+
+    def close(self):
+        if self.h:
+            lib().slg_rex_close(self.h)
+            self.h = None
 
 
 class Server:
