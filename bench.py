@@ -150,6 +150,34 @@ def gpu_run(dta, bnt, trks, steps, warmup, path=None, rows=16, graph=0, events=T
     return dict(dt=dt, cells=cells, ms=ms, cnt=cnt, sched_s=t_sched, tune=tune, streams=nstreams)
 
 
+# ----------------------------------------------------------------------- rex
+def rex_frames_bench(dta, bnt, trks, reps=5):
+    """rex v2 frames of every storm of the ensemble encoded on the device
+    (slg_batch_rex_frames: HHB of the last step, run-length coded, copied back)
+    after 20 staged steps; time per frame set, over reps calls after one
+    untimed call."""
+    B = sloshgpu.Basin(dta, bnt)
+    X = sloshgpu.Batch(B, trks)
+    X.upload()
+    X.set_path(sloshgpu.PATH_STAGED, 16)
+    X.step(20)
+    X.sync()
+    fr = X.rex_frames("HHB")
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fr = X.rex_frames("HHB")
+    dt = (time.perf_counter() - t0) / reps
+    n = X.n
+    cells = (B.imxb - 1) * (B.jmxb - 1)
+    X.close()
+    B.close()
+    nbytes = sum(len(f) for f in fr)
+    return {"storms": n, "ms_per_frame_set": dt * 1e3, "frames_per_s": n / dt,
+            "bytes_out_per_frame": nbytes / n, "cells_per_frame": cells,
+            "note": "HHB = min(HB, HMX) per cell (8 B read in FP32) to rex v2 run-length records, "
+                    "copied to the host; includes the host copy"}
+
+
 # -------------------------------------------------------------------- uneven
 def uneven(root, nstorm, steps, warmup, graph=20):
     """nstorm storms over the six basins (share proportional to cells), one
@@ -330,8 +358,9 @@ def main():
                     help="timed steps of the batched run (48 h runs = 5760 steps; the autotuner's trials "
                          "take 80 and the warm-up W of them)")
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--parts", default="batched,plain,uneven,serve,cpu",
-                    help="comma list of batched, plain, uneven, serve, cpu (and copy: the counter calibration copy)")
+    ap.add_argument("--parts", default="batched,rex,plain,uneven,serve,cpu",
+                    help="comma list of batched, rex, plain, uneven, serve, cpu (and copy: the counter "
+                         "calibration copy)")
     ap.add_argument("--storms", type=int, default=1024)
     ap.add_argument("--plain-steps", type=int, default=8600, help="timed steps of the plain run (72 h = 8640)")
     ap.add_argument("--cpu-steps", type=int, default=2000)
@@ -420,6 +449,10 @@ def main():
             out["roofline"]["storms_per_launch"] = per_launch
             out["kernel_avg_ms"] = avg
             out["step_roofline_frac"] = rate * b_alg_step(a.storms) / (HBM_PEAK_GBS * 1e9)
+        if "rex" in parts:
+            rx = rex_frames_bench(paths["dta"], paths["bntline"], trks)
+            log("rex: %d frames in %.2f ms" % (rx["storms"], rx["ms_per_frame_set"]))
+            out["rex"] = rx
         if "plain" in parts:
             # plain single storm: elliptic 250 x 300, spline track (fused; and graphs)
             b2, t2 = synth.config2()

@@ -1221,6 +1221,37 @@ extern "C" int slg_batch_envelopes(slg_batch *X, short *env, short *mom, char *e
                         : batch_envelopes<float>(X, env, mom, err, errlen);
 }
 
+static void rex_free(slg_batch *X)
+{
+   for (void *p : {X->rex_scr, X->rex_out, X->rex_pack, X->rex_nb, X->rex_off})
+      if (p) (void) hipFree(p);
+   if (X->rex_host) (void) hipHostFree(X->rex_host);
+   X->rex_scr = X->rex_out = X->rex_pack = X->rex_nb = X->rex_off = nullptr;
+   X->rex_host = nullptr;
+   X->rex_chunk = 0;
+   X->rex_cap = 0;
+}
+
+// the batch's rex buffers for chunk storms of cap bytes (per: scratch bytes a
+// storm), allocated on first use and when a call needs more
+static bool rex_buffers(slg_batch *X, int chunk, long cap, size_t per)
+{
+   if (X->rex_chunk >= chunk && X->rex_cap >= cap) return true;
+   (void) hipStreamSynchronize(X->st);
+   rex_free(X);
+   const size_t fr = (size_t) cap * chunk;
+   if (hipMalloc(&X->rex_scr, per * chunk) != hipSuccess || hipMalloc(&X->rex_out, fr) != hipSuccess ||
+       hipMalloc(&X->rex_pack, fr) != hipSuccess || hipMalloc(&X->rex_nb, sizeof(int) * chunk) != hipSuccess ||
+       hipMalloc(&X->rex_off, sizeof(long) * chunk) != hipSuccess ||
+       hipHostMalloc((void **) &X->rex_host, fr, 0) != hipSuccess) {
+      rex_free(X);
+      return false;
+   }
+   X->rex_chunk = chunk;
+   X->rex_cap = cap;
+   return true;
+}
+
 // rex frames of storms s0 .. s0+ns-1, in chunks whose scratch (2N+1 words a
 // storm for run starts and records) stays near 256 MB
 template <class R>
@@ -1235,14 +1266,17 @@ static int batch_rex(slg_batch *X, bool hhb, int s0, int ns, unsigned char *buf,
    const size_t per = (size_t) (2 * N + 1) * 4;
    int chunk = (int) std::max<size_t>(1, ((size_t) 256 << 20) / per);
    if (chunk > ns) chunk = ns;
-   unsigned *scr = nullptr;
-   unsigned char *dout = nullptr;
-   int *dnb = nullptr;
+   if (!rex_buffers(X, chunk, cap, per)) {
+      seterr(err, errlen, "%s", "no device memory for the rex frame buffers");
+      return SLG_EHIP;
+   }
+   unsigned *scr = (unsigned *) X->rex_scr;
+   unsigned char *dout = (unsigned char *) X->rex_out, *dpack = (unsigned char *) X->rex_pack;
+   int *dnb = (int *) X->rex_nb;
+   long *doff = (long *) X->rex_off;
    int rc = 0;
    HIPOK(hipStreamSynchronize(X->st));
-   if (hipMalloc((void **) &scr, per * chunk) != hipSuccess || hipMalloc((void **) &dout, (size_t) cap * chunk) !=
-       hipSuccess || hipMalloc((void **) &dnb, sizeof(int) * chunk) != hipSuccess)
-      rc = SLG_EHIP;
+   std::vector<long> off(chunk);
    for (int c = 0; c < ns && !rc; c += chunk) {
       const int n = std::min(chunk, ns - c);
       int r = dev_rex(&img(B, R()).dev, &D, hbs, img(B, R()).zbd, s0 + c, n, dout, cap, dnb, scr, X->st);
@@ -1251,19 +1285,24 @@ static int batch_rex(slg_batch *X, bool hhb, int s0, int ns, unsigned char *buf,
          rc = SLG_EHIP;
          break;
       }
-      for (int k = 0; k < n && !rc; k++) {
-         if (nbytes[c + k] < 0) {
-            rc = SLG_ELIMIT;
-            break;
-         }
-         if (hipMemcpy(buf + (size_t) (c + k) * cap, dout + (size_t) k * cap, (size_t) nbytes[c + k],
-                       hipMemcpyDeviceToHost) != hipSuccess)
-            rc = SLG_EHIP;
+      long tot = 0;
+      for (int k = 0; k < n; k++) {
+         if (nbytes[c + k] < 0) rc = SLG_ELIMIT;
+         off[k] = tot;
+         tot += nbytes[c + k] > 0 ? nbytes[c + k] : 0;
       }
+      if (rc) break;
+      // the chunk's frames back to back on the device, then one copy
+      if (hipMemcpyAsync(doff, off.data(), sizeof(long) * n, hipMemcpyHostToDevice, X->st) != hipSuccess ||
+          sl_dev_rex_pack(dout, cap, dnb, doff, n, dpack, X->st) ||
+          hipMemcpyAsync(X->rex_host, dpack, (size_t) tot, hipMemcpyDeviceToHost, X->st) != hipSuccess ||
+          hipStreamSynchronize(X->st) != hipSuccess) {
+         rc = SLG_EHIP;
+         break;
+      }
+      for (int k = 0; k < n; k++)
+         memcpy(buf + (size_t) (c + k) * cap, X->rex_host + off[k], (size_t) nbytes[c + k]);
    }
-   if (scr) (void) hipFree(scr);
-   if (dout) (void) hipFree(dout);
-   if (dnb) (void) hipFree(dnb);
    if (rc == SLG_ELIMIT) seterr(err, errlen, "a rex frame is longer than cap = %s%d bytes", "", (int) cap);
    else if (rc) seterr(err, errlen, "%s", "rex frame kernel or copy failed");
    return rc;
@@ -1395,6 +1434,7 @@ extern "C" void slg_batch_close(slg_batch *X)
 {
    if (!X) return;
    drop_graph(X);
+   rex_free(X);
    for (int p = 0; p < SL_MAXSPLIT - 1; p++)
       if (X->stx[p]) {
          (void) hipStreamSynchronize(X->stx[p]);

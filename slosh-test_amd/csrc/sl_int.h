@@ -107,6 +107,13 @@ struct slg_batch {
    // part's latency-bound cascade replay overlaps the others' stencil kernels.
    // Part 0 runs on st, part p > 0 on stx[p - 1].
    int nsplit = 1;
+   // rex frame buffers, kept from one slg_batch_rex_frames call to the next:
+   // scratch, frames, packed frames (device), sizes and offsets, and pinned
+   // host staging for the packed copy; for rex_chunk storms of rex_cap bytes
+   void *rex_scr = nullptr, *rex_out = nullptr, *rex_pack = nullptr, *rex_nb = nullptr, *rex_off = nullptr;
+   unsigned char *rex_host = nullptr;
+   int rex_chunk = 0;
+   long rex_cap = 0;
    hipStream_t stx[SL_MAXSPLIT - 1] = {};
    hipEvent_t ev_join[SL_MAXSPLIT - 1] = {};
    hipEvent_t ev_phase[SL_MAXSPLIT][SLG_NKERN + 1] = {};   // marks of each part's first step
@@ -136,6 +143,8 @@ extern "C" int sl_dev_rex_f32(const SlBasinDevT<float> *B, const SlRunDevT<float
 extern "C" int sl_dev_rex_f64(const SlBasinDevT<double> *B, const SlRunDevT<double> *R, const double *hbs,
                               const double *zbd, int s0, int ns, unsigned char *out, long cap, int *nbytes,
                               unsigned *scr, hipStream_t st);
+extern "C" int sl_dev_rex_pack(const unsigned char *src, long cap, const int *nbytes, const long *off, int ns,
+                               unsigned char *dst, hipStream_t st);
 static inline int dev_rex(const SlBasinDevT<float> *B, const SlRunDevT<float> *R, const float *hbs,
                           const double *zbd, int s0, int ns, unsigned char *out, long cap, int *nbytes, unsigned *scr,
                           hipStream_t st)

@@ -1946,6 +1946,16 @@ __global__ __launch_bounds__(RX_T) void k_rex(SlBasinDev B, SlRunDev R, const re
    if (t == 0) nbytes[blockIdx.x] = (int) nb;
 }
 
+// frame k (nb[k] bytes at src + k*cap) to dst + off[k]: the frames of a chunk
+// back to back, for one copy to the host
+__global__ void k_rex_pack(const uint8_t *src, long cap, const int *nb, const long *off, uint8_t *dst)
+{
+   const int k = blockIdx.x;
+   const uint8_t *s = src + (size_t) k * cap;
+   uint8_t *d = dst + off[k];
+   for (long b = threadIdx.x; b < nb[k]; b += blockDim.x) d[b] = s[b];
+}
+
 }   // namespace SL_NS
 
 using namespace SL_NS;
@@ -1984,6 +1994,17 @@ extern "C" int SL_REX(const SlBasinDev *B, const SlRunDev *R, const real *hbs, c
    hipError_t e = hipGetLastError();
    return e == hipSuccess ? 0 : -(int) e;
 }
+
+#ifndef SL_F64
+// frames of one chunk back to back (both precisions use this one)
+extern "C" int sl_dev_rex_pack(const unsigned char *src, long cap, const int *nbytes, const long *off, int ns,
+                               unsigned char *dst, hipStream_t st)
+{
+   hipLaunchKernelGGL(k_rex_pack, dim3(ns), dim3(256), 0, st, src, cap, nbytes, off, dst);
+   hipError_t e = hipGetLastError();
+   return e == hipSuccess ? 0 : -(int) e;
+}
+#endif
 
 extern "C" int SL_LAUNCH(const SlBasinDev *B, const SlRunDev *R, int smpt, int filt, int dyn, int parity,
                          int path, int bth, hipStream_t st, hipEvent_t *ev)
