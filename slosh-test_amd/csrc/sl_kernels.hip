@@ -967,10 +967,12 @@ __device__ static void mntmbd(const SlBasinDev &B, const real *P, const real *T,
 
 // Momentum point (i,j) with F_MOM set; hb gives the final HB of the step on the
 // four squares around it.  Writes UB/VB at (i,j) and at the boundary points
-// this point owns.
+// this point owns.  ect, edt: the metrics ELPCT(I), ELPDT(I) of the point's
+// column (loaded once by a caller that visits several rows of one column).
 template <class HB, class Z>
 __device__ __forceinline__ void momntm_pt(const SlBasinDev &B, const real *P, const real *T, uint32_t fl,
-                                          int i, int j, const HB &hb, const Z &z, real *ub, real *vb)
+                                          int i, int j, const HB &hb, const Z &z, real *ub, real *vb, real ect,
+                                          real edt)
 {
    const uint32_t p = IDX(B, i, j);
    const real h1 = hb(i - 1, j - 1), h2 = hb(i, j - 1), h3 = hb(i - 1, j), h4 = hb(i, j);
@@ -1075,10 +1077,10 @@ __device__ __forceinline__ void momntm_pt(const SlBasinDev &B, const real *P, co
             }
          }
          real fxbp, fybp;
-         real xr = B.elpct[i - 1] * B.cost[j - 1], yr = B.elpdt[i - 1] * B.sint[j - 1];
+         real xr = ect * B.cost[j - 1], yr = edt * B.sint[j - 1];
          frcpnt(B, P, T, xr, yr, fxbp, fybp, ncatg, cshltr, 1, F);
-         xr = B.elpdt[i - 1] * B.cost[j - 1];
-         yr = B.elpct[i - 1] * B.sint[j - 1];
+         xr = edt * B.cost[j - 1];
+         yr = ect * B.sint[j - 1];
          fxb = (xr * fxbp + yr * fybp);
          fyb = (xr * fybp - yr * fxbp);
          xttx = P[P_DELT] * fxb;
@@ -1587,7 +1589,8 @@ __global__ void __launch_bounds__(TX * TY) k_momntm(SlBasinDev B, SlRunDev R, St
    if (!(fl & F_MOM)) return;
    const size_t so = (size_t) s * R.fstride;
    const real *P = par(R, s, step);
-   momntm_pt(B, P, tabv(R, s, P), fl, i, j, GF{A.hc + so, B.pitch}, ZG{B.zb, B.zbm, B.pitch}, R.ub + so, R.vb + so);
+   momntm_pt(B, P, tabv(R, s, P), fl, i, j, GF{A.hc + so, B.pitch}, ZG{B.zb, B.zbm, B.pitch}, R.ub + so, R.vb + so,
+             B.elpct[i - 1], B.elpdt[i - 1]);
 }
 
 // ------------------------------------------------------------ fused stage B
@@ -1765,11 +1768,16 @@ k_stepB(SlBasinDev B, SlRunDev R, StepArgs A, int ntx, int nty, int dyn)
                filt ? RF::w : RS::w};
    real *ub = R.ub + so, *vb = R.vb + so;
    for (int e = tid; e < TX * TH; e += 256) {
-      const int gi = I0 + e % TX, gj = J0 + e / TX;
+      int gi = I0 + e % TX;
+      // the column is the same on every pass of this loop; kept opaque so the
+      // compiler loads its metrics on each pass instead of holding their
+      // addresses across MOMNTM (which spills them at 64 VGPRs)
+      asm volatile("" : "+v"(gi));
+      const int gj = J0 + e / TX;
       if (gi <= IM && gj <= JM) {
          hn_g[IDX(B, gi, gj)] = fL(gi, gj);
          const uint32_t fl = B.flags[IDX(B, gi, gj)];
-         if (fl & F_MOM) momntm_pt(B, P, T, fl, gi, gj, fL, zL, ub, vb);
+         if (fl & F_MOM) momntm_pt(B, P, T, fl, gi, gj, fL, zL, ub, vb, B.elpct[gi - 1], B.elpdt[gi - 1]);
       }
    }
 }
