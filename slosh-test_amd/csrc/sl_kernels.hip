@@ -248,10 +248,13 @@ __global__ void __launch_bounds__(TX * TY) k_contty(SlBasinDev B, SlRunDev R, St
    A.hn[so + IDX(B, i, j)] = contty_sq(B, R, s, step, i, j, A.hc + so, R.ub + so, R.vb + so, nullptr);
 }
 
-// Fused path: a 64 x 16 tile per workgroup (each thread four rows), numbered
+// Fused path: a 64 x 16 tile per workgroup (each thread ATH / TYA rows), numbered
 // XCD by XCD; also writes the U==0 mask of every point for k_stepB.
 #define ATH 16
-__global__ void __launch_bounds__(TX * TY) k_conttyA(SlBasinDev B, SlRunDev R, StepArgs A, int ntx,
+#ifndef TYA
+#define TYA 8   /* rows of threads: each thread takes ATH / TYA rows */
+#endif
+__global__ void __launch_bounds__(TX * TYA) k_conttyA(SlBasinDev B, SlRunDev R, StepArgs A, int ntx,
                                                      int nty)
 {
    int tx, ty, s;
@@ -262,10 +265,89 @@ __global__ void __launch_bounds__(TX * TY) k_conttyA(SlBasinDev B, SlRunDev R, S
    const real *hc = A.hc + so;
    real *hn = A.hn + so, *ub = R.ub + so, *vb = R.vb + so;
    uint8_t *uz = R.uz + so;
-   for (int r = threadIdx.y; r < ATH; r += TY) {
-      const int j = ty * ATH + r + 1;
+   // The loads of the thread's rows are issued before any of its arithmetic:
+   // rows do not depend on each other (a transport its owner flushes reads as
+   // 0 to every square through the same |UB| <= 1e-10 test), so the values
+   // are those contty_sq reads, without a dependent chain per row.
+   constexpr int NR = ATH / TYA;
+   struct Row {
+      uint32_t fl;
+      real u1, v1, u2, v2, u3, v3, u4, v4, hc, zb, sl;
+   };
+   Row L[NR];
+   const bool in_i = i <= B.imxb - 1;
+   const real ed = in_i ? B.elpdl2[i - 1] : real(0.), ec = in_i ? B.elpcl2[i - 1] : real(0.);
+   for (int k = 0; k < NR; k++) {
+      const int j = ty * ATH + threadIdx.y + k * TYA + 1;
+      Row &l = L[k];
+      l.fl = 0;
+      l.u1 = l.v1 = l.u2 = l.v2 = l.u3 = l.v3 = l.u4 = l.v4 = l.hc = l.zb = l.sl = real(0.);
+      if (j > B.jmxb) continue;
+      const uint32_t p = IDX(B, i, j);
+      l.u1 = ub[p];
+      l.hc = hc[p];
+      if (in_i && j <= B.jmxb - 1) {
+         const uint32_t p3 = p + B.pitch;
+         l.fl = B.flags[p];
+         l.v1 = vb[p];
+         l.u2 = ub[p + 1];
+         l.v2 = vb[p + 1];
+         l.u3 = ub[p3 + 1];
+         l.v3 = vb[p3 + 1];
+         l.u4 = ub[p3];
+         l.v4 = vb[p3];
+         l.zb = B.zb[p];
+         l.sl = B.sinl2[j - 1];
+      }
+   }
+   for (int k = 0; k < NR; k++) {
+      const int j = ty * ATH + threadIdx.y + k * TYA + 1;
       if (j > B.jmxb) break;
-      hn[IDX(B, i, j)] = contty_sq(B, R, s, step, i, j, hc, ub, vb, uz);
+      const Row &l = L[k];
+      const uint32_t p = IDX(B, i, j);
+      if (j == B.jmxb || !in_i) {   // the last row or column (contty_sq's first cases)
+         if (j == B.jmxb && in_i) R.hsubtop[(size_t) s * B.pitch + (i - 1)] = l.hc;
+         uz[p] = l.u1 == real(0.);
+         hn[p] = l.hc;
+         continue;
+      }
+      real u1 = l.u1, v1 = l.v1, u2 = l.u2, v2 = l.v2, u3 = l.u3, v3 = l.v3, u4 = l.u4, v4 = l.v4;
+      if ((l.fl & F_ZR) && fabs(u1) <= real(1.E-10)) {
+         if (!same_bits(u1, real(0.))) ub[p] = real(0.);
+         if (!same_bits(v1, real(0.))) vb[p] = real(0.);
+         u1 = real(0.);
+         v1 = real(0.);
+      }
+      uz[p] = u1 == real(0.);
+      real out = l.hc;
+      if (l.fl & F_ACT) {
+         if (zr(B, i + 1, j) && fabs(u2) <= real(1.E-10)) { u2 = real(0.); v2 = real(0.); }
+         if (zr(B, i + 1, j + 1) && fabs(u3) <= real(1.E-10)) { u3 = real(0.); v3 = real(0.); }
+         if (zr(B, i, j + 1) && fabs(u4) <= real(1.E-10)) { u4 = real(0.); v4 = real(0.); }
+         const real dux = -u1 + u2 + u3 - u4;
+         const real zb = l.zb;
+         if (!(zb <= real(30.) && dux == real(0.))) {
+            const real *P = par(R, s, step);
+            const real hpst = l.hc;
+            const real z = ed + (ec - ed) * l.sl;
+            const real fct = P[P_X1B5] / z;
+            const real dvy = -v1 - v2 + v3 + v4;
+            const real hbj = hpst - fct * (dux + dvy);
+            const real chng = hbj + zb;
+            if (chng < real(0.)) {
+               if (-chng < real(1E-3)) out = -zb;
+               else if (l.hc == -zb) out = l.hc;
+               else {
+                  out = -zb;   // final value comes from the replay
+                  int q = atomicAdd(&R.evcnt[s], 1);
+                  if (q < SL_EVCAP) R.evkey[(size_t) s * SL_EVCAP + q] = ((uint32_t) j << 16) | (uint32_t) i;
+               }
+            } else {
+               out = hbj;
+            }
+         }
+      }
+      hn[p] = out;
    }
 }
 
@@ -2031,7 +2113,7 @@ extern "C" int SL_LAUNCH(const SlBasinDev *B, const SlRunDev *R, int smpt, int f
       a.hc = h0;
       a.hn = h1;
       MARK(0);
-      hipLaunchKernelGGL(k_conttyA, dim3(ntx * nty * R->nstorm), dim3(TX, TY), 0, st, *B, *R, a, ntx, nty);
+      hipLaunchKernelGGL(k_conttyA, dim3(ntx * nty * R->nstorm), dim3(TX, TYA), 0, st, *B, *R, a, ntx, nty);
       MARK(1);
       hipLaunchKernelGGL(k_replay, dim3(R->nstorm), dim3(64), 0, st, *B, *R, a, 1);
       MARK(2);
