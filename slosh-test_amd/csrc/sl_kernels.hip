@@ -1737,25 +1737,39 @@ k_stepB(SlBasinDev B, SlRunDev R, StepArgs A, int ntx, int nty, int dyn)
    const bool filt = FILT && (!dyn || (sf & SF_FILT));
    if (tid < 4) sh_cnt[tid] = 0;
 
-   // HB after CONTTY (and the U==0 mask) with halo; the static ZB and ZBM
-   for (int e = tid; e < RH::n; e += 256) {
-      const int gi = I0 - RH::lo + e % RH::w, gj = J0 - RH::lo + e / RH::w;
-      sh_hs[e] = (gi >= 1 && gi <= IM && gj >= 1 && gj <= JM) ? hs_g[IDX(B, gi, gj)] : real(0.);
+   // HB after CONTTY (and the U==0 mask) with halo; the static ZB and ZBM.
+   // The loops are unrolled over a compile-time element count so each
+   // thread's loads are in flight together.
+#pragma unroll
+   for (int k = 0; k < (RH::n + 255) / 256; k++) {
+      const int e = tid + 256 * k;
+      if (e < RH::n) {
+         const int gi = I0 - RH::lo + e % RH::w, gj = J0 - RH::lo + e / RH::w;
+         sh_hs[e] = (gi >= 1 && gi <= IM && gj >= 1 && gj <= JM) ? hs_g[IDX(B, gi, gj)] : real(0.);
+      }
    }
-   for (int e = tid; e < RZ::n; e += 256) {
-      const int gi = I0 - RZ::lo + e % RZ::w, gj = J0 - RZ::lo + e / RZ::w;
-      const bool in = gi >= 1 && gi <= IM && gj >= 1 && gj <= JM;
-      sh_zb[e] = in ? B.zb[IDX(B, gi, gj)] : real(0.);
-      sh_zbm[e] = in ? B.zbm[IDX(B, gi, gj)] : real(0.);
+#pragma unroll
+   for (int k = 0; k < (RZ::n + 255) / 256; k++) {
+      const int e = tid + 256 * k;
+      if (e < RZ::n) {
+         const int gi = I0 - RZ::lo + e % RZ::w, gj = J0 - RZ::lo + e / RZ::w;
+         const bool in = gi >= 1 && gi <= IM && gj >= 1 && gj <= JM;
+         sh_zb[e] = in ? B.zb[IDX(B, gi, gj)] : real(0.);
+         sh_zbm[e] = in ? B.zbm[IDX(B, gi, gj)] : real(0.);
+      }
    }
    int anyz = 0;
    if (smpt) {
       const uint8_t *uz_g = R.uz + so;
-      for (int e = tid; e < RU::n; e += 256) {
-         const int gi = I0 - RU::lo + e % RU::w, gj = J0 - RU::lo + e / RU::w;
-         const uint8_t u = (gi >= 1 && gi <= IM && gj >= 1 && gj <= JM) ? uz_g[IDX(B, gi, gj)] : 0;
-         sh_uz[e] = u;
-         anyz |= u;
+#pragma unroll
+      for (int k = 0; k < (RU::n + 255) / 256; k++) {
+         const int e = tid + 256 * k;
+         if (e < RU::n) {
+            const int gi = I0 - RU::lo + e % RU::w, gj = J0 - RU::lo + e / RU::w;
+            const uint8_t u = (gi >= 1 && gi <= IM && gj >= 1 && gj <= JM) ? uz_g[IDX(B, gi, gj)] : 0;
+            sh_uz[e] = u;
+            anyz |= u;
+         }
       }
    }
    // Without a U == 0 point in reach no momentum point is flagged, no height
