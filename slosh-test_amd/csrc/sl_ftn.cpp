@@ -15,6 +15,7 @@
 //   DUMB7.HB   on the first step (mask init)     -> initial water, by inital_
 //   hb (HHB)   after TMSTPC2 when FPASS          -> min(HB, HMX) after HMXSV
 //   hb         after CLNUP                       -> HMX (the envelope)
+//   DUMB7.HB   after every step with -f_stat     -> HB, by tmstpc2_ (see below)
 //   EGTH.DELT  on tide-only steps                -> kept current by tmstpm_
 //   LLXFLE     written by memSetLLx before INITAL -> read by inital_ (Coriolis)
 // Paths this build does not model (tides, hot start, waves, full depth) stop
@@ -65,7 +66,8 @@ struct Run {
    slg_basin *basin = nullptr;
    slg_batch *batch = nullptr;
    int imxb = 0, jmxb = 0, nsteps = 0, step = 0;
-   std::vector<double> buf;   // [jmxb][imxb] staging
+   bool hb_each_step = false;   // the driver reads COMMON HB after every step
+   std::vector<double> buf;     // [jmxb][imxb] staging
 };
 Run g_run;
 
@@ -96,6 +98,31 @@ extern "C" {
 int initalcommon_(void)
 {
    return 0;   // the mirrors start zeroed
+}
+
+// With -f_stat N (N != 0; usrparse.c:357-363) the driver's windowed maxima
+// (MaxStatUpdateGrid, slosh2.c:1264-1316) read COMMON /DUMB7/ HB after every
+// step.  The driver passes the model no sign of it, so the library looks for
+// the option on the process's command line.
+static bool cmdline_f_stat()
+{
+   FILE *fp = fopen("/proc/self/cmdline", "rb");
+   if (!fp) return false;
+   std::vector<std::string> args;
+   std::string a;
+   int c;
+   while ((c = fgetc(fp)) != EOF) {
+      if (c == 0) {
+         args.push_back(a);
+         a.clear();
+      } else {
+         a.push_back((char) c);
+      }
+   }
+   fclose(fp);
+   for (size_t k = 0; k + 1 < args.size(); k++)
+      if (args[k] == "-f_stat" && atoi(args[k + 1].c_str()) != 0) return true;
+   return false;
 }
 
 int inital_(int *mhalt, int *imxb, int *jmxb, double zb[BAS_Y][BAS_X], int *hr, int *mn, int *dy,
@@ -133,6 +160,7 @@ int inital_(int *mhalt, int *imxb, int *jmxb, double zb[BAS_Y][BAS_X], int *hr, 
    if (slg_batch_upload(g_run.batch, err, sizeof err)) fatal("device", err);
    g_run.nsteps = slg_batch_nsteps(g_run.batch, 0);
    g_run.step = 0;
+   g_run.hb_each_step = cmdline_f_stat();
    // outputs (inital.f:204-272)
    *mhalt = g_run.nsteps;
    *imxb = ni;
@@ -198,6 +226,12 @@ int tmstpc2_(double hb[BAS_Y][BAS_X], short *fpass, short *restart)
             const double h = g_run.buf[(size_t) j * ni + i], m = hmx[(size_t) j * ni + i];
             hb[j][i] = (m < h) ? m : h;
          }
+   }
+   if (g_run.hb_each_step) {   // HB at the end of the step, as COMMON holds it after TMSTPM
+      get("HB");
+      const int ni = g_run.imxb, nj = g_run.jmxb;
+      for (int j = 0; j < nj; j++)
+         for (int i = 0; i < ni; i++) dumb7_.HB[j][i] = g_run.buf[(size_t) j * ni + i];
    }
    return 0;
 }

@@ -50,11 +50,11 @@ def test_gpu_cli_polar_100x90_vs_reference(tmp_path):
     assert (wet_g == wet_r).mean() >= 0.99
 
 
-def _run_gpu64(root, bsn):
+def _run_gpu64(root, bsn, tag="gpu64", extra=()):
     old = os.environ.get("SLG_PRECISION")
     os.environ["SLG_PRECISION"] = "64"
     try:
-        return helpers.run_cli(helpers.GPU_CLI, root, bsn, "gpu64")
+        return helpers.run_cli(helpers.GPU_CLI, root, bsn, tag, extra=extra)
     finally:
         if old is None:
             del os.environ["SLG_PRECISION"]
@@ -80,3 +80,37 @@ def test_gpu_cli_fp64_polar_100x90_matches_reference_cli(tmp_path):
     env_g, rex_g = _run_gpu64(root, bsn)
     env_r, rex_r = helpers.run_cli(helpers.REF_CLI, root, bsn, "ref")
     assert helpers.same_cli_outputs(env_g, rex_g, env_r, rex_r) == []
+
+
+def test_gpu_cli_fp64_f_stat_matches_reference_cli(tmp_path):
+    """P-Surge windowed maxima (-f_stat 1): the driver's MaxStatUpdateGrid
+    (slosh2.c:1264-1316) reads COMMON HB after every step, so the drop-in keeps
+    it current in this mode.  The env file of every window (ExtendEnvName,
+    slosh2.c:1998-2011) and the rex file of the window maxima equal the
+    reference CLI's."""
+    import glob
+    import subprocess
+    b, t = synth.tiny(seed=5, ni=30, nj=26, hours=6)
+    outs = {}
+    for tag, exe in (("gpu64", helpers.GPU_CLI), ("ref", helpers.REF_CLI)):
+        root = str(tmp_path / tag)
+        bsn = synth.write_case(root, b, t)["basin"]
+        env = dict(os.environ, SLG_PRECISION="64")
+        r = subprocess.run([exe, "-basin", bsn, "-rootDir", root, "-trk", os.path.join(root, "storm.trk"),
+                            "-rex", os.path.join(root, "out.rex"), "-env", os.path.join(root, "out.env"),
+                            "-verbose", "0", "-f_stat", "1"], cwd=root, capture_output=True, text=True,
+                           timeout=600, env=env, preexec_fn=helpers._big_stack)
+        assert r.returncode == 0, (tag, r.stdout[-500:], r.stderr[-500:])
+        outs[tag] = (root, sorted(os.path.basename(f) for f in glob.glob(os.path.join(root, "*.env"))))
+    (rg, fg), (rr, fr) = outs["gpu64"], outs["ref"]
+    assert fg == fr and len(fr) >= 3, (fg, fr)
+    for f in fr:
+        a, c = helpers.read_env(os.path.join(rg, f)), helpers.read_env(os.path.join(rr, f))
+        assert a[:3] == c[:3] and a[4] == c[4], f
+        assert np.array_equal(a[3], c[3]), f
+    ra, rb = open(os.path.join(rg, "out.rex"), "rb").read(), open(os.path.join(rr, "out.rex"), "rb").read()
+    ia, ib = ra.rfind(b"envl"), rb.rfind(b"envl")
+    assert len(ra) == len(rb) and ia == ib
+    if ia >= 0:   # the track records Data_WriteTrk copies are unset for this track layout
+        ra, rb = ra[:ia + 4] + ra[ia + 4 + 2400:], rb[:ib + 4] + rb[ib + 4 + 2400:]
+    assert ra == rb
