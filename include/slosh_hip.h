@@ -146,11 +146,24 @@ int slg_track_datums(const char *trk_path, float *ht1, float *ht2);
  * field "HHB": HHB = min(HB, HMX) of the last step, the array TMSTPC2 hands
  * the C driver for a rex frame (tmstpc2.f:74-85; needs HB before FILTER, so
  * the last step ran on the staged path or did not filter); "HMX": the
- * envelope, the last frame RexSaveEnv writes.  Frame k goes to buf + k*cap
+ * envelope, the last frame RexSaveEnv writes; "HBMAX": the windowed maxima
+ * (slg_batch_set_maxstat), the frames of the -f_stat mode (slosh2.c:2460).  Frame k goes to buf + k*cap
  * and its length to nbytes[k]; cap = (18*(imxb-1)*(jmxb-1) + 7) / 8 always
  * suffices (SLG_ELIMIT if a frame does not fit). */
 int slg_batch_rex_frames(slg_batch *x, const char *field, int s0, int ns, unsigned char *buf, long cap,
                          int *nbytes, char *err, int errlen);
+/* P-Surge windowed maxima of every run (slosh2.c:1227-1316; the reference's
+ * -f_stat mode): on = 1 keeps, in double as the C driver does, the maximum of
+ * HB after each step on every square, starting from -ZB (MaxStatInitGrid);
+ * on = 0 drops them.  Call after slg_batch_upload.  reset starts a new window
+ * (MaxStatInitGrid flag 1); env gives the EnvSave values of the window's
+ * maxima (slosh2.c:1611-1671, as PerformRun saves them per window,
+ * slosh2.c:2460-2475), env[nstorm][jmxb][imxb]; their rex frames come from
+ * slg_batch_rex_frames(x, "HBMAX", ...) and their values from
+ * slg_batch_get(x, s, "HBMAX", ...). */
+int slg_batch_set_maxstat(slg_batch *x, int on);
+int slg_batch_maxstat_reset(slg_batch *x);
+int slg_batch_maxstat_env(slg_batch *x, short *env, char *err, int errlen);
 /* The file around the frames, laid out as the reference's rex writer does:
  * open = RexOpen (version 2) + RexSaveHeader (rex.c:150-180, 264-379); frame =
  * RexSaveStep's record (rex.c:387-447: lat, lon, wind speed, wind direction,

@@ -781,17 +781,34 @@ static int launch_split(slg_batch *X, const SlBasinDevT<R> &B, const SlRunDevT<R
 {
    const int N = X->nsplit, n = X->nstorm;
    hipEvent_t *prev = nullptr;
+   int r = 0;
    for (int p = 0; p < N; p++) {
       const int s0 = (int) ((long) n * p / N), s1 = (int) ((long) n * (p + 1) / N);
       const SlRunDevT<R> V = run_view(D, B, s0, s1 - s0);
       hipStream_t st = p ? X->stx[p - 1] : X->st;
       hipEvent_t *evp = (p == 0 && ev) ? ev : (first ? X->ev_phase[p] : nullptr);
       if (p && first && hipStreamWaitEvent(st, prev[2], 0) != hipSuccess) return -(int) hipErrorUnknown;
-      const int r = dev_launch(&B, &V, smpt, filt, dyn, X->parity, X->path, X->tile_rows, st, evp);
+      r = dev_launch(&B, &V, smpt, filt, dyn, X->parity, X->path, X->tile_rows, st, evp);
       if (r < 0) return r;
+      if (X->hbmax) {
+         const int m = dev_maxstat(&B, &V, r ? V.hb1 : V.hb0, img(X->b, R()).zbd,
+                                   X->hbmax + (size_t) s0 * D.fstride, 0, st);
+         if (m < 0) return m;
+      }
       prev = evp;
    }
-   return 0;
+   return r;
+}
+
+// the whole batch on its stream; then the windowed maxima when they are on
+template <class R>
+static int launch_whole(slg_batch *X, const SlBasinDevT<R> &B, const SlRunDevT<R> &D, int smpt, int filt,
+                        hipEvent_t *ev, int dyn)
+{
+   const int r = dev_launch(&B, &D, smpt, filt, dyn, X->parity, X->path, X->tile_rows, X->st, ev);
+   if (r < 0 || !X->hbmax) return r;
+   const int m = dev_maxstat(&B, &D, r ? D.hb1 : D.hb0, img(X->b, R()).zbd, X->hbmax, 0, X->st);
+   return m < 0 ? m : r;
 }
 
 static int launch_one(slg_batch *X, int smpt, int filt, hipEvent_t *ev, int dyn = 0, bool first = false)
@@ -800,9 +817,8 @@ static int launch_one(slg_batch *X, int smpt, int filt, hipEvent_t *ev, int dyn 
       if (X->prec == 64) return launch_split(X, X->b->i64.dev, X->d64, smpt, filt, ev, dyn, first);
       return launch_split(X, X->b->i32.dev, X->d32, smpt, filt, ev, dyn, first);
    }
-   if (X->prec == 64)
-      return dev_launch(&X->b->i64.dev, &X->d64, smpt, filt, dyn, X->parity, X->path, X->tile_rows, X->st, ev);
-   return dev_launch(&X->b->i32.dev, &X->d32, smpt, filt, dyn, X->parity, X->path, X->tile_rows, X->st, ev);
+   if (X->prec == 64) return launch_whole(X, X->b->i64.dev, X->d64, smpt, filt, ev, dyn);
+   return launch_whole(X, X->b->i32.dev, X->d32, smpt, filt, ev, dyn);
 }
 
 // the batch stream waits for the other parts' steps: callers keep seeing one
@@ -1127,6 +1143,20 @@ static int batch_get(slg_batch *X, int storm, const char *name, double *dst, cha
       for (int l = 0; l < nl; l++) dst[l] = (double) f[l];
       return 0;
    }
+   if (!strcmp(name, "HBMAX")) {   // windowed maxima, held in double
+      if (!X->hbmax) {
+         seterr(err, errlen, "%s", "windowed maxima are off");
+         return SLG_EINPUT;
+      }
+      std::vector<double> f(D.fstride);
+      HIPOK(hipStreamSynchronize(X->st));
+      HIPOK(hipMemcpy(f.data(), X->hbmax + (size_t) storm * D.fstride, D.fstride * 8, hipMemcpyDeviceToHost));
+      const int P = X->b->pitch;
+      for (int j = 1; j <= b->jmxb; j++)
+         for (int i = 1; i <= b->imxb; i++)
+            dst[PIX(b, i, j)] = (i < b->imxb && j < b->jmxb) ? f[(size_t) (j - 1) * P + (i - 1)] : 0.;
+      return 0;
+   }
    if (!strcmp(name, "HB")) src = X->last_par[storm] ? D.hb1 : D.hb0;
    else if (!strcmp(name, "HBS")) {
       if (X->hbs_buf < 0 || X->sc[storm].nsteps < X->step) {
@@ -1255,14 +1285,15 @@ static bool rex_buffers(slg_batch *X, int chunk, long cap, size_t per)
 // rex frames of storms s0 .. s0+ns-1, in chunks whose scratch (2N+1 words a
 // storm for run starts and records) stays near 256 MB
 template <class R>
-static int batch_rex(slg_batch *X, bool hhb, int s0, int ns, unsigned char *buf, long cap, int *nbytes, char *err,
+static int batch_rex(slg_batch *X, int src, int s0, int ns, unsigned char *buf, long cap, int *nbytes, char *err,
                      int errlen)
 {
    slg_basin *B = X->b;
    const SlRunDevT<R> &D = run(X, R());
    const long N = (long) (B->h.imxb - 1) * (B->h.jmxb - 1);
    const R *hbs = nullptr;
-   if (hhb) hbs = X->hbs_buf ? D.hb1 : D.hb0;
+   if (src == 0) hbs = X->hbs_buf ? D.hb1 : D.hb0;
+   const double *hmax = src == 2 ? X->hbmax : nullptr;
    const size_t per = (size_t) (2 * N + 1) * 4;
    int chunk = (int) std::max<size_t>(1, ((size_t) 256 << 20) / per);
    if (chunk > ns) chunk = ns;
@@ -1279,7 +1310,7 @@ static int batch_rex(slg_batch *X, bool hhb, int s0, int ns, unsigned char *buf,
    std::vector<long> off(chunk);
    for (int c = 0; c < ns && !rc; c += chunk) {
       const int n = std::min(chunk, ns - c);
-      int r = dev_rex(&img(B, R()).dev, &D, hbs, img(B, R()).zbd, s0 + c, n, dout, cap, dnb, scr, X->st);
+      int r = dev_rex(&img(B, R()).dev, &D, hbs, hmax, img(B, R()).zbd, s0 + c, n, dout, cap, dnb, scr, X->st);
       if (r || hipMemcpyAsync(nbytes + c, dnb, sizeof(int) * n, hipMemcpyDeviceToHost, X->st) != hipSuccess ||
           hipStreamSynchronize(X->st) != hipSuccess) {
          rc = SLG_EHIP;
@@ -1321,8 +1352,13 @@ extern "C" int slg_batch_rex_frames(slg_batch *X, const char *field, int s0, int
       return SLG_EINPUT;
    }
    const bool hhb = !strcmp(field, "HHB");
-   if (!hhb && strcmp(field, "HMX")) {
-      seterr(err, errlen, "unknown rex field %s (HHB or HMX)", field);
+   const int src = hhb ? 0 : !strcmp(field, "HMX") ? 1 : !strcmp(field, "HBMAX") ? 2 : -1;
+   if (src < 0) {
+      seterr(err, errlen, "unknown rex field %s (HHB, HMX or HBMAX)", field);
+      return SLG_EINPUT;
+   }
+   if (src == 2 && !X->hbmax) {
+      seterr(err, errlen, "%s", "windowed maxima are off (slg_batch_set_maxstat)");
       return SLG_EINPUT;
    }
    if (hhb) {
@@ -1335,8 +1371,70 @@ extern "C" int slg_batch_rex_frames(slg_batch *X, const char *field, int s0, int
       }
    }
    if (ns == 0) return 0;
-   return X->prec == 64 ? batch_rex<double>(X, hhb, s0, ns, buf, cap, nbytes, err, errlen)
-                        : batch_rex<float>(X, hhb, s0, ns, buf, cap, nbytes, err, errlen);
+   return X->prec == 64 ? batch_rex<double>(X, src, s0, ns, buf, cap, nbytes, err, errlen)
+                        : batch_rex<float>(X, src, s0, ns, buf, cap, nbytes, err, errlen);
+}
+
+// ---- P-Surge windowed maxima (slosh2.c:1227-1316) --------------------------
+template <class R>
+static int maxstat_init(slg_batch *X)
+{
+   int r = dev_maxstat(&img(X->b, R()).dev, &run(X, R()), (const R *) nullptr, img(X->b, R()).zbd, X->hbmax, 1,
+                       X->st);
+   return (r || hipStreamSynchronize(X->st) != hipSuccess) ? SLG_EHIP : 0;
+}
+
+extern "C" int slg_batch_maxstat_reset(slg_batch *X)
+{
+   if (!X->dmem || !X->hbmax) return SLG_EINPUT;
+   return X->prec == 64 ? maxstat_init<double>(X) : maxstat_init<float>(X);
+}
+
+extern "C" int slg_batch_set_maxstat(slg_batch *X, int on)
+{
+   if (!X->dmem) return SLG_EINPUT;
+   if (hipStreamSynchronize(X->st) != hipSuccess) return SLG_EHIP;
+   drop_graph(X);   // captured steps hold the update or not
+   if (!on) {
+      if (X->hbmax) (void) hipFree(X->hbmax);
+      X->hbmax = nullptr;
+      return 0;
+   }
+   const size_t fs = X->prec == 64 ? X->d64.fstride : X->d32.fstride;
+   if (!X->hbmax && hipMalloc((void **) &X->hbmax, sizeof(double) * fs * X->nstorm) != hipSuccess) {
+      X->hbmax = nullptr;
+      return SLG_EHIP;
+   }
+   return slg_batch_maxstat_reset(X);
+}
+
+template <class R>
+static int maxstat_env(slg_batch *X, short *env, char *err, int errlen)
+{
+   const sl_basin *b = &X->b->h;
+   const size_t n = (size_t) b->imxb * b->jmxb;
+   short *denv = nullptr;
+   HIPOK(hipMalloc((void **) &denv, n * X->nstorm * 2));
+   int rc = dev_envelopes(&img(X->b, R()).dev, &run(X, R()), img(X->b, R()).zbd, 0, X->nstorm, denv, nullptr,
+                          X->st, X->hbmax);
+   if (rc == 0 && (hipMemcpyAsync(env, denv, n * X->nstorm * 2, hipMemcpyDeviceToHost, X->st) != hipSuccess ||
+                   hipStreamSynchronize(X->st) != hipSuccess))
+      rc = -1;
+   (void) hipFree(denv);
+   if (rc) {
+      seterr(err, errlen, "%s", "windowed-maxima env kernel or copy failed");
+      return SLG_EHIP;
+   }
+   return 0;
+}
+
+extern "C" int slg_batch_maxstat_env(slg_batch *X, short *env, char *err, int errlen)
+{
+   if (!X->dmem || !X->hbmax || !env) {
+      seterr(err, errlen, "%s", "windowed maxima are off (slg_batch_set_maxstat) or no buffer");
+      return SLG_EINPUT;
+   }
+   return X->prec == 64 ? maxstat_env<double>(X, env, err, errlen) : maxstat_env<float>(X, env, err, errlen);
 }
 
 // Datums of a run's track for the env trailer (ReadTrkFile, slosh2.c:1885-1975):
@@ -1435,6 +1533,7 @@ extern "C" void slg_batch_close(slg_batch *X)
    if (!X) return;
    drop_graph(X);
    rex_free(X);
+   if (X->hbmax) (void) hipFree(X->hbmax);
    for (int p = 0; p < SL_MAXSPLIT - 1; p++)
       if (X->stx[p]) {
          (void) hipStreamSynchronize(X->stx[p]);

@@ -107,6 +107,7 @@ struct slg_batch {
    // part's latency-bound cascade replay overlaps the others' stencil kernels.
    // Part 0 runs on st, part p > 0 on stx[p - 1].
    int nsplit = 1;
+   double *hbmax = nullptr;   // P-Surge windowed maxima, [nstorm][fstride], when on
    // rex frame buffers, kept from one slg_batch_rex_frames call to the next:
    // scratch, frames, packed frames (device), sizes and offsets, and pinned
    // host staging for the packed copy; for rex_chunk storms of rex_cap bytes
@@ -122,40 +123,54 @@ static inline SlRunDevT<float> &run(slg_batch *X, float) { return X->d32; }
 static inline SlRunDevT<double> &run(slg_batch *X, double) { return X->d64; }
 
 
-extern "C" int sl_dev_envelopes_f32(const SlBasinDevT<float> *B, const SlRunDevT<float> *R, const double *zbd,
-                                    int s0, int ns, short *env, short *mom, hipStream_t st);
-extern "C" int sl_dev_envelopes_f64(const SlBasinDevT<double> *B, const SlRunDevT<double> *R, const double *zbd,
-                                    int s0, int ns, short *env, short *mom, hipStream_t st);
-static inline int dev_envelopes(const SlBasinDevT<float> *B, const SlRunDevT<float> *R, const double *zbd,
-                                int s0, int ns, short *env, short *mom, hipStream_t st)
+extern "C" int sl_dev_envelopes_f32(const SlBasinDevT<float> *B, const SlRunDevT<float> *R, const double *hx2,
+                                    const double *zbd, int s0, int ns, short *env, short *mom, hipStream_t st);
+extern "C" int sl_dev_envelopes_f64(const SlBasinDevT<double> *B, const SlRunDevT<double> *R, const double *hx2,
+                                    const double *zbd, int s0, int ns, short *env, short *mom, hipStream_t st);
+// env values (EnvSave) of HMX, or of hx2 (double [nstorm][fstride]) when given
+static inline int dev_envelopes(const SlBasinDevT<float> *B, const SlRunDevT<float> *R, const double *zbd, int s0,
+                                int ns, short *env, short *mom, hipStream_t st, const double *hx2 = nullptr)
 {
-   return sl_dev_envelopes_f32(B, R, zbd, s0, ns, env, mom, st);
+   return sl_dev_envelopes_f32(B, R, hx2, zbd, s0, ns, env, mom, st);
 }
-static inline int dev_envelopes(const SlBasinDevT<double> *B, const SlRunDevT<double> *R, const double *zbd,
-                                int s0, int ns, short *env, short *mom, hipStream_t st)
+static inline int dev_envelopes(const SlBasinDevT<double> *B, const SlRunDevT<double> *R, const double *zbd, int s0,
+                                int ns, short *env, short *mom, hipStream_t st, const double *hx2 = nullptr)
 {
-   return sl_dev_envelopes_f64(B, R, zbd, s0, ns, env, mom, st);
+   return sl_dev_envelopes_f64(B, R, hx2, zbd, s0, ns, env, mom, st);
 }
-
+extern "C" int sl_dev_maxstat_f32(const SlBasinDevT<float> *B, const SlRunDevT<float> *R, const float *hb,
+                                  const double *zbd, double *hmax, int init, hipStream_t st);
+extern "C" int sl_dev_maxstat_f64(const SlBasinDevT<double> *B, const SlRunDevT<double> *R, const double *hb,
+                                  const double *zbd, double *hmax, int init, hipStream_t st);
+static inline int dev_maxstat(const SlBasinDevT<float> *B, const SlRunDevT<float> *R, const float *hb,
+                              const double *zbd, double *hmax, int init, hipStream_t st)
+{
+   return sl_dev_maxstat_f32(B, R, hb, zbd, hmax, init, st);
+}
+static inline int dev_maxstat(const SlBasinDevT<double> *B, const SlRunDevT<double> *R, const double *hb,
+                              const double *zbd, double *hmax, int init, hipStream_t st)
+{
+   return sl_dev_maxstat_f64(B, R, hb, zbd, hmax, init, st);
+}
 extern "C" int sl_dev_rex_f32(const SlBasinDevT<float> *B, const SlRunDevT<float> *R, const float *hbs,
-                              const double *zbd, int s0, int ns, unsigned char *out, long cap, int *nbytes,
-                              unsigned *scr, hipStream_t st);
+                              const double *hmax, const double *zbd, int s0, int ns, unsigned char *out, long cap,
+                              int *nbytes, unsigned *scr, hipStream_t st);
 extern "C" int sl_dev_rex_f64(const SlBasinDevT<double> *B, const SlRunDevT<double> *R, const double *hbs,
-                              const double *zbd, int s0, int ns, unsigned char *out, long cap, int *nbytes,
-                              unsigned *scr, hipStream_t st);
+                              const double *hmax, const double *zbd, int s0, int ns, unsigned char *out, long cap,
+                              int *nbytes, unsigned *scr, hipStream_t st);
 extern "C" int sl_dev_rex_pack(const unsigned char *src, long cap, const int *nbytes, const long *off, int ns,
                                unsigned char *dst, hipStream_t st);
 static inline int dev_rex(const SlBasinDevT<float> *B, const SlRunDevT<float> *R, const float *hbs,
-                          const double *zbd, int s0, int ns, unsigned char *out, long cap, int *nbytes, unsigned *scr,
-                          hipStream_t st)
+                          const double *hmax, const double *zbd, int s0, int ns, unsigned char *out, long cap,
+                          int *nbytes, unsigned *scr, hipStream_t st)
 {
-   return sl_dev_rex_f32(B, R, hbs, zbd, s0, ns, out, cap, nbytes, scr, st);
+   return sl_dev_rex_f32(B, R, hbs, hmax, zbd, s0, ns, out, cap, nbytes, scr, st);
 }
 static inline int dev_rex(const SlBasinDevT<double> *B, const SlRunDevT<double> *R, const double *hbs,
-                          const double *zbd, int s0, int ns, unsigned char *out, long cap, int *nbytes, unsigned *scr,
-                          hipStream_t st)
+                          const double *hmax, const double *zbd, int s0, int ns, unsigned char *out, long cap,
+                          int *nbytes, unsigned *scr, hipStream_t st)
 {
-   return sl_dev_rex_f64(B, R, hbs, zbd, s0, ns, out, cap, nbytes, scr, st);
+   return sl_dev_rex_f64(B, R, hbs, hmax, zbd, s0, ns, out, cap, nbytes, scr, st);
 }
 
 // storm schedule of one run (host, double): rows kept when keep_rows

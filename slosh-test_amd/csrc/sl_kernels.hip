@@ -1895,7 +1895,8 @@ static void launch_B(int smpt, int filt, int dyn, dim3 g, hipStream_t st, const 
 // 1611-1671): int16 tenths of a foot, (short)(HMX*10+.5) in double, 999 where
 // HMX+ZB == 0 (dry), 0 on the last row and column.  zbd is ZB in double, as
 // the reference's C driver holds it.
-__device__ __forceinline__ short env_val(const SlBasinDev &B, const real *hmx, const double *zbd, int i, int j)
+template <class T>
+__device__ __forceinline__ short env_val(const SlBasinDev &B, const T *hmx, const double *zbd, int i, int j)
 {
    if (i == B.imxb || j == B.jmxb) return 0;
    const uint32_t p = IDX(B, i, j);
@@ -1907,7 +1908,8 @@ __device__ __forceinline__ short env_val(const SlBasinDev &B, const real *hmx, c
 // Envelopes of storms s0..s0+ns-1 (env, [ns][jmxb][imxb], may be null) and
 // their MOM, the maximum of MEOWs (gui/tclsrc/meowgen.tcl:34-110): per point
 // the largest envelope value below 50 ft over the storms, 999 where none.
-__global__ void k_envelopes(SlBasinDev B, SlRunDev R, const double *zbd, int s0, int ns, short *env, short *mom)
+__global__ void k_envelopes(SlBasinDev B, SlRunDev R, const double *hx2, const double *zbd, int s0, int ns,
+                            short *env, short *mom)
 {
    const int cell = blockIdx.x * blockDim.x + threadIdx.x, n = B.imxb * B.jmxb;
    if (cell >= n) return;
@@ -1915,7 +1917,8 @@ __global__ void k_envelopes(SlBasinDev B, SlRunDev R, const double *zbd, int s0,
    int m = 999;
    bool any = false;
    for (int s = s0; s < s0 + ns; s++) {
-      const short v = env_val(B, R.hmx + (size_t) s * R.fstride, zbd, i, j);
+      const short v = hx2 ? env_val(B, hx2 + (size_t) s * R.fstride, zbd, i, j)
+                          : env_val(B, R.hmx + (size_t) s * R.fstride, zbd, i, j);
       if (env) env[(size_t) (s - s0) * n + cell] = v;
       if (v < 500 && (!any || v > m)) {
          m = v;
@@ -1943,18 +1946,25 @@ __global__ void k_advance(SlRunDev R)
 #define RX_T 1024
 
 // code of frame cell k: HHB = HMX where HMX < HB, else HB (tmstpc2.f:74-85),
-// or HMX alone (hbs == nullptr); in double, as the C driver's arrays
-__device__ __forceinline__ int rex_code(const SlBasinDev &B, const real *hbs, const real *hmx, const double *zbd,
-                                        int k, int J1)
+// HMX alone (hbs == nullptr), or the windowed maxima (hx2, double); in double,
+// as the C driver's arrays
+__device__ __forceinline__ int rex_code(const SlBasinDev &B, const real *hbs, const real *hmx, const double *hx2,
+                                        const double *zbd, int k, int J1)
 {
    const int i = k / J1, j = k - i * J1;
    const uint32_t p = IDX(B, i + 1, j + 1);
-   real h = hmx[p];
-   if (hbs) {
-      const real b = hbs[p];
-      if (!(h < b)) h = b;
+   double hd;
+   if (hx2) {
+      hd = hx2[p];
+   } else {
+      real h = hmx[p];
+      if (hbs) {
+         const real b = hbs[p];
+         if (!(h < b)) h = b;
+      }
+      hd = (double) h;
    }
-   const double hd = (double) h, zd = zbd[p];
+   const double zd = zbd[p];
    int v = (hd + zd == 0.0) ? 999 : (int) (hd * 10 + .5);
    if (v < -320) v = -320;
    else if (v > 700 && v != 999) v = 700;
@@ -1989,21 +1999,23 @@ __device__ static int blk_scan(int v, int *sh, int *total)
 
 // storms s0 + blockIdx.x; scr holds 2N+1 words per storm; frame bytes to
 // out + blockIdx.x * cap, their count to nbytes (-1: over cap)
-__global__ __launch_bounds__(RX_T) void k_rex(SlBasinDev B, SlRunDev R, const real *hbs, const double *zbd,
-                                             int s0, uint8_t *out, long cap, int *nbytes, uint32_t *scr)
+__global__ __launch_bounds__(RX_T) void k_rex(SlBasinDev B, SlRunDev R, const real *hbs, const double *hmax,
+                                             const double *zbd, int s0, uint8_t *out, long cap, int *nbytes,
+                                             uint32_t *scr)
 {
    __shared__ int sh[RX_T / 64];
    const int s = s0 + blockIdx.x, t = threadIdx.x;
    const int J1 = B.jmxb - 1, N = (B.imxb - 1) * J1;
    const size_t so = (size_t) s * R.fstride;
    const real *hx = R.hmx + so, *hb = hbs ? hbs + so : nullptr;
+   const double *h2 = hmax ? hmax + so : nullptr;
    uint32_t *starts = scr + (size_t) blockIdx.x * (2 * (size_t) N + 1), *rec = starts + N + 1;
    uint8_t *o = out + (size_t) blockIdx.x * cap;
    int nrun = 0, tot;
    for (int c = 0; c < N; c += RX_T) {
       const int k = c + t;
       int f = 0;
-      if (k < N) f = k == 0 || rex_code(B, hb, hx, zbd, k, J1) != rex_code(B, hb, hx, zbd, k - 1, J1);
+      if (k < N) f = k == 0 || rex_code(B, hb, hx, h2, zbd, k, J1) != rex_code(B, hb, hx, h2, zbd, k - 1, J1);
       const int pos = blk_scan(f, sh, &tot);
       if (f) starts[nrun + pos] = (uint32_t) k;
       nrun += tot;
@@ -2020,7 +2032,7 @@ __global__ __launch_bounds__(RX_T) void k_rex(SlBasinDev B, SlRunDev R, const re
       }
       const int pos = blk_scan(n, sh, &tot);
       if (r < nrun) {
-         const uint32_t code = (uint32_t) rex_code(B, hb, hx, zbd, (int) starts[r], J1);
+         const uint32_t code = (uint32_t) rex_code(B, hb, hx, h2, zbd, (int) starts[r], J1);
          for (int q = 0; q < n; q++) {
             const int l = len - 256 * q < 256 ? len - 256 * q : 256;
             rec[nrec + pos + q] = ((uint32_t) (l - 1) << 10) | code;
@@ -2050,6 +2062,27 @@ __global__ __launch_bounds__(RX_T) void k_rex(SlBasinDev B, SlRunDev R, const re
    if (t == 0) nbytes[blockIdx.x] = (int) nb;
 }
 
+// ------------------------------------------------------ windowed maxima
+// P-Surge windowed maxima of every storm (slosh2.c:1227-1316), in double as
+// the C driver holds them, on the squares (I <= IMXB-1, J <= JMXB-1):
+// MaxStatInitGrid sets -ZB, MaxStatUpdateGrid takes the maximum with HB after
+// each step.
+__global__ void k_maxstat(SlBasinDev B, SlRunDev R, const real *hb, const double *zbd, double *hmax, int init)
+{
+   const int s = blockIdx.y;
+   const int i = (blockIdx.x % ((B.imxb + 63) / 64)) * 64 + threadIdx.x + 1;
+   const int j = blockIdx.x / ((B.imxb + 63) / 64) + 1;
+   if (s >= R.nstorm || i > B.imxb - 1 || j > B.jmxb - 1) return;
+   const uint32_t p = IDX(B, i, j);
+   double *m = hmax + (size_t) s * R.fstride + p;
+   if (init) {
+      *m = -1 * zbd[p];
+      return;
+   }
+   const double h = (double) hb[(size_t) s * R.fstride + p];
+   if (*m < h) *m = h;
+}
+
 // frame k (nb[k] bytes at src + k*cap) to dst + off[k]: the frames of a chunk
 // back to back, for one copy to the host
 __global__ void k_rex_pack(const uint8_t *src, long cap, const int *nb, const long *off, uint8_t *dst)
@@ -2075,11 +2108,11 @@ using namespace SL_NS;
 #else
 #define SL_ENVS sl_dev_envelopes_f32
 #endif
-extern "C" int SL_ENVS(const SlBasinDev *B, const SlRunDev *R, const double *zbd, int s0, int ns, short *env,
-                       short *mom, hipStream_t st)
+extern "C" int SL_ENVS(const SlBasinDev *B, const SlRunDev *R, const double *hx2, const double *zbd, int s0,
+                       int ns, short *env, short *mom, hipStream_t st)
 {
    const int n = B->imxb * B->jmxb;
-   hipLaunchKernelGGL(k_envelopes, dim3((n + 255) / 256), dim3(256), 0, st, *B, *R, zbd, s0, ns, env, mom);
+   hipLaunchKernelGGL(k_envelopes, dim3((n + 255) / 256), dim3(256), 0, st, *B, *R, hx2, zbd, s0, ns, env, mom);
    hipError_t e = hipGetLastError();
    return e == hipSuccess ? 0 : -(int) e;
 }
@@ -2091,10 +2124,26 @@ extern "C" int SL_ENVS(const SlBasinDev *B, const SlRunDev *R, const double *zbd
 #endif
 // rex frames of storms s0 .. s0+ns-1 from HHB (hbs: the HB buffer before
 // FILTER) or from HMX (hbs == nullptr); scr: ns * (2N+1) words
-extern "C" int SL_REX(const SlBasinDev *B, const SlRunDev *R, const real *hbs, const double *zbd, int s0, int ns,
-                      unsigned char *out, long cap, int *nbytes, unsigned *scr, hipStream_t st)
+extern "C" int SL_REX(const SlBasinDev *B, const SlRunDev *R, const real *hbs, const double *hmax,
+                      const double *zbd, int s0, int ns, unsigned char *out, long cap, int *nbytes, unsigned *scr,
+                      hipStream_t st)
 {
-   hipLaunchKernelGGL(k_rex, dim3(ns), dim3(RX_T), 0, st, *B, *R, hbs, zbd, s0, out, cap, nbytes, scr);
+   hipLaunchKernelGGL(k_rex, dim3(ns), dim3(RX_T), 0, st, *B, *R, hbs, hmax, zbd, s0, out, cap, nbytes, scr);
+   hipError_t e = hipGetLastError();
+   return e == hipSuccess ? 0 : -(int) e;
+}
+
+#ifdef SL_F64
+#define SL_MAXSTAT sl_dev_maxstat_f64
+#else
+#define SL_MAXSTAT sl_dev_maxstat_f32
+#endif
+// windowed maxima of the storms of R: init (-ZB) or the maximum with hb
+extern "C" int SL_MAXSTAT(const SlBasinDev *B, const SlRunDev *R, const real *hb, const double *zbd, double *hmax,
+                          int init, hipStream_t st)
+{
+   const int nx = (B->imxb + 63) / 64;
+   hipLaunchKernelGGL(k_maxstat, dim3(nx * (B->jmxb - 1), R->nstorm), dim3(64), 0, st, *B, *R, hb, zbd, hmax, init);
    hipError_t e = hipGetLastError();
    return e == hipSuccess ? 0 : -(int) e;
 }

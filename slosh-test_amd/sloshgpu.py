@@ -56,6 +56,9 @@ def lib():
         L.slg_batch_rex_frames.argtypes = [vp, cp, i, i, vp, ctypes.c_long, vp, cp, i]
         L.slg_batch_track.argtypes = [vp, i, i, vp]
         L.slg_basin_field.argtypes = [vp, cp, vp]
+        L.slg_batch_set_maxstat.argtypes = [vp, i]
+        L.slg_batch_maxstat_reset.argtypes = [vp]
+        L.slg_batch_maxstat_env.argtypes = [vp, vp, cp, i]
         L.slg_batch_start_date.argtypes = [vp, i, vp]
         L.slg_rex_open.argtypes = [cp, cp, cp, i, i, ctypes.POINTER(vp)]
         L.slg_rex_frame.argtypes = [vp, vp, ctypes.c_double, vp, i]
@@ -238,6 +241,22 @@ class Batch:
         if lib().slg_batch_start_date(self.h, storm, out.ctypes.data_as(ctypes.c_void_p)):
             raise SloshError("no start date for storm %d" % storm)
         return [int(v) for v in out]
+
+    def set_maxstat(self, on=True):
+        """Keep the P-Surge windowed maxima of HB (the -f_stat mode)."""
+        if lib().slg_batch_set_maxstat(self.h, 1 if on else 0):
+            raise SloshError("cannot switch the windowed maxima (upload first)")
+
+    def maxstat_reset(self):
+        if lib().slg_batch_maxstat_reset(self.h):
+            raise SloshError("windowed maxima are off")
+
+    def maxstat_env(self):
+        """EnvSave values of every run's window maxima, [nstorm, jmxb, imxb]."""
+        env = np.zeros((self.n, self.basin.jmxb, self.basin.imxb), dtype=np.int16)
+        err = ctypes.create_string_buffer(512)
+        _chk(lib().slg_batch_maxstat_env(self.h, env.ctypes.data_as(ctypes.c_void_p), err, 512), err)
+        return env
 
     def rex_frames(self, field="HHB", s0=0, ns=None):
         """rex v2 frame bit streams of storms s0..s0+ns-1, encoded on the

@@ -146,3 +146,133 @@ def test_batched_rex_file_equals_reference_cli(tmp_path, cid):
     mine, ref = open(path, "rb").read(), open(rex_ref, "rb").read()
     assert len(mine) == len(ref)
     assert mine == ref
+
+
+def test_maxstat_values(tmp_path):
+    """Windowed maxima on the device: -ZB at the start and after a reset, then
+    the maximum of HB after each step, on every square (MaxStatInitGrid /
+    MaxStatUpdateGrid, slosh2.c:1227-1316); FP32 runs keep them in double."""
+    b, t = synth.tiny(seed=73, ni=30, nj=26, hours=6)
+    case = helpers.make_case(str(tmp_path), b, t)
+    B = sloshgpu.Basin(case["dta"], case["bntline"])
+    zb = B.field("ZB")
+    X = sloshgpu.Batch(B, [case["trk"]])
+    X.upload()
+    X.set_maxstat(True)
+    sq = (slice(0, B.jmxb - 1), slice(0, B.imxb - 1))
+    want = -1 * zb
+    assert np.array_equal(X.get(0, "HBMAX")[sq], want[sq])
+    for n in range(40):
+        X.step(1)
+        want = np.maximum(want, X.get(0, "HB"))
+        if n == 25:
+            X.maxstat_reset()
+            want = -1 * zb
+    got = X.get(0, "HBMAX")
+    assert np.array_equal(got[sq], want[sq])
+    X.close()
+    B.close()
+
+
+@pytest.mark.skipif(not os.path.exists(helpers.REF_CLI), reason="reference CLI (oracle/_ref) not built")
+def test_batched_f_stat_windows_equal_reference_cli(tmp_path):
+    """The reference's -f_stat mode from a batch: windows of 10 model minutes
+    as PerformRun keeps them (slosh2.c:2366-2386, 2420-2520, the trailing
+    window 2597-2640); every window's env file (ExtendEnvName, slosh2.c:
+    1998-2011) and the rex file of the window maxima equal the reference
+    CLI's."""
+    import subprocess
+    b, t = synth.tiny(seed=5, ni=30, nj=26, hours=6)
+    root = str(tmp_path / "ref")
+    case = helpers.make_case(root, b, t)
+    rexutil.trk_canonical(t, case["trk"])
+    r = subprocess.run([helpers.REF_CLI, "-basin", case["basin"], "-rootDir", root, "-trk", case["trk"],
+                        "-rex", os.path.join(root, "out.rex"), "-env", os.path.join(root, "out.env"), "-verbose",
+                        "0", "-f_stat", "1"], cwd=root, capture_output=True, text=True, timeout=600,
+                       preexec_fn=helpers._big_stack)
+    assert r.returncode == 0, r.stderr[-400:]
+    mine = str(tmp_path / "mine")
+    os.makedirs(mine)
+    B = sloshgpu.Basin(case["dta"], case["bntline"])
+    X = sloshgpu.Batch(B, [case["trk"]], precision=64)
+    X.upload()
+    X.set_maxstat(True)
+    ht1, ht2 = sloshgpu.track_datums(case["trk"])
+    y, mo, d, hr, mi = X.start_date(0)
+    start = (datetime.date(y, mo, d) - datetime.date(1970, 1, 1)).days * 86400.0
+    start = start + hr * 3600.
+    start = start + mi * 60
+    clock = start
+    rex_clock = 0.0
+    while rex_clock < start:
+        rex_clock += 10 * 60
+    R = sloshgpu.Rex(os.path.join(mine, "out.rex"), "FileVersion=2013", rexutil.rex_abbrev(case["basin"]),
+                     B.imxb, B.jmxb)
+    names = set()
+
+    def save(t6, at):
+        R.frame(t6, at, X.rex_frames("HBMAX")[0])
+        name = "out%03d.env" % int((at - start) / 3600. + .5)
+        sloshgpu.write_env(os.path.join(mine, name), X.maxstat_env()[0], ht1, ht2)
+        names.add(name)
+
+    t6 = None
+    for n in range(1, X.nsteps[0] + 1):
+        X.step(1)
+        tr = X.track(0, n)
+        clock = clock + tr[2]
+        t6 = [tr[3], -tr[4], tr[7], tr[8], tr[5], tr[6]]
+        if clock >= rex_clock:
+            save(t6, clock)
+            rex_clock += 10 * 60
+            X.maxstat_reset()
+    if rex_clock != clock + 10 * 60:
+        save(t6, rex_clock)
+    R.env(case["trk"], X.rex_frames("HMX")[0])
+    R.close()
+    X.close()
+    B.close()
+    ref_names = sorted(f for f in os.listdir(root) if f.startswith("out") and f.endswith(".env"))
+    assert sorted(names) == ref_names and len(ref_names) >= 3
+    for f in ref_names:
+        a, c = helpers.read_env(os.path.join(mine, f)), helpers.read_env(os.path.join(root, f))
+        assert a[:2] == c[:2] and a[4] == c[4], f
+        assert np.array_equal(a[3], c[3]), f
+    assert open(os.path.join(mine, "out.rex"), "rb").read() == open(os.path.join(root, "out.rex"), "rb").read()
+
+
+def test_maxstat_streams_and_graphs(tmp_path):
+    """The windowed maxima are updated on every launch path: the batch in 3
+    parts on 3 streams and the step loop replayed as hipGraphs give the
+    one-stream values, storm by storm."""
+    b, t = synth.tiny(seed=74, ni=30, nj=26, hours=8)
+    root = str(tmp_path)
+    case = helpers.make_case(root, b, t)
+    rng = np.random.default_rng(74)
+    trks = [case["trk"]]
+    for k in range(6):
+        p = os.path.join(root, "m%d.trk" % k)
+        with open(p, "w") as fh:
+            fh.write(synth.trk_text(synth.random_track(b, rng, hours_range=(6, 8))))
+        trks.append(p)
+    B = sloshgpu.Basin(case["dta"], case["bntline"])
+    res = []
+    for mode in ("one", "parts", "graphs"):
+        X = sloshgpu.Batch(B, trks)
+        X.upload()
+        X.set_path(sloshgpu.PATH_FUSED, 8)
+        if mode == "parts":
+            X.set_streams(3)
+        if mode == "graphs":
+            X.set_graph(20)
+        X.set_maxstat(True)
+        X.step(130)
+        X.maxstat_reset()
+        X.step(170)
+        X.sync()
+        res.append([X.get(s, "HBMAX") for s in range(X.n)])
+        X.close()
+    B.close()
+    for r in res[1:]:
+        for s in range(len(trks)):
+            assert np.array_equal(res[0][s], r[s]), s
