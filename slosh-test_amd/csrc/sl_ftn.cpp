@@ -6,7 +6,7 @@
 // TMSTPM and CLNUP and reads a few COMMON blocks directly (slosh2.c:24-303,
 // savellx.c:686-700).  Linking that driver against this library instead of
 // the Fortran objects runs the same CLI -- dta/bnt/trk in, env and rex files
-// out, written by the driver's own EnvSave / RexSaveStep -- with the time step
+// out, written by the C driver's own EnvSave / RexSaveStep -- with the time step
 // on the GPU.  The ABI is the reference's default build (-DDOUBLE_FORTRAN):
 // double arrays [BAS_Y][BAS_X] = [1655][999], #pragma pack(2) structs.
 //
@@ -100,7 +100,7 @@ int initalcommon_(void)
    return 0;   // the mirrors start zeroed
 }
 
-// With -f_stat N (N != 0; usrparse.c:357-363) the driver's windowed maxima
+// With -f_stat N (N != 0; usrparse.c:357-363) the C driver's windowed maxima
 // (MaxStatUpdateGrid, slosh2.c:1264-1316) read COMMON /DUMB7/ HB after every
 // step.  The driver passes the model no sign of it, so the library looks for
 // the option on the process's command line.

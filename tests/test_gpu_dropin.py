@@ -83,7 +83,7 @@ def test_gpu_cli_fp64_polar_100x90_matches_reference_cli(tmp_path):
 
 
 def test_gpu_cli_fp64_f_stat_matches_reference_cli(tmp_path):
-    """P-Surge windowed maxima (-f_stat 1): the driver's MaxStatUpdateGrid
+    """P-Surge windowed maxima (-f_stat 1): the C driver's MaxStatUpdateGrid
     (slosh2.c:1264-1316) reads COMMON HB after every step, so the drop-in keeps
     it current in this mode.  The env file of every window (ExtendEnvName,
     slosh2.c:1998-2011) and the rex file of the window maxima equal the
