@@ -77,15 +77,16 @@ int slg_batch_sync(slg_batch *x, char *err, int errlen);
 #define SLG_K_STEPB 8    /* fused path: SMPT2G..MOMNTM in one tiled kernel */
 #define SLG_NKERN 9
 /* Launch paths.  FUSED (default): CONTTY, the cascade replay, then one kernel
- * for SMPT2G/FLT2G/BDRYHT/HMXSV/FILTER/MOMNTM on LDS tiles of 64 x tile_rows
- * points.  STAGED: one kernel per routine (slots 0-7).  Results are identical.
+ * for SMPT2G/FLT2G/BDRYHT/HMXSV/FILTER/MOMNTM on LDS tiles of 512 points,
+ * (512 / tile_rows) x tile_rows.  STAGED: one kernel per routine (slots 0-7).  Results are identical.
  * Basins with sub-grid 1-D flow sides (1-D points, weirs, cuts: FLW1DM_C,
  * flw1dm_block_c.f:83-435, and FLW1DM_M, flw1dm_block_m.f:1-403, plus
  * SMPT2G's side pass, smpt2g.f:101-123) step on the staged path only: FUSED
  * and hipGraphs return SLG_EUNSUP for them. */
 #define SLG_PATH_FUSED 0
 #define SLG_PATH_STAGED 1
-/* Choose the launch path and the fused tile height (8, 16 or 32 rows). */
+/* Choose the launch path and the fused tile height: 8, 16 or 32 rows, for
+ * tiles of 64 x 8, 32 x 16 or 16 x 32 points. */
 int slg_batch_set_path(slg_batch *x, int path, int tile_rows);
 /* Replay the fused step loop as hipGraphs of steps_per_graph steps (0: off).
  * The captured steps let every storm's own step flags decide, so one graph

@@ -1479,8 +1479,8 @@ extern "C" int slg_write_env(const char *path, int imxb, int jmxb, const char *c
    return ok ? 0 : SLG_EINPUT;
 }
 
-// Online autotuner: time each launch configuration (fused with 8, 16 and 32
-// row tiles, and staged) on consecutive chunks of the run's own steps -- every
+// Online autotuner: time each launch configuration (fused with 64 x 8, 32 x 16
+// and 16 x 32 tiles, and staged) on consecutive chunks of the run's own steps -- every
 // configuration computes identical fields, so the trial steps are real
 // progress -- and keep the fastest.  ms_out (4 entries, may be NULL) receives
 // the time per step of each candidate.

@@ -1676,8 +1676,9 @@ __global__ void __launch_bounds__(TX * TY) k_momntm(SlBasinDev B, SlRunDev R, St
 }
 
 // ------------------------------------------------------------ fused stage B
-// One workgroup = one 64 x TH tile of one storm.  Regions (halo below / above
-// the tile, in points, each direction):
+// One workgroup = one tile of one storm: 512 points, (512 / TH) x TH (64 x 8,
+// 32 x 16 or 16 x 32; the autotuner picks).  Regions (halo below / above the
+// tile, in points, each direction):
 //   F  final HB MOMNTM reads                 1 / 0
 //   S  HB after FLT2G+BDRYHT (HMXSV, filter) SLO / SHI  (2/2 with a filter, else 1/1)
 //   C  height classes FLT2G reads            S + 1
@@ -1690,19 +1691,20 @@ __global__ void __launch_bounds__(TX * TY) k_momntm(SlBasinDev B, SlRunDev R, St
 // point's inputs are in LDS; only the tile's own points are written back.
 template <int LO, int HI, int TH>
 struct Reg {
-   static constexpr int lo = LO, hi = HI, w = TX + LO + HI, h = TH + LO + HI, n = w * h;
+   static constexpr int lo = LO, hi = HI, w = 512 / TH + LO + HI, h = TH + LO + HI, n = w * h;
 };
 
 // SMPT / FILT size the regions and, with dyn == 0, decide the stages; with
 // dyn != 0 (storms at different steps in one launch) each storm's own step
 // flags decide, inside the regions of the <TH, true, true> instance.
-// The FP32 TH=8 instances without FILTER are held to 64 VGPRs, so that 8
+// The FP32 instances without FILTER are held to 64 VGPRs, so that 8
 // waves fit on a SIMD (their LDS allows it; the kernel is latency-bound and
 // runs 7 % faster so).
 template <int TH, bool SMPT, bool FILT>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TH == 8 && !FILT && sizeof(real) == 4 ? 8 : 1)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(!FILT && sizeof(real) == 4 ? 8 : 1)))
 k_stepB(SlBasinDev B, SlRunDev R, StepArgs A, int ntx, int nty, int dyn)
 {
+   constexpr int TW = 512 / TH;   // tile width: every tile has 512 points
    constexpr int SLO = FILT ? 2 : 1, SHI = FILT ? 2 : 1;
    using RS = Reg<SLO, SHI, TH>;
    using RC = Reg<SLO + 1, SHI + 1, TH>;
@@ -1726,7 +1728,7 @@ k_stepB(SlBasinDev B, SlRunDev R, StepArgs A, int ntx, int nty, int dyn)
    const int step = storm_step(R, s);
    if (!step) return;   // the whole workgroup: its storm's run is complete
    const int tid = threadIdx.x;
-   const int I0 = tx * TX + 1, J0 = ty * TH + 1;
+   const int I0 = tx * TW + 1, J0 = ty * TH + 1;
    const int IM = B.imxb, JM = B.jmxb;
    const size_t so = (size_t) s * R.fstride;
    const real *hs_g = A.hc + so;
@@ -1788,7 +1790,7 @@ k_stepB(SlBasinDev B, SlRunDev R, StepArgs A, int ntx, int nty, int dyn)
          if (gi >= 1 && gi <= IM && gj >= 1 && gj <= JM) {
             int nd;
             f = pflag_pt(B, gi, gj, uzL, hsL, zL, &nd);
-            if (gi >= I0 && gi < I0 + TX && gj >= J0 && gj < J0 + TH) {
+            if (gi >= I0 && gi < I0 + TW && gj >= J0 && gj < J0 + TH) {
                np += f != 0;
                ndt += nd;
             }
@@ -1809,7 +1811,7 @@ k_stepB(SlBasinDev B, SlRunDev R, StepArgs A, int ntx, int nty, int dyn)
          uint8_t c = 0;
          if (gi >= 1 && gi <= IM && gj >= 1 && gj <= JM) {
             c = hclass_pt(B, gi, gj, hsL, pfL, zL);
-            if (gi >= I0 && gi < I0 + TX && gj >= J0 && gj < J0 + TH) {
+            if (gi >= I0 && gi < I0 + TW && gj >= J0 && gj < J0 + TH) {
                n1 += c == 1;
                n2 += c == 2;
             }
@@ -1839,8 +1841,8 @@ k_stepB(SlBasinDev B, SlRunDev R, StepArgs A, int ntx, int nty, int dyn)
    const LF sL{sh_s, I0 - RS::lo, J0 - RS::lo, RS::w};
    // HMXSV on the tile's own squares
    real *hmx = R.hmx + so;
-   for (int e = tid; e < TX * TH; e += 256) {
-      const int gi = I0 + e % TX, gj = J0 + e / TX;
+   for (int e = tid; e < TW * TH; e += 256) {
+      const int gi = I0 + e % TW, gj = J0 + e / TW;
       if (gi <= IM - 1 && gj <= JM - 1) {
          const uint32_t p = IDX(B, gi, gj);
          const real hx = hmx[p];
@@ -1863,13 +1865,13 @@ k_stepB(SlBasinDev B, SlRunDev R, StepArgs A, int ntx, int nty, int dyn)
    const LF fL{filt ? sh_f : sh_s, I0 - (filt ? RF::lo : RS::lo), J0 - (filt ? RF::lo : RS::lo),
                filt ? RF::w : RS::w};
    real *ub = R.ub + so, *vb = R.vb + so;
-   for (int e = tid; e < TX * TH; e += 256) {
-      int gi = I0 + e % TX;
+   for (int e = tid; e < TW * TH; e += 256) {
+      int gi = I0 + e % TW;
       // the column is the same on every pass of this loop; kept opaque so the
       // compiler loads its metrics on each pass instead of holding their
       // addresses across MOMNTM (which spills them at 64 VGPRs)
       asm volatile("" : "+v"(gi));
-      const int gj = J0 + e / TX;
+      const int gj = J0 + e / TW;
       if (gi <= IM && gj <= JM) {
          hn_g[IDX(B, gi, gj)] = fL(gi, gj);
          const uint32_t fl = B.flags[IDX(B, gi, gj)];
@@ -2183,11 +2185,12 @@ extern "C" int SL_LAUNCH(const SlBasinDev *B, const SlRunDev *R, int smpt, int f
       a.hc = h1;
       a.hn = h0;
       const int th = (bth == 8 || bth == 32) ? bth : 16;
+      const int nbx = (B->imxb + 512 / th - 1) / (512 / th);
       nty = (B->jmxb + th - 1) / th;
-      const dim3 g(ntx * nty * R->nstorm);
-      if (th == 8) launch_B<8>(smpt, filt, dyn, g, st, *B, *R, a, ntx, nty);
-      else if (th == 32) launch_B<32>(smpt, filt, dyn, g, st, *B, *R, a, ntx, nty);
-      else launch_B<16>(smpt, filt, dyn, g, st, *B, *R, a, ntx, nty);
+      const dim3 g(nbx * nty * R->nstorm);
+      if (th == 8) launch_B<8>(smpt, filt, dyn, g, st, *B, *R, a, nbx, nty);
+      else if (th == 32) launch_B<32>(smpt, filt, dyn, g, st, *B, *R, a, nbx, nty);
+      else launch_B<16>(smpt, filt, dyn, g, st, *B, *R, a, nbx, nty);
       MARK(3);
       hipLaunchKernelGGL(k_advance, gadv, badv, 0, st, *R);
       hipError_t e = hipGetLastError();

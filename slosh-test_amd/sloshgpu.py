@@ -132,6 +132,7 @@ class Batch:
         _chk(L.slg_batch_open_prec(basin.h, self.n, arr, precision, ctypes.byref(self.h), err, 512), err)
         self.precision = precision
         self.nsteps = [L.slg_batch_nsteps(self.h, s) for s in range(self.n)]
+        self._rexbuf = None
 
     def export(self, storm: int, init_path: str, rows_path: str, tables_path: str):
         L = lib()
@@ -158,7 +159,7 @@ class Batch:
         return lib().slg_batch_streams(self.h)
 
     def autotune(self, steps_per_trial: int = 20):
-        """Times fused 8/16/32-row tiles and the staged path on real steps and
+        """Times fused 64x8/32x16/16x32 tiles and the staged path on real steps and
         keeps the fastest; returns (tile_rows, path, ms per step of each)."""
         rows, path = ctypes.c_int(), ctypes.c_int()
         ms = np.zeros(4)
@@ -265,7 +266,12 @@ class Batch:
         ns = self.n - s0 if ns is None else ns
         nj, ni = self.basin.jmxb, self.basin.imxb
         cap = (18 * (ni - 1) * (nj - 1) + 7) // 8
-        buf = np.zeros(ns * cap, dtype=np.uint8)
+        # one staging buffer per batch, kept between calls (a frame of 18 bits
+        # a cell is the longest possible; fresh pages of that size each call
+        # cost more than the encoding)
+        if self._rexbuf is None or self._rexbuf.size < ns * cap:
+            self._rexbuf = np.empty(ns * cap, dtype=np.uint8)
+        buf = self._rexbuf
         nb = np.zeros(ns, dtype=np.int32)
         err = ctypes.create_string_buffer(512)
         _chk(lib().slg_batch_rex_frames(self.h, field.encode(), s0, ns, buf.ctypes.data_as(ctypes.c_void_p), cap,
@@ -273,6 +279,7 @@ class Batch:
         return [bytes(buf[k * cap:k * cap + nb[k]]) for k in range(ns)]
 
     def close(self):
+        self._rexbuf = None
         if self.h:
             lib().slg_batch_close(self.h)
             self.h = ctypes.c_void_p()
