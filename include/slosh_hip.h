@@ -22,6 +22,7 @@ extern "C" {
 #define SLG_EUNSUP (-2)  /* basin or track feature this build does not implement */
 #define SLG_EHIP (-3)    /* HIP runtime error (device missing, out of memory, ...) */
 #define SLG_ELIMIT (-4)  /* a reference hard limit was reached (the reference STOPs) */
+#define SLG_MAX_POINTS 1048576 /* squares of one point series (slg_batch_rex_points) */
 
 typedef struct slg_basin slg_basin;
 typedef struct slg_batch slg_batch;
@@ -176,6 +177,21 @@ int slg_rex_open(const char *path, const char *comment, const char *abbrev, int 
 int slg_rex_frame(slg_rex *r, const double track[6], double clock, const unsigned char *bits, int nbytes);
 int slg_rex_env(slg_rex *r, const char *trk_path, const unsigned char *bits, int nbytes);
 void slg_rex_close(slg_rex *r);
+/* Rex point series without the frames.  slg_batch_rex_points gives the value
+ * a rex frame of field ("HHB", "HMX" or "HBMAX", as slg_batch_rex_frames)
+ * holds at each of npts squares xy[2p], xy[2p+1] (1-based I, J, at most
+ * IMXB-1, JMXB-1): tenths of a foot, -320..700, 999 dry; out[ns][npts].
+ * These are the values the reference's rexout reads back from a rex file
+ * (sorc/rexout.cd/rexout.c:707-740).  slg_pnt_* write them as rexout's
+ * style 0 text (rexout.c:631-639, 736-742), one row per frame with the
+ * model clock's date. */
+int slg_batch_rex_points(slg_batch *x, const char *field, int s0, int ns, int npts, const int *xy, short *out,
+                         char *err, int errlen);
+typedef struct slg_pnt slg_pnt;
+int slg_pnt_open(const char *path, const char *abbrev, const char *storm, const char *rexfile, int npts,
+                 const int *xy, slg_pnt **out);
+int slg_pnt_row(slg_pnt *p, double clock, const short *vals);
+void slg_pnt_close(slg_pnt *p);
 /* Write an env file in EnvSave's layout (slosh2.c:1611-1671). */
 int slg_write_env(const char *path, int imxb, int jmxb, const char *comment, const short *vals, float ht1,
                   float ht2);

@@ -1340,15 +1340,16 @@ static int batch_rex(slg_batch *X, int src, int s0, int ns, unsigned char *buf, 
 }
 
 // rex v2 frames of storms s0 .. s0+ns-1 from HHB of the last step or HMX
-extern "C" int slg_batch_rex_frames(slg_batch *X, const char *field, int s0, int ns, unsigned char *buf, long cap,
-                                    int *nbytes, char *err, int errlen)
+// the source of rex values: 0 HHB of the last step, 1 HMX, 2 the windowed
+// maxima; negative: an SLG_E* code, err set
+static int rex_source(slg_batch *X, const char *field, int s0, int ns, char *err, int errlen)
 {
    if (!X->dmem) {
       seterr(err, errlen, "%s", "batch not uploaded");
       return SLG_EINPUT;
    }
-   if (!field || !buf || !nbytes || s0 < 0 || ns < 0 || s0 + ns > X->nstorm || cap < 1) {
-      seterr(err, errlen, "%s", "bad storm range or buffer");
+   if (!field || s0 < 0 || ns < 0 || s0 + ns > X->nstorm) {
+      seterr(err, errlen, "%s", "bad storm range");
       return SLG_EINPUT;
    }
    const bool hhb = !strcmp(field, "HHB");
@@ -1370,9 +1371,74 @@ extern "C" int slg_batch_rex_frames(slg_batch *X, const char *field, int s0, int
          return SLG_EINPUT;
       }
    }
+   return src;
+}
+
+extern "C" int slg_batch_rex_frames(slg_batch *X, const char *field, int s0, int ns, unsigned char *buf, long cap,
+                                    int *nbytes, char *err, int errlen)
+{
+   if (!buf || !nbytes || cap < 1) {
+      seterr(err, errlen, "%s", "bad buffer");
+      return SLG_EINPUT;
+   }
+   const int src = rex_source(X, field, s0, ns, err, errlen);
+   if (src < 0) return src;
    if (ns == 0) return 0;
    return X->prec == 64 ? batch_rex<double>(X, src, s0, ns, buf, cap, nbytes, err, errlen)
                         : batch_rex<float>(X, src, s0, ns, buf, cap, nbytes, err, errlen);
+}
+
+template <class R>
+static int batch_rex_points(slg_batch *X, int src, int s0, int ns, int npts, const int *k, short *out)
+{
+   slg_basin *B = X->b;
+   const SlRunDevT<R> &D = run(X, R());
+   const R *hbs = src == 0 ? (X->hbs_buf ? D.hb1 : D.hb0) : nullptr;
+   const size_t need = sizeof(int) * (size_t) npts + sizeof(int16_t) * (size_t) npts * ns;
+   if (X->pt_cap < need) {
+      if (hipStreamSynchronize(X->st) != hipSuccess) return SLG_EHIP;
+      if (X->pt_buf) (void) hipFree(X->pt_buf);
+      X->pt_buf = nullptr;
+      X->pt_cap = 0;
+      if (hipMalloc(&X->pt_buf, need) != hipSuccess) return SLG_EHIP;
+      X->pt_cap = need;
+   }
+   int *dk = (int *) X->pt_buf;
+   int16_t *dout = (int16_t *) (dk + npts);
+   if (hipMemcpyAsync(dk, k, sizeof(int) * (size_t) npts, hipMemcpyHostToDevice, X->st) != hipSuccess ||
+       dev_rex_points(&img(B, R()).dev, &D, hbs, src == 2 ? X->hbmax : nullptr, img(B, R()).zbd, s0, ns, dk, npts,
+                      dout, X->st) ||
+       hipMemcpyAsync(out, dout, sizeof(int16_t) * (size_t) npts * ns, hipMemcpyDeviceToHost, X->st) != hipSuccess ||
+       hipStreamSynchronize(X->st) != hipSuccess)
+      return SLG_EHIP;
+   return 0;
+}
+
+// rex values at listed squares of storms s0 .. s0+ns-1 (rexout's series)
+extern "C" int slg_batch_rex_points(slg_batch *X, const char *field, int s0, int ns, int npts, const int *xy,
+                                    short *out, char *err, int errlen)
+{
+   if (npts < 0 || npts > SLG_MAX_POINTS || (npts && (!xy || !out))) {
+      seterr(err, errlen, "%s", "bad point list or output buffer");
+      return SLG_EINPUT;
+   }
+   const int src = rex_source(X, field, s0, ns, err, errlen);
+   if (src < 0) return src;
+   if (ns == 0 || npts == 0) return 0;
+   const int I1 = X->b->h.imxb - 1, J1 = X->b->h.jmxb - 1;
+   std::vector<int> k(npts);
+   for (int p = 0; p < npts; p++) {
+      const int x = xy[2 * p], y = xy[2 * p + 1];
+      if (x < 1 || x > I1 || y < 1 || y > J1) {
+         seterr(err, errlen, "point %s%d is outside the rex frame's squares", "", p);
+         return SLG_EINPUT;
+      }
+      k[p] = (x - 1) * J1 + (y - 1);
+   }
+   const int r = X->prec == 64 ? batch_rex_points<double>(X, src, s0, ns, npts, k.data(), out)
+                               : batch_rex_points<float>(X, src, s0, ns, npts, k.data(), out);
+   if (r) seterr(err, errlen, "%s", "rex point kernel or copy failed");
+   return r;
 }
 
 // ---- P-Surge windowed maxima (slosh2.c:1227-1316) --------------------------
@@ -1534,6 +1600,7 @@ extern "C" void slg_batch_close(slg_batch *X)
    drop_graph(X);
    rex_free(X);
    if (X->hbmax) (void) hipFree(X->hbmax);
+   if (X->pt_buf) (void) hipFree(X->pt_buf);
    for (int p = 0; p < SL_MAXSPLIT - 1; p++)
       if (X->stx[p]) {
          (void) hipStreamSynchronize(X->stx[p]);

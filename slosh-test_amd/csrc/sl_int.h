@@ -115,6 +115,9 @@ struct slg_batch {
    unsigned char *rex_host = nullptr;
    int rex_chunk = 0;
    long rex_cap = 0;
+   // device buffer of slg_batch_rex_points: square indices, then the values
+   void *pt_buf = nullptr;
+   size_t pt_cap = 0;
    hipStream_t stx[SL_MAXSPLIT - 1] = {};
    hipEvent_t ev_join[SL_MAXSPLIT - 1] = {};
    hipEvent_t ev_phase[SL_MAXSPLIT][SLG_NKERN + 1] = {};   // marks of each part's first step
@@ -158,6 +161,24 @@ extern "C" int sl_dev_rex_f32(const SlBasinDevT<float> *B, const SlRunDevT<float
 extern "C" int sl_dev_rex_f64(const SlBasinDevT<double> *B, const SlRunDevT<double> *R, const double *hbs,
                               const double *hmax, const double *zbd, int s0, int ns, unsigned char *out, long cap,
                               int *nbytes, unsigned *scr, hipStream_t st);
+extern "C" int sl_dev_rex_points_f32(const SlBasinDevT<float> *B, const SlRunDevT<float> *R, const float *hbs,
+                                     const double *hmax, const double *zbd, int s0, int ns, const int *ks, int npts,
+                                     int16_t *out, hipStream_t st);
+extern "C" int sl_dev_rex_points_f64(const SlBasinDevT<double> *B, const SlRunDevT<double> *R, const double *hbs,
+                                     const double *hmax, const double *zbd, int s0, int ns, const int *ks, int npts,
+                                     int16_t *out, hipStream_t st);
+static inline int dev_rex_points(const SlBasinDevT<float> *B, const SlRunDevT<float> *R, const float *hbs,
+                                 const double *hmax, const double *zbd, int s0, int ns, const int *ks, int npts,
+                                 int16_t *out, hipStream_t st)
+{
+   return sl_dev_rex_points_f32(B, R, hbs, hmax, zbd, s0, ns, ks, npts, out, st);
+}
+static inline int dev_rex_points(const SlBasinDevT<double> *B, const SlRunDevT<double> *R, const double *hbs,
+                                 const double *hmax, const double *zbd, int s0, int ns, const int *ks, int npts,
+                                 int16_t *out, hipStream_t st)
+{
+   return sl_dev_rex_points_f64(B, R, hbs, hmax, zbd, s0, ns, ks, npts, out, st);
+}
 extern "C" int sl_dev_rex_pack(const unsigned char *src, long cap, const int *nbytes, const long *off, int ns,
                                unsigned char *dst, hipStream_t st);
 static inline int dev_rex(const SlBasinDevT<float> *B, const SlRunDevT<float> *R, const float *hbs,

@@ -2135,6 +2135,36 @@ extern "C" int SL_REX(const SlBasinDev *B, const SlRunDev *R, const real *hbs, c
    return e == hipSuccess ? 0 : -(int) e;
 }
 
+// rex values at listed squares, without the frames: storm s0 + blockIdx.y,
+// square ks[p] in frame order ((I-1)(JMXB-1) + J-1); the value a rex frame
+// holds there (tenths of a foot, -320..700, 999 dry), as rexout reads it back
+// (sorc/rexout.cd/rexout.c:707-740)
+__global__ __launch_bounds__(256) void k_rex_points(SlBasinDev B, SlRunDev R, const real *hbs, const double *hmax,
+                                                    const double *zbd, int s0, const int *ks, int npts,
+                                                    int16_t *out)
+{
+   const int p = blockIdx.x * 256 + threadIdx.x;
+   if (p >= npts) return;
+   const size_t so = (size_t) (s0 + blockIdx.y) * R.fstride;
+   const int c = rex_code(B, hbs ? hbs + so : nullptr, R.hmx + so, hmax ? hmax + so : nullptr, zbd, ks[p],
+                          B.jmxb - 1) - 320;
+   out[(size_t) blockIdx.y * npts + p] = (int16_t) (c == 701 ? 999 : c);
+}
+
+#ifdef SL_F64
+#define SL_REXPTS sl_dev_rex_points_f64
+#else
+#define SL_REXPTS sl_dev_rex_points_f32
+#endif
+extern "C" int SL_REXPTS(const SlBasinDev *B, const SlRunDev *R, const real *hbs, const double *hmax,
+                         const double *zbd, int s0, int ns, const int *ks, int npts, int16_t *out, hipStream_t st)
+{
+   hipLaunchKernelGGL(k_rex_points, dim3((npts + 255) / 256, ns), dim3(256), 0, st, *B, *R, hbs, hmax, zbd, s0, ks,
+                      npts, out);
+   hipError_t e = hipGetLastError();
+   return e == hipSuccess ? 0 : -(int) e;
+}
+
 #ifdef SL_F64
 #define SL_MAXSTAT sl_dev_maxstat_f64
 #else

@@ -285,3 +285,68 @@ void slg_rex_close(slg_rex *r)
    if (r->fp) fclose(r->fp);
    free(r);
 }
+
+/* Point series in rexout's style 0 (sorc/rexout.cd/rexout.c:631-639 header,
+ * 736-742 rows): the values a rex file's frames hold at the listed squares,
+ * written without the frames.  The basin field is the rex header's 4-byte
+ * abbreviation as rexout prints it (a leading blank dropped, a trailing blank
+ * cut, rexout.c:534-543); each row gives the frame's date from the model clock
+ * (as slg_rex_frame stores it) and value / 10 in float with one decimal. */
+struct slg_pnt {
+   FILE *fp;
+   int npts;
+};
+
+int slg_pnt_open(const char *path, const char *abbrev, const char *storm, const char *rexfile, int npts,
+                 const int *xy, slg_pnt **out)
+{
+   *out = NULL;
+   if (!path || npts < 0 || npts > SLG_MAX_POINTS || (npts && !xy)) return SLG_EINPUT;
+   char abv[5] = {0, 0, 0, 0, 0};
+   if (abbrev) {
+      const size_t la = strlen(abbrev);
+      memcpy(abv, abbrev, la < 4 ? la : 4);
+   }
+   if (abv[0] == ' ') {
+      abv[0] = abv[1];
+      abv[1] = abv[2];
+      abv[2] = abv[3];
+      abv[3] = ' ';
+   }
+   if (abv[3] == ' ') abv[3] = '\0';
+   slg_pnt *p = (slg_pnt *) calloc(1, sizeof *p);
+   if (!p) return SLG_EINPUT;
+   p->fp = fopen(path, "w");
+   p->npts = npts;
+   if (!p->fp) {
+      free(p);
+      return SLG_EINPUT;
+   }
+   fprintf(p->fp, "Basin=%s\nStorm=%s\nRexFile=%s\nDate, Time", abv, storm ? storm : "", rexfile ? rexfile : "");
+   for (int k = 0; k < npts; k++) fprintf(p->fp, ", (%d %d)", xy[2 * k], xy[2 * k + 1]);
+   fprintf(p->fp, "\n");
+   *out = p;
+   return ferror(p->fp) ? SLG_EINPUT : 0;
+}
+
+int slg_pnt_row(slg_pnt *p, double clock, const short *vals)
+{
+   if (!p || (p->npts && !vals)) return SLG_EINPUT;
+   int year, mon, day, hr, mn;
+   double sec;
+   clock_date(clock, &year, &mon, &day, &hr, &mn, &sec);
+   fprintf(p->fp, "%02d/%02d/%d, %02d:%02d:%02d", mon, day, year, hr, mn, (int) (unsigned char) sec);
+   for (int k = 0; k < p->npts; k++) {
+      const float g = (float) (vals[k] / 10.0);
+      fprintf(p->fp, ", %.1f", g);
+   }
+   fprintf(p->fp, "\n");
+   return ferror(p->fp) ? SLG_EINPUT : 0;
+}
+
+void slg_pnt_close(slg_pnt *p)
+{
+   if (!p) return;
+   if (p->fp) fclose(p->fp);
+   free(p);
+}

@@ -65,6 +65,11 @@ def lib():
         L.slg_rex_env.argtypes = [vp, cp, vp, i]
         L.slg_rex_close.argtypes = [vp]
         L.slg_rex_close.restype = None
+        L.slg_batch_rex_points.argtypes = [vp, cp, i, i, i, vp, vp, cp, i]
+        L.slg_pnt_open.argtypes = [cp, cp, cp, cp, i, vp, ctypes.POINTER(vp)]
+        L.slg_pnt_row.argtypes = [vp, ctypes.c_double, vp]
+        L.slg_pnt_close.argtypes = [vp]
+        L.slg_pnt_close.restype = None
         L.slg_write_env.argtypes = [cp, i, i, cp, vp, ctypes.c_float, ctypes.c_float]
         L.slg_server_open.argtypes = [i, ctypes.c_double, ctypes.POINTER(vp), cp, i]
         L.slg_server_add_basin.argtypes = [vp, cp, cp, i, i, i, ctypes.POINTER(i), cp, i]
@@ -278,10 +283,45 @@ class Batch:
                                         nb.ctypes.data_as(ctypes.c_void_p), err, 512), err)
         return [bytes(buf[k * cap:k * cap + nb[k]]) for k in range(ns)]
 
+    def rex_points(self, xy, field="HHB", s0=0, ns=None):
+        """rex values (tenths of a foot, 999 dry) at squares xy [(I, J), ...]
+        (1-based) of storms s0..s0+ns-1, computed on the device from the same
+        field as rex_frames; int16 [ns, npts]."""
+        ns = self.n - s0 if ns is None else ns
+        pts = np.ascontiguousarray(np.asarray(xy, dtype=np.int32).reshape(-1, 2))
+        out = np.zeros((ns, len(pts)), dtype=np.int16)
+        err = ctypes.create_string_buffer(512)
+        _chk(lib().slg_batch_rex_points(self.h, field.encode(), s0, ns, len(pts), pts.ctypes.data_as(ctypes.c_void_p),
+                                        out.ctypes.data_as(ctypes.c_void_p), err, 512), err)
+        return out
+
     def close(self):
         self._rexbuf = None
         if self.h:
             lib().slg_batch_close(self.h)
+            self.h = ctypes.c_void_p()
+
+
+class PointSeries:
+    """rexout style-0 text of a point series (slg_pnt_*): the header, then one
+    row per frame time (model clock, seconds since 1970) with the values of
+    Batch.rex_points for one storm."""
+
+    def __init__(self, path: str, abbrev: str, storm: str, rexfile: str, xy):
+        self.pts = np.ascontiguousarray(np.asarray(xy, dtype=np.int32).reshape(-1, 2))
+        self.h = ctypes.c_void_p()
+        if lib().slg_pnt_open(path.encode(), abbrev.encode(), storm.encode(), rexfile.encode(), len(self.pts),
+                              self.pts.ctypes.data_as(ctypes.c_void_p), ctypes.byref(self.h)):
+            raise SloshError("cannot open point series %s" % path)
+
+    def row(self, clock: float, vals):
+        v = np.ascontiguousarray(vals, dtype=np.int16)
+        if v.size != len(self.pts) or lib().slg_pnt_row(self.h, clock, v.ctypes.data_as(ctypes.c_void_p)):
+            raise SloshError("point series row failed")
+
+    def close(self):
+        if self.h:
+            lib().slg_pnt_close(self.h)
             self.h = ctypes.c_void_p()
 
 

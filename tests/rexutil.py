@@ -16,8 +16,54 @@ import sloshgpu
 REXREF = os.path.join(helpers.REPO, "oracle", "_ref", "rexref64")
 
 
+REXOUT = os.path.join(helpers.REPO, "oracle", "_ref", "rexout")
+
+
 def have_rexref():
     return os.path.exists(REXREF)
+
+
+def have_rexout():
+    return os.path.exists(REXOUT)
+
+
+def rexout_abbrev(abbrev):
+    """The rex header's abbreviation as rexout normalises it (rexout.c:534-543):
+    the pnt file's first line must equal it (rexout.c:544-547)."""
+    a = (abbrev + "\0" * 4)[:4]
+    a = a[1:] + " " if a[0] == " " else a
+    return (a[:3] if a[3] == " " else a).split("\0")[0]
+
+
+def reference_rexout(rexfile, abbrev, xy, root):
+    """The reference's rexout -style 0 over rexfile at squares xy (its pnt
+    file: the basin abbreviation, then one "I J" line per square); the text."""
+    pnt, out = os.path.join(root, "pts.pnt"), os.path.join(root, "pts.ref.txt")
+    with open(pnt, "w") as fh:
+        fh.write(abbrev + "\n" + "".join("%d %d\n" % (x, y) for x, y in xy))
+    subprocess.run([REXOUT, "-rex", rexfile, "-pnt", pnt, "-style", "0", "-out", out], check=True, timeout=60,
+                   stdout=subprocess.DEVNULL)
+    with open(out) as fh:
+        return fh.read()
+
+
+def rexout_values(text):
+    """Values of rexout style-0 text: float [frames, points], tenths as int."""
+    rows = [ln.split(", ")[2:] for ln in text.splitlines()[4:]]
+    return np.array([[round(float(v) * 10) for v in r] for r in rows], dtype=np.int64)
+
+
+def compare_series(mine, ref, storm):
+    """rexout's style-0 text against the product's: equal line by line, except
+    the Storm line.  rexout builds the storm name with an overlapping strcpy
+    (rexout.c:296-298, undefined behaviour) that garbles it with this C
+    library; the product prints the name the header holds."""
+    a, b = mine.splitlines(), ref.splitlines()
+    assert len(a) == len(b)
+    assert a[1] == "Storm=" + storm and b[1].startswith("Storm=") and len(b[1]) == len(a[1])
+    for k, (x, y) in enumerate(zip(a, b)):
+        if k != 1:
+            assert x == y, (k, x, y)
 
 
 def oracle_frame(hb, zb):

@@ -148,6 +148,101 @@ def test_batched_rex_file_equals_reference_cli(tmp_path, cid):
     assert mine == ref
 
 
+@pytest.mark.skipif(not (os.path.exists(helpers.REF_CLI) and rexutil.have_rexout()),
+                    reason="reference CLI or rexout (oracle/_ref) not built")
+def test_batched_point_series_equals_reference_rexout(tmp_path):
+    """The rex point series of a batched FP64 run (slg_batch_rex_points at
+    every rex save, written by slg_pnt_*) equals the reference's rexout over
+    the reference CLI's own rex file of the same run."""
+    b, t = synth.tiny(seed=5, ni=30, nj=26, hours=6)
+    root = str(tmp_path)
+    case = helpers.make_case(root, b, t)
+    rexutil.trk_canonical(t, case["trk"])
+    _, rex_ref = helpers.run_cli(helpers.REF_CLI, root, case["basin"], "ref")
+    B = sloshgpu.Basin(case["dta"], case["bntline"])
+    X = sloshgpu.Batch(B, [case["trk"]], precision=64)
+    X.upload()
+    y, mo, d, hr, mi = X.start_date(0)
+    clock = (datetime.date(y, mo, d) - datetime.date(1970, 1, 1)).days * 86400.0 + hr * 3600. + mi * 60
+    rng = np.random.default_rng(11)
+    xy = [(1, 1), (B.imxb - 1, B.jmxb - 1), (1, B.jmxb - 1), (B.imxb - 1, 1)]
+    xy += [(int(x), int(y)) for x, y in zip(rng.integers(1, B.imxb, 60), rng.integers(1, B.jmxb, 60))]
+    abbrev = rexutil.rex_abbrev(case["basin"])
+    series = os.path.join(root, "pts.txt")
+    P = sloshgpu.PointSeries(series, abbrev, "FileVersion=2013", rex_ref, xy)
+    rex_clock, del_t, vals = clock, 0.0, None
+    for n in range(1, X.nsteps[0] + 1):
+        f_pass = n == 1 or clock + del_t >= rex_clock
+        X.set_path(sloshgpu.PATH_STAGED if f_pass else sloshgpu.PATH_FUSED, 16)
+        X.step(1)
+        if f_pass:
+            vals = X.rex_points(xy, "HHB")[0]
+        del_t = X.track(0, n)[2]
+        clock = clock + del_t
+        if clock >= rex_clock:
+            P.row(clock, vals)
+            rex_clock += 10 * 60
+    P.close()
+    X.close()
+    B.close()
+    ref = rexutil.reference_rexout(rex_ref, rexutil.rexout_abbrev(abbrev), xy, root)
+    rexutil.compare_series(open(series).read(), ref, "FileVersion=2013")
+    v = rexutil.rexout_values(ref)
+    assert len(v) > 30 and (v == 999).any() and (v != 999).any()
+
+
+@pytest.mark.skipif(not rexutil.have_rexout(), reason="reference rexout (oracle/_ref) not built")
+@pytest.mark.parametrize("prec", [32, 64])
+def test_rex_points_equal_rexout_of_device_frames(tmp_path, prec):
+    """slg_batch_rex_points of HHB, HMX and HBMAX for every storm of a batch
+    equals what rexout reads back from the device's own rex frames of the same
+    fields; points outside the frame are refused."""
+    root = str(tmp_path)
+    rng = np.random.default_rng(prec)
+    b, t = synth.tiny(seed=7, ni=41, nj=33, hours=8)
+    case = helpers.make_case(root, b, t)
+    trks = [case["trk"]]
+    for k in range(2):
+        p = os.path.join(root, "s%d.trk" % k)
+        with open(p, "w") as fh:
+            fh.write(synth.trk_text(synth.random_track(b, rng, hours_range=(6, 8))))
+        trks.append(p)
+    B = sloshgpu.Basin(case["dta"], case["bntline"])
+    X = sloshgpu.Batch(B, trks, precision=prec)
+    X.upload()
+    X.set_maxstat(1)
+    X.set_path(sloshgpu.PATH_STAGED, 16)
+    xy = [(1, 1), (B.imxb - 1, B.jmxb - 1)]
+    xy += [(int(x), int(y)) for x, y in zip(rng.integers(1, B.imxb, 50), rng.integers(1, B.jmxb, 50))]
+    abbrev = "tny"
+    frames = {f: [[] for _ in trks] for f in ("HHB", "HMX", "HBMAX")}
+    pts = {f: [] for f in frames}
+    done = 0
+    for n in (1, 60, 200):
+        X.step(n - done)
+        done = n
+        print("rex points prec %d step %d" % (prec, n), flush=True)
+        for f in frames:
+            for s, bits in enumerate(X.rex_frames(f)):
+                frames[f][s].append(bits)
+            pts[f].append(X.rex_points(xy, f))
+    zb = B.field("ZB")
+    for f in frames:
+        for s in range(len(trks)):
+            rex = os.path.join(root, "%s%d.rex" % (f, s))
+            rexutil.product_rex(rex, zb, [(np.zeros(6), 1.7e9 + 600 * k, bits) for k, bits in
+                                          enumerate(frames[f][s])], trks[s], frames[f][s][-1], abbrev=abbrev)
+            got = rexutil.rexout_values(rexutil.reference_rexout(rex, abbrev, xy, root))
+            want = np.array([p[s] for p in pts[f]], dtype=np.int64)
+            assert np.array_equal(got, want), (f, s)
+    with pytest.raises(sloshgpu.SloshError):
+        X.rex_points([(B.imxb, 1)], "HMX")
+    with pytest.raises(sloshgpu.SloshError):
+        X.rex_points([(1, 0)], "HMX")
+    X.close()
+    B.close()
+
+
 def test_maxstat_values(tmp_path):
     """Windowed maxima on the device: -ZB at the start and after a reset, then
     the maximum of HB after each step, on every square (MaxStatInitGrid /
