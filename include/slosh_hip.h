@@ -23,6 +23,7 @@ extern "C" {
 #define SLG_EHIP (-3)    /* HIP runtime error (device missing, out of memory, ...) */
 #define SLG_ELIMIT (-4)  /* a reference hard limit was reached (the reference STOPs) */
 #define SLG_MAX_POINTS 1048576 /* squares of one point series (slg_batch_rex_points) */
+#define SLG_MAX_STORMS 65535   /* runs of one batch (a launch grid dimension holds them) */
 
 typedef struct slg_basin slg_basin;
 typedef struct slg_batch slg_batch;
@@ -47,9 +48,10 @@ void slg_basin_close(slg_basin *b);
  * depsfc.f:83-138) or "YLT" (/LLXFLE/). */
 int slg_basin_field(const slg_basin *b, const char *field, double *dst);
 
-/* Open nstorm runs on one basin, one trk file each (INPUTN2, STERON, INTVAL,
- * SETCMP, INTLHT: inputn2.f, steron.f, intval.f, setcmp.f, intlht.f).  The
- * storm schedule of every run is computed on the host here. */
+/* Open nstorm runs (1 .. SLG_MAX_STORMS) on one basin, one trk file each
+ * (INPUTN2, STERON, INTVAL, SETCMP, INTLHT: inputn2.f, steron.f, intval.f,
+ * setcmp.f, intlht.f).  The storm schedule of every run is computed on the
+ * host here. */
 int slg_batch_open(slg_basin *b, int nstorm, const char *const *trk_paths, slg_batch **out,
                    char *err, int errlen);
 /* As slg_batch_open, choosing the arithmetic of the time step: 32 (FP32) or 64
@@ -206,7 +208,7 @@ int slg_write_env(const char *path, int imxb, int jmxb, const char *comment, con
  * waiting runs and launches up to k further steps on every pool. */
 typedef struct slg_server slg_server;
 int slg_server_open(int precision, double hbm_fraction, slg_server **out, char *err, int errlen);
-/* Register a basin with up to max_slots slots of max_steps steps and an arena
+/* Register a basin with up to max_slots (<= SLG_MAX_STORMS) slots of max_steps steps and an arena
  * of tables_per_slot radial-table sets per slot; *basin_id identifies it. */
 int slg_server_add_basin(slg_server *sv, const char *dta_path, const char *bnt_line, int max_slots,
                          int max_steps, int tables_per_slot, int *basin_id, char *err, int errlen);

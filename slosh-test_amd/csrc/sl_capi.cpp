@@ -549,8 +549,8 @@ extern "C" int slg_batch_open_prec(slg_basin *B, int nstorm, const char *const *
                                    slg_batch **out, char *err, int errlen)
 {
    *out = nullptr;
-   if (nstorm < 1) {
-      seterr(err, errlen, "%s", "nstorm must be >= 1");
+   if (nstorm < 1 || nstorm > SLG_MAX_STORMS) {
+      seterr(err, errlen, "nstorm must be 1 .. %s%d (storms index a launch grid dimension)", "", SLG_MAX_STORMS);
       return SLG_EINPUT;
    }
    if (precision != 32 && precision != 64) {
@@ -1405,10 +1405,16 @@ static int batch_rex_points(slg_batch *X, int src, int s0, int ns, int npts, con
    }
    int *dk = (int *) X->pt_buf;
    int16_t *dout = (int16_t *) (dk + npts);
-   if (hipMemcpyAsync(dk, k, sizeof(int) * (size_t) npts, hipMemcpyHostToDevice, X->st) != hipSuccess ||
-       dev_rex_points(&img(B, R()).dev, &D, hbs, src == 2 ? X->hbmax : nullptr, img(B, R()).zbd, s0, ns, dk, npts,
-                      dout, X->st) ||
-       hipMemcpyAsync(out, dout, sizeof(int16_t) * (size_t) npts * ns, hipMemcpyDeviceToHost, X->st) != hipSuccess ||
+   if (hipMemcpyAsync(dk, k, sizeof(int) * (size_t) npts, hipMemcpyHostToDevice, X->st) != hipSuccess)
+      return SLG_EHIP;
+   // storms go on the grid's y dimension: at most 65 535 a launch
+   for (int c = 0; c < ns; c += 65535) {
+      const int n = std::min(65535, ns - c);
+      if (dev_rex_points(&img(B, R()).dev, &D, hbs, src == 2 ? X->hbmax : nullptr, img(B, R()).zbd, s0 + c, n, dk,
+                         npts, dout + (size_t) c * npts, X->st))
+         return SLG_EHIP;
+   }
+   if (hipMemcpyAsync(out, dout, sizeof(int16_t) * (size_t) npts * ns, hipMemcpyDeviceToHost, X->st) != hipSuccess ||
        hipStreamSynchronize(X->st) != hipSuccess)
       return SLG_EHIP;
    return 0;

@@ -225,8 +225,8 @@ extern "C" int slg_server_open(int precision, double hbm_fraction, slg_server **
 extern "C" int slg_server_add_basin(slg_server *S, const char *dta, const char *bnt_line, int max_slots,
                                     int max_steps, int tables_per_slot, int *basin_id, char *err, int errlen)
 {
-   if (max_slots < 1 || max_steps < 1 || tables_per_slot < 1) {
-      seterr(err, errlen, "%s", "max_slots, max_steps and tables_per_slot must be >= 1");
+   if (max_slots < 1 || max_slots > SLG_MAX_STORMS || max_steps < 1 || tables_per_slot < 1) {
+      seterr(err, errlen, "%s", "max_slots must be 1 .. SLG_MAX_STORMS; max_steps and tables_per_slot >= 1");
       return SLG_EINPUT;
    }
    slg_basin *B = nullptr;
