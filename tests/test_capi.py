@@ -73,3 +73,15 @@ def test_sides_basin_steps_on_staged_path(tmp_path):
     X.set_path(sloshgpu.PATH_STAGED, 16)
     X.close()
     B.close()
+
+
+def test_batch_size_limit(tmp_path):
+    """A batch holds 1 .. SLG_MAX_STORMS (65 535) runs: storms index a launch
+    grid dimension.  More is refused at open with a message, before any track
+    is read."""
+    b, t = helpers.synth.tiny(seed=4)
+    case = helpers.make_case(str(tmp_path), b, t)
+    B = sloshgpu.Basin(case["dta"], case["bntline"])
+    with pytest.raises(sloshgpu.SloshError, match="65535"):
+        sloshgpu.Batch(B, [case["trk"]] * 65536)
+    B.close()
