@@ -113,6 +113,16 @@ int slg_batch_autotune(slg_batch *x, int steps_per_trial, int *rows_out, int *pa
  * (the host waits once every 64 steps): ms[k] += kernel k time, launches[k] += 1. */
 int slg_batch_step_timed(slg_batch *x, int n, double *ms, int *launches, char *err, int errlen);
 int slg_batch_current_step(const slg_batch *x);
+/* For checking the CONTTY drying-cascade replay (contty.f:146-213, the serial
+ * back-step sweep) on ordinary runs: evcap (1 .. 4096) drying squares are
+ * listed per storm before the replay falls back to visiting every square (the
+ * same results), and budget (> 0; 0 = the default bound of 64 visits per
+ * square) visits before a run is stopped and flagged.  A flagged run's fields
+ * are refused with SLG_ELIMIT by slg_batch_get, slg_batch_envelopes,
+ * slg_batch_rex_frames, slg_batch_rex_points and slg_batch_maxstat_env (the
+ * reference STOPs on its hard limits, e.g. smpt2g.f:34-38).  Call after
+ * slg_batch_upload. */
+int slg_batch_set_replay_limits(slg_batch *x, int evcap, long budget);
 /* Copy one run's field to the host as double [jmxb][imxb]: "HB", "UB", "VB" or
  * "HMX" (COMMON /DUMB7/ and /DUMB10/, parm.for:39-45); "HB0" the initial water
  * (INTLHT, intlht.f:53-185); "HBS" HB of the last step after HMXSV and before
@@ -214,10 +224,20 @@ int slg_server_add_basin(slg_server *sv, const char *dta_path, const char *bnt_l
                          int max_steps, int tables_per_slot, int *basin_id, char *err, int errlen);
 int slg_server_slots(const slg_server *sv, int basin_id);
 int slg_server_submit(slg_server *sv, int basin_id, const char *trk_path, int *run_id, char *err, int errlen);
-/* Returns the number of runs retired this round (>= 0) or an error code. */
+/* Returns the number of runs retired this round (>= 0) or an error code.  The
+ * host waits for no stream: admissions and envelopes pass through pinned
+ * staging behind events, and a pool is held to two pumps of queued steps. */
 int slg_server_pump(slg_server *sv, int k, char *err, int errlen);
-/* 0 queued, 1 running, 2 done (then env [jmxb][imxb] and latency are filled). */
+/* 0 queued, 1 running, 2 done (then env [jmxb][imxb] and latency are filled),
+ * 3 failed (its track could not be read, it does not fit the pool's slots, or
+ * its drying-cascade replay ran out of visits); slg_server_run_error gives the
+ * message. */
 int slg_server_result(const slg_server *sv, int run_id, short *env, double *latency_s);
+int slg_server_run_error(const slg_server *sv, int run_id, char *err, int errlen);
+/* Tile rows (8, 16 or 32) each basin's pool steps with, out[n]: every pool
+ * times the three tiles on its first launch rounds (real steps; identical
+ * fields) and keeps the fastest per slot-step.  Returns the number of pools. */
+int slg_server_tiles(const slg_server *sv, int *out, int n);
 /* [0] queued [1] running [2] done [3] slots [4] pool bytes [5] storm.cell.steps */
 int slg_server_stats(const slg_server *sv, long long *out6);
 void slg_server_close(slg_server *sv);

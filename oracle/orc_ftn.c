@@ -134,6 +134,14 @@ int inital_(int *mhalt, int *imxb, int *jmxb, double zb[BAS_Y][BAS_X], int *hr, 
    if ((wave && *wave) || (full_depth && *full_depth)) fatal("unsupported option", "wave/fullDepth");
    fstr(dpath, dta, *len_dta);
    fstr(tpath, trk, *len_trk);
+   /* a list run (cstart.c:92-245) calls INITAL .. CLNUP once per storm in one
+    * process: nothing of the previous run may carry over */
+   free(G.rows);
+   free(G.tab_step);
+   free(G.tabs);
+   free(G.buf);
+   free(G.hbs);
+   memset(&G, 0, sizeof G);
    if (slg_basin_open(dpath, NULL, &G.basin, err, sizeof err)) fatal("basin", err);
    slg_basin_dims(G.basin, &ni, &nj, NULL);
    G.imxb = ni;

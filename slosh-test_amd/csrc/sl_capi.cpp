@@ -698,6 +698,8 @@ static int batch_upload(slg_batch *X, char *err, int errlen)
    D.f1dact = (uint8_t *) (d + off[18]);
    D.hsq = (R *) (d + off[19]);
    D.chq = (R *) (d + off[20]);       // Q = 0 (channl.f:73)
+   D.evcap = SL_EVCAP;
+   D.vbudget = 0;
    HIPOK(hipMemset(d, 0, tot));
    // initial water (INTLHT) per storm: HB, HMX=HB, UB=VB=0
    std::vector<R> f(fstride, R(0));
@@ -894,6 +896,41 @@ extern "C" int slg_batch_set_graph(slg_batch *X, int steps_per_graph)
    return 0;
 }
 
+// Lower the cascade replay's limits, for checking its two fallbacks on
+// ordinary runs: evcap drying squares listed per storm (1 .. SL_EVCAP) before
+// the walk visits every square, and a visit budget (0: the default bound).
+extern "C" int slg_batch_set_replay_limits(slg_batch *X, int evcap, long budget)
+{
+   if (evcap < 1 || evcap > SL_EVCAP || budget < 0) return SLG_EINPUT;
+   if (!X->dmem) return SLG_EINPUT;   // after slg_batch_upload
+   X->d32.evcap = X->d64.evcap = evcap;
+   X->d32.vbudget = X->d64.vbudget = budget;
+   drop_graph(X);                     // captured launches hold the old values
+   return 0;
+}
+
+// The cascade replay stops a run that exhausts its visit budget and flags it
+// (stat[7]); such a run's fields are incomplete and are not handed out.
+template <class R>
+static int check_runs(slg_batch *X, int s0, int ns, char *err, int errlen)
+{
+   std::vector<int> st((size_t) ns * 8);
+   HIPOK(hipStreamSynchronize(X->st));
+   HIPOK(hipMemcpy(st.data(), run(X, R()).stat + (size_t) s0 * 8, st.size() * 4, hipMemcpyDeviceToHost));
+   for (int s = 0; s < ns; s++)
+      if (st[(size_t) s * 8 + 7]) {
+         seterr(err, errlen, "run %s%d: the drying-cascade replay ran out of visits; its fields are incomplete",
+                "", s0 + s);
+         return SLG_ELIMIT;
+      }
+   return 0;
+}
+
+static int check_runs_any(slg_batch *X, int s0, int ns, char *err, int errlen)
+{
+   return X->prec == 64 ? check_runs<double>(X, s0, ns, err, errlen) : check_runs<float>(X, s0, ns, err, errlen);
+}
+
 // after a lockstep step: where each storm still running left its HB
 static void note_step(slg_batch *X, int step, int rc)
 {
@@ -908,6 +945,7 @@ extern "C" int slg_batch_step(slg_batch *X, int n, char *err, int errlen)
    const int lim = X->maxsteps;   // storms with shorter runs stop at their own end
    if (X->graph_steps > 0 && X->path == SLG_PATH_FUSED && X->nsplit == 1) {
       // whole graphs while they fit, then single steps with the same kernels
+      const int step0 = X->step;
       if (!X->gexec) {
          int rc = build_graph(X, err, errlen);
          if (rc) return rc;
@@ -925,8 +963,12 @@ extern "C" int slg_batch_step(slg_batch *X, int n, char *err, int errlen)
          }
          X->step++;
       }
+      // a fused step leaves HB in the buffer it started from (X->parity, which
+      // an earlier staged step, e.g. an autotuner trial, may have set to 1):
+      // every storm that stepped in this call ends there
       X->hbs_buf = -1;
-      note_step(X, X->step, 0);
+      for (int s = 0; s < X->nstorm; s++)
+         if (X->sc[s].nsteps > step0) X->last_par[s] = X->parity;
       return 0;
    }
    for (int k = 0; k < n && X->step < lim; k++) {
@@ -1199,6 +1241,12 @@ extern "C" int slg_batch_get(slg_batch *X, int storm, const char *name, double *
       memcpy(dst, &X->hb0[(size_t) storm * np], np * sizeof(double));
       return 0;
    }
+   if (!X->dmem) {
+      seterr(err, errlen, "%s", "batch not uploaded");
+      return SLG_EINPUT;
+   }
+   const int rc = check_runs_any(X, storm, 1, err, errlen);
+   if (rc) return rc;
    return X->prec == 64 ? batch_get<double>(X, storm, name, dst, err, errlen)
                         : batch_get<float>(X, storm, name, dst, err, errlen);
 }
@@ -1247,6 +1295,8 @@ extern "C" int slg_batch_envelopes(slg_batch *X, short *env, short *mom, char *e
       seterr(err, errlen, "%s", "batch not uploaded");
       return SLG_EINPUT;
    }
+   const int rc = check_runs_any(X, 0, X->nstorm, err, errlen);
+   if (rc) return rc;
    return X->prec == 64 ? batch_envelopes<double>(X, env, mom, err, errlen)
                         : batch_envelopes<float>(X, env, mom, err, errlen);
 }
@@ -1371,7 +1421,8 @@ static int rex_source(slg_batch *X, const char *field, int s0, int ns, char *err
          return SLG_EINPUT;
       }
    }
-   return src;
+   const int rc = ns ? check_runs_any(X, s0, ns, err, errlen) : 0;
+   return rc ? rc : src;
 }
 
 extern "C" int slg_batch_rex_frames(slg_batch *X, const char *field, int s0, int ns, unsigned char *buf, long cap,
@@ -1506,6 +1557,8 @@ extern "C" int slg_batch_maxstat_env(slg_batch *X, short *env, char *err, int er
       seterr(err, errlen, "%s", "windowed maxima are off (slg_batch_set_maxstat) or no buffer");
       return SLG_EINPUT;
    }
+   const int rc = check_runs_any(X, 0, X->nstorm, err, errlen);
+   if (rc) return rc;
    return X->prec == 64 ? maxstat_env<double>(X, env, err, errlen) : maxstat_env<float>(X, env, err, errlen);
 }
 

@@ -97,6 +97,9 @@ struct SlRunDevT {
    uint8_t *f1dact;           /* [nstorm][nsqrwc]: 1 where FLW1DM_C left water on both sides */
    R *hsq;                    /* [nstorm][nsq]: HSUB of the 1-D flow squares (FLW1DM_C -> _M) */
    R *chq;                    /* [nstorm][5]: each channel's flow, kept from step to step */
+   int evcap;                 /* drying squares listed per storm before the replay sweeps
+                                 every square (1 .. SL_EVCAP; SL_EVCAP in production) */
+   long vbudget;              /* visits one replay may make (0: 64 * squares + 4096) */
 };
 
 #endif

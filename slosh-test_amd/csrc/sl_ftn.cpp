@@ -15,7 +15,7 @@
 //   DUMB7.HB   on the first step (mask init)     -> initial water, by inital_
 //   hb (HHB)   after TMSTPC2 when FPASS          -> min(HB, HMX) after HMXSV
 //   hb         after CLNUP                       -> HMX (the envelope)
-//   DUMB7.HB   after every step with -f_stat     -> HB, by tmstpc2_ (see below)
+//   DUMB7.HB   after every step (-f_stat reads it) -> HB, by tmstpc2_ (see below)
 //   EGTH.DELT  on tide-only steps                -> kept current by tmstpm_
 //   LLXFLE     written by memSetLLx before INITAL -> read by inital_ (Coriolis)
 // Paths this build does not model (tides, hot start, waves, full depth) stop
@@ -100,29 +100,17 @@ int initalcommon_(void)
    return 0;   // the mirrors start zeroed
 }
 
-// With -f_stat N (N != 0; usrparse.c:357-363) the C driver's windowed maxima
-// (MaxStatUpdateGrid, slosh2.c:1264-1316) read COMMON /DUMB7/ HB after every
-// step.  The driver passes the model no sign of it, so the library looks for
-// the option on the process's command line.
-static bool cmdline_f_stat()
+// COMMON /DUMB7/ HB holds the step's result after every step, as the
+// reference's Fortran leaves it.  The C driver reads it between steps in the
+// -f_stat mode (MaxStatUpdateGrid, slosh2.c:1264-1316, called from
+// RunLoopStep whatever the caller: the CLI, list runs, the Tcl binding), and
+// the model is not told that mode is on, so the mirror is kept after every
+// step.  SLG_HB_MIRROR=0 drops it for drivers known not to read HB between
+// steps (a copy of one field per step saved).
+static bool hb_mirror_on()
 {
-   FILE *fp = fopen("/proc/self/cmdline", "rb");
-   if (!fp) return false;
-   std::vector<std::string> args;
-   std::string a;
-   int c;
-   while ((c = fgetc(fp)) != EOF) {
-      if (c == 0) {
-         args.push_back(a);
-         a.clear();
-      } else {
-         a.push_back((char) c);
-      }
-   }
-   fclose(fp);
-   for (size_t k = 0; k + 1 < args.size(); k++)
-      if (args[k] == "-f_stat" && atoi(args[k + 1].c_str()) != 0) return true;
-   return false;
+   const char *e = getenv("SLG_HB_MIRROR");
+   return !(e && e[0] == '0' && e[1] == 0);
 }
 
 int inital_(int *mhalt, int *imxb, int *jmxb, double zb[BAS_Y][BAS_X], int *hr, int *mn, int *dy,
@@ -160,7 +148,7 @@ int inital_(int *mhalt, int *imxb, int *jmxb, double zb[BAS_Y][BAS_X], int *hr, 
    if (slg_batch_upload(g_run.batch, err, sizeof err)) fatal("device", err);
    g_run.nsteps = slg_batch_nsteps(g_run.batch, 0);
    g_run.step = 0;
-   g_run.hb_each_step = cmdline_f_stat();
+   g_run.hb_each_step = hb_mirror_on();
    // outputs (inital.f:204-272)
    *mhalt = g_run.nsteps;
    *imxb = ni;

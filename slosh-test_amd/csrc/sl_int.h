@@ -194,6 +194,21 @@ static inline int dev_rex(const SlBasinDevT<double> *B, const SlRunDevT<double> 
    return sl_dev_rex_f64(B, R, hbs, hmax, zbd, s0, ns, out, cap, nbytes, scr, st);
 }
 
+extern "C" int sl_dev_slot_init_f32(const SlBasinDevT<float> *B, const SlRunDevT<float> *R, int slot, int tstart,
+                                    int nsteps, hipStream_t st);
+extern "C" int sl_dev_slot_init_f64(const SlBasinDevT<double> *B, const SlRunDevT<double> *R, int slot, int tstart,
+                                    int nsteps, hipStream_t st);
+static inline int dev_slot_init(const SlBasinDevT<float> *B, const SlRunDevT<float> *R, int slot, int tstart,
+                                int nsteps, hipStream_t st)
+{
+   return sl_dev_slot_init_f32(B, R, slot, tstart, nsteps, st);
+}
+static inline int dev_slot_init(const SlBasinDevT<double> *B, const SlRunDevT<double> *R, int slot, int tstart,
+                                int nsteps, hipStream_t st)
+{
+   return sl_dev_slot_init_f64(B, R, slot, tstart, nsteps, st);
+}
+
 // storm schedule of one run (host, double): rows kept when keep_rows
 int sl_make_schedule(const slg_basin *B, const char *trk, storm_sched &S, bool keep_rows, int prec, char *err,
                      int errlen);

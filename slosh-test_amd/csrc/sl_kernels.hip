@@ -228,7 +228,7 @@ __device__ __forceinline__ real contty_sq(const SlBasinDev &B, const SlRunDev &R
             else {
                out = -zb;   // final value comes from the replay
                int k = atomicAdd(&R.evcnt[s], 1);
-               if (k < SL_EVCAP) R.evkey[(size_t) s * SL_EVCAP + k] = ((uint32_t) j << 16) | (uint32_t) i;
+               if (k < R.evcap) R.evkey[(size_t) s * SL_EVCAP + k] = ((uint32_t) j << 16) | (uint32_t) i;
             }
          } else {
             out = hbj;
@@ -340,7 +340,7 @@ __global__ void __launch_bounds__(TX * TYA) k_conttyA(SlBasinDev B, SlRunDev R, 
                else {
                   out = -zb;   // final value comes from the replay
                   int q = atomicAdd(&R.evcnt[s], 1);
-                  if (q < SL_EVCAP) R.evkey[(size_t) s * SL_EVCAP + q] = ((uint32_t) j << 16) | (uint32_t) i;
+                  if (q < R.evcap) R.evkey[(size_t) s * SL_EVCAP + q] = ((uint32_t) j << 16) | (uint32_t) i;
                }
             } else {
                out = hbj;
@@ -373,7 +373,7 @@ struct Replay {
    real *hn, *ub, *vb;
    uint8_t *uz;
    uint32_t *key;             // sorted candidate keys (j << 16 | i), LDS
-   int n, full;
+   int n, full, cap;          // keys held, full-sweep mode, key capacity
    uint32_t frontier;         // largest key the sweep has reached
    real x1b5;
 };
@@ -454,7 +454,7 @@ __device__ static bool dirty9(Replay &X, int i, int j, int lane)
    const uint64_t nm = __ballot(isnew);
    const int m = __popcll(nm);
    if (!m) return below;
-   if (X.n + m > SL_EVCAP) {
+   if (X.n + m > X.cap) {
       go_full(X, lane);
       return below;
    }
@@ -559,7 +559,7 @@ __global__ __launch_bounds__(64) void k_replay(SlBasinDev B, SlRunDev R, StepArg
    const int step = storm_step(R, s);
    int n = R.evcnt[s];
    if (n == 0 || !step) return;
-   if (n <= SL_EVCAP) {
+   if (n <= R.evcap) {
       const uint32_t *gk = R.evkey + (size_t) s * SL_EVCAP;
       int m = 1;
       while (m < n) m <<= 1;
@@ -590,8 +590,9 @@ __global__ __launch_bounds__(64) void k_replay(SlBasinDev B, SlRunDev R, StepArg
    X.uz = use_uz ? R.uz + so : nullptr;
    X.key = sk;
    X.full = 0;
+   X.cap = R.evcap;
    X.x1b5 = par(R, s, step)[P_X1B5];
-   if (n > SL_EVCAP) {
+   if (n > R.evcap) {
       n = 0;
       X.full = 1;
    }
@@ -609,8 +610,9 @@ __global__ __launch_bounds__(64) void k_replay(SlBasinDev B, SlRunDev R, StepArg
    ib = B.is[j - 1];
    ifn = B.ie[j - 1];
    // The reference sweep ends after a bounded number of visits; the budget
-   // only guards the device against a malformed bound table.
-   long budget = 64L * B.imxb * B.jmxb + 4096;
+   // only guards the device against a malformed bound table.  A run that
+   // exhausts it is flagged (stat[7]) and its results are refused by the host.
+   long budget = R.vbudget > 0 ? R.vbudget : 64L * B.imxb * B.jmxb + 4096;
 L130:
    if (--budget < 0) {
       if (lane == 0) atomicOr(&R.stat[s * 8 + 7], 1);
@@ -1930,6 +1932,33 @@ __global__ void k_envelopes(SlBasinDev B, SlRunDev R, const double *hx2, const d
    if (mom) mom[cell] = (short) (any ? m : 999);
 }
 
+// A serving slot takes a new run whose initial water (INTLHT) was copied into
+// hb0: HB in both buffers, HMX = HB on squares and 0 on the last row and column
+// (intlht.f:172-177), UB = VB = 0, the slot's counters cleared, and its step
+// count and first radial-table set set.  Queued on the pool's stream after the
+// steps of the slot's previous run, so no launch sees a half-made slot.
+__global__ void k_slot_init(SlBasinDev B, SlRunDev R, int slot, int tstart, int nsteps)
+{
+   const int p = blockIdx.x * blockDim.x + threadIdx.x;
+   const size_t so = (size_t) slot * R.fstride;
+   if (p < B.npts) {
+      const int i = p % B.pitch + 1, j = p / B.pitch + 1;
+      const real h = R.hb0[so + p];
+      R.hb1[so + p] = h;
+      R.hmx[so + p] = (i < B.imxb && j < B.jmxb) ? h : real(0.);
+      R.ub[so + p] = real(0.);
+      R.vb[so + p] = real(0.);
+   }
+   if (p < B.pitch) R.hsubtop[(size_t) slot * B.pitch + p] = real(0.);
+   if (p < 8) R.stat[slot * 8 + p] = 0;
+   if (p == 0) {
+      R.evcnt[slot] = 0;
+      R.cur[slot] = 0;
+      ((int *) R.nsteps)[slot] = nsteps;
+      ((int *) R.tbase)[slot] = tstart;
+   }
+}
+
 // End of a step: every storm still running has completed one more step.
 __global__ void k_advance(SlRunDev R)
 {
@@ -2115,6 +2144,19 @@ extern "C" int SL_ENVS(const SlBasinDev *B, const SlRunDev *R, const double *hx2
 {
    const int n = B->imxb * B->jmxb;
    hipLaunchKernelGGL(k_envelopes, dim3((n + 255) / 256), dim3(256), 0, st, *B, *R, hx2, zbd, s0, ns, env, mom);
+   hipError_t e = hipGetLastError();
+   return e == hipSuccess ? 0 : -(int) e;
+}
+
+#ifdef SL_F64
+#define SL_SLOTINIT sl_dev_slot_init_f64
+#else
+#define SL_SLOTINIT sl_dev_slot_init_f32
+#endif
+extern "C" int SL_SLOTINIT(const SlBasinDev *B, const SlRunDev *R, int slot, int tstart, int nsteps, hipStream_t st)
+{
+   const int n = B->npts > B->pitch ? B->npts : B->pitch;
+   hipLaunchKernelGGL(k_slot_init, dim3((n + 255) / 256), dim3(256), 0, st, *B, *R, slot, tstart, nsteps);
    hipError_t e = hipGetLastError();
    return e == hipSuccess ? 0 : -(int) e;
 }

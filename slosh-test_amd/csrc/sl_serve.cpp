@@ -2,19 +2,30 @@
 //
 // Each registered basin gets a pool: device memory for a fixed number of run
 // slots (fields, schedule, radial-table arena), sized from the free HBM at
-// registration.  Runs are submitted as they arrive (their storm schedule and
-// initial water are computed on the host at submission), wait in a FIFO, and
+// registration, and its own HIP stream.  Runs are submitted as they arrive;
+// their storm schedule (the host-side TTIMER/STMVAL/FORCE1 sequence) and
+// initial water are prepared on worker threads, then they wait in a FIFO and
 // are admitted into a free slot of their basin's pool while the other slots
-// keep stepping.  Every pump advances all pools by up to k steps, each pool on
-// its own HIP stream so the pools' kernels overlap; the kernels read each
-// slot's own step counter (runs in one pool are at different steps) and skip
-// empty or finished slots.  A finished run's envelope (EnvSave quantisation)
-// is computed on the device and copied out; its slot and tables are freed.
-// A run's storm schedule (the host-side TTIMER/STMVAL/FORCE1 sequence) and
-// initial water are prepared on worker threads between submission and
-// admission, so arrivals do not stall the stepping.
-// A run's fields depend only on its own track and basin, so its result is the
-// same alone or in any mix.
+// keep stepping.  The kernels read each slot's own step counter (runs in one
+// pool are at different steps) and skip empty or finished slots.  A run's
+// fields depend only on its own track and basin, so its result is the same
+// alone or in any mix.
+//
+// The host never waits for a whole stream.  Everything the device needs from
+// the host goes through a pinned staging ring: an admission copies the run's
+// initial water, schedule and tables into the ring, queues the copies and a
+// slot-initialisation kernel on the pool's stream, and records an event that
+// releases the ring space once the copies are done.  A run whose last step
+// has been queued is retired in the same way: its envelope (EnvSave
+// quantisation) is computed on the device and copied into the ring behind an
+// event; its slot and tables are free at once (anything later on the stream
+// runs after the envelope kernel), and the run is reported done when a later
+// pump finds the event complete.  Each pool keeps at most two pumps of steps
+// queued, which bounds how far the host runs ahead of the device.
+//
+// Launch shape: each pool tunes its tile on its first launch rounds, timing
+// the 64 x 8, 32 x 16 and 16 x 32 tiles with events on real steps (all three
+// compute identical fields) and keeping the fastest per slot-step.
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -31,6 +42,63 @@ double now_s()
    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+// Pinned host memory handed out in FIFO order; a range is reused once the
+// device work that reads or writes it is complete (its event) and the host is
+// done with it (consumed).
+struct Ring {
+   char *base = nullptr;
+   size_t cap = 0, head = 0;
+   struct Chunk {
+      size_t off, len;
+      hipEvent_t ev;
+      bool consumed;
+   };
+   std::deque<Chunk> busy;
+   std::vector<hipEvent_t> spare;
+
+   hipEvent_t event()
+   {
+      if (!spare.empty()) {
+         hipEvent_t e = spare.back();
+         spare.pop_back();
+         return e;
+      }
+      hipEvent_t e = nullptr;
+      if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return nullptr;
+      return e;
+   }
+   void reclaim()
+   {
+      while (!busy.empty() && busy.front().consumed && hipEventQuery(busy.front().ev) == hipSuccess) {
+         spare.push_back(busy.front().ev);
+         busy.pop_front();
+      }
+   }
+   // a range of len bytes (multiple of 256), or -1 when the ring is too full
+   long take(size_t len)
+   {
+      reclaim();
+      if (busy.empty()) head = 0;
+      const size_t tail = busy.empty() ? 0 : busy.front().off;
+      if (busy.empty() || head > tail) {
+         if (cap - head >= len) {
+            head += len;
+            return (long) (head - len);
+         }
+         if (!busy.empty() && len < tail) {   // wrap to the start
+            head = len;
+            return 0;
+         }
+         return -1;
+      }
+      if (tail - head > len) {   // wrapped: strictly below the oldest range
+         head += len;
+         return (long) (head - len);
+      }
+      return -1;
+   }
+};
+
 struct Pool {
    slg_basin *basin = nullptr;
    int slots = 0, maxsteps = 0, tsets = 0;
@@ -40,20 +108,32 @@ struct Pool {
    SlRunDevT<double> d64;
    hipStream_t st = nullptr;
    std::vector<int> run;     // run id in each slot, -1 free
-   std::vector<int> left;    // steps the run in the slot still has to go
+   std::vector<int> left;    // steps of the slot's run not yet queued
    std::vector<std::pair<int, int>> free_ext;   // free table-set extents (start, count)
-   short *denv = nullptr;    // device envelope of one slot
+   short *denv = nullptr;    // device envelope of one slot (reused in stream order)
    size_t cells = 0;         // computational cells (IMXB1*JMXB1)
+   // launch rounds in flight (back-pressure) and the tile tuner
+   hipEvent_t ev_round[2] = {};
+   int inflight = 0;
+   int tile_rows = 32;
+   int trial = 0;            // tiles tried so far (3: tuned)
+   hipEvent_t ev_t0[3] = {}, ev_t1[3] = {};
+   double trial_units[3] = {};
+   bool trial_pending[3] = {};
 };
 
 struct Run {
    int basin = 0, slot = -1;
    std::atomic<int> state{-1};   // -1 being prepared, 0 queued, 1 running, 2 done, 3 failed
+   // copied from the pool at submission: worker threads never touch the pools
+   slg_basin *bas = nullptr;
+   int maxsteps = 0, tsets = 0;
    std::string trk, err;
    storm_sched sched;
    std::vector<double> hb0;
    std::vector<short> env;
    int tstart = -1, ntab = 0;
+   long ring_off = -1;          // the envelope's range in the ring while it is being copied back
    double t_submit = 0, t_admit = 0, t_done = 0;
 };
 
@@ -83,15 +163,21 @@ void give_tables(Pool &P, int start, int n)
    P.free_ext.swap(m);
 }
 
+size_t up256(size_t n)
+{
+   return (n + 255) & ~(size_t) 255;
+}
+
 }   // namespace
 
 struct slg_server {
    int prec = 32;
    double frac = 0.5;
-   int tile_rows = 16;
-   std::vector<Pool> pools;
+   std::vector<std::unique_ptr<Pool>> pools;
    std::vector<std::unique_ptr<Run>> runs;
-   std::deque<int> queue;   // submitted, not yet admitted (submission order)
+   std::deque<int> queue;      // submitted, not yet admitted (submission order)
+   std::vector<int> finishing; // retired runs whose envelope is still being copied back
+   Ring ring;
    long long cell_steps = 0;
    // schedule preparation workers
    std::mutex mu;
@@ -103,11 +189,10 @@ struct slg_server {
 
 static void prepare(slg_server *S, Run *r)
 {
-   Pool &P = S->pools[r->basin];
    char e[512];
    e[0] = 0;
-   int rc = sl_make_schedule(P.basin, r->trk.c_str(), r->sched, false, S->prec, e, sizeof e);
-   if (!rc && (r->sched.nsteps > P.maxsteps || (int) r->sched.tab_step.size() > P.tsets)) {
+   int rc = sl_make_schedule(r->bas, r->trk.c_str(), r->sched, false, S->prec, e, sizeof e);
+   if (!rc && (r->sched.nsteps > r->maxsteps || (int) r->sched.tab_step.size() > r->tsets)) {
       snprintf(e, sizeof e, "%s", "run longer (or with more table sets) than the pool's slots hold");
       rc = SLG_ELIMIT;
    }
@@ -116,9 +201,9 @@ static void prepare(slg_server *S, Run *r)
       r->state = 3;
       return;
    }
-   const size_t npt = (size_t) P.basin->h.imxb * P.basin->h.jmxb;
+   const size_t npt = (size_t) r->bas->h.imxb * r->bas->h.jmxb;
    r->hb0.resize(npt);
-   sl_initial_water(&P.basin->h, &r->sched.s0, r->hb0.data());
+   sl_initial_water(&r->bas->h, &r->sched.s0, r->hb0.data());
    r->ntab = (int) r->sched.tab_step.size();
    r->state = 0;
 }
@@ -146,7 +231,13 @@ static size_t slot_bytes(const slg_basin *B, int maxsteps, int tsets_per_slot, i
 }
 
 template <class R>
-static int pool_alloc(slg_server *S, Pool &P, char *err, int errlen)
+static SlRunDevT<R> &pool_run(Pool &P)
+{
+   return *(SlRunDevT<R> *) (sizeof(R) == 8 ? (void *) &P.d64 : (void *) &P.d32);
+}
+
+template <class R>
+static int pool_alloc(Pool &P, char *err, int errlen)
 {
    const slg_basin *B = P.basin;
    const int n = P.slots;
@@ -155,7 +246,7 @@ static int pool_alloc(slg_server *S, Pool &P, char *err, int errlen)
    int k = 0;
    auto add = [&](size_t bytes) {
       off[k++] = tot;
-      tot += (bytes + 255) & ~(size_t) 255;
+      tot += up256(bytes);
    };
    add(ns * rs); add(ns * rs); add(ns * rs); add(ns * rs); add(ns * rs);   // hb0 hb1 ub vb hmx
    add((size_t) n * B->pitch * rs);                                       // hsubtop
@@ -174,7 +265,7 @@ static int pool_alloc(slg_server *S, Pool &P, char *err, int errlen)
    HIPOK(hipMemset(d, 0, tot));   // nsteps = 0: every slot empty
    P.dmem = d;
    P.bytes = tot;
-   SlRunDevT<R> &D = *(SlRunDevT<R> *) (sizeof(R) == 8 ? (void *) &P.d64 : (void *) &P.d32);
+   SlRunDevT<R> &D = pool_run<R>(P);
    memset(&D, 0, sizeof D);
    D.nstorm = n;
    D.fstride = np;
@@ -196,9 +287,15 @@ static int pool_alloc(slg_server *S, Pool &P, char *err, int errlen)
    D.tbase = (const int *) (d + off[14]);
    D.nsteps = (const int *) (d + off[15]);
    D.cur = (int *) (d + off[16]);
+   D.evcap = SL_EVCAP;
+   D.vbudget = 0;
    P.denv = (short *) (d + off[17]);
    HIPOK(hipStreamCreate(&P.st));
-   (void) S;
+   for (auto &e : P.ev_round) HIPOK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+   for (int c = 0; c < 3; c++) {
+      HIPOK(hipEventCreate(&P.ev_t0[c]));
+      HIPOK(hipEventCreate(&P.ev_t1[c]));
+   }
    return 0;
 }
 
@@ -209,9 +306,16 @@ extern "C" int slg_server_open(int precision, double hbm_fraction, slg_server **
       seterr(err, errlen, "%s", "precision must be 32 or 64, hbm_fraction in (0, 1]");
       return SLG_EINPUT;
    }
+   // pinned staging: admissions (initial water, schedule, tables) and
+   // envelopes on their way back
+   const size_t ring_bytes = (size_t) 512 << 20;
+   char *base = nullptr;
+   HIPOK(hipHostMalloc((void **) &base, ring_bytes, hipHostMallocDefault));
    slg_server *S = new slg_server();
    S->prec = precision;
    S->frac = hbm_fraction;
+   S->ring.base = base;
+   S->ring.cap = ring_bytes;
    const int nw = std::max(1, std::min<int>(8, (int) std::thread::hardware_concurrency()));
    for (int k = 0; k < nw; k++) S->workers.emplace_back(worker, S);
    *out = S;
@@ -252,28 +356,28 @@ extern "C" int slg_server_add_basin(slg_server *S, const char *dta, const char *
       seterr(err, errlen, "%s", "not enough free HBM for one run slot");
       return SLG_ELIMIT;
    }
-   Pool P;
-   P.basin = B;
-   P.slots = slots;
-   P.maxsteps = max_steps;
-   P.tsets = slots * tables_per_slot;
-   P.cells = (size_t) B->h.imxb1 * B->h.jmxb1;
-   P.run.assign(slots, -1);
-   P.left.assign(slots, 0);
-   P.free_ext.push_back({0, P.tsets});
-   rc = S->prec == 64 ? pool_alloc<double>(S, P, err, errlen) : pool_alloc<float>(S, P, err, errlen);
+   std::unique_ptr<Pool> P(new Pool());
+   P->basin = B;
+   P->slots = slots;
+   P->maxsteps = max_steps;
+   P->tsets = slots * tables_per_slot;
+   P->cells = (size_t) B->h.imxb1 * B->h.jmxb1;
+   P->run.assign(slots, -1);
+   P->left.assign(slots, 0);
+   P->free_ext.push_back({0, P->tsets});
+   rc = S->prec == 64 ? pool_alloc<double>(*P, err, errlen) : pool_alloc<float>(*P, err, errlen);
    if (rc) {
       slg_basin_close(B);
       return rc;
    }
-   S->pools.push_back(P);
+   S->pools.push_back(std::move(P));
    *basin_id = (int) S->pools.size() - 1;
    return 0;
 }
 
 extern "C" int slg_server_slots(const slg_server *S, int basin_id)
 {
-   return (basin_id >= 0 && basin_id < (int) S->pools.size()) ? S->pools[basin_id].slots : -1;
+   return (basin_id >= 0 && basin_id < (int) S->pools.size()) ? S->pools[basin_id]->slots : -1;
 }
 
 // Submit a run (trk file) on a registered basin; its schedule is prepared on a
@@ -285,8 +389,12 @@ extern "C" int slg_server_submit(slg_server *S, int basin_id, const char *trk, i
       seterr(err, errlen, "%s", "unknown basin");
       return SLG_EINPUT;
    }
+   const Pool &P = *S->pools[basin_id];
    std::unique_ptr<Run> r(new Run());
    r->basin = basin_id;
+   r->bas = P.basin;
+   r->maxsteps = P.maxsteps;
+   r->tsets = P.tsets;
    r->trk = trk;
    r->t_submit = now_s();
    Run *rp = r.get();
@@ -301,93 +409,195 @@ extern "C" int slg_server_submit(slg_server *S, int basin_id, const char *trk, i
    return 0;
 }
 
+// Queue a run's admission into a slot: its initial water, schedule and tables
+// through the ring, then the slot-initialisation kernel.  Returns 1 when the
+// ring has no room (the run waits for a later pump).
 template <class R>
-static int admit(slg_server *S, Pool &P, int slot, int id, char *err, int errlen)
+static int admit(slg_server *S, Pool &P, int slot, Run &r, char *err, int errlen)
 {
-   Run &r = *S->runs[id];
    const slg_basin *B = P.basin;
    const sl_basin *b = &B->h;
    const int PI = B->pitch, NJ = b->jmxb;
    const size_t np = (size_t) PI * NJ;
-   SlRunDevT<R> &D = *(SlRunDevT<R> *) (sizeof(R) == 8 ? (void *) &P.d64 : (void *) &P.d32);
-   std::vector<R> f(np, R(0)), h(np, R(0));
-   for (int j = 1; j <= NJ; j++)
-      for (int i = 1; i <= b->imxb; i++) {
-         const R v = (R) r.hb0[PIX(b, i, j)];
-         f[(size_t) (j - 1) * PI + (i - 1)] = v;
-         if (i < b->imxb && j < NJ) h[(size_t) (j - 1) * PI + (i - 1)] = v;   // HMX = HB on squares
-      }
-   const size_t so = (size_t) slot * np;
-   HIPOK(hipMemcpyAsync(D.hb0 + so, f.data(), np * sizeof(R), hipMemcpyHostToDevice, P.st));
-   HIPOK(hipMemcpyAsync(D.hb1 + so, f.data(), np * sizeof(R), hipMemcpyHostToDevice, P.st));
-   HIPOK(hipMemcpyAsync(D.hmx + so, h.data(), np * sizeof(R), hipMemcpyHostToDevice, P.st));
-   HIPOK(hipMemsetAsync(D.ub + so, 0, np * sizeof(R), P.st));
-   HIPOK(hipMemsetAsync(D.vb + so, 0, np * sizeof(R), P.st));
-   HIPOK(hipMemsetAsync(D.hsubtop + (size_t) slot * PI, 0, PI * sizeof(R), P.st));
-   HIPOK(hipMemsetAsync(D.evcnt + slot, 0, 4, P.st));
-   HIPOK(hipMemsetAsync(D.stat + slot * 8, 0, 32, P.st));
+   SlRunDevT<R> &D = pool_run<R>(P);
    const std::vector<R> &sp = sched_params(r.sched, R()), &stb = sched_tables(r.sched, R());
-   HIPOK(hipMemcpyAsync((R *) D.params + (size_t) slot * P.maxsteps * SL_NPAR, sp.data(), sp.size() * sizeof(R),
+   const size_t lf = up256(np * sizeof(R)), lp = up256(sp.size() * sizeof(R)), lt = up256(stb.size() * sizeof(R));
+   const long off = S->ring.take(lf + lp + lt);
+   if (off < 0) return 1;
+   char *h = S->ring.base + off;
+   R *f = (R *) h;
+   std::fill(f, f + np, R(0));
+   for (int j = 1; j <= NJ; j++)
+      for (int i = 1; i <= b->imxb; i++) f[(size_t) (j - 1) * PI + (i - 1)] = (R) r.hb0[PIX(b, i, j)];
+   memcpy(h + lf, sp.data(), sp.size() * sizeof(R));
+   memcpy(h + lf + lp, stb.data(), stb.size() * sizeof(R));
+   const size_t so = (size_t) slot * np;
+   HIPOK(hipMemcpyAsync(D.hb0 + so, h, np * sizeof(R), hipMemcpyHostToDevice, P.st));
+   HIPOK(hipMemcpyAsync((R *) D.params + (size_t) slot * P.maxsteps * SL_NPAR, h + lf, sp.size() * sizeof(R),
                         hipMemcpyHostToDevice, P.st));
-   HIPOK(hipMemcpyAsync((R *) D.tables + (size_t) r.tstart * SL_NTABV * 8, stb.data(), stb.size() * sizeof(R),
+   HIPOK(hipMemcpyAsync((R *) D.tables + (size_t) r.tstart * SL_NTABV * 8, h + lf + lp, stb.size() * sizeof(R),
                         hipMemcpyHostToDevice, P.st));
-   const int zero = 0;
-   HIPOK(hipMemcpyAsync((int *) D.tbase + slot, &r.tstart, 4, hipMemcpyHostToDevice, P.st));
-   HIPOK(hipMemcpyAsync(D.cur + slot, &zero, 4, hipMemcpyHostToDevice, P.st));
-   HIPOK(hipMemcpyAsync((int *) D.nsteps + slot, &r.sched.nsteps, 4, hipMemcpyHostToDevice, P.st));
-   // the host buffers above are locals: finish the copies before they go
-   HIPOK(hipStreamSynchronize(P.st));
+   const int rc = dev_slot_init(&img(P.basin, R()).dev, &D, slot, r.tstart, r.sched.nsteps, P.st);
+   if (rc) {
+      seterr(err, errlen, "slot initialisation failed (%s%d)", "", -rc);
+      return SLG_EHIP;
+   }
+   hipEvent_t ev = S->ring.event();
+   if (!ev) {
+      seterr(err, errlen, "%s", "event creation failed");
+      return SLG_EHIP;
+   }
+   HIPOK(hipEventRecord(ev, P.st));
+   S->ring.busy.push_back({(size_t) off, lf + lp + lt, ev, true});
    return 0;
 }
 
+// Queue a finished run's envelope and error flag into the ring; the slot and
+// its tables are free from here on.  Returns 1 when the ring has no room.
 template <class R>
 static int retire(slg_server *S, Pool &P, int slot, char *err, int errlen)
 {
    const int id = P.run[slot];
    Run &r = *S->runs[id];
    BasinImg<R> &I = img(P.basin, R());
-   SlRunDevT<R> &D = *(SlRunDevT<R> *) (sizeof(R) == 8 ? (void *) &P.d64 : (void *) &P.d32);
-   const size_t n = (size_t) P.basin->h.imxb * P.basin->h.jmxb;
+   SlRunDevT<R> &D = pool_run<R>(P);
+   const size_t n = (size_t) P.basin->h.imxb * P.basin->h.jmxb, le = up256(n * 2);
+   const long off = S->ring.take(le + 256);
+   if (off < 0) return 1;
    int rc = dev_envelopes(&I.dev, &D, I.zbd, slot, 1, P.denv, nullptr, P.st);
    if (rc) {
       seterr(err, errlen, "envelope kernel failed (%s%d)", "", -rc);
       return SLG_EHIP;
    }
-   r.env.resize(n);
-   HIPOK(hipMemcpyAsync(r.env.data(), P.denv, n * 2, hipMemcpyDeviceToHost, P.st));
-   HIPOK(hipStreamSynchronize(P.st));
-   r.state = 2;
-   r.t_done = now_s();
+   char *h = S->ring.base + off;
+   HIPOK(hipMemcpyAsync(h, P.denv, n * 2, hipMemcpyDeviceToHost, P.st));
+   HIPOK(hipMemcpyAsync(h + le, D.stat + (size_t) slot * 8, 32, hipMemcpyDeviceToHost, P.st));
+   hipEvent_t ev = S->ring.event();
+   if (!ev) {
+      seterr(err, errlen, "%s", "event creation failed");
+      return SLG_EHIP;
+   }
+   HIPOK(hipEventRecord(ev, P.st));
+   S->ring.busy.push_back({(size_t) off, le + 256, ev, false});
+   r.ring_off = off;
    give_tables(P, r.tstart, r.ntab);
    P.run[slot] = -1;
-   // the schedule is no longer needed
-   r.sched.p32.clear();
-   r.sched.p32.shrink_to_fit();
-   r.sched.p64.clear();
-   r.sched.p64.shrink_to_fit();
-   r.sched.t32.clear();
-   r.sched.t32.shrink_to_fit();
-   r.sched.t64.clear();
-   r.sched.t64.shrink_to_fit();
-   r.hb0.clear();
-   r.hb0.shrink_to_fit();
+   S->finishing.push_back(id);
    return 0;
 }
 
-// One round: finish what the last round launched, retire finished runs, admit
-// queued runs into free slots, and launch up to k steps on every pool that has
-// runs (asynchronous).  Returns the number of runs retired, or an error code.
+// Runs whose envelope has arrived in the ring are done (or failed, when the
+// cascade replay flagged them).
+static void finish_runs(slg_server *S)
+{
+   std::vector<int> still;
+   for (int id : S->finishing) {
+      Run &r = *S->runs[id];
+      Ring::Chunk *c = nullptr;
+      for (auto &k : S->ring.busy)
+         if (!k.consumed && (long) k.off == r.ring_off) c = &k;
+      if (!c || hipEventQuery(c->ev) != hipSuccess) {
+         still.push_back(id);
+         continue;
+      }
+      const slg_basin *B = r.bas;
+      const size_t n = (size_t) B->h.imxb * B->h.jmxb;
+      const char *h = S->ring.base + c->off;
+      const int *st = (const int *) (h + up256(n * 2));
+      r.t_done = now_s();
+      if (st[7]) {
+         r.err = "the drying-cascade replay ran out of visits; the run's fields are incomplete";
+         r.state = 3;
+      } else {
+         r.env.assign((const short *) h, (const short *) h + n);
+         r.state = 2;
+      }
+      c->consumed = true;
+      r.ring_off = -1;
+      // the schedule is no longer needed
+      r.sched.p32 = std::vector<float>();
+      r.sched.p64 = std::vector<double>();
+      r.sched.t32 = std::vector<float>();
+      r.sched.t64 = std::vector<double>();
+      r.hb0 = std::vector<double>();
+   }
+   S->finishing.swap(still);
+   S->ring.reclaim();
+}
+
+// The pool's next launch round: k steps over slots [0, hi).  The first three
+// rounds time the three tiles; from then on the fastest per slot-step runs.
+template <class R>
+static int launch_round(Pool &P, int n, int hi, char *err, int errlen)
+{
+   SlRunDevT<R> d = pool_run<R>(P);
+   d.nstorm = hi;
+   const int tiles[3] = {32, 16, 8};
+   int c = -1;
+   if (P.trial < 3) {
+      c = P.trial++;
+      P.tile_rows = tiles[c];
+      P.trial_units[c] = (double) n * hi;
+      P.trial_pending[c] = true;
+      HIPOK(hipEventRecord(P.ev_t0[c], P.st));
+   }
+   for (int t = 0; t < n; t++) {
+      const int rc = dev_launch(&img(P.basin, R()).dev, &d, 0, 0, 1, 0, SLG_PATH_FUSED, P.tile_rows, P.st, nullptr);
+      if (rc < 0) {
+         seterr(err, errlen, "kernel launch failed (%s%d)", "", -rc);
+         return SLG_EHIP;
+      }
+   }
+   if (c >= 0) HIPOK(hipEventRecord(P.ev_t1[c], P.st));
+   return 0;
+}
+
+// After the trial rounds have run: keep the tile with the least time per
+// slot-step.
+static void settle_tile(Pool &P)
+{
+   if (P.trial < 3) return;
+   const int tiles[3] = {32, 16, 8};
+   double best = 1e300;
+   int bi = -1;
+   for (int c = 0; c < 3; c++) {
+      if (!P.trial_pending[c]) continue;
+      if (hipEventQuery(P.ev_t1[c]) != hipSuccess) return;   // not measured yet
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, P.ev_t0[c], P.ev_t1[c]) != hipSuccess) continue;
+      const double per = ms / std::max(1.0, P.trial_units[c]);
+      if (per < best) {
+         best = per;
+         bi = c;
+      }
+   }
+   if (bi >= 0) P.tile_rows = tiles[bi];
+   for (auto &f : P.trial_pending) f = false;
+}
+
+// One round: collect finished envelopes, retire runs whose last step is
+// queued, admit queued runs into free slots, and queue up to k further steps
+// on every pool that has runs.  Returns the number of runs retired, or an
+// error code.
 extern "C" int slg_server_pump(slg_server *S, int k, char *err, int errlen)
 {
+   if (k < 0) {
+      seterr(err, errlen, "%s", "k must be >= 0");
+      return SLG_EINPUT;
+   }
    int retired = 0;
-   for (auto &P : S->pools) HIPOK(hipStreamSynchronize(P.st));
-   for (auto &P : S->pools)
+   finish_runs(S);
+   for (auto &PP : S->pools) {
+      Pool &P = *PP;
+      settle_tile(P);
       for (int slot = 0; slot < P.slots; slot++)
          if (P.run[slot] >= 0 && P.left[slot] == 0) {
-            int rc = S->prec == 64 ? retire<double>(S, P, slot, err, errlen) : retire<float>(S, P, slot, err, errlen);
-            if (rc) return rc;
-            retired++;
+            const int rc = S->prec == 64 ? retire<double>(S, P, slot, err, errlen)
+                                         : retire<float>(S, P, slot, err, errlen);
+            if (rc < 0) return rc;
+            if (rc == 0) retired++;
          }
+   }
    // admission: FIFO per basin; a run that does not fit waits, later runs of
    // other basins may go ahead of it
    std::deque<int> wait;
@@ -400,7 +610,7 @@ extern "C" int slg_server_pump(slg_server *S, int k, char *err, int errlen)
          wait.push_back(id);
          continue;
       }
-      Pool &P = S->pools[r.basin];
+      Pool &P = *S->pools[r.basin];
       int slot = -1;
       for (int q = 0; q < P.slots && slot < 0; q++)
          if (P.run[q] < 0) slot = q;
@@ -410,8 +620,13 @@ extern "C" int slg_server_pump(slg_server *S, int k, char *err, int errlen)
          continue;
       }
       r.tstart = ts;
-      int rc = S->prec == 64 ? admit<double>(S, P, slot, id, err, errlen) : admit<float>(S, P, slot, id, err, errlen);
-      if (rc) return rc;
+      const int rc = S->prec == 64 ? admit<double>(S, P, slot, r, err, errlen) : admit<float>(S, P, slot, r, err, errlen);
+      if (rc < 0) return rc;
+      if (rc == 1) {   // no staging room: give the tables back and wait
+         give_tables(P, ts, r.ntab);
+         wait.push_back(id);
+         continue;
+      }
       P.run[slot] = id;
       P.left[slot] = r.sched.nsteps;
       r.slot = slot;
@@ -419,7 +634,8 @@ extern "C" int slg_server_pump(slg_server *S, int k, char *err, int errlen)
       r.t_admit = now_s();
    }
    S->queue.swap(wait);
-   for (auto &P : S->pools) {
+   for (auto &PP : S->pools) {
+      Pool &P = *PP;
       // launch over slots [0, hi): admission fills the lowest free slot, so the
       // grid shrinks with the load and empty slots beyond hi cost nothing
       int most = 0, hi = 0;
@@ -429,18 +645,18 @@ extern "C" int slg_server_pump(slg_server *S, int k, char *err, int errlen)
             hi = slot + 1;
          }
       const int n = std::min(k, most);
-      SlRunDevT<float> d32 = P.d32;
-      SlRunDevT<double> d64 = P.d64;
-      d32.nstorm = hi;
-      d64.nstorm = hi;
-      for (int t = 0; t < n; t++) {
-         int rc = S->prec == 64 ? dev_launch(&P.basin->i64.dev, &d64, 0, 0, 1, 0, SLG_PATH_FUSED, S->tile_rows, P.st, nullptr)
-                                : dev_launch(&P.basin->i32.dev, &d32, 0, 0, 1, 0, SLG_PATH_FUSED, S->tile_rows, P.st, nullptr);
-         if (rc < 0) {
-            seterr(err, errlen, "kernel launch failed (%s%d)", "", -rc);
-            return SLG_EHIP;
-         }
+      if (n == 0) continue;
+      // at most two rounds queued per pool
+      if (P.inflight == 2) {
+         HIPOK(hipEventSynchronize(P.ev_round[0]));
+         std::swap(P.ev_round[0], P.ev_round[1]);
+         P.inflight = 1;
       }
+      const int rc = S->prec == 64 ? launch_round<double>(P, n, hi, err, errlen)
+                                   : launch_round<float>(P, n, hi, err, errlen);
+      if (rc) return rc;
+      HIPOK(hipEventRecord(P.ev_round[P.inflight], P.st));
+      P.inflight++;
       for (int slot = 0; slot < P.slots; slot++)
          if (P.run[slot] >= 0) {
             const int d = std::min(n, P.left[slot]);
@@ -451,21 +667,31 @@ extern "C" int slg_server_pump(slg_server *S, int k, char *err, int errlen)
    return retired;
 }
 
-// Run state (0 queued, 1 running, 2 done) and, when done, its envelope
-// [jmxb][imxb] (EnvSave quantisation) and latency from submission (s).
+// Run state (0 queued, 1 running, 2 done, 3 failed) and, when done, its
+// envelope [jmxb][imxb] (EnvSave quantisation) and latency from submission (s).
 extern "C" int slg_server_result(const slg_server *S, int run_id, short *env, double *latency_s)
 {
    if (run_id < 0 || run_id >= (int) S->runs.size()) return SLG_EINPUT;
    const Run &r = *S->runs[run_id];
-   if (r.state == 2) {
+   const int st = r.state;
+   if (st == 2) {
       if (env) memcpy(env, r.env.data(), r.env.size() * 2);
       if (latency_s) *latency_s = r.t_done - r.t_submit;
    }
-   return r.state;
+   return st < 0 ? 0 : st;
+}
+
+// The message of a failed run (state 3).
+extern "C" int slg_server_run_error(const slg_server *S, int run_id, char *err, int errlen)
+{
+   if (run_id < 0 || run_id >= (int) S->runs.size()) return SLG_EINPUT;
+   const Run &r = *S->runs[run_id];
+   seterr(err, errlen, "%s", r.state == 3 ? r.err.c_str() : "");
+   return 0;
 }
 
 // Counters: [0] queued, [1] running, [2] done, [3] slots, [4] device bytes of
-// the pools, [5] storm.cell.steps advanced.
+// the pools, [5] storm.cell.steps queued.
 extern "C" int slg_server_stats(const slg_server *S, long long *out6)
 {
    long long q = (long long) S->queue.size(), run = 0, done = 0, slots = 0, bytes = 0;
@@ -474,8 +700,8 @@ extern "C" int slg_server_stats(const slg_server *S, long long *out6)
       done += r->state == 2;
    }
    for (auto &P : S->pools) {
-      slots += P.slots;
-      bytes += (long long) P.bytes;
+      slots += P->slots;
+      bytes += (long long) P->bytes;
    }
    out6[0] = q;
    out6[1] = run;
@@ -484,6 +710,15 @@ extern "C" int slg_server_stats(const slg_server *S, long long *out6)
    out6[4] = bytes;
    out6[5] = S->cell_steps;
    return 0;
+}
+
+// Tile rows each pool runs (8, 16 or 32; the tuner's choice once it has
+// measured), out[npools]; returns the number of pools.
+extern "C" int slg_server_tiles(const slg_server *S, int *out, int n)
+{
+   const int np = (int) S->pools.size();
+   for (int p = 0; p < np && p < n; p++) out[p] = S->pools[p]->tile_rows;
+   return np;
 }
 
 extern "C" void slg_server_close(slg_server *S)
@@ -495,13 +730,23 @@ extern "C" void slg_server_close(slg_server *S)
    }
    S->cv.notify_all();
    for (auto &w : S->workers) w.join();
-   for (auto &P : S->pools) {
+   for (auto &PP : S->pools) {
+      Pool &P = *PP;
       if (P.st) {
          (void) hipStreamSynchronize(P.st);
          (void) hipStreamDestroy(P.st);
       }
+      for (auto &e : P.ev_round)
+         if (e) (void) hipEventDestroy(e);
+      for (int c = 0; c < 3; c++) {
+         if (P.ev_t0[c]) (void) hipEventDestroy(P.ev_t0[c]);
+         if (P.ev_t1[c]) (void) hipEventDestroy(P.ev_t1[c]);
+      }
       if (P.dmem) (void) hipFree(P.dmem);
       slg_basin_close(P.basin);
    }
+   for (auto &c : S->ring.busy) (void) hipEventDestroy(c.ev);
+   for (auto &e : S->ring.spare) (void) hipEventDestroy(e);
+   if (S->ring.base) (void) hipHostFree(S->ring.base);
    delete S;
 }

@@ -78,6 +78,9 @@ def lib():
         L.slg_server_pump.argtypes = [vp, i, cp, i]
         L.slg_server_result.argtypes = [vp, i, vp, ctypes.POINTER(ctypes.c_double)]
         L.slg_server_stats.argtypes = [vp, vp]
+        L.slg_server_run_error.argtypes = [vp, i, cp, i]
+        L.slg_server_tiles.argtypes = [vp, vp, i]
+        L.slg_batch_set_replay_limits.argtypes = [vp, i, ctypes.c_long]
         L.slg_server_close.argtypes = [vp]
         L.slg_server_close.restype = None
         L.slg_batch_current_step.argtypes = [vp]
@@ -176,6 +179,12 @@ class Batch:
     def upload(self):
         err = ctypes.create_string_buffer(512)
         _chk(lib().slg_batch_upload(self.h, err, 512), err)
+
+    def set_replay_limits(self, evcap: int = 4096, budget: int = 0):
+        """Lower the cascade replay's list capacity (it then sweeps every square)
+        and visit budget (a run that exhausts it is flagged; its results refused)."""
+        if lib().slg_batch_set_replay_limits(self.h, evcap, budget):
+            raise SloshError("bad replay limits %r %r (upload first)" % (evcap, budget))
 
     def step(self, n: int):
         err = ctypes.create_string_buffer(512)
@@ -410,8 +419,16 @@ class Server:
         lat = ctypes.c_double()
         st = lib().slg_server_result(self.h, run_id, env.ctypes.data_as(ctypes.c_void_p), ctypes.byref(lat))
         if st == 3:
-            raise SloshError("run %d failed in preparation" % run_id)
+            err = ctypes.create_string_buffer(512)
+            lib().slg_server_run_error(self.h, run_id, err, 512)
+            raise SloshError("run %d failed: %s" % (run_id, err.value.decode(errors="replace")))
         return (st, env, lat.value) if st == 2 else (st, None, None)
+
+    def tiles(self):
+        """Tile rows each pool steps with (the pool's tuner choice once measured)."""
+        out = np.zeros(64, dtype=np.int32)
+        n = lib().slg_server_tiles(self.h, out.ctypes.data_as(ctypes.c_void_p), 64)
+        return [int(x) for x in out[:n]]
 
     def stats(self):
         out = np.zeros(6, dtype=np.int64)

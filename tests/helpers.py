@@ -187,14 +187,15 @@ def tmpdir(prefix="slg_"):
     return tempfile.mkdtemp(prefix=prefix)
 
 
-def gpu_vs_oracle(case, nsteps, checks, storms=None, path=0, tile_rows=16, precision=32):
+def gpu_vs_oracle(case, nsteps, checks, storms=None, path=0, tile_rows=16, precision=32, setup=None):
     """Run one storm of a case on the GPU and the FP32 oracle side by side.
 
     Returns {step: {field: (gpu, oracle)}} at each checkpoint (plus the last
     step) and the oracle's drying-cascade event count.  ``storms`` lists extra
     track files batched with the case's track (the case's own storm is 0);
     ``path``/``tile_rows`` select the launch path (sloshgpu.PATH_*); ``precision``
-    the arithmetic (32 against the FP32 oracle, 64 against the FP64 one)."""
+    the arithmetic (32 against the FP32 oracle, 64 against the FP64 one);
+    ``setup(batch)`` runs after the upload."""
     root = tmpdir("gvo_")
     rows, tabs, total = product_export(case, root)
     n = min(nsteps, total)
@@ -202,6 +203,8 @@ def gpu_vs_oracle(case, nsteps, checks, storms=None, path=0, tile_rows=16, preci
     X = sloshgpu.Batch(B, [case["trk"]] + list(storms or []), precision=precision)
     X.set_path(path, tile_rows)
     X.upload()
+    if setup:
+        setup(X)
     o = Oracle(str(precision), root + "/init.bin")
     out, events = {}, 0
     want = set(c for c in checks if c <= n) | {n}
@@ -349,3 +352,37 @@ def same_cli_outputs(env_a, rex_a, env_b, rex_b):
             d = next(k for k in range(len(ra)) if ra[k] != rb[k])
             out.append("rex differs from byte %d" % d)
     return out
+
+
+def list_run_case(root):
+    """Two storms on two basins (polar 'tst', elliptic 'etnz') written for a
+    list run: a list file of the reference's version-0 layout (cstart.c:92-245:
+    three header lines, then basin,trk,rex,env per run).  Consecutive runs must
+    name different basins (setFileNames, setup.c, refuses a repeat)."""
+    b1, t1 = synth.tiny(seed=71, ni=28, nj=24, hours=6)
+    b2, t2 = synth.tiny(seed=72, gtype="$", ni=26, nj=22, hours=8)
+    b2.name = "tnz"
+    c1 = synth.write_case(root, b1, t1, trk_name="run1.trk")
+    c2 = synth.write_case(root, b2, t2, trk_name="run2.trk")
+    return [c1, c2]
+
+
+def run_cli_list(exe, root, cases, tag, timeout=900, env_extra=None):
+    """Run a reference-style CLI over a list file (-lst, lstType 0) of the
+    cases; returns [(env, rex)] per run."""
+    lst = os.path.join(root, tag + ".lst")
+    outs = []
+    with open(lst, "w") as fh:
+        fh.write("list run\nbasin,trk,rex,env\n-----\n")
+        for k, c in enumerate(cases):
+            rex = os.path.join(root, "%s_%d.rex" % (tag, k))
+            envf = os.path.join(root, "%s_%d.env" % (tag, k))
+            fh.write("%s,%s,%s,%s\n" % (c["basin"], c["trk"], rex, envf))
+            outs.append((envf, rex))
+    env = dict(os.environ)
+    env.update(env_extra or {})
+    r = subprocess.run([exe, "-lst", lst, "-rootDir", root, "-rex", "yes", "-env", "yes", "-verbose", "0"],
+                       cwd=root, capture_output=True, text=True, timeout=timeout, preexec_fn=_big_stack, env=env)
+    if r.returncode != 0 or not all(os.path.exists(e) for e, _ in outs):
+        raise RuntimeError("%s list run failed (%d): %s %s" % (exe, r.returncode, r.stdout[-800:], r.stderr[-800:]))
+    return outs

@@ -58,3 +58,15 @@ def test_polar_100x90_cli(tmp_path):
     env_r, rex_r = helpers.run_cli(helpers.REF_CLI, root, bsn, "ref")
     env_o, rex_o = helpers.run_cli(helpers.ORC_CLI["64"], root, bsn, "orc")
     assert helpers.same_cli_outputs(env_r, rex_r, env_o, rex_o) == []
+
+
+def test_list_run_matches_reference(tmp_path):
+    """A list run (-lst, DoStormListV0, cstart.c:92-245): two storms on two
+    basins in one process, so INITAL .. CLNUP run twice through the same glue;
+    every run's env and rex equal the reference CLI's."""
+    root = str(tmp_path)
+    cases = helpers.list_run_case(root)
+    ref = helpers.run_cli_list(helpers.REF_CLI, root, cases, "ref")
+    orc = helpers.run_cli_list(helpers.ORC_CLI["64"], root, cases, "orc64")
+    for (ea, ra), (eb, rb) in zip(ref, orc):
+        assert helpers.same_cli_outputs(ea, ra, eb, rb) == []

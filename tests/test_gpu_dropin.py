@@ -114,3 +114,44 @@ def test_gpu_cli_fp64_f_stat_matches_reference_cli(tmp_path):
     if ia >= 0:   # the track records Data_WriteTrk copies are unset for this track layout
         ra, rb = ra[:ia + 4] + ra[ia + 4 + 2400:], rb[:ib + 4] + rb[ib + 4 + 2400:]
     assert ra == rb
+
+
+def test_gpu_list_run_matches_reference(tmp_path):
+    """A list run (-lst, DoStormListV0, cstart.c:92-245; two storms on two
+    basins in one process, so the library is initialised and cleaned up twice):
+    with the FP64 step every run's env and rex equal the reference CLI's; with
+    the FP32 step they equal the same driver over the FP32 oracle."""
+    root = str(tmp_path)
+    cases = helpers.list_run_case(root)
+    print("list run fp64", flush=True)
+    gpu64 = helpers.run_cli_list(helpers.GPU_CLI, root, cases, "gpu64", env_extra={"SLG_PRECISION": "64"})
+    ref = helpers.run_cli_list(helpers.REF_CLI, root, cases, "ref")
+    for (ea, ra), (eb, rb) in zip(gpu64, ref):
+        assert helpers.same_cli_outputs(ea, ra, eb, rb) == []
+    print("list run fp32", flush=True)
+    gpu32 = helpers.run_cli_list(helpers.GPU_CLI, root, cases, "gpu32", env_extra={"SLG_PRECISION": "32"})
+    orc32 = helpers.run_cli_list(helpers.ORC_CLI["32"], root, cases, "orc32")
+    for (ea, ra), (eb, rb) in zip(gpu32, orc32):
+        assert helpers.same_cli_outputs(ea, ra, eb, rb) == []
+
+
+def test_gpu_f_stat_without_hb_mirror_differs(tmp_path):
+    """The HB mirror is what the -f_stat windows read: with SLG_HB_MIRROR=0 the
+    drop-in skips it, and the window maxima then come from a stale COMMON HB
+    (so the default keeps it on for every driver)."""
+    import subprocess
+    b, t = synth.tiny(seed=5, ni=30, nj=26, hours=6)
+    envs = {}
+    for tag, mirror in (("on", "1"), ("off", "0")):
+        root = str(tmp_path / tag)
+        bsn = synth.write_case(root, b, t)["basin"]
+        env = dict(os.environ, SLG_PRECISION="64", SLG_HB_MIRROR=mirror)
+        r = subprocess.run([helpers.GPU_CLI, "-basin", bsn, "-rootDir", root, "-trk", os.path.join(root, "storm.trk"),
+                            "-env", os.path.join(root, "out.env"), "-verbose", "0", "-f_stat", "1"], cwd=root,
+                           capture_output=True, text=True, timeout=600, env=env, preexec_fn=helpers._big_stack)
+        assert r.returncode == 0, (tag, r.stdout[-500:], r.stderr[-500:])
+        envs[tag] = sorted(os.listdir(root))
+        envs[tag + "_vals"] = [helpers.read_env(os.path.join(root, f))[3] for f in envs[tag] if f.endswith(".env")
+                               and f != "out.env"]
+    assert envs["on"] == envs["off"]
+    assert any(not np.array_equal(a, c) for a, c in zip(envs["on_vals"], envs["off_vals"]))

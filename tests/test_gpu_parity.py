@@ -270,9 +270,11 @@ def test_uneven_run_lengths_equal_alone(tmp_path):
 
 @pytest.mark.parametrize("precision", [32, 64])
 def test_graph_replay_and_autotune_bitwise(precision, tmp_path):
-    """The step loop replayed as hipGraphs (storm-own step flags), and a run whose
-    first steps went to the autotuner's trials, both give the fields of the
-    plain fused loop."""
+    """The step loop replayed as hipGraphs (storm-own step flags), a run whose
+    first steps went to the autotuner's trials, and a tuned run then replayed as
+    hipGraphs (the tuner's staged trial filters on step 79 and leaves HB in the
+    second buffer, where the graph's fused steps keep it) all give the fields
+    of the plain fused loop."""
     b, t = synth.tiny(seed=51, ni=30, nj=26, hours=6)
     case = helpers.make_case(str(tmp_path), b, t)
     tracks = synth.meow_tracks(b, n=1024, hours=6, landfall=3)[::200]
@@ -283,20 +285,23 @@ def test_graph_replay_and_autotune_bitwise(precision, tmp_path):
         paths.append(p)
     B = sloshgpu.Basin(case["dta"], case["bntline"])
     res = {}
-    for mode in ("plain", "graph", "tuned"):
+    for mode in ("plain", "graph", "tuned", "tuned_graph"):
         X = sloshgpu.Batch(B, paths, precision=precision)
         X.upload()
         if mode == "graph":
             X.set_graph(16)
-        if mode == "tuned":
+        if mode.startswith("tuned"):
             rows, path, ms = X.autotune(20)
             print("autotune", rows, path, ms, flush=True)
+            if mode == "tuned_graph":
+                X.set_path(sloshgpu.PATH_FUSED, rows)
+                X.set_graph(16)
         X.step(max(X.nsteps) - X.current_step)
         X.sync()
         res[mode] = [{f: X.get(s, f) for f in helpers.FIELDS} for s in range(len(paths))]
         X.close()
     B.close()
-    for mode in ("graph", "tuned"):
+    for mode in ("graph", "tuned", "tuned_graph"):
         for s in range(len(paths)):
             for f in helpers.FIELDS:
                 assert np.array_equal(res[mode][s][f], res["plain"][s][f]), (mode, s, f)
