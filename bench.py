@@ -275,12 +275,18 @@ def uneven(root, nstorm, prec, graph=20):
         work += sum(max(0, s - X.current_step) for s in X.nsteps) * B.cells
     chunk = max(1, graph)
     t0 = time.perf_counter()
+    rounds = 0
     while True:
         active = [(B, X) for B, X, n in runs if X.current_step < max(X.nsteps)]
         if not active:
             break
         for B, X in active:
             X.step(min(chunk, max(X.nsteps) - X.current_step))
+        rounds += 1
+        if rounds % 50 == 0:   # a progress line every 1000 steps (the host runs ahead by at most that)
+            active[0][1].sync()
+            log("uneven: step %d, %d of 6 basins running, %.1f s" % (rounds * chunk, len(active),
+                                                                     time.perf_counter() - t0))
     for B, X, n in runs:
         X.sync()
     dt = time.perf_counter() - t0

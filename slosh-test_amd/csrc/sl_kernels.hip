@@ -1701,9 +1701,11 @@ struct Reg {
 // flags decide, inside the regions of the <TH, true, true> instance.
 // The FP32 instances without FILTER are held to 64 VGPRs, so that 8
 // waves fit on a SIMD (their LDS allows it; the kernel is latency-bound and
-// runs 7 % faster so).
+// runs 7 % faster so); the FP64 instances to 96 VGPRs, 5 waves (their LDS
+// allows five workgroups of four waves on a CU).
 template <int TH, bool SMPT, bool FILT>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(!FILT && sizeof(real) == 4 ? 8 : 1)))
+__global__ void __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(sizeof(real) == 8 ? 5 : !FILT ? 8 : 1)))
 k_stepB(SlBasinDev B, SlRunDev R, StepArgs A, int ntx, int nty, int dyn)
 {
    constexpr int TW = 512 / TH;   // tile width: every tile has 512 points
@@ -1715,9 +1717,12 @@ k_stepB(SlBasinDev B, SlRunDev R, StepArgs A, int ntx, int nty, int dyn)
    using RH = Reg<SMPT ? SLO + 2 : SLO, SMPT ? SHI + 2 : SHI, TH>;
    using RF = Reg<1, 0, TH>;
    using RZ = Reg<SLO + 2, SHI + 2, TH>;
-   __shared__ real sh_hs[RH::n];
+   // HB after CONTTY is read only up to the FLT2G stage; the filtered HB that
+   // FILTER writes afterwards takes its place (less LDS: FP64 tiles fit five
+   // workgroups on a CU instead of four)
+   __shared__ real sh_hs[RH::n > RF::n ? RH::n : RF::n];
    __shared__ real sh_s[RS::n];
-   __shared__ real sh_f[FILT ? RF::n : 1];
+   real *const sh_f = sh_hs;
    __shared__ uint8_t sh_uz[SMPT ? RU::n : 1];
    __shared__ uint8_t sh_pf[SMPT ? RP::n : 1];
    __shared__ uint8_t sh_cl[SMPT ? RC::n : 1];
