@@ -149,6 +149,14 @@ void slg_batch_close(slg_batch *x);
  * 34-110: per cell the largest envelope below 50 ft, 999 where none),
  * mom[jmxb][imxb]; computed on the device, either output may be NULL. */
 int slg_batch_envelopes(slg_batch *x, short *env, short *mom, char *err, int errlen);
+/* As slg_batch_envelopes, allocating nothing.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, serving loops that forbid hipMalloc).  The device staging is
+ * a fixed 32 MiB scratch array of the library, one per GPU, held until the
+ * process ends; storms go through it in chunks.  SLG_ELIMIT if one run's
+ * envelope and the MOM do not fit it (basins above 8 million points).  Calls
+ * that overlap on one GPU share the scratch and race. */
+int slg_batch_envelopes_noalloc(slg_batch *x, short *env, short *mom, char *err, int errlen);
 /* Datums a track carries for the env trailer (ReadTrkFile, slosh2.c:1885-1975),
  * after the surge-only InitWater_TideModeOverride (slosh2.c:408-482, 2238). */
 int slg_track_datums(const char *trk_path, float *ht1, float *ht2);

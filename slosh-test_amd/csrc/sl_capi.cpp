@@ -8,6 +8,7 @@
 // computed on the host up front and uploaded once; stepping then needs no host
 // round trip.
 #include <map>
+#include <mutex>
 #include "sl_int.h"
 
 // --------------------------------------------------------------------------
@@ -914,15 +915,18 @@ extern "C" int slg_batch_set_replay_limits(slg_batch *X, int evcap, long budget)
 template <class R>
 static int check_runs(slg_batch *X, int s0, int ns, char *err, int errlen)
 {
-   std::vector<int> st((size_t) ns * 8);
+   int st[64 * 8];   // in chunks of 64 storms: no host or device allocation
    HIPOK(hipStreamSynchronize(X->st));
-   HIPOK(hipMemcpy(st.data(), run(X, R()).stat + (size_t) s0 * 8, st.size() * 4, hipMemcpyDeviceToHost));
-   for (int s = 0; s < ns; s++)
-      if (st[(size_t) s * 8 + 7]) {
-         seterr(err, errlen, "run %s%d: the drying-cascade replay ran out of visits; its fields are incomplete",
-                "", s0 + s);
-         return SLG_ELIMIT;
-      }
+   for (int c = 0; c < ns; c += 64) {
+      const int m = std::min(64, ns - c);
+      HIPOK(hipMemcpy(st, run(X, R()).stat + (size_t) (s0 + c) * 8, (size_t) m * 32, hipMemcpyDeviceToHost));
+      for (int s = 0; s < m; s++)
+         if (st[s * 8 + 7]) {
+            seterr(err, errlen, "run %s%d: the drying-cascade replay ran out of visits; its fields are incomplete",
+                   "", s0 + c + s);
+            return SLG_ELIMIT;
+         }
+   }
    return 0;
 }
 
@@ -1299,6 +1303,83 @@ extern "C" int slg_batch_envelopes(slg_batch *X, short *env, short *mom, char *e
    if (rc) return rc;
    return X->prec == 64 ? batch_envelopes<double>(X, env, mom, err, errlen)
                         : batch_envelopes<float>(X, env, mom, err, errlen);
+}
+
+// ---- envelopes without allocation ----------------------------------------
+// No counterpart in the reference.  For callers that cannot allocate device
+// memory (code under HIP graph capture, serving loops that forbid hipMalloc),
+// slg_batch_envelopes_noalloc stages the device envelopes and the MOM in this
+// fixed array instead of the buffers slg_batch_envelopes allocates: per GPU
+// one copy at its own address, held until the process ends.  Its device
+// address is taken once per device (hipGetSymbolAddress, with that device
+// current); host code never passes the array's own name to a kernel or a copy.
+#define SLG_NOALLOC_BYTES ((size_t) 32 << 20)
+static __device__ __align__(256) unsigned char scratch[SLG_NOALLOC_BYTES];
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[64];
+
+static int scratch_of_device(void **out, char *err, int errlen)
+{
+   int dev = -1;
+   HIPOK(hipGetDevice(&dev));
+   if (dev < 0 || dev >= 64) {
+      seterr(err, errlen, "device %s%d out of range for the scratch table", "", dev);
+      return SLG_EHIP;
+   }
+   std::lock_guard<std::mutex> lk(g_scratch_mu);
+   if (!g_scratch_ptr[dev]) {
+      void *p = nullptr;
+      HIPOK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+      g_scratch_ptr[dev] = p;
+   }
+   *out = g_scratch_ptr[dev];
+   return 0;
+}
+
+template <class R>
+static int batch_envelopes_noalloc(slg_batch *X, short *env, short *mom, char *err, int errlen)
+{
+   slg_basin *B = X->b;
+   const sl_basin *b = &B->h;
+   const size_t n = (size_t) b->imxb * b->jmxb, nb = (n * 2 + 255) & ~(size_t) 255;
+   if (2 * nb > SLG_NOALLOC_BYTES) {
+      seterr(err, errlen, "basin of %s%d points: its envelope and MOM do not fit the 32 MiB scratch", "", (int) n);
+      return SLG_ELIMIT;
+   }
+   void *scr = nullptr;
+   int rc = scratch_of_device(&scr, err, errlen);
+   if (rc) return rc;
+   short *dmom = (short *) scr, *denv = (short *) ((char *) scr + nb);
+   // storms per chunk: the envelopes of a chunk share the scratch after the MOM
+   const int per = (int) std::min<size_t>((size_t) X->nstorm, (SLG_NOALLOC_BYTES - nb) / (n * 2));
+   for (int s0 = 0; s0 < X->nstorm; s0 += per) {
+      const int ns = std::min(per, X->nstorm - s0);
+      rc = dev_envelopes(&img(B, R()).dev, &run(X, R()), img(B, R()).zbd, s0, ns, env ? denv : nullptr,
+                         mom ? dmom : nullptr, X->st, nullptr, s0 > 0);
+      if (rc) {
+         seterr(err, errlen, "envelope kernel failed (%s%d)", "", -rc);
+         return SLG_EHIP;
+      }
+      // the next chunk's kernel follows this copy on the stream
+      if (env) HIPOK(hipMemcpyAsync(env + (size_t) s0 * n, denv, (size_t) ns * n * 2, hipMemcpyDeviceToHost, X->st));
+   }
+   if (mom) HIPOK(hipMemcpyAsync(mom, dmom, n * 2, hipMemcpyDeviceToHost, X->st));
+   HIPOK(hipStreamSynchronize(X->st));
+   return 0;
+}
+
+// As slg_batch_envelopes, allocating nothing (the staging is the fixed scratch
+// above).  Calls that overlap on one GPU share that scratch and race.
+extern "C" int slg_batch_envelopes_noalloc(slg_batch *X, short *env, short *mom, char *err, int errlen)
+{
+   if (!X->dmem) {
+      seterr(err, errlen, "%s", "batch not uploaded");
+      return SLG_EINPUT;
+   }
+   const int rc = check_runs_any(X, 0, X->nstorm, err, errlen);
+   if (rc) return rc;
+   return X->prec == 64 ? batch_envelopes_noalloc<double>(X, env, mom, err, errlen)
+                        : batch_envelopes_noalloc<float>(X, env, mom, err, errlen);
 }
 
 static void rex_free(slg_batch *X)

@@ -127,19 +127,24 @@ static inline SlRunDevT<double> &run(slg_batch *X, double) { return X->d64; }
 
 
 extern "C" int sl_dev_envelopes_f32(const SlBasinDevT<float> *B, const SlRunDevT<float> *R, const double *hx2,
-                                    const double *zbd, int s0, int ns, short *env, short *mom, hipStream_t st);
+                                    const double *zbd, int s0, int ns, short *env, short *mom, int acc,
+                                    hipStream_t st);
 extern "C" int sl_dev_envelopes_f64(const SlBasinDevT<double> *B, const SlRunDevT<double> *R, const double *hx2,
-                                    const double *zbd, int s0, int ns, short *env, short *mom, hipStream_t st);
-// env values (EnvSave) of HMX, or of hx2 (double [nstorm][fstride]) when given
+                                    const double *zbd, int s0, int ns, short *env, short *mom, int acc,
+                                    hipStream_t st);
+// env values (EnvSave) of HMX, or of hx2 (double [nstorm][fstride]) when given;
+// with acc the MOM continues from what mom holds
 static inline int dev_envelopes(const SlBasinDevT<float> *B, const SlRunDevT<float> *R, const double *zbd, int s0,
-                                int ns, short *env, short *mom, hipStream_t st, const double *hx2 = nullptr)
+                                int ns, short *env, short *mom, hipStream_t st, const double *hx2 = nullptr,
+                                int acc = 0)
 {
-   return sl_dev_envelopes_f32(B, R, hx2, zbd, s0, ns, env, mom, st);
+   return sl_dev_envelopes_f32(B, R, hx2, zbd, s0, ns, env, mom, acc, st);
 }
 static inline int dev_envelopes(const SlBasinDevT<double> *B, const SlRunDevT<double> *R, const double *zbd, int s0,
-                                int ns, short *env, short *mom, hipStream_t st, const double *hx2 = nullptr)
+                                int ns, short *env, short *mom, hipStream_t st, const double *hx2 = nullptr,
+                                int acc = 0)
 {
-   return sl_dev_envelopes_f64(B, R, hx2, zbd, s0, ns, env, mom, st);
+   return sl_dev_envelopes_f64(B, R, hx2, zbd, s0, ns, env, mom, acc, st);
 }
 extern "C" int sl_dev_maxstat_f32(const SlBasinDevT<float> *B, const SlRunDevT<float> *R, const float *hb,
                                   const double *zbd, double *hmax, int init, hipStream_t st);

@@ -1917,14 +1917,20 @@ __device__ __forceinline__ short env_val(const SlBasinDev &B, const T *hmx, cons
 // Envelopes of storms s0..s0+ns-1 (env, [ns][jmxb][imxb], may be null) and
 // their MOM, the maximum of MEOWs (gui/tclsrc/meowgen.tcl:34-110): per point
 // the largest envelope value below 50 ft over the storms, 999 where none.
+// With acc the MOM continues from the values already in mom (storms taken in
+// chunks).
 __global__ void k_envelopes(SlBasinDev B, SlRunDev R, const double *hx2, const double *zbd, int s0, int ns,
-                            short *env, short *mom)
+                            short *env, short *mom, int acc)
 {
    const int cell = blockIdx.x * blockDim.x + threadIdx.x, n = B.imxb * B.jmxb;
    if (cell >= n) return;
    const int i = cell % B.imxb + 1, j = cell / B.imxb + 1;
    int m = 999;
    bool any = false;
+   if (mom && acc) {
+      m = mom[cell];
+      any = m != 999;
+   }
    for (int s = s0; s < s0 + ns; s++) {
       const short v = hx2 ? env_val(B, hx2 + (size_t) s * R.fstride, zbd, i, j)
                           : env_val(B, R.hmx + (size_t) s * R.fstride, zbd, i, j);
@@ -2145,10 +2151,11 @@ using namespace SL_NS;
 #define SL_ENVS sl_dev_envelopes_f32
 #endif
 extern "C" int SL_ENVS(const SlBasinDev *B, const SlRunDev *R, const double *hx2, const double *zbd, int s0,
-                       int ns, short *env, short *mom, hipStream_t st)
+                       int ns, short *env, short *mom, int acc, hipStream_t st)
 {
    const int n = B->imxb * B->jmxb;
-   hipLaunchKernelGGL(k_envelopes, dim3((n + 255) / 256), dim3(256), 0, st, *B, *R, hx2, zbd, s0, ns, env, mom);
+   hipLaunchKernelGGL(k_envelopes, dim3((n + 255) / 256), dim3(256), 0, st, *B, *R, hx2, zbd, s0, ns, env, mom,
+                      acc);
    hipError_t e = hipGetLastError();
    return e == hipSuccess ? 0 : -(int) e;
 }
