@@ -174,8 +174,9 @@ L350:
 }
 
 /* ------------------------------------------------------------ SMPT2G/FLT2G */
-static short i2g[M2G_ + 1], j2g[M2G_ + 1];
-static short mct[MCT_ + 1], i2gg[MCT_ + 1], j2gg[MCT_ + 1];
+/* per thread: several oracle instances may step at once, one per thread */
+static _Thread_local short i2g[M2G_ + 1], j2g[M2G_ + 1];
+static _Thread_local short mct[MCT_ + 1], i2gg[MCT_ + 1], j2gg[MCT_ + 1];
 
 static void flt2g(const orc_basin *b, orc_state *st, int mm2g, int mcmx)
 {

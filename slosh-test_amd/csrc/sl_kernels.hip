@@ -138,18 +138,23 @@ struct GUZ {  // "UB(i,j) == 0" straight from the transport field
    __device__ bool operator()(int i, int j) const { return ub[(uint32_t) ((j - 1) * pitch + (i - 1))] == real(0.); }
 };
 
-// Static fields ZB (squares) and ZBM (momentum points): global or an LDS tile.
+// Static fields ZB (squares), ZBM (momentum points) and the per-point flag
+// word: global or an LDS tile.
 struct ZG {
    const real *zb, *zbm;
+   const uint32_t *fl;
    int pitch;
    __device__ real b(int i, int j) const { return zb[(uint32_t) ((j - 1) * pitch + (i - 1))]; }
    __device__ real m(int i, int j) const { return zbm[(uint32_t) ((j - 1) * pitch + (i - 1))]; }
+   __device__ uint32_t f(int i, int j) const { return fl[(uint32_t) ((j - 1) * pitch + (i - 1))]; }
 };
 struct ZL {
    const real *zb, *zbm;
+   const uint32_t *fl;
    int i0, j0, w;
    __device__ real b(int i, int j) const { return zb[(j - j0) * w + (i - i0)]; }
    __device__ real m(int i, int j) const { return zbm[(j - j0) * w + (i - i0)]; }
+   __device__ uint32_t f(int i, int j) const { return fl[(j - j0) * w + (i - i0)]; }
 };
 
 // Workgroup number -> (tile x, tile y, storm).  Hardware hands workgroup g to
@@ -226,7 +231,10 @@ __device__ __forceinline__ real contty_sq(const SlBasinDev &B, const SlRunDev &R
             if (-chng < real(1E-3)) out = -zb;
             else if (hc[p] == -zb) out = hc[p];
             else {
-               out = -zb;   // final value comes from the replay
+               // the replay sets the final value; until then the square holds
+               // what the serial sweep leaves in a square it has not assigned
+               // (HSUB): a cascade may revisit it and store nothing
+               out = hc[p];
                int k = atomicAdd(&R.evcnt[s], 1);
                if (k < R.evcap) R.evkey[(size_t) s * SL_EVCAP + k] = ((uint32_t) j << 16) | (uint32_t) i;
             }
@@ -338,7 +346,7 @@ __global__ void __launch_bounds__(TX * TYA) k_conttyA(SlBasinDev B, SlRunDev R, 
                if (-chng < real(1E-3)) out = -zb;
                else if (l.hc == -zb) out = l.hc;
                else {
-                  out = -zb;   // final value comes from the replay
+                  out = l.hc;   // HSUB until the replay sets it (see contty_sq)
                   int q = atomicAdd(&R.evcnt[s], 1);
                   if (q < R.evcap) R.evkey[(size_t) s * SL_EVCAP + q] = ((uint32_t) j << 16) | (uint32_t) i;
                }
@@ -712,7 +720,7 @@ __device__ __forceinline__ uint8_t pflag_pt(const SlBasinDev &B, int i, int j, c
 {
    uint8_t f = 0;
    *ndry = 0;
-   if (B.flags[IDX(B, i, j)] & F_PFC) {
+   if (z.f(i, j) & F_PFC) {
       if (!uz(i, j)) {
          if (uz(i - 1, j) || uz(i, j - 1) || uz(i, j + 1) || uz(i + 1, j)) {
             int nd = 0;
@@ -801,7 +809,7 @@ __device__ __forceinline__ real smooth_pt(const SlBasinDev &B, const real *P, co
          }
       }
    }
-   if (B.flags[p] & F_BDRY) {
+   if (z.f(i, j) & F_BDRY) {
       const real xr = B.elpcl[i - 1] * B.cosl[j - 1], yr = B.elpdl[i - 1] * B.sinl[j - 1];
       const real x = xr - P[P_C1] - P[P_AX], y = yr - P[P_C2] - P[P_AY];
       const real rsq = x * x + y * y, r1 = sqrt(rsq) / real(5280.) + real(1.);
@@ -1052,11 +1060,13 @@ __device__ static void mntmbd(const SlBasinDev &B, const real *P, const real *T,
 // Momentum point (i,j) with F_MOM set; hb gives the final HB of the step on the
 // four squares around it.  Writes UB/VB at (i,j) and at the boundary points
 // this point owns.  ect, edt: the metrics ELPCT(I), ELPDT(I) of the point's
-// column (loaded once by a caller that visits several rows of one column).
+// column (loaded once by a caller that visits several rows of one column);
+// u0, v0, zcor: UB, VB and the Coriolis factor at (i,j) as the step found them
+// (no other point writes them, so a caller may load them early).
 template <class HB, class Z>
 __device__ __forceinline__ void momntm_pt(const SlBasinDev &B, const real *P, const real *T, uint32_t fl,
                                           int i, int j, const HB &hb, const Z &z, real *ub, real *vb, real ect,
-                                          real edt)
+                                          real edt, real u0, real v0, real zcor)
 {
    const uint32_t p = IDX(B, i, j);
    const real h1 = hb(i - 1, j - 1), h2 = hb(i, j - 1), h3 = hb(i - 1, j), h4 = hb(i, j);
@@ -1126,6 +1136,7 @@ __device__ __forceinline__ void momntm_pt(const SlBasinDev &B, const real *P, co
          aintrp = real(.25) * aintrp;
       }
    }
+   real uij = real(0.);
    if (blocked) {
       ub[p] = real(0.);
       vb[p] = real(0.);
@@ -1134,8 +1145,8 @@ __device__ __forceinline__ void momntm_pt(const SlBasinDev &B, const real *P, co
    } else {
       aii = real(1.) + P[P_X1B4] * aintrp;
       const real *E = EREC(B, F.iik, F.id);
-      arj = E[E_AR2] * P[P_X1B4] * B.zcor[p];
-      const real aiu = aii * ub[p], aiv = aii * vb[p];
+      arj = E[E_AR2] * P[P_X1B4] * zcor;
+      const real aiu = aii * u0, aiv = aii * v0;
       const real fct = P[P_X1B6] * F.dph;
       real xttx, xtty;
       if (jump != 2) {
@@ -1173,16 +1184,16 @@ __device__ __forceinline__ void momntm_pt(const SlBasinDev &B, const real *P, co
          xttx = real(0.);
          xtty = real(0.);
       }
-      ubj = ub[p];
-      vbj = vb[p];
+      ubj = u0;
+      vbj = v0;
       const real br = E[E_BR2], bi = E[E_BI2];
-      ub[p] = aiu - fct * (br * dhx - bi * dhy) + arj * vb[p] + xttx;
-      vb[p] = aiv - fct * (br * dhy + bi * dhx) - arj * ub[p] + xtty;
+      uij = aiu - fct * (br * dhx - bi * dhy) + arj * v0 + xttx;
+      ub[p] = uij;
+      vb[p] = aiv - fct * (br * dhy + bi * dhx) - arj * uij + xtty;
    }
    // boundary transports (momntm.f:384-458)
    const uint32_t bop = F_BOP(fl);
    if (bop == 0) return;
-   const real uij = ub[p];
    int ijump = 1;
    if (bop <= 4) {
       const int isb = i + cISBS[bop], jsb = j + cJSBS[bop];
@@ -1230,7 +1241,7 @@ __global__ void __launch_bounds__(TX * TY) k_pflag(SlBasinDev B, SlRunDev R, Ste
    const size_t so = (size_t) s * R.fstride;
    uint8_t f = 0;
    int nd = 0;
-   if (in) f = pflag_pt(B, i, j, GUZ{R.ub + so, B.pitch}, GF{A.hc + so, B.pitch}, ZG{B.zb, B.zbm, B.pitch}, &nd);
+   if (in) f = pflag_pt(B, i, j, GUZ{R.ub + so, B.pitch}, GF{A.hc + so, B.pitch}, ZG{B.zb, B.zbm, B.flags, B.pitch}, &nd);
    const int np = wave_sum(f != 0);
    nd = wave_sum(nd);
    if (threadIdx.x == 0) {
@@ -1248,7 +1259,7 @@ __global__ void __launch_bounds__(TX * TY) k_hclass(SlBasinDev B, SlRunDev R, St
    const bool in = (i <= B.imxb && j <= B.jmxb);
    const size_t so = (size_t) s * R.fstride;
    uint8_t c = 0;
-   if (in) c = hclass_pt(B, i, j, GF{A.hc + so, B.pitch}, GB{R.pflag + so, B.pitch}, ZG{B.zb, B.zbm, B.pitch});
+   if (in) c = hclass_pt(B, i, j, GF{A.hc + so, B.pitch}, GB{R.pflag + so, B.pitch}, ZG{B.zb, B.zbm, B.flags, B.pitch});
    const int n1 = wave_sum(c == 1), n2 = wave_sum(c == 2);
    if (threadIdx.x == 0) {
       if (n1) atomicAdd(&R.stat[s * 8 + 2], n1);
@@ -1273,7 +1284,7 @@ __global__ void __launch_bounds__(TX * TY) k_smooth(SlBasinDev B, SlRunDev R, St
    }
    const bool smpt = (int) P[P_FLAGS] & SF_SMPT;
    A.hn[so + p] = smooth_pt(B, P, tabv(R, s, P), smpt, i, j, GF{hs, B.pitch}, GB{R.hclass + so, B.pitch},
-                            ZG{B.zb, B.zbm, B.pitch});
+                            ZG{B.zb, B.zbm, B.flags, B.pitch});
 }
 
 // ------------------------------------------------------- sub-grid 1-D flow
@@ -1647,7 +1658,7 @@ __global__ void __launch_bounds__(TX * TY) k_hmxsv(SlBasinDev B, SlRunDev R, Ste
    if (i > B.imxb - 1 || j > B.jmxb - 1 || !storm_step(R, s)) return;
    const size_t so = (size_t) s * R.fstride, p = IDX(B, i, j);
    const real hx = R.hmx[so + p];
-   const real v = hmxsv_pt(B, i, j, GF{A.hc + so, B.pitch}, hx, ZG{B.zb, B.zbm, B.pitch});
+   const real v = hmxsv_pt(B, i, j, GF{A.hc + so, B.pitch}, hx, ZG{B.zb, B.zbm, B.flags, B.pitch});
    if (!same_bits(v, hx)) R.hmx[so + p] = v;
 }
 
@@ -1660,7 +1671,7 @@ __global__ void __launch_bounds__(TX * TY) k_filter(SlBasinDev B, SlRunDev R, St
    const real *hs = A.hc + so;
    real out = hs[p];
    if (i <= B.imxb - 1 && j <= B.jmxb - 1)
-      out = filter_pt(B, i, j, GF{hs, B.pitch}, R.hsubtop + (size_t) s * B.pitch, ZG{B.zb, B.zbm, B.pitch});
+      out = filter_pt(B, i, j, GF{hs, B.pitch}, R.hsubtop + (size_t) s * B.pitch, ZG{B.zb, B.zbm, B.flags, B.pitch});
    A.hn[so + p] = out;
 }
 
@@ -1673,8 +1684,9 @@ __global__ void __launch_bounds__(TX * TY) k_momntm(SlBasinDev B, SlRunDev R, St
    if (!(fl & F_MOM)) return;
    const size_t so = (size_t) s * R.fstride;
    const real *P = par(R, s, step);
-   momntm_pt(B, P, tabv(R, s, P), fl, i, j, GF{A.hc + so, B.pitch}, ZG{B.zb, B.zbm, B.pitch}, R.ub + so, R.vb + so,
-             B.elpct[i - 1], B.elpdt[i - 1]);
+   const uint32_t p = IDX(B, i, j);
+   momntm_pt(B, P, tabv(R, s, P), fl, i, j, GF{A.hc + so, B.pitch}, ZG{B.zb, B.zbm, B.flags, B.pitch}, R.ub + so, R.vb + so,
+             B.elpct[i - 1], B.elpdt[i - 1], R.ub[so + p], R.vb[so + p], B.zcor[p]);
 }
 
 // ------------------------------------------------------------ fused stage B
@@ -1728,6 +1740,7 @@ k_stepB(SlBasinDev B, SlRunDev R, StepArgs A, int ntx, int nty, int dyn)
    __shared__ uint8_t sh_cl[SMPT ? RC::n : 1];
    __shared__ real sh_zb[RZ::n];
    __shared__ real sh_zbm[RZ::n];
+   __shared__ uint32_t sh_fl[RZ::n];
    __shared__ int sh_cnt[4];
 
    int tx, ty, s;
@@ -1765,7 +1778,18 @@ k_stepB(SlBasinDev B, SlRunDev R, StepArgs A, int ntx, int nty, int dyn)
          const bool in = gi >= 1 && gi <= IM && gj >= 1 && gj <= JM;
          sh_zb[e] = in ? B.zb[IDX(B, gi, gj)] : real(0.);
          sh_zbm[e] = in ? B.zbm[IDX(B, gi, gj)] : real(0.);
+         sh_fl[e] = in ? B.flags[IDX(B, gi, gj)] : 0u;
       }
+   }
+   // the envelope of the thread's two tile squares: only this thread reads or
+   // writes them, so their loads go out now and land while SMPT2G runs
+   static_assert(TW * TH == 512, "two tile points per thread");
+   real *hmx = R.hmx + so;
+   real hx2[2];
+#pragma unroll
+   for (int k = 0; k < 2; k++) {
+      const int e = tid + 256 * k, gi = I0 + e % TW, gj = J0 + e / TW;
+      hx2[k] = (gi <= IM - 1 && gj <= JM - 1) ? hmx[IDX(B, gi, gj)] : real(0.);
    }
    int anyz = 0;
    if (smpt) {
@@ -1787,7 +1811,7 @@ k_stepB(SlBasinDev B, SlRunDev R, StepArgs A, int ntx, int nty, int dyn)
    if (!__syncthreads_or(anyz)) smpt = false;
    const LF hsL{sh_hs, I0 - RH::lo, J0 - RH::lo, RH::w};
    const LB clL{sh_cl, I0 - RC::lo, J0 - RC::lo, RC::w};
-   const ZL zL{sh_zb, sh_zbm, I0 - RZ::lo, J0 - RZ::lo, RZ::w};
+   const ZL zL{sh_zb, sh_zbm, sh_fl, I0 - RZ::lo, J0 - RZ::lo, RZ::w};
    if (smpt) {
       const LB uzL{sh_uz, I0 - RU::lo, J0 - RU::lo, RU::w};
       int np = 0, ndt = 0;
@@ -1846,15 +1870,29 @@ k_stepB(SlBasinDev B, SlRunDev R, StepArgs A, int ntx, int nty, int dyn)
    }
    __syncthreads();
    const LF sL{sh_s, I0 - RS::lo, J0 - RS::lo, RS::w};
-   // HMXSV on the tile's own squares
-   real *hmx = R.hmx + so;
-   for (int e = tid; e < TW * TH; e += 256) {
-      const int gi = I0 + e % TW, gj = J0 + e / TW;
-      if (gi <= IM - 1 && gj <= JM - 1) {
+   // UB, VB and the Coriolis factor of the thread's two momentum points: only
+   // this thread reads or writes them (MNTMBD writes boundary points, which are
+   // no momentum points), so they load now and land during HMXSV and FILTER
+   real *ub = R.ub + so, *vb = R.vb + so;
+   real u0[2], v0[2], zc0[2];
+#pragma unroll
+   for (int k = 0; k < 2; k++) {
+      const int e = tid + 256 * k, gi = I0 + e % TW, gj = J0 + e / TW;
+      u0[k] = v0[k] = zc0[k] = real(0.);
+      if (gi <= IM && gj <= JM && (zL.f(gi, gj) & F_MOM)) {
          const uint32_t p = IDX(B, gi, gj);
-         const real hx = hmx[p];
-         const real v = hmxsv_pt(B, gi, gj, sL, hx, zL);
-         if (!same_bits(v, hx)) hmx[p] = v;
+         u0[k] = ub[p];
+         v0[k] = vb[p];
+         zc0[k] = B.zcor[p];
+      }
+   }
+   // HMXSV on the tile's own squares (HMX loaded at the start)
+#pragma unroll
+   for (int k = 0; k < 2; k++) {
+      const int e = tid + 256 * k, gi = I0 + e % TW, gj = J0 + e / TW;
+      if (gi <= IM - 1 && gj <= JM - 1) {
+         const real v = hmxsv_pt(B, gi, gj, sL, hx2[k], zL);
+         if (!same_bits(v, hx2[k])) hmx[IDX(B, gi, gj)] = v;
       }
    }
    real *hn_g = A.hn + so;
@@ -1871,18 +1909,20 @@ k_stepB(SlBasinDev B, SlRunDev R, StepArgs A, int ntx, int nty, int dyn)
    }
    const LF fL{filt ? sh_f : sh_s, I0 - (filt ? RF::lo : RS::lo), J0 - (filt ? RF::lo : RS::lo),
                filt ? RF::w : RS::w};
-   real *ub = R.ub + so, *vb = R.vb + so;
-   for (int e = tid; e < TW * TH; e += 256) {
+#pragma unroll
+   for (int k = 0; k < 2; k++) {
+      const int e = tid + 256 * k;
       int gi = I0 + e % TW;
-      // the column is the same on every pass of this loop; kept opaque so the
-      // compiler loads its metrics on each pass instead of holding their
-      // addresses across MOMNTM (which spills them at 64 VGPRs)
+      // the column is the same on both passes; kept opaque so the compiler
+      // loads its metrics on each pass instead of holding their addresses
+      // across MOMNTM (which spills them at 64 VGPRs)
       asm volatile("" : "+v"(gi));
       const int gj = J0 + e / TW;
       if (gi <= IM && gj <= JM) {
          hn_g[IDX(B, gi, gj)] = fL(gi, gj);
-         const uint32_t fl = B.flags[IDX(B, gi, gj)];
-         if (fl & F_MOM) momntm_pt(B, P, T, fl, gi, gj, fL, zL, ub, vb, B.elpct[gi - 1], B.elpdt[gi - 1]);
+         const uint32_t fl = zL.f(gi, gj);
+         if (fl & F_MOM)
+            momntm_pt(B, P, T, fl, gi, gj, fL, zL, ub, vb, B.elpct[gi - 1], B.elpdt[gi - 1], u0[k], v0[k], zc0[k]);
       }
    }
 }

@@ -386,3 +386,30 @@ def run_cli_list(exe, root, cases, tag, timeout=900, env_extra=None):
     if r.returncode != 0 or not all(os.path.exists(e) for e, _ in outs):
         raise RuntimeError("%s list run failed (%d): %s %s" % (exe, r.returncode, r.stdout[-800:], r.stderr[-800:]))
     return outs
+
+
+def oracle_checkpoints(case, prec, checkpoints, root):
+    """The oracle's HB/UB/VB/HMX of one storm (case: dta, bntline, trk) at each
+    checkpoint step up to its run's end, and at the end of the last step run
+    ("final"; the whole run when no checkpoint falls inside it); the host
+    initialisation is the product's (exported)."""
+    os.makedirs(root, exist_ok=True)
+    rows, tabs, n = product_export(case, root)
+    cps = sorted(c for c in checkpoints if c <= n)
+    return run_oracle(prec, root + "/init.bin", rows, tabs, cps[-1] if cps else n, cps)
+
+
+def oracle_many(jobs, workers=8):
+    """oracle_checkpoints for several (case, prec, checkpoints, root) jobs in
+    threads (the oracle's C step releases the GIL)."""
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(max_workers=workers) as ex:
+        futs = [ex.submit(oracle_checkpoints, *j) for j in jobs]
+        return [f.result() for f in futs]
+
+
+def track_case(case, trk_path):
+    """The case of one of a batch's storms: the same basin, another track."""
+    c = dict(case)
+    c["trk"] = trk_path
+    return c

@@ -71,3 +71,23 @@ def test_exhausted_visit_budget_is_refused(tmp_path):
     X.envelopes()
     X.close()
     B.close()
+
+
+@pytest.mark.parametrize("path", [sloshgpu.PATH_FUSED, sloshgpu.PATH_STAGED], ids=["fused", "staged"])
+def test_nested_cascade_leaves_hsub(path, tmp_path):
+    """Storm 254 of the MEOW ensemble (250x300, FP64) at step 3880: square
+    (9, 97) dries with a zero rescale factor, the back-step dries (10, 96) a
+    second time, so row 97 is swept again from its start and the revisit of
+    (9, 97), its corner transports now zero, stores nothing (contty.f:128-133):
+    the square keeps HSUB.  Checked against the oracle just after it."""
+    root = str(tmp_path)
+    b = synth.config3_basin()
+    tracks = synth.meow_tracks(b, n=1024)
+    case = synth.write_case(root, b, tracks[0], trk_name="s0000.trk")
+    p = root + "/s254.trk"
+    with open(p, "w") as fh:
+        fh.write(synth.trk_text(tracks[254]))
+    c = helpers.track_case(case, p)
+    print("storm 254 to step 3900, path", path, flush=True)
+    out, events = helpers.gpu_vs_oracle(c, 3900, [3870, 3881, 3900], path=path, tile_rows=32, precision=64)
+    _assert_identical(out, "s254")
