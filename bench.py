@@ -144,7 +144,7 @@ def build_ensemble(root, nstorm):
     return b, paths, trks
 
 
-def batched_run(dta, bnt, trks, prec, steps, warmup, streams, events, path=None, rows=32):
+def batched_run(dta, bnt, trks, prec, steps, warmup, streams, events, path=None, rows=32, span=0):
     """Open and upload the batch, autotune (or fix the path), warm up, then time
     `steps` steps in windows spread to the end of the run.  With events the
     timed steps carry HIP events around every kernel on the batch's stream
@@ -168,6 +168,8 @@ def batched_run(dta, bnt, trks, prec, steps, warmup, streams, events, path=None,
     X.step(warmup)
     X.sync()
     start, end = X.current_step, min(X.nsteps)
+    if span > 0:
+        end = min(end, start + span)
     wins = windows(start, end, steps)
     if sum(b - a for a, b in wins) != steps:
         raise SystemExit("bench.py: %d timed steps asked, the run has %d left after tuning and warm-up"
@@ -518,6 +520,9 @@ def main():
                     help="timed steps of the batched run (48 h runs = 5760 steps; the autotuner's trials "
                          "take 80 and the warm-up W of them); fewer are spread over the run in windows")
     ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--span", type=int, default=0,
+                    help="take the timed windows from the first SPAN steps after the warm-up only (0: to the end of "
+                         "the run); for short profiling runs")
     ap.add_argument("--parts", default="batched,kernels,fp32,rex,plain,uneven,serve,cpu",
                     help="comma list of batched, kernels, fp32, rex, plain, uneven, serve, cpu (and copy: "
                          "the counter calibration copy)")
@@ -534,7 +539,7 @@ def main():
                     help="step the batched ensemble in this many parts of storms, each on its own stream")
     ap.add_argument("--uneven-storms", type=int, default=4096)
     ap.add_argument("--serve-rates", default="5,20,50")
-    ap.add_argument("--serve-seconds", type=float, default=20.0)
+    ap.add_argument("--serve-seconds", type=float, default=15.0)
     ap.add_argument("--serve-hours", type=int, default=72)
     ap.add_argument("--serve-slots", type=int, default=512)
     ap.add_argument("--serve-pump", type=int, default=64)
@@ -594,7 +599,7 @@ def main():
             log("batched %d storms on %dx%d polar basin, FP%d, %d streams" % (a.storms, b.imxb - 1, b.jmxb - 1,
                                                                               prec, a.streams))
             r = batched_run(paths["dta"], paths["bntline"], trks, prec, a.steps, a.warmup, a.streams, False,
-                            path, a.tile_rows)
+                            path, a.tile_rows, a.span)
             rate[prec] = a.storms * r["cells"] * a.steps / r["dt"]
             log("batched %.3e storm.cell.steps/s (%s)" % (rate[prec], r["tune"]))
             out.update(value=rate[prec], ms_per_step=r["dt"] * 1e3 / a.steps)
@@ -610,7 +615,7 @@ def main():
         if "kernels" in parts:
             log("kernels: the same batch at one stream with events, FP%d" % prec)
             rk = batched_run(paths["dta"], paths["bntline"], trks, prec, a.steps, a.warmup, 1, True,
-                             path, a.tile_rows)
+                             path, a.tile_rows, a.span)
             kr = kernel_roofline(rk, prec, a.storms)
             one = a.storms * rk["cells"] * a.steps / rk["dt"]
             out["kernels"] = {"streams": 1, "value": one, "ms_per_step": rk["dt"] * 1e3 / a.steps,
@@ -627,7 +632,7 @@ def main():
         if "fp32" in parts and prec == 64:
             log("batched FP32, %d streams" % a.streams)
             r32 = batched_run(paths["dta"], paths["bntline"], trks, 32, a.steps, a.warmup, a.streams, False,
-                              path, a.tile_rows)
+                              path, a.tile_rows, a.span)
             rate[32] = a.storms * r32["cells"] * a.steps / r32["dt"]
             out["fp32"] = {"value": rate[32], "ms_per_step": r32["dt"] * 1e3 / a.steps,
                            "windows": r32["windows"], "tune": r32["tune"],

@@ -106,13 +106,22 @@ int slg_batch_streams(const slg_batch *x);
 /* Online autotuner: times fused tiles of 8/16/32 rows and the staged path on
  * consecutive chunks of steps_per_trial real steps (all give identical fields)
  * and keeps the fastest; ms_out[4] (may be NULL) gets each candidate's time per
- * step. */
+ * step.  With the fused path it then times storm groups of 1, 8 and 16 against
+ * the default 4 (slg_batch_storm_group gives the choice). */
 int slg_batch_autotune(slg_batch *x, int steps_per_trial, int *rows_out, int *path_out, double *ms_out,
                        char *err, int errlen);
 /* slg_batch_step with HIP events around every kernel on the batch's stream
  * (the host waits once every 64 steps): ms[k] += kernel k time, launches[k] += 1. */
 int slg_batch_step_timed(slg_batch *x, int n, double *ms, int *launches, char *err, int errlen);
 int slg_batch_current_step(const slg_batch *x);
+/* Storms whose tiles follow each other in the fused kernels' workgroup order
+ * (1 .. 64; default 4; call after slg_batch_upload): the tiles of a group of
+ * storms go out tile by tile, so a tile's static basin data (ZB, ZBM, flags,
+ * Coriolis factor), shared by every storm, is read from HBM once per group and
+ * from L2 by the others (the CONTTY pass keeps storm order).  Fields are
+ * identical for any value; the autotuner picks one of 1, 4, 8 and 16. */
+int slg_batch_set_storm_group(slg_batch *x, int sg);
+int slg_batch_storm_group(const slg_batch *x);
 /* For checking the CONTTY drying-cascade replay (contty.f:146-213, the serial
  * back-step sweep) on ordinary runs: evcap (1 .. 4096) drying squares are
  * listed per storm before the replay falls back to visiting every square (the

@@ -701,6 +701,7 @@ static int batch_upload(slg_batch *X, char *err, int errlen)
    D.chq = (R *) (d + off[20]);       // Q = 0 (channl.f:73)
    D.evcap = SL_EVCAP;
    D.vbudget = 0;
+   D.sgroup = SL_SGROUP;
    HIPOK(hipMemset(d, 0, tot));
    // initial water (INTLHT) per storm: HB, HMX=HB, UB=VB=0
    std::vector<R> f(fstride, R(0));
@@ -895,6 +896,21 @@ extern "C" int slg_batch_set_graph(slg_batch *X, int steps_per_graph)
    drop_graph(X);
    X->graph_steps = steps_per_graph;
    return 0;
+}
+
+// Storms whose tiles follow each other in the fused kernels' workgroup order
+// (1 .. 64; fields are identical for any value).
+extern "C" int slg_batch_set_storm_group(slg_batch *X, int sg)
+{
+   if (sg < 1 || sg > 64 || !X->dmem) return SLG_EINPUT;
+   X->d32.sgroup = X->d64.sgroup = sg;
+   drop_graph(X);                     // captured launches hold the old value
+   return 0;
+}
+
+extern "C" int slg_batch_storm_group(const slg_batch *X)
+{
+   return X->prec == 64 ? X->d64.sgroup : X->d32.sgroup;
 }
 
 // Lower the cascade replay's limits, for checking its two fallbacks on
@@ -1724,10 +1740,31 @@ extern "C" int slg_batch_autotune(slg_batch *X, int steps_per_trial, int *rows_o
          bi = c;
       }
    }
-   (void) hipEventDestroy(e0);
-   (void) hipEventDestroy(e1);
    X->path = cand[bi][0];
    X->tile_rows = cand[bi][1];
+   // then the storm group of the fused tile (the trials above ran SL_SGROUP)
+   if (X->path == SLG_PATH_FUSED && X->nstorm > 1) {
+      const int groups[3] = {1, 8, 16};
+      int bg = slg_batch_storm_group(X);
+      for (int c = 0; c < 3; c++) {
+         if (X->step + steps_per_trial > X->maxsteps) break;
+         X->d32.sgroup = X->d64.sgroup = groups[c];
+         HIPOK(hipEventRecord(e0, X->st));
+         int rc = slg_batch_step(X, steps_per_trial, err, errlen);
+         if (rc) return rc;
+         HIPOK(hipEventRecord(e1, X->st));
+         HIPOK(hipEventSynchronize(e1));
+         float ms = 0.f;
+         HIPOK(hipEventElapsedTime(&ms, e0, e1));
+         if (ms / steps_per_trial < best) {
+            best = ms / steps_per_trial;
+            bg = groups[c];
+         }
+      }
+      X->d32.sgroup = X->d64.sgroup = bg;
+   }
+   (void) hipEventDestroy(e0);
+   (void) hipEventDestroy(e1);
    X->graph_steps = keep_graph;
    if (rows_out) *rows_out = X->tile_rows;
    if (path_out) *path_out = X->path;

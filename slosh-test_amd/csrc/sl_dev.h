@@ -13,6 +13,7 @@
 #define SL_NPAR 26
 #define SL_EVCAP 4096         /* per-storm CONTTY event / dirty-square capacity */
 #define SL_MAXSPLIT 4         /* parts of a batch on their own streams */
+#define SL_SGROUP 4           /* default storms per group in the fused kernels' workgroup order */
 #define T_AI2_OFF 147000      /* offset of AI2 in the [6][245][600] table block */
 #define SL_M2G 30000          /* corner-smoothing list limits, smpt2g.f:17 / parm.for:26 */
 #define SL_MCT 8000
@@ -100,6 +101,8 @@ struct SlRunDevT {
    int evcap;                 /* drying squares listed per storm before the replay sweeps
                                  every square (1 .. SL_EVCAP; SL_EVCAP in production) */
    long vbudget;              /* visits one replay may make (0: 64 * squares + 4096) */
+   int sgroup;                /* storms whose tiles follow each other in the fused
+                                 kernels' workgroup order (1: storm by storm) */
 };
 
 #endif

@@ -159,15 +159,32 @@ struct ZL {
 
 // Workgroup number -> (tile x, tile y, storm).  Hardware hands workgroup g to
 // XCD g mod 8; the k-th workgroup of an XCD takes item x*q + min(x, r) + k of
-// the natural (storm, ty, tx) order, a bijection onto 0..G-1.
-__device__ __forceinline__ void tile_of(int g, int G, int ntx, int nty, int &tx, int &ty, int &s)
+// a bijection onto 0..G-1 that walks storms in groups of sg: every tile of a
+// group, each tile for the group's sg storms in a row (so the tile's static
+// basin data, read by each of them, is still in the XCD's L2), then the next
+// group.  sg = 1 is the plain (storm, ty, tx) order.
+__device__ __forceinline__ void tile_of(int g, int G, int ntx, int nty, int sg, int &tx, int &ty, int &s)
 {
    const int x = g & 7, k = g >> 3, q = G >> 3, r = G & 7;
    const int lin = x * q + (x < r ? x : r) + k;
-   tx = lin % ntx;
-   const int t = lin / ntx;
-   ty = t % nty;
-   s = t / nty;
+   const int T = ntx * nty, N = G / T;
+   if (sg < 1) sg = 1;
+   const int full = N / sg, F = full * T * sg;
+   int tile, si, s0;
+   if (lin < F) {
+      const int grp = lin / (T * sg), w = lin - grp * T * sg;
+      tile = w / sg;
+      si = w - tile * sg;
+      s0 = grp * sg;
+   } else {   // the last, partial group of N % sg storms
+      const int rem = N - full * sg, w = lin - F;
+      tile = w / rem;
+      si = w - tile * rem;
+      s0 = full * sg;
+   }
+   s = s0 + si;
+   tx = tile % ntx;
+   ty = tile / ntx;
 }
 
 __device__ __forceinline__ int wave_sum(int v)
@@ -266,7 +283,7 @@ __global__ void __launch_bounds__(TX * TYA) k_conttyA(SlBasinDev B, SlRunDev R, 
                                                      int nty)
 {
    int tx, ty, s;
-   tile_of(blockIdx.x, gridDim.x, ntx, nty, tx, ty, s);
+   tile_of(blockIdx.x, gridDim.x, ntx, nty, 1, tx, ty, s);   // little static data: storm by storm
    const int i = tx * TX + threadIdx.x + 1, step = storm_step(R, s);
    if (i > B.imxb || !step) return;
    const size_t so = (size_t) s * R.fstride;
@@ -1744,7 +1761,7 @@ k_stepB(SlBasinDev B, SlRunDev R, StepArgs A, int ntx, int nty, int dyn)
    __shared__ int sh_cnt[4];
 
    int tx, ty, s;
-   tile_of(blockIdx.x, gridDim.x, ntx, nty, tx, ty, s);
+   tile_of(blockIdx.x, gridDim.x, ntx, nty, R.sgroup, tx, ty, s);
    const int step = storm_step(R, s);
    if (!step) return;   // the whole workgroup: its storm's run is complete
    const int tid = threadIdx.x;

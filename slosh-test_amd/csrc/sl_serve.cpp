@@ -289,6 +289,7 @@ static int pool_alloc(Pool &P, char *err, int errlen)
    D.cur = (int *) (d + off[16]);
    D.evcap = SL_EVCAP;
    D.vbudget = 0;
+   D.sgroup = SL_SGROUP;
    P.denv = (short *) (d + off[17]);
    HIPOK(hipStreamCreate(&P.st));
    for (auto &e : P.ev_round) HIPOK(hipEventCreateWithFlags(&e, hipEventDisableTiming));

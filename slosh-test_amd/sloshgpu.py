@@ -81,6 +81,8 @@ def lib():
         L.slg_server_run_error.argtypes = [vp, i, cp, i]
         L.slg_server_tiles.argtypes = [vp, vp, i]
         L.slg_batch_set_replay_limits.argtypes = [vp, i, ctypes.c_long]
+        L.slg_batch_set_storm_group.argtypes = [vp, i]
+        L.slg_batch_storm_group.argtypes = [vp]
         L.slg_server_close.argtypes = [vp]
         L.slg_server_close.restype = None
         L.slg_batch_current_step.argtypes = [vp]
@@ -179,6 +181,15 @@ class Batch:
     def upload(self):
         err = ctypes.create_string_buffer(512)
         _chk(lib().slg_batch_upload(self.h, err, 512), err)
+
+    def set_storm_group(self, sg: int):
+        """Storms whose tiles follow each other in the fused kernels' workgroup order."""
+        if lib().slg_batch_set_storm_group(self.h, sg):
+            raise SloshError("bad storm group %r (1-64, after upload)" % sg)
+
+    @property
+    def storm_group(self):
+        return lib().slg_batch_storm_group(self.h)
 
     def set_replay_limits(self, evcap: int = 4096, budget: int = 0):
         """Lower the cascade replay's list capacity (it then sweeps every square)

@@ -343,3 +343,42 @@ def test_split_streams_bitwise(tmp_path):
         for s in range(len(trks)):
             for k, f in enumerate(helpers.FIELDS):
                 assert np.array_equal(res[0][s][k], r[s][k]), (s, f)
+
+
+@pytest.mark.parametrize("precision", [32, 64])
+def test_storm_groups_bitwise(precision, tmp_path):
+    """The fused kernels' workgroup order in storm groups of 1, 3, 4 or 16
+    (groups that do not divide the batch, a group larger than it) gives the
+    same fields storm by storm; the autotuner's choice is one of 1, 4, 8, 16."""
+    b, t = synth.tiny(seed=71, ni=40, nj=34, hours=6)
+    root = str(tmp_path)
+    case = helpers.make_case(root, b, t)
+    rng = np.random.default_rng(71)
+    trks = [case["trk"]]
+    for k in range(10):
+        p = "%s/q%d.trk" % (root, k)
+        with open(p, "w") as fh:
+            fh.write(synth.trk_text(synth.random_track(b, rng, hours_range=(5, 6))))
+        trks.append(p)
+    B = sloshgpu.Basin(case["dta"], case["bntline"])
+    res = []
+    for sg in (1, 3, 4, 16):
+        X = sloshgpu.Batch(B, trks, precision=precision)
+        X.upload()
+        X.set_path(sloshgpu.PATH_FUSED, 16)
+        X.set_storm_group(sg)
+        assert X.storm_group == sg
+        X.step(max(X.nsteps))
+        X.sync()
+        res.append([[X.get(s, f) for f in helpers.FIELDS] for s in range(len(trks))])
+        X.close()
+    X = sloshgpu.Batch(B, trks, precision=precision)
+    X.upload()
+    X.autotune(10)
+    assert X.storm_group in (1, 4, 8, 16)
+    X.close()
+    B.close()
+    for r in res[1:]:
+        for s in range(len(trks)):
+            for k, f in enumerate(helpers.FIELDS):
+                assert np.array_equal(res[0][s][k], r[s][k]), (s, f)
