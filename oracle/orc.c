@@ -541,6 +541,68 @@ void orc_filter(const orc_basin *b, const orc_storm *s, orc_state *st)
 }
 
 /* ----------------------------------------------------------------- XFILTER */
+/* FLTER2, flter2.f:1-102: FILTER's smoother on a closed island basin (DOLLAR
+ * '$ ', periodic in J): a neighbour row J-1 = 0 is row JMXB1, row JMXB is a
+ * neighbour like any other, and HB(I,JMXB) = HB(I,1) after the pass.  The two
+ * segments of the reference's loop (I = 2..IMXB2, then I = 1 and IMXB1) cover
+ * every square once and read only HSUB, so one sweep in any order is the same. */
+void orc_flter2(const orc_basin *b, const orc_storm *s, orc_state *st)
+{
+   const int imxb = b->imxb, jmxb = b->jmxb, imxb1 = imxb - 1, jmxb1 = jmxb - 1;
+   const R hcrt = RC(0.5);
+   int i, j, k;
+   (void) s;
+   for (j = 1; j <= jmxb; j++)
+      for (i = 1; i <= imxb; i++) HSUB(i, j) = HB(i, j);
+   for (j = 1; j <= jmxb1; j++)
+      for (i = 1; i <= imxb1; i++) {
+         R hst, hsm, hdif, gam0, gamf, hzz, htemp;
+         if (ZB(i, j) <= -b->hter) continue;
+         hst = HSUB(i, j) + ZB(i, j);
+         if (hst < hcrt) continue;
+         hsm = 0;
+         hdif = 0;
+         gam0 = ELPDL2(i) + (ELPCL2(i) - ELPDL2(i)) * SINL2(j);
+         for (k = 1; k <= 4; k++) {
+            int ii = i + IIH[k], jj = j + JJH[k];
+            if (jj == 0) jj = jmxb1;
+            if (ii == 0) {
+               gamf = 0;
+               hsm = hsm + HSUB(i, j);
+               hzz = HSUB(i, j);
+               hdif = hdif + hzz * gamf;
+               continue;
+            }
+            if (b->ebsn == '$' || b->ebsn == '+') {
+               R gamfk = ELPDL2(ii) + (ELPCL2(ii) - ELPDL2(ii)) * SINL2(jj);
+               R z = gamfk / gam0;
+               gamf = RC(2.) * z / (RC(1.) + z) - RC(1.);
+            } else {
+               gamf = b->gamfp[k - 1];
+            }
+            if (ii == imxb || HSUB(ii, jj) + ZB(ii, jj) < hcrt) {
+               hsm = hsm + HSUB(i, j);
+               hzz = HSUB(i, j);
+            } else {
+               int kk = k % 4 + 1;
+               int ia = i + IP[k], ibb = i + IP[kk], ja = j + JP[k], jb = j + JP[kk];
+               R z1 = ZBM(ia, ja), zz = ZBM(ibb, jb), zzz = AMIN1(z1, zz) + hcrt;
+               if (HSUB(i, j) < zzz || HSUB(ii, jj) < zzz) {
+                  hsm = hsm + HSUB(i, j);
+                  hzz = HSUB(i, j);
+               } else {
+                  hsm = hsm + HSUB(ii, jj);
+                  hzz = HSUB(ii, jj);
+               }
+            }
+            hdif = hdif + hzz * gamf;
+         }
+         htemp = RC(.5) * HSUB(i, j) + (hsm + hdif) / RC(8.);
+         HB(i, j) = AMAX1(-ZB(i, j), htemp);
+      }
+   for (i = 1; i <= imxb; i++) HB(i, jmxb) = HB(i, 1);
+}
+
 void orc_xfilter(const orc_basin *b, const orc_storm *s, orc_state *st, int iversion)
 {
    const int imxb = b->imxb, jmxb = b->jmxb, imxb1 = imxb - 1, jmxb1 = jmxb - 1;
@@ -1378,8 +1440,8 @@ int orc_step(const orc_basin *b, const orc_storm *s, orc_state *st, orc_diag *d,
    if ((s->itime + 1) % incsm == 0 || e1 == '+') {
       int isl = (b->dollar[0] == '$' && b->dollar[1] == ' ');
       if (d) d->filtered = 1;
-      if (isl) return -2;                       /* FLTER2: island basins, not restated */
-      if (b->ismth == 1) orc_filter(b, s, st);
+      if (isl) orc_flter2(b, s, st);            /* closed island (tmstpc2.f:101-111) */
+      else if (b->ismth == 1) orc_filter(b, s, st);
       else orc_xfilter(b, s, st, strncmp(b->sta, "CH2 (2008)", 10) == 0 ? 1 : 2);
    }
    /* TMSTPM */

@@ -512,9 +512,22 @@ static int read_body(frec *r, sl_basin *b, char err[256])
             b->itree[PIX(b, i, j)] = fr_char(r);
          }
       }
+      /* Manning's n (times 1000) of every point, Ver2019-03 basins: a count
+       * line, then (20I3) records per J (crdrd2.f:379-385) */
       if (b->iver == 201903) {
-         snprintf(err, 256, "Manning-n (Ver2019-03) basins are not supported by this build");
-         return -1;
+         fr_next(r);
+         (void) fr_int(r, 3);
+         for (j = 1; j <= jmxb; j++) {
+            fr_next(r);
+            for (i = 1; i <= imxb; i++) {
+               if (i > 1 && (i - 1) % 20 == 0) fr_next(r);
+               b->imann[PIX(b, i, j)] = fr_int(r, 3);
+            }
+         }
+         if (r->eof) {
+            snprintf(err, 256, "the Manning-n map ends early");
+            return -1;
+         }
       }
       if (strcmp(b->dollar, "$ ") && strcmp(b->dollar, "A ")) {
          for (j = 1; j <= jmxb; j += b->jmxb1)

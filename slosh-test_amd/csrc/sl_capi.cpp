@@ -45,8 +45,10 @@ static uint32_t flag_bits(const slg_basin *B, int i, int j)
    if (j >= 2 && j <= B->jend && i >= b->ms[j - 1] && i <= b->me[j - 1] && it != '4' && it != '6') {
       int iskip = 2, bop = 0;
       f |= F_MOM;
-      if (j == 2) iskip = 1;
-      if (j == B->jend) iskip = 3;
+      if (!B->island) {   // an island basin has no side boundaries in J (momntm.f:105-112)
+         if (j == 2) iskip = 1;
+         if (j == B->jend) iskip = 3;
+      }
       if (iskip == 3) bop = (i != b->ms[j - 1]) ? 3 : 6;
       else if (iskip == 1) bop = (i != b->ms[j - 1]) ? 1 : 5;
       else if (i == b->ms[j - 1]) bop = 2;
@@ -247,6 +249,7 @@ static int basin_upload(slg_basin *B, char *err, int errlen)
    D.pitch = P;
    D.npts = (int) np;
    D.jend = B->jend;
+   D.island = B->island;
    D.ebsn_ell = (b->ebsn == '$' || b->ebsn == '+');
    D.ebsn1_amp = (b->ebsn1 == '&');
    D.ismth = b->ismth;
@@ -332,12 +335,6 @@ extern "C" int slg_basin_open(const char *dta, const char *bnt_line, slg_basin *
       return SLG_EINPUT;
    }
    sl_basin *b = &B->h;
-   if (!strcmp(b->dollar, "$ ")) {
-      seterr(err, errlen, "%s", "island (periodic) basins are not supported by this build");
-      sl_basin_free(b);
-      delete B;
-      return SLG_EUNSUP;
-   }
    if (bnt_line && bnt_line[0]) {
       if (sl_basin_llx(b, bnt_line, e)) {
          seterr(err, errlen, "%s", e);
@@ -348,7 +345,10 @@ extern "C" int slg_basin_open(const char *dta, const char *bnt_line, slg_basin *
       strncpy(B->bnt, bnt_line, sizeof B->bnt - 1);
    }
    B->pitch = (b->imxb + 63) & ~63;
-   B->jend = b->jmxb1;
+   // MOMNTM's last row: JMXB on a closed island basin, whose row 1 repeats row
+   // JMXB (momntm.f:98-99, 464-469)
+   B->island = !strcmp(b->dollar, "$ ");
+   B->jend = B->island ? b->jmxb : b->jmxb1;
    // MOMNTM reads the four squares around a point and writes the boundary point
    // beside it; the momentum rows must leave column 1 and column IMXB for those.
    for (int j = 2; j <= B->jend; j++)

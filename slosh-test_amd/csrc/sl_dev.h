@@ -38,6 +38,8 @@ struct SlBasinDevT {
    int npts;                  /* pitch * jmxb: elements of one field */
    int ebsn_ell;              /* grid type '$' or '+': metric GAMF in FILTER/FLT2G */
    int ebsn1_amp;             /* EBSN1 == '&' (HMXSV: no smoothing) */
+   int island;                /* DOLLAR '$ ': closed island, periodic in J (FLTER2,
+                                 MOMNTM to row JMXB, row 1 repeats row JMXB) */
    int ismth, xver;           /* FILTER (1) or XFILTER hcrt code; XFILTER version */
    R hter, hcrt, exclude;
    R gamfp[4];

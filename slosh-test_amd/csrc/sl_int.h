@@ -67,6 +67,7 @@ struct slg_basin {
    sl_basin h;
    char bnt[512];
    int pitch, jend;
+   int island;                         // DOLLAR '$ ': closed island, periodic in J
    std::vector<int> is, ie, ms, me;
    int incsm, ismpt2g;
    BasinImg<float> i32;

@@ -983,6 +983,56 @@ __device__ __forceinline__ real filter_pt(const SlBasinDev &B, int i, int j, con
    return out;
 }
 
+// FLTER2 (flter2.f:17-95) at square (i,j), i <= IMXB-1, j <= JMXB-1, of a
+// closed island basin: FILTER's weights with the angular direction periodic
+// (a neighbour in row 0 is in row JMXB1; row JMXB is an ordinary neighbour).
+// hs is the HSUB snapshot, rows 1..JMXB.
+template <class HS, class Z>
+__device__ __forceinline__ real flter2_pt(const SlBasinDev &B, int i, int j, const HS &hs, const Z &z)
+{
+   const real zb = z.b(i, j), hsub = hs(i, j), hcrt = real(0.5);
+   if (zb <= -B.hter) return hsub;
+   if (hsub + zb < hcrt) return hsub;
+   real hsm = real(0.), hdif = real(0.), hzz, gamf;
+   const real gam0 = B.elpdl2[i - 1] + (B.elpcl2[i - 1] - B.elpdl2[i - 1]) * B.sinl2[j - 1];
+   for (int k = 1; k <= 4; k++) {
+      const int ii = i + cIIH[k];
+      int jj = j + cJJH[k];
+      if (jj == 0) jj = B.jmxb - 1;
+      if (ii == 0) {
+         gamf = real(0.);
+         hsm = hsm + hsub;
+         hzz = hsub;
+      } else {
+         if (B.ebsn_ell) {
+            const real gamfk = B.elpdl2[ii - 1] + (B.elpcl2[ii - 1] - B.elpdl2[ii - 1]) * B.sinl2[jj - 1];
+            const real zz = gamfk / gam0;
+            gamf = real(2.) * zz / (real(1.) + zz) - real(1.);
+         } else {
+            gamf = B.gamfp[k - 1];
+         }
+         if (ii == B.imxb || hs(ii, jj) + z.b(ii, jj) < hcrt) {
+            hsm = hsm + hsub;
+            hzz = hsub;
+         } else {
+            const int kk = k % 4 + 1;
+            const real z1 = z.m(i + cIP[k], j + cJP[k]), z2 = z.m(i + cIP[kk], j + cJP[kk]);
+            const real zzz = fmin(z1, z2) + hcrt, hq = hs(ii, jj);
+            if (hsub < zzz || hq < zzz) {
+               hsm = hsm + hsub;
+               hzz = hsub;
+            } else {
+               hsm = hsm + hq;
+               hzz = hq;
+            }
+         }
+      }
+      hdif = hdif + hzz * gamf;
+   }
+   const real htemp = real(.5) * hsub + (hsm + hdif) / real(8.);
+   return fmax(-zb, htemp);
+}
+
 // ------------------------------------------------------------------ MOMNTM
 struct Forcing {
    real dph;
@@ -1706,6 +1756,52 @@ __global__ void __launch_bounds__(TX * TY) k_momntm(SlBasinDev B, SlRunDev R, St
              B.elpct[i - 1], B.elpdt[i - 1], R.ub[so + p], R.vb[so + p], B.zcor[p]);
 }
 
+// ------------------------------------------------------ closed island basins
+// Does storm s filter on its current step (dyn: its own flag; else filt)?
+__device__ __forceinline__ bool filters(const SlRunDev &R, int s, int step, int dyn, int filt)
+{
+   return dyn ? ((int) par(R, s, step)[P_FLAGS] & SF_FILT) != 0 : filt != 0;
+}
+
+// FLTER2 pass: HB after HMXSV (hc, the HSUB snapshot) -> hn; the last column
+// and row keep HSUB (row JMXB then takes the filtered row 1, k_island_hb).
+__global__ void __launch_bounds__(TX * TY) k_filter2(SlBasinDev B, SlRunDev R, StepArgs A, int dyn, int filt)
+{
+   const int i = blockIdx.x * TX + threadIdx.x + 1, j = blockIdx.y * TY + threadIdx.y + 1;
+   const int s = blockIdx.z, step = storm_step(R, s);
+   if (i > B.imxb || j > B.jmxb || !step || !filters(R, s, step, dyn, filt)) return;
+   const size_t so = (size_t) s * R.fstride, p = IDX(B, i, j);
+   const real *hs = A.hc + so;
+   real out = hs[p];
+   if (i <= B.imxb - 1 && j <= B.jmxb - 1) out = flter2_pt(B, i, j, GF{hs, B.pitch}, ZG{B.zb, B.zbm, B.flags, B.pitch});
+   A.hn[so + p] = out;
+}
+
+// After FLTER2: HB(I,JMXB) = HB(I,1) (flter2.f:96-97), from the filtered
+// field src into dst: the whole field with the row copy (fused path: back into
+// the buffer the step's HB lives in), or row JMXB only (rowonly; staged path,
+// src == dst).
+__global__ void __launch_bounds__(TX * TY) k_island_hb(SlBasinDev B, SlRunDev R, const real *src, real *dst,
+                                                       int dyn, int filt, int rowonly)
+{
+   const int i = blockIdx.x * TX + threadIdx.x + 1, j = blockIdx.y * TY + threadIdx.y + 1;
+   const int s = blockIdx.z, step = storm_step(R, s);
+   if (i > B.imxb || j > B.jmxb || !step || !filters(R, s, step, dyn, filt)) return;
+   if (rowonly && j != B.jmxb) return;
+   const size_t so = (size_t) s * R.fstride;
+   dst[so + IDX(B, i, j)] = src[so + (j == B.jmxb ? IDX(B, i, 1) : IDX(B, i, j))];
+}
+
+// After MOMNTM: UB(I,1) = UB(I,JMXB), VB(I,1) = VB(I,JMXB) (momntm.f:464-469).
+__global__ void k_island_uv(SlBasinDev B, SlRunDev R)
+{
+   const int i = blockIdx.x * blockDim.x + threadIdx.x + 1, s = blockIdx.y;
+   if (i > B.imxb || !storm_step(R, s)) return;
+   const size_t so = (size_t) s * R.fstride;
+   R.ub[so + IDX(B, i, 1)] = R.ub[so + IDX(B, i, B.jmxb)];
+   R.vb[so + IDX(B, i, 1)] = R.vb[so + IDX(B, i, B.jmxb)];
+}
+
 // ------------------------------------------------------------ fused stage B
 // One workgroup = one tile of one storm: 512 points, (512 / TH) x TH (64 x 8,
 // 32 x 16 or 16 x 32; the autotuner picks).  Regions (halo below / above the
@@ -1735,7 +1831,7 @@ struct Reg {
 template <int TH, bool SMPT, bool FILT>
 __global__ void __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu(sizeof(real) == 8 ? 5 : !FILT ? 8 : 1)))
-k_stepB(SlBasinDev B, SlRunDev R, StepArgs A, int ntx, int nty, int dyn)
+k_stepB(SlBasinDev B, SlRunDev R, StepArgs A, int ntx, int nty, int dyn, int isl)
 {
    constexpr int TW = 512 / TH;   // tile width: every tile has 512 points
    constexpr int SLO = FILT ? 2 : 1, SHI = FILT ? 2 : 1;
@@ -1773,7 +1869,9 @@ k_stepB(SlBasinDev B, SlRunDev R, StepArgs A, int ntx, int nty, int dyn)
    const real *T = tabv(R, s, P);
    const int sf = (int) P[P_FLAGS];
    bool smpt = SMPT && (!dyn || (sf & SF_SMPT));
-   const bool filt = FILT && (!dyn || (sf & SF_FILT));
+   // a closed island basin filters (FLTER2) and runs MOMNTM in kernels of
+   // their own: its periodic neighbours and row copies cross tiles
+   const bool filt = FILT && !isl && (!dyn || (sf & SF_FILT));
    if (tid < 4) sh_cnt[tid] = 0;
 
    // HB after CONTTY (and the U==0 mask) with halo; the static ZB and ZBM.
@@ -1896,7 +1994,7 @@ k_stepB(SlBasinDev B, SlRunDev R, StepArgs A, int ntx, int nty, int dyn)
    for (int k = 0; k < 2; k++) {
       const int e = tid + 256 * k, gi = I0 + e % TW, gj = J0 + e / TW;
       u0[k] = v0[k] = zc0[k] = real(0.);
-      if (gi <= IM && gj <= JM && (zL.f(gi, gj) & F_MOM)) {
+      if (!isl && gi <= IM && gj <= JM && (zL.f(gi, gj) & F_MOM)) {
          const uint32_t p = IDX(B, gi, gj);
          u0[k] = ub[p];
          v0[k] = vb[p];
@@ -1938,7 +2036,7 @@ k_stepB(SlBasinDev B, SlRunDev R, StepArgs A, int ntx, int nty, int dyn)
       if (gi <= IM && gj <= JM) {
          hn_g[IDX(B, gi, gj)] = fL(gi, gj);
          const uint32_t fl = zL.f(gi, gj);
-         if (fl & F_MOM)
+         if (!isl && (fl & F_MOM))
             momntm_pt(B, P, T, fl, gi, gj, fL, zL, ub, vb, B.elpct[gi - 1], B.elpdt[gi - 1], u0[k], v0[k], zc0[k]);
       }
    }
@@ -1949,11 +2047,13 @@ template <int TH>
 static void launch_B(int smpt, int filt, int dyn, dim3 g, hipStream_t st, const SlBasinDev &B,
                      const SlRunDev &R, StepArgs a, int ntx, int nty)
 {
+   const int isl = B.island;
+   if (isl) filt = 0;   // FLTER2 runs in its own kernel
    if (dyn || (smpt && filt))
-      hipLaunchKernelGGL((k_stepB<TH, true, true>), g, dim3(256), 0, st, B, R, a, ntx, nty, dyn);
-   else if (smpt) hipLaunchKernelGGL((k_stepB<TH, true, false>), g, dim3(256), 0, st, B, R, a, ntx, nty, 0);
-   else if (filt) hipLaunchKernelGGL((k_stepB<TH, false, true>), g, dim3(256), 0, st, B, R, a, ntx, nty, 0);
-   else hipLaunchKernelGGL((k_stepB<TH, false, false>), g, dim3(256), 0, st, B, R, a, ntx, nty, 0);
+      hipLaunchKernelGGL((k_stepB<TH, true, true>), g, dim3(256), 0, st, B, R, a, ntx, nty, dyn, isl);
+   else if (smpt) hipLaunchKernelGGL((k_stepB<TH, true, false>), g, dim3(256), 0, st, B, R, a, ntx, nty, 0, isl);
+   else if (filt) hipLaunchKernelGGL((k_stepB<TH, false, true>), g, dim3(256), 0, st, B, R, a, ntx, nty, 0, isl);
+   else hipLaunchKernelGGL((k_stepB<TH, false, false>), g, dim3(256), 0, st, B, R, a, ntx, nty, 0, isl);
 }
 
 // ------------------------------------------------------------------ products
@@ -2332,6 +2432,20 @@ extern "C" int SL_LAUNCH(const SlBasinDev *B, const SlRunDev *R, int smpt, int f
       if (th == 8) launch_B<8>(smpt, filt, dyn, g, st, *B, *R, a, nbx, nty);
       else if (th == 32) launch_B<32>(smpt, filt, dyn, g, st, *B, *R, a, nbx, nty);
       else launch_B<16>(smpt, filt, dyn, g, st, *B, *R, a, nbx, nty);
+      if (B->island) {
+         // HB after HMXSV is in h0: FLTER2 into h1 and back (with the row
+         // copy) for the storms that filter, then MOMNTM and the periodic row
+         const dim3 blk(TX, TY), grd((B->imxb + TX - 1) / TX, (B->jmxb + TY - 1) / TY, R->nstorm);
+         StepArgs f;
+         f.hc = h0;
+         f.hn = h1;
+         if (dyn || filt) {
+            hipLaunchKernelGGL(k_filter2, grd, blk, 0, st, *B, *R, f, dyn, filt);
+            hipLaunchKernelGGL(k_island_hb, grd, blk, 0, st, *B, *R, (const real *) h1, h0, dyn, filt, 0);
+         }
+         hipLaunchKernelGGL(k_momntm, grd, blk, 0, st, *B, *R, f);
+         hipLaunchKernelGGL(k_island_uv, dim3((B->imxb + 255) / 256, R->nstorm), dim3(256), 0, st, *B, *R);
+      }
       MARK(3);
       hipLaunchKernelGGL(k_advance, gadv, badv, 0, st, *R);
       hipError_t e = hipGetLastError();
@@ -2374,12 +2488,19 @@ extern "C" int SL_LAUNCH(const SlBasinDev *B, const SlRunDev *R, int smpt, int f
    if (filt) {
       a.hc = h0;
       a.hn = h1;
-      hipLaunchKernelGGL(k_filter, grd, blk, 0, st, *B, *R, a);
+      if (B->island) {   // FLTER2, then HB(I,JMXB) = HB(I,1)
+         hipLaunchKernelGGL(k_filter2, grd, blk, 0, st, *B, *R, a, 0, 1);
+         hipLaunchKernelGGL(k_island_hb, grd, blk, 0, st, *B, *R, (const real *) h1, h1, 0, 1, 1);
+      } else {
+         hipLaunchKernelGGL(k_filter, grd, blk, 0, st, *B, *R, a);
+      }
       fin = h1;
    }
    MARK(7);
    a.hc = fin;
    hipLaunchKernelGGL(k_momntm, grd, blk, 0, st, *B, *R, a);
+   if (B->island)
+      hipLaunchKernelGGL(k_island_uv, dim3((B->imxb + 255) / 256, R->nstorm), dim3(256), 0, st, *B, *R);
    MARK(8);
    hipLaunchKernelGGL(k_advance, gadv, badv, 0, st, *R);
 #undef MARK

@@ -283,14 +283,14 @@ def _f10(*vals):
 def dta_text(b: Basin) -> str:
     """The basin as a Ver2014-08 dta file (crdrd1.f:102-213, crdrd2.f:65-470)."""
     L = []
-    L.append("Ver2014-08")
+    L.append("Ver2019-03" if "manning" in b.extra else "Ver2014-08")
     L.append("Proj=Clarke")
     nm = ("SYN " + b.name.upper())[:10].ljust(10)
     # col 12 grid type, 13-14 island flag, 15 smoothing, 16 order, 17 hemi, 18-21 opts
     # col 15 EBSN1 (smoothing cadence), col 21 X-filter code (crdrd1.f:72-142)
     # (A10,1X,A1,A2,8A1): STA, EBSN, DOLLAR, EBSN1, EBSN2, ISOUTH, NOFLD, NOF1D, ITERRAIN, ISMOOTH
-    L.append(nm + " " + b.gtype + "  " + b.extra.get("ebsn1", " ") + " " * 3 + b.extra.get("nof1d", " ") +
-             " " + b.extra.get("ismooth", " "))
+    L.append(nm + " " + b.gtype + b.extra.get("dollar", "  ") + b.extra.get("ebsn1", " ") + " " * 3 +
+             b.extra.get("nof1d", " ") + " " + b.extra.get("ismooth", " "))
     L.append(_f10(b.azmth))
     if b.gtype in "$+":
         L.append(_f10(b.alto, b.alno))                       # TLAT TLONG (unused)
@@ -344,6 +344,12 @@ def dta_text(b: Basin) -> str:
         row = "".join(b.itree[j, :])
         for k in range(0, len(row), 60):
             L.append(row[k:k + 60])
+    if "manning" in b.extra:                                 # Ver2019-03: Manning's n x 1000 (20I3)
+        L.append("  1")
+        for j in range(b.jmxb):
+            row = [int(v) for v in b.extra["manning"][j, :]]
+            for k in range(0, len(row), 20):
+                L.append("".join("%3d" % v for v in row[k:k + 20]))
     L.append("  1 X")                                        # print sheets
     L.append("%3d" % 1)
     L.append("%3d" % min(b.jmxb - 1, 999))
@@ -552,6 +558,16 @@ def tiny(seed=5, gtype=" ", ni=24, nj=20, dt=60.0, hours=6):
     b = make_basin("tny", gtype, ni, nj, seed=seed, land_frac=0.25,
                    n_barriers=max(4, ni * nj // 60), dt=dt, size_km=(1.0, 3.0), corr=4.0)
     return b, straight_track(b, 12.0, 330.0, 60.0, 15.0, hours=hours, landfall=hours // 2)
+
+
+def add_manning(b: Basin, seed=13):
+    """Make b a Ver2019-03 basin with a seeded Manning's n map (n x 1000, I3):
+    land mostly 100-300 (slip classes 95-245), some below 100 (class 95)."""
+    rng = np.random.default_rng(seed)
+    m = rng.integers(100, 300, size=(b.jmxb, b.imxb))
+    m[rng.random((b.jmxb, b.imxb)) < 0.2] = rng.integers(20, 99)
+    b.extra["manning"] = m
+    return b
 
 
 # uneven ensemble: six basins of uneven size (name, grid type, ni, nj, barrier edges)

@@ -253,6 +253,46 @@ def variant_case(root, idx, ni=28, nj=24, hours=6):
     return make_case(root, b, t)
 
 
+# Closed island basins (DOLLAR '$ ', crdrd1.f:76: periodic in J): FLTER2
+# instead of FILTER/XFILTER (tmstpc2.f:101-111), MOMNTM to row JMXB with no J
+# side boundaries and row 1 copied from row JMXB (momntm.f:96-112, 464-469):
+# (grid type, ebsn1, ismooth).
+ISLAND_VARIANTS = [(" ", " ", " "), (" ", "!", "@"), ("$", " ", " "), ("+", "+", " ")]
+
+
+def island_id(idx):
+    g, e, m = ISLAND_VARIANTS[idx]
+    return "island-%s-e%02x-s%02x" % ({" ": "polar", "$": "ellip", "+": "hyper"}[g], ord(e), ord(m))
+
+
+def island_case(root, idx, ni=30, nj=26, hours=6):
+    g, e, m = ISLAND_VARIANTS[idx]
+    b, t = synth.tiny(seed=200 + idx, gtype=g, ni=ni, nj=nj, hours=hours)
+    t.dp = [85.0] * len(t.dp)
+    b.extra["ebsn1"] = e
+    b.extra["ismooth"] = m
+    b.extra["dollar"] = "$ "
+    return make_case(root, b, t)
+
+
+# Ver2019-03 basins with a Manning's n map (crdrd2.f:379-385): the bottom
+# stress class IIK of each point follows its n (inital.f), (grid type, seed).
+MANNING_VARIANTS = [(" ", 301), ("$", 302), ("+", 303)]
+
+
+def manning_id(idx):
+    g, _ = MANNING_VARIANTS[idx]
+    return "manning-%s" % {" ": "polar", "$": "ellip", "+": "hyper"}[g]
+
+
+def manning_case(root, idx, ni=30, nj=26, hours=6):
+    g, seed = MANNING_VARIANTS[idx]
+    b, t = synth.tiny(seed=seed, gtype=g, ni=ni, nj=nj, hours=hours)
+    t.dp = [85.0] * len(t.dp)
+    synth.add_manning(b, seed)
+    return make_case(root, b, t)
+
+
 # Basins with sub-grid 1-D flow sides (1-D sides with and without banks, raised
 # weirs, cuts; 'T' canal sheltering and rotated stress directions) and
 # channels: (grid type, seed, ebsn1, ismooth, nof1d, channels).  NOF1D '+'

@@ -36,6 +36,9 @@ def cli_cases():
     for idx in (0, 1):                   # sub-grid 1-D flow sides, banks, weirs, cuts, channels
         b, t = helpers.edge_basin(idx)
         yield "sides-" + helpers.edge_id(idx), b, t
+    b, t = synth.tiny(seed=205, ni=30, nj=26, hours=6)   # closed island: FLTER2, periodic rows
+    b.extra["dollar"] = "$ "
+    yield "island-polar", b, t
 
 
 CASES = list(cli_cases())

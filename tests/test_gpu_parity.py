@@ -350,7 +350,7 @@ def test_storm_groups_bitwise(precision, tmp_path):
     """The fused kernels' workgroup order in storm groups of 1, 3, 4 or 16
     (groups that do not divide the batch, a group larger than it) gives the
     same fields storm by storm; the autotuner's choice is one of 1, 4, 8, 16."""
-    b, t = synth.tiny(seed=71, ni=40, nj=34, hours=6)
+    b, t = synth.tiny(seed=71, ni=40, nj=34, hours=8)
     root = str(tmp_path)
     case = helpers.make_case(root, b, t)
     rng = np.random.default_rng(71)
@@ -358,7 +358,7 @@ def test_storm_groups_bitwise(precision, tmp_path):
     for k in range(10):
         p = "%s/q%d.trk" % (root, k)
         with open(p, "w") as fh:
-            fh.write(synth.trk_text(synth.random_track(b, rng, hours_range=(5, 6))))
+            fh.write(synth.trk_text(synth.random_track(b, rng, hours_range=(8, 9))))
         trks.append(p)
     B = sloshgpu.Basin(case["dta"], case["bntline"])
     res = []
@@ -382,3 +382,62 @@ def test_storm_groups_bitwise(precision, tmp_path):
         for s in range(len(trks)):
             for k, f in enumerate(helpers.FIELDS):
                 assert np.array_equal(res[0][s][k], r[s][k]), (s, f)
+
+
+# ----------------------------------------------------- closed island basins
+# FLTER2, MOMNTM to row JMXB without J side boundaries and the periodic row
+# copies (flter2.f, momntm.f:96-112, 464-469), on both launch paths.
+
+@pytest.mark.parametrize("precision", [32, 64])
+@pytest.mark.parametrize("path", [sloshgpu.PATH_FUSED, sloshgpu.PATH_STAGED], ids=["fused", "staged"])
+@pytest.mark.parametrize("idx", range(len(helpers.ISLAND_VARIANTS)), ids=helpers.island_id)
+def test_island_bitwise(idx, path, precision, tmp_path):
+    case = helpers.island_case(str(tmp_path), idx)
+    print("island", helpers.island_id(idx), "path", path, "fp%d" % precision, flush=True)
+    out, _ = helpers.gpu_vs_oracle(case, 300, [1, 19, 20, 40, 100, 200], path=path, precision=precision)
+    _assert_identical(out, helpers.island_id(idx))
+
+
+def test_island_graph_and_batch_equal_alone(tmp_path):
+    """Island storms batched (runs of different lengths, hipGraph replay with
+    each storm's own filter steps) equal the same storms run alone."""
+    b, t = synth.tiny(seed=204, ni=30, nj=26, hours=9)
+    b.extra["dollar"] = "$ "
+    root = str(tmp_path)
+    case = helpers.make_case(root, b, t)
+    rng = np.random.default_rng(204)
+    trks = [case["trk"]]
+    for k in range(5):
+        p = "%s/i%d.trk" % (root, k)
+        with open(p, "w") as fh:
+            fh.write(synth.trk_text(synth.random_track(b, rng, hours_range=(8, 10))))
+        trks.append(p)
+    B = sloshgpu.Basin(case["dta"], case["bntline"])
+    X = sloshgpu.Batch(B, trks)
+    X.upload()
+    X.set_path(sloshgpu.PATH_FUSED, 16)
+    X.set_graph(8)
+    X.step(max(X.nsteps))
+    X.sync()
+    mix = [{f: X.get(s, f) for f in helpers.FIELDS} for s in range(len(trks))]
+    X.close()
+    for s, p in enumerate(trks):
+        A = sloshgpu.Batch(B, [p])
+        A.set_path(sloshgpu.PATH_STAGED, 16)
+        A.upload()
+        A.step(A.nsteps[0])
+        A.sync()
+        for f in helpers.FIELDS:
+            assert np.array_equal(A.get(0, f), mix[s][f]), (s, f)
+        A.close()
+    B.close()
+
+
+# ------------------------------------------- Ver2019-03 basins (Manning's n)
+@pytest.mark.parametrize("precision", [32, 64])
+@pytest.mark.parametrize("idx", range(len(helpers.MANNING_VARIANTS)), ids=helpers.manning_id)
+def test_manning_bitwise(idx, precision, tmp_path):
+    case = helpers.manning_case(str(tmp_path), idx)
+    print("manning", helpers.manning_id(idx), "fp%d" % precision, flush=True)
+    out, _ = helpers.gpu_vs_oracle(case, 300, [1, 20, 100, 200], precision=precision)
+    _assert_identical(out, helpers.manning_id(idx))

@@ -32,6 +32,79 @@ def test_variant_oracle64_and_host_match_reference(idx, tmp_path):
             assert np.array_equal(snaps[c][f].ravel(), st[f]), "step %d %s differs" % (c, f)
 
 
+@pytest.mark.parametrize("idx", range(len(helpers.ISLAND_VARIANTS)), ids=helpers.island_id)
+def test_island_oracle64_and_host_match_reference(idx, tmp_path):
+    """Closed island basins: the product's initialisation and the FP64 oracle
+    (FLTER2, MOMNTM's island rows and periodic copy) equal the reference build
+    at every checkpoint, filter steps included."""
+    root = str(tmp_path)
+    case = helpers.island_case(root, idx)
+    ref = helpers.run_reference(case, root + "/ref", 300, ",".join(map(str, CHECKS)))
+    rows, tabs, _ = helpers.product_export(case, root)
+    ri = helpers.read_rec(ref + "/init.bin")
+    mi = helpers.read_rec(root + "/init.bin")
+    for k in mi:
+        assert np.array_equal(ri[k], mi[k]), "init record %s differs" % k
+    snaps = helpers.run_oracle("64", root + "/init.bin", rows, tabs, 300, set(CHECKS))
+    for c in CHECKS:
+        st = helpers.read_rec(ref + "/s%06d.bin" % c)
+        for f in helpers.FIELDS:
+            assert np.array_equal(snaps[c][f].ravel(), st[f]), "step %d %s differs" % (c, f)
+
+
+@pytest.mark.parametrize("idx", range(len(helpers.MANNING_VARIANTS)), ids=helpers.manning_id)
+def test_manning_oracle64_and_host_match_reference(idx, tmp_path):
+    """Ver2019-03 basins: the Manning's n map read by the product's host side
+    sets the same initial records (bottom stress classes included) as the
+    reference build, and the FP64 oracle then equals it at every checkpoint."""
+    root = str(tmp_path)
+    case = helpers.manning_case(root, idx)
+    ref = helpers.run_reference(case, root + "/ref", 300, ",".join(map(str, CHECKS)))
+    rows, tabs, _ = helpers.product_export(case, root)
+    ri = helpers.read_rec(ref + "/init.bin")
+    mi = helpers.read_rec(root + "/init.bin")
+    for k in mi:
+        assert np.array_equal(ri[k], mi[k]), "init record %s differs" % k
+    snaps = helpers.run_oracle("64", root + "/init.bin", rows, tabs, 300, set(CHECKS))
+    for c in CHECKS:
+        st = helpers.read_rec(ref + "/s%06d.bin" % c)
+        for f in helpers.FIELDS:
+            assert np.array_equal(snaps[c][f].ravel(), st[f]), "step %d %s differs" % (c, f)
+
+
+def test_manning_map_changes_the_run(tmp_path):
+    """The same basin read as Ver2014-08 (no map: every class from the fixed
+    default) and as Ver2019-03 with a map gives different initial records: the
+    Manning pinning above covers a map that matters."""
+    root = str(tmp_path)
+    a = helpers.manning_case(root + "/a", 0)
+    g, seed = helpers.MANNING_VARIANTS[0]
+    b, t = helpers.synth.tiny(seed=seed, gtype=g, ni=30, nj=26, hours=6)
+    t.dp = [85.0] * len(t.dp)
+    c = helpers.make_case(root + "/b", b, t)
+    helpers.product_export(a, root + "/a")
+    helpers.product_export(c, root + "/b")
+    ra = helpers.read_rec(root + "/a/init.bin")
+    rb = helpers.read_rec(root + "/b/init.bin")
+    assert any(not np.array_equal(ra[k], rb[k]) for k in ra)
+
+
+def test_island_planted_mistake_is_caught(tmp_path):
+    """FLTER2 run as the plain FILTER (no periodic neighbour, no row copy) on an
+    island basin must change the run: the island tests see the periodic rules."""
+    root = str(tmp_path)
+    case = helpers.island_case(root, 0)
+    rows, tabs, _ = helpers.product_export(case, root)
+    a = helpers.run_oracle("64", root + "/init.bin", rows, tabs, 200, {200})
+    recs = {k: np.array(v) for k, v in helpers.read_rec(root + "/init.bin").items()}
+    fl = recs["FLAGS"].copy()
+    fl[0:2] = np.frombuffer(b"  ", dtype=np.uint8)   # DOLLAR '$ ' -> '  '
+    recs["FLAGS"] = fl
+    helpers.write_rec(root + "/init2.bin", recs)
+    b = helpers.run_oracle("64", root + "/init2.bin", rows, tabs, 200, {200})
+    assert any(not np.array_equal(a[200][f], b[200][f]) for f in helpers.FIELDS)
+
+
 @pytest.mark.parametrize("idx", range(len(helpers.EDGE_VARIANTS)), ids=helpers.edge_id)
 def test_sides_oracle64_and_host_match_reference(idx, tmp_path):
     """Basins with sub-grid 1-D flow sides and channels: the product's reading
