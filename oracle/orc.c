@@ -811,7 +811,11 @@ static void mntmbd(const orc_basin *b, const orc_storm *s, orc_state *st, int ju
 }
 
 /* ------------------------------------------------------------------ MOMNTM */
-void orc_momntm(const orc_basin *b, const orc_storm *s, orc_state *st)
+/* Returns -2 when a point it evaluates lies outside the Coriolis table: LLAT =
+ * INT(YLT) must be 0..88 for CORL(LLAT+1) and CORL(LLAT+2) (momntm.f:294-295);
+ * beyond it the reference reads outside COMMON /DUMB4I/ and has no defined
+ * result. */
+int orc_momntm(const orc_basin *b, const orc_storm *s, orc_state *st)
 {
    const int imxb = b->imxb, jmxb = b->jmxb, jmxb1 = jmxb - 1;
    static const int ISBS[5] = {0, 0, -1, 0, 1}, JSBS[5] = {0, -1, 0, 1, 0};
@@ -919,7 +923,9 @@ void orc_momntm(const orc_basin *b, const orc_storm *s, orc_state *st)
          aii = RC(1.) + s->x1b4 * aintrp;
          {
             int llat = (int) YLT(i, j);
-            R zcor = b->corl[llat] + (YLT(i, j) - llat) * (b->corl[llat + 1] - b->corl[llat]);
+            R zcor;
+            if (llat < 0 || llat > 88) return -2;
+            zcor = b->corl[llat] + (YLT(i, j) - llat) * (b->corl[llat + 1] - b->corl[llat]);
             arj = TAB(ar2, iik, id) * s->x1b4 * zcor;
          }
          aiu = aii * UB(i, j);
@@ -1032,6 +1038,7 @@ void orc_momntm(const orc_basin *b, const orc_storm *s, orc_state *st)
          VB(i, 1) = VB(i, jmxb);
       }
    (void) hpd;
+   return 0;
 }
 
 /* ------------------------------------------------------------------ CHANNL */
@@ -1445,6 +1452,5 @@ int orc_step(const orc_basin *b, const orc_storm *s, orc_state *st, orc_diag *d,
       else orc_xfilter(b, s, st, strncmp(b->sta, "CH2 (2008)", 10) == 0 ? 1 : 2);
    }
    /* TMSTPM */
-   orc_momntm(b, s, st);
-   return 0;
+   return orc_momntm(b, s, st);
 }

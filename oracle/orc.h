@@ -83,7 +83,7 @@ void orc_bdryht(const orc_basin *b, const orc_storm *s, orc_state *st);
 void orc_hmxsv(const orc_basin *b, const orc_storm *s, orc_state *st);
 void orc_filter(const orc_basin *b, const orc_storm *s, orc_state *st);
 void orc_xfilter(const orc_basin *b, const orc_storm *s, orc_state *st, int iversion);
-void orc_momntm(const orc_basin *b, const orc_storm *s, orc_state *st);
+int orc_momntm(const orc_basin *b, const orc_storm *s, orc_state *st);
 void orc_flw1dm_c(const orc_basin *b, const orc_storm *s, orc_state *st);
 void orc_channl(const orc_basin *b, const orc_storm *s, orc_state *st);
 void orc_flw1dm_m(const orc_basin *b, const orc_storm *s, orc_state *st);

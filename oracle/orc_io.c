@@ -386,7 +386,7 @@ int orc_call(void *vc, const char *routine, const double *row, int *diag)
    else if (!strcmp(routine, "filter")) orc_filter(&c->b, &s, &c->st);
    else if (!strcmp(routine, "xfilter1")) orc_xfilter(&c->b, &s, &c->st, 1);
    else if (!strcmp(routine, "xfilter2")) orc_xfilter(&c->b, &s, &c->st, 2);
-   else if (!strcmp(routine, "momntm")) orc_momntm(&c->b, &s, &c->st);
+   else if (!strcmp(routine, "momntm")) rc = orc_momntm(&c->b, &s, &c->st);
    else return -9;
    if (diag) {
       diag[0] = c->diag.drying_events;

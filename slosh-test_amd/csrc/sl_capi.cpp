@@ -96,6 +96,11 @@ static void basin_images(slg_basin *B)
          int llat = (int) ylt;
          if (llat >= 0 && llat <= 88)
             I.zcor[q] = corl[llat] + (ylt - (R) llat) * (corl[llat + 1] - corl[llat]);
+         else if ((I.flags[q] & F_MOM) && !I.lat_i) {   // outside CORL(1..90): refused at upload
+            I.lat_i = i;
+            I.lat_j = j;
+            I.lat_v = (double) ylt;
+         }
       }
    // BDRYHT squares (the ISH/JSH list DEPSFC builds, depsfc.f:376-390)
    for (int n = 0; n < b->nbcpt; n++)
@@ -209,6 +214,15 @@ static int basin_upload(slg_basin *B, char *err, int errlen)
 {
    BasinImg<R> &I = img(B, R());
    if (!I.built) basin_images<R>(B);
+   if (I.lat_i) {
+      // MOMNTM takes the Coriolis factor from CORL(LLAT+1), CORL(LLAT+2) with
+      // LLAT = INT(YLT) (momntm.f:294-295): a momentum point whose latitude
+      // truncates outside 0..88 reads outside the table and has no defined value
+      if (err && errlen > 0)
+         snprintf(err, errlen, "grid point (%d,%d) lies at latitude %.3f, outside the Coriolis table (0 to 89)",
+                  I.lat_i, I.lat_j, I.lat_v);
+      return SLG_EINPUT;
+   }
    sl_basin *b = &B->h;
    const int P = B->pitch, NJ = b->jmxb;
    const size_t np = (size_t) P * NJ, rs = sizeof(R);

@@ -58,6 +58,8 @@ struct BasinImg {
    std::vector<R> ereal, ecut;     // [nsqrwc][8], [ncut][4]
    int nsq = 0;                    // distinct squares the sides join
    bool built = false, on_dev = false;
+   int lat_i = 0, lat_j = 0;       // first momentum point outside the Coriolis table
+   double lat_v = 0;
    void *dmem = nullptr;
    SlBasinDevT<R> dev;
    const double *zbd = nullptr;   // device ZB in double (envelope dry test, as EnvSave)

@@ -93,10 +93,13 @@ def _grf(rng, shape, corr):
 
 def make_basin(name="tst", gtype=" ", ni=100, nj=90, seed=1, land_frac=0.20,
                n_barriers=50, dt=60.0, size_km=(0.5, 3.0), alto=29.5, alno=89.5,
-               depth_m=60.0, land_m=5.0, corr=10.0, cfl=0.45) -> Basin:
+               depth_m=60.0, land_m=5.0, corr=10.0, cfl=0.45, hyp_axes=(60.0, 40.0)) -> Basin:
     """Generate a basin of ni x nj cells (height points), as in the workloads:
     polar with radial cell size growing geometrically from size_km[0] to size_km[1],
-    elliptic ('$') or hyperbolic ('+') with the same depth/land generator."""
+    elliptic ('$') or hyperbolic ('+') with the same depth/land generator.
+    hyp_axes: the hyperbolic grid's AAXIS and BAXIS (miles); with GSIZE fixed
+    the cell size near the focal line stays the same and larger axes make the
+    cells grow more slowly with I."""
     rng = np.random.default_rng(seed)
     imxb, jmxb = ni + 1, nj + 1
     extra = {}
@@ -125,7 +128,7 @@ def make_basin(name="tst", gtype=" ", ni=100, nj=90, seed=1, land_frac=0.20,
     elif gtype == "+":
         # hyperbolic (elliptic II): DELA = 2 GSIZE/(AAXIS+BAXIS) (crdrd1.f:188-195)
         ndb = nj // 2
-        aaxis, baxis = 60.0, 40.0
+        aaxis, baxis = hyp_axes
         gsize = 0.6
         dela = 2.0 * gsize / (aaxis + baxis)
         irb = -int(round(0.1 / dela))
@@ -580,7 +583,11 @@ def config4_basins(seed=31, dt=30.0):
     200x250, 300x300 and 400x350, hyperbolic 350x200 (seeded depth fields)."""
     out = []
     for k, (nm, g, ni, nj, nbar) in enumerate(C4_BASINS):
-        out.append(make_basin(nm, g, ni, nj, seed=seed + k, land_frac=0.25, n_barriers=nbar, dt=dt))
+        # 350 cells along I: axes of 260 x 174 mi keep the cells 0.55-1.6 mi and
+        # the grid within 24-30 N (60 x 40 mi would grow the cells to 44 mi and
+        # take the grid south of the equator, outside the Coriolis table)
+        out.append(make_basin(nm, g, ni, nj, seed=seed + k, land_frac=0.25, n_barriers=nbar, dt=dt,
+                              hyp_axes=(260.0, 174.0)))
     return out
 
 

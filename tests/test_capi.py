@@ -5,6 +5,7 @@ import glob
 import os
 import re
 
+import numpy as np
 import pytest
 
 import helpers
@@ -84,4 +85,27 @@ def test_batch_size_limit(tmp_path):
     B = sloshgpu.Basin(case["dta"], case["bntline"])
     with pytest.raises(sloshgpu.SloshError, match="65535"):
         sloshgpu.Batch(B, [case["trk"]] * 65536)
+    B.close()
+
+
+def test_latitude_outside_coriolis_table_refused(tmp_path):
+    """MOMNTM takes its Coriolis factor from CORL(INT(YLT)+1..+2)
+    (momntm.f:294-295), defined for latitudes -1 < YLT < 89 only; beyond that
+    the reference reads outside the table.  A basin whose momentum points
+    reach such latitudes is refused at upload, before any device work, and the
+    oracle refuses the step (-2) instead of reading out of bounds."""
+    root = str(tmp_path)
+    # a hyperbolic grid 350 cells long with small axes: cells grow to tens of
+    # miles and the grid runs south of the equator
+    b = helpers.synth.make_basin("hyb", "+", 350, 200, seed=35, land_frac=0.25, n_barriers=50, dt=30.0)
+    rng = np.random.default_rng(3)
+    case = helpers.synth.write_case(root, b, helpers.synth.random_track(b, rng, hours_range=(8, 9)))
+    B = sloshgpu.Basin(case["dta"], case["bntline"])
+    X = sloshgpu.Batch(B, [case["trk"]])
+    with pytest.raises(sloshgpu.SloshError, match="outside the Coriolis table"):
+        X.upload()
+    rows, tabs, _ = helpers.product_export(case, root)
+    with pytest.raises(RuntimeError, match="oracle step 1 returned -2"):
+        helpers.run_oracle("32", root + "/init.bin", rows, tabs, 1)
+    X.close()
     B.close()
