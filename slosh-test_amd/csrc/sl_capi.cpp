@@ -419,7 +419,8 @@ extern "C" void slg_basin_close(slg_basin *B)
    delete B;
 }
 
-// basins with sub-grid 1-D flow sides (their passes need the staged path)
+// basins with sub-grid 1-D flow sides or channels (their passes run in kernels
+// of their own between SMPT2G and HMXSV, on either path)
 static bool has_sides(const slg_basin *B)
 {
    return B->h.nsqrw > 0 || (B->h.nsqrwc > 0 && B->h.nof1d != '+') || B->h.npss > 0;
@@ -610,9 +611,7 @@ extern "C" int slg_batch_open_prec(slg_basin *B, int nstorm, const char *const *
    for (int s = 0; s < nstorm; s++) sl_initial_water(&B->h, &X->sc[s].s0, &X->hb0[(size_t) s * np]);
    X->step = 0;
    X->parity = 0;
-   // basins with 1-D flow sides step on the staged path (their side passes sit
-   // between routines the fused kernel runs in one pass)
-   X->path = has_sides(B) ? SLG_PATH_STAGED : SLG_PATH_FUSED;
+   X->path = SLG_PATH_FUSED;
    X->tile_rows = 16;
    X->hbs_buf = -1;
    X->last_par.assign(nstorm, 0);
@@ -638,7 +637,6 @@ extern "C" int slg_batch_set_path(slg_batch *X, int path, int tile_rows)
    if ((path != SLG_PATH_FUSED && path != SLG_PATH_STAGED) ||
        (tile_rows != 8 && tile_rows != 16 && tile_rows != 32))
       return SLG_EINPUT;
-   if (path == SLG_PATH_FUSED && has_sides(X->b)) return SLG_EUNSUP;
    X->path = path;
    X->tile_rows = tile_rows;
    drop_graph(X);
@@ -688,6 +686,8 @@ static int batch_upload(slg_batch *X, char *err, int errlen)
    add((size_t) S * nl);                                               // f1dact
    add((size_t) S * nq * rs);                                          // hsq
    add((size_t) S * 5 * rs);                                           // chq
+   const bool sides = has_sides(B);
+   add(sides ? np * rs : 0);                                           // hb2
    char *d = nullptr;
    HIPOK(hipMalloc((void **) &d, tot));
    X->dmem = d;
@@ -713,6 +713,7 @@ static int batch_upload(slg_batch *X, char *err, int errlen)
    D.f1dact = (uint8_t *) (d + off[18]);
    D.hsq = (R *) (d + off[19]);
    D.chq = (R *) (d + off[20]);       // Q = 0 (channl.f:73)
+   D.hb2 = sides ? (R *) (d + off[21]) : nullptr;
    D.evcap = SL_EVCAP;
    D.vbudget = 0;
    D.sgroup = SL_SGROUP;
@@ -785,6 +786,7 @@ static SlRunDevT<R> run_view(const SlRunDevT<R> &D, const SlBasinDevT<R> &B, int
    V.f1dact += s * (size_t) B.nsqrwc;
    V.hsq += s * (size_t) B.nsq;
    V.chq += s * 5;
+   if (V.hb2) V.hb2 += s * fs;
    return V;
 }
 
@@ -906,7 +908,6 @@ static int build_graph(slg_batch *X, char *err, int errlen)
 extern "C" int slg_batch_set_graph(slg_batch *X, int steps_per_graph)
 {
    if (steps_per_graph < 0) return SLG_EINPUT;
-   if (steps_per_graph > 0 && has_sides(X->b)) return SLG_EUNSUP;   // graphs replay fused steps
    drop_graph(X);
    X->graph_steps = steps_per_graph;
    return 0;
@@ -1725,7 +1726,7 @@ extern "C" int slg_batch_autotune(slg_batch *X, int steps_per_trial, int *rows_o
 {
    const int cand[4][2] = {{SLG_PATH_FUSED, 8}, {SLG_PATH_FUSED, 16}, {SLG_PATH_FUSED, 32}, {SLG_PATH_STAGED, 16}};
    double best = 1e300;
-   int bi = has_sides(X->b) ? 3 : 1;
+   int bi = 1;
    const int keep_graph = X->graph_steps;
    X->graph_steps = 0;
    drop_graph(X);
@@ -1734,10 +1735,6 @@ extern "C" int slg_batch_autotune(slg_batch *X, int steps_per_trial, int *rows_o
    HIPOK(hipEventCreate(&e1));
    for (int c = 0; c < 4; c++) {
       if (X->step + steps_per_trial > X->maxsteps) break;
-      if (cand[c][0] == SLG_PATH_FUSED && has_sides(X->b)) {
-         if (ms_out) ms_out[c] = 0.;
-         continue;
-      }
       X->path = cand[c][0];
       X->tile_rows = cand[c][1];
       HIPOK(hipEventRecord(e0, X->st));

@@ -83,6 +83,8 @@ struct SlRunDevT {
    int nstorm;
    size_t fstride;            /* elements per storm field */
    R *hb0, *hb1, *ub, *vb, *hmx;   /* [nstorm][fstride] */
+   R *hb2;                    /* [nstorm][fstride]: HB after TMSTPC1 on the fused path of
+                                 basins with 1-D flow sides or channels (else null) */
    R *hsubtop;                /* [nstorm][pitch]: HSUB row JMXB for XFILTER */
    uint8_t *pflag, *hclass;   /* [nstorm][fstride] (staged path) */
    uint8_t *uz;               /* [nstorm][fstride]: UB == 0 after CONTTY (fused path) */

@@ -1299,11 +1299,12 @@ __device__ __forceinline__ void momntm_pt(const SlBasinDev &B, const real *P, co
 // ----------------------------------------------------------- staged kernels
 // The counters feed the checks against the reference's list limits (the
 // 30000 / 8000 entries of smpt2g.f:31-38, 57-62, 78-96); one atomic per wave.
+// Both run for the storms whose own step runs SMPT2G (the flag k_smooth reads).
 __global__ void __launch_bounds__(TX * TY) k_pflag(SlBasinDev B, SlRunDev R, StepArgs A)
 {
    const int i = blockIdx.x * TX + threadIdx.x + 1, j = blockIdx.y * TY + threadIdx.y + 1;
-   const int s = blockIdx.z;
-   if (!storm_step(R, s)) return;
+   const int s = blockIdx.z, step = storm_step(R, s);
+   if (!step || !((int) par(R, s, step)[P_FLAGS] & SF_SMPT)) return;
    const bool in = (i <= B.imxb && j <= B.jmxb);
    const size_t so = (size_t) s * R.fstride;
    uint8_t f = 0;
@@ -1321,8 +1322,8 @@ __global__ void __launch_bounds__(TX * TY) k_pflag(SlBasinDev B, SlRunDev R, Ste
 __global__ void __launch_bounds__(TX * TY) k_hclass(SlBasinDev B, SlRunDev R, StepArgs A)
 {
    const int i = blockIdx.x * TX + threadIdx.x + 1, j = blockIdx.y * TY + threadIdx.y + 1;
-   const int s = blockIdx.z;
-   if (!storm_step(R, s)) return;
+   const int s = blockIdx.z, step = storm_step(R, s);
+   if (!step || !((int) par(R, s, step)[P_FLAGS] & SF_SMPT)) return;
    const bool in = (i <= B.imxb && j <= B.jmxb);
    const size_t so = (size_t) s * R.fstride;
    uint8_t c = 0;
@@ -1831,7 +1832,7 @@ struct Reg {
 template <int TH, bool SMPT, bool FILT>
 __global__ void __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu(sizeof(real) == 8 ? 5 : !FILT ? 8 : 1)))
-k_stepB(SlBasinDev B, SlRunDev R, StepArgs A, int ntx, int nty, int dyn, int isl)
+k_stepB(SlBasinDev B, SlRunDev R, StepArgs A, int ntx, int nty, int dyn, int isl, int post)
 {
    constexpr int TW = 512 / TH;   // tile width: every tile has 512 points
    constexpr int SLO = FILT ? 2 : 1, SHI = FILT ? 2 : 1;
@@ -1876,11 +1877,14 @@ k_stepB(SlBasinDev B, SlRunDev R, StepArgs A, int ntx, int nty, int dyn, int isl
 
    // HB after CONTTY (and the U==0 mask) with halo; the static ZB and ZBM.
    // The loops are unrolled over a compile-time element count so each
-   // thread's loads are in flight together.
+   // thread's loads are in flight together.  With post (basins with 1-D flow
+   // sides or channels) A.hc already holds HB after TMSTPC1: SMPT2G, FLT2G,
+   // BDRYHT and the side passes ran in kernels of their own, and this kernel
+   // starts at HMXSV.
 #pragma unroll
    for (int k = 0; k < (RH::n + 255) / 256; k++) {
       const int e = tid + 256 * k;
-      if (e < RH::n) {
+      if (e < RH::n && !post) {
          const int gi = I0 - RH::lo + e % RH::w, gj = J0 - RH::lo + e / RH::w;
          sh_hs[e] = (gi >= 1 && gi <= IM && gj >= 1 && gj <= JM) ? hs_g[IDX(B, gi, gj)] : real(0.);
       }
@@ -1978,7 +1982,8 @@ k_stepB(SlBasinDev B, SlRunDev R, StepArgs A, int ntx, int nty, int dyn, int isl
       const int gi = I0 - RS::lo + e % RS::w, gj = J0 - RS::lo + e / RS::w;
       real v = real(0.);
       if (gi >= 1 && gi <= IM && gj >= 1 && gj <= JM) {
-         if (gj == JM) v = (gi <= IM - 1) ? hs_g[IDX(B, gi, 1)] : hsL(gi, gj);
+         if (post) v = hs_g[IDX(B, gi, gj)];
+         else if (gj == JM) v = (gi <= IM - 1) ? hs_g[IDX(B, gi, 1)] : hsL(gi, gj);
          else v = smooth_pt(B, P, T, smpt, gi, gj, hsL, clL, zL);
       }
       sh_s[e] = v;
@@ -2044,16 +2049,20 @@ k_stepB(SlBasinDev B, SlRunDev R, StepArgs A, int ntx, int nty, int dyn, int isl
 
 // ------------------------------------------------------------------ launch
 template <int TH>
-static void launch_B(int smpt, int filt, int dyn, dim3 g, hipStream_t st, const SlBasinDev &B,
+static void launch_B(int smpt, int filt, int dyn, int post, dim3 g, hipStream_t st, const SlBasinDev &B,
                      const SlRunDev &R, StepArgs a, int ntx, int nty)
 {
    const int isl = B.island;
    if (isl) filt = 0;   // FLTER2 runs in its own kernel
-   if (dyn || (smpt && filt))
-      hipLaunchKernelGGL((k_stepB<TH, true, true>), g, dim3(256), 0, st, B, R, a, ntx, nty, dyn, isl);
-   else if (smpt) hipLaunchKernelGGL((k_stepB<TH, true, false>), g, dim3(256), 0, st, B, R, a, ntx, nty, 0, isl);
-   else if (filt) hipLaunchKernelGGL((k_stepB<TH, false, true>), g, dim3(256), 0, st, B, R, a, ntx, nty, 0, isl);
-   else hipLaunchKernelGGL((k_stepB<TH, false, false>), g, dim3(256), 0, st, B, R, a, ntx, nty, 0, isl);
+   if (post) {          // from HMXSV on: no SMPT2G regions
+      if (dyn || filt)
+         hipLaunchKernelGGL((k_stepB<TH, false, true>), g, dim3(256), 0, st, B, R, a, ntx, nty, dyn, isl, 1);
+      else hipLaunchKernelGGL((k_stepB<TH, false, false>), g, dim3(256), 0, st, B, R, a, ntx, nty, 0, isl, 1);
+   } else if (dyn || (smpt && filt))
+      hipLaunchKernelGGL((k_stepB<TH, true, true>), g, dim3(256), 0, st, B, R, a, ntx, nty, dyn, isl, 0);
+   else if (smpt) hipLaunchKernelGGL((k_stepB<TH, true, false>), g, dim3(256), 0, st, B, R, a, ntx, nty, 0, isl, 0);
+   else if (filt) hipLaunchKernelGGL((k_stepB<TH, false, true>), g, dim3(256), 0, st, B, R, a, ntx, nty, 0, isl, 0);
+   else hipLaunchKernelGGL((k_stepB<TH, false, false>), g, dim3(256), 0, st, B, R, a, ntx, nty, 0, isl, 0);
 }
 
 // ------------------------------------------------------------------ products
@@ -2119,6 +2128,14 @@ __global__ void k_slot_init(SlBasinDev B, SlRunDev R, int slot, int tstart, int 
    }
    if (p < B.pitch) R.hsubtop[(size_t) slot * B.pitch + p] = real(0.);
    if (p < 8) R.stat[slot * 8 + p] = 0;
+   // 1-D flow state: FLWSQR = 0 (intlht.f:190-191), F1DACT not 'T', Q = 0
+   // (channl.f:73)
+   if (p < B.nsqrwc) {
+      R.flwsqr[(size_t) slot * B.nsqrwc + p] = real(0.);
+      R.f1dact[(size_t) slot * B.nsqrwc + p] = 0;
+   }
+   if (p < B.nsq) R.hsq[(size_t) slot * B.nsq + p] = real(0.);
+   if (p < 5) R.chq[slot * 5 + p] = real(0.);
    if (p == 0) {
       R.evcnt[slot] = 0;
       R.cur[slot] = 0;
@@ -2324,7 +2341,7 @@ extern "C" int SL_ENVS(const SlBasinDev *B, const SlRunDev *R, const double *hx2
 #endif
 extern "C" int SL_SLOTINIT(const SlBasinDev *B, const SlRunDev *R, int slot, int tstart, int nsteps, hipStream_t st)
 {
-   const int n = B->npts > B->pitch ? B->npts : B->pitch;
+   const int n = std::max(std::max(B->npts, B->pitch), std::max(B->nsqrwc, B->nsq));
    hipLaunchKernelGGL(k_slot_init, dim3((n + 255) / 256), dim3(256), 0, st, *B, *R, slot, tstart, nsteps);
    hipError_t e = hipGetLastError();
    return e == hipSuccess ? 0 : -(int) e;
@@ -2407,9 +2424,10 @@ extern "C" int SL_LAUNCH(const SlBasinDev *B, const SlRunDev *R, int smpt, int f
 {
    real *h0 = parity ? R->hb1 : R->hb0, *h1 = parity ? R->hb0 : R->hb1;
    const dim3 gadv((R->nstorm + 255) / 256), badv(256);
-   // basins with 1-D flow sides or channels step on the staged path (their
-   // passes sit between routines the fused kernel runs in one pass)
-   if (path == 0 && (B->nsqrw > 0 || B->f1d_on || B->npss)) return -(int) hipErrorNotSupported;
+   // basins with 1-D flow sides or channels: their passes sit between SMPT2G
+   // and HMXSV, so the fused path runs SMPT2G/FLT2G/BDRYHT into hb2, the side
+   // passes there, and the step kernel from HMXSV on
+   const bool sides = B->nsqrw > 0 || B->f1d_on || B->npss;
    StepArgs a;
 #define MARK(k) \
    if (ev) (void) hipEventRecord(ev[k], st)
@@ -2425,13 +2443,32 @@ extern "C" int SL_LAUNCH(const SlBasinDev *B, const SlRunDev *R, int smpt, int f
       MARK(2);
       a.hc = h1;
       a.hn = h0;
+      if (sides) {
+         // HB after CONTTY (h1) -> hb2: SMPT2G's classes and side pass, FLT2G,
+         // BDRYHT and the row copy; then CHANNL and FLW1DM_C/M on hb2; the
+         // step kernel takes hb2 from HMXSV on into h0
+         const dim3 blk(TX, TY), grd((B->imxb + TX - 1) / TX, (B->jmxb + TY - 1) / TY, R->nstorm);
+         StepArgs c;
+         c.hc = h1;
+         c.hn = R->hb2;
+         if (smpt || dyn) {
+            hipLaunchKernelGGL(k_pflag, grd, blk, 0, st, *B, *R, c);
+            hipLaunchKernelGGL(k_hclass, grd, blk, 0, st, *B, *R, c);
+            if (B->nsqrw > 0)
+               hipLaunchKernelGGL(k_f1d_mark, dim3((R->nstorm + 63) / 64), dim3(64), 0, st, *B, *R, c);
+         }
+         hipLaunchKernelGGL(k_smooth, grd, blk, 0, st, *B, *R, c);
+         if (B->f1d_on || B->npss)
+            hipLaunchKernelGGL(k_sides, dim3((R->nstorm + 63) / 64), dim3(64), 0, st, *B, *R, c);
+         a.hc = R->hb2;
+      }
       const int th = (bth == 8 || bth == 32) ? bth : 16;
       const int nbx = (B->imxb + 512 / th - 1) / (512 / th);
       nty = (B->jmxb + th - 1) / th;
       const dim3 g(nbx * nty * R->nstorm);
-      if (th == 8) launch_B<8>(smpt, filt, dyn, g, st, *B, *R, a, nbx, nty);
-      else if (th == 32) launch_B<32>(smpt, filt, dyn, g, st, *B, *R, a, nbx, nty);
-      else launch_B<16>(smpt, filt, dyn, g, st, *B, *R, a, nbx, nty);
+      if (th == 8) launch_B<8>(smpt, filt, dyn, sides, g, st, *B, *R, a, nbx, nty);
+      else if (th == 32) launch_B<32>(smpt, filt, dyn, sides, g, st, *B, *R, a, nbx, nty);
+      else launch_B<16>(smpt, filt, dyn, sides, g, st, *B, *R, a, nbx, nty);
       if (B->island) {
          // HB after HMXSV is in h0: FLTER2 into h1 and back (with the row
          // copy) for the storms that filter, then MOMNTM and the periodic row

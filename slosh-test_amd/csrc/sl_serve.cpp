@@ -223,11 +223,22 @@ static void worker(slg_server *S)
    }
 }
 
+// basins with 1-D flow sides or channels: per-slot side state and a third HB
+static bool pool_sides(const slg_basin *B)
+{
+   return B->h.nsqrw > 0 || (B->h.nsqrwc > 0 && B->h.nof1d != '+') || B->h.npss > 0;
+}
+
 static size_t slot_bytes(const slg_basin *B, int maxsteps, int tsets_per_slot, int prec)
 {
    const size_t rs = prec == 64 ? 8 : 4, np = (size_t) B->pitch * B->h.jmxb;
-   return np * rs * 5 + np * 3 + (size_t) B->pitch * rs + SL_EVCAP * 4 + 64 + (size_t) maxsteps * SL_NPAR * rs +
-          (size_t) tsets_per_slot * SL_NTABV * 8 * rs + 4096;
+   const size_t nl = (size_t) std::max(B->h.nsqrwc, 1);
+   // fields (a third HB with sides), masks, the HSUB top row, the event list,
+   // counters, schedule and tables, FLWSQR + F1DACT, the side squares' HSUB
+   // (at most two per side), the channel flows
+   return np * rs * (pool_sides(B) ? 6 : 5) + np * 3 + (size_t) B->pitch * rs + SL_EVCAP * 4 + 64 +
+          (size_t) maxsteps * SL_NPAR * rs + (size_t) tsets_per_slot * SL_NTABV * 8 * rs + nl * (rs + 1) +
+          2 * nl * rs + 5 * rs + 4096;
 }
 
 template <class R>
@@ -242,7 +253,7 @@ static int pool_alloc(Pool &P, char *err, int errlen)
    const slg_basin *B = P.basin;
    const int n = P.slots;
    const size_t np = (size_t) B->pitch * B->h.jmxb, ns = np * n, rs = sizeof(R);
-   size_t off[20], tot = 0;
+   size_t off[32], tot = 0;
    int k = 0;
    auto add = [&](size_t bytes) {
       off[k++] = tot;
@@ -260,6 +271,14 @@ static int pool_alloc(Pool &P, char *err, int errlen)
    add((size_t) n * 4);                                                   // nsteps
    add((size_t) n * 4);                                                   // cur
    add((size_t) B->h.imxb * B->h.jmxb * 2);                               // one envelope
+   const BasinImg<R> &I = img(const_cast<slg_basin *>(B), R());
+   const size_t nl = (size_t) std::max(B->h.nsqrwc, 1), nq = (size_t) std::max(I.nsq, 1);
+   add((size_t) n * nl * rs);                                             // flwsqr
+   add((size_t) n * nl);                                                  // f1dact
+   add((size_t) n * nq * rs);                                             // hsq
+   add((size_t) n * 5 * rs);                                              // chq
+   const bool sides = pool_sides(B);
+   add(sides ? ns * rs : 0);                                              // hb2
    char *d = nullptr;
    HIPOK(hipMalloc((void **) &d, tot));
    HIPOK(hipMemset(d, 0, tot));   // nsteps = 0: every slot empty
@@ -291,6 +310,11 @@ static int pool_alloc(Pool &P, char *err, int errlen)
    D.vbudget = 0;
    D.sgroup = SL_SGROUP;
    P.denv = (short *) (d + off[17]);
+   D.flwsqr = (R *) (d + off[18]);
+   D.f1dact = (uint8_t *) (d + off[19]);
+   D.hsq = (R *) (d + off[20]);
+   D.chq = (R *) (d + off[21]);
+   D.hb2 = sides ? (R *) (d + off[22]) : nullptr;
    HIPOK(hipStreamCreate(&P.st));
    for (auto &e : P.ev_round) HIPOK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
    for (int c = 0; c < 3; c++) {
@@ -337,11 +361,6 @@ extern "C" int slg_server_add_basin(slg_server *S, const char *dta, const char *
    slg_basin *B = nullptr;
    int rc = slg_basin_open(dta, bnt_line, &B, err, errlen);
    if (rc) return rc;
-   if (B->h.nsqrw > 0 || (B->h.nsqrwc > 0 && B->h.nof1d != '+') || B->h.npss > 0) {
-      slg_basin_close(B);
-      seterr(err, errlen, "%s", "basins with 1-D flow sides or channels are stepped through the batch API, not served");
-      return SLG_EUNSUP;
-   }
    rc = sl_basin_ensure(B, S->prec, err, errlen);
    if (rc) {
       slg_basin_close(B);

@@ -59,18 +59,16 @@ def test_host_state_without_gpu(tmp_path):
     assert nsteps == rows.shape[0] - 1 and len(tabs) >= 1
 
 
-def test_sides_basin_steps_on_staged_path(tmp_path):
-    """A basin with sub-grid 1-D flow sides opens with its sides counted, starts
-    on the staged path and refuses the fused path and hipGraphs (no GPU needed)."""
+def test_sides_basin_takes_every_path(tmp_path):
+    """A basin with sub-grid 1-D flow sides opens with its sides counted and
+    accepts both launch paths and hipGraphs (no GPU needed)."""
     import helpers
     case = helpers.edge_case(str(tmp_path), 0)
     B = sloshgpu.Basin(case["dta"], case["bntline"])
     assert B.nsides >= 15
     X = sloshgpu.Batch(B, [case["trk"]])
-    with pytest.raises(sloshgpu.SloshError):
-        X.set_path(sloshgpu.PATH_FUSED, 16)
-    with pytest.raises(sloshgpu.SloshError):
-        X.set_graph(16)
+    X.set_path(sloshgpu.PATH_FUSED, 16)
+    X.set_graph(16)
     X.set_path(sloshgpu.PATH_STAGED, 16)
     X.close()
     B.close()

@@ -159,26 +159,33 @@ def test_fused_equals_staged_ensemble(tmp_path):
 
 
 # ------------------------------------------------- sub-grid 1-D flow sides
-# Basins with 1-D flow sides, banks, raised weirs and cuts step on the staged
-# path with SMPT2G's side pass and FLW1DM_C / FLW1DM_M between its routines.
+# Basins with 1-D flow sides, banks, raised weirs and cuts: SMPT2G's side pass
+# and CHANNL / FLW1DM_C / FLW1DM_M between its routines, on the staged path and
+# on the fused one (SMPT2G, FLT2G and BDRYHT into a third HB buffer, the side
+# passes there, the step kernel from HMXSV on).
 
+SIDE_PATHS = [(sloshgpu.PATH_STAGED, 16), (sloshgpu.PATH_FUSED, 16), (sloshgpu.PATH_FUSED, 32)]
+
+
+@pytest.mark.parametrize("path", SIDE_PATHS, ids=["staged", "fused16", "fused32"])
 @pytest.mark.parametrize("idx", range(len(helpers.EDGE_VARIANTS)), ids=helpers.edge_id)
-def test_sides_bitwise(idx, tmp_path):
+def test_sides_bitwise(idx, path, tmp_path):
     case = helpers.edge_case(str(tmp_path), idx)
-    print("sides", helpers.edge_id(idx), flush=True)
-    out, _ = helpers.gpu_vs_oracle(case, 480, [1, 5, 31, 100, 200, 400], path=sloshgpu.PATH_STAGED)
+    print("sides", helpers.edge_id(idx), path, flush=True)
+    out, _ = helpers.gpu_vs_oracle(case, 480, [1, 5, 31, 100, 200, 400], path=path[0], tile_rows=path[1])
     assert all("FLWSQR" in f for f in out.values())
     _assert_identical(out, helpers.edge_id(idx))
 
 
+@pytest.mark.parametrize("path", SIDE_PATHS[:2], ids=["staged", "fused16"])
 @pytest.mark.parametrize("idx", range(len(helpers.EDGE_VARIANTS)), ids=helpers.edge_id)
-def test_fp64_sides_bitwise(idx, tmp_path):
+def test_fp64_sides_bitwise(idx, path, tmp_path):
     """FP64: bit for bit without channels.  CHANNL's fractional powers use the
     device pow, which can differ from the host libm's in the last bit, so with
     channels the FP64 step is held to 1e-6 ft / 1e-6 relative instead."""
     case = helpers.edge_case(str(tmp_path), idx)
-    print("fp64 sides", helpers.edge_id(idx), flush=True)
-    out, _ = helpers.gpu_vs_oracle(case, 480, [1, 31, 200, 400], path=sloshgpu.PATH_STAGED, precision=64)
+    print("fp64 sides", helpers.edge_id(idx), path, flush=True)
+    out, _ = helpers.gpu_vs_oracle(case, 480, [1, 31, 200, 400], path=path[0], tile_rows=path[1], precision=64)
     if not helpers.EDGE_VARIANTS[idx][5]:
         _assert_identical(out, helpers.edge_id(idx))
         return
@@ -441,3 +448,47 @@ def test_manning_bitwise(idx, precision, tmp_path):
     print("manning", helpers.manning_id(idx), "fp%d" % precision, flush=True)
     out, _ = helpers.gpu_vs_oracle(case, 300, [1, 20, 100, 200], precision=precision)
     _assert_identical(out, helpers.manning_id(idx))
+
+
+@pytest.mark.parametrize("idx", [1, 3, 5], ids=lambda i: helpers.edge_id(i))
+def test_sides_graph_and_batch_equal_alone(idx, tmp_path):
+    """Side basins batched on the fused path with runs of different lengths,
+    replayed as hipGraphs (each storm's own SMPT2G and FILTER steps), and split
+    over streams: every storm's fields and 1-D transports equal the same storm
+    run alone on the staged path."""
+    b, t = helpers.edge_basin(idx, hours=9)
+    root = str(tmp_path)
+    case = helpers.make_case(root, b, t)
+    rng = np.random.default_rng(300 + idx)
+    trks = [case["trk"]]
+    for k in range(5):
+        p = "%s/e%d.trk" % (root, k)
+        with open(p, "w") as fh:
+            fh.write(synth.trk_text(synth.random_track(b, rng, hours_range=(8, 10))))
+        trks.append(p)
+    B = sloshgpu.Basin(case["dta"], case["bntline"])
+    runs = []
+    for graph, streams in ((8, 1), (0, 2)):
+        X = sloshgpu.Batch(B, trks)
+        X.upload()
+        X.set_path(sloshgpu.PATH_FUSED, 16)
+        if streams > 1:
+            X.set_streams(streams)
+        X.set_graph(graph)
+        X.step(max(X.nsteps))
+        X.sync()
+        runs.append(([{f: X.get(s, f) for f in helpers.FIELDS} for s in range(len(trks))],
+                     [X.flwsqr(s) for s in range(len(trks))]))
+        X.close()
+    for s, p in enumerate(trks):
+        A = sloshgpu.Batch(B, [p])
+        A.set_path(sloshgpu.PATH_STAGED, 16)
+        A.upload()
+        A.step(A.nsteps[0])
+        A.sync()
+        for fields, flw in runs:
+            for f in helpers.FIELDS:
+                assert np.array_equal(A.get(0, f), fields[s][f]), (s, f)
+            assert np.array_equal(A.flwsqr(0), flw[s]), (s, "FLWSQR")
+        A.close()
+    B.close()

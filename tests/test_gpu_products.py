@@ -126,3 +126,44 @@ def test_served_run_same_alone_and_in_mix(precision, tmp_path):
     X.close()
     B.close()
     assert np.array_equal(env[0], alone)
+
+
+@pytest.mark.parametrize("precision", [32, 64])
+def test_served_side_basin_runs_equal_staged_alone(precision, tmp_path):
+    """A basin with 1-D flow sides and channels is served beside an ordinary
+    basin (slots fewer than runs, arrivals spread over the pumps): every
+    served envelope equals the same storm run alone on the staged path."""
+    root = str(tmp_path)
+    bs, ts = helpers.edge_basin(5, hours=6)     # sides with NOF1D '+' and a channel
+    bs.name = "tsd"
+    be, te = helpers.edge_basin(3, hours=6)     # sides, banks, weirs, cuts, 3 channels
+    be.name = "tse"
+    cs = synth.write_case(os.path.join(root, "bs"), bs, ts)
+    ce = synth.write_case(os.path.join(root, "be"), be, te)
+    trs = _tracks(bs, root, 5, seed=11)
+    tre = _tracks(be, root, 6, seed=12)
+    print("served sides precision", precision, flush=True)
+    S = sloshgpu.Server(precision=precision)
+    b_s = S.add_basin(cs["dta"], cs["bntline"], max_slots=3, max_steps=800)
+    b_e = S.add_basin(ce["dta"], ce["bntline"], max_slots=3, max_steps=800)
+    arrivals = [(b_s, p, cs) for p in trs] + [(b_e, p, ce) for p in tre]
+    rng = np.random.default_rng(13)
+    rng.shuffle(arrivals)
+    pending, ids = list(arrivals), []
+    while pending or any(S.result(r, bb)[0] != 2 for r, bb, _, _ in ids):
+        for _ in range(min(len(pending), 2)):
+            bb, p, c = pending.pop()
+            ids.append((S.submit(bb, p), bb, p, c))
+        S.pump(29)
+    served = [(S.result(r, bb)[1], p, c) for r, bb, p, c in ids]
+    S.close()
+    for env, p, c in served:
+        B = sloshgpu.Basin(c["dta"], c["bntline"])
+        X = sloshgpu.Batch(B, [p], precision=precision)
+        X.set_path(sloshgpu.PATH_STAGED, 16)
+        X.upload()
+        X.step(X.nsteps[0])
+        alone, _ = X.envelopes(want_mom=False)
+        X.close()
+        B.close()
+        assert np.array_equal(env, alone[0]), p
