@@ -195,6 +195,13 @@ int slg_batch_rex_frames(slg_batch *x, const char *field, int s0, int ns, unsign
 int slg_batch_set_maxstat(slg_batch *x, int on);
 int slg_batch_maxstat_reset(slg_batch *x);
 int slg_batch_maxstat_env(slg_batch *x, short *env, char *err, int errlen);
+/* The second window of the -envSave2 mode (hb_max2, slosh2.c:1264-1311): the
+ * same maxima over a window of its own, which PerformRun restarts only when
+ * an envSave2 save falls due (MaxStatInitGrid flag 3, slosh2.c:2514-2520) and
+ * saves to the envDir2 files (slosh2.c:2477-2508, 2639-2668). */
+int slg_batch_set_maxstat2(slg_batch *x, int on);
+int slg_batch_maxstat2_reset(slg_batch *x);
+int slg_batch_maxstat2_env(slg_batch *x, short *env, char *err, int errlen);
 /* The file around the frames, laid out as the reference's rex writer does:
  * open = RexOpen (version 2) + RexSaveHeader (rex.c:150-180, 264-379); frame =
  * RexSaveStep's record (rex.c:387-447: lat, lon, wind speed, wind direction,
@@ -241,6 +248,13 @@ int slg_server_add_basin(slg_server *sv, const char *dta_path, const char *bnt_l
                          int max_steps, int tables_per_slot, int *basin_id, char *err, int errlen);
 int slg_server_slots(const slg_server *sv, int basin_id);
 int slg_server_submit(slg_server *sv, int basin_id, const char *trk_path, int *run_id, char *err, int errlen);
+/* Admission order of the queued runs: SLG_ORDER_FIFO (submission order, the
+ * default) or SLG_ORDER_HARDEST (the most work first, steps x cells, as the
+ * reference's job list is sorted for its worker queue: dev/b1.sortJobList.sh:
+ * 20-73 with leader.c:48-71 handing idle workers the sorted list). */
+#define SLG_ORDER_FIFO 0
+#define SLG_ORDER_HARDEST 1
+int slg_server_set_order(slg_server *sv, int order);
 /* Returns the number of runs retired this round (>= 0) or an error code.  The
  * host waits for no stream: admissions and envelopes pass through pinned
  * staging behind events, and a pool is held to two pumps of queued steps. */

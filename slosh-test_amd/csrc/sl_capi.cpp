@@ -810,9 +810,10 @@ static int launch_split(slg_batch *X, const SlBasinDevT<R> &B, const SlRunDevT<R
       if (p && first && hipStreamWaitEvent(st, prev[2], 0) != hipSuccess) return -(int) hipErrorUnknown;
       r = dev_launch(&B, &V, smpt, filt, dyn, X->parity, X->path, X->tile_rows, st, evp);
       if (r < 0) return r;
-      if (X->hbmax) {
-         const int m = dev_maxstat(&B, &V, r ? V.hb1 : V.hb0, img(X->b, R()).zbd,
-                                   X->hbmax + (size_t) s0 * D.fstride, 0, st);
+      if (X->hbmax || X->hbmax2) {
+         const size_t o = (size_t) s0 * D.fstride;
+         const int m = dev_maxstat(&B, &V, r ? V.hb1 : V.hb0, img(X->b, R()).zbd, X->hbmax ? X->hbmax + o : nullptr,
+                                   X->hbmax2 ? X->hbmax2 + o : nullptr, 0, st);
          if (m < 0) return m;
       }
       prev = evp;
@@ -826,8 +827,8 @@ static int launch_whole(slg_batch *X, const SlBasinDevT<R> &B, const SlRunDevT<R
                         hipEvent_t *ev, int dyn)
 {
    const int r = dev_launch(&B, &D, smpt, filt, dyn, X->parity, X->path, X->tile_rows, X->st, ev);
-   if (r < 0 || !X->hbmax) return r;
-   const int m = dev_maxstat(&B, &D, r ? D.hb1 : D.hb0, img(X->b, R()).zbd, X->hbmax, 0, X->st);
+   if (r < 0 || !(X->hbmax || X->hbmax2)) return r;
+   const int m = dev_maxstat(&B, &D, r ? D.hb1 : D.hb0, img(X->b, R()).zbd, X->hbmax, X->hbmax2, 0, X->st);
    return m < 0 ? m : r;
 }
 
@@ -1612,46 +1613,67 @@ extern "C" int slg_batch_rex_points(slg_batch *X, const char *field, int s0, int
 
 // ---- P-Surge windowed maxima (slosh2.c:1227-1316) --------------------------
 template <class R>
-static int maxstat_init(slg_batch *X)
+static int maxstat_init(slg_batch *X, double *h)
 {
-   int r = dev_maxstat(&img(X->b, R()).dev, &run(X, R()), (const R *) nullptr, img(X->b, R()).zbd, X->hbmax, 1,
+   int r = dev_maxstat(&img(X->b, R()).dev, &run(X, R()), (const R *) nullptr, img(X->b, R()).zbd, h, nullptr, 1,
                        X->st);
    return (r || hipStreamSynchronize(X->st) != hipSuccess) ? SLG_EHIP : 0;
 }
 
-extern "C" int slg_batch_maxstat_reset(slg_batch *X)
+static int maxstat_reset(slg_batch *X, double *h)
 {
-   if (!X->dmem || !X->hbmax) return SLG_EINPUT;
-   return X->prec == 64 ? maxstat_init<double>(X) : maxstat_init<float>(X);
+   if (!X->dmem || !h) return SLG_EINPUT;
+   return X->prec == 64 ? maxstat_init<double>(X, h) : maxstat_init<float>(X, h);
 }
 
-extern "C" int slg_batch_set_maxstat(slg_batch *X, int on)
+extern "C" int slg_batch_maxstat_reset(slg_batch *X)
+{
+   return maxstat_reset(X, X->hbmax);
+}
+
+extern "C" int slg_batch_maxstat2_reset(slg_batch *X)
+{
+   return maxstat_reset(X, X->hbmax2);
+}
+
+// one window's buffer on (allocated and started) or off (freed)
+static int maxstat_switch(slg_batch *X, double **h, int on)
 {
    if (!X->dmem) return SLG_EINPUT;
    if (hipStreamSynchronize(X->st) != hipSuccess) return SLG_EHIP;
    drop_graph(X);   // captured steps hold the update or not
    if (!on) {
-      if (X->hbmax) (void) hipFree(X->hbmax);
-      X->hbmax = nullptr;
+      if (*h) (void) hipFree(*h);
+      *h = nullptr;
       return 0;
    }
    const size_t fs = X->prec == 64 ? X->d64.fstride : X->d32.fstride;
-   if (!X->hbmax && hipMalloc((void **) &X->hbmax, sizeof(double) * fs * X->nstorm) != hipSuccess) {
-      X->hbmax = nullptr;
+   if (!*h && hipMalloc((void **) h, sizeof(double) * fs * X->nstorm) != hipSuccess) {
+      *h = nullptr;
       return SLG_EHIP;
    }
-   return slg_batch_maxstat_reset(X);
+   return maxstat_reset(X, *h);
+}
+
+extern "C" int slg_batch_set_maxstat(slg_batch *X, int on)
+{
+   return maxstat_switch(X, &X->hbmax, on);
+}
+
+extern "C" int slg_batch_set_maxstat2(slg_batch *X, int on)
+{
+   return maxstat_switch(X, &X->hbmax2, on);
 }
 
 template <class R>
-static int maxstat_env(slg_batch *X, short *env, char *err, int errlen)
+static int maxstat_env(slg_batch *X, const double *h, short *env, char *err, int errlen)
 {
    const sl_basin *b = &X->b->h;
    const size_t n = (size_t) b->imxb * b->jmxb;
    short *denv = nullptr;
    HIPOK(hipMalloc((void **) &denv, n * X->nstorm * 2));
    int rc = dev_envelopes(&img(X->b, R()).dev, &run(X, R()), img(X->b, R()).zbd, 0, X->nstorm, denv, nullptr,
-                          X->st, X->hbmax);
+                          X->st, h);
    if (rc == 0 && (hipMemcpyAsync(env, denv, n * X->nstorm * 2, hipMemcpyDeviceToHost, X->st) != hipSuccess ||
                    hipStreamSynchronize(X->st) != hipSuccess))
       rc = -1;
@@ -1663,15 +1685,25 @@ static int maxstat_env(slg_batch *X, short *env, char *err, int errlen)
    return 0;
 }
 
-extern "C" int slg_batch_maxstat_env(slg_batch *X, short *env, char *err, int errlen)
+static int maxstat_env_any(slg_batch *X, const double *h, short *env, char *err, int errlen)
 {
-   if (!X->dmem || !X->hbmax || !env) {
-      seterr(err, errlen, "%s", "windowed maxima are off (slg_batch_set_maxstat) or no buffer");
+   if (!X->dmem || !h || !env) {
+      seterr(err, errlen, "%s", "windowed maxima are off (slg_batch_set_maxstat / _maxstat2) or no buffer");
       return SLG_EINPUT;
    }
    const int rc = check_runs_any(X, 0, X->nstorm, err, errlen);
    if (rc) return rc;
-   return X->prec == 64 ? maxstat_env<double>(X, env, err, errlen) : maxstat_env<float>(X, env, err, errlen);
+   return X->prec == 64 ? maxstat_env<double>(X, h, env, err, errlen) : maxstat_env<float>(X, h, env, err, errlen);
+}
+
+extern "C" int slg_batch_maxstat_env(slg_batch *X, short *env, char *err, int errlen)
+{
+   return maxstat_env_any(X, X->hbmax, env, err, errlen);
+}
+
+extern "C" int slg_batch_maxstat2_env(slg_batch *X, short *env, char *err, int errlen)
+{
+   return maxstat_env_any(X, X->hbmax2, env, err, errlen);
 }
 
 // Datums of a run's track for the env trailer (ReadTrkFile, slosh2.c:1885-1975):
@@ -1788,6 +1820,7 @@ extern "C" void slg_batch_close(slg_batch *X)
    drop_graph(X);
    rex_free(X);
    if (X->hbmax) (void) hipFree(X->hbmax);
+   if (X->hbmax2) (void) hipFree(X->hbmax2);
    if (X->pt_buf) (void) hipFree(X->pt_buf);
    for (int p = 0; p < SL_MAXSPLIT - 1; p++)
       if (X->stx[p]) {

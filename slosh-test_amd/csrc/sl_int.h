@@ -111,6 +111,7 @@ struct slg_batch {
    // Part 0 runs on st, part p > 0 on stx[p - 1].
    int nsplit = 1;
    double *hbmax = nullptr;   // P-Surge windowed maxima, [nstorm][fstride], when on
+   double *hbmax2 = nullptr;  // their second window (-envSave2), when on
    // rex frame buffers, kept from one slg_batch_rex_frames call to the next:
    // scratch, frames, packed frames (device), sizes and offsets, and pinned
    // host staging for the packed copy; for rex_chunk storms of rex_cap bytes
@@ -150,18 +151,18 @@ static inline int dev_envelopes(const SlBasinDevT<double> *B, const SlRunDevT<do
    return sl_dev_envelopes_f64(B, R, hx2, zbd, s0, ns, env, mom, acc, st);
 }
 extern "C" int sl_dev_maxstat_f32(const SlBasinDevT<float> *B, const SlRunDevT<float> *R, const float *hb,
-                                  const double *zbd, double *hmax, int init, hipStream_t st);
+                                  const double *zbd, double *hmax, double *hmax2, int init, hipStream_t st);
 extern "C" int sl_dev_maxstat_f64(const SlBasinDevT<double> *B, const SlRunDevT<double> *R, const double *hb,
-                                  const double *zbd, double *hmax, int init, hipStream_t st);
+                                  const double *zbd, double *hmax, double *hmax2, int init, hipStream_t st);
 static inline int dev_maxstat(const SlBasinDevT<float> *B, const SlRunDevT<float> *R, const float *hb,
-                              const double *zbd, double *hmax, int init, hipStream_t st)
+                              const double *zbd, double *hmax, double *hmax2, int init, hipStream_t st)
 {
-   return sl_dev_maxstat_f32(B, R, hb, zbd, hmax, init, st);
+   return sl_dev_maxstat_f32(B, R, hb, zbd, hmax, hmax2, init, st);
 }
 static inline int dev_maxstat(const SlBasinDevT<double> *B, const SlRunDevT<double> *R, const double *hb,
-                              const double *zbd, double *hmax, int init, hipStream_t st)
+                              const double *zbd, double *hmax, double *hmax2, int init, hipStream_t st)
 {
-   return sl_dev_maxstat_f64(B, R, hb, zbd, hmax, init, st);
+   return sl_dev_maxstat_f64(B, R, hb, zbd, hmax, hmax2, init, st);
 }
 extern "C" int sl_dev_rex_f32(const SlBasinDevT<float> *B, const SlRunDevT<float> *R, const float *hbs,
                               const double *hmax, const double *zbd, int s0, int ns, unsigned char *out, long cap,

@@ -2283,20 +2283,25 @@ __global__ __launch_bounds__(RX_T) void k_rex(SlBasinDev B, SlRunDev R, const re
 // the C driver holds them, on the squares (I <= IMXB-1, J <= JMXB-1):
 // MaxStatInitGrid sets -ZB, MaxStatUpdateGrid takes the maximum with HB after
 // each step.
-__global__ void k_maxstat(SlBasinDev B, SlRunDev R, const real *hb, const double *zbd, double *hmax, int init)
+__global__ void k_maxstat(SlBasinDev B, SlRunDev R, const real *hb, const double *zbd, double *hmax,
+                          double *hmax2, int init)
 {
    const int s = blockIdx.y;
    const int i = (blockIdx.x % ((B.imxb + 63) / 64)) * 64 + threadIdx.x + 1;
    const int j = blockIdx.x / ((B.imxb + 63) / 64) + 1;
    if (s >= R.nstorm || i > B.imxb - 1 || j > B.jmxb - 1) return;
    const uint32_t p = IDX(B, i, j);
-   double *m = hmax + (size_t) s * R.fstride + p;
+   const size_t q = (size_t) s * R.fstride + p;
    if (init) {
-      *m = -1 * zbd[p];
+      if (hmax) hmax[q] = -1 * zbd[p];
+      if (hmax2) hmax2[q] = -1 * zbd[p];
       return;
    }
-   const double h = (double) hb[(size_t) s * R.fstride + p];
-   if (*m < h) *m = h;
+   // both windows take the same HB (MaxStatUpdateGrid with f_max2,
+   // slosh2.c:1295-1306)
+   const double h = (double) hb[q];
+   if (hmax && hmax[q] < h) hmax[q] = h;
+   if (hmax2 && hmax2[q] < h) hmax2[q] = h;
 }
 
 // frame k (nb[k] bytes at src + k*cap) to dst + off[k]: the frames of a chunk
@@ -2398,12 +2403,14 @@ extern "C" int SL_REXPTS(const SlBasinDev *B, const SlRunDev *R, const real *hbs
 #else
 #define SL_MAXSTAT sl_dev_maxstat_f32
 #endif
-// windowed maxima of the storms of R: init (-ZB) or the maximum with hb
+// windowed maxima of the storms of R (either buffer may be null): init (-ZB)
+// or the maximum with hb
 extern "C" int SL_MAXSTAT(const SlBasinDev *B, const SlRunDev *R, const real *hb, const double *zbd, double *hmax,
-                          int init, hipStream_t st)
+                          double *hmax2, int init, hipStream_t st)
 {
    const int nx = (B->imxb + 63) / 64;
-   hipLaunchKernelGGL(k_maxstat, dim3(nx * (B->jmxb - 1), R->nstorm), dim3(64), 0, st, *B, *R, hb, zbd, hmax, init);
+   hipLaunchKernelGGL(k_maxstat, dim3(nx * (B->jmxb - 1), R->nstorm), dim3(64), 0, st, *B, *R, hb, zbd, hmax,
+                      hmax2, init);
    hipError_t e = hipGetLastError();
    return e == hipSuccess ? 0 : -(int) e;
 }

@@ -176,6 +176,7 @@ struct slg_server {
    std::vector<std::unique_ptr<Pool>> pools;
    std::vector<std::unique_ptr<Run>> runs;
    std::deque<int> queue;      // submitted, not yet admitted (submission order)
+   int order = SLG_ORDER_FIFO; // admission order of the queued runs
    std::vector<int> finishing; // retired runs whose envelope is still being copied back
    Ring ring;
    long long cell_steps = 0;
@@ -599,6 +600,13 @@ static void settle_tile(Pool &P)
 // queued, admit queued runs into free slots, and queue up to k further steps
 // on every pool that has runs.  Returns the number of runs retired, or an
 // error code.
+extern "C" int slg_server_set_order(slg_server *S, int order)
+{
+   if (order != SLG_ORDER_FIFO && order != SLG_ORDER_HARDEST) return SLG_EINPUT;
+   S->order = order;
+   return 0;
+}
+
 extern "C" int slg_server_pump(slg_server *S, int k, char *err, int errlen)
 {
    if (k < 0) {
@@ -618,8 +626,25 @@ extern "C" int slg_server_pump(slg_server *S, int k, char *err, int errlen)
             if (rc == 0) retired++;
          }
    }
-   // admission: FIFO per basin; a run that does not fit waits, later runs of
-   // other basins may go ahead of it
+   // admission: in submission order, or hardest first: the prepared runs by
+   // their work (steps x cells, the measure of the reference's job sort,
+   // dev/b1.sortJobList.sh:20-73), largest first, ties in submission order.
+   // A run that does not fit waits; later runs may go ahead of it.
+   if (S->order == SLG_ORDER_HARDEST) {
+      auto work = [&](int id) -> double {
+         const Run &r = *S->runs[id];
+         if (r.state != 0) return -1.;   // not prepared yet: after the prepared ones
+         return (double) r.sched.nsteps * (double) (r.bas->h.imxb - 1) * (double) (r.bas->h.jmxb - 1);
+      };
+      std::vector<std::pair<double, int>> key;
+      key.reserve(S->queue.size());
+      for (int id : S->queue) key.push_back({work(id), id});
+      std::stable_sort(key.begin(), key.end(), [](const std::pair<double, int> &a, const std::pair<double, int> &b) {
+         return a.first > b.first;
+      });
+      S->queue.clear();
+      for (auto &k2 : key) S->queue.push_back(k2.second);
+   }
    std::deque<int> wait;
    while (!S->queue.empty()) {
       const int id = S->queue.front();

@@ -59,6 +59,9 @@ def lib():
         L.slg_batch_set_maxstat.argtypes = [vp, i]
         L.slg_batch_maxstat_reset.argtypes = [vp]
         L.slg_batch_maxstat_env.argtypes = [vp, vp, cp, i]
+        L.slg_batch_set_maxstat2.argtypes = [vp, i]
+        L.slg_batch_maxstat2_reset.argtypes = [vp]
+        L.slg_batch_maxstat2_env.argtypes = [vp, vp, cp, i]
         L.slg_batch_start_date.argtypes = [vp, i, vp]
         L.slg_rex_open.argtypes = [cp, cp, cp, i, i, ctypes.POINTER(vp)]
         L.slg_rex_frame.argtypes = [vp, vp, ctypes.c_double, vp, i]
@@ -80,6 +83,7 @@ def lib():
         L.slg_server_stats.argtypes = [vp, vp]
         L.slg_server_run_error.argtypes = [vp, i, cp, i]
         L.slg_server_tiles.argtypes = [vp, vp, i]
+        L.slg_server_set_order.argtypes = [vp, i]
         L.slg_batch_set_replay_limits.argtypes = [vp, i, ctypes.c_long]
         L.slg_batch_set_storm_group.argtypes = [vp, i]
         L.slg_batch_storm_group.argtypes = [vp]
@@ -284,6 +288,22 @@ class Batch:
         _chk(lib().slg_batch_maxstat_env(self.h, env.ctypes.data_as(ctypes.c_void_p), err, 512), err)
         return env
 
+    def set_maxstat2(self, on=True):
+        """Keep the second window of maxima (the -envSave2 mode)."""
+        if lib().slg_batch_set_maxstat2(self.h, 1 if on else 0):
+            raise SloshError("cannot switch the second window (upload first)")
+
+    def maxstat2_reset(self):
+        if lib().slg_batch_maxstat2_reset(self.h):
+            raise SloshError("the second window is off")
+
+    def maxstat2_env(self):
+        """EnvSave values of every run's second-window maxima, [nstorm, jmxb, imxb]."""
+        env = np.zeros((self.n, self.basin.jmxb, self.basin.imxb), dtype=np.int16)
+        err = ctypes.create_string_buffer(512)
+        _chk(lib().slg_batch_maxstat2_env(self.h, env.ctypes.data_as(ctypes.c_void_p), err, 512), err)
+        return env
+
     def rex_frames(self, field="HHB", s0=0, ns=None):
         """rex v2 frame bit streams of storms s0..s0+ns-1, encoded on the
         device from HHB = min(HB, HMX) of the last step or from HMX; a list of
@@ -409,6 +429,11 @@ class Server:
 
     def slots(self, basin_id: int) -> int:
         return lib().slg_server_slots(self.h, basin_id)
+
+    def set_order(self, hardest_first: bool):
+        """Admit queued runs in submission order, or the most work first."""
+        if lib().slg_server_set_order(self.h, 1 if hardest_first else 0):
+            raise SloshError("bad admission order")
 
     def submit(self, basin_id: int, trk: str) -> int:
         rid = ctypes.c_int()

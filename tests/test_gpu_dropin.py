@@ -116,6 +116,34 @@ def test_gpu_cli_fp64_f_stat_matches_reference_cli(tmp_path):
     assert ra == rb
 
 
+def test_gpu_cli_fp64_envsave2_matches_reference_cli(tmp_path):
+    """-f_stat with -envSave2 60 (slosh2.c:2477-2520, 2639-2668): the envDir
+    and envDir2 files of the drop-in equal the reference CLI's."""
+    import subprocess
+    b, t = synth.tiny(seed=6, ni=30, nj=26, hours=6)
+    outs = {}
+    for tag, exe in (("gpu64", helpers.GPU_CLI), ("ref", helpers.REF_CLI)):
+        root = str(tmp_path / tag)
+        bsn = synth.write_case(root, b, t)["basin"]
+        for d in ("e1", "e2"):
+            os.makedirs(os.path.join(root, d, b.name.lower()))
+        env = dict(os.environ, SLG_PRECISION="64")
+        r = subprocess.run([exe, "-basin", bsn, "-rootDir", root, "-trk", os.path.join(root, "storm.trk"),
+                            "-env", "yes", "-envDir", os.path.join(root, "e1"), "-envDir2", os.path.join(root, "e2"),
+                            "-envSave2", "60", "-verbose", "0", "-f_stat", "1"], cwd=root, capture_output=True,
+                           text=True, timeout=600, env=env, preexec_fn=helpers._big_stack)
+        assert r.returncode == 0, (tag, r.stdout[-500:], r.stderr[-500:])
+        outs[tag] = root
+    for d in ("e1", "e2"):
+        rg, rr = (os.path.join(outs[k], d, b.name.lower()) for k in ("gpu64", "ref"))
+        fg, fr = sorted(os.listdir(rg)), sorted(os.listdir(rr))
+        assert fg == fr and len(fr) >= 5, (d, fg, fr)
+        for f in fr:
+            a, c = helpers.read_env(os.path.join(rg, f)), helpers.read_env(os.path.join(rr, f))
+            assert a[:3] == c[:3] and a[4] == c[4], (d, f)
+            assert np.array_equal(a[3], c[3]), (d, f)
+
+
 def test_gpu_list_run_matches_reference(tmp_path):
     """A list run (-lst, DoStormListV0, cstart.c:92-245; two storms on two
     basins in one process, so the library is initialised and cleaned up twice):

@@ -167,3 +167,40 @@ def test_served_side_basin_runs_equal_staged_alone(precision, tmp_path):
         X.close()
         B.close()
         assert np.array_equal(env, alone[0]), p
+
+
+def test_server_hardest_first_admission(tmp_path):
+    """With the hardest-first order and one slot, queued runs are admitted by
+    their work, largest first (the reference's job sort), and each run's
+    envelope is the one the default submission order gives."""
+    import time
+    root = str(tmp_path)
+    b, t = synth.tiny(seed=31, ni=30, nj=26, hours=6)
+    case = synth.write_case(os.path.join(root, "b"), b, t)
+    hours = [6, 12, 9, 7]
+    paths = []
+    for k, h in enumerate(hours):
+        tr = synth.straight_track(b, 10.0 + k, 300.0 + 10 * k, 60.0, 20.0, hours=h, landfall=h // 2)
+        p = os.path.join(root, "h%02d.trk" % h)
+        open(p, "w").write(synth.trk_text(tr))
+        paths.append(p)
+    envs = []
+    for hardest in (False, True):
+        S = sloshgpu.Server(precision=64)
+        bid = S.add_basin(case["dta"], case["bntline"], max_slots=1, max_steps=1500)
+        S.set_order(hardest)
+        ids = [S.submit(bid, p) for p in paths]
+        time.sleep(1.0)   # every schedule prepared before the first admission
+        done = []
+        while len(done) < len(ids):
+            S.pump(40)
+            for r in ids:
+                if r not in done and S.result(r, bid)[0] == 2:
+                    done.append(r)
+        envs.append([S.result(r, bid)[1] for r in ids])
+        S.close()
+        got = [hours[ids.index(r)] for r in done]
+        print("hardest" if hardest else "fifo", "finish order", got, flush=True)
+        assert got == (sorted(hours, reverse=True) if hardest else hours)
+    for a, c in zip(envs[0], envs[1]):
+        assert np.array_equal(a, c)

@@ -336,10 +336,92 @@ def test_batched_f_stat_windows_equal_reference_cli(tmp_path):
     assert open(os.path.join(mine, "out.rex"), "rb").read() == open(os.path.join(root, "out.rex"), "rb").read()
 
 
+def test_batched_envsave2_windows_equal_reference_cli(tmp_path):
+    """-f_stat with -envSave2 60 over a batch of two storms: the first window
+    restarts at every 10-minute rex save, the second only when an hourly
+    envSave2 save falls due (MaxStatInitGrid flags 1 and 3, slosh2.c:
+    2514-2520); every envDir file (ExtendEnvName) and envDir2 file
+    (ExtendEnvName2, slosh2.c:2014-2027), the trailing ones included
+    (slosh2.c:2597-2668), equals the reference CLI's for each storm."""
+    import subprocess
+    b, t = synth.tiny(seed=5, ni=30, nj=26, hours=6)
+    root = str(tmp_path / "ref")
+    case = helpers.make_case(root, b, t)
+    rexutil.trk_canonical(t, case["trk"])
+    t2 = synth.straight_track(b, 9.0, 300.0, 70.0, 20.0, hours=6, landfall=3)
+    trk2 = os.path.join(root, "second.trk")
+    open(trk2, "w").write(synth.trk_text(t2))
+    rexutil.trk_canonical(t2, trk2)
+    trks = [case["trk"], trk2]
+    sub = b.name.lower()   # the basin's folder under -envDir (setup.c:241-251)
+    for trk in trks:
+        for d in ("e1", "e2"):
+            os.makedirs(os.path.join(root, d, sub), exist_ok=True)
+        r = subprocess.run([helpers.REF_CLI, "-basin", case["basin"], "-rootDir", root, "-trk", trk, "-env", "yes",
+                            "-envDir", os.path.join(root, "e1"), "-envDir2", os.path.join(root, "e2"),
+                            "-envSave2", "60", "-verbose", "0", "-f_stat", "1"], cwd=root, capture_output=True,
+                           text=True, timeout=600, preexec_fn=helpers._big_stack)
+        assert r.returncode == 0, r.stderr[-400:]
+    B = sloshgpu.Basin(case["dta"], case["bntline"])
+    X = sloshgpu.Batch(B, trks, precision=64)
+    X.upload()
+    X.set_maxstat(True)
+    X.set_maxstat2(True)
+    assert X.nsteps[0] == X.nsteps[1] and X.start_date(0) == X.start_date(1)
+    ht = [sloshgpu.track_datums(p) for p in trks]
+    y, mo, d, hr, mi = X.start_date(0)
+    start = (datetime.date(y, mo, d) - datetime.date(1970, 1, 1)).days * 86400.0
+    start = start + hr * 3600.
+    start = start + mi * 60
+    clock, rex_clock, env2_clock = start, 0.0, 0.0
+    while rex_clock < start:
+        rex_clock += 10 * 60
+    while env2_clock < start:
+        env2_clock += 60 * 60
+    mine = {"e1": {}, "e2": {}}
+
+    def save(which, at):
+        env = X.maxstat_env() if which == "e1" else X.maxstat2_env()
+        for s in range(2):
+            name = "%s%03d.env" % (os.path.basename(trks[s])[:-4], int((at - start) / 3600. + .5))
+            mine[which][name] = env[s]
+
+    for n in range(1, X.nsteps[0] + 1):
+        X.step(1)
+        clock = clock + X.track(0, n)[2]
+        if clock >= rex_clock:
+            save("e1", clock)
+            due2 = clock >= env2_clock
+            if due2:
+                save("e2", clock)
+            rex_clock += 10 * 60
+            X.maxstat_reset()
+            if due2:
+                env2_clock += 60 * 60
+                X.maxstat2_reset()
+    if rex_clock != clock + 10 * 60:
+        save("e1", rex_clock)
+    if env2_clock != clock + 60 * 60:
+        save("e2", env2_clock)
+    X.close()
+    B.close()
+    for which in ("e1", "e2"):
+        d = os.path.join(root, which, sub)
+        ref_names = sorted(os.listdir(d))
+        assert sorted(mine[which]) == ref_names and len(ref_names) >= 10, which
+        for f in ref_names:
+            s = 0 if f.startswith("storm") else 1
+            path = os.path.join(str(tmp_path), which + f)
+            sloshgpu.write_env(path, mine[which][f], *ht[s])
+            a, c = helpers.read_env(path), helpers.read_env(os.path.join(d, f))
+            assert a[:2] == c[:2] and a[4] == c[4], (which, f)
+            assert np.array_equal(a[3], c[3]), (which, f)
+
+
 def test_maxstat_streams_and_graphs(tmp_path):
-    """The windowed maxima are updated on every launch path: the batch in 3
-    parts on 3 streams and the step loop replayed as hipGraphs give the
-    one-stream values, storm by storm."""
+    """The windowed maxima (both windows) are updated on every launch path:
+    the batch in 3 parts on 3 streams and the step loop replayed as hipGraphs
+    give the one-stream values, storm by storm."""
     b, t = synth.tiny(seed=74, ni=30, nj=26, hours=8)
     root = str(tmp_path)
     case = helpers.make_case(root, b, t)
@@ -361,13 +443,15 @@ def test_maxstat_streams_and_graphs(tmp_path):
         if mode == "graphs":
             X.set_graph(20)
         X.set_maxstat(True)
+        X.set_maxstat2(True)
         X.step(130)
         X.maxstat_reset()
         X.step(170)
         X.sync()
-        res.append([X.get(s, "HBMAX") for s in range(X.n)])
+        res.append(([X.get(s, "HBMAX") for s in range(X.n)], X.maxstat2_env()))
         X.close()
     B.close()
     for r in res[1:]:
         for s in range(len(trks)):
-            assert np.array_equal(res[0][s], r[s]), s
+            assert np.array_equal(res[0][0][s], r[0][s]), s
+        assert np.array_equal(res[0][1], r[1])
