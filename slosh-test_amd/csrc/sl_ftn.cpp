@@ -140,10 +140,11 @@ int inital_(int *mhalt, int *imxb, int *jmxb, double zb[BAS_Y][BAS_X], int *hr, 
       for (int i = 0; i < ni; i++) g_run.buf[(size_t) j * ni + i] = llxfle_.YLT[j][i];
    slg_basin_set_ylt(g_run.basin, g_run.buf.data());
    const char *trks[1] = {trk_path.c_str()};
-   // SLG_PRECISION=64 runs the step in double, the reference default build's
-   // arithmetic (its env and rex files then match that build's byte for byte)
+   // The step runs in double, the reference default build's arithmetic (its env
+   // and rex files then match that build's byte for byte); SLG_PRECISION=32
+   // runs the FP32 step instead
    const char *pe = getenv("SLG_PRECISION");
-   const int prec = (pe && atoi(pe) == 64) ? 64 : 32;
+   const int prec = (pe && atoi(pe) == 32) ? 32 : 64;
    if (slg_batch_open_prec(g_run.basin, 1, trks, prec, &g_run.batch, err, sizeof err)) fatal("track", err);
    if (slg_batch_upload(g_run.batch, err, sizeof err)) fatal("device", err);
    g_run.nsteps = slg_batch_nsteps(g_run.batch, 0);

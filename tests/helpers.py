@@ -337,16 +337,17 @@ def _big_stack():
     resource.setrlimit(resource.RLIMIT_STACK, (hard, hard))
 
 
-def run_cli(exe, case_root, basin, tag, timeout=900, extra=()):
+def run_cli(exe, case_root, basin, tag, timeout=900, extra=(), env_extra=None):
     """Run a reference-style CLI (slosh -basin -rootDir -trk -rex -env) on a case
-    written by synth.write_case, with extra options; returns (env_path, rex_path)."""
+    written by synth.write_case, with extra options and environment variables;
+    returns (env_path, rex_path)."""
     env = os.path.join(case_root, tag + ".env")
     rex = os.path.join(case_root, tag + ".rex")
     r = subprocess.run([exe, "-basin", basin, "-rootDir", case_root, "-trk",
                         os.path.join(case_root, "storm.trk"), "-rex", rex, "-env", env, "-verbose", "0"]
                        + list(extra),
                        cwd=case_root, capture_output=True, text=True, timeout=timeout,
-                       preexec_fn=_big_stack)
+                       preexec_fn=_big_stack, env=dict(os.environ, **(env_extra or {})))
     if r.returncode != 0 or not os.path.exists(env):
         raise RuntimeError("%s failed (%d): %s %s" % (exe, r.returncode, r.stdout[-800:], r.stderr[-800:]))
     return env, rex

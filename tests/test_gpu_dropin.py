@@ -22,7 +22,7 @@ pytestmark = [pytest.mark.gpu,
 def _run_pair(b, t, root):
     bsn = synth.write_case(root, b, t)["basin"]
     print("drop-in", bsn, b.imxb, b.jmxb, flush=True)
-    env_g, rex_g = helpers.run_cli(helpers.GPU_CLI, root, bsn, "gpu")
+    env_g, rex_g = helpers.run_cli(helpers.GPU_CLI, root, bsn, "gpu", env_extra={"SLG_PRECISION": "32"})
     env_o, rex_o = helpers.run_cli(helpers.ORC_CLI["32"], root, bsn, "orc32")
     return bsn, env_g, rex_g, env_o, rex_o
 

@@ -37,7 +37,7 @@ def test_env_file_equals_dropin_cli(tmp_path):
     b, t = synth.tiny(seed=5)
     root = str(tmp_path)
     case = synth.write_case(root, b, t)
-    env_cli, _ = helpers.run_cli(helpers.GPU_CLI, root, case["basin"], "cli")
+    env_cli, _ = helpers.run_cli(helpers.GPU_CLI, root, case["basin"], "cli", env_extra={"SLG_PRECISION": "32"})
     B = sloshgpu.Basin(case["dta"], case["bntline"])
     X = sloshgpu.Batch(B, [case["trk"]])
     X.upload()
