@@ -273,6 +273,23 @@ int slg_server_tiles(const slg_server *sv, int *out, int n);
 int slg_server_stats(const slg_server *sv, long long *out6);
 void slg_server_close(slg_server *sv);
 
+/* ---- tide coupling through the C driver (slosh2.c:813-1027) ----------------
+ * The reference's driver adds the astronomical tide between steps: to the water
+ * level of every wet square (COMMON HB) and as TIDESH at every BDRYHT square
+ * (COMMON /BCPTS/, bdryht.f:85).  The Fortran-ABI library passes both on with
+ * these calls; a batch can use them the same way.
+ * boundary: the BDRYHT squares (ISH, JSH, 1-based; DEPSFC, depsfc.f:376-390), up
+ * to cap of them; returns their number.  is_subgrid: ISSUBGRID (issubgrid.f:
+ * 14-33).  put: overwrite a storm's HB (double [jmxb][imxb]) where its next
+ * step reads it.  set_boundary_tide: TIDESH of a storm from its next step on,
+ * n = the number of BDRYHT squares (0 until set).  static_height: STATICHT
+ * (staticht.f:14-41) at (i,j) with the storm values of its next step. */
+int slg_basin_boundary(const slg_basin *b, int *ish, int *jsh, int cap);
+int slg_basin_is_subgrid(const slg_basin *b, int i, int j);
+int slg_batch_put(slg_batch *x, int storm, const char *name, const double *src, char *err, int errlen);
+int slg_batch_set_boundary_tide(slg_batch *x, int storm, const double *tide, int n, char *err, int errlen);
+int slg_batch_static_height(const slg_batch *x, int storm, int i, int j, double *ht);
+
 /* Write the host initialisation of one run, and its storm schedule, in the
  * tagged-record layout used for the reference's state dumps (for checking). */
 int slg_export_init(const slg_batch *x, int storm, const char *path);

@@ -393,7 +393,7 @@ void orc_bdryht(const orc_basin *b, const orc_storm *s, orc_state *st)
       r2 = k;
       dr = r1 - r2;
       if (k > 790) k = 790;
-      HB(i, j) = s->DELP[k] + dr * (s->DELP[k + 1] - s->DELP[k]) + s->seadtm + RC(0.);
+      HB(i, j) = s->DELP[k] + dr * (s->DELP[k + 1] - s->DELP[k]) + s->seadtm + (st->tidesh ? st->tidesh[n] : RC(0.));
    }
 }
 

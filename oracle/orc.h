@@ -66,6 +66,7 @@ typedef struct {
    REAL *flwsqr;              /* transport across each 1-D flow side (nsqrwc) */
    char *f1dact;              /* 'T' where both squares of a side held water after FLW1DM_C */
    REAL chq[5];               /* each channel's flow (CHANNL's saved Q) */
+   REAL *tidesh;              /* tide height at each BDRYHT square (BCPTS TIDESH, nbcpt), or NULL: 0 */
 } orc_state;
 
 typedef struct {

@@ -28,6 +28,9 @@ int orc_info(void *c, int *imxb, int *jmxb);
 int orc_set_tables(void *c, const double *tab);
 int orc_advance_hbs(void *c, const double *row, int *diag, double *hbs);
 int orc_get(void *c, const char *name, double *out);
+int orc_put(void *c, const char *name, const double *in);
+int orc_set_tidesh(void *c, const double *t, int n);
+int orc_staticht(void *c, const double *row, const double *tab, int i, int j, double *out);
 void orc_close(void *c);
 
 #pragma pack(push, 2)
@@ -64,7 +67,41 @@ static struct {
    int ntab, *tab_step;     /* table sets and the step after which each applies */
    double *tabs;            /* [ntab][6*800] in S C P DELP SW CW order */
    double *buf, *hbs;
+   double *hb_last;         /* COMMON HB as this file last wrote it */
+   double tide_last[NBCPTS];
 } G;
+
+static void fatal(const char *what, const char *detail);
+
+/* COMMON HB after the step, as the reference's Fortran leaves it: the C
+ * driver's tide coupling (slosh2.c:813-1027) and windowed maxima read it
+ * between steps. */
+static void mirror_hb(void)
+{
+   int i, j;
+   orc_get(G.orc, "HB", G.buf);
+   for (j = 0; j < G.jmxb; j++)
+      for (i = 0; i < G.imxb; i++) dumb7_.HB[j][i] = G.hb_last[(size_t) j * G.imxb + i] = G.buf[(size_t) j * G.imxb + i];
+}
+
+/* Before a step: what the C driver changed in COMMON HB and TIDESH since the
+ * last step (the tide coupling adds the tide to wet squares and sets the
+ * boundary squares' tide) goes into the oracle's state. */
+static void take_common(void)
+{
+   int i, j, n = bcpts_.NBCPT, hb_changed = 0;
+   for (j = 0; j < G.jmxb && !hb_changed; j++)
+      if (memcmp(dumb7_.HB[j], G.hb_last + (size_t) j * G.imxb, sizeof(double) * G.imxb)) hb_changed = 1;
+   if (hb_changed) {
+      for (j = 0; j < G.jmxb; j++)
+         for (i = 0; i < G.imxb; i++) G.buf[(size_t) j * G.imxb + i] = dumb7_.HB[j][i];
+      orc_put(G.orc, "HB", G.buf);
+   }
+   if (memcmp(bcpts_.TIDESH, G.tide_last, sizeof(double) * n)) {
+      memcpy(G.tide_last, bcpts_.TIDESH, sizeof(double) * n);
+      if (orc_set_tidesh(G.orc, G.tide_last, n)) fatal("oracle", "TIDESH");
+   }
+}
 
 static void fatal(const char *what, const char *detail)
 {
@@ -141,6 +178,7 @@ int inital_(int *mhalt, int *imxb, int *jmxb, double zb[BAS_Y][BAS_X], int *hr, 
    free(G.tabs);
    free(G.buf);
    free(G.hbs);
+   free(G.hb_last);
    memset(&G, 0, sizeof G);
    if (slg_basin_open(dpath, NULL, &G.basin, err, sizeof err)) fatal("basin", err);
    slg_basin_dims(G.basin, &ni, &nj, NULL);
@@ -148,6 +186,7 @@ int inital_(int *mhalt, int *imxb, int *jmxb, double zb[BAS_Y][BAS_X], int *hr, 
    G.jmxb = nj;
    G.buf = calloc((size_t) ni * nj, sizeof(double));
    G.hbs = calloc((size_t) ni * nj, sizeof(double));
+   G.hb_last = calloc((size_t) ni * nj, sizeof(double));
    for (j = 0; j < nj; j++)
       for (i = 0; i < ni; i++) G.buf[(size_t) j * ni + i] = llxfle_.YLT[j][i];
    slg_basin_set_ylt(G.basin, G.buf);
@@ -183,9 +222,11 @@ int inital_(int *mhalt, int *imxb, int *jmxb, double zb[BAS_Y][BAS_X], int *hr, 
    slg_basin_field(G.basin, "ZBM", G.buf);
    for (j = 0; j < nj; j++)
       for (i = 0; i < ni; i++) dumb5_.ZBM[j][i] = G.buf[(size_t) j * ni + i];
-   orc_get(G.orc, "HB", G.buf);
-   for (j = 0; j < nj; j++)
-      for (i = 0; i < ni; i++) dumb7_.HB[j][i] = G.buf[(size_t) j * ni + i];
+   mirror_hb();
+   /* the BDRYHT squares (DEPSFC, depsfc.f:376-390) with no tide yet */
+   bcpts_.NBCPT = slg_basin_boundary(G.basin, bcpts_.ISH, bcpts_.JSH, NBCPTS);
+   if (bcpts_.NBCPT > NBCPTS) fatal("basin", "more than 12000 boundary squares");
+   memset(bcpts_.TIDESH, 0, sizeof bcpts_.TIDESH);
    slg_batch_start_date(G.batch, 0, d);
    *yr = d[0];
    *mo = d[1];
@@ -212,10 +253,12 @@ int tmstpc2_(double hb[BAS_Y][BAS_X], short *fpass, short *restart)
 {
    int k = G.step, n, i, j, diag[8];
    (void) restart;
+   take_common();
    for (n = 0; n < G.ntab; n++)
       if (G.tab_step[n] == k) orc_set_tables(G.orc, G.tabs + (size_t) n * 4800);
    if (orc_advance_hbs(G.orc, G.rows + (size_t) k * 90, diag, G.hbs)) fatal("oracle step", "");
    G.step++;
+   mirror_hb();
    if (fpass && *fpass) {
       orc_get(G.orc, "HMX", G.buf);
       for (j = 0; j < G.jmxb; j++)
@@ -259,8 +302,27 @@ int clnup_(double hb[BAS_Y][BAS_X], int *f_save_env)
    return 0;
 }
 
-int staticht_(int *i, int *j, double *a) { (void) i; (void) j; (void) a; fatal("unsupported", "STATICHT"); return 0; }
-int issubgrid_(int *i, int *j, int *a) { (void) i; (void) j; (void) a; fatal("unsupported", "ISSUBGRID"); return 0; }
+/* STATICHT (staticht.f:14-41) between steps: the storm values the next step
+ * reads (the driver calls it after TMSTPM, from its tide coupling) and the
+ * table set in effect for that step: one that takes effect before it, else the
+ * oracle's current one. */
+int staticht_(int *i, int *j, double *a)
+{
+   int n;
+   const double *tab = NULL;
+   for (n = 0; n < G.ntab; n++)
+      if (G.tab_step[n] == G.step) tab = G.tabs + (size_t) n * 4800;
+   orc_staticht(G.orc, G.rows + (size_t) G.step * 90, tab, *i, *j, a);
+   return 0;
+}
+
+/* ISSUBGRID (issubgrid.f:14-33): 1 when (i,j) is a 1-D side square whose cut
+ * or side narrows the flow below a full cell. */
+int issubgrid_(int *i, int *j, int *a)
+{
+   *a = slg_basin_is_subgrid(G.basin, *i, *j);
+   return 0;
+}
 int tmstp2_(void) { fatal("unsupported", "TMSTP2"); return 0; }
 int intlhtsub_(void) { fatal("unsupported", "INTLHTSUB"); return 0; }
 int intlhtadd_(void) { fatal("unsupported", "INTLHTADD"); return 0; }

@@ -304,6 +304,47 @@ int orc_set_tables(void *vc, const double *tab)
    return 0;
 }
 
+static void make_storm(ctx_t *c, const double *row, orc_storm *s);
+
+/* The tide height at each BDRYHT square (COMMON /BCPTS/ TIDESH, which the C
+ * driver's tide coupling sets between steps, slosh2.c:998-1025). */
+int orc_set_tidesh(void *vc, const double *t, int n)
+{
+   ctx_t *c = vc;
+   int k;
+   if (n != c->b.nbcpt) return -1;
+   if (!c->st.tidesh) c->st.tidesh = calloc((size_t) (n > 0 ? n : 1), sizeof(R));
+   for (k = 0; k < n; k++) c->st.tidesh[k] = (R) t[k];
+   return 0;
+}
+
+/* STATICHT (staticht.f:14-41): the static height at square (i,j) for the storm
+ * values of row and the DELP column of the table set tab (S C P DELP SW CW, 800
+ * each; NULL: the set in effect), in REAL. */
+int orc_staticht(void *vc, const double *row, const double *tab, int i, int j, double *out)
+{
+   ctx_t *c = vc;
+   const orc_basin *b = &c->b;
+   orc_storm s;
+   R xr, yr, x, y, rsq, r1, r2, dr, d0, d1;
+   int k;
+   make_storm(c, row, &s);
+   xr = b->elpcl[i - 1] * b->cosl[j - 1];
+   yr = b->elpdl[i - 1] * b->sinl[j - 1];
+   x = xr - s.c1 - s.ax;
+   y = yr - s.c2 - s.ay;
+   rsq = x * x + y * y;
+   r1 = (R) sqrt(rsq) / (R) 5280. + (R) 1.;
+   k = (int) r1;
+   r2 = (R) k;
+   dr = r1 - r2;
+   if (k > 790) k = 790;
+   d0 = tab ? (R) tab[3 * 800 + k - 1] : c->DELP[k];
+   d1 = tab ? (R) tab[3 * 800 + k] : c->DELP[k + 1];
+   *out = (double) (d0 + dr * (d1 - d0) + s.seadtm);
+   return 0;
+}
+
 static void make_storm(ctx_t *c, const double *row, orc_storm *s)
 {
    int k;
@@ -440,6 +481,7 @@ void orc_close(void *vc)
    ctx_t *c = vc;
    int k;
    if (!c) return;
+   free(c->st.tidesh);
    free(c->zb); free(c->zbm); free(c->ylt); free(c->hb); free(c->ub); free(c->vb);
    free(c->hmx); free(c->hsub); free(c->itree); free(c->kskp); free(c->imann);
    free(c->is); free(c->ie); free(c->ms); free(c->me); free(c->ish); free(c->jsh);

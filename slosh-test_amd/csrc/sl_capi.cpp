@@ -245,6 +245,7 @@ static int basin_upload(slg_basin *B, char *err, int errlen)
    add(I.ereal.size() * rs);
    add(I.ecut.size() * rs);
    add(np * 8);        // ZB in double
+   add(np * 4);        // BDRYHT list index per point
    char *d = nullptr;
    HIPOK(hipMalloc((void **) &d, tot));
    I.dmem = d;
@@ -324,6 +325,11 @@ static int basin_upload(slg_basin *B, char *err, int errlen)
       for (int j = 1; j <= NJ; j++)
          for (int i = 1; i <= b->imxb; i++) zd[(size_t) (j - 1) * P + (i - 1)] = b->zb[PIX(b, i, j)];
       I.zbd = (const double *) up(zd.data(), np * 8);
+   }
+   {
+      std::vector<int> bi(np, -1);
+      for (int n = 0; n < b->nbcpt; n++) bi[(size_t) (b->jsh[n] - 1) * P + (b->ish[n] - 1)] = n;
+      D.bcidx = (const int *) up(bi.data(), np * 4);
    }
    HIPOK(upe);
    HIPOK(hipDeviceSynchronize());
@@ -787,6 +793,7 @@ static SlRunDevT<R> run_view(const SlRunDevT<R> &D, const SlBasinDevT<R> &B, int
    V.hsq += s * (size_t) B.nsq;
    V.chq += s * 5;
    if (V.hb2) V.hb2 += s * fs;
+   if (V.tidesh) V.tidesh += s * (size_t) B.nbcpt;
    return V;
 }
 
@@ -1134,6 +1141,132 @@ extern "C" int slg_basin_field(const slg_basin *B, const char *name, double *dst
 // ITIME, MHALT, DELT, storm latitude and longitude (XY2LLX of (C1+AX, C2+AY)),
 // DELP (YDELP), size (YC24), speed (C21), direction (AZMTH - C22 in degrees).
 // Step 0 gives the state after initialisation.  Kept for batches of <= 8 runs.
+// The BDRYHT squares (DEPSFC's ISH/JSH list, depsfc.f:376-390), 1-based, up to
+// cap of them; returns their number.
+extern "C" int slg_basin_boundary(const slg_basin *B, int *ish, int *jsh, int cap)
+{
+   const sl_basin *b = &B->h;
+   for (int n = 0; n < b->nbcpt && n < cap; n++) {
+      ish[n] = b->ish[n];
+      jsh[n] = b->jsh[n];
+   }
+   return b->nbcpt;
+}
+
+// ISSUBGRID (issubgrid.f:14-33): 1 when square (i,j) holds a 1-D flow side
+// whose cut width (DELCUT) or cut interior/exterior length (CUTLI, CUTLE) is
+// below one cell, else 0.
+extern "C" int slg_basin_is_subgrid(const slg_basin *B, int i, int j)
+{
+   const sl_basin *b = &B->h;
+   for (int l = 1; l <= b->nsqrwc; l++) {
+      if (b->isqr[l - 1] != i || b->jsqr[l - 1] != j) continue;
+      if (l <= b->nsqrw) {
+         if (b->delcut[l - 1] < 1.) return 1;
+      } else {
+         const int ll = l - b->nsqrw;
+         if (b->cutli[ll - 1] < 1. || b->cutle[ll - 1] < 1.) return 1;
+      }
+   }
+   return 0;
+}
+
+// Overwrite a storm's HB (double [jmxb][imxb]) where its next step reads it:
+// for a driver that changes the water level between steps (the tide coupling,
+// slosh2.c:813-1027, adds the tide to every wet square).
+template <class R>
+static int batch_put_hb(slg_batch *X, int storm, const double *src, char *err, int errlen)
+{
+   const sl_basin *b = &X->b->h;
+   SlRunDevT<R> &D = run(X, R());
+   const int P = X->b->pitch;
+   std::vector<R> f(D.fstride, R(0));
+   for (int j = 1; j <= b->jmxb; j++)
+      for (int i = 1; i <= b->imxb; i++) f[(size_t) (j - 1) * P + (i - 1)] = (R) src[PIX(b, i, j)];
+   R *dst = (X->last_par[storm] ? D.hb1 : D.hb0) + (size_t) storm * D.fstride;
+   HIPOK(hipStreamSynchronize(X->st));
+   HIPOK(hipMemcpy(dst, f.data(), D.fstride * sizeof(R), hipMemcpyHostToDevice));
+   return 0;
+}
+
+extern "C" int slg_batch_put(slg_batch *X, int storm, const char *name, const double *src, char *err, int errlen)
+{
+   if (!X->dmem || storm < 0 || storm >= X->nstorm || strcmp(name, "HB")) {
+      seterr(err, errlen, "%s", "HB of an uploaded storm only");
+      return SLG_EINPUT;
+   }
+   if (X->last_par[storm] != X->parity && X->sc[storm].nsteps > X->step) {
+      seterr(err, errlen, "%s", "storm not at the batch's step");
+      return SLG_EINPUT;
+   }
+   return X->prec == 64 ? batch_put_hb<double>(X, storm, src, err, errlen)
+                        : batch_put_hb<float>(X, storm, src, err, errlen);
+}
+
+// The tide height BDRYHT adds at each boundary square (COMMON /BCPTS/ TIDESH,
+// bdryht.f:85), n = the basin's number of BDRYHT squares, from the storm's next
+// step on.  The first call gives every storm of the batch a row of zeros.
+template <class R>
+static int batch_tide(slg_batch *X, int storm, const double *tide, int n, char *err, int errlen)
+{
+   SlRunDevT<R> &D = run(X, R());
+   const size_t nb = (size_t) std::max(n, 1);
+   HIPOK(hipStreamSynchronize(X->st));
+   if (!D.tidesh) {
+      R *t = nullptr;
+      HIPOK(hipMalloc((void **) &t, nb * X->nstorm * sizeof(R)));
+      HIPOK(hipMemset(t, 0, nb * X->nstorm * sizeof(R)));
+      D.tidesh = t;
+      drop_graph(X);   // captured launches hold the old pointer
+   }
+   std::vector<R> v(nb, R(0));
+   for (int k = 0; k < n; k++) v[k] = (R) tide[k];
+   HIPOK(hipMemcpy(D.tidesh + (size_t) storm * n, v.data(), (size_t) n * sizeof(R), hipMemcpyHostToDevice));
+   return 0;
+}
+
+extern "C" int slg_batch_set_boundary_tide(slg_batch *X, int storm, const double *tide, int n, char *err,
+                                           int errlen)
+{
+   if (!X->dmem || storm < 0 || storm >= X->nstorm || n != X->b->h.nbcpt) {
+      seterr(err, errlen, "%s", "an uploaded storm and one value per BDRYHT square");
+      return SLG_EINPUT;
+   }
+   return X->prec == 64 ? batch_tide<double>(X, storm, tide, n, err, errlen)
+                        : batch_tide<float>(X, storm, tide, n, err, errlen);
+}
+
+// STATICHT (staticht.f:14-41): the static height at square (i,j) from the
+// storm values and radial table of the storm's next step, in the batch's
+// precision, as BDRYHT computes it without the tide term.
+template <class R>
+static double static_height(const slg_batch *X, int storm, int i, int j)
+{
+   const storm_sched &S = X->sc[storm];
+   const BasinImg<R> &I = img(X->b, R());
+   const int k0 = std::min(X->step, S.nsteps - 1);
+   const R *P = &sched_params(S, R())[(size_t) k0 * SL_NPAR];
+   const R *T = &sched_tables(S, R())[(size_t) (int) P[P_TVER] * SL_NTABV * 8];
+   const R xr = I.vi[2][i - 1] * I.vj[3][j - 1], yr = I.vi[3][i - 1] * I.vj[2][j - 1];
+   const R x = xr - P[P_C1] - P[P_AX], y = yr - P[P_C2] - P[P_AY];
+   const R rsq = x * x + y * y, r1 = std::sqrt(rsq) / R(5280.) + R(1.);
+   int k = (int) r1;
+   const R r2 = (R) k, dr = r1 - r2;
+   if (k > 790) k = 790;
+   const R *Rk = T + (size_t) k * 8, *Rk1 = Rk + 8;
+   return (double) (Rk[5] + dr * (Rk1[5] - Rk[5]) + P[P_SEA]);   // column 5: DELP
+}
+
+extern "C" int slg_batch_static_height(const slg_batch *X, int storm, int i, int j, double *ht)
+{
+   const sl_basin *b = &X->b->h;
+   if (storm < 0 || storm >= X->nstorm || i < 1 || i > b->imxb || j < 1 || j > b->jmxb ||
+       !(X->prec == 64 ? X->b->i64.built : X->b->i32.built))
+      return SLG_EINPUT;
+   *ht = X->prec == 64 ? static_height<double>(X, storm, i, j) : static_height<float>(X, storm, i, j);
+   return 0;
+}
+
 extern "C" int slg_batch_track(const slg_batch *X, int storm, int n, double *out9)
 {
    if (storm < 0 || storm >= X->nstorm) return SLG_EINPUT;

@@ -51,6 +51,7 @@ struct SlBasinDevT {
    const R *sint, *cost, *sinl, *cosl, *sinl2;                        /* [jmxb] */
    const R *tab8;             /* [245][600][8]: AI2 SLPAI AR2 BR2 BI2 CR2 CI2 0 per (iik, id) */
    const R *g4;               /* [4][pitch*jmxb]: metric weight GAMF of neighbour k (IIH/JJH order) */
+   const int *bcidx;          /* [pitch*jmxb]: index n of a BDRYHT square in the ISH/JSH list, else -1 */
    /* sub-grid 1-D flow sides, L = 1..nsqrwc at index L-1 (1-D sides, weirs
     * from nsqrs+1, cuts from nsqrw+1); f1d_on: FLW1DM runs (sides and NOF1D
     * not '+'); sfl: STA starts with "SOUTH FL" */
@@ -85,6 +86,8 @@ struct SlRunDevT {
    R *hb0, *hb1, *ub, *vb, *hmx;   /* [nstorm][fstride] */
    R *hb2;                    /* [nstorm][fstride]: HB after TMSTPC1 on the fused path of
                                  basins with 1-D flow sides or channels (else null) */
+   R *tidesh;                 /* [nstorm][nbcpt]: tide height BDRYHT adds at each boundary
+                                 square (COMMON /BCPTS/ TIDESH), or null: 0 */
    R *hsubtop;                /* [nstorm][pitch]: HSUB row JMXB for XFILTER */
    uint8_t *pflag, *hclass;   /* [nstorm][fstride] (staged path) */
    uint8_t *uz;               /* [nstorm][fstride]: UB == 0 after CONTTY (fused path) */

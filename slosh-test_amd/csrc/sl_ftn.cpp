@@ -68,6 +68,8 @@ struct Run {
    int imxb = 0, jmxb = 0, nsteps = 0, step = 0;
    bool hb_each_step = false;   // the driver reads COMMON HB after every step
    std::vector<double> buf;     // [jmxb][imxb] staging
+   std::vector<double> hb_last; // COMMON HB as this library last wrote it
+   std::vector<double> tide_last;   // TIDESH as last passed to the device
 };
 Run g_run;
 
@@ -89,6 +91,44 @@ void get(const char *field)
 {
    char err[256];
    if (slg_batch_get(g_run.batch, 0, field, g_run.buf.data(), err, sizeof err)) fatal("field copy failed", err);
+}
+
+// COMMON HB from the staging buffer; what was written is remembered, so that a
+// change the driver makes before the next step can be told apart
+void mirror_hb()
+{
+   const int ni = g_run.imxb, nj = g_run.jmxb;
+   for (int j = 0; j < nj; j++)
+      for (int i = 0; i < ni; i++) dumb7_.HB[j][i] = g_run.buf[(size_t) j * ni + i];
+   g_run.hb_last = g_run.buf;
+}
+
+// Before a step: the C driver's tide coupling (slosh2.c:813-1027) changes
+// COMMON HB (the tide added to every wet square) and TIDESH (the tide at the
+// BDRYHT squares) between steps, as the reference's Fortran sees them in
+// COMMON; both go to the device when they changed.  HB is compared only while
+// the mirror is kept (without it COMMON HB is not the model's).
+void take_common()
+{
+   char err[512];
+   const int ni = g_run.imxb, nj = g_run.jmxb;
+   if (g_run.hb_each_step) {
+      bool changed = false;
+      for (int j = 0; j < nj && !changed; j++)
+         changed = memcmp(dumb7_.HB[j], &g_run.hb_last[(size_t) j * ni], sizeof(double) * ni) != 0;
+      if (changed) {
+         for (int j = 0; j < nj; j++)
+            for (int i = 0; i < ni; i++) g_run.buf[(size_t) j * ni + i] = dumb7_.HB[j][i];
+         if (slg_batch_put(g_run.batch, 0, "HB", g_run.buf.data(), err, sizeof err)) fatal("HB", err);
+         g_run.hb_last = g_run.buf;
+      }
+   }
+   const int n = bcpts_.NBCPT;
+   if (memcmp(bcpts_.TIDESH, g_run.tide_last.data(), sizeof(double) * n)) {
+      memcpy(g_run.tide_last.data(), bcpts_.TIDESH, sizeof(double) * n);
+      if (slg_batch_set_boundary_tide(g_run.batch, 0, g_run.tide_last.data(), n, err, sizeof err))
+         fatal("TIDESH", err);
+   }
 }
 
 }   // namespace
@@ -164,8 +204,13 @@ int inital_(int *mhalt, int *imxb, int *jmxb, double zb[BAS_Y][BAS_X], int *hr, 
    for (int j = 0; j < nj; j++)
       for (int i = 0; i < ni; i++) dumb5_.ZBM[j][i] = g_run.buf[(size_t) j * ni + i];
    get("HB0");
-   for (int j = 0; j < nj; j++)
-      for (int i = 0; i < ni; i++) dumb7_.HB[j][i] = g_run.buf[(size_t) j * ni + i];
+   mirror_hb();
+   // the BDRYHT squares (DEPSFC, depsfc.f:376-390) for the driver's tide
+   // coupling, with no tide yet
+   bcpts_.NBCPT = slg_basin_boundary(g_run.basin, bcpts_.ISH, bcpts_.JSH, NBCPTS);
+   if (bcpts_.NBCPT > NBCPTS) fatal("basin", "more than 12000 boundary squares");
+   memset(bcpts_.TIDESH, 0, sizeof bcpts_.TIDESH);
+   g_run.tide_last.assign(NBCPTS, 0.);
    int d[5];
    slg_batch_start_date(g_run.batch, 0, d);
    *yr = d[0];
@@ -201,6 +246,7 @@ int tmstpc2_(double hb[BAS_Y][BAS_X], short *fpass, short *restart)
    (void) restart;
    char err[512];
    const bool pass = fpass && *fpass;
+   take_common();
    // the staged path keeps HB before FILTER, which HHB needs on filter steps
    slg_batch_set_path(g_run.batch, pass ? SLG_PATH_STAGED : SLG_PATH_FUSED, 16);
    if (slg_batch_step(g_run.batch, 1, err, sizeof err)) fatal("step", err);
@@ -218,9 +264,7 @@ int tmstpc2_(double hb[BAS_Y][BAS_X], short *fpass, short *restart)
    }
    if (g_run.hb_each_step) {   // HB at the end of the step, as COMMON holds it after TMSTPM
       get("HB");
-      const int ni = g_run.imxb, nj = g_run.jmxb;
-      for (int j = 0; j < nj; j++)
-         for (int i = 0; i < ni; i++) dumb7_.HB[j][i] = g_run.buf[(size_t) j * ni + i];
+      mirror_hb();
    }
    return 0;
 }
@@ -266,21 +310,21 @@ int clnup_(double hb[BAS_Y][BAS_X], int *f_save_env)
    return 0;
 }
 
-// Tide and sub-grid entry points: reached only with -f_tide or a tide spin-up.
+// STATICHT (staticht.f:14-41), called by the tide coupling between steps: the
+// static height at (i,j) with the storm values of the next step.
 int staticht_(int *i, int *j, double *ans)
 {
-   (void) i;
-   (void) j;
-   (void) ans;
-   fatal("tides (STATICHT) are not supported by this build", "");
+   if (slg_batch_static_height(g_run.batch, 0, *i, *j, ans)) fatal("STATICHT", "square outside the basin");
+   return 0;
 }
+// ISSUBGRID (issubgrid.f:14-33): whether (i,j) is a narrowed 1-D side square.
 int issubgrid_(int *i, int *j, int *ans)
 {
-   (void) i;
-   (void) j;
-   (void) ans;
-   fatal("tides (ISSUBGRID) are not supported by this build", "");
+   *ans = slg_basin_is_subgrid(g_run.basin, *i, *j);
+   return 0;
 }
+// The tide spin-up (SpinUpTideGrid, slosh2.c:1029-1225: TMSTP2, INTLHTSUB,
+// INTLHTADD) is not part of this build.
 int tmstp2_(void)
 {
    fatal("tide spin-up (TMSTP2) is not supported by this build", "");

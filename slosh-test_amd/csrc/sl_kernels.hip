@@ -778,7 +778,8 @@ __device__ __forceinline__ real gamf_of(const SlBasinDev &B, int k, uint32_t p)
 // FLT2G (both passes, when smpt) + BDRYHT for square (i,j), j <= JMXB-1.
 template <class HS, class CL, class Z>
 __device__ __forceinline__ real smooth_pt(const SlBasinDev &B, const real *P, const real *T, bool smpt,
-                                           int i, int j, const HS &hs, const CL &cl, const Z &z)
+                                           int i, int j, const HS &hs, const CL &cl, const Z &z,
+                                           const real *tide)
 {
    const uint32_t p = IDX(B, i, j);
    real out = hs(i, j);
@@ -834,7 +835,7 @@ __device__ __forceinline__ real smooth_pt(const SlBasinDev &B, const real *P, co
       const real r2 = (real) k, dr = r1 - r2;
       if (k > 790) k = 790;
       const real *Rk = T + (uint32_t) k * 8, *Rk1 = Rk + 8;
-      out = Rk[R_DELP] + dr * (Rk1[R_DELP] - Rk[R_DELP]) + P[P_SEA] + real(0.);
+      out = Rk[R_DELP] + dr * (Rk1[R_DELP] - Rk[R_DELP]) + P[P_SEA] + (tide ? tide[B.bcidx[p]] : real(0.));
    }
    return out;
 }
@@ -1352,7 +1353,7 @@ __global__ void __launch_bounds__(TX * TY) k_smooth(SlBasinDev B, SlRunDev R, St
    }
    const bool smpt = (int) P[P_FLAGS] & SF_SMPT;
    A.hn[so + p] = smooth_pt(B, P, tabv(R, s, P), smpt, i, j, GF{hs, B.pitch}, GB{R.hclass + so, B.pitch},
-                            ZG{B.zb, B.zbm, B.flags, B.pitch});
+                            ZG{B.zb, B.zbm, B.flags, B.pitch}, R.tidesh ? R.tidesh + (size_t) s * B.nbcpt : nullptr);
 }
 
 // ------------------------------------------------------- sub-grid 1-D flow
@@ -1869,6 +1870,7 @@ k_stepB(SlBasinDev B, SlRunDev R, StepArgs A, int ntx, int nty, int dyn, int isl
    const real *P = par(R, s, step);
    const real *T = tabv(R, s, P);
    const int sf = (int) P[P_FLAGS];
+   const real *tide = R.tidesh ? R.tidesh + (size_t) s * B.nbcpt : nullptr;
    bool smpt = SMPT && (!dyn || (sf & SF_SMPT));
    // a closed island basin filters (FLTER2) and runs MOMNTM in kernels of
    // their own: its periodic neighbours and row copies cross tiles
@@ -1984,7 +1986,7 @@ k_stepB(SlBasinDev B, SlRunDev R, StepArgs A, int ntx, int nty, int dyn, int isl
       if (gi >= 1 && gi <= IM && gj >= 1 && gj <= JM) {
          if (post) v = hs_g[IDX(B, gi, gj)];
          else if (gj == JM) v = (gi <= IM - 1) ? hs_g[IDX(B, gi, 1)] : hsL(gi, gj);
-         else v = smooth_pt(B, P, T, smpt, gi, gj, hsL, clL, zL);
+         else v = smooth_pt(B, P, T, smpt, gi, gj, hsL, clL, zL, tide);
       }
       sh_s[e] = v;
    }

@@ -629,3 +629,79 @@ def forecast_track(b: Basin, rng, hours=72):
         la.append(b.alto + (py - cy) / 69.05)
         lo.append(-b.alno + (px - cx) / (69.05 * math.cos(math.radians(b.alto))))
     return Track(H, la, lo, dps, rms, 0, hours, H[kl], interp="Linear")
+
+
+# ---------------------------------------------------------------- tide inputs
+# The 37 constituents in the order the reference's harmonic files list them
+# ("Tides and Currents" order, tideutil.c:155-163) with their speeds in degrees
+# per hour (tideutil.c:301-315); the synthetic amplitudes favour the
+# semidiurnal and diurnal ones, as coastal tides do.
+TIDE_NAMES = ["M2", "S2", "N2", "K1", "M4", "O1", "M6", "MK3", "S4", "MN4", "NU2", "S6", "MU2", "2N2", "OO1",
+              "LAM2", "S1", "M1", "J1", "MM", "SSA", "SA", "MSF", "MF", "RHO", "Q1", "T2", "R2", "2Q1", "P1",
+              "2SM2", "M3", "L2", "2MK3", "K2", "M8", "MS4"]
+_TIDE_SCALE = {"M2": 1.6, "S2": 0.4, "N2": 0.35, "K1": 0.5, "O1": 0.45, "P1": 0.15, "K2": 0.1, "Q1": 0.08}
+
+
+def ft03_text(seed=3, year0=2000, nyears=41) -> str:
+    """Yearly node factors and equilibrium arguments in the layout LoadYearlyConstit
+    reads (tideutil.c:82-283): five lines per year of 8, 8, 8, 8 and 5
+    constituents (ft03 order), each line 'YYYY 0 L' then per constituent the node
+    factor x 1000 (I4) and V+U x 10 (I4); the first line's year starts the file."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for y in range(year0, year0 + nyears):
+        xode = rng.integers(850, 1150, size=37)
+        vpu = rng.integers(0, 3600, size=37)
+        k = 0
+        for line, n in enumerate((8, 8, 8, 8, 5), start=1):
+            s = "%4d 0 %1d" % (y, line)
+            for _ in range(n):
+                s += "%4d%4d" % (xode[k], vpu[k])
+                k += 1
+            out.append(s)
+    return "\n".join(out) + "\n"
+
+
+def bhc_bytes(b: Basin, abbrev: str, seed=4) -> bytes:
+    """A harmonic-constant file as ReadHarmonicHeader/ReadHarmonicBody read it
+    (tideutil.c:470-645): 'HarmConstit' + NUL, version 1 (int16), the basin
+    abbreviation (5 bytes), the cell counts IMXB-1 and JMXB-1 (int16), 37 names
+    of 5 bytes, then per cell (I fastest, tideInd, slosh2.c:492-518) longitude
+    and latitude x 1e6 and per constituent amplitude (ft) and phase (deg)
+    x 1e6, all little-endian int32.  Amplitudes vary smoothly over the basin."""
+    import struct
+    rng = np.random.default_rng(seed)
+    ni, nj = b.imxb - 1, b.jmxb - 1
+    head = b"HarmConstit\0" + struct.pack("<h", 1) + abbrev.encode()[:5].ljust(5, b"\0")
+    head += struct.pack("<hh", ni, nj) + b"".join(n.encode().ljust(5, b"\0") for n in TIDE_NAMES)
+    base_amp = np.array([_TIDE_SCALE.get(n, 0.02) for n in TIDE_NAMES])
+    base_ph = rng.uniform(0, 360, size=37)
+    gi = np.linspace(0, 1, ni)[None, :]
+    gj = np.linspace(0, 1, nj)[:, None]
+    cells = np.zeros((nj, ni, 2 + 74), dtype=np.int64)
+    cells[:, :, 0] = int(-b.alno * 1e6)
+    cells[:, :, 1] = int(b.alto * 1e6)
+    for k in range(37):
+        amp = base_amp[k] * (0.8 + 0.4 * gi + 0.1 * gj)
+        ph = (base_ph[k] + 20.0 * gi - 10.0 * gj) % 360.0
+        cells[:, :, 2 + 2 * k] = np.round(amp * 1e6)
+        cells[:, :, 3 + 2 * k] = np.round(ph * 1e6)
+    return head + cells.astype("<i4").tobytes()
+
+
+def write_tide_db(root: str, b: Basin, abbrev: str, seed=4, database=2014, flavor=None):
+    """The tide inputs -f_tide reads under -rootDir (usrparse.c:432-436,
+    slosh2.c:2173-2217): <root>/tidefile.ec<database>/ft03.dta and <abbrev>.bhc
+    (no datum-adjustment file: the driver then takes the adjustment as 0), and
+    with flavor the basin's line of tide_flavor.txt for -f_tide VDEF
+    (slosh2.c:2098-2160)."""
+    d = os.path.join(root, "tidefile.ec%4d" % database)
+    os.makedirs(d, exist_ok=True)
+    if flavor:
+        with open(os.path.join(d, "tide_flavor.txt"), "w") as fh:
+            fh.write("# basin:tide version\n%s:%s\n" % (abbrev.strip(), flavor))
+    with open(os.path.join(d, "ft03.dta"), "w") as fh:
+        fh.write(ft03_text(seed))
+    with open(os.path.join(d, abbrev + ".bhc"), "wb") as fh:
+        fh.write(bhc_bytes(b, abbrev, seed))
+    return d

@@ -293,6 +293,38 @@ def manning_case(root, idx, ni=30, nj=26, hours=6):
     return make_case(root, b, t)
 
 
+# Tide coupling through the reference's C driver (-f_tide; slosh2.c:813-1027):
+# (id, -f_tide value, basin kind).  'deep' has squares at least 290 ft deep,
+# which V2.1 / V2.2 set to STATICHT + tide every step; 'sides' has 1-D sides
+# with cuts narrower than a cell, which V2.2 / V2.4 take out of the tide
+# (ISSUBGRID); VDEF takes the version from tide_flavor.txt.
+TIDE_CASES = [("v2-polar", "V2", "polar"), ("v2-hyper", "V2", "hyper"), ("v21-deep", "V2.1.-5", "deep"),
+              ("v22-sides", "V2.2.-3", "sides"), ("v24-sides", "V2.4", "sides"), ("vdef-deep", "VDEF", "deep")]
+
+
+def tide_basin(kind, hours=6):
+    if kind == "polar":
+        return synth.tiny(seed=41, ni=30, nj=26, hours=hours)
+    if kind == "hyper":
+        return synth.tiny(seed=42, gtype="+", ni=30, nj=26, hours=hours)
+    if kind == "deep":
+        b = synth.make_basin("tdp", " ", 30, 26, seed=43, land_frac=0.2, n_barriers=20, dt=30.0,
+                             size_km=(1.5, 4.0), depth_m=150.0)
+        return b, synth.straight_track(b, 12.0, 330.0, 60.0, 15.0, hours=hours, landfall=hours // 2)
+    b, t = edge_basin(1, hours=hours)
+    return b, t
+
+
+def tide_case(root, idx, hours=6):
+    """A case with its tide inputs; returns (case, -f_tide value)."""
+    cid, flag, kind = TIDE_CASES[idx]
+    b, t = tide_basin(kind, hours)
+    case = make_case(root, b, t)
+    synth.write_tide_db(root, b, case["basin"], seed=50 + idx, flavor="V2.2.-3" if flag == "VDEF" else None)
+    case["b"] = b
+    return case, flag
+
+
 # Basins with sub-grid 1-D flow sides (1-D sides with and without banks, raised
 # weirs, cuts; 'T' canal sheltering and rotated stress directions) and
 # channels: (grid type, seed, ebsn1, ismooth, nof1d, channels).  NOF1D '+'

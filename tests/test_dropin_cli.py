@@ -73,3 +73,36 @@ def test_list_run_matches_reference(tmp_path):
     orc = helpers.run_cli_list(helpers.ORC_CLI["64"], root, cases, "orc64")
     for (ea, ra), (eb, rb) in zip(ref, orc):
         assert helpers.same_cli_outputs(ea, ra, eb, rb) == []
+
+
+@pytest.mark.parametrize("idx", range(len(helpers.TIDE_CASES)), ids=[c[0] for c in helpers.TIDE_CASES])
+def test_tide_coupling_over_oracle64_matches_reference(idx, tmp_path):
+    """-f_tide: the reference's driver adds the tide between steps to COMMON HB
+    and TIDESH (CalcTideGrid2, slosh2.c:813-1027) and calls STATICHT and
+    ISSUBGRID; the glue hands those changes to the model before the next step
+    and BDRYHT adds TIDESH (bdryht.f:85).  The env and rex files equal the
+    reference CLI's byte for byte."""
+    root = str(tmp_path)
+    case, flag = helpers.tide_case(root, idx)
+    env_r, rex_r = helpers.run_cli(helpers.REF_CLI, root, case["basin"], "ref", extra=["-f_tide", flag])
+    env_o, rex_o = helpers.run_cli(helpers.ORC_CLI["64"], root, case["basin"], "orc", extra=["-f_tide", flag])
+    assert helpers.same_cli_outputs(env_r, rex_r, env_o, rex_o) == []
+
+
+def test_tide_changes_the_run(tmp_path):
+    """The tide cases are sensitive: with the tide the reference's env differs
+    from its surge-only run on many squares (a glue that dropped the driver's
+    changes would write the surge-only files), and the deep case has squares
+    that V2.1 sets to STATICHT + tide."""
+    import numpy as np
+    root = str(tmp_path)
+    case, flag = helpers.tide_case(root, 0)
+    env_t, _ = helpers.run_cli(helpers.REF_CLI, root, case["basin"], "tide", extra=["-f_tide", flag])
+    env_s, _ = helpers.run_cli(helpers.REF_CLI, root, case["basin"], "surge")
+    a, c = helpers.read_env(env_t)[3], helpers.read_env(env_s)[3]
+    assert int((a != c).sum()) > 50
+    # the deep case forces STATICHT + tide on its deep squares
+    root2 = str(tmp_path / "deep")
+    case2, flag2 = helpers.tide_case(root2, 2)
+    zb = np.array(case2["b"].zb)
+    assert (zb >= 290).sum() > 20

@@ -183,3 +183,24 @@ def test_gpu_f_stat_without_hb_mirror_differs(tmp_path):
                                and f != "out.env"]
     assert envs["on"] == envs["off"]
     assert any(not np.array_equal(a, c) for a, c in zip(envs["on_vals"], envs["off_vals"]))
+
+
+@pytest.mark.parametrize("idx", range(len(helpers.TIDE_CASES)), ids=[c[0] for c in helpers.TIDE_CASES])
+def test_gpu_cli_tide_coupling(idx, tmp_path):
+    """-f_tide through the drop-in: the driver's changes to COMMON HB and
+    TIDESH between steps reach the device (slg_batch_put,
+    slg_batch_set_boundary_tide), STATICHT and ISSUBGRID answer from the
+    product.  FP64: env and rex byte-identical to the reference CLI; FP32:
+    byte-identical to the same driver over the FP32 oracle."""
+    root = str(tmp_path)
+    case, flag = helpers.tide_case(root, idx)
+    print("tide", helpers.TIDE_CASES[idx], flush=True)
+    bsn = case["basin"]
+    env_r, rex_r = helpers.run_cli(helpers.REF_CLI, root, bsn, "ref", extra=["-f_tide", flag])
+    env_g, rex_g = helpers.run_cli(helpers.GPU_CLI, root, bsn, "gpu64", extra=["-f_tide", flag],
+                                   env_extra={"SLG_PRECISION": "64"})
+    assert helpers.same_cli_outputs(env_g, rex_g, env_r, rex_r) == []
+    env_o, rex_o = helpers.run_cli(helpers.ORC_CLI["32"], root, bsn, "orc32", extra=["-f_tide", flag])
+    env_h, rex_h = helpers.run_cli(helpers.GPU_CLI, root, bsn, "gpu32", extra=["-f_tide", flag],
+                                   env_extra={"SLG_PRECISION": "32"})
+    assert helpers.same_cli_outputs(env_h, rex_h, env_o, rex_o) == []
