@@ -35,6 +35,12 @@ typedef struct slg_batch slg_batch;
  * bnt_line may be NULL if slg_basin_set_ylt supplies the cell latitudes. */
 int slg_basin_open(const char *dta_path, const char *bnt_line, slg_basin **out, char *err,
                    int errlen);
+/* As slg_basin_open, with options: SLG_OPEN_FULL_DEPTH takes the depths as read
+ * (the -fullDepth option, INITAL's MFULLDEP: crdrd2.f:441-444) instead of
+ * capping them at 600 ft. */
+#define SLG_OPEN_FULL_DEPTH 1
+int slg_basin_open_opts(const char *dta_path, const char *bnt_line, int flags, slg_basin **out, char *err,
+                        int errlen);
 /* Cell latitudes YLT (double, [jmxb][imxb], I fastest) from the caller, as the
  * reference's C side fills COMMON /LLXFLE/ before INITAL (savellx.c:757-762). */
 int slg_basin_set_ylt(slg_basin *b, const double *ylt);

@@ -168,7 +168,7 @@ int inital_(int *mhalt, int *imxb, int *jmxb, double zb[BAS_Y][BAS_X], int *hr, 
    FILE *fp;
    long nb;
    (void) env; (void) len_env; (void) ft40; (void) len_ft40; (void) bsn; (void) len_bsn;
-   if ((wave && *wave) || (full_depth && *full_depth)) fatal("unsupported option", "wave/fullDepth");
+   if (wave && *wave) fatal("unsupported option", "wave");
    fstr(dpath, dta, *len_dta);
    fstr(tpath, trk, *len_trk);
    /* a list run (cstart.c:92-245) calls INITAL .. CLNUP once per storm in one
@@ -180,7 +180,9 @@ int inital_(int *mhalt, int *imxb, int *jmxb, double zb[BAS_Y][BAS_X], int *hr, 
    free(G.hbs);
    free(G.hb_last);
    memset(&G, 0, sizeof G);
-   if (slg_basin_open(dpath, NULL, &G.basin, err, sizeof err)) fatal("basin", err);
+   if (slg_basin_open_opts(dpath, NULL, (full_depth && *full_depth) ? SLG_OPEN_FULL_DEPTH : 0, &G.basin, err,
+                           sizeof err))
+      fatal("basin", err);
    slg_basin_dims(G.basin, &ni, &nj, NULL);
    G.imxb = ni;
    G.jmxb = nj;

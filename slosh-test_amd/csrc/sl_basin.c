@@ -559,7 +559,7 @@ static int read_body(frec *r, sl_basin *b, char err[256])
             }
             for (i = ist; i <= ifn; i++) {
                double z = fr_real(r, w, 0);
-               if (z > 600.) z = 600.;       /* MFULLDEP = 0 (crdrd2.f:441-444) */
+               if (z > 600. && !b->full_depth) z = 600.;   /* MFULLDEP = 0 (crdrd2.f:441-444) */
                b->zb[PIX(b, i, j)] = z;
             }
          }
@@ -981,8 +981,16 @@ static void cnstnt(sl_basin *b)
 
 int sl_basin_load(const char *dta_path, sl_basin *b, char err[256])
 {
+   return sl_basin_load_opts(dta_path, b, 0, err);
+}
+
+/* full_depth: the -fullDepth option (INITAL's MFULLDEP, crdrd2.f:441-444): the
+ * depths are taken as read instead of capped at 600 ft. */
+int sl_basin_load_opts(const char *dta_path, sl_basin *b, int full_depth, char err[256])
+{
    frec r;
    memset(b, 0, sizeof *b);
+   b->full_depth = full_depth;
    err[0] = 0;
    r.fp = fopen(dta_path, "r");
    r.eof = 0;

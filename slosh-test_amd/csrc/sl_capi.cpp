@@ -346,10 +346,20 @@ int sl_basin_ensure(slg_basin *B, int prec, char *err, int errlen)
 extern "C" int slg_basin_open(const char *dta, const char *bnt_line, slg_basin **out, char *err,
                               int errlen)
 {
+   return slg_basin_open_opts(dta, bnt_line, 0, out, err, errlen);
+}
+
+extern "C" int slg_basin_open_opts(const char *dta, const char *bnt_line, int flags, slg_basin **out, char *err,
+                                   int errlen)
+{
    char e[256];
    *out = nullptr;
+   if (flags & ~SLG_OPEN_FULL_DEPTH) {
+      seterr(err, errlen, "%s", "unknown basin option");
+      return SLG_EINPUT;
+   }
    slg_basin *B = new slg_basin();
-   if (sl_basin_load(dta, &B->h, e)) {
+   if (sl_basin_load_opts(dta, &B->h, (flags & SLG_OPEN_FULL_DEPTH) != 0, e)) {
       seterr(err, errlen, "%s", e);
       delete B;
       return SLG_EINPUT;

@@ -166,9 +166,10 @@ int inital_(int *mhalt, int *imxb, int *jmxb, double zb[BAS_Y][BAS_X], int *hr, 
    (void) len_bsn;
    char err[512];
    if (wave && *wave) fatal("the wave model is not part of this build", "");
-   if (full_depth && *full_depth) fatal("-fullDepth is not supported by this build", "");
    const std::string dta_path = fstr(dta, *len_dta), trk_path = fstr(trk, *len_trk);
-   if (slg_basin_open(dta_path.c_str(), nullptr, &g_run.basin, err, sizeof err)) fatal("basin", err);
+   if (slg_basin_open_opts(dta_path.c_str(), nullptr, (full_depth && *full_depth) ? SLG_OPEN_FULL_DEPTH : 0,
+                           &g_run.basin, err, sizeof err))
+      fatal("basin", err);
    int ni, nj;
    slg_basin_dims(g_run.basin, &ni, &nj, nullptr);
    if (ni > BAS_X || nj > BAS_Y) fatal("basin larger than the compiled dimensions", dta_path.c_str());

@@ -29,6 +29,7 @@ typedef struct sl_basin {
    char sta[17];
    char ebsn, dollar[3], ebsn1, ebsn2, isouth, nofld, nof1d, iterrain, ismooth;
    int iver, iprj, ismth;
+   int full_depth;                    /* -fullDepth: depths not capped at 600 ft */
    double hter, fsouth;
    double azmth, alto, alno, degree, dela, rmouth, aaxis, baxis, abqab;
    double xmouth, ymouth, dstnt, asmpt, gsize;
@@ -96,6 +97,7 @@ typedef struct sl_storm {
 /* Basin: parse the dta file and derive everything the step needs.
  * Returns 0, or -1 with a message in err (e.g. an unsupported feature). */
 int sl_basin_load(const char *dta_path, sl_basin *b, char err[256]);
+int sl_basin_load_opts(const char *dta_path, sl_basin *b, int full_depth, char err[256]);
 /* Cell latitudes for the Coriolis term from the ?basins.dta line
  * (restates memSetLLx, savellx.c:704-765).  0 on success. */
 int sl_basin_llx(sl_basin *b, const char *bnt_line, char err[256]);

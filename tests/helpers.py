@@ -315,6 +315,17 @@ def tide_basin(kind, hours=6):
     return b, t
 
 
+def full_depth_case(root, hours=6):
+    """A basin with squares deeper than 600 ft (cells of 3-8 km at dt = 20 s keep
+    them within the gravity-wave limit), for -fullDepth (crdrd2.f:441-444)."""
+    b = synth.make_basin("tfd", " ", 30, 26, seed=44, land_frac=0.2, n_barriers=20, dt=20.0,
+                         size_km=(3.0, 8.0), depth_m=300.0)
+    t = synth.straight_track(b, 12.0, 330.0, 60.0, 20.0, hours=hours, landfall=hours // 2)
+    case = make_case(root, b, t)
+    case["b"] = b
+    return case
+
+
 def tide_case(root, idx, hours=6):
     """A case with its tide inputs; returns (case, -f_tide value)."""
     cid, flag, kind = TIDE_CASES[idx]

@@ -106,3 +106,19 @@ def test_tide_changes_the_run(tmp_path):
     case2, flag2 = helpers.tide_case(root2, 2)
     zb = np.array(case2["b"].zb)
     assert (zb >= 290).sum() > 20
+
+
+def test_full_depth_over_oracle64_matches_reference(tmp_path):
+    """-fullDepth: the depths are taken as read instead of capped at 600 ft
+    (crdrd2.f:441-444).  The env and rex files equal the reference CLI's, and
+    differ from the capped run's (the basin has squares deeper than 600 ft)."""
+    import numpy as np
+    root = str(tmp_path)
+    case = helpers.full_depth_case(root)
+    assert (np.array(case["b"].zb) > 600).sum() > 20
+    bsn = case["basin"]
+    env_r, rex_r = helpers.run_cli(helpers.REF_CLI, root, bsn, "ref", extra=["-fullDepth", "1"])
+    env_o, rex_o = helpers.run_cli(helpers.ORC_CLI["64"], root, bsn, "orc", extra=["-fullDepth", "1"])
+    assert helpers.same_cli_outputs(env_r, rex_r, env_o, rex_o) == []
+    env_c, rex_c = helpers.run_cli(helpers.REF_CLI, root, bsn, "capped")
+    assert open(rex_c, "rb").read() != open(rex_r, "rb").read()

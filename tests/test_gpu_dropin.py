@@ -204,3 +204,15 @@ def test_gpu_cli_tide_coupling(idx, tmp_path):
     env_h, rex_h = helpers.run_cli(helpers.GPU_CLI, root, bsn, "gpu32", extra=["-f_tide", flag],
                                    env_extra={"SLG_PRECISION": "32"})
     assert helpers.same_cli_outputs(env_h, rex_h, env_o, rex_o) == []
+
+
+def test_gpu_cli_full_depth_matches_reference_cli(tmp_path):
+    """-fullDepth through the drop-in (depths deeper than 600 ft kept): FP64 env
+    and rex byte-identical to the reference CLI's."""
+    root = str(tmp_path)
+    case = helpers.full_depth_case(root)
+    bsn = case["basin"]
+    env_r, rex_r = helpers.run_cli(helpers.REF_CLI, root, bsn, "ref", extra=["-fullDepth", "1"])
+    env_g, rex_g = helpers.run_cli(helpers.GPU_CLI, root, bsn, "gpu64", extra=["-fullDepth", "1"],
+                                   env_extra={"SLG_PRECISION": "64"})
+    assert helpers.same_cli_outputs(env_g, rex_g, env_r, rex_r) == []
