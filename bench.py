@@ -196,6 +196,33 @@ def batched_run(dta, bnt, trks, prec, steps, warmup, streams, events, path=None,
     return dict(dt=dt, cells=cells, ms=ms, cnt=cnt, sched_s=t_sched, tune=tune, streams=nstreams, windows=wins)
 
 
+def setup_bench(dta, bnt, trks, tracks, prec):
+    """Open and upload of the ensemble three ways: the per-step storm values
+    and the initial water computed on the host and uploaded (SLG_SCHED_HOST);
+    generated on the device from the storm programs; and the same from tracks
+    handed over in memory instead of trk files.  The basin's static fields are
+    uploaded beforehand (untimed), so each line times the runs alone."""
+    B = sloshgpu.Basin(dta, bnt)
+    W = sloshgpu.Batch(B, trks[:1], precision=prec)
+    W.upload()
+    W.close()
+    out = {}
+    for name, kw in (("host", dict(trks=trks, sched_host=True)), ("device", dict(trks=trks)),
+                     ("device_memory_tracks", dict(tracks=tracks))):
+        t0 = time.perf_counter()
+        X = sloshgpu.Batch(B, precision=prec, **kw)
+        t1 = time.perf_counter()
+        X.upload()
+        X.sync()
+        t2 = time.perf_counter()
+        out[name] = {"open_s": t1 - t0, "upload_s": t2 - t1, "total_s": t2 - t0}
+        X.close()
+        log("setup %s: open %.2f s, upload %.2f s" % (name, t1 - t0, t2 - t1))
+    B.close()
+    out["storms"] = len(trks)
+    return out
+
+
 def kernel_roofline(r, prec, storms):
     """Per-kernel average launch time (HIP events on the batch's one stream) and
     achieved algorithmic GB/s against the HBM peak."""
@@ -523,7 +550,7 @@ def main():
     ap.add_argument("--span", type=int, default=0,
                     help="take the timed windows from the first SPAN steps after the warm-up only (0: to the end of "
                          "the run); for short profiling runs")
-    ap.add_argument("--parts", default="batched,kernels,fp32,rex,plain,uneven,serve,cpu",
+    ap.add_argument("--parts", default="batched,kernels,fp32,setup,rex,plain,uneven,serve,cpu",
                     help="comma list of batched, kernels, fp32, rex, plain, uneven, serve, cpu (and copy: "
                          "the counter calibration copy)")
     ap.add_argument("--precision", type=int, default=64, choices=[32, 64],
@@ -638,6 +665,9 @@ def main():
                            "windows": r32["windows"], "tune": r32["tune"],
                            "step_roofline_frac": rate[32] * b_alg_step(a.storms, 32) / (HBM_PEAK_GBS * 1e9)}
             log("batched FP32 %.3e storm.cell.steps/s" % rate[32])
+        if "setup" in parts:
+            tracks = [synth.track_dict(t) for t in synth.meow_tracks(b, n=a.storms)]
+            out["setup"] = setup_bench(paths["dta"], paths["bntline"], trks, tracks, prec)
         if "rex" in parts:
             rx = rex_frames_bench(paths["dta"], paths["bntline"], trks, prec)
             log("rex: %d frames in %.2f ms" % (rx["storms"], rx["ms_per_frame_set"]))

@@ -56,8 +56,10 @@ int slg_basin_field(const slg_basin *b, const char *field, double *dst);
 
 /* Open nstorm runs (1 .. SLG_MAX_STORMS) on one basin, one trk file each
  * (INPUTN2, STERON, INTVAL, SETCMP, INTLHT: inputn2.f, steron.f, intval.f,
- * setcmp.f, intlht.f).  The storm schedule of every run is computed on the
- * host here. */
+ * setcmp.f, intlht.f).  Each run's storm is walked once on the host here
+ * (track, STERON, SETCMP, STMVAL with its FORCE1 table refreshes) into a
+ * compact storm program; slg_batch_upload expands the programs into the
+ * per-step storm values on the device (see SLG_SCHED_HOST). */
 int slg_batch_open(slg_basin *b, int nstorm, const char *const *trk_paths, slg_batch **out,
                    char *err, int errlen);
 /* As slg_batch_open, choosing the arithmetic of the time step: 32 (FP32) or 64
@@ -65,6 +67,71 @@ int slg_batch_open(slg_basin *b, int nstorm, const char *const *trk_paths, slg_b
  * Makefile:15-21; its results reproduce that build bit for bit). */
 int slg_batch_open_prec(slg_basin *b, int nstorm, const char *const *trk_paths, int precision,
                         slg_batch **out, char *err, int errlen);
+/* As slg_batch_open_prec, with flags.  SLG_SCHED_HOST: compute and keep the
+ * per-step storm values (TTIMER's BT, the storm position and the forcing
+ * constants each STMVAL call leaves for the next step: ttimer.f:33-42,
+ * stmval.f:95-236) on the host, as the reference's driver loop produces them,
+ * and upload them; slg_batch_static_height needs this.  Without it they are
+ * generated on the device at upload.  The values are identical either way. */
+#define SLG_SCHED_HOST 1
+int slg_batch_open_opts(slg_basin *b, int nstorm, const char *const *trk_paths, int precision, int flags,
+                        slg_batch **out, char *err, int errlen);
+/* A track in memory: what a FileVersion=2013 trk file holds (inputn2.f:110-605)
+ * without the file.  Rows as the file's rows: hour, latitude (deg N), longitude
+ * (deg E, negative west), pressure deficit (mb; PressureType 0) and RMW (mi);
+ * a row whose hour is 999 ends the track.  wind_factor and rmw_factor: 1 when
+ * a file gives none.  spline: InterpolationType Spline (1) or Linear (0). */
+typedef struct slg_track {
+   int year, month, day, hour_of_day, minute;   /* DateTime */
+   int date_hour, landfall_hour, begin_hour, end_hour;
+   double ocean_datum, lake_datum;              /* ft */
+   double wind_factor, rmw_factor;
+   int spline;
+   int npts;
+   const double *hour, *lat, *lon, *delp, *rmw;  /* [npts] */
+} slg_track;
+/* As slg_batch_open_opts with the tracks in memory (no per-storm files). */
+int slg_batch_open_tracks(slg_basin *b, int nstorm, const slg_track *tracks, int precision, int flags,
+                          slg_batch **out, char *err, int errlen);
+/* MEOW track sets in memory.  The seed points are the landfall points of
+ * parallel tracks heading dir across the basin, apart miles apart, covering
+ * the basin's extent across the heading widened by r34 on both sides, as the
+ * reference's MEOW generator places them (conMeowGenCmd,
+ * sorc/rexout.cd/convert.c:1181-1288; xyflag 0 steps along the basin's
+ * tangent plane, 1 along great circles).  Each track runs straight through
+ * its seed at speed mi/h on the heading, hours 1..100 as the GUI's MEOW tracks
+ * (CreateMEOWTrk, gui/tclsrc/meow.tcl:290-344), passing the seed at
+ * landfall_hour, its points stepped along great circles (Earth_Jump,
+ * convert.c:1127-1151), with constant delp and rmw.  lat/lon of the central
+ * point and of the seeds: degrees, longitude positive west as the basin's
+ * ?basins.dta line holds it.  Needs a basin opened with its bnt line. */
+typedef struct slg_meow {
+   int xyflag;
+   double lat, lon;           /* central seed point */
+   double dir;                /* heading of motion, deg from north */
+   double apart, r34;         /* mi */
+   int num;                   /* at most num tracks (-1: as many as the extent holds) */
+   double speed;              /* mi/h */
+   double delp, rmw;          /* mb, mi */
+   int landfall_hour, begin_hour, end_hour;   /* within 1..100 */
+   double ocean_datum, lake_datum;            /* ft */
+} slg_meow;
+#define SLG_MEOW_HOURS 100
+/* Seed points into idx (the reference's numbering: 0, 1, 2, ..., then -1,
+ * -2, ...), lat, lon [cap]; returns their count or a negative SLG_E*. */
+int slg_meow_seeds(const slg_basin *b, const slg_meow *m, int cap, int *idx, double *lat, double *lon,
+                   char *err, int errlen);
+/* One track per seed, in seed order, into tracks[cap]; rows must hold
+ * cap * 5 * SLG_MEOW_HOURS doubles, which the tracks point into (hour, lat,
+ * lon east positive, delp, rmw).  Returns the count or a negative SLG_E*. */
+int slg_meow_tracks(const slg_basin *b, const slg_meow *m, int cap, slg_track *tracks, double *rows, char *err,
+                    int errlen);
+/* The per-step storm values of one run as its steps read them,
+ * [nsteps][SLG_NPAR] widened to double: from the device after
+ * slg_batch_upload (from_device != 0), or from the host schedule of a batch
+ * opened with SLG_SCHED_HOST. */
+#define SLG_NPAR 26
+int slg_batch_schedule(const slg_batch *x, int storm, int from_device, double *dst, char *err, int errlen);
 int slg_batch_precision(const slg_batch *x);
 /* Steps of one run (MHALT after any time-step change, intval.f:68). */
 int slg_batch_nsteps(const slg_batch *x, int storm);

@@ -1034,7 +1034,7 @@ void sl_basin_free(sl_basin *b)
 #define LL_ARAD 6.378294E6
 #define LL_METPSM 1609.344
 
-static void ll_cl2cnf(double clk, double *cnf, double *scale)
+void sl_cl2cnf(double clk, double *cnf, double *scale)
 {
    double s = sin(clk * LL_RADPDG), t0 = LL_ECCEN * s;
    double t1 = (1. - s) * pow((1. + t0), LL_ECCEN);
@@ -1043,7 +1043,7 @@ static void ll_cl2cnf(double clk, double *cnf, double *scale)
    *scale = pow((1 - t0 * t0), ((1 + LL_ECCEN) * .5)) * 2. / (t1 + t2);
 }
 
-static void ll_cnf2cl(double cnf, double *clk)
+void sl_cnf2cl(double cnf, double *clk)
 {
    double s, t0, t1, t2, tl, grad;
    int i;
@@ -1119,8 +1119,8 @@ int sl_basin_llx(sl_basin *b, const char *line, char err[256])
    yjmax = ll_fatof(s + 98, 5, 1);
    if (type == 'h') elipty = ll_fatof(s + 123, 6, 5);
    /* tangent plane at G on the conformal sphere */
-   ll_cl2cnf(plat, &plat, &tmp);
-   ll_cl2cnf(glat, &glat, &tscald);
+   sl_cl2cnf(plat, &plat, &tmp);
+   sl_cl2cnf(glat, &glat, &tscald);
    stl = sin(LL_RADPDG * glat);
    ctl = cos(LL_RADPDG * glat);
    {
@@ -1155,6 +1155,23 @@ int sl_basin_llx(sl_basin *b, const char *line, char err[256])
       t1 = gsize / (2. * rg2p);
       delrg = 2. * log(sqrt(1.0 + t1 * t1) + t1);
    }
+   b->proj.type = type;
+   b->proj.glat = glat;
+   b->proj.glon = glon;
+   b->proj.tscald = tscald;
+   b->proj.stl = stl;
+   b->proj.ctl = ctl;
+   b->proj.xppt = xppt;
+   b->proj.yppt = yppt;
+   b->proj.rg2p = rg2p;
+   b->proj.thtg2p = thtg2p;
+   b->proj.abq = abq;
+   b->proj.delrg = delrg;
+   b->proj.xig = xig;
+   b->proj.yjg = yjg;
+   b->proj.ximax = ximax;
+   b->proj.yjmax = yjmax;
+   b->proj.have = 1;
    if ((int) ximax != b->imxb || (int) yjmax != b->jmxb) {
       snprintf(err, 256, "bnt line grid %g x %g does not match the dta grid %d x %d",
                ximax, yjmax, b->imxb, b->jmxb);
@@ -1163,33 +1180,62 @@ int sl_basin_llx(sl_basin *b, const char *line, char err[256])
    for (j = 0; j < b->jmxb; j++)
       for (i = 0; i < b->imxb; i++) {
          /* pq2xy then sxy2ll then cnf2cl (savellx.c:195-227, 146-183, 68-92) */
-         double xa = ((i + 1) - xig) * delrg, ya = (yjg - (j + 1)) * delrg;
-         double ex = exp(xa), zr = ex * cos(ya), zi = ex * sin(ya);
-         double den = zr * zr + zi * zi, ir = zr / den, ii = -zi / den;
-         double z2r = rg2p * (zr + abq * ir), z2i = rg2p * (zi + abq * ii);
-         double x1 = z2i, y1 = z2r, cs = cos(thtg2p * LL_RADPDG), ss = sin(thtg2p * LL_RADPDG);
-         double x = xppt + x1 * cs + y1 * ss, y = yppt - x1 * ss + y1 * cs;
-         double xp = x * tscald, yp = y * tscald, rr = sqrt(xp * xp + yp * yp);
-         double sn, cn, tt, tt2, sp, cp, lat;
-         if (rr > 0) {
-            sn = xp / rr;
-            cn = yp / rr;
-         } else {
-            sn = 0;
-            cn = 1;
-         }
-         tt = 0.5 * rr * LL_METPSM / LL_ARAD;
-         tt2 = tt * tt;
-         sp = 2. * tt / (1. + tt2);
-         cp = (1. - tt2) / (1. + tt2);
-         xp = cp * ctl - sp * stl * cn;
-         yp = -1 * sp * sn;
-         rr = sqrt(xp * xp + yp * yp);
-         tt = cp * stl + sp * ctl * cn;
-         lat = atan2(tt, rr) / LL_RADPDG;
-         ll_cnf2cl(lat, &lat);
+         double x, y, lat, lon;
+         sl_proj_pq2xy(b, i + 1, j + 1, &x, &y);
+         sl_proj_sxy2ll(b, x, y, &lat, &lon);
+         sl_cnf2cl(lat, &lat);
          b->ylt[PIX(b, i + 1, j + 1)] = lat;
       }
    b->have_ylt = 1;
    return 0;
+}
+
+/* pq2xy, savellx.c:195-227: Z = exp(xa + i ya), Z2 = RG2P (Z + ABQ / Z) */
+void sl_proj_pq2xy(const sl_basin *b, double p, double q, double *x, double *y)
+{
+   const double xa = (p - b->proj.xig) * b->proj.delrg, ya = (b->proj.yjg - q) * b->proj.delrg;
+   const double ex = exp(xa), zr = ex * cos(ya), zi = ex * sin(ya);
+   const double den = zr * zr + zi * zi, ir = zr / den, ii = -zi / den;
+   const double z2r = b->proj.rg2p * (zr + b->proj.abq * ir), z2i = b->proj.rg2p * (zi + b->proj.abq * ii);
+   const double x1 = z2i, y1 = z2r;
+   const double cs = cos(b->proj.thtg2p * LL_RADPDG), ss = sin(b->proj.thtg2p * LL_RADPDG);
+   *x = b->proj.xppt + x1 * cs + y1 * ss;
+   *y = b->proj.yppt - x1 * ss + y1 * cs;
+}
+
+/* sxy2ll, savellx.c:146-183: plane -> conformal lat/lon */
+void sl_proj_sxy2ll(const sl_basin *b, double x, double y, double *lat, double *lon)
+{
+   double xp = x * b->proj.tscald, yp = y * b->proj.tscald, rr = sqrt(xp * xp + yp * yp);
+   double sn, cn, tt, tt2, sp, cp;
+   if (rr > 0) {
+      sn = xp / rr;
+      cn = yp / rr;
+   } else {
+      sn = 0;
+      cn = 1;
+   }
+   tt = 0.5 * rr * LL_METPSM / LL_ARAD;
+   tt2 = tt * tt;
+   sp = 2. * tt / (1. + tt2);
+   cp = (1. - tt2) / (1. + tt2);
+   xp = cp * b->proj.ctl - sp * b->proj.stl * cn;
+   yp = -1 * sp * sn;
+   rr = sqrt(xp * xp + yp * yp);
+   *lon = (rr > 0) ? b->proj.glon + atan2(yp, xp) / LL_RADPDG : b->proj.glon;
+   tt = cp * b->proj.stl + sp * b->proj.ctl * cn;
+   *lat = atan2(tt, rr) / LL_RADPDG;
+}
+
+/* sll2xy, savellx.c:113-129: conformal lat/lon -> plane */
+void sl_proj_sll2xy(const sl_basin *b, double lat, double lon, double *x, double *y)
+{
+   const double sl = sin(lat * LL_RADPDG), cl = cos(lat * LL_RADPDG);
+   const double dl = lon - b->proj.glon, cdl = cos(dl * LL_RADPDG);
+   const double q1 = 1. + sl * b->proj.stl + cl * b->proj.ctl * cdl;
+   const double q2 = -1 * cl * sin(dl * LL_RADPDG);
+   const double q3 = sl * b->proj.ctl - cl * b->proj.stl * cdl;
+   const double q4 = ((LL_ARAD / LL_METPSM) * (2. / q1) / b->proj.tscald);
+   *x = q4 * q2;
+   *y = q4 * q3;
 }

@@ -5,8 +5,11 @@
 // that basin with their fields resident on the device.  Each storm's forcing
 // depends only on its track, so the whole per-step storm schedule (the values
 // STMVAL leaves for the next step, and every radial table FORCE1 produces) is
-// computed on the host up front and uploaded once; stepping then needs no host
-// round trip.
+// fixed before the first step.  The host walks each storm once (track,
+// STERON, SETCMP, STMVAL with its FORCE1 table refreshes) and keeps a compact
+// storm program: the state after each STMVAL call that changed it.  At upload
+// the device expands the programs into the per-step values of every run
+// (k_sched); stepping then needs no host round trip.
 #include <map>
 #include <mutex>
 #include "sl_int.h"
@@ -480,12 +483,55 @@ static void push_tables(storm_sched &S, const sl_storm *s, int step, int prec)
 template <class R>
 static void step_params(const slg_basin *B, const double *r, int tv, R *P);
 
+// the storm state the per-step values come from (SQ_* order, sl_dev.h)
+static void sched_state(const sl_storm *s, int tv, double *q)
+{
+   q[SQ_DELT] = s->delt;
+   q[SQ_X1B0] = s->x1b[0];
+   q[SQ_X1B1] = s->x1b[1];
+   q[SQ_X1B3] = s->x1b[3];
+   q[SQ_X1B4] = s->x1b[4];
+   q[SQ_X1B5] = s->x1b[5];
+   q[SQ_C1] = s->c1;
+   q[SQ_C2] = s->c2;
+   q[SQ_X4] = s->x12[4];
+   q[SQ_X9] = s->x12[9];
+   q[SQ_X10] = s->x12[10];
+   q[SQ_X11] = s->x12[11];
+   q[SQ_X12] = s->x12[12];
+   q[SQ_X15] = s->x12[15];
+   q[SQ_X20] = s->x12[20];
+   q[SQ_X21] = s->x12[21];
+   q[SQ_X23] = s->x12[23];
+   q[SQ_X24] = s->x12[24];
+   q[SQ_W0] = s->w1218[0];
+   q[SQ_W1] = s->w1218[1];
+   q[SQ_SEA] = s->seadtm;
+   q[SQ_TV] = tv;
+   q[SQ_AX] = s->ax;
+   q[SQ_AY] = s->ay;
+}
+
+static double ttimer_bt(int itime, int ic19);
+
 int sl_make_schedule(const slg_basin *B, const char *trk, storm_sched &S, bool keep_rows, int prec,
-                     char *err, int errlen)
+                     char *err, int errlen, bool hostp, const slg_track *track)
 {
    char e[256];
-   double row[90];
-   if (sl_storm_load(trk, &B->h, &S.s0, e)) {
+   double row[90], q[SQ_N], q0[SQ_N];
+   int rc;
+   if (track) {
+      sl_trk *t = (sl_trk *) malloc(sizeof *t);
+      if (!t) {
+         seterr(err, errlen, "%s", "out of host memory");
+         return SLG_EINPUT;
+      }
+      rc = sl_track_from_mem(track, t, e);
+      if (!rc) rc = sl_storm_init(t, &B->h, &S.s0, e);
+      free(t);
+   } else
+      rc = sl_storm_load(trk, &B->h, &S.s0, e);
+   if (rc) {
       seterr(err, errlen, "%s", e);
       return SLG_EINPUT;
    }
@@ -496,20 +542,33 @@ int sl_make_schedule(const slg_basin *B, const char *trk, storm_sched &S, bool k
    S.t32.clear();
    S.t64.clear();
    S.tab_step.clear();
+   S.prog.clear();
+   S.hostp = hostp;
    storm_row(&s, &B->h, row);
    if (keep_rows) S.rows.insert(S.rows.end(), row, row + 90);
    push_tables(S, &s, 0, prec);
-   int gen = s.table_gen, n = 0;
-   if (prec == 64) S.p64.reserve((size_t) (s.mhalt + 1) * SL_NPAR);
-   else S.p32.reserve((size_t) (s.mhalt + 1) * SL_NPAR);
+   // storm program: header, the state after SETCMP, TTIMER's ramp; events follow
+   const int ic19 = s.ic19;
+   S.prog.assign(SQ_HDR, 0.);
+   S.prog[1] = ic19;
+   S.prog[3] = s.dtmlak;
+   sched_state(&s, 0, q0);
+   S.prog.insert(S.prog.end(), q0, q0 + SQ_N);
+   S.prog.insert(S.prog.end(), s.DELP, s.DELP + SL_NTABV);   // INTLHT's table (the first hour's)
+   for (int it = 1; it < ic19; it++) S.prog.push_back(ttimer_bt(it, ic19));
+   int gen = s.table_gen, n = 0, nev = 0;
+   if (hostp) {
+      if (prec == 64) S.p64.reserve((size_t) (s.mhalt + 1) * SL_NPAR);
+      else S.p32.reserve((size_t) (s.mhalt + 1) * SL_NPAR);
+   }
    while (s.itime < s.mhalt) {
       // values step n+1 reads: the row after step n, and the table set in force
       const int tv = (int) S.tab_step.size() - 1;
-      if (prec == 64) {
+      if (hostp && prec == 64) {
          size_t o = S.p64.size();
          S.p64.resize(o + SL_NPAR);
          step_params(B, row, tv, &S.p64[o]);
-      } else {
+      } else if (hostp) {
          size_t o = S.p32.size();
          S.p32.resize(o + SL_NPAR);
          step_params(B, row, tv, &S.p32[o]);
@@ -520,28 +579,45 @@ int sl_make_schedule(const slg_basin *B, const char *trk, storm_sched &S, bool k
          return SLG_EINPUT;
       }
       n++;
-      storm_row(&s, &B->h, row);
+      if (keep_rows || hostp) storm_row(&s, &B->h, row);
       if (keep_rows) S.rows.insert(S.rows.end(), row, row + 90);
       if (s.table_gen != gen) {
          push_tables(S, &s, n, prec);
          gen = s.table_gen;
       }
+      // an event wherever this STMVAL call changed more than AX and AY
+      sched_state(&s, (int) S.tab_step.size() - 1, q);
+      if (memcmp(q, q0, sizeof(double) * SQ_AX)) {
+         S.prog.push_back(n);
+         S.prog.insert(S.prog.end(), q, q + SQ_AX);
+         memcpy(q0, q, sizeof q0);
+         nev++;
+      }
    }
    S.nsteps = n;
+   S.prog[0] = n;
+   S.prog[2] = nev;
    return 0;
 }
 
 // device parameters of a step from the ref-format row left by the step before
 // (the storm schedule is computed in double and rounded to R where it is used)
+// TTIMER's ramp, ttimer.f:36-41
+static double ttimer_bt(int itime, int ic19)
+{
+   double bt = 1.;
+   if (itime < ic19) {
+      double time = itime, ang = time / ic19;
+      bt = .5 * (1. - cos(ang * 3.14159265358979323846));
+   }
+   return bt;
+}
+
 template <class R>
 static void step_params(const slg_basin *B, const double *r, int tv, R *P)
 {
    int itime = (int) r[0] + 1, ic19 = (int) r[27];
-   double bt = 1.;
-   if (itime < ic19) {   // TTIMER, ttimer.f:36-41
-      double time = itime, ang = time / ic19;
-      bt = .5 * (1. - cos(ang * 3.14159265358979323846));
-   }
+   const double bt = ttimer_bt(itime, ic19);
    for (int k = 0; k < SL_NPAR; k++) P[k] = R(0);
    P[P_BT] = (R) bt;
    P[P_DELT] = (R) r[2];
@@ -577,10 +653,51 @@ static void step_params(const slg_basin *B, const double *r, int tv, R *P)
 // --------------------------------------------------------------------------
 // batch
 
-extern "C" int slg_batch_open_prec(slg_basin *B, int nstorm, const char *const *trk, int precision,
-                                   slg_batch **out, char *err, int errlen)
+// A track given in memory as the rows INPUTN2 would read from a file
+// (inputn2.f:560-605; PressureType 0: the deficit itself), with its header.
+extern "C" int sl_track_from_mem(const slg_track *m, sl_trk *t, char e[256])
+{
+   sl_trk_init(t);
+   e[0] = 0;
+   if (!m || m->npts < 0 || (m->npts > 0 && (!m->hour || !m->lat || !m->lon || !m->delp || !m->rmw))) {
+      snprintf(e, 256, "track: rows missing");
+      return -1;
+   }
+   t->date_yr = m->year;
+   t->date_mo = m->month;
+   t->date_dy = m->day;
+   t->date_hr = m->hour_of_day;
+   t->date_mn = m->minute;
+   t->dhr = m->date_hour;
+   t->land = m->landfall_hour;
+   t->beg = m->begin_hour;
+   t->end = m->end_hour;
+   t->ocean = m->ocean_datum;
+   t->lake = m->lake_datum;
+   t->windf = m->wind_factor;
+   t->rmwf = m->rmw_factor;
+   t->spline = m->spline != 0;
+   for (int k = 0; k < m->npts; k++) {
+      if ((int) m->hour[k] == 999) break;   // the file's terminator row
+      const int rc = sl_trk_row(t, m->hour[k], m->lat[k], m->lon[k], m->delp[k], m->rmw[k], e);
+      if (rc < 0) return -1;
+      if (rc > 0) break;
+   }
+   return 0;
+}
+
+static int batch_open(slg_basin *B, int nstorm, const char *const *trk, const slg_track *tracks, int precision,
+                      int flags, slg_batch **out, char *err, int errlen)
 {
    *out = nullptr;
+   if (!B || (!trk && !tracks)) {
+      seterr(err, errlen, "%s", "a basin and nstorm tracks");
+      return SLG_EINPUT;
+   }
+   if (flags & ~SLG_SCHED_HOST) {
+      seterr(err, errlen, "%s", "unknown batch flags");
+      return SLG_EINPUT;
+   }
    if (nstorm < 1 || nstorm > SLG_MAX_STORMS) {
       seterr(err, errlen, "nstorm must be 1 .. %s%d (storms index a launch grid dimension)", "", SLG_MAX_STORMS);
       return SLG_EINPUT;
@@ -607,7 +724,8 @@ extern "C" int slg_batch_open_prec(slg_basin *B, int nstorm, const char *const *
             char e[512];
             for (int s = t; s < nstorm; s += nt) {
                e[0] = 0;
-               rcs[s] = sl_make_schedule(B, trk[s], X->sc[s], nstorm <= 8, precision, e, sizeof e);
+               rcs[s] = sl_make_schedule(B, trk ? trk[s] : nullptr, X->sc[s], nstorm <= 8, precision, e,
+                                         sizeof e, (flags & SLG_SCHED_HOST) != 0, tracks ? &tracks[s] : nullptr);
                if (rcs[s]) msgs[s] = e;
             }
          });
@@ -621,10 +739,13 @@ extern "C" int slg_batch_open_prec(slg_basin *B, int nstorm, const char *const *
          if (X->sc[s].nsteps > X->maxsteps) X->maxsteps = X->sc[s].nsteps;
       }
    }
-   // initial water (INTLHT) of every run, kept in double on the host
-   const size_t np = (size_t) B->h.imxb * B->h.jmxb;
-   X->hb0.resize((size_t) nstorm * np);
-   for (int s = 0; s < nstorm; s++) sl_initial_water(&B->h, &X->sc[s].s0, &X->hb0[(size_t) s * np]);
+   // initial water (INTLHT) of every run: kept in double on the host with the
+   // host schedule, else computed on the device at upload (k_intlht)
+   if (flags & SLG_SCHED_HOST) {
+      const size_t np = (size_t) B->h.imxb * B->h.jmxb;
+      X->hb0.resize((size_t) nstorm * np);
+      for (int s = 0; s < nstorm; s++) sl_initial_water(&B->h, &X->sc[s].s0, &X->hb0[(size_t) s * np]);
+   }
    X->step = 0;
    X->parity = 0;
    X->path = SLG_PATH_FUSED;
@@ -633,6 +754,24 @@ extern "C" int slg_batch_open_prec(slg_basin *B, int nstorm, const char *const *
    X->last_par.assign(nstorm, 0);
    *out = X;
    return 0;
+}
+
+extern "C" int slg_batch_open_opts(slg_basin *B, int nstorm, const char *const *trk, int precision, int flags,
+                                   slg_batch **out, char *err, int errlen)
+{
+   return batch_open(B, nstorm, trk, nullptr, precision, flags, out, err, errlen);
+}
+
+extern "C" int slg_batch_open_tracks(slg_basin *B, int nstorm, const slg_track *tracks, int precision, int flags,
+                                     slg_batch **out, char *err, int errlen)
+{
+   return batch_open(B, nstorm, nullptr, tracks, precision, flags, out, err, errlen);
+}
+
+extern "C" int slg_batch_open_prec(slg_basin *B, int nstorm, const char *const *trk, int precision,
+                                   slg_batch **out, char *err, int errlen)
+{
+   return batch_open(B, nstorm, trk, nullptr, precision, 0, out, err, errlen);
 }
 
 extern "C" int slg_batch_open(slg_basin *B, int nstorm, const char *const *trk, slg_batch **out,
@@ -657,6 +796,130 @@ extern "C" int slg_batch_set_path(slg_batch *X, int path, int tile_rows)
    X->tile_rows = tile_rows;
    drop_graph(X);
    return 0;
+}
+
+// --------------------------------------------------------------------------
+// MEOW track sets
+
+static int meow_check(const slg_basin *B, const slg_meow *m, int cap, char *err, int errlen)
+{
+   if (!B || !m || cap < 1) {
+      seterr(err, errlen, "%s", "a basin, MEOW parameters and room for at least one track");
+      return SLG_EINPUT;
+   }
+   if (!B->h.proj.have) {
+      seterr(err, errlen, "%s", "MEOW seeds need the basin's ?basins.dta line (slg_basin_open with bnt_line)");
+      return SLG_EINPUT;
+   }
+   if (!(m->apart > 0.) || !(m->r34 >= 0.) || (m->xyflag != 0 && m->xyflag != 1) || m->num == 0 ||
+       m->num < -1) {
+      seterr(err, errlen, "%s", "MEOW: apart > 0, r34 >= 0, xyflag 0 or 1, num -1 or positive");
+      return SLG_EINPUT;
+   }
+   return 0;
+}
+
+extern "C" int slg_meow_seeds(const slg_basin *B, const slg_meow *m, int cap, int *idx, double *lat, double *lon,
+                              char *err, int errlen)
+{
+   int rc = meow_check(B, m, cap, err, errlen);
+   if (rc) return rc;
+   const int n = sl_meow_seeds(&B->h, m->xyflag, m->lat, m->lon, m->dir, m->apart, m->num, m->r34, cap, idx, lat,
+                               lon);
+   if (n < 0) {
+      seterr(err, errlen, "MEOW: more seed points than the %s%d given room for", "", cap);
+      return SLG_EINPUT;
+   }
+   return n;
+}
+
+extern "C" int slg_meow_tracks(const slg_basin *B, const slg_meow *m, int cap, slg_track *tracks, double *rows,
+                               char *err, int errlen)
+{
+   int rc = meow_check(B, m, cap, err, errlen);
+   if (rc) return rc;
+   if (!tracks || !rows || !(m->speed >= 0.) || m->landfall_hour < 1 || m->landfall_hour > SLG_MEOW_HOURS ||
+       m->begin_hour < 1 || m->end_hour > SLG_MEOW_HOURS - 1 || m->begin_hour >= m->end_hour) {
+      seterr(err, errlen, "MEOW tracks: speed >= 0 and 1 <= begin < end < landfall ... hours within 1..%s%d", "",
+             SLG_MEOW_HOURS);
+      return SLG_EINPUT;
+   }
+   std::vector<int> idx(cap);
+   std::vector<double> la(cap), lo(cap);
+   const int n = sl_meow_seeds(&B->h, m->xyflag, m->lat, m->lon, m->dir, m->apart, m->num, m->r34, cap, idx.data(),
+                               la.data(), lo.data());
+   if (n < 0) {
+      seterr(err, errlen, "MEOW: more tracks than the %s%d given room for", "", cap);
+      return SLG_EINPUT;
+   }
+   const int H = SLG_MEOW_HOURS;
+   for (int t = 0; t < n; t++) {
+      double *h = rows + (size_t) t * 5 * H, *a = h + H, *o = a + H, *d = o + H, *r = d + H;
+      for (int j = 1; j <= H; j++) {
+         // the point speed * (j - landfall) miles along the heading from the
+         // seed (CreateMEOWTrk, meow.tcl:327-331), longitude east positive
+         double lat2, lon2;
+         sl_earth_jump(la[t], -lo[t], m->speed * (j - m->landfall_hour), m->dir, &lat2, &lon2);
+         h[j - 1] = j;
+         a[j - 1] = lat2;
+         o[j - 1] = lon2;
+         d[j - 1] = m->delp;
+         r[j - 1] = m->rmw;
+      }
+      slg_track &T = tracks[t];
+      memset(&T, 0, sizeof T);
+      T.year = 2000;
+      T.month = 1;
+      T.day = 1;
+      T.date_hour = 1;
+      T.landfall_hour = m->landfall_hour;
+      T.begin_hour = m->begin_hour;
+      T.end_hour = m->end_hour;
+      T.ocean_datum = m->ocean_datum;
+      T.lake_datum = m->lake_datum;
+      T.wind_factor = 1.;
+      T.rmw_factor = 1.;
+      T.spline = 0;
+      T.npts = H;
+      T.hour = h;
+      T.lat = a;
+      T.lon = o;
+      T.delp = d;
+      T.rmw = r;
+   }
+   return n;
+}
+
+template <class R>
+static int batch_schedule(const slg_batch *X, int storm, int from_device, double *dst, char *err, int errlen)
+{
+   const storm_sched &S = X->sc[storm];
+   const size_t n = (size_t) S.nsteps * SL_NPAR;
+   std::vector<R> v(n);
+   if (from_device) {
+      const SlRunDevT<R> &D = X->prec == 64 ? *(const SlRunDevT<R> *) &X->d64 : *(const SlRunDevT<R> *) &X->d32;
+      HIPOK(hipMemcpy(v.data(), D.params + (size_t) storm * D.maxsteps * SL_NPAR, n * sizeof(R),
+                      hipMemcpyDeviceToHost));
+   } else {
+      if (!S.hostp) {
+         seterr(err, errlen, "%s", "the host keeps no per-step values (open with SLG_SCHED_HOST)");
+         return SLG_EINPUT;
+      }
+      memcpy(v.data(), sched_params(S, R()).data(), n * sizeof(R));
+   }
+   for (size_t k = 0; k < n; k++) dst[k] = (double) v[k];
+   return 0;
+}
+
+extern "C" int slg_batch_schedule(const slg_batch *X, int storm, int from_device, double *dst, char *err,
+                                  int errlen)
+{
+   if (!X || storm < 0 || storm >= X->nstorm || !dst || (from_device && !X->dmem)) {
+      seterr(err, errlen, "%s", "an uploaded batch (from_device) and a storm of it");
+      return SLG_EINPUT;
+   }
+   return X->prec == 64 ? batch_schedule<double>(X, storm, from_device, dst, err, errlen)
+                        : batch_schedule<float>(X, storm, from_device, dst, err, errlen);
 }
 
 extern "C" int slg_batch_nsteps(const slg_batch *X, int storm)
@@ -734,28 +997,84 @@ static int batch_upload(slg_batch *X, char *err, int errlen)
    D.vbudget = 0;
    D.sgroup = SL_SGROUP;
    HIPOK(hipMemset(d, 0, tot));
-   // initial water (INTLHT) per storm: HB, HMX=HB, UB=VB=0
-   std::vector<R> f(fstride, R(0));
-   std::vector<R> params((size_t) S * X->maxsteps * SL_NPAR, R(0));
+   // initial water (INTLHT) per storm: HB, HMX=HB, UB=VB=0 (from the host's,
+   // or on the device below)
+   const bool hostw = !X->hb0.empty();
+   std::vector<R> f(hostw ? fstride : 0, R(0));
+   std::vector<R> params(X->sc[0].hostp ? (size_t) S * X->maxsteps * SL_NPAR : 0, R(0));
    std::vector<R> tables;
    std::vector<int> tbase(S), nst(S);
    for (int s = 0; s < S; s++) {
-      const double *hb = &X->hb0[(size_t) s * b->imxb * NJ];
-      for (int j = 1; j <= NJ; j++)
-         for (int i = 1; i <= b->imxb; i++) f[(size_t) (j - 1) * P + (i - 1)] = (R) hb[PIX(b, i, j)];
-      HIPOK(hipMemcpy(D.hb0 + (size_t) s * fstride, f.data(), fstride * rs, hipMemcpyHostToDevice));
-      HIPOK(hipMemcpy(D.hb1 + (size_t) s * fstride, f.data(), fstride * rs, hipMemcpyHostToDevice));
-      for (int j = 1; j <= NJ; j++)       // HMX = HB on squares only (intlht.f:172-177)
-         for (int i = 1; i <= b->imxb; i++)
-            if (i == b->imxb || j == NJ) f[(size_t) (j - 1) * P + (i - 1)] = R(0);
-      HIPOK(hipMemcpy(D.hmx + (size_t) s * fstride, f.data(), fstride * rs, hipMemcpyHostToDevice));
+      if (hostw) {
+         const double *hb = &X->hb0[(size_t) s * b->imxb * NJ];
+         for (int j = 1; j <= NJ; j++)
+            for (int i = 1; i <= b->imxb; i++) f[(size_t) (j - 1) * P + (i - 1)] = (R) hb[PIX(b, i, j)];
+         HIPOK(hipMemcpy(D.hb0 + (size_t) s * fstride, f.data(), fstride * rs, hipMemcpyHostToDevice));
+         HIPOK(hipMemcpy(D.hb1 + (size_t) s * fstride, f.data(), fstride * rs, hipMemcpyHostToDevice));
+         for (int j = 1; j <= NJ; j++)       // HMX = HB on squares only (intlht.f:172-177)
+            for (int i = 1; i <= b->imxb; i++)
+               if (i == b->imxb || j == NJ) f[(size_t) (j - 1) * P + (i - 1)] = R(0);
+         HIPOK(hipMemcpy(D.hmx + (size_t) s * fstride, f.data(), fstride * rs, hipMemcpyHostToDevice));
+      }
       const std::vector<R> &sp = sched_params(X->sc[s], R()), &st = sched_tables(X->sc[s], R());
-      memcpy(&params[(size_t) s * X->maxsteps * SL_NPAR], sp.data(), sp.size() * sizeof(R));
+      if (X->sc[s].hostp) memcpy(&params[(size_t) s * X->maxsteps * SL_NPAR], sp.data(), sp.size() * sizeof(R));
       tbase[s] = (int) (tables.size() / ((size_t) SL_NTABV * 8));
       tables.insert(tables.end(), st.begin(), st.end());
       nst[s] = X->sc[s].nsteps;
    }
-   HIPOK(hipMemcpy(d + off[12], params.data(), params.size() * rs, hipMemcpyHostToDevice));
+   if (X->sc[0].hostp) HIPOK(hipMemcpy(d + off[12], params.data(), params.size() * rs, hipMemcpyHostToDevice));
+   if (!X->sc[0].hostp || !hostw) {
+      // the storm programs to the device, with (for the initial water) the
+      // basin metrics ELPCL ELPDL COSL SINL, ZB and the squares in reach of
+      // ocean winds, all in double; the per-step values and the initial water
+      // are generated from them there
+      std::vector<long> poff(S);
+      size_t np2 = 0;
+      for (int s = 0; s < S; s++) {
+         poff[s] = (long) np2;
+         np2 += X->sc[s].prog.size();
+      }
+      const int ni = b->imxb, nj = b->jmxb;
+      const size_t nb = 2 * (size_t) ni + 2 * (size_t) nj + (size_t) ni * nj;
+      std::vector<double> buf;
+      buf.reserve(np2 + nb);
+      for (int s = 0; s < S; s++) buf.insert(buf.end(), X->sc[s].prog.begin(), X->sc[s].prog.end());
+      buf.insert(buf.end(), b->elpcl, b->elpcl + ni);
+      buf.insert(buf.end(), b->elpdl, b->elpdl + ni);
+      buf.insert(buf.end(), b->cosl, b->cosl + nj);
+      buf.insert(buf.end(), b->sinl, b->sinl + nj);
+      buf.insert(buf.end(), b->zb, b->zb + (size_t) ni * nj);
+      std::vector<unsigned char> ocn((size_t) ni * nj, 0);
+      for (int j = 1; j <= b->jmxb1; j++)     // intlht.f:59-62
+         for (int i = 1; i <= b->imxb1; i++) {
+            const char *it = b->itree;
+            const char c[4] = {it[PIX(b, i, j)], it[PIX(b, i + 1, j)], it[PIX(b, i, j + 1)],
+                               it[PIX(b, i + 1, j + 1)]};
+            for (int k = 0; k < 4; k++)
+               if (c[k] == '2' || c[k] == '5') ocn[PIX(b, i, j)] = 1;
+         }
+      char *pd = nullptr;
+      const size_t nbytes = buf.size() * sizeof(double) + (size_t) S * sizeof(long) + ocn.size();
+      HIPOK(hipMalloc((void **) &pd, nbytes));
+      long *offd = (long *) (pd + buf.size() * sizeof(double));
+      unsigned char *ocnd = (unsigned char *) (offd + S);
+      hipError_t he = hipMemcpy(pd, buf.data(), buf.size() * sizeof(double), hipMemcpyHostToDevice);
+      if (he == hipSuccess) he = hipMemcpy(offd, poff.data(), (size_t) S * sizeof(long), hipMemcpyHostToDevice);
+      if (he == hipSuccess) he = hipMemcpy(ocnd, ocn.data(), ocn.size(), hipMemcpyHostToDevice);
+      int krc = he == hipSuccess ? 0 : -(int) he;
+      if (!krc && !X->sc[0].hostp)
+         krc = dev_sched((const double *) pd, offd, S, (R *) (d + off[12]), X->maxsteps, B->ismpt2g, B->incsm,
+                         b->ebsn1 == '+', 0);
+      if (!krc && !hostw)
+         krc = dev_intlht((const double *) pd, offd, (const double *) pd + np2, ocnd, ni, nj, b->imxb1, b->jmxb1, P,
+                          &D, 0);
+      if (!krc) krc = -(int) hipDeviceSynchronize();
+      (void) hipFree(pd);
+      if (krc) {
+         seterr(err, errlen, "%s", "device storm schedule / initial water failed");
+         return SLG_EHIP;
+      }
+   }
    HIPOK(hipMemcpy(d + off[13], tables.data(), tables.size() * rs, hipMemcpyHostToDevice));
    HIPOK(hipMemcpy(d + off[14], tbase.data(), (size_t) S * 4, hipMemcpyHostToDevice));
    HIPOK(hipMemcpy(d + off[15], nst.data(), (size_t) S * 4, hipMemcpyHostToDevice));
@@ -1273,6 +1592,7 @@ extern "C" int slg_batch_static_height(const slg_batch *X, int storm, int i, int
    if (storm < 0 || storm >= X->nstorm || i < 1 || i > b->imxb || j < 1 || j > b->jmxb ||
        !(X->prec == 64 ? X->b->i64.built : X->b->i32.built))
       return SLG_EINPUT;
+   if (!X->sc[storm].hostp) return SLG_EUNSUP;   // the values live on the device (open with SLG_SCHED_HOST)
    *ht = X->prec == 64 ? static_height<double>(X, storm, i, j) : static_height<float>(X, storm, i, j);
    return 0;
 }
@@ -1415,9 +1735,10 @@ extern "C" int slg_batch_get(slg_batch *X, int storm, const char *name, double *
       seterr(err, errlen, "storm %s%d out of range", "", storm);
       return SLG_EINPUT;
    }
-   if (!strcmp(name, "HB0")) {   // initial water (INTLHT), double, kept on the host
+   if (!strcmp(name, "HB0")) {   // initial water (INTLHT), double, on the host
       const size_t np = (size_t) b->imxb * b->jmxb;
-      memcpy(dst, &X->hb0[(size_t) storm * np], np * sizeof(double));
+      if (X->hb0.empty()) sl_initial_water(b, &X->sc[storm].s0, dst);
+      else memcpy(dst, &X->hb0[(size_t) storm * np], np * sizeof(double));
       return 0;
    }
    if (!X->dmem) {
@@ -2014,8 +2335,12 @@ extern "C" int slg_export_init(const slg_batch *X, int storm, const char *path)
    rec(fp, "IMANN", 'i', (long) np, b->imann);
    rec(fp, "KSKP", 'c', (long) np, b->kskp);
    rec(fp, "YLT", 'd', (long) np, b->ylt);
-   const double *hb = &X->hb0[(size_t) storm * np];
-   std::vector<double> z(np, 0.);
+   std::vector<double> z(np, 0.), w;
+   if (X->hb0.empty()) {
+      w.resize(np);
+      sl_initial_water(b, &X->sc[storm].s0, w.data());
+   }
+   const double *hb = X->hb0.empty() ? w.data() : &X->hb0[(size_t) storm * np];
    rec(fp, "HB", 'd', (long) np, hb);
    rec(fp, "UB", 'd', (long) np, z.data());
    rec(fp, "VB", 'd', (long) np, z.data());

@@ -79,6 +79,23 @@ enum {
 #define SF_SMPT (1 << 0)      /* SMPT2G/FLT2G this step */
 #define SF_FILT (1 << 1)      /* FILTER/XFILTER this step */
 
+/* Storm program: what the device needs to generate a run's initial water and
+ * per-step values (the P_* above) itself.  Doubles, per storm:
+ *   [0] nsteps  [1] IC19  [2] nev  [3] DTMLAK (the lake datum)
+ *   [4 .. 4+SQ_N)  the storm state after SETCMP, in SQ_* order
+ *   DELP(0 .. SL_NTABV-1): the static-height table of the first hour (INTLHT)
+ *   BT(ITIME) for ITIME = 1 .. IC19-1 (TTIMER's ramp)
+ *   nev events of SQ_EV doubles: n, then the state (SQ_* up to SQ_AX) left by
+ *   the n-th STMVAL call wherever that call changed any of it */
+enum {
+   SQ_DELT, SQ_X1B0, SQ_X1B1, SQ_X1B3, SQ_X1B4, SQ_X1B5, SQ_C1, SQ_C2, SQ_X4, SQ_X9, SQ_X10, SQ_X11, SQ_X12,
+   SQ_X15, SQ_X20, SQ_X21, SQ_X23, SQ_X24, SQ_W0, SQ_W1, SQ_SEA, SQ_TV, SQ_AX, SQ_AY, SQ_N
+};
+#define SQ_HDR 4
+#define SQ_DELP (SQ_HDR + SQ_N)                   /* offset of the DELP table */
+#define SQ_BT (SQ_DELP + SL_NTABV)                /* offset of BT(1) */
+#define SQ_EV (SQ_AX + 1)
+
 template <class R>
 struct SlRunDevT {
    int nstorm;

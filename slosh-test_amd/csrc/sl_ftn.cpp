@@ -186,7 +186,9 @@ int inital_(int *mhalt, int *imxb, int *jmxb, double zb[BAS_Y][BAS_X], int *hr, 
    // runs the FP32 step instead
    const char *pe = getenv("SLG_PRECISION");
    const int prec = (pe && atoi(pe) == 32) ? 32 : 64;
-   if (slg_batch_open_prec(g_run.basin, 1, trks, prec, &g_run.batch, err, sizeof err)) fatal("track", err);
+   // the per-step storm values stay on the host: STATICHT reads them there
+   if (slg_batch_open_opts(g_run.basin, 1, trks, prec, SLG_SCHED_HOST, &g_run.batch, err, sizeof err))
+      fatal("track", err);
    if (slg_batch_upload(g_run.batch, err, sizeof err)) fatal("device", err);
    g_run.nsteps = slg_batch_nsteps(g_run.batch, 0);
    g_run.step = 0;

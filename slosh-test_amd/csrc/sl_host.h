@@ -69,6 +69,16 @@ typedef struct sl_basin {
    double aman, bman, gb, agb, agb13;
    int ipn[10], jpn[10];
    int have_ylt;
+   /* the grid's tangent-plane projection from the ?basins.dta line
+    * (load_bsndta / memSetLLx, savellx.c:704-765, rexout.cd/convert.c:404-533):
+    * grid type 'p' 'e' 'h'; G (conformal latitude, longitude as the line
+    * holds it, deg), the map scale at G, sin / cos of G's latitude, P on the
+    * plane, the grid origin and spacing */
+   struct {
+      int have;
+      char type;
+      double glat, glon, tscald, stl, ctl, xppt, yppt, rg2p, thtg2p, abq, delrg, xig, yjg, ximax, yjmax;
+   } proj;
 } sl_basin;
 
 typedef struct sl_storm {
@@ -103,9 +113,49 @@ int sl_basin_load_opts(const char *dta_path, sl_basin *b, int full_depth, char e
 int sl_basin_llx(sl_basin *b, const char *bnt_line, char err[256]);
 void sl_basin_free(sl_basin *b);
 
+/* A track as INPUTN2 reads it (rows 1..n, hours counted from 1, longitudes
+ * west positive), before the hourly interpolation. */
+typedef struct sl_trk {
+   int date_yr, date_mo, date_dy, date_hr, date_mn;
+   int dhr, land, beg, end, adj, spline, n;
+   double ocean, lake, windf, rmwf;
+   int hrs[SL_NT + 2];
+   double fla[SL_NT + 2], flo[SL_NT + 2], fdp[SL_NT + 2], frm[SL_NT + 2];
+} sl_trk;
+void sl_trk_init(sl_trk *t);
+/* Add a row (longitude east positive, as a trk file holds it); 0, 1 when the
+ * track is full (the row is not read), -1 with a message when out of bounds. */
+int sl_trk_row(sl_trk *t, double hour, double lat, double lon, double dp, double rmw, char err[256]);
+int sl_trk_parse(const char *path, sl_trk *t, char err[256]);
+int sl_trk_build(const sl_trk *t, sl_storm *s, char err[256]);
+
+/* The conformal sphere and the basin's tangent plane (savellx.c:40-227):
+ * Clarke <-> conformal latitude; lat/lon (conformal) <-> plane x, y (statute
+ * miles, origin at G, y to local north); grid (p, q) -> plane. */
+void sl_cl2cnf(double clk, double *cnf, double *scale);
+void sl_cnf2cl(double cnf, double *clk);
+void sl_proj_sll2xy(const sl_basin *b, double lat, double lon, double *x, double *y);
+void sl_proj_sxy2ll(const sl_basin *b, double x, double y, double *lat, double *lon);
+void sl_proj_pq2xy(const sl_basin *b, double p, double q, double *x, double *y);
+
+/* MEOW seed points (conMeowGenCmd, rexout.cd/convert.c:1181-1288): the
+ * landfall points of parallel tracks heading dir (deg from north) spaced
+ * apart (mi) across the basin's shadow widened by r34 (mi), about the point
+ * (lat, lon in the basin line's convention, deg).  xyflag 0 steps along the
+ * tangent plane, 1 along great circles (Earth_Jump, convert.c:1127-1151).
+ * num: at most num points (-1: as many as the shadow holds).  Points in the
+ * reference's order into idx/lat/lon[cap]; returns their count, or -1. */
+int sl_meow_seeds(const sl_basin *b, int xyflag, double lat, double lon, double dir, double apart, int num,
+                  double r34, int cap, int *idx, double *slat, double *slon);
+/* Earth_Jump (convert.c:1127-1151): the point dist statute miles from
+ * (lat, lon) (deg) on bearing bear (deg from north). */
+void sl_earth_jump(double lat, double lon, double dist, double bear, double *lat2, double *lon2);
+
 /* Track: parse the trk file and initialise the storm (INPUTN2, STERON, INTDLT,
- * INTVAL, SETCMP).  Needs the loaded basin. */
+ * INTVAL, SETCMP).  Needs the loaded basin.  sl_storm_init does the same from
+ * a track already parsed or built in memory. */
 int sl_storm_load(const char *trk_path, const sl_basin *b, sl_storm *s, char err[256]);
+int sl_storm_init(const sl_trk *t, const sl_basin *b, sl_storm *s, char err[256]);
 /* Initial water HB (INTLHT, intlht.f:53-185) into hb[jmxb*imxb]; also zeroes
  * nothing else -- UB=VB=0 and HMX=HB are the caller's. */
 void sl_initial_water(const sl_basin *b, const sl_storm *s, double *hb);

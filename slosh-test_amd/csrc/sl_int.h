@@ -87,6 +87,8 @@ struct storm_sched {
    std::vector<double> p64, t64;
    std::vector<int> tab_step;          // step after which each table set took effect
    int nsteps;
+   bool hostp = false;                 // p32 / p64 filled (else the device generates them)
+   std::vector<double> prog;           // the storm program (sl_dev.h) the device generates them from
 };
 
 struct slg_batch {
@@ -203,6 +205,40 @@ static inline int dev_rex(const SlBasinDevT<double> *B, const SlRunDevT<double> 
    return sl_dev_rex_f64(B, R, hbs, hmax, zbd, s0, ns, out, cap, nbytes, scr, st);
 }
 
+extern "C" int sl_dev_sched_f32(const double *prog, const long *off, int nstorm, float *params, int maxsteps,
+                                int ismpt2g, int incsm, int filt_always, hipStream_t st);
+extern "C" int sl_dev_sched_f64(const double *prog, const long *off, int nstorm, double *params, int maxsteps,
+                                int ismpt2g, int incsm, int filt_always, hipStream_t st);
+static inline int dev_sched(const double *prog, const long *off, int nstorm, float *params, int maxsteps,
+                            int ismpt2g, int incsm, int filt_always, hipStream_t st)
+{
+   return sl_dev_sched_f32(prog, off, nstorm, params, maxsteps, ismpt2g, incsm, filt_always, st);
+}
+static inline int dev_sched(const double *prog, const long *off, int nstorm, double *params, int maxsteps,
+                            int ismpt2g, int incsm, int filt_always, hipStream_t st)
+{
+   return sl_dev_sched_f64(prog, off, nstorm, params, maxsteps, ismpt2g, incsm, filt_always, st);
+}
+
+extern "C" int sl_dev_intlht_f32(const double *prog, const long *off, const double *bm, const unsigned char *ocn,
+                                 int imxb, int jmxb, int imxb1, int jmxb1, int pitch, const SlRunDevT<float> *R,
+                                 hipStream_t st);
+extern "C" int sl_dev_intlht_f64(const double *prog, const long *off, const double *bm, const unsigned char *ocn,
+                                 int imxb, int jmxb, int imxb1, int jmxb1, int pitch, const SlRunDevT<double> *R,
+                                 hipStream_t st);
+static inline int dev_intlht(const double *prog, const long *off, const double *bm, const unsigned char *ocn,
+                             int imxb, int jmxb, int imxb1, int jmxb1, int pitch, const SlRunDevT<float> *R,
+                             hipStream_t st)
+{
+   return sl_dev_intlht_f32(prog, off, bm, ocn, imxb, jmxb, imxb1, jmxb1, pitch, R, st);
+}
+static inline int dev_intlht(const double *prog, const long *off, const double *bm, const unsigned char *ocn,
+                             int imxb, int jmxb, int imxb1, int jmxb1, int pitch, const SlRunDevT<double> *R,
+                             hipStream_t st)
+{
+   return sl_dev_intlht_f64(prog, off, bm, ocn, imxb, jmxb, imxb1, jmxb1, pitch, R, st);
+}
+
 extern "C" int sl_dev_slot_init_f32(const SlBasinDevT<float> *B, const SlRunDevT<float> *R, int slot, int tstart,
                                     int nsteps, hipStream_t st);
 extern "C" int sl_dev_slot_init_f64(const SlBasinDevT<double> *B, const SlRunDevT<double> *R, int slot, int tstart,
@@ -218,9 +254,14 @@ static inline int dev_slot_init(const SlBasinDevT<double> *B, const SlRunDevT<do
    return sl_dev_slot_init_f64(B, R, slot, tstart, nsteps, st);
 }
 
-// storm schedule of one run (host, double): rows kept when keep_rows
+// a track given in memory, as sl_trk_parse leaves a file's
+extern "C" int sl_track_from_mem(const slg_track *m, sl_trk *t, char e[256]);
+// storm schedule of one run (host, double): rows kept when keep_rows; the
+// per-step values in the batch's precision when hostp, else only the storm
+// program the device generates them from.  trk: a file path, or with track
+// set, the track itself (slg_batch_open_tracks).
 int sl_make_schedule(const slg_basin *B, const char *trk, storm_sched &S, bool keep_rows, int prec, char *err,
-                     int errlen);
+                     int errlen, bool hostp = true, const slg_track *track = nullptr);
 // the basin's static fields on the device in precision prec (uploads once)
 int sl_basin_ensure(slg_basin *B, int prec, char *err, int errlen);
 static inline const std::vector<float> &sched_params(const storm_sched &S, float) { return S.p32; }

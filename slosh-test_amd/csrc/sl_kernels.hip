@@ -2153,6 +2153,78 @@ __global__ void k_advance(SlRunDev R)
    if (s < R.nstorm && R.cur[s] < R.nsteps[s]) R.cur[s]++;
 }
 
+// ------------------------------------------------------------- storm schedule
+// The per-step storm values of every run, generated here from each storm's
+// program (layout in sl_dev.h) instead of on the host.  One thread per storm
+// walks its steps in order.  Step n+1 reads the state STMVAL left after its
+// n-th call (stmval.f:95-236): the call's hourly advance and table refresh
+// arrive as events (the whole state after that call), and every call moves
+// the storm, AX = AX + X12(11), AY = AY + X12(12) (stmval.f:175-178), in
+// double and in that order, so the sums are the host's bit for bit.  BT is
+// TTIMER's ramp (ttimer.f:36-41) from the program's table; the step flags are
+// TMSTPC1's (tmstpc1.f:55-92).  Values are rounded to R as the host rounds them.
+__global__ void __launch_bounds__(64) k_sched(const double *prog, const long *off, int nstorm, real *params,
+                                              int maxsteps, int ismpt2g, int incsm, int filt_always)
+{
+   const int s = blockIdx.x * blockDim.x + threadIdx.x;
+   if (s >= nstorm) return;
+   const double *g = prog + off[s];
+   const int nsteps = (int) g[0], ic19 = (int) g[1], nev = (int) g[2];
+   double st[SQ_N];
+#pragma unroll
+   for (int k = 0; k < SQ_N; k++) st[k] = g[SQ_HDR + k];
+   const double *bt = g + SQ_BT;                         // bt[ITIME - 1]
+   const double *ev = bt + (ic19 > 1 ? ic19 - 1 : 0);
+   int e = 0, next = nev > 0 ? (int) ev[0] : -1;
+   real *P = params + (size_t) s * maxsteps * SL_NPAR;
+   for (int n = 0; n < nsteps; n++) {
+      if (n > 0) {   // STMVAL call n
+         if (n == next) {
+#pragma unroll
+            for (int k = 0; k < SQ_AX; k++) st[k] = ev[(size_t) e * SQ_EV + 1 + k];
+            e++;
+            next = e < nev ? (int) ev[(size_t) e * SQ_EV] : -1;
+         }
+         st[SQ_AX] = st[SQ_AX] + st[SQ_X11];
+         st[SQ_AY] = st[SQ_AY] + st[SQ_X12];
+      }
+      const int itime = n + 1;
+      int fl = 0;
+      if (ismpt2g && itime > 5 && incsm != 1) fl |= SF_SMPT;
+      if ((itime + 1) % incsm == 0 || filt_always) fl |= SF_FILT;
+      real v[SL_NPAR];
+      v[P_BT] = (real) (itime < ic19 ? bt[itime - 1] : 1.);
+      v[P_DELT] = (real) st[SQ_DELT];
+      v[P_X1B4] = (real) st[SQ_X1B3];
+      v[P_X1B5] = (real) st[SQ_X1B4];
+      v[P_X1B6] = (real) st[SQ_X1B5];
+      v[P_C1] = (real) st[SQ_C1];
+      v[P_C2] = (real) st[SQ_C2];
+      v[P_AX] = (real) st[SQ_AX];
+      v[P_AY] = (real) st[SQ_AY];
+      v[P_X4] = (real) st[SQ_X4];
+      v[P_X9] = (real) st[SQ_X9];
+      v[P_X10] = (real) st[SQ_X10];
+      v[P_X15] = (real) st[SQ_X15];
+      v[P_X20] = (real) st[SQ_X20];
+      v[P_X21] = (real) st[SQ_X21];
+      v[P_X23] = (real) st[SQ_X23];
+      v[P_X24] = (real) st[SQ_X24];
+      v[P_W0] = (real) st[SQ_W0];
+      v[P_W1] = (real) st[SQ_W1];
+      v[P_SEA] = (real) st[SQ_SEA];
+      v[P_ITIME] = (real) itime;
+      v[P_TVER] = (real) st[SQ_TV];
+      v[P_FLAGS] = (real) fl;
+      v[P_X1B1] = (real) st[SQ_X1B0];
+      v[P_X1B2] = (real) st[SQ_X1B1];
+      v[P_PAD] = real(0.);
+      real *p = P + (size_t) n * SL_NPAR;
+#pragma unroll
+      for (int k = 0; k < SL_NPAR; k++) p[k] = v[k];
+   }
+}
+
 // ------------------------------------------------------------------ rex frames
 // One rex v2 frame per storm: the values of RexSaveStep's basin loop
 // (rex.c:453-499, cells I-outer and J-inner over (IMXB-1) x (JMXB-1)) as the
@@ -2350,6 +2422,83 @@ extern "C" int SL_SLOTINIT(const SlBasinDev *B, const SlRunDev *R, int slot, int
 {
    const int n = std::max(std::max(B->npts, B->pitch), std::max(B->nsqrwc, B->nsq));
    hipLaunchKernelGGL(k_slot_init, dim3((n + 255) / 256), dim3(256), 0, st, *B, *R, slot, tstart, nsteps);
+   hipError_t e = hipGetLastError();
+   return e == hipSuccess ? 0 : -(int) e;
+}
+
+// INTLHT's standard branch (intlht.f:53-94, 161, 172-177) for every run, on
+// the device: squares in reach of ocean winds (ITREE '2' or '5' at a corner,
+// ocn) take the static height of the first hour's storm at the square, at
+// least -ZB; the others the lake datum, at least -ZB.  Row JMXB repeats row 1.
+// In double (the host's operations in its order), rounded to R into HB (both
+// buffers) and HMX (squares only).  bm: ELPCL[imxb], ELPDL[imxb], COSL[jmxb],
+// SINL[jmxb], then ZB[jmxb][imxb] (packed, I fastest); ocn [jmxb][imxb].
+__global__ void __launch_bounds__(256) k_intlht(const double *prog, const long *off, const double *bm,
+                                                const uint8_t *ocn, int imxb, int jmxb, int imxb1, int jmxb1,
+                                                int pitch, SlRunDev R)
+{
+   const int s = blockIdx.y, p = blockIdx.x * 256 + threadIdx.x;
+   if (p >= imxb * jmxb) return;
+   const int i = p % imxb + 1, j = p / imxb + 1;
+   const double *g = prog + off[s];
+   const double *elpcl = bm, *elpdl = bm + imxb, *cosl = bm + 2 * imxb, *sinl = cosl + jmxb, *zbv = sinl + jmxb;
+   const int jj = (j == jmxb) ? 1 : j;
+   double hb = 0.;
+   if (i <= imxb1 && jj <= jmxb1) {
+      const int q = (jj - 1) * imxb + (i - 1);
+      const double zb = zbv[q];
+      if (ocn[q]) {
+         const double *st = g + SQ_HDR, *delp = g + SQ_DELP;
+         const double xr = elpcl[i - 1] * cosl[jj - 1], yr = elpdl[i - 1] * sinl[jj - 1];
+         const double x = xr - st[SQ_C1] - st[SQ_AX], y = yr - st[SQ_C2] - st[SQ_AY];
+         const double rsq = x * x + y * y, r1 = sqrt(rsq) / 5280. + 1.;
+         int k = (int) r1;
+         const double r2 = k, dr = r1 - r2;
+         if (k > 790) k = 790;
+         const double h = delp[k] + dr * (delp[k + 1] - delp[k]) + st[SQ_SEA];
+         hb = h > -zb ? h : -zb;
+      } else {
+         const double t = g[3] > -zb ? g[3] : -zb;
+         hb = -zb > t ? -zb : t;
+      }
+   }
+   const real v = (real) hb;
+   const size_t o = (size_t) s * R.fstride + (size_t) (j - 1) * pitch + (i - 1);
+   R.hb0[o] = v;
+   R.hb1[o] = v;
+   R.hmx[o] = (i == imxb || j == jmxb) ? real(0.) : v;
+}
+
+#ifdef SL_F64
+#define SL_INTLHT sl_dev_intlht_f64
+#else
+#define SL_INTLHT sl_dev_intlht_f32
+#endif
+extern "C" int SL_INTLHT(const double *prog, const long *off, const double *bm, const unsigned char *ocn, int imxb,
+                         int jmxb, int imxb1, int jmxb1, int pitch, const SlRunDev *R, hipStream_t st)
+{
+   if (R->nstorm < 1 || R->nstorm > 65535 || imxb < 2 || jmxb < 2 || pitch < imxb ||
+       R->fstride < (size_t) pitch * jmxb)
+      return -1;
+   hipLaunchKernelGGL(k_intlht, dim3((imxb * jmxb + 255) / 256, R->nstorm), dim3(256), 0, st, prog, off, bm, ocn,
+                      imxb, jmxb, imxb1, jmxb1, pitch, *R);
+   hipError_t e = hipGetLastError();
+   return e == hipSuccess ? 0 : -(int) e;
+}
+
+#ifdef SL_F64
+#define SL_SCHED sl_dev_sched_f64
+#else
+#define SL_SCHED sl_dev_sched_f32
+#endif
+// per-step storm values [nstorm][maxsteps][SL_NPAR] of every run from the
+// storm programs (prog + off[s], each run's nsteps <= maxsteps)
+extern "C" int SL_SCHED(const double *prog, const long *off, int nstorm, real *params, int maxsteps, int ismpt2g,
+                        int incsm, int filt_always, hipStream_t st)
+{
+   if (nstorm < 1 || incsm < 1) return -1;
+   hipLaunchKernelGGL(k_sched, dim3((nstorm + 63) / 64), dim3(64), 0, st, prog, off, nstorm, params, maxsteps,
+                      ismpt2g, incsm, filt_always);
    hipError_t e = hipGetLastError();
    return e == hipSuccess ? 0 : -(int) e;
 }

@@ -77,28 +77,54 @@ static int starts(const char *s, const char *p)
    return strncmp(s, p, strlen(p)) == 0;
 }
 
-/* INPUTN2 2013 branch (inputn2.f:110-575) */
-static int read_trk(const char *path, sl_storm *s, char err[256])
+/* One track row (inputn2.f:560-605): bounds checked as INPUTN2 checks them;
+ * the hour counted from the first row's hour 1, the longitude negated (west
+ * positive inside the model, inputn2.f:439).  dp: the pressure deficit after
+ * the PressureType conversion. */
+int sl_trk_row(sl_trk *t, double hour, double lat, double lon, double dp, double rmw, char err[256])
+{
+   int n;
+   if (hour < 0 || hour > 999 || lat < 0. || lat > 90. || lon < -180. || lon > 180. || rmw < 1. ||
+       rmw > 500. || dp > 213. || dp < 0.1) {
+      snprintf(err, 256, "track row out of bounds (hour %d)", (int) hour);
+      return -1;
+   }
+   if (t->n >= SL_NT) return 1;   /* rows beyond NT_ are not read */
+   n = ++t->n;
+   if (n == 1) t->adj = 1 - (int) hour;
+   t->hrs[n] = (int) hour + t->adj;
+   t->fla[n] = lat;
+   t->flo[n] = lon * (-1.0);
+   t->fdp[n] = dp;
+   t->frm[n] = rmw;
+   if (n > 1 && t->hrs[n] <= t->hrs[n - 1]) {
+      snprintf(err, 256, "track hours not increasing");
+      return -1;
+   }
+   return 0;
+}
+
+void sl_trk_init(sl_trk *t)
+{
+   memset(t, 0, sizeof *t);
+   t->beg = t->end = t->land = t->dhr = -999;
+   t->ocean = t->lake = -999.9;
+   t->windf = t->rmwf = 1.0;
+}
+
+/* INPUTN2 2013 branch (inputn2.f:110-575): the trk file into t */
+int sl_trk_parse(const char *path, sl_trk *t, char err[256])
 {
    FILE *fp = fopen(path, "r");
    char line[512];
-   int beg = -999, end = -999, land = -999, dhr = -999, press = 0, ntrk = 0, adj = 0;
-   double ocean = -999.9, lake = -999.9, windf = 1.0, rmwf = 1.0, amb = -999.9;
-   int spline_mode = 0, in_data = 0, i, jj;
-   /* per call (tracks may be read on several threads at once) */
-   int hrs[SL_NT + 2];
-   double fla[SL_NT + 2], flo[SL_NT + 2], fdp[SL_NT + 2], frm[SL_NT + 2];
-   double hla[SL_NT + 2], hlo[SL_NT + 2], hdp[SL_NT + 2], hrm[SL_NT + 2];
-   int ihour = 0;
+   int press = 0, in_data = 0, rc;
+   double amb = -999.9;
 
+   sl_trk_init(t);
    if (!fp) {
       snprintf(err, 256, "cannot open track file %s", path);
       return -1;
    }
-   memset(hla, 0, sizeof hla);
-   memset(hlo, 0, sizeof hlo);
-   memset(hdp, 0, sizeof hdp);
-   memset(hrm, 0, sizeof hrm);
    if (!fgets(line, sizeof line, fp) || !strstr(line, "FileVersion") || !strstr(line, "2013")) {
       fclose(fp);
       snprintf(err, 256, "only FileVersion=2013 track files are supported by this build");
@@ -117,23 +143,23 @@ static int read_trk(const char *path, sl_storm *s, char err[256])
             continue;
          }
          if (starts(line, "DateTime=")) {
-            sscanf(line + 9, "%4d", &s->date_yr);
-            sscanf(line + 14, "%2d", &s->date_mo);
-            sscanf(line + 17, "%2d", &s->date_dy);
-            sscanf(line + 20, "%2d", &s->date_hr);
-            sscanf(line + 23, "%2d", &s->date_mn);
-         } else if (starts(line, "DateHour=")) dhr = atoi(line + 9);
-         else if (starts(line, "LandfallHour=")) land = atoi(line + 13);
-         else if (starts(line, "BeginHour=")) beg = atoi(line + 10);
-         else if (starts(line, "EndHour=")) end = atoi(line + 8);
+            sscanf(line + 9, "%4d", &t->date_yr);
+            sscanf(line + 14, "%2d", &t->date_mo);
+            sscanf(line + 17, "%2d", &t->date_dy);
+            sscanf(line + 20, "%2d", &t->date_hr);
+            sscanf(line + 23, "%2d", &t->date_mn);
+         } else if (starts(line, "DateHour=")) t->dhr = atoi(line + 9);
+         else if (starts(line, "LandfallHour=")) t->land = atoi(line + 13);
+         else if (starts(line, "BeginHour=")) t->beg = atoi(line + 10);
+         else if (starts(line, "EndHour=")) t->end = atoi(line + 8);
          else if (starts(line, "AmbientPressure=")) amb = strtod(line + 16, NULL);
-         else if (starts(line, "OceanDatum=")) ocean = strtod(line + 11, NULL);
-         else if (starts(line, "LakeDatum=")) lake = strtod(line + 10, NULL);
-         else if (starts(line, "WindFactor=")) windf = strtod(line + 11, NULL);
-         else if (starts(line, "RMWFactor=")) rmwf = strtod(line + 10, NULL);
+         else if (starts(line, "OceanDatum=")) t->ocean = strtod(line + 11, NULL);
+         else if (starts(line, "LakeDatum=")) t->lake = strtod(line + 10, NULL);
+         else if (starts(line, "WindFactor=")) t->windf = strtod(line + 11, NULL);
+         else if (starts(line, "RMWFactor=")) t->rmwf = strtod(line + 10, NULL);
          else if (starts(line, "InterpolationType=")) {
-            spline_mode = (starts(line + 18, "Spline") || starts(line + 18, "spline"));
-            if (!spline_mode && !starts(line + 18, "Linear") && !starts(line + 18, "linear")) {
+            t->spline = (starts(line + 18, "Spline") || starts(line + 18, "spline"));
+            if (!t->spline && !starts(line + 18, "Linear") && !starts(line + 18, "linear")) {
                fclose(fp);
                snprintf(err, 256, "unknown InterpolationType");
                return -1;
@@ -163,35 +189,48 @@ static int read_trk(const char *path, sl_storm *s, char err[256])
          if (press == 0) dp = v[3];
          else if (press == 1) dp = amb - v[3];
          else dp = v[5] - v[3];
-         if (v[0] < 0 || v[0] > 999 || v[1] < 0. || v[1] > 90. || v[2] < -180. || v[2] > 180. ||
-             v[4] < 1. || v[4] > 500. || dp > 213. || dp < 0.1) {
+         rc = sl_trk_row(t, v[0], v[1], v[2], dp, v[4], err);
+         if (rc < 0) {
             fclose(fp);
-            snprintf(err, 256, "track row out of bounds (hour %d)", (int) v[0]);
             return -1;
          }
-         ntrk++;
-         if (ntrk > SL_NT) break;
-         if (ntrk == 1) adj = 1 - (int) v[0];
-         hrs[ntrk] = (int) v[0] + adj;
-         fla[ntrk] = v[1];
-         flo[ntrk] = v[2] * (-1.0);
-         fdp[ntrk] = dp;
-         frm[ntrk] = v[4];
-         if (ntrk > 1 && hrs[ntrk] <= hrs[ntrk - 1]) {
-            fclose(fp);
-            snprintf(err, 256, "track hours not increasing");
-            return -1;
-         }
+         if (rc > 0) break;
       }
    }
    fclose(fp);
-   if (beg < 0 || beg > 999 || end < 0 || end > 999 || dhr < 0 || land < 0 || land > 999 ||
-       ocean < -50. || ocean > 50. || lake < -50. || lake > 50. || beg >= end || ntrk < 2) {
+   return 0;
+}
+
+/* INPUTN2 after the rows (inputn2.f:440-575): the header checked, the track
+ * interpolated to every hour (linearly, or by the cubic spline), the wind and
+ * RMW factors and the datums applied. */
+int sl_trk_build(const sl_trk *t, sl_storm *s, char err[256])
+{
+   const int ntrk = t->n;
+   int i, jj, ihour = 0;
+   /* per call (tracks may be built on several threads at once) */
+   int hrs[SL_NT + 2];
+   double hla[SL_NT + 2], hlo[SL_NT + 2], hdp[SL_NT + 2], hrm[SL_NT + 2];
+   const double *fla = t->fla, *flo = t->flo, *fdp = t->fdp, *frm = t->frm;
+
+   if (t->beg < 0 || t->beg > 999 || t->end < 0 || t->end > 999 || t->dhr < 0 || t->land < 0 ||
+       t->land > 999 || t->ocean < -50. || t->ocean > 50. || t->lake < -50. || t->lake > 50. ||
+       t->beg >= t->end || ntrk < 2) {
       snprintf(err, 256, "track header incomplete or out of bounds");
       return -1;
    }
+   s->date_yr = t->date_yr;
+   s->date_mo = t->date_mo;
+   s->date_dy = t->date_dy;
+   s->date_hr = t->date_hr;
+   s->date_mn = t->date_mn;
+   memset(hla, 0, sizeof hla);
+   memset(hlo, 0, sizeof hlo);
+   memset(hdp, 0, sizeof hdp);
+   memset(hrm, 0, sizeof hrm);
+   memcpy(hrs, t->hrs, sizeof(int) * (size_t) (ntrk + 1));
    hrs[ntrk + 1] = 0;
-   if (!spline_mode) {
+   if (!t->spline) {
       for (i = 1; i <= ntrk; i++) {
          ihour = hrs[i];
          hla[ihour] = fla[i];
@@ -235,17 +274,17 @@ static int read_trk(const char *path, sl_storm *s, char err[256])
          }
       }
    }
-   s->seadtm = ocean;
-   s->dtmlak = lake;
-   s->ibgnt = beg + adj;
-   s->itend = end + adj;
-   s->jhr = land + adj;
+   s->seadtm = t->ocean;
+   s->dtmlak = t->lake;
+   s->ibgnt = t->beg + t->adj;
+   s->itend = t->end + t->adj;
+   s->jhr = t->land + t->adj;
    s->trklen = ihour;
    for (i = 1; i <= s->trklen; i++) {
       s->XLAT[i] = hla[i];
       s->YLONG[i] = hlo[i];
-      s->PT[i] = hdp[i] * windf;
-      s->R[i] = hrm[i] * rmwf;
+      s->PT[i] = hdp[i] * t->windf;
+      s->R[i] = hrm[i] * t->rmwf;
    }
    if ((int) (s->seadtm * 10 + .5) == 999) {
       s->seadtm = 0;
@@ -253,7 +292,7 @@ static int read_trk(const char *path, sl_storm *s, char err[256])
    }
    if ((s->seadtm >= 150) || (s->seadtm <= -250)) s->seadtm = fmod((s->seadtm + 50), 100.) - 50;
    if (s->itend + 1 > s->trklen || s->ibgnt < 1 || s->jhr < 4) {
-      snprintf(err, 256, "track does not cover hours %d..%d", beg, end + 1);
+      snprintf(err, 256, "track does not cover hours %d..%d", t->beg, t->end + 1);
       return -1;
    }
    return 0;
@@ -464,11 +503,26 @@ static void cnsdlt(sl_storm *s, const sl_basin *b)
 
 int sl_storm_load(const char *trk_path, const sl_basin *b, sl_storm *s, char err[256])
 {
+   sl_trk *t = (sl_trk *) malloc(sizeof *t);
+   int rc;
+   err[0] = 0;
+   if (!t) {
+      snprintf(err, 256, "out of host memory");
+      return -1;
+   }
+   rc = sl_trk_parse(trk_path, t, err);
+   if (!rc) rc = sl_storm_init(t, b, s, err);
+   free(t);
+   return rc;
+}
+
+int sl_storm_init(const sl_trk *trk, const sl_basin *b, sl_storm *s, char err[256])
+{
    int i, ncatg, igrwth;
    double t, vmax;
    memset(s, 0, sizeof *s);
    err[0] = 0;
-   if (read_trk(trk_path, s, err)) return -1;
+   if (sl_trk_build(trk, s, err)) return -1;
    steron(s, b);
    /* INTDLT, intdlt.f:40-62 */
    s->dltb = b->dltin[1];

@@ -40,6 +40,9 @@ def lib():
         L.slg_basin_close.restype = None
         L.slg_batch_open.argtypes = [vp, i, ctypes.POINTER(cp), ctypes.POINTER(vp), cp, i]
         L.slg_batch_open_prec.argtypes = [vp, i, ctypes.POINTER(cp), i, ctypes.POINTER(vp), cp, i]
+        L.slg_batch_open_opts.argtypes = [vp, i, ctypes.POINTER(cp), i, i, ctypes.POINTER(vp), cp, i]
+        L.slg_batch_open_tracks.argtypes = [vp, i, vp, i, i, ctypes.POINTER(vp), cp, i]
+        L.slg_batch_schedule.argtypes = [vp, i, i, vp, cp, i]
         L.slg_batch_precision.argtypes = [vp]
         L.slg_batch_nsteps.argtypes = [vp, i]
         L.slg_batch_upload.argtypes = [vp, cp, i]
@@ -135,15 +138,58 @@ class Basin:
             self.h = ctypes.c_void_p()
 
 
+SCHED_HOST = 1
+NPAR = 26
+
+
+class TrackC(ctypes.Structure):
+    """struct slg_track (include/slosh_hip.h): a trk file's contents in memory."""
+    _dp = ctypes.POINTER(ctypes.c_double)
+    _fields_ = [("year", ctypes.c_int), ("month", ctypes.c_int), ("day", ctypes.c_int),
+                ("hour_of_day", ctypes.c_int), ("minute", ctypes.c_int), ("date_hour", ctypes.c_int),
+                ("landfall_hour", ctypes.c_int), ("begin_hour", ctypes.c_int), ("end_hour", ctypes.c_int),
+                ("ocean_datum", ctypes.c_double), ("lake_datum", ctypes.c_double),
+                ("wind_factor", ctypes.c_double), ("rmw_factor", ctypes.c_double), ("spline", ctypes.c_int),
+                ("npts", ctypes.c_int), ("hour", _dp), ("lat", _dp), ("lon", _dp), ("delp", _dp), ("rmw", _dp)]
+
+
+def track_struct(t: dict):
+    """A TrackC from a dict with the trk header fields (year, month, day,
+    hour_of_day, minute, date_hour, landfall_hour, begin_hour, end_hour,
+    ocean_datum, lake_datum, optional wind_factor / rmw_factor / spline) and row
+    arrays hour, lat, lon (deg E), delp (mb), rmw (mi).  Returns (struct, the
+    arrays it points into, which the caller keeps alive)."""
+    rows = [np.ascontiguousarray(t[k], dtype=np.float64) for k in ("hour", "lat", "lon", "delp", "rmw")]
+    dp = ctypes.POINTER(ctypes.c_double)
+    c = TrackC(int(t["year"]), int(t["month"]), int(t["day"]), int(t["hour_of_day"]), int(t["minute"]),
+               int(t["date_hour"]), int(t["landfall_hour"]), int(t["begin_hour"]), int(t["end_hour"]),
+               float(t["ocean_datum"]), float(t["lake_datum"]), float(t.get("wind_factor", 1.0)),
+               float(t.get("rmw_factor", 1.0)), int(t.get("spline", 0)), len(rows[0]),
+               *[r.ctypes.data_as(dp) for r in rows])
+    return c, rows
+
+
 class Batch:
-    def __init__(self, basin: Basin, trks, precision: int = 32):
+    def __init__(self, basin: Basin, trks=None, precision: int = 32, sched_host: bool = False, tracks=None):
+        """Runs on one basin from trk files (trks) or from tracks in memory
+        (tracks: dicts as track_struct takes).  sched_host: the per-step storm
+        values computed on the host (SLG_SCHED_HOST) instead of on the device."""
         L = lib()
         self.basin = basin
-        self.n = len(trks)
-        arr = (ctypes.c_char_p * self.n)(*[t.encode() for t in trks])
         self.h = ctypes.c_void_p()
         err = ctypes.create_string_buffer(512)
-        _chk(L.slg_batch_open_prec(basin.h, self.n, arr, precision, ctypes.byref(self.h), err, 512), err)
+        flags = SCHED_HOST if sched_host else 0
+        if tracks is not None:
+            self.n = len(tracks)
+            held = [track_struct(t) for t in tracks]
+            arr = (TrackC * self.n)(*[c for c, _ in held])
+            _chk(L.slg_batch_open_tracks(basin.h, self.n, arr, precision, flags, ctypes.byref(self.h), err, 512),
+                 err)
+        else:
+            self.n = len(trks)
+            arr = (ctypes.c_char_p * self.n)(*[t.encode() for t in trks])
+            _chk(L.slg_batch_open_opts(basin.h, self.n, arr, precision, flags, ctypes.byref(self.h), err, 512),
+                 err)
         self.precision = precision
         self.nsteps = [L.slg_batch_nsteps(self.h, s) for s in range(self.n)]
         self._rexbuf = None
@@ -185,6 +231,14 @@ class Batch:
     def upload(self):
         err = ctypes.create_string_buffer(512)
         _chk(lib().slg_batch_upload(self.h, err, 512), err)
+
+    def schedule(self, storm: int, from_device: bool = True):
+        """The per-step storm values of one run, [nsteps][NPAR] (double)."""
+        out = np.zeros((self.nsteps[storm], NPAR))
+        err = ctypes.create_string_buffer(512)
+        _chk(lib().slg_batch_schedule(self.h, storm, 1 if from_device else 0,
+                                      out.ctypes.data_as(ctypes.c_void_p), err, 512), err)
+        return out
 
     def set_storm_group(self, sg: int):
         """Storms whose tiles follow each other in the fused kernels' workgroup order."""

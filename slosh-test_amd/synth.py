@@ -307,7 +307,7 @@ def dta_text(b: Basin) -> str:
     ire = b.imxb - 1 - b.irb
     nde = b.jmxb - 1 - b.ndb
     L.append("%5d%5d%5d%5d%5d" % (b.irb, ire, b.ndb, nde, 0))
-    L.append(_f10(b.dltin, b.dltin, b.dltin))
+    L.append(_f10(*(tuple(b.dltin) if isinstance(b.dltin, (tuple, list)) else (b.dltin,) * 3)))
     if b.gtype != "+":                                       # XMOUTH YMOUTH
         L.append(_f10(b.extra.get("xmouth", b.rmouth), b.extra.get("ymouth", 0.0)))
     # barriers (crdrd2.f:78-100): count with the letter code for thousands
@@ -500,6 +500,17 @@ def trk_text(t: Track) -> str:
         L.append("%d,%.4f,%.4f,%.2f,%.2f" % (h, la, lo, d, r))
     L.append("999,0,0,0,0")
     return "\n".join(L) + "\n"
+
+
+def track_dict(t: Track) -> dict:
+    """The track as slg_track fields (sloshgpu.track_struct): the values a trk
+    file of trk_text(t) holds, rounded as that file writes them."""
+    return dict(year=2025, month=9, day=1, hour_of_day=0, minute=0, date_hour=t.hours[0],
+                landfall_hour=t.landfall, begin_hour=t.begin, end_hour=t.end, ocean_datum=0.0, lake_datum=0.0,
+                spline=1 if t.interp == "Spline" else 0,
+                hour=[float(h) for h in t.hours], lat=[float("%.4f" % x) for x in t.lat],
+                lon=[float("%.4f" % x) for x in t.lon], delp=[float("%.2f" % x) for x in t.dp],
+                rmw=[float("%.2f" % x) for x in t.rmw])
 
 
 def write_case(root: str, b: Basin, t: Track, trk_name="storm.trk"):
