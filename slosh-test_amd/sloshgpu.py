@@ -43,6 +43,8 @@ def lib():
         L.slg_batch_open_opts.argtypes = [vp, i, ctypes.POINTER(cp), i, i, ctypes.POINTER(vp), cp, i]
         L.slg_batch_open_tracks.argtypes = [vp, i, vp, i, i, ctypes.POINTER(vp), cp, i]
         L.slg_batch_schedule.argtypes = [vp, i, i, vp, cp, i]
+        L.slg_meow_seeds.argtypes = [vp, vp, i, vp, vp, vp, cp, i]
+        L.slg_meow_tracks.argtypes = [vp, vp, i, vp, vp, cp, i]
         L.slg_batch_precision.argtypes = [vp]
         L.slg_batch_nsteps.argtypes = [vp, i]
         L.slg_batch_upload.argtypes = [vp, cp, i]
@@ -167,6 +169,54 @@ def track_struct(t: dict):
                float(t.get("rmw_factor", 1.0)), int(t.get("spline", 0)), len(rows[0]),
                *[r.ctypes.data_as(dp) for r in rows])
     return c, rows
+
+
+class MeowC(ctypes.Structure):
+    """struct slg_meow (include/slosh_hip.h)."""
+    _fields_ = [("xyflag", ctypes.c_int), ("lat", ctypes.c_double), ("lon", ctypes.c_double),
+                ("dir", ctypes.c_double), ("apart", ctypes.c_double), ("r34", ctypes.c_double),
+                ("num", ctypes.c_int), ("speed", ctypes.c_double), ("delp", ctypes.c_double),
+                ("rmw", ctypes.c_double), ("landfall_hour", ctypes.c_int), ("begin_hour", ctypes.c_int),
+                ("end_hour", ctypes.c_int), ("ocean_datum", ctypes.c_double), ("lake_datum", ctypes.c_double)]
+
+
+MEOW_HOURS = 100
+
+
+def meow(xyflag=0, lat=0.0, lon=0.0, dir=0.0, apart=10.0, r34=0.0, num=-1, speed=10.0, delp=50.0, rmw=20.0,
+         landfall_hour=50, begin_hour=20, end_hour=70, ocean_datum=0.0, lake_datum=0.0):
+    return MeowC(xyflag, lat, lon, dir, apart, r34, num, speed, delp, rmw, landfall_hour, begin_hour, end_hour,
+                 ocean_datum, lake_datum)
+
+
+def meow_seeds(basin, m: MeowC, cap=4096):
+    """MEOW seed points (slg_meow_seeds): (index, lat, lon) arrays."""
+    idx = np.zeros(cap, dtype=np.int32)
+    la, lo = np.zeros(cap), np.zeros(cap)
+    err = ctypes.create_string_buffer(512)
+    n = lib().slg_meow_seeds(basin.h, ctypes.byref(m), cap, idx.ctypes.data_as(ctypes.c_void_p),
+                             la.ctypes.data_as(ctypes.c_void_p), lo.ctypes.data_as(ctypes.c_void_p), err, 512)
+    _chk(min(n, 0), err)
+    return idx[:n], la[:n], lo[:n]
+
+
+def meow_tracks(basin, m: MeowC, cap=4096):
+    """MEOW tracks (slg_meow_tracks) as dicts Batch(tracks=...) takes."""
+    tr = (TrackC * cap)()
+    rows = np.zeros(cap * 5 * MEOW_HOURS)
+    err = ctypes.create_string_buffer(512)
+    n = lib().slg_meow_tracks(basin.h, ctypes.byref(m), cap, tr, rows.ctypes.data_as(ctypes.c_void_p), err, 512)
+    _chk(min(n, 0), err)
+    out = []
+    for t in range(n):
+        T = tr[t]
+        r = rows[t * 5 * MEOW_HOURS:(t + 1) * 5 * MEOW_HOURS].reshape(5, MEOW_HOURS).copy()
+        out.append(dict(year=T.year, month=T.month, day=T.day, hour_of_day=T.hour_of_day, minute=T.minute,
+                        date_hour=T.date_hour, landfall_hour=T.landfall_hour, begin_hour=T.begin_hour,
+                        end_hour=T.end_hour, ocean_datum=T.ocean_datum, lake_datum=T.lake_datum,
+                        wind_factor=T.wind_factor, rmw_factor=T.rmw_factor, spline=T.spline,
+                        hour=r[0], lat=r[1], lon=r[2], delp=r[3], rmw=r[4]))
+    return out
 
 
 class Batch:

@@ -108,3 +108,33 @@ def test_memory_tracks_run_equal_files(tmp_path, prec):
     F.close()
     M.close()
     B.close()
+
+
+@pytest.mark.parametrize("prec", [32, 64])
+def test_meow_set_from_memory(tmp_path, prec):
+    """A MEOW track set generated in memory (slg_meow_tracks) opens and steps
+    with the device schedule exactly as with the host's."""
+    b = synth.config3_basin()
+    case = synth.write_case(str(tmp_path), b, synth.straight_track(b))
+    B = sloshgpu.Basin(case["dta"], case["bntline"])
+    m = sloshgpu.meow(xyflag=1, lat=b.alto, lon=b.alno, dir=315.0, apart=12.0, r34=15.0, speed=11.5, delp=60.0,
+                      rmw=20.0, landfall_hour=40, begin_hour=16, end_hour=64)
+    tracks = sloshgpu.meow_tracks(B, m)
+    print("MEOW set of %d tracks, FP%d" % (len(tracks), prec), flush=True)
+    assert len(tracks) > 8
+    H = sloshgpu.Batch(B, precision=prec, sched_host=True, tracks=tracks)
+    D = sloshgpu.Batch(B, precision=prec, tracks=tracks)
+    for X in (H, D):
+        X.upload()
+        X.set_path(sloshgpu.PATH_FUSED, 32)
+    for s in range(len(tracks)):
+        assert np.array_equal(D.schedule(s, True), H.schedule(s, False)), "storm %d" % s
+    for X in (H, D):
+        X.step(240)
+        X.sync()
+    for s in range(0, len(tracks), 3):
+        for f in ("HB", "UB", "VB", "HMX"):
+            assert np.array_equal(H.get(s, f), D.get(s, f)), "storm %d field %s" % (s, f)
+    H.close()
+    D.close()
+    B.close()
