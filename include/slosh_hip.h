@@ -5,7 +5,8 @@
  * time step (sorc/slosh.fd/slosh2.c:1400, 1540, 1542) on state held in Fortran
  * COMMON blocks.  This library replaces that path.  Its handle API owns the
  * basin statics and any number of storm runs on one basin, resident in HBM and
- * stepped together in FP32 by hand-written HIP kernels.
+ * stepped together by hand-written HIP kernels, in FP64 (the reference's
+ * default build) or FP32.
  *
  * Plain pointers and sizes only; no torch or HIP types cross this boundary.
  * Functions return 0 on success or a negative SLG_E* code, with a message in
@@ -157,8 +158,8 @@ int slg_batch_sync(slg_batch *x, char *err, int errlen);
  * (512 / tile_rows) x tile_rows.  STAGED: one kernel per routine (slots 0-7).  Results are identical.
  * Basins with sub-grid 1-D flow sides (1-D points, weirs, cuts: FLW1DM_C,
  * flw1dm_block_c.f:83-435, and FLW1DM_M, flw1dm_block_m.f:1-403, plus
- * SMPT2G's side pass, smpt2g.f:101-123) step on the staged path only: FUSED
- * and hipGraphs return SLG_EUNSUP for them. */
+ * SMPT2G's side pass, smpt2g.f:101-123) or channels run those passes in
+ * kernels of their own between SMPT2G/FLT2G/BDRYHT and HMXSV on either path. */
 #define SLG_PATH_FUSED 0
 #define SLG_PATH_STAGED 1
 /* Choose the launch path and the fused tile height: 8, 16 or 32 rows, for

@@ -565,14 +565,64 @@ static int read_body(frec *r, sl_basin *b, char err[256])
          }
       }
    }
-   if (!strcmp(b->dollar, "1$") || !strcmp(b->dollar, "2$") || b->dollar[0] == 'A') {
-      snprintf(err, 256, "basin variant '%s' is not supported by this build", b->dollar);
+   if (b->dollar[0] == 'A') {
+      snprintf(err, 256, "basin variant '%s' (the MPI team's island strip) is not supported by this build",
+               b->dollar);
       return -1;
+   }
+   /* '1$': the squares dried at the start (crdrd2.f:447-461): NODRY (I4),
+    * then, when more than one, pairs (9(I3,1X,I3,1X)) or (9(I4,1X,I4,1X)) */
+   b->nodry = 0;
+   if (!strcmp(b->dollar, "1$")) {
+      int n, k, w = (b->iver == 199201) ? 3 : 4;
+      fr_next(r);
+      n = (int) fr_int(r, 4);
+      if (r->eof || n > 6000 || n < 0) {
+         snprintf(err, 256, "dry cells: %d listed (at most 6000)", n);
+         return -1;
+      }
+      b->nodry = n;
+      b->idry = (int *) calloc((size_t) (n > 0 ? n : 1), sizeof(int));
+      b->jdry = (int *) calloc((size_t) (n > 0 ? n : 1), sizeof(int));
+      if (n > 1) {
+         for (k = 0; k < n; k++) {
+            if (k % 9 == 0) fr_next(r);
+            b->idry[k] = (int) fr_int(r, w);
+            (void) fr_char(r);
+            b->jdry[k] = (int) fr_int(r, w);
+            (void) fr_char(r);
+         }
+         if (r->eof) {
+            snprintf(err, 256, "the dry-cell list ends early");
+            return -1;
+         }
+      }
    }
    if (!strcmp(b->dollar, "$ ")) {
       for (i = 1; i <= imxb; i++) b->zb[PIX(b, i, jmxb)] = b->zb[PIX(b, i, 1)];
    } else {
       for (i = 1; i <= imxb; i++) b->zb[PIX(b, i, jmxb)] = b->zb[PIX(b, i, b->jmxb1)];
+   }
+   /* '2$' (the MSY basin): DEPMSY after CRDRD2 (initlz.f:52, depmsy.f:8-24)
+    * lowers ZB of rows 1..JSUB1 by ZSUB(J), and the lake datum by ZSUBCE;
+    * JSUB = 38, JSUB1 = 50, ZSUBCE = -1 (flw1dm_block_c.f:60) */
+   b->dtmlak_add = 0.;
+   if (!strcmp(b->dollar, "2$")) {
+      const int jsub = 38, jsub1 = 50;
+      const double zsubce = -1.;
+      for (j = 1; j <= jsub1 && j <= jmxb; j++) {
+         double zsub = 0.;
+         if (j <= jsub) zsub = zsubce;
+         else {
+            double z = jsub1 - j;
+            int izs;
+            z = z / (jsub1 - jsub);
+            izs = (int) (-zsubce * z * 10.);
+            zsub = -izs * .1;
+         }
+         for (i = 1; i <= b->imxb1; i++) b->zb[PIX(b, i, j)] = b->zb[PIX(b, i, j)] - zsub;
+      }
+      b->dtmlak_add = zsubce;
    }
    (void) nsheet;
    return 0;
@@ -1021,7 +1071,7 @@ void sl_basin_free(sl_basin *b)
                 b->elpct, b->elpdt, b->elpcl, b->elpdl, b->elpcl2, b->elpdl2, b->ish,
                 b->jsh, b->tidesh, b->ar2, b->ai2, b->br2, b->bi2, b->cr2, b->ci2,
                 b->isqr, b->jsqr, b->iside, b->iang, b->ktree, b->hweir, b->zbmin, b->bank1,
-                b->bank2, b->delcut, b->cos1d, b->sin1d, b->cutl, b->cutli, b->cutle};
+                b->bank2, b->delcut, b->cos1d, b->sin1d, b->cutl, b->cutli, b->cutle, b->idry, b->jdry};
    size_t k;
    for (k = 0; k < sizeof p / sizeof p[0]; k++) free(p[k]);
    memset(b, 0, sizeof *b);

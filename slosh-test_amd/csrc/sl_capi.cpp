@@ -1044,15 +1044,9 @@ static int batch_upload(slg_batch *X, char *err, int errlen)
       buf.insert(buf.end(), b->cosl, b->cosl + nj);
       buf.insert(buf.end(), b->sinl, b->sinl + nj);
       buf.insert(buf.end(), b->zb, b->zb + (size_t) ni * nj);
-      std::vector<unsigned char> ocn((size_t) ni * nj, 0);
-      for (int j = 1; j <= b->jmxb1; j++)     // intlht.f:59-62
-         for (int i = 1; i <= b->imxb1; i++) {
-            const char *it = b->itree;
-            const char c[4] = {it[PIX(b, i, j)], it[PIX(b, i + 1, j)], it[PIX(b, i, j + 1)],
-                               it[PIX(b, i + 1, j + 1)]};
-            for (int k = 0; k < 4; k++)
-               if (c[k] == '2' || c[k] == '5') ocn[PIX(b, i, j)] = 1;
-         }
+      std::vector<unsigned char> ocn((size_t) ni * nj, 0);   // how INTLHT fills each square
+      for (int j = 1; j <= b->jmxb1; j++)
+         for (int i = 1; i <= b->imxb1; i++) ocn[PIX(b, i, j)] = (unsigned char) sl_intlht_class(b, i, j);
       char *pd = nullptr;
       const size_t nbytes = buf.size() * sizeof(double) + (size_t) S * sizeof(long) + ocn.size();
       HIPOK(hipMalloc((void **) &pd, nbytes));

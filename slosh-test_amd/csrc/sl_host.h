@@ -57,6 +57,8 @@ typedef struct sl_basin {
     * 1..nsqrs 1-D flow sides, nsqrs+1..nsqrw raised weirs, nsqrw+1..nsqrwc
     * cuts (cut data cutl/cutli/cutle at index L-nsqrw-1). */
    int nsqrs, nsqrw, nsqrwc, npss, ncut, nodry;
+   int *idry, *jdry;                  /* '1$' basins: squares dried at the start (COMMON /LDRY/) */
+   double dtmlak_add;                 /* '2$' basins: DEPMSY's change to the lake datum */
    int *isqr, *jsqr, *iside, *iang;
    char *ktree;
    double *hweir, *zbmin, *bank1, *bank2, *delcut, *cos1d, *sin1d;
@@ -159,6 +161,13 @@ int sl_storm_init(const sl_trk *t, const sl_basin *b, sl_storm *s, char err[256]
 /* Initial water HB (INTLHT, intlht.f:53-185) into hb[jmxb*imxb]; also zeroes
  * nothing else -- UB=VB=0 and HMX=HB are the caller's. */
 void sl_initial_water(const sl_basin *b, const sl_storm *s, double *hb);
+/* How INTLHT fills square (i,j), i <= IMXB1, j <= JMXB1: SL_IW_STORM the
+ * storm's static height (else a datum: the lake datum, or with SL_IW_SEA the
+ * sea datum), SL_IW_DRY dried afterwards (-ZB). */
+#define SL_IW_STORM 1
+#define SL_IW_DRY 2
+#define SL_IW_SEA 4
+int sl_intlht_class(const sl_basin *b, int i, int j);
 
 /* Per-step storm machinery.  Order inside one step, as TMSTPC1/TMSTPM run it:
  * sl_storm_ttimer() at the start (ITIME+1, ramp BT), then the field kernels,

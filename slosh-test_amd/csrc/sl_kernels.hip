@@ -2426,13 +2426,15 @@ extern "C" int SL_SLOTINIT(const SlBasinDev *B, const SlRunDev *R, int slot, int
    return e == hipSuccess ? 0 : -(int) e;
 }
 
-// INTLHT's standard branch (intlht.f:53-94, 161, 172-177) for every run, on
-// the device: squares in reach of ocean winds (ITREE '2' or '5' at a corner,
-// ocn) take the static height of the first hour's storm at the square, at
-// least -ZB; the others the lake datum, at least -ZB.  Row JMXB repeats row 1.
-// In double (the host's operations in its order), rounded to R into HB (both
-// buffers) and HMX (squares only).  bm: ELPCL[imxb], ELPDL[imxb], COSL[jmxb],
-// SINL[jmxb], then ZB[jmxb][imxb] (packed, I fastest); ocn [jmxb][imxb].
+// INTLHT (intlht.f:53-185 without the Okeechobee branch; 161, 172-177) for
+// every run, on the device.  ocn [jmxb][imxb] says how the host's reading of
+// the basin fills each square (sl_intlht_class): squares in reach of ocean
+// winds take the static height of the first hour's storm at the square, at
+// least -ZB; the others a datum (the lake's, or the sea's on MSY's open
+// squares), at least -ZB; listed squares dry out (-ZB).  Row JMXB repeats row
+// 1.  In double (the host's operations in its order), rounded to R into HB
+// (both buffers) and HMX (squares only).  bm: ELPCL[imxb], ELPDL[imxb],
+// COSL[jmxb], SINL[jmxb], then ZB[jmxb][imxb] (packed, I fastest).
 __global__ void __launch_bounds__(256) k_intlht(const double *prog, const long *off, const double *bm,
                                                 const uint8_t *ocn, int imxb, int jmxb, int imxb1, int jmxb1,
                                                 int pitch, SlRunDev R)
@@ -2447,7 +2449,8 @@ __global__ void __launch_bounds__(256) k_intlht(const double *prog, const long *
    if (i <= imxb1 && jj <= jmxb1) {
       const int q = (jj - 1) * imxb + (i - 1);
       const double zb = zbv[q];
-      if (ocn[q]) {
+      const int cls = ocn[q];
+      if (cls & 1) {
          const double *st = g + SQ_HDR, *delp = g + SQ_DELP;
          const double xr = elpcl[i - 1] * cosl[jj - 1], yr = elpdl[i - 1] * sinl[jj - 1];
          const double x = xr - st[SQ_C1] - st[SQ_AX], y = yr - st[SQ_C2] - st[SQ_AY];
@@ -2458,9 +2461,11 @@ __global__ void __launch_bounds__(256) k_intlht(const double *prog, const long *
          const double h = delp[k] + dr * (delp[k + 1] - delp[k]) + st[SQ_SEA];
          hb = h > -zb ? h : -zb;
       } else {
-         const double t = g[3] > -zb ? g[3] : -zb;
+         const double dtm = (cls & 4) ? g[SQ_HDR + SQ_SEA] : g[3];
+         const double t = dtm > -zb ? dtm : -zb;
          hb = -zb > t ? -zb : t;
       }
+      if (cls & 2) hb = -zb;
    }
    const real v = (real) hb;
    const size_t o = (size_t) s * R.fstride + (size_t) (j - 1) * pitch + (i - 1);

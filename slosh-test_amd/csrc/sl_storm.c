@@ -523,6 +523,7 @@ int sl_storm_init(const sl_trk *trk, const sl_basin *b, sl_storm *s, char err[25
    memset(s, 0, sizeof *s);
    err[0] = 0;
    if (sl_trk_build(trk, s, err)) return -1;
+   s->dtmlak = s->dtmlak + b->dtmlak_add;   /* DEPMSY, depmsy.f:24 ('2$' basins) */
    steron(s, b);
    /* INTDLT, intdlt.f:40-62 */
    s->dltb = b->dltin[1];
@@ -722,19 +723,45 @@ void sl_storm_latlon(const sl_storm *s, const sl_basin *b, double *lat, double *
    *lon = dlong / 1.74532925199433E-2;
 }
 
-/* INTLHT standard branch, intlht.f:53-94 (+ the J=JMXB copy :161) */
+/* MSY's first lake column of each row (LDTMG, flw1dm_block_c.f:67-69) */
+static int ldtmg(int j)
+{
+   static const int t[32] = {33, 33, 33, 33, 33, 35, 35, 35, 35, 35, 35, 35, 35, 34, 34, 34,
+                             34, 34, 30, 29, 29, 28, 27, 26, 26, 24, 22, 20, 21, 22, 22, 21};
+   return (j >= 1 && j <= 32) ? t[j - 1] : 1;
+}
+
+int sl_intlht_class(const sl_basin *b, int i, int j)
+{
+   const char *it = b->itree;
+   char a = it[PIX(b, i, j)], c = it[PIX(b, i + 1, j)], d = it[PIX(b, i, j + 1)], e = it[PIX(b, i + 1, j + 1)];
+   int k, cls;
+   if (!strcmp(b->dollar, "2$")) {   /* the MSY branch, intlht.f:96-121 */
+      if (i <= ldtmg(j)) cls = 0;
+      else if (a == '2' || d == '2' || c == '2' || e == '2') cls = SL_IW_STORM;
+      else cls = SL_IW_SEA;
+   } else {                          /* intlht.f:55-78 */
+      cls = (a == '2' || a == '5' || c == '2' || c == '5' || d == '2' || d == '5' || e == '2' || e == '5')
+               ? SL_IW_STORM : 0;
+   }
+   /* '1$': the listed squares dry out (intlht.f:81-92); MSY's list is empty
+    * once INITALCOMMON has cleared COMMON /LDRY/ (initalcommon.f:129-131) */
+   if (!strcmp(b->dollar, "1$"))
+      for (k = 0; k < b->nodry; k++)
+         if (b->idry[k] == i && b->jdry[k] == j) cls |= SL_IW_DRY;
+   return cls;
+}
+
+/* INTLHT, intlht.f:53-185 without the Okeechobee branch (+ the J=JMXB copy :161) */
 void sl_initial_water(const sl_basin *b, const sl_storm *s, double *hb)
 {
    int i, j;
    memset(hb, 0, sizeof(double) * (size_t) b->imxb * b->jmxb);
    for (j = 1; j <= b->jmxb1; j++)
       for (i = 1; i <= b->imxb1; i++) {
-         const char *it = b->itree;
-         char a = it[PIX(b, i, j)], c = it[PIX(b, i + 1, j)], d = it[PIX(b, i, j + 1)],
-              e = it[PIX(b, i + 1, j + 1)];
+         const int cls = sl_intlht_class(b, i, j);
          double zb = b->zb[PIX(b, i, j)], h;
-         if (a == '2' || a == '5' || c == '2' || c == '5' || d == '2' || d == '5' || e == '2' ||
-             e == '5') {
+         if (cls & SL_IW_STORM) {
             double xr = b->elpcl[i - 1] * b->cosl[j - 1], yr = b->elpdl[i - 1] * b->sinl[j - 1];
             double x = xr - s->c1 - s->ax, y = yr - s->c2 - s->ay;
             double rsq = x * x + y * y, r1 = sqrt(rsq) / 5280. + 1., r2, dr;
@@ -745,9 +772,14 @@ void sl_initial_water(const sl_basin *b, const sl_storm *s, double *hb)
             h = s->DELP[k] + dr * (s->DELP[k + 1] - s->DELP[k]) + s->seadtm;
             hb[PIX(b, i, j)] = h > -zb ? h : -zb;
          } else {
-            double t = s->dtmlak > -zb ? s->dtmlak : -zb;
+            const double dtm = (cls & SL_IW_SEA) ? s->seadtm : s->dtmlak;
+            double t = dtm > -zb ? dtm : -zb;
             hb[PIX(b, i, j)] = -zb > t ? -zb : t;
          }
       }
+   if (b->nodry > 0)
+      for (j = 1; j <= b->jmxb1; j++)
+         for (i = 1; i <= b->imxb1; i++)
+            if (sl_intlht_class(b, i, j) & SL_IW_DRY) hb[PIX(b, i, j)] = -b->zb[PIX(b, i, j)];
    for (i = 1; i <= b->imxb1; i++) hb[PIX(b, i, b->jmxb)] = hb[PIX(b, i, 1)];
 }

@@ -360,6 +360,11 @@ def dta_text(b: Basin) -> str:
         for j in range(b.jmxb):
             vals = b.zb[j, k:min(k + 15, b.imxb)]
             L.append("".join("%5d" % v for v in vals))
+    if b.extra.get("dollar") == "1$":                        # dry cells (I4), then (9(I4,1X,I4,1X))
+        dry = b.extra.get("dry", [])
+        L.append("%4d" % len(dry))
+        for k in range(0, len(dry), 9):
+            L.append("".join("%4d %4d " % p for p in dry[k:k + 9]))
     return "\n".join(L) + "\n"
 
 
@@ -429,6 +434,8 @@ class Track:
     end: int
     landfall: int
     interp: str = "Linear"
+    ocean: float = 0.0  # OceanDatum, LakeDatum (ft)
+    lake: float = 0.0
 
 
 def straight_track(b: Basin, speed_kt=12.0, heading=330.0, dp=60.0, rmw=25.0,
@@ -491,8 +498,8 @@ def trk_text(t: Track) -> str:
          "LandfallHour=%d" % t.landfall,
          "BeginHour=%d" % t.begin,
          "EndHour=%d" % t.end,
-         "OceanDatum=0.0",
-         "LakeDatum=0.0",
+         "OceanDatum=%.1f" % t.ocean,
+         "LakeDatum=%.1f" % t.lake,
          "InterpolationType=%s" % t.interp,
          "PressureType=0",
          "Hour,Lat,Lon,DeltaP,RMW"]
@@ -506,7 +513,8 @@ def track_dict(t: Track) -> dict:
     """The track as slg_track fields (sloshgpu.track_struct): the values a trk
     file of trk_text(t) holds, rounded as that file writes them."""
     return dict(year=2025, month=9, day=1, hour_of_day=0, minute=0, date_hour=t.hours[0],
-                landfall_hour=t.landfall, begin_hour=t.begin, end_hour=t.end, ocean_datum=0.0, lake_datum=0.0,
+                landfall_hour=t.landfall, begin_hour=t.begin, end_hour=t.end,
+                ocean_datum=float("%.1f" % t.ocean), lake_datum=float("%.1f" % t.lake),
                 spline=1 if t.interp == "Spline" else 0,
                 hour=[float(h) for h in t.hours], lat=[float("%.4f" % x) for x in t.lat],
                 lon=[float("%.4f" % x) for x in t.lon], delp=[float("%.2f" % x) for x in t.dp],

@@ -275,6 +275,40 @@ def island_case(root, idx, ni=30, nj=26, hours=6):
     return make_case(root, b, t)
 
 
+# The DOLLAR variants of CRDRD1's line 1 that change the initialisation:
+# '1$' lists squares dried at the start (crdrd2.f:447-461, intlht.f:81-92);
+# '2$' is the MSY basin: DEPMSY lowers ZB of rows 1..50 and the lake datum
+# (initlz.f:52, depmsy.f), and INTLHT fills it by LDTMG's lake columns, ITREE
+# '2' ocean and the sea datum elsewhere (intlht.f:96-121).  (flag, grid type).
+DOLLAR_VARIANTS = [("1$", " "), ("1$", "$"), ("2$", " "), ("2$", "+")]
+
+
+def dollar_id(idx):
+    d, g = DOLLAR_VARIANTS[idx]
+    return "dollar%s-%s" % (d[0], {" ": "polar", "$": "ellip", "+": "hyper"}[g])
+
+
+def dollar_basin(idx, hours=6):
+    d, g = DOLLAR_VARIANTS[idx]
+    ni, nj = (40, 56) if d == "2$" else (30, 26)
+    b, t = synth.tiny(seed=400 + idx, gtype=g, ni=ni, nj=nj, hours=hours)
+    t.dp = [85.0] * len(t.dp)
+    t.ocean, t.lake = 0.7, 1.5
+    b.extra["dollar"] = d
+    if d == "1$":
+        rng = np.random.default_rng(400 + idx)
+        # wet squares below the datum, plus one outside the grid (INTLHT skips it)
+        wet = [(i + 1, j + 1) for j in range(nj - 1) for i in range(ni - 1) if b.zb[j, i] > 0]
+        pick = rng.choice(len(wet), size=min(23, len(wet)), replace=False)
+        b.extra["dry"] = [wet[k] for k in sorted(pick)] + [(ni + 3, 2)]
+    return b, t
+
+
+def dollar_case(root, idx, hours=6):
+    b, t = dollar_basin(idx, hours)
+    return make_case(root, b, t)
+
+
 # Ver2019-03 basins with a Manning's n map (crdrd2.f:379-385): the bottom
 # stress class IIK of each point follows its n (inital.f), (grid type, seed).
 MANNING_VARIANTS = [(" ", 301), ("$", 302), ("+", 303)]

@@ -405,6 +405,18 @@ def test_island_bitwise(idx, path, precision, tmp_path):
     _assert_identical(out, helpers.island_id(idx))
 
 
+# '1$' (squares dried at the start) and '2$' (MSY: DEPMSY, INTLHT's MSY
+# branch) basins: the initial water made on the device and the steps after it
+# equal the oracle from the host's initialisation, bit for bit.
+@pytest.mark.parametrize("precision", [32, 64])
+@pytest.mark.parametrize("idx", range(len(helpers.DOLLAR_VARIANTS)), ids=helpers.dollar_id)
+def test_dollar_bitwise(idx, precision, tmp_path):
+    case = helpers.dollar_case(str(tmp_path), idx)
+    print("variant", helpers.dollar_id(idx), "fp%d" % precision, flush=True)
+    out, _ = helpers.gpu_vs_oracle(case, 300, [1, 20, 100, 300], path=sloshgpu.PATH_FUSED, precision=precision)
+    _assert_identical(out, helpers.dollar_id(idx))
+
+
 def test_island_graph_and_batch_equal_alone(tmp_path):
     """Island storms batched (runs of different lengths, hipGraph replay with
     each storm's own filter steps) equal the same storms run alone."""

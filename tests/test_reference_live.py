@@ -176,3 +176,46 @@ def test_channel_planted_mistake_is_caught(tmp_path):
     helpers.write_rec(root + "/init2.bin", recs)
     b = helpers.run_oracle("64", root + "/init2.bin", rows, tabs, 200, {200})
     assert any(not np.array_equal(a[200][f], b[200][f]) for f in helpers.FIELDS)
+
+
+@pytest.mark.parametrize("idx", range(len(helpers.DOLLAR_VARIANTS)), ids=helpers.dollar_id)
+def test_dollar_oracle64_and_host_match_reference(idx, tmp_path):
+    """'1$' basins (squares dried at the start) and '2$' (MSY: DEPMSY and
+    INTLHT's MSY branch): the product's initialisation equals the reference
+    build's, and the FP64 oracle from it equals the reference at every
+    checkpoint."""
+    root = str(tmp_path)
+    case = helpers.dollar_case(root, idx)
+    ref = helpers.run_reference(case, root + "/ref", 300, ",".join(map(str, CHECKS)))
+    rows, tabs, _ = helpers.product_export(case, root)
+    ri = helpers.read_rec(ref + "/init.bin")
+    mi = helpers.read_rec(root + "/init.bin")
+    for k in mi:
+        assert np.array_equal(ri[k], mi[k]), "init record %s differs" % k
+    rrows = np.fromfile(ref + "/storm.bin").reshape(-1, 90)
+    assert np.array_equal(rows[:len(rrows)], rrows)
+    snaps = helpers.run_oracle("64", root + "/init.bin", rows, tabs, 300, set(CHECKS))
+    for c in CHECKS:
+        st = helpers.read_rec(ref + "/s%06d.bin" % c)
+        for f in helpers.FIELDS:
+            assert np.array_equal(snaps[c][f].ravel(), st[f]), "step %d %s differs" % (c, f)
+
+
+def test_dollar_variants_change_the_start(tmp_path):
+    """The variants are seen: the dried squares start dry, and MSY's rows
+    1..50 are lowered and filled from both datums."""
+    for idx in range(len(helpers.DOLLAR_VARIANTS)):
+        d, g = helpers.DOLLAR_VARIANTS[idx]
+        root = str(tmp_path / ("v%d" % idx))
+        case = helpers.dollar_case(root, idx)
+        helpers.product_export(case, root)
+        mi = helpers.read_rec(root + "/init.bin")
+        ni, nj = mi["DUMB3"][0], mi["DUMB3"][1]
+        hb = mi["HB"].reshape(nj, ni)
+        zb = mi["ZB"].reshape(nj, ni)
+        if d == "1$":
+            b, _ = helpers.dollar_basin(idx)
+            dried = [(i, j) for i, j in b.extra["dry"] if i <= ni - 1 and j <= nj - 1]
+            assert dried and all(hb[j - 1, i - 1] == -zb[j - 1, i - 1] for i, j in dried)
+        else:
+            assert np.any(hb[:nj - 1, :ni - 1] == 1.5 - 1.0) and np.any(hb[:nj - 1, :ni - 1] == 0.7)
