@@ -35,7 +35,14 @@ def test_gpu_cli_matches_oracle32_cli(cid, b, t, tmp_path):
 
 def test_gpu_cli_polar_100x90_vs_reference(tmp_path):
     """Polar 100x90, 48 h: identical bytes to the FP32 oracle through the same
-    driver, and an envelope that agrees with the reference FP64 CLI."""
+    driver, and the envelope's agreement with the reference FP64 CLI.
+
+    FP32 is not the precision that meets the 99.9 % wet/dry bar: it agrees on
+    99.53 % of this run's squares, while one ulp of input moves the reference
+    build itself by 1-3 squares (99.97-99.99 %, test_sensitivity).  The FP64
+    step, which is the reference's default build, equals it bit for bit
+    (test_gpu_cli_fp64_polar_100x90_matches_reference).  The bound here only
+    guards the FP32 path against regressions."""
     b, t = synth.config1()
     root = str(tmp_path)
     bsn, env_g, rex_g, env_o, rex_o = _run_pair(b, t, root)
