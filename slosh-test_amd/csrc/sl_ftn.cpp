@@ -327,18 +327,21 @@ int issubgrid_(int *i, int *j, int *ans)
    return 0;
 }
 // The tide spin-up (SpinUpTideGrid, slosh2.c:1029-1225: TMSTP2, INTLHTSUB,
-// INTLHTADD) is not part of this build.
+// INTLHTADD) is not part of this build: its momentum step MOMNTM2 adds the
+// wind-forcing terms XTTX/XTTY to every wet point but, with the forcing
+// commented out, sets them only on partially flooded points (momntm2.f:
+// 352-372), so it reads them uninitialised and has no defined result to match.
 int tmstp2_(void)
 {
-   fatal("tide spin-up (TMSTP2) is not supported by this build", "");
+   fatal("tide spin-up (TMSTP2) is not supported by this build", "MOMNTM2 reads XTTX/XTTY uninitialised");
 }
 int intlhtsub_(void)
 {
-   fatal("tides (INTLHTSUB) are not supported by this build", "");
+   fatal("tide spin-up (INTLHTSUB) is not supported by this build", "");
 }
 int intlhtadd_(void)
 {
-   fatal("tides (INTLHTADD) are not supported by this build", "");
+   fatal("tide spin-up (INTLHTADD) is not supported by this build", "");
 }
 // CLN_HS (cln_hs.f) deletes the hot-start files; this build writes none.  The
 // driver calls it at the end of every run (slosh2.c:2771).
