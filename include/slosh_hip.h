@@ -364,6 +364,30 @@ int slg_batch_put(slg_batch *x, int storm, const char *name, const double *src, 
 int slg_batch_set_boundary_tide(slg_batch *x, int storm, const double *tide, int n, char *err, int errlen);
 int slg_batch_static_height(const slg_batch *x, int storm, int i, int j, double *ht);
 
+/* The gridded astronomical tide of the -f_tide V2 modes inside the batch, with
+ * no driver between steps: the tide coupling of RunInit and RunLoopStep
+ * (slosh2.c:1789-1869, 1553-1582), SetTideUsableFlag (:717-744) and
+ * CalcTideGrid2 (:813-1027), applied on the device after every step (and once
+ * to the initial water at the upload).  Replaces the reference's
+ * InitTideGrid / ReadHarmonicBody / ReadAdjDatum (tideutil.c:148-734) and
+ * CalcTideGrid2 per run.
+ * ft03: the yearly constituent file; bhc: the basin's harmonic constants
+ * (HarmConstit v1, one record per square of the (IMXB-1) x (JMXB-1) grid);
+ * adj: the MTL -> NAVD88 adjustment text, or NULL / a missing file (0).
+ * mode: 2, 21, 22 or 24 (-f_tide V2, V2.1.x, V2.2.x, V2.4.x) and thresh the x
+ * (ParseTide, usrparse.c:14-52; squares with ZB <= thresh take no tide in
+ * modes 21, 22, 24).  Each run's model clock starts at its start date and
+ * moves by its DELT each step; the tables are its year's.  Call once, before
+ * slg_batch_upload.  hipGraphs and storm parts on several streams are not
+ * used while the tide is on (the steps launch one by one on the batch's
+ * stream).  SLG_EUNSUP for the hsf basins' 6-minute tide cadence
+ * (slosh2.c:1565-1577).  The tide spin-up (-spinUp) is not provided.
+ * datums_tide: slg_track_datums for such a run (InitWater_TideModeOverride
+ * with f_tide = mode). */
+int slg_batch_set_tide(slg_batch *x, const char *ft03, const char *bhc, const char *adj, int mode, int thresh,
+                       char *err, int errlen);
+int slg_track_datums_tide(const char *trk_path, int mode, float *ht1, float *ht2);
+
 /* Write the host initialisation of one run, and its storm schedule, in the
  * tagged-record layout used for the reference's state dumps (for checking). */
 int slg_export_init(const slg_batch *x, int storm, const char *path);

@@ -1000,6 +1000,20 @@ static int batch_upload(slg_batch *X, char *err, int errlen)
    // initial water (INTLHT) per storm: HB, HMX=HB, UB=VB=0 (from the host's,
    // or on the device below)
    const bool hostw = !X->hb0.empty();
+   // with the tide in single precision, that water also in double for the
+   // tide's first call (the driver adds it to the double value)
+   double *hbd = nullptr;
+   if (X->tide_mode && sizeof(R) == 4) HIPOK(hipMalloc((void **) &hbd, np * sizeof(double)));
+   auto fail = [&](hipError_t he) {
+      if (hbd) (void) hipFree(hbd);
+      seterr(err, errlen, "HIP error %s (%d)", hipGetErrorString(he), (int) he);
+      return SLG_EHIP;
+   };
+#define HIPOK_T(x)                                                                                             \
+   do {                                                                                                        \
+      hipError_t he_ = (x);                                                                                    \
+      if (he_ != hipSuccess) return fail(he_);                                                                 \
+   } while (0)
    std::vector<R> f(hostw ? fstride : 0, R(0));
    std::vector<R> params(X->sc[0].hostp ? (size_t) S * X->maxsteps * SL_NPAR : 0, R(0));
    std::vector<R> tables;
@@ -1009,12 +1023,18 @@ static int batch_upload(slg_batch *X, char *err, int errlen)
          const double *hb = &X->hb0[(size_t) s * b->imxb * NJ];
          for (int j = 1; j <= NJ; j++)
             for (int i = 1; i <= b->imxb; i++) f[(size_t) (j - 1) * P + (i - 1)] = (R) hb[PIX(b, i, j)];
-         HIPOK(hipMemcpy(D.hb0 + (size_t) s * fstride, f.data(), fstride * rs, hipMemcpyHostToDevice));
-         HIPOK(hipMemcpy(D.hb1 + (size_t) s * fstride, f.data(), fstride * rs, hipMemcpyHostToDevice));
+         HIPOK_T(hipMemcpy(D.hb0 + (size_t) s * fstride, f.data(), fstride * rs, hipMemcpyHostToDevice));
+         HIPOK_T(hipMemcpy(D.hb1 + (size_t) s * fstride, f.data(), fstride * rs, hipMemcpyHostToDevice));
          for (int j = 1; j <= NJ; j++)       // HMX = HB on squares only (intlht.f:172-177)
             for (int i = 1; i <= b->imxb; i++)
                if (i == b->imxb || j == NJ) f[(size_t) (j - 1) * P + (i - 1)] = R(0);
-         HIPOK(hipMemcpy(D.hmx + (size_t) s * fstride, f.data(), fstride * rs, hipMemcpyHostToDevice));
+         HIPOK_T(hipMemcpy(D.hmx + (size_t) s * fstride, f.data(), fstride * rs, hipMemcpyHostToDevice));
+         if (hbd) {
+            std::vector<double> fd(fstride, 0.);
+            for (int j = 1; j <= NJ; j++)
+               for (int i = 1; i <= b->imxb; i++) fd[(size_t) (j - 1) * P + (i - 1)] = hb[PIX(b, i, j)];
+            HIPOK_T(hipMemcpy(hbd + (size_t) s * fstride, fd.data(), fstride * 8, hipMemcpyHostToDevice));
+         }
       }
       const std::vector<R> &sp = sched_params(X->sc[s], R()), &st = sched_tables(X->sc[s], R());
       if (X->sc[s].hostp) memcpy(&params[(size_t) s * X->maxsteps * SL_NPAR], sp.data(), sp.size() * sizeof(R));
@@ -1022,7 +1042,7 @@ static int batch_upload(slg_batch *X, char *err, int errlen)
       tables.insert(tables.end(), st.begin(), st.end());
       nst[s] = X->sc[s].nsteps;
    }
-   if (X->sc[0].hostp) HIPOK(hipMemcpy(d + off[12], params.data(), params.size() * rs, hipMemcpyHostToDevice));
+   if (X->sc[0].hostp) HIPOK_T(hipMemcpy(d + off[12], params.data(), params.size() * rs, hipMemcpyHostToDevice));
    if (!X->sc[0].hostp || !hostw) {
       // the storm programs to the device, with (for the initial water) the
       // basin metrics ELPCL ELPDL COSL SINL, ZB and the squares in reach of
@@ -1049,7 +1069,7 @@ static int batch_upload(slg_batch *X, char *err, int errlen)
          for (int i = 1; i <= b->imxb1; i++) ocn[PIX(b, i, j)] = (unsigned char) sl_intlht_class(b, i, j);
       char *pd = nullptr;
       const size_t nbytes = buf.size() * sizeof(double) + (size_t) S * sizeof(long) + ocn.size();
-      HIPOK(hipMalloc((void **) &pd, nbytes));
+      HIPOK_T(hipMalloc((void **) &pd, nbytes));
       long *offd = (long *) (pd + buf.size() * sizeof(double));
       unsigned char *ocnd = (unsigned char *) (offd + S);
       hipError_t he = hipMemcpy(pd, buf.data(), buf.size() * sizeof(double), hipMemcpyHostToDevice);
@@ -1061,25 +1081,40 @@ static int batch_upload(slg_batch *X, char *err, int errlen)
                          b->ebsn1 == '+', 0);
       if (!krc && !hostw)
          krc = dev_intlht((const double *) pd, offd, (const double *) pd + np2, ocnd, ni, nj, b->imxb1, b->jmxb1, P,
-                          &D, 0);
+                          &D, hbd, 0);
       if (!krc) krc = -(int) hipDeviceSynchronize();
       (void) hipFree(pd);
       if (krc) {
+         if (hbd) (void) hipFree(hbd);
          seterr(err, errlen, "%s", "device storm schedule / initial water failed");
          return SLG_EHIP;
       }
    }
-   HIPOK(hipMemcpy(d + off[13], tables.data(), tables.size() * rs, hipMemcpyHostToDevice));
-   HIPOK(hipMemcpy(d + off[14], tbase.data(), (size_t) S * 4, hipMemcpyHostToDevice));
-   HIPOK(hipMemcpy(d + off[15], nst.data(), (size_t) S * 4, hipMemcpyHostToDevice));
+   HIPOK_T(hipMemcpy(d + off[13], tables.data(), tables.size() * rs, hipMemcpyHostToDevice));
+   HIPOK_T(hipMemcpy(d + off[14], tbase.data(), (size_t) S * 4, hipMemcpyHostToDevice));
+   HIPOK_T(hipMemcpy(d + off[15], nst.data(), (size_t) S * 4, hipMemcpyHostToDevice));
    D.params = (const R *) (d + off[12]);
    D.tables = (const R *) (d + off[13]);
    D.tbase = (const int *) (d + off[14]);
    D.nsteps = (const int *) (d + off[15]);
    D.cur = (int *) (d + off[16]);   // zeroed with the rest
-   HIPOK(hipStreamCreate(&X->st));
+   HIPOK_T(hipStreamCreate(&X->st));
    X->step = 0;
    X->parity = 0;
+   if (X->tide_mode) {
+      // TIDESH of every run, then the tide's first call on the initial water
+      // (RunInit, slosh2.c:1851-1868)
+      const size_t nb = (size_t) std::max(b->nbcpt, 1) * S;
+      R *t = nullptr;
+      HIPOK_T(hipMalloc((void **) &t, nb * rs));
+      D.tidesh = t;
+      HIPOK_T(hipMemset(t, 0, nb * rs));
+      const int krc = dev_tide(&I.dev, &D, &X->tdev, D.hb0, hbd, 0, 1, X->st);
+      HIPOK_T(krc < 0 ? (hipError_t) -krc : (krc > 0 ? hipErrorInvalidValue : hipSuccess));
+      HIPOK_T(hipStreamSynchronize(X->st));
+   }
+   if (hbd) (void) hipFree(hbd);
+#undef HIPOK_T
    return 0;
 }
 
@@ -1157,14 +1192,19 @@ static int launch_whole(slg_batch *X, const SlBasinDevT<R> &B, const SlRunDevT<R
                         hipEvent_t *ev, int dyn)
 {
    const int r = dev_launch(&B, &D, smpt, filt, dyn, X->parity, X->path, X->tile_rows, X->st, ev);
-   if (r < 0 || !(X->hbmax || X->hbmax2)) return r;
+   if (r < 0) return r;
+   if (X->tide_mode) {   // the driver's tide after the step, before the maxima see HB
+      const int t = dev_tide(&B, &D, &X->tdev, r ? D.hb1 : D.hb0, nullptr, X->step + 1, 0, X->st);
+      if (t < 0) return t;
+   }
+   if (!(X->hbmax || X->hbmax2)) return r;
    const int m = dev_maxstat(&B, &D, r ? D.hb1 : D.hb0, img(X->b, R()).zbd, X->hbmax, X->hbmax2, 0, X->st);
    return m < 0 ? m : r;
 }
 
 static int launch_one(slg_batch *X, int smpt, int filt, hipEvent_t *ev, int dyn = 0, bool first = false)
 {
-   if (X->nsplit > 1 && X->path == SLG_PATH_FUSED) {
+   if (X->nsplit > 1 && X->path == SLG_PATH_FUSED && !X->tide_mode) {
       if (X->prec == 64) return launch_split(X, X->b->i64.dev, X->d64, smpt, filt, ev, dyn, first);
       return launch_split(X, X->b->i32.dev, X->d32, smpt, filt, ev, dyn, first);
    }
@@ -1309,7 +1349,7 @@ extern "C" int slg_batch_step(slg_batch *X, int n, char *err, int errlen)
 {
    const slg_basin *B = X->b;
    const int lim = X->maxsteps;   // storms with shorter runs stop at their own end
-   if (X->graph_steps > 0 && X->path == SLG_PATH_FUSED && X->nsplit == 1) {
+   if (X->graph_steps > 0 && X->path == SLG_PATH_FUSED && X->nsplit == 1 && !X->tide_mode) {
       // whole graphs while they fit, then single steps with the same kernels
       const int step0 = X->step;
       if (!X->gexec) {
@@ -1557,6 +1597,198 @@ extern "C" int slg_batch_set_boundary_tide(slg_batch *X, int storm, const double
    }
    return X->prec == 64 ? batch_tide<double>(X, storm, tide, n, err, errlen)
                         : batch_tide<float>(X, storm, tide, n, err, errlen);
+}
+
+// ---------------------------------------------------------------- gridded tide
+// DELT after each STMVAL call n = 1 .. nsteps of a run (what TMSTPM hands the
+// driver, which moves its model clock by it; slosh2.c:1553), from the storm
+// program: the state after SETCMP, changed by the events.
+static void run_delts(const storm_sched &S, std::vector<double> &dt)
+{
+   const std::vector<double> &g = S.prog;
+   const int n = (int) g[0], ic19 = (int) g[1], nev = (int) g[2];
+   const double *ev = g.data() + SQ_BT + (ic19 > 1 ? ic19 - 1 : 0);
+   double d = g[SQ_HDR + SQ_DELT];
+   dt.assign((size_t) n + 1, 0.);
+   for (int k = 1, e = 0; k <= n; k++) {
+      for (; e < nev && (int) ev[(size_t) e * SQ_EV] <= k; e++) d = ev[(size_t) e * SQ_EV + 1 + SQ_DELT];
+      dt[k] = d;
+   }
+}
+
+// The gridded tide of the reference's -f_tide V2 modes, applied on the device
+// between steps as its driver applies it (RunInit, slosh2.c:1789-1869;
+// SetTideUsableFlag, :717-744; CalcTideGrid2, :813-1027; RunLoopStep,
+// :1553-1582).  Each run's model clock starts at its start date (INITAL's
+// date, Clock_ScanDate + hour + minute) and moves by DELT every step; the
+// tide tables are the run's year's.  The cos / sin of every constituent's
+// angle at every step are taken here on the host (libm, as the driver takes
+// them), once per group of runs with the same year and clock.
+extern "C" int slg_batch_set_tide(slg_batch *X, const char *ft03, const char *bhc, const char *adj, int mode,
+                                  int thresh, char *err, int errlen)
+{
+   if (X->dmem || X->tide_mode) {
+      seterr(err, errlen, "%s", "the tide is set once, before the upload");
+      return SLG_EINPUT;
+   }
+   if (mode != 2 && mode != 21 && mode != 22 && mode != 24) {
+      seterr(err, errlen, "tide mode %s%d: 2, 21, 22 or 24 (-f_tide V2, V2.1, V2.2, V2.4)", "", mode);
+      return SLG_EINPUT;
+   }
+   static_assert(SL_NTIDE == SL_NUMT, "constituent count");
+   const sl_basin *b = &X->b->h;
+   const int S = X->nstorm;
+   char e[256];
+   sl_tide t;
+   if (sl_tide_open(&t, bhc, adj, b->imxb, b->jmxb, e)) {
+      seterr(err, errlen, "%s", e);
+      return SLG_EINPUT;
+   }
+   if (!strncmp(t.abbrev, "hsf", 3)) {   // RunLoopStep calls CalcTideGrid2 every 6 minutes there
+      sl_tide_free(&t);
+      seterr(err, errlen, "%s", "the 6-minute tide cadence of the hsf basins is not supported");
+      return SLG_EUNSUP;
+   }
+   const int ni = t.ni, nc = t.ncell;
+   // clock groups: key = year, then the clock at every step
+   std::map<std::vector<double>, int> gidx;
+   std::vector<int> sgrp(S), years, gyear, gmax;
+   std::vector<std::vector<double>> gclk;
+   std::vector<double> dt, key;
+   for (int s = 0; s < S; s++) {
+      int d[5];
+      slg_batch_start_date(X, s, d);
+      double c = sl_clock_scan_date(d[0], d[1], d[2]);
+      c = c + (d[3] * 3600.);
+      c = c + (d[4] * 60);
+      run_delts(X->sc[s], dt);
+      key.assign(1, (double) d[0]);
+      key.push_back(c);
+      for (size_t k = 1; k < dt.size(); k++) {
+         c = c + dt[k];
+         key.push_back(c);
+      }
+      auto it = gidx.find(key);
+      if (it == gidx.end()) {
+         const int g = (int) gclk.size();
+         it = gidx.emplace(key, g).first;
+         gclk.emplace_back(key.begin() + 1, key.end());
+         int y = 0;
+         while (y < (int) years.size() && years[y] != d[0]) y++;
+         if (y == (int) years.size()) years.push_back(d[0]);
+         gyear.push_back(y);
+         gmax.push_back((int) gclk.back().size() - 1);
+      }
+      sgrp[s] = it->second;
+   }
+   const int G = (int) gclk.size(), Y = (int) years.size();
+   // cos / sin of each constituent's angle at each step of each group
+   double ang[SL_NUMT];
+   for (int k = 0; k < SL_NUMT; k++) ang[k] = sl_tide_speed(k);
+   std::vector<long> goff(G);
+   size_t ncs = 0;
+   for (int g = 0; g < G; g++) {
+      goff[g] = (long) ncs;
+      ncs += gclk[g].size() * 2 * SL_NUMT;
+   }
+   // (the reference's build computes each pair with one sincos call, and
+   // glibc's sincos differs from separate sin and cos in about 0.1 % of
+   // arguments, so it is taken the same way here)
+   std::vector<double> casa(ncs);
+   for (int g = 0; g < G; g++) {
+      const double ac = sl_clock_scan_date(years[gyear[g]], 1, 1);
+      double *o = &casa[goff[g]];
+      for (double clk : gclk[g]) {
+         const double now = (clk - ac) / 3600.;
+         for (int k = 0; k < SL_NUMT; k++) sincos(ang[k] * now, &o[SL_NUMT + k], &o[k]);
+         o += 2 * SL_NUMT;
+      }
+   }
+   // squares: where the tide is added, and where it replaces HB over the
+   // static height; ZB in double
+   std::vector<unsigned char> sub((size_t) b->imxb * b->jmxb, 0), cfl(nc);
+   for (int l = 1; l <= b->nsqrwc; l++) {   // ISSUBGRID (issubgrid.f:14-33)
+      const bool narrow = l <= b->nsqrw ? b->delcut[l - 1] < 1.
+                                        : (b->cutli[l - b->nsqrw - 1] < 1. || b->cutle[l - b->nsqrw - 1] < 1.);
+      const int i = b->isqr[l - 1], j = b->jsqr[l - 1];
+      if (narrow && i >= 1 && i <= b->imxb && j >= 1 && j <= b->jmxb) sub[PIX(b, i, j)] = 1;
+   }
+   std::vector<double> zb(nc);
+   for (int c = 0; c < nc; c++) {
+      const int i = c % ni + 1, j = c / ni + 1;
+      const double z = b->zb[PIX(b, i, j)];
+      bool use = true;
+      if (mode != 2) {
+         if (z <= thresh) use = false;
+         else if (mode != 21 && sub[PIX(b, i, j)]) use = false;
+      }
+      cfl[c] = ((mode == 21 || mode == 22) && z >= 290) ? 2 : use ? 1 : 0;
+      zb[c] = z;
+   }
+   const int nb = b->nbcpt;
+   std::vector<int> bcc((size_t) std::max(nb, 1), 0);
+   for (int n = 0; n < nb; n++) bcc[n] = (b->jsh[n] - 1) * ni + (b->ish[n] - 1);
+   // one allocation: year tables, per-square values, groups, fields
+   const size_t tab = (size_t) SL_NUMT * nc;
+   size_t off[12], tot = 0;
+   int n = 0;
+   auto add = [&](size_t bytes) {
+      off[n++] = tot;
+      tot += (bytes + 255) & ~(size_t) 255;
+   };
+   add(Y * tab * 8); add(Y * tab * 8); add(Y * tab * 8);    // amp cv sv
+   add((size_t) nc * 8); add((size_t) nc * 8); add(nc);     // mtl zb cfl
+   add(bcc.size() * 4);                                     // bccell
+   add(ncs * 8); add((size_t) G * 8);                        // casa goff
+   add((size_t) G * 4); add((size_t) G * 4); add((size_t) S * 4);   // gyear gmax sgrp
+   const size_t foff = tot;
+   tot += (size_t) G * 2 * nc * 8;                          // field
+   char *d = nullptr;
+   std::vector<double> amp(tab), cv(tab), sv(tab);
+   hipError_t he = hipMalloc((void **) &d, tot);
+   for (int y = 0; y < Y && he == hipSuccess; y++) {
+      if (sl_tide_year(&t, ft03, years[y], amp.data(), cv.data(), sv.data(), e)) {
+         sl_tide_free(&t);
+         (void) hipFree(d);
+         seterr(err, errlen, "%s", e);
+         return SLG_EINPUT;
+      }
+      he = hipMemcpy(d + off[0] + y * tab * 8, amp.data(), tab * 8, hipMemcpyHostToDevice);
+      if (he == hipSuccess) he = hipMemcpy(d + off[1] + y * tab * 8, cv.data(), tab * 8, hipMemcpyHostToDevice);
+      if (he == hipSuccess) he = hipMemcpy(d + off[2] + y * tab * 8, sv.data(), tab * 8, hipMemcpyHostToDevice);
+   }
+   const void *src[12] = {nullptr, nullptr, nullptr, t.mtl, zb.data(), cfl.data(), bcc.data(), casa.data(),
+                          goff.data(), gyear.data(), gmax.data(), sgrp.data()};
+   const size_t len[12] = {0, 0, 0, (size_t) nc * 8, (size_t) nc * 8, (size_t) nc, bcc.size() * 4, ncs * 8,
+                           (size_t) G * 8, (size_t) G * 4, (size_t) G * 4, (size_t) S * 4};
+   for (int k = 3; k < 12 && he == hipSuccess; k++) he = hipMemcpy(d + off[k], src[k], len[k], hipMemcpyHostToDevice);
+   sl_tide_free(&t);
+   if (he != hipSuccess) {
+      if (d) (void) hipFree(d);
+      seterr(err, errlen, "HIP error %s (%d)", hipGetErrorString(he), (int) he);
+      return SLG_EHIP;
+   }
+   SlTideDev &T = X->tdev;
+   T.ni = ni;
+   T.ncell = nc;
+   T.nbcpt = nb;
+   T.ngrp = G;
+   T.amp = (const double *) (d + off[0]);
+   T.cv = (const double *) (d + off[1]);
+   T.sv = (const double *) (d + off[2]);
+   T.mtl = (const double *) (d + off[3]);
+   T.zb = (const double *) (d + off[4]);
+   T.cfl = (const uint8_t *) (d + off[5]);
+   T.bccell = (const int *) (d + off[6]);
+   T.casa = (const double *) (d + off[7]);
+   T.goff = (const long *) (d + off[8]);
+   T.gyear = (const int *) (d + off[9]);
+   T.gmax = (const int *) (d + off[10]);
+   T.sgrp = (const int *) (d + off[11]);
+   T.field = (double *) (d + foff);
+   X->tide_mem = d;
+   X->tide_mode = mode;
+   return 0;
 }
 
 // STATICHT (staticht.f:14-41): the static height at square (i,j) from the
@@ -2166,7 +2398,7 @@ extern "C" int slg_batch_maxstat2_env(slg_batch *X, short *env, char *err, int e
 
 // Datums of a run's track for the env trailer (ReadTrkFile, slosh2.c:1885-1975):
 // OceanDatum and LakeDatum; with a CanalDatum line, LakeDatum and CanalDatum.
-extern "C" int slg_track_datums(const char *trk_path, float *ht1, float *ht2)
+static int read_datums(const char *trk_path, float *ht1, float *ht2)
 {
    FILE *fp = fopen(trk_path, "r");
    char line[512];
@@ -2183,7 +2415,22 @@ extern "C" int slg_track_datums(const char *trk_path, float *ht1, float *ht2)
    const bool canal = cn > -100;   // Okeechobee-style datums: lake and canal
    *ht1 = canal ? lk : oc;
    *ht2 = canal ? cn : lk;
+   return 0;
+}
+
+extern "C" int slg_track_datums(const char *trk_path, float *ht1, float *ht2)
+{
+   if (read_datums(trk_path, ht1, ht2)) return SLG_EINPUT;
    sl_datum_override(ht1, ht2);    // slosh2.c:2238, a surge-only run
+   return 0;
+}
+
+// The same for a run with the gridded tide (InitWater_TideModeOverride,
+// slosh2.c:408-482, f_tide = mode).
+extern "C" int slg_track_datums_tide(const char *trk_path, int mode, float *ht1, float *ht2)
+{
+   if ((mode != 2 && mode != 21 && mode != 22 && mode != 24) || read_datums(trk_path, ht1, ht2)) return SLG_EINPUT;
+   sl_datum_override_tide(ht1, ht2, mode);
    return 0;
 }
 
@@ -2294,6 +2541,9 @@ extern "C" void slg_batch_close(slg_batch *X)
       (void) hipStreamDestroy(X->st);
       (void) hipFree(X->dmem);
    }
+   if (X->d32.tidesh) (void) hipFree(X->d32.tidesh);
+   if (X->d64.tidesh) (void) hipFree(X->d64.tidesh);
+   if (X->tide_mem) (void) hipFree(X->tide_mem);
    delete X;
 }
 

@@ -129,4 +129,29 @@ struct SlRunDevT {
                                  kernels' workgroup order (1: storm by storm) */
 };
 
+/* The gridded tide between steps (-f_tide V2 modes; CalcTideGrid2,
+ * slosh2.c:813-1027).  Runs that share a year and a model clock form a clock
+ * group; a group's tide field is summed once per step and every run of the
+ * group takes it from there.  Squares are the (IMXB-1) x (JMXB-1) grid of the
+ * harmonic constants, I fastest. */
+#define SL_NTIDE 37
+struct SlTideDev {
+   int ni, ncell;             /* squares per row (IMXB-1), squares */
+   int nbcpt, ngrp;
+   const double *amp, *cv, *sv;   /* [nyear][SL_NTIDE][ncell]: nodal factor x amplitude,
+                                     cos and sin of (equilibrium argument - phase) */
+   const double *mtl;         /* [ncell]: NAVD88 - MTL */
+   const double *zb;          /* [ncell]: ZB in double */
+   const uint8_t *cfl;        /* [ncell]: 1 the tide is added (SetTideUsableFlag),
+                                 2 HB = STATICHT + tide (V2.1 / V2.2, ZB >= 290) */
+   const int *bccell;         /* [nbcpt]: the square of each BDRYHT point */
+   const double *casa;        /* group g from goff[g]: per step 0 .. gmax[g] the cos, then
+                                 the sin, of each constituent's speed x hours */
+   const long *goff;          /* [ngrp] */
+   const int *gyear;          /* [ngrp]: index of the group's year in amp, cv, sv */
+   const int *gmax;           /* [ngrp]: last step of the group's longest run */
+   const int *sgrp;           /* [nstorm]: the group of each run */
+   double *field;             /* [ngrp][2][ncell]: tide at the group's step n in slot n & 1 */
+};
+
 #endif

@@ -14,6 +14,8 @@
 #ifndef SL_HOST_H
 #define SL_HOST_H
 
+#include <stdint.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -153,6 +155,30 @@ int sl_meow_seeds(const sl_basin *b, int xyflag, double lat, double lon, double 
  * (lat, lon) (deg) on bearing bear (deg from north). */
 void sl_earth_jump(double lat, double lon, double dist, double bear, double *lat2, double *lon2);
 
+/* The basin's harmonic constants for the gridded astronomical tide (-f_tide
+ * V2 modes; tideutil.c): per square of the (IMXB-1) x (JMXB-1) grid, in the
+ * file's order (I fastest), the 37 amplitudes (1e-6 ft) and phases (1e-6 deg)
+ * as read, the NAVD88 - MTL offset and the square's lon / lat. */
+#define SL_NUMT 37
+typedef struct sl_tide {
+   int ni, nj, ncell;
+   char abbrev[6];
+   int32_t *amp, *phs;           /* [ncell][SL_NUMT] */
+   double *mtl, *lon, *lat;      /* [ncell] */
+} sl_tide;
+/* seconds from 1970-01-01 00:00 to 00:00 of the date (Clock_ScanDate) */
+double sl_clock_scan_date(int year, int month, int day);
+/* constituent k's speed in radians per hour */
+double sl_tide_speed(int k);
+/* bhc (HarmConstit v1) and adj (datum adjustment text; NULL or missing: 0)
+ * for a basin of imxb x jmxb points */
+int sl_tide_open(sl_tide *t, const char *bhc, const char *adj, int imxb, int jmxb, char err[256]);
+/* one year's tables from ft03, [SL_NUMT][ncell] each: nodal factor x
+ * amplitude, cos and sin of (equilibrium argument - phase) */
+int sl_tide_year(const sl_tide *t, const char *ft03, int year, double *xamp, double *cosv, double *sinv,
+                 char err[256]);
+void sl_tide_free(sl_tide *t);
+
 /* Track: parse the trk file and initialise the storm (INPUTN2, STERON, INTDLT,
  * INTVAL, SETCMP).  Needs the loaded basin.  sl_storm_init does the same from
  * a track already parsed or built in memory. */
@@ -186,6 +212,8 @@ double sl_intwnd(double pdrop, double rmax, double alto);
 /* The datums of a surge-only run after InitWater_TideModeOverride
  * (slosh2.c:345-482, f_tide 0), as the env and rex trailers carry them. */
 void sl_datum_override(float *ht1, float *ht2);
+/* The same for a run with the gridded tide -f_tide f_tide (2, 21, 22, 24). */
+void sl_datum_override_tide(float *ht1, float *ht2, int f_tide);
 
 #ifdef __cplusplus
 }

@@ -127,6 +127,12 @@ struct slg_batch {
    hipStream_t stx[SL_MAXSPLIT - 1] = {};
    hipEvent_t ev_join[SL_MAXSPLIT - 1] = {};
    hipEvent_t ev_phase[SL_MAXSPLIT][SLG_NKERN + 1] = {};   // marks of each part's first step
+   // the gridded tide between steps (slg_batch_set_tide): mode 2, 21, 22 or
+   // 24, or 0; its tables and fields on the device (tide_mem), the host's
+   // copy of the harmonic constants until the upload
+   int tide_mode = 0;
+   void *tide_mem = nullptr;
+   SlTideDev tdev = {};
 };
 static inline SlRunDevT<float> &run(slg_batch *X, float) { return X->d32; }
 static inline SlRunDevT<double> &run(slg_batch *X, double) { return X->d64; }
@@ -222,21 +228,35 @@ static inline int dev_sched(const double *prog, const long *off, int nstorm, dou
 
 extern "C" int sl_dev_intlht_f32(const double *prog, const long *off, const double *bm, const unsigned char *ocn,
                                  int imxb, int jmxb, int imxb1, int jmxb1, int pitch, const SlRunDevT<float> *R,
-                                 hipStream_t st);
+                                 double *hbd, hipStream_t st);
 extern "C" int sl_dev_intlht_f64(const double *prog, const long *off, const double *bm, const unsigned char *ocn,
                                  int imxb, int jmxb, int imxb1, int jmxb1, int pitch, const SlRunDevT<double> *R,
-                                 hipStream_t st);
+                                 double *hbd, hipStream_t st);
 static inline int dev_intlht(const double *prog, const long *off, const double *bm, const unsigned char *ocn,
                              int imxb, int jmxb, int imxb1, int jmxb1, int pitch, const SlRunDevT<float> *R,
-                             hipStream_t st)
+                             double *hbd, hipStream_t st)
 {
-   return sl_dev_intlht_f32(prog, off, bm, ocn, imxb, jmxb, imxb1, jmxb1, pitch, R, st);
+   return sl_dev_intlht_f32(prog, off, bm, ocn, imxb, jmxb, imxb1, jmxb1, pitch, R, hbd, st);
 }
 static inline int dev_intlht(const double *prog, const long *off, const double *bm, const unsigned char *ocn,
                              int imxb, int jmxb, int imxb1, int jmxb1, int pitch, const SlRunDevT<double> *R,
-                             hipStream_t st)
+                             double *hbd, hipStream_t st)
 {
-   return sl_dev_intlht_f64(prog, off, bm, ocn, imxb, jmxb, imxb1, jmxb1, pitch, R, st);
+   return sl_dev_intlht_f64(prog, off, bm, ocn, imxb, jmxb, imxb1, jmxb1, pitch, R, hbd, st);
+}
+extern "C" int sl_dev_tide_f32(const SlBasinDevT<float> *B, const SlRunDevT<float> *R, const SlTideDev *T, float *hb,
+                               const double *hbd, int n, int first, hipStream_t st);
+extern "C" int sl_dev_tide_f64(const SlBasinDevT<double> *B, const SlRunDevT<double> *R, const SlTideDev *T,
+                               double *hb, const double *hbd, int n, int first, hipStream_t st);
+static inline int dev_tide(const SlBasinDevT<float> *B, const SlRunDevT<float> *R, const SlTideDev *T, float *hb,
+                           const double *hbd, int n, int first, hipStream_t st)
+{
+   return sl_dev_tide_f32(B, R, T, hb, hbd, n, first, st);
+}
+static inline int dev_tide(const SlBasinDevT<double> *B, const SlRunDevT<double> *R, const SlTideDev *T, double *hb,
+                           const double *hbd, int n, int first, hipStream_t st)
+{
+   return sl_dev_tide_f64(B, R, T, hb, hbd, n, first, st);
 }
 
 extern "C" int sl_dev_slot_init_f32(const SlBasinDevT<float> *B, const SlRunDevT<float> *R, int slot, int tstart,

@@ -2388,6 +2388,84 @@ __global__ void k_rex_pack(const uint8_t *src, long cap, const int *nb, const lo
    for (long b = threadIdx.x; b < nb[k]; b += blockDim.x) d[b] = s[b];
 }
 
+// ------------------------------------------------------------------ gridded tide
+// The tide of every square of clock group g at the group's step n: the sum of
+// CalcTideGrid2 (slosh2.c:875-885, 946-956), in double and in its order:
+// -NAVD88_MTL, then for each constituent amplitude x (CA cos - SA sin).
+__global__ void __launch_bounds__(256) k_tide_field(SlTideDev T, int n)
+{
+   const int c = blockIdx.x * 256 + threadIdx.x, g = blockIdx.y;
+   if (c >= T.ncell || n > T.gmax[g]) return;
+   const double *ca = T.casa + T.goff[g] + (size_t) n * 2 * SL_NTIDE, *sa = ca + SL_NTIDE;
+   const size_t yo = (size_t) T.gyear[g] * SL_NTIDE * T.ncell + c;
+   const double *a = T.amp + yo, *cv = T.cv + yo, *sv = T.sv + yo;
+   double t = -T.mtl[c];
+   for (int k = 0; k < SL_NTIDE; k++) {
+      const size_t o = (size_t) k * T.ncell;
+      t = t + a[o] * (ca[k] * cv[o] - sa[k] * sv[o]);
+   }
+   T.field[((size_t) g * 2 + (n & 1)) * T.ncell + c] = t;
+}
+
+// STATICHT (staticht.f:14-41) at square (i,j) as the tide coupling calls it
+// after step n: from the storm values and radial table of the run's next step
+// (after its last step, of that step)
+__device__ real static_height(const SlBasinDev &B, const SlRunDev &R, int s, int n, int i, int j)
+{
+   const int k0 = min(n, R.nsteps[s] - 1);
+   const real *P = par(R, s, k0 + 1), *Tb = tabv(R, s, P);
+   const real xr = B.elpcl[i - 1] * B.cosl[j - 1], yr = B.elpdl[i - 1] * B.sinl[j - 1];
+   const real x = xr - P[P_C1] - P[P_AX], y = yr - P[P_C2] - P[P_AY];
+   const real rsq = x * x + y * y, r1 = sqrt(rsq) / real(5280.) + real(1.);
+   int k = (int) r1;
+   const real r2 = (real) k, dr = r1 - r2;
+   if (k > 790) k = 790;
+   const real *Rk = Tb + (uint32_t) k * 8, *Rk1 = Rk + 8;
+   return Rk[R_DELP] + dr * (Rk1[R_DELP] - Rk[R_DELP]) + P[P_SEA];
+}
+
+// CalcTideGrid2's change to HB and TIDESH (slosh2.c:859-1025) for every run
+// still running at step n; first: the call before the first step, on the
+// initial water (hbd, when given, holds that water in double).  hb is the
+// buffer that holds each run's HB.  HB goes through double as the driver's
+// COMMON copy does, and is rounded to real where the device keeps it.  The
+// old tide of a square is the group's field of step n-1: the driver's stored
+// value and its recomputation from the previous call's angles are that sum.
+__global__ void __launch_bounds__(256) k_tide_apply(SlBasinDev B, SlRunDev R, SlTideDev T, real *hb,
+                                                    const double *hbd, int n, int first)
+{
+   const int p = blockIdx.x * 256 + threadIdx.x, s = blockIdx.y;
+   if (p >= T.ncell + T.nbcpt || (!first && R.nsteps[s] < n)) return;
+   const int g = T.sgrp[s];
+   const double *fn = T.field + ((size_t) g * 2 + (n & 1)) * T.ncell;
+   if (p >= T.ncell) {   // the boundary squares: TIDESH = the tide at step n
+      const int b = p - T.ncell;
+      R.tidesh[(size_t) s * T.nbcpt + b] = (real) fn[T.bccell[b]];
+      return;
+   }
+   const int i = p % T.ni, j = p / T.ni;
+   const size_t o = (size_t) s * R.fstride + (size_t) j * B.pitch + i;
+   const double zb = T.zb[p];
+   if (first) {   // every wet square (slosh2.c:859-896)
+      const double h = hbd ? hbd[o] : (double) hb[o];
+      if (h + zb != 0.) {
+         const double x = h + fn[p];
+         hb[o] = (x + zb >= 0.) ? (real) x : (real) (-zb);
+      }
+      return;
+   }
+   const uint8_t f = T.cfl[p];
+   if (f & 2) {   // V2.1 / V2.2 deep squares (slosh2.c:904-931)
+      hb[o] = (real) ((double) static_height(B, R, s, n, i + 1, j + 1) + fn[p]);
+   } else if (f & 1) {   // usable wet squares: subtract the old tide, add the new
+      const double h = (double) hb[o];
+      if (h + zb != 0.) {
+         const double x = h - T.field[((size_t) g * 2 + ((n - 1) & 1)) * T.ncell + p] + fn[p];
+         hb[o] = (x + zb >= 0.) ? (real) x : (real) (-zb);
+      }
+   }
+}
+
 }   // namespace SL_NS
 
 using namespace SL_NS;
@@ -2433,11 +2511,12 @@ extern "C" int SL_SLOTINIT(const SlBasinDev *B, const SlRunDev *R, int slot, int
 // least -ZB; the others a datum (the lake's, or the sea's on MSY's open
 // squares), at least -ZB; listed squares dry out (-ZB).  Row JMXB repeats row
 // 1.  In double (the host's operations in its order), rounded to R into HB
-// (both buffers) and HMX (squares only).  bm: ELPCL[imxb], ELPDL[imxb],
+// (both buffers) and HMX (squares only), and kept in double in hbd when given
+// (the tide's first call adds to that value).  bm: ELPCL[imxb], ELPDL[imxb],
 // COSL[jmxb], SINL[jmxb], then ZB[jmxb][imxb] (packed, I fastest).
 __global__ void __launch_bounds__(256) k_intlht(const double *prog, const long *off, const double *bm,
                                                 const uint8_t *ocn, int imxb, int jmxb, int imxb1, int jmxb1,
-                                                int pitch, SlRunDev R)
+                                                int pitch, SlRunDev R, double *hbd)
 {
    const int s = blockIdx.y, p = blockIdx.x * 256 + threadIdx.x;
    if (p >= imxb * jmxb) return;
@@ -2472,6 +2551,7 @@ __global__ void __launch_bounds__(256) k_intlht(const double *prog, const long *
    R.hb0[o] = v;
    R.hb1[o] = v;
    R.hmx[o] = (i == imxb || j == jmxb) ? real(0.) : v;
+   if (hbd) hbd[o] = hb;
 }
 
 #ifdef SL_F64
@@ -2480,13 +2560,13 @@ __global__ void __launch_bounds__(256) k_intlht(const double *prog, const long *
 #define SL_INTLHT sl_dev_intlht_f32
 #endif
 extern "C" int SL_INTLHT(const double *prog, const long *off, const double *bm, const unsigned char *ocn, int imxb,
-                         int jmxb, int imxb1, int jmxb1, int pitch, const SlRunDev *R, hipStream_t st)
+                         int jmxb, int imxb1, int jmxb1, int pitch, const SlRunDev *R, double *hbd, hipStream_t st)
 {
    if (R->nstorm < 1 || R->nstorm > 65535 || imxb < 2 || jmxb < 2 || pitch < imxb ||
        R->fstride < (size_t) pitch * jmxb)
       return -1;
    hipLaunchKernelGGL(k_intlht, dim3((imxb * jmxb + 255) / 256, R->nstorm), dim3(256), 0, st, prog, off, bm, ocn,
-                      imxb, jmxb, imxb1, jmxb1, pitch, *R);
+                      imxb, jmxb, imxb1, jmxb1, pitch, *R, hbd);
    hipError_t e = hipGetLastError();
    return e == hipSuccess ? 0 : -(int) e;
 }
@@ -2567,6 +2647,25 @@ extern "C" int SL_MAXSTAT(const SlBasinDev *B, const SlRunDev *R, const real *hb
    const int nx = (B->imxb + 63) / 64;
    hipLaunchKernelGGL(k_maxstat, dim3(nx * (B->jmxb - 1), R->nstorm), dim3(64), 0, st, *B, *R, hb, zbd, hmax,
                       hmax2, init);
+   hipError_t e = hipGetLastError();
+   return e == hipSuccess ? 0 : -(int) e;
+}
+
+#ifdef SL_F64
+#define SL_TIDE sl_dev_tide_f64
+#else
+#define SL_TIDE sl_dev_tide_f32
+#endif
+// the tide after step n (first: before the first step) of every run of R
+extern "C" int SL_TIDE(const SlBasinDev *B, const SlRunDev *R, const SlTideDev *T, real *hb, const double *hbd, int n,
+                       int first, hipStream_t st)
+{
+   if (T->ngrp < 1 || T->ngrp > 65535 || R->nstorm < 1 || R->nstorm > 65535 || T->ncell < 1 || !R->tidesh ||
+       T->ni < 1 || T->ncell % T->ni || T->ni > B->imxb - 1 || T->ncell / T->ni > B->jmxb - 1)
+      return -1;
+   hipLaunchKernelGGL(k_tide_field, dim3((T->ncell + 255) / 256, T->ngrp), dim3(256), 0, st, *T, n);
+   hipLaunchKernelGGL(k_tide_apply, dim3((T->ncell + T->nbcpt + 255) / 256, R->nstorm), dim3(256), 0, st, *B, *R,
+                      *T, hb, hbd, n, first);
    hipError_t e = hipGetLastError();
    return e == hipSuccess ? 0 : -(int) e;
 }

@@ -162,7 +162,11 @@ int slg_rex_frame(slg_rex *r, const double track[6], double clock, const unsigne
  * surge-only run, in float as the reference holds them. */
 void sl_datum_override(float *ht1, float *ht2)
 {
-   (void) ht2;   /* changed only by the gridded-tide branches */
+   sl_datum_override_tide(ht1, ht2, 0);
+}
+
+void sl_datum_override_tide(float *ht1, float *ht2, int f_tide)
+{
    const float h = *ht1;
    float anom;
    if ((double) h == 99.9 || (double) h == 999.9) anom = 99.9f;
@@ -176,12 +180,21 @@ void sl_datum_override(float *ht1, float *ht2)
       const int adjust = (int) ((h + 50.f) / 100.f);
       mode = (adjust >= 0 && adjust <= 8 && adjust != 1) ? by_adjust[adjust] : 0;
    }
-   if (mode == 1 || mode == 2 || mode == 3 || mode == -1 || mode == -2 || mode == -3) mode = 9;
+   if (f_tide == 0) {
+      if (mode == 1 || mode == 2 || mode == 3 || mode == -1 || mode == -2 || mode == -3) mode = 9;
+   } else {   /* the gridded tide V2 modes (slosh2.c:444-452) */
+      if (mode == 10 || mode == 11) {
+         anom = 0.f;
+         *ht2 = 0.f;
+      }
+      mode = 2;
+   }
    int a;
-   switch (mode) {
+   switch (mode) {   /* InitWater_New, slosh2.c:345-381 */
       case 10: *ht1 = 99.9f; return;
       case 11: *ht1 = 999.9f; return;
       case 9: a = 2; break;
+      case 2: a = 6; break;
       default: a = 0; break;   /* mode 0 */
    }
    *ht1 = (float) (a * 100) + anom;

@@ -58,6 +58,8 @@ def lib():
         L.slg_batch_autotune.argtypes = [vp, i, ctypes.POINTER(i), ctypes.POINTER(i), vp, cp, i]
         L.slg_batch_envelopes.argtypes = [vp, vp, vp, cp, i]
         L.slg_track_datums.argtypes = [cp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]
+        L.slg_track_datums_tide.argtypes = [cp, i, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]
+        L.slg_batch_set_tide.argtypes = [vp, cp, cp, cp, i, i, cp, i]
         L.slg_batch_rex_frames.argtypes = [vp, cp, i, i, vp, ctypes.c_long, vp, cp, i]
         L.slg_batch_track.argtypes = [vp, i, i, vp]
         L.slg_basin_field.argtypes = [vp, cp, vp]
@@ -278,6 +280,12 @@ class Batch:
                                       ms.ctypes.data_as(ctypes.c_void_p), err, 512), err)
         return rows.value, path.value, ms
 
+    def set_tide(self, ft03: str, bhc: str, adj: str | None, mode: int, thresh: int = 0):
+        """The gridded tide between steps (-f_tide V2 modes); before upload()."""
+        err = ctypes.create_string_buffer(512)
+        _chk(lib().slg_batch_set_tide(self.h, ft03.encode(), bhc.encode(), adj.encode() if adj else None, mode,
+                                      thresh, err, 512), err)
+
     def upload(self):
         err = ctypes.create_string_buffer(512)
         _chk(lib().slg_batch_upload(self.h, err, 512), err)
@@ -469,11 +477,28 @@ class PointSeries:
             self.h = ctypes.c_void_p()
 
 
-def track_datums(trk: str):
+def track_datums(trk: str, tide_mode: int = 0):
+    """The env / rex trailer datums of a run from its trk file; tide_mode: those
+    of a run with the gridded tide in that mode (2, 21, 22, 24)."""
     a, b = ctypes.c_float(), ctypes.c_float()
-    if lib().slg_track_datums(trk.encode(), ctypes.byref(a), ctypes.byref(b)):
+    if tide_mode:
+        rc = lib().slg_track_datums_tide(trk.encode(), tide_mode, ctypes.byref(a), ctypes.byref(b))
+    else:
+        rc = lib().slg_track_datums(trk.encode(), ctypes.byref(a), ctypes.byref(b))
+    if rc:
         raise SloshError("cannot read %s" % trk)
     return a.value, b.value
+
+
+def parse_tide(flag: str):
+    """(mode, thresh) of an -f_tide V2 value as ParseTide reads it
+    (usrparse.c:14-52): V2, V2.1[.x], V2.2[.x], V2.4[.x]."""
+    if flag == "V2":
+        return 2, 0
+    for head, mode in (("V2.1", 21), ("V2.2", 22), ("V2.4", 24)):
+        if flag.startswith(head):
+            return mode, (int(flag[5:]) if len(flag) > 4 else 0)
+    raise SloshError("not a gridded-tide V2 mode: %r" % flag)
 
 
 def write_env(path: str, vals: np.ndarray, ht1: float, ht2: float, comment: str = "FileVersion=2013"):

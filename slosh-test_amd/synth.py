@@ -436,6 +436,7 @@ class Track:
     interp: str = "Linear"
     ocean: float = 0.0  # OceanDatum, LakeDatum (ft)
     lake: float = 0.0
+    date: tuple = (2025, 9, 1, 0, 0)   # DateTime: year, month, day, hour, minute
 
 
 def straight_track(b: Basin, speed_kt=12.0, heading=330.0, dp=60.0, rmw=25.0,
@@ -493,7 +494,7 @@ def spline_track(b: Basin, seed=2, hours=72, dp=(40.0, 80.0), rmw=(15.0, 30.0),
 def trk_text(t: Track) -> str:
     L = ["FileVersion=2013",
          "# synthetic track (seeded)",
-         "DateTime=2025-09-01T00:00:00",
+         "DateTime=%04d-%02d-%02dT%02d:%02d:00" % tuple(t.date),
          "DateHour=%d" % t.hours[0],
          "LandfallHour=%d" % t.landfall,
          "BeginHour=%d" % t.begin,
@@ -512,7 +513,8 @@ def trk_text(t: Track) -> str:
 def track_dict(t: Track) -> dict:
     """The track as slg_track fields (sloshgpu.track_struct): the values a trk
     file of trk_text(t) holds, rounded as that file writes them."""
-    return dict(year=2025, month=9, day=1, hour_of_day=0, minute=0, date_hour=t.hours[0],
+    y, mo, d, h, mi = t.date
+    return dict(year=y, month=mo, day=d, hour_of_day=h, minute=mi, date_hour=t.hours[0],
                 landfall_hour=t.landfall, begin_hour=t.begin, end_hour=t.end,
                 ocean_datum=float("%.1f" % t.ocean), lake_datum=float("%.1f" % t.lake),
                 spline=1 if t.interp == "Spline" else 0,
@@ -708,14 +710,33 @@ def bhc_bytes(b: Basin, abbrev: str, seed=4) -> bytes:
     return head + cells.astype("<i4").tobytes()
 
 
-def write_tide_db(root: str, b: Basin, abbrev: str, seed=4, database=2014, flavor=None):
+def adj_text(b: Basin, seed=4) -> str:
+    """A datum-adjustment file as ReadAdjDatum reads it (tideutil.c:647-734):
+    the header line, then per square of the harmonic grid (I fastest) its
+    longitude, latitude, I, J and the NAVD88 - MTL offset (ft)."""
+    rng = np.random.default_rng(seed + 1000)
+    ni, nj = b.imxb - 1, b.jmxb - 1
+    lon, lat = int(-b.alno * 1e6) / 1e6, int(b.alto * 1e6) / 1e6
+    adj = np.round(rng.uniform(-0.6, 1.2) + 0.3 * np.linspace(0, 1, ni)[None, :]
+                   - 0.2 * np.linspace(0, 1, nj)[:, None], 3)
+    rows = ["Longitude, Latitude, I, J, Adj"]
+    for j in range(nj):
+        for i in range(ni):
+            rows.append("%.6f, %.6f, %d, %d, %.3f" % (lon, lat, i + 1, j + 1, adj[j, i]))
+    return "\n".join(rows) + "\n"
+
+
+def write_tide_db(root: str, b: Basin, abbrev: str, seed=4, database=2014, flavor=None, adj=False):
     """The tide inputs -f_tide reads under -rootDir (usrparse.c:432-436,
-    slosh2.c:2173-2217): <root>/tidefile.ec<database>/ft03.dta and <abbrev>.bhc
-    (no datum-adjustment file: the driver then takes the adjustment as 0), and
-    with flavor the basin's line of tide_flavor.txt for -f_tide VDEF
+    slosh2.c:2173-2217): <root>/tidefile.ec<database>/ft03.dta and <abbrev>.bhc,
+    with adj <abbrev>.adj (without it the driver takes the adjustment as 0),
+    and with flavor the basin's line of tide_flavor.txt for -f_tide VDEF
     (slosh2.c:2098-2160)."""
     d = os.path.join(root, "tidefile.ec%4d" % database)
     os.makedirs(d, exist_ok=True)
+    if adj:
+        with open(os.path.join(d, abbrev + ".adj"), "w") as fh:
+            fh.write(adj_text(b, seed))
     if flavor:
         with open(os.path.join(d, "tide_flavor.txt"), "w") as fh:
             fh.write("# basin:tide version\n%s:%s\n" % (abbrev.strip(), flavor))

@@ -360,13 +360,17 @@ def full_depth_case(root, hours=6):
     return case
 
 
-def tide_case(root, idx, hours=6):
-    """A case with its tide inputs; returns (case, -f_tide value)."""
+def tide_case(root, idx, hours=6, adj=False):
+    """A case with its tide inputs (adj: with a datum-adjustment file); returns
+    (case, -f_tide value)."""
     cid, flag, kind = TIDE_CASES[idx]
     b, t = tide_basin(kind, hours)
     case = make_case(root, b, t)
-    synth.write_tide_db(root, b, case["basin"], seed=50 + idx, flavor="V2.2.-3" if flag == "VDEF" else None)
+    d = synth.write_tide_db(root, b, case["basin"], seed=50 + idx, flavor="V2.2.-3" if flag == "VDEF" else None,
+                            adj=adj)
     case["b"] = b
+    case["t"] = t
+    case["tidedir"] = d
     return case, flag
 
 
