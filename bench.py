@@ -144,15 +144,21 @@ def build_ensemble(root, nstorm):
     return b, paths, trks
 
 
-def batched_run(dta, bnt, trks, prec, steps, warmup, streams, events, path=None, rows=32, span=0):
+def batched_run(dta, bnt, trks, prec, steps, warmup, streams, events, path=None, rows=32, span=0, tide=None):
     """Open and upload the batch, autotune (or fix the path), warm up, then time
     `steps` steps in windows spread to the end of the run.  With events the
     timed steps carry HIP events around every kernel on the batch's stream
-    (one stream), which give the per-kernel durations of those steps."""
+    (one stream), which give the per-kernel durations of those steps.  tide:
+    (ft03, bhc, adj, mode, thresh) of the gridded tide between steps."""
     B = sloshgpu.Basin(dta, bnt)
     t0 = time.time()
     X = sloshgpu.Batch(B, trks, precision=prec)
     t_sched = time.time() - t0
+    t_tide = None
+    if tide is not None:
+        t0 = time.time()
+        X.set_tide(*tide)
+        t_tide = time.time() - t0
     X.upload()
     tune = None
     if path is None:   # the online autotuner picks the launch configuration
@@ -193,7 +199,8 @@ def batched_run(dta, bnt, trks, prec, steps, warmup, streams, events, path=None,
     X.close()
     cells = B.cells
     B.close()
-    return dict(dt=dt, cells=cells, ms=ms, cnt=cnt, sched_s=t_sched, tune=tune, streams=nstreams, windows=wins)
+    return dict(dt=dt, cells=cells, ms=ms, cnt=cnt, sched_s=t_sched, tune=tune, streams=nstreams, windows=wins,
+                tide_s=t_tide)
 
 
 def setup_bench(dta, bnt, trks, tracks, prec):
@@ -550,9 +557,9 @@ def main():
     ap.add_argument("--span", type=int, default=0,
                     help="take the timed windows from the first SPAN steps after the warm-up only (0: to the end of "
                          "the run); for short profiling runs")
-    ap.add_argument("--parts", default="batched,kernels,fp32,setup,rex,plain,uneven,serve,cpu",
-                    help="comma list of batched, kernels, fp32, rex, plain, uneven, serve, cpu (and copy: "
-                         "the counter calibration copy)")
+    ap.add_argument("--parts", default="batched,kernels,fp32,tide,setup,rex,plain,uneven,serve,cpu",
+                    help="comma list of batched, kernels, fp32, tide, setup, rex, plain, uneven, serve, cpu "
+                         "(and copy: the counter calibration copy)")
     ap.add_argument("--precision", type=int, default=64, choices=[32, 64],
                     help="arithmetic of the headline (64: the reference's default build, bit for bit)")
     ap.add_argument("--storms", type=int, default=1024)
@@ -665,6 +672,21 @@ def main():
                            "windows": r32["windows"], "tune": r32["tune"],
                            "step_roofline_frac": rate[32] * b_alg_step(a.storms, 32) / (HBM_PEAK_GBS * 1e9)}
             log("batched FP32 %.3e storm.cell.steps/s" % rate[32])
+        if "tide" in parts:
+            # the same ensemble with the gridded tide (-f_tide V2.1.-5: STATICHT +
+            # tide on squares 290 ft deep, none above 5 ft of land) between steps
+            td = synth.write_tide_db(os.path.join(root, "tide"), b, "cfg3", seed=9, adj=True)
+            tide = (os.path.join(td, "ft03.dta"), os.path.join(td, "cfg3.bhc"), os.path.join(td, "cfg3.adj"), 21, -5)
+            log("batched with the gridded tide, FP%d" % prec)
+            rt = batched_run(paths["dta"], paths["bntline"], trks, prec, a.steps, a.warmup, 1, False,
+                             path, a.tile_rows, a.span, tide=tide)
+            vt = a.storms * rt["cells"] * a.steps / rt["dt"]
+            out["tide"] = {"workload": "the batched ensemble with -f_tide V2.1.-5 (37 constituents)",
+                           "value": vt, "ms_per_step": rt["dt"] * 1e3 / a.steps, "set_tide_s": rt["tide_s"],
+                           "tune": rt["tune"], "windows": rt["windows"]}
+            if prec in rate:
+                out["tide"]["vs_without_tide"] = vt / rate[prec]
+            log("tide: %.3e storm.cell.steps/s, set_tide %.2f s" % (vt, rt["tide_s"]))
         if "setup" in parts:
             tracks = [synth.track_dict(t) for t in synth.meow_tracks(b, n=a.storms)]
             out["setup"] = setup_bench(paths["dta"], paths["bntline"], trks, tracks, prec)

@@ -1000,20 +1000,6 @@ static int batch_upload(slg_batch *X, char *err, int errlen)
    // initial water (INTLHT) per storm: HB, HMX=HB, UB=VB=0 (from the host's,
    // or on the device below)
    const bool hostw = !X->hb0.empty();
-   // with the tide in single precision, that water also in double for the
-   // tide's first call (the driver adds it to the double value)
-   double *hbd = nullptr;
-   if (X->tide_mode && sizeof(R) == 4) HIPOK(hipMalloc((void **) &hbd, np * sizeof(double)));
-   auto fail = [&](hipError_t he) {
-      if (hbd) (void) hipFree(hbd);
-      seterr(err, errlen, "HIP error %s (%d)", hipGetErrorString(he), (int) he);
-      return SLG_EHIP;
-   };
-#define HIPOK_T(x)                                                                                             \
-   do {                                                                                                        \
-      hipError_t he_ = (x);                                                                                    \
-      if (he_ != hipSuccess) return fail(he_);                                                                 \
-   } while (0)
    std::vector<R> f(hostw ? fstride : 0, R(0));
    std::vector<R> params(X->sc[0].hostp ? (size_t) S * X->maxsteps * SL_NPAR : 0, R(0));
    std::vector<R> tables;
@@ -1023,18 +1009,12 @@ static int batch_upload(slg_batch *X, char *err, int errlen)
          const double *hb = &X->hb0[(size_t) s * b->imxb * NJ];
          for (int j = 1; j <= NJ; j++)
             for (int i = 1; i <= b->imxb; i++) f[(size_t) (j - 1) * P + (i - 1)] = (R) hb[PIX(b, i, j)];
-         HIPOK_T(hipMemcpy(D.hb0 + (size_t) s * fstride, f.data(), fstride * rs, hipMemcpyHostToDevice));
-         HIPOK_T(hipMemcpy(D.hb1 + (size_t) s * fstride, f.data(), fstride * rs, hipMemcpyHostToDevice));
+         HIPOK(hipMemcpy(D.hb0 + (size_t) s * fstride, f.data(), fstride * rs, hipMemcpyHostToDevice));
+         HIPOK(hipMemcpy(D.hb1 + (size_t) s * fstride, f.data(), fstride * rs, hipMemcpyHostToDevice));
          for (int j = 1; j <= NJ; j++)       // HMX = HB on squares only (intlht.f:172-177)
             for (int i = 1; i <= b->imxb; i++)
                if (i == b->imxb || j == NJ) f[(size_t) (j - 1) * P + (i - 1)] = R(0);
-         HIPOK_T(hipMemcpy(D.hmx + (size_t) s * fstride, f.data(), fstride * rs, hipMemcpyHostToDevice));
-         if (hbd) {
-            std::vector<double> fd(fstride, 0.);
-            for (int j = 1; j <= NJ; j++)
-               for (int i = 1; i <= b->imxb; i++) fd[(size_t) (j - 1) * P + (i - 1)] = hb[PIX(b, i, j)];
-            HIPOK_T(hipMemcpy(hbd + (size_t) s * fstride, fd.data(), fstride * 8, hipMemcpyHostToDevice));
-         }
+         HIPOK(hipMemcpy(D.hmx + (size_t) s * fstride, f.data(), fstride * rs, hipMemcpyHostToDevice));
       }
       const std::vector<R> &sp = sched_params(X->sc[s], R()), &st = sched_tables(X->sc[s], R());
       if (X->sc[s].hostp) memcpy(&params[(size_t) s * X->maxsteps * SL_NPAR], sp.data(), sp.size() * sizeof(R));
@@ -1042,7 +1022,7 @@ static int batch_upload(slg_batch *X, char *err, int errlen)
       tables.insert(tables.end(), st.begin(), st.end());
       nst[s] = X->sc[s].nsteps;
    }
-   if (X->sc[0].hostp) HIPOK_T(hipMemcpy(d + off[12], params.data(), params.size() * rs, hipMemcpyHostToDevice));
+   if (X->sc[0].hostp) HIPOK(hipMemcpy(d + off[12], params.data(), params.size() * rs, hipMemcpyHostToDevice));
    if (!X->sc[0].hostp || !hostw) {
       // the storm programs to the device, with (for the initial water) the
       // basin metrics ELPCL ELPDL COSL SINL, ZB and the squares in reach of
@@ -1069,7 +1049,7 @@ static int batch_upload(slg_batch *X, char *err, int errlen)
          for (int i = 1; i <= b->imxb1; i++) ocn[PIX(b, i, j)] = (unsigned char) sl_intlht_class(b, i, j);
       char *pd = nullptr;
       const size_t nbytes = buf.size() * sizeof(double) + (size_t) S * sizeof(long) + ocn.size();
-      HIPOK_T(hipMalloc((void **) &pd, nbytes));
+      HIPOK(hipMalloc((void **) &pd, nbytes));
       long *offd = (long *) (pd + buf.size() * sizeof(double));
       unsigned char *ocnd = (unsigned char *) (offd + S);
       hipError_t he = hipMemcpy(pd, buf.data(), buf.size() * sizeof(double), hipMemcpyHostToDevice);
@@ -1081,24 +1061,23 @@ static int batch_upload(slg_batch *X, char *err, int errlen)
                          b->ebsn1 == '+', 0);
       if (!krc && !hostw)
          krc = dev_intlht((const double *) pd, offd, (const double *) pd + np2, ocnd, ni, nj, b->imxb1, b->jmxb1, P,
-                          &D, hbd, 0);
+                          &D, 0);
       if (!krc) krc = -(int) hipDeviceSynchronize();
       (void) hipFree(pd);
       if (krc) {
-         if (hbd) (void) hipFree(hbd);
          seterr(err, errlen, "%s", "device storm schedule / initial water failed");
          return SLG_EHIP;
       }
    }
-   HIPOK_T(hipMemcpy(d + off[13], tables.data(), tables.size() * rs, hipMemcpyHostToDevice));
-   HIPOK_T(hipMemcpy(d + off[14], tbase.data(), (size_t) S * 4, hipMemcpyHostToDevice));
-   HIPOK_T(hipMemcpy(d + off[15], nst.data(), (size_t) S * 4, hipMemcpyHostToDevice));
+   HIPOK(hipMemcpy(d + off[13], tables.data(), tables.size() * rs, hipMemcpyHostToDevice));
+   HIPOK(hipMemcpy(d + off[14], tbase.data(), (size_t) S * 4, hipMemcpyHostToDevice));
+   HIPOK(hipMemcpy(d + off[15], nst.data(), (size_t) S * 4, hipMemcpyHostToDevice));
    D.params = (const R *) (d + off[12]);
    D.tables = (const R *) (d + off[13]);
    D.tbase = (const int *) (d + off[14]);
    D.nsteps = (const int *) (d + off[15]);
    D.cur = (int *) (d + off[16]);   // zeroed with the rest
-   HIPOK_T(hipStreamCreate(&X->st));
+   HIPOK(hipStreamCreate(&X->st));
    X->step = 0;
    X->parity = 0;
    if (X->tide_mode) {
@@ -1106,15 +1085,13 @@ static int batch_upload(slg_batch *X, char *err, int errlen)
       // (RunInit, slosh2.c:1851-1868)
       const size_t nb = (size_t) std::max(b->nbcpt, 1) * S;
       R *t = nullptr;
-      HIPOK_T(hipMalloc((void **) &t, nb * rs));
+      HIPOK(hipMalloc((void **) &t, nb * rs));
       D.tidesh = t;
-      HIPOK_T(hipMemset(t, 0, nb * rs));
-      const int krc = dev_tide(&I.dev, &D, &X->tdev, D.hb0, hbd, 0, 1, X->st);
-      HIPOK_T(krc < 0 ? (hipError_t) -krc : (krc > 0 ? hipErrorInvalidValue : hipSuccess));
-      HIPOK_T(hipStreamSynchronize(X->st));
+      HIPOK(hipMemset(t, 0, nb * rs));
+      const int krc = dev_tide(&I.dev, &D, &X->tdev, D.hb0, 0, 1, X->st);
+      HIPOK(krc < 0 ? (hipError_t) -krc : (krc > 0 ? hipErrorInvalidValue : hipSuccess));
+      HIPOK(hipStreamSynchronize(X->st));
    }
-   if (hbd) (void) hipFree(hbd);
-#undef HIPOK_T
    return 0;
 }
 
@@ -1194,7 +1171,7 @@ static int launch_whole(slg_batch *X, const SlBasinDevT<R> &B, const SlRunDevT<R
    const int r = dev_launch(&B, &D, smpt, filt, dyn, X->parity, X->path, X->tile_rows, X->st, ev);
    if (r < 0) return r;
    if (X->tide_mode) {   // the driver's tide after the step, before the maxima see HB
-      const int t = dev_tide(&B, &D, &X->tdev, r ? D.hb1 : D.hb0, nullptr, X->step + 1, 0, X->st);
+      const int t = dev_tide(&B, &D, &X->tdev, r ? D.hb1 : D.hb0, X->step + 1, 0, X->st);
       if (t < 0) return t;
    }
    if (!(X->hbmax || X->hbmax2)) return r;

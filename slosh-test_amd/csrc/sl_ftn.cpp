@@ -206,7 +206,9 @@ int inital_(int *mhalt, int *imxb, int *jmxb, double zb[BAS_Y][BAS_X], int *hr, 
    slg_basin_field(g_run.basin, "ZBM", g_run.buf.data());
    for (int j = 0; j < nj; j++)
       for (int i = 0; i < ni; i++) dumb5_.ZBM[j][i] = g_run.buf[(size_t) j * ni + i];
-   get("HB0");
+   // COMMON HB: the initial water as the step holds it (in the FP32 step,
+   // rounded to float), which the driver's first tide call adds to
+   get("HB");
    mirror_hb();
    // the BDRYHT squares (DEPSFC, depsfc.f:376-390) for the driver's tide
    // coupling, with no tide yet

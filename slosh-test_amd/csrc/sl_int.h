@@ -228,35 +228,35 @@ static inline int dev_sched(const double *prog, const long *off, int nstorm, dou
 
 extern "C" int sl_dev_intlht_f32(const double *prog, const long *off, const double *bm, const unsigned char *ocn,
                                  int imxb, int jmxb, int imxb1, int jmxb1, int pitch, const SlRunDevT<float> *R,
-                                 double *hbd, hipStream_t st);
+                                 hipStream_t st);
 extern "C" int sl_dev_intlht_f64(const double *prog, const long *off, const double *bm, const unsigned char *ocn,
                                  int imxb, int jmxb, int imxb1, int jmxb1, int pitch, const SlRunDevT<double> *R,
-                                 double *hbd, hipStream_t st);
+                                 hipStream_t st);
 static inline int dev_intlht(const double *prog, const long *off, const double *bm, const unsigned char *ocn,
                              int imxb, int jmxb, int imxb1, int jmxb1, int pitch, const SlRunDevT<float> *R,
-                             double *hbd, hipStream_t st)
+                             hipStream_t st)
 {
-   return sl_dev_intlht_f32(prog, off, bm, ocn, imxb, jmxb, imxb1, jmxb1, pitch, R, hbd, st);
+   return sl_dev_intlht_f32(prog, off, bm, ocn, imxb, jmxb, imxb1, jmxb1, pitch, R, st);
 }
 static inline int dev_intlht(const double *prog, const long *off, const double *bm, const unsigned char *ocn,
                              int imxb, int jmxb, int imxb1, int jmxb1, int pitch, const SlRunDevT<double> *R,
-                             double *hbd, hipStream_t st)
+                             hipStream_t st)
 {
-   return sl_dev_intlht_f64(prog, off, bm, ocn, imxb, jmxb, imxb1, jmxb1, pitch, R, hbd, st);
+   return sl_dev_intlht_f64(prog, off, bm, ocn, imxb, jmxb, imxb1, jmxb1, pitch, R, st);
 }
 extern "C" int sl_dev_tide_f32(const SlBasinDevT<float> *B, const SlRunDevT<float> *R, const SlTideDev *T, float *hb,
-                               const double *hbd, int n, int first, hipStream_t st);
+                               int n, int first, hipStream_t st);
 extern "C" int sl_dev_tide_f64(const SlBasinDevT<double> *B, const SlRunDevT<double> *R, const SlTideDev *T,
-                               double *hb, const double *hbd, int n, int first, hipStream_t st);
-static inline int dev_tide(const SlBasinDevT<float> *B, const SlRunDevT<float> *R, const SlTideDev *T, float *hb,
-                           const double *hbd, int n, int first, hipStream_t st)
+                               double *hb, int n, int first, hipStream_t st);
+static inline int dev_tide(const SlBasinDevT<float> *B, const SlRunDevT<float> *R, const SlTideDev *T, float *hb, int n,
+                           int first, hipStream_t st)
 {
-   return sl_dev_tide_f32(B, R, T, hb, hbd, n, first, st);
+   return sl_dev_tide_f32(B, R, T, hb, n, first, st);
 }
 static inline int dev_tide(const SlBasinDevT<double> *B, const SlRunDevT<double> *R, const SlTideDev *T, double *hb,
-                           const double *hbd, int n, int first, hipStream_t st)
+                           int n, int first, hipStream_t st)
 {
-   return sl_dev_tide_f64(B, R, T, hb, hbd, n, first, st);
+   return sl_dev_tide_f64(B, R, T, hb, n, first, st);
 }
 
 extern "C" int sl_dev_slot_init_f32(const SlBasinDevT<float> *B, const SlRunDevT<float> *R, int slot, int tstart,

@@ -2426,41 +2426,54 @@ __device__ real static_height(const SlBasinDev &B, const SlRunDev &R, int s, int
 
 // CalcTideGrid2's change to HB and TIDESH (slosh2.c:859-1025) for every run
 // still running at step n; first: the call before the first step, on the
-// initial water (hbd, when given, holds that water in double).  hb is the
-// buffer that holds each run's HB.  HB goes through double as the driver's
-// COMMON copy does, and is rounded to real where the device keeps it.  The
+// initial water.  hb is the buffer that holds each run's HB.  HB goes through
+// double as the driver's COMMON copy does, and is rounded to real where the
+// device keeps it.  The
 // old tide of a square is the group's field of step n-1: the driver's stored
 // value and its recomputation from the previous call's angles are that sum.
-__global__ void __launch_bounds__(256) k_tide_apply(SlBasinDev B, SlRunDev R, SlTideDev T, real *hb,
-                                                    const double *hbd, int n, int first)
+// One thread takes one square (or BDRYHT point) of SL_TSPT runs: the square's
+// ZB and flags are read once, and the runs' HB loads are in flight together.
+#define SL_TSPT 4
+__global__ void __launch_bounds__(256) k_tide_apply(SlBasinDev B, SlRunDev R, SlTideDev T, real *hb, int n,
+                                                    int first)
 {
-   const int p = blockIdx.x * 256 + threadIdx.x, s = blockIdx.y;
-   if (p >= T.ncell + T.nbcpt || (!first && R.nsteps[s] < n)) return;
-   const int g = T.sgrp[s];
-   const double *fn = T.field + ((size_t) g * 2 + (n & 1)) * T.ncell;
+   const int p = blockIdx.x * 256 + threadIdx.x, s0 = blockIdx.y * SL_TSPT;
+   if (p >= T.ncell + T.nbcpt) return;
+   const int ns = min(SL_TSPT, R.nstorm - s0);
+   const size_t now = (size_t) (n & 1) * T.ncell, old = (size_t) ((n - 1) & 1) * T.ncell;
    if (p >= T.ncell) {   // the boundary squares: TIDESH = the tide at step n
-      const int b = p - T.ncell;
-      R.tidesh[(size_t) s * T.nbcpt + b] = (real) fn[T.bccell[b]];
-      return;
-   }
-   const int i = p % T.ni, j = p / T.ni;
-   const size_t o = (size_t) s * R.fstride + (size_t) j * B.pitch + i;
-   const double zb = T.zb[p];
-   if (first) {   // every wet square (slosh2.c:859-896)
-      const double h = hbd ? hbd[o] : (double) hb[o];
-      if (h + zb != 0.) {
-         const double x = h + fn[p];
-         hb[o] = (x + zb >= 0.) ? (real) x : (real) (-zb);
+      const int b = p - T.ncell, c = T.bccell[b];
+      for (int t = 0; t < ns; t++) {
+         const int s = s0 + t;
+         if (first || R.nsteps[s] >= n)
+            R.tidesh[(size_t) s * T.nbcpt + b] = (real) T.field[(size_t) T.sgrp[s] * 2 * T.ncell + now + c];
       }
       return;
    }
-   const uint8_t f = T.cfl[p];
-   if (f & 2) {   // V2.1 / V2.2 deep squares (slosh2.c:904-931)
-      hb[o] = (real) ((double) static_height(B, R, s, n, i + 1, j + 1) + fn[p]);
-   } else if (f & 1) {   // usable wet squares: subtract the old tide, add the new
-      const double h = (double) hb[o];
-      if (h + zb != 0.) {
-         const double x = h - T.field[((size_t) g * 2 + ((n - 1) & 1)) * T.ncell + p] + fn[p];
+   const uint8_t f = first ? 1 : T.cfl[p];   // the first call: every wet square (slosh2.c:859-896)
+   if (!(f & 3)) return;                      // not usable: HB keeps its value
+   const int i = p % T.ni, j = p / T.ni;
+   const size_t q = (size_t) j * B.pitch + i;
+   const double zb = T.zb[p];
+   double h[SL_TSPT];
+   bool on[SL_TSPT];
+#pragma unroll
+   for (int t = 0; t < SL_TSPT; t++) {
+      const int s = s0 + t;
+      on[t] = t < ns && (first || R.nsteps[s] >= n);
+      h[t] = 0.;
+      if (on[t] && !(f & 2)) h[t] = (double) hb[(size_t) s * R.fstride + q];
+   }
+#pragma unroll
+   for (int t = 0; t < SL_TSPT; t++) {
+      if (!on[t]) continue;
+      const int s = s0 + t;
+      const size_t o = (size_t) s * R.fstride + q;
+      const double *fg = T.field + (size_t) T.sgrp[s] * 2 * T.ncell + p;
+      if (f & 2) {   // V2.1 / V2.2 deep squares (slosh2.c:904-931)
+         hb[o] = (real) ((double) static_height(B, R, s, n, i + 1, j + 1) + fg[now]);
+      } else if (h[t] + zb != 0.) {   // wet: subtract the old tide (none before the first call), add the new
+         const double x = first ? h[t] + fg[now] : h[t] - fg[old] + fg[now];
          hb[o] = (x + zb >= 0.) ? (real) x : (real) (-zb);
       }
    }
@@ -2511,12 +2524,11 @@ extern "C" int SL_SLOTINIT(const SlBasinDev *B, const SlRunDev *R, int slot, int
 // least -ZB; the others a datum (the lake's, or the sea's on MSY's open
 // squares), at least -ZB; listed squares dry out (-ZB).  Row JMXB repeats row
 // 1.  In double (the host's operations in its order), rounded to R into HB
-// (both buffers) and HMX (squares only), and kept in double in hbd when given
-// (the tide's first call adds to that value).  bm: ELPCL[imxb], ELPDL[imxb],
+// (both buffers) and HMX (squares only).  bm: ELPCL[imxb], ELPDL[imxb],
 // COSL[jmxb], SINL[jmxb], then ZB[jmxb][imxb] (packed, I fastest).
 __global__ void __launch_bounds__(256) k_intlht(const double *prog, const long *off, const double *bm,
                                                 const uint8_t *ocn, int imxb, int jmxb, int imxb1, int jmxb1,
-                                                int pitch, SlRunDev R, double *hbd)
+                                                int pitch, SlRunDev R)
 {
    const int s = blockIdx.y, p = blockIdx.x * 256 + threadIdx.x;
    if (p >= imxb * jmxb) return;
@@ -2551,7 +2563,6 @@ __global__ void __launch_bounds__(256) k_intlht(const double *prog, const long *
    R.hb0[o] = v;
    R.hb1[o] = v;
    R.hmx[o] = (i == imxb || j == jmxb) ? real(0.) : v;
-   if (hbd) hbd[o] = hb;
 }
 
 #ifdef SL_F64
@@ -2560,13 +2571,13 @@ __global__ void __launch_bounds__(256) k_intlht(const double *prog, const long *
 #define SL_INTLHT sl_dev_intlht_f32
 #endif
 extern "C" int SL_INTLHT(const double *prog, const long *off, const double *bm, const unsigned char *ocn, int imxb,
-                         int jmxb, int imxb1, int jmxb1, int pitch, const SlRunDev *R, double *hbd, hipStream_t st)
+                         int jmxb, int imxb1, int jmxb1, int pitch, const SlRunDev *R, hipStream_t st)
 {
    if (R->nstorm < 1 || R->nstorm > 65535 || imxb < 2 || jmxb < 2 || pitch < imxb ||
        R->fstride < (size_t) pitch * jmxb)
       return -1;
    hipLaunchKernelGGL(k_intlht, dim3((imxb * jmxb + 255) / 256, R->nstorm), dim3(256), 0, st, prog, off, bm, ocn,
-                      imxb, jmxb, imxb1, jmxb1, pitch, *R, hbd);
+                      imxb, jmxb, imxb1, jmxb1, pitch, *R);
    hipError_t e = hipGetLastError();
    return e == hipSuccess ? 0 : -(int) e;
 }
@@ -2657,15 +2668,15 @@ extern "C" int SL_MAXSTAT(const SlBasinDev *B, const SlRunDev *R, const real *hb
 #define SL_TIDE sl_dev_tide_f32
 #endif
 // the tide after step n (first: before the first step) of every run of R
-extern "C" int SL_TIDE(const SlBasinDev *B, const SlRunDev *R, const SlTideDev *T, real *hb, const double *hbd, int n,
-                       int first, hipStream_t st)
+extern "C" int SL_TIDE(const SlBasinDev *B, const SlRunDev *R, const SlTideDev *T, real *hb, int n, int first,
+                       hipStream_t st)
 {
    if (T->ngrp < 1 || T->ngrp > 65535 || R->nstorm < 1 || R->nstorm > 65535 || T->ncell < 1 || !R->tidesh ||
        T->ni < 1 || T->ncell % T->ni || T->ni > B->imxb - 1 || T->ncell / T->ni > B->jmxb - 1)
       return -1;
    hipLaunchKernelGGL(k_tide_field, dim3((T->ncell + 255) / 256, T->ngrp), dim3(256), 0, st, *T, n);
-   hipLaunchKernelGGL(k_tide_apply, dim3((T->ncell + T->nbcpt + 255) / 256, R->nstorm), dim3(256), 0, st, *B, *R,
-                      *T, hb, hbd, n, first);
+   hipLaunchKernelGGL(k_tide_apply, dim3((T->ncell + T->nbcpt + 255) / 256, (R->nstorm + SL_TSPT - 1) / SL_TSPT),
+                      dim3(256), 0, st, *B, *R, *T, hb, n, first);
    hipError_t e = hipGetLastError();
    return e == hipSuccess ? 0 : -(int) e;
 }
