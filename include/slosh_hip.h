@@ -240,9 +240,14 @@ int slg_batch_envelopes(slg_batch *x, short *env, short *mom, char *err, int err
  * envelope and the MOM do not fit it (basins above 8 million points).  Calls
  * that overlap on one GPU share the scratch and race. */
 int slg_batch_envelopes_noalloc(slg_batch *x, short *env, short *mom, char *err, int errlen);
-/* Datums a track carries for the env trailer (ReadTrkFile, slosh2.c:1885-1975),
+/* Datums a track carries for the env trailer (ReadTrkFile, slosh2.c:1885-1975;
+ * both track formats),
  * after the surge-only InitWater_TideModeOverride (slosh2.c:408-482, 2238). */
 int slg_track_datums(const char *trk_path, float *ht1, float *ht2);
+/* The env comment of a run (ReadTrkFile, slosh2.c:1884-1975): a FileVersion=2013
+ * file's ExtraInfo= value or "FileVersion=2013"; a 1992-format file's two title
+ * lines, each cut or blank-padded to 80 columns.  out: at least 161 bytes. */
+int slg_track_env_comment(const char *trk_path, char *out, int cap);
 
 /* ---- rex v2 files of batched runs ---------------------------------------
  * The frames' bit streams are encoded on the device: the basin loop of

@@ -2381,6 +2381,27 @@ static int read_datums(const char *trk_path, float *ht1, float *ht2)
    char line[512];
    float oc = 0.f, lk = 0.f, cn = -999.f;
    if (!fp) return SLG_EINPUT;
+   if (fgets(line, sizeof line, fp) && strncmp(line, "FileVersion=", 12)) {
+      // the 1992 format: the datum line after the titles, 100 rows, IBGNT
+      // ITEND JHR and the landfall time (ReadTrkFile, slosh2.c:1940-1990)
+      int k = 1;
+      while (k < 105 && fgets(line, sizeof line, fp)) k++;
+      fclose(fp);
+      if (k < 105) return SLG_EINPUT;
+      if (line[10] == 'x' || line[10] == 'X') {
+         line[16] = 0;
+         *ht2 = (float) atof(line + 11);
+         line[10] = 0;
+         *ht1 = (float) atof(line + 5);
+      } else {
+         line[10] = 0;
+         *ht2 = (float) atof(line + 6);
+         line[5] = 0;
+         *ht1 = (float) atof(line);
+      }
+      return 0;
+   }
+   rewind(fp);
    while (fgets(line, sizeof line, fp)) {
       if (!strncmp(line, "Hour,", 5)) break;
       if (line[0] == '#') continue;
@@ -2392,6 +2413,36 @@ static int read_datums(const char *trk_path, float *ht1, float *ht2)
    const bool canal = cn > -100;   // Okeechobee-style datums: lake and canal
    *ht1 = canal ? lk : oc;
    *ht2 = canal ? cn : lk;
+   return 0;
+}
+
+// The comment EnvSave writes for a run (ReadTrkFile, slosh2.c:1884-1975):
+// FileVersion=2013 files give their ExtraInfo= value (or "FileVersion=2013"),
+// 1992 files their two title lines, each cut or blank-padded to 80 columns.
+extern "C" int slg_track_env_comment(const char *trk_path, char *out, int cap)
+{
+   FILE *fp = fopen(trk_path, "r");
+   char h1[512], h2[512], b[512];
+   if (!fp || cap < 161) {
+      if (fp) fclose(fp);
+      return SLG_EINPUT;
+   }
+   memset(out, 0, (size_t) cap);
+   if (!fgets(h1, 200, fp)) h1[0] = 0;
+   if (!strncmp(h1, "FileVersion=", 12)) {
+      snprintf(out, (size_t) cap, "%s", "FileVersion=2013");
+      while (fgets(b, 200, fp)) {
+         if (!strncmp(b, "Hour,", 5)) break;
+         if (b[0] == '#') continue;
+         if (!strncmp(b, "ExtraInfo=", 10)) snprintf(out, (size_t) cap, "%.160s", b + 10);
+      }
+   } else {
+      if (!fgets(h2, 200, fp)) h2[0] = 0;
+      h1[strcspn(h1, "\n")] = 0;
+      h2[strcspn(h2, "\n")] = 0;
+      snprintf(out, (size_t) cap, "%-80.80s%-80.80s", h1, h2);
+   }
+   fclose(fp);
    return 0;
 }
 

@@ -122,6 +122,7 @@ void sl_basin_free(sl_basin *b);
 typedef struct sl_trk {
    int date_yr, date_mo, date_dy, date_hr, date_mn;
    int dhr, land, beg, end, adj, spline, n;
+   int v1992;          /* the original (1992) track format: 100 hourly rows, no factors */
    double ocean, lake, windf, rmwf;
    int hrs[SL_NT + 2];
    double fla[SL_NT + 2], flo[SL_NT + 2], fdp[SL_NT + 2], frm[SL_NT + 2];
@@ -130,6 +131,8 @@ void sl_trk_init(sl_trk *t);
 /* Add a row (longitude east positive, as a trk file holds it); 0, 1 when the
  * track is full (the row is not read), -1 with a message when out of bounds. */
 int sl_trk_row(sl_trk *t, double hour, double lat, double lon, double dp, double rmw, char err[256]);
+/* A trk file in either of INPUTN2's formats (FileVersion=2013, or the 1992
+ * format of 100 hourly rows). */
 int sl_trk_parse(const char *path, sl_trk *t, char err[256]);
 int sl_trk_build(const sl_trk *t, sl_storm *s, char err[256]);
 

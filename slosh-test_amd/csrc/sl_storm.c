@@ -112,7 +112,93 @@ void sl_trk_init(sl_trk *t)
    t->windf = t->rmwf = 1.0;
 }
 
-/* INPUTN2 2013 branch (inputn2.f:110-575): the trk file into t */
+/* A Fortran Fw.d input field: columns [c0, c0 + w) of line (short lines read
+ * as blanks), blanks ignored, an all-blank field 0, and d implied decimals
+ * when the digits carry no decimal point. */
+static double fort_f(const char *line, int c0, int w, int d)
+{
+   char b[64];
+   int k, n = 0, dot = 0;
+   const int len = (int) strlen(line);
+   for (k = c0; k < c0 + w && k < len && n < 62; k++) {
+      const char c = line[k];
+      if (c == ' ' || c == '\n' || c == '\r') continue;
+      if (c == '.' || c == 'e' || c == 'E' || c == 'd' || c == 'D') dot = 1;
+      b[n++] = (c == 'd' || c == 'D') ? 'e' : c;
+   }
+   b[n] = 0;
+   if (!n) return 0.;
+   return dot ? strtod(b, NULL) : strtod(b, NULL) / (d == 1 ? 10. : d == 2 ? 100. : pow(10., d));
+}
+
+/* a Fortran Iw field of columns [c0, c0 + w) */
+static int fort_i(const char *line, int c0, int w)
+{
+   char b[32];
+   int k, n = 0;
+   const int len = (int) strlen(line);
+   for (k = c0; k < c0 + w && k < len && n < 30; k++)
+      if (line[k] != ' ' && line[k] != '\n' && line[k] != '\r') b[n++] = line[k];
+   b[n] = 0;
+   return n ? atoi(b) : 0;
+}
+
+/* INPUTN2's 1992 branch (inputn2.f:610-627): two title lines, 100 hourly rows
+ * (15X,I5,8F8.2: hour, latitude, longitude west positive, speed, direction,
+ * pressure deficit, RMW), IBGNT ITEND JHR (3I3), the landfall time INITAL
+ * reads (inital.f:206-215: 2X,2I2.2,1X,I2.2,1X,A3,1X,I4) and the datums
+ * (2F5.1,A1,F5.1).  No interpolation and no wind / RMW factors. */
+static int trk_parse_1992(FILE *fp, sl_trk *t, char err[256])
+{
+   static const char *mon[12] = {"JAN", "FEB", "MAR", "APR", "MAY", "JUN",
+                                 "JUL", "AUG", "SEP", "OCT", "NOV", "DEC"};
+   char line[512];
+   int i, m;
+   for (i = 0; i < 2; i++)
+      if (!fgets(line, sizeof line, fp)) goto short_file;
+   t->v1992 = 1;
+   t->n = 100;
+   for (i = 1; i <= 100; i++) {
+      if (!fgets(line, sizeof line, fp)) goto short_file;
+      t->hrs[i] = i;
+      t->fla[i] = fort_f(line, 20, 8, 2);
+      t->flo[i] = fort_f(line, 28, 8, 2);
+      t->fdp[i] = fort_f(line, 52, 8, 2);
+      t->frm[i] = fort_f(line, 60, 8, 2);
+   }
+   if (!fgets(line, sizeof line, fp)) goto short_file;
+   t->beg = fort_i(line, 0, 3);
+   t->end = fort_i(line, 3, 3);
+   t->land = fort_i(line, 6, 3);
+   if (!fgets(line, sizeof line, fp)) goto short_file;
+   t->date_hr = fort_i(line, 2, 2);
+   t->date_mn = fort_i(line, 4, 2);
+   t->date_dy = fort_i(line, 7, 2);
+   t->date_yr = fort_i(line, 14, 4);
+   for (m = 0; m < 12; m++)
+      if ((int) strlen(line) >= 13 && !strncmp(line + 10, mon[m], 3)) break;
+   if (m == 12) {
+      snprintf(err, 256, "track landfall time: no month in '%.20s'", line);
+      return -1;
+   }
+   t->date_mo = m + 1;
+   if (!fgets(line, sizeof line, fp)) goto short_file;
+   t->ocean = fort_f(line, 0, 5, 1);
+   t->lake = fort_f(line, 5, 5, 1);
+   if ((int) strlen(line) > 10 && (line[10] == 'X' || line[10] == 'x')) {
+      snprintf(err, 256, "Okeechobee canal datums (XOKE) are not supported");
+      return -1;
+   }
+   t->dhr = 0;
+   t->adj = 0;
+   return 0;
+short_file:
+   snprintf(err, 256, "the track file ends early (1992 format: 2 titles, 100 rows, 3 lines)");
+   return -1;
+}
+
+/* INPUTN2 (inputn2.f:84-627): the trk file into t; the first line says the
+ * format (FileVersion=2013, else the 1992 format) */
 int sl_trk_parse(const char *path, sl_trk *t, char err[256])
 {
    FILE *fp = fopen(path, "r");
@@ -125,12 +211,26 @@ int sl_trk_parse(const char *path, sl_trk *t, char err[256])
       snprintf(err, 256, "cannot open track file %s", path);
       return -1;
    }
-   if (!fgets(line, sizeof line, fp) || !strstr(line, "FileVersion") || !strstr(line, "2013")) {
+   if (!fgets(line, sizeof line, fp)) {
       fclose(fp);
-      snprintf(err, 256, "only FileVersion=2013 track files are supported by this build");
+      snprintf(err, 256, "empty track file %s", path);
       return -1;
    }
-   rewind(fp);
+   {
+      const char *v = starts(line, "FileVersion=") ? line + 12 : starts(line, " FileVersion=") ? line + 13
+                    : starts(line, " FileVersion =") ? line + 14 : NULL;
+      rewind(fp);
+      if (!v) {
+         rc = trk_parse_1992(fp, t, err);
+         fclose(fp);
+         return rc;
+      }
+      if (atoi(v) != 2013) {
+         fclose(fp);
+         snprintf(err, 256, "track file version %d: 1992 or 2013", atoi(v));
+         return -1;
+      }
+   }
    while (fgets(line, sizeof line, fp)) {
       char *l = line;
       line[strcspn(line, "\r\n")] = 0;
@@ -214,7 +314,7 @@ int sl_trk_build(const sl_trk *t, sl_storm *s, char err[256])
    const double *fla = t->fla, *flo = t->flo, *fdp = t->fdp, *frm = t->frm;
 
    if (t->beg < 0 || t->beg > 999 || t->end < 0 || t->end > 999 || t->dhr < 0 || t->land < 0 ||
-       t->land > 999 || t->ocean < -50. || t->ocean > 50. || t->lake < -50. || t->lake > 50. ||
+       t->land > 999 || (!t->v1992 && (t->ocean < -50. || t->ocean > 50. || t->lake < -50. || t->lake > 50.)) ||
        t->beg >= t->end || ntrk < 2) {
       snprintf(err, 256, "track header incomplete or out of bounds");
       return -1;

@@ -58,6 +58,7 @@ def lib():
         L.slg_batch_autotune.argtypes = [vp, i, ctypes.POINTER(i), ctypes.POINTER(i), vp, cp, i]
         L.slg_batch_envelopes.argtypes = [vp, vp, vp, cp, i]
         L.slg_track_datums.argtypes = [cp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]
+        L.slg_track_env_comment.argtypes = [cp, cp, i]
         L.slg_track_datums_tide.argtypes = [cp, i, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]
         L.slg_batch_set_tide.argtypes = [vp, cp, cp, cp, i, i, cp, i]
         L.slg_batch_rex_frames.argtypes = [vp, cp, i, i, vp, ctypes.c_long, vp, cp, i]
@@ -488,6 +489,14 @@ def track_datums(trk: str, tide_mode: int = 0):
     if rc:
         raise SloshError("cannot read %s" % trk)
     return a.value, b.value
+
+
+def track_env_comment(trk: str) -> str:
+    """The comment the reference's EnvSave writes for a run of this trk file."""
+    b = ctypes.create_string_buffer(256)
+    if lib().slg_track_env_comment(trk.encode(), b, 256):
+        raise SloshError("cannot read %s" % trk)
+    return b.value.decode("latin-1")
 
 
 def parse_tide(flag: str):

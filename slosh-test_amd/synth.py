@@ -510,6 +510,29 @@ def trk_text(t: Track) -> str:
     return "\n".join(L) + "\n"
 
 
+def trk1992_text(t: Track, beg: int, end: int, land: int, title="SYNTHETIC STORM (SEEDED)") -> str:
+    """The track in INPUTN2's original (1992) format (inputn2.f:610-627), laid out
+    as the reference's stm2trk writes it (sorc/stm2trk.cd/stm2trk.c:447-502):
+    two title lines, 100 hourly rows (hour I = 1..100: latitude, longitude west
+    positive, forward speed, direction, deficit, RMW), IBGNT ITEND JHR, the
+    landfall time (t.date is taken as the time of hour JHR) and the datums.
+    t must hold hours 1..100."""
+    at = {int(h): k for k, h in enumerate(t.hours)}
+    L = ["HUR   %-60s" % title, " DELTA-P = %.0fMB ; RMW= %.0f ST. MI." % (t.dp[0], t.rmw[0])]
+    for i in range(1, 101):
+        k = at[i]
+        row = "%17s%3d%8.4f%8.3f%8.2f%8.2f%8.2f%8.2f%5d" % ("NAP-----  " if i == land else " ", i, t.lat[k],
+                                                         -t.lon[k], 0.0, 0.0, t.dp[k], t.rmw[k], i)
+        L.append(row + (" ---NAP" if i == land else ""))
+    L.append("%3d%3d%3d%16sIBGNT ITEND JHR" % (beg, end, land, " "))
+    y, mo, d, h, mi = t.date
+    L.append("HR%02d%02d %02d %3s %04d%7sNEAREST APPROACH, OR LANDFALL, TIME"
+             % (h, mi, d, ["JAN", "FEB", "MAR", "APR", "MAY", "JUN", "JUL", "AUG", "SEP", "OCT", "NOV",
+                           "DEC"][mo - 1], y, " "))
+    L.append("%5.1f%5.1f%15sSEA AND LAKE DATUM" % (t.ocean, t.lake, " "))
+    return "\n".join(L) + "\n"
+
+
 def track_dict(t: Track) -> dict:
     """The track as slg_track fields (sloshgpu.track_struct): the values a trk
     file of trk_text(t) holds, rounded as that file writes them."""

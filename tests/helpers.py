@@ -349,6 +349,30 @@ def tide_basin(kind, hours=6):
     return b, t
 
 
+# Tracks in INPUTN2's original (1992) format: (grid, seed, IBGNT, ITEND, JHR,
+# ocean datum, lake datum)
+TRK1992_CASES = [(" ", 61, 40, 52, 46, 1.2, 0.8), ("+", 62, 30, 44, 38, 0.0, 0.0), ("$", 63, 44, 54, 48, 99.9, 2.0)]
+
+
+def trk1992_id(idx):
+    g, seed, beg, end, land, oc, lk = TRK1992_CASES[idx]
+    return "trk1992-%s-s%d" % ({" ": "polar", "$": "ellip", "+": "hyper"}[g], seed)
+
+
+def trk1992_case(root, idx):
+    """A case whose trk file is in the 1992 format (the reference's default)."""
+    g, seed, beg, end, land, oc, lk = TRK1992_CASES[idx]
+    b, _ = synth.tiny(seed=seed, gtype=g, ni=30, nj=26)
+    t = synth.straight_track(b, 11.0, 320.0, 65.0, 18.0, hours=99, landfall=land)
+    t.ocean, t.lake = oc, lk
+    t.date = (2024, 9, 12, 6, 30)
+    case = make_case(root, b, t)
+    with open(case["trk"], "w") as fh:
+        fh.write(synth.trk1992_text(t, beg, end, land))
+    case["b"] = b
+    return case
+
+
 def full_depth_case(root, hours=6):
     """A basin with squares deeper than 600 ft (cells of 3-8 km at dt = 20 s keep
     them within the gravity-wave limit), for -fullDepth (crdrd2.f:441-444)."""

@@ -11,6 +11,7 @@ import numpy as np
 import pytest
 
 import helpers
+import sloshgpu
 from helpers import synth
 from test_dropin_cli import CASES
 
@@ -223,3 +224,29 @@ def test_gpu_cli_full_depth_matches_reference_cli(tmp_path):
     env_g, rex_g = helpers.run_cli(helpers.GPU_CLI, root, bsn, "gpu64", extra=["-fullDepth", "1"],
                                    env_extra={"SLG_PRECISION": "64"})
     assert helpers.same_cli_outputs(env_g, rex_g, env_r, rex_r) == []
+
+
+@pytest.mark.parametrize("idx", range(len(helpers.TRK1992_CASES)), ids=helpers.trk1992_id)
+def test_gpu_cli_and_batch_trk1992(idx, tmp_path):
+    """Tracks in the original 1992 format (the reference's default): the drop-in
+    CLI's env and rex files equal the reference CLI's byte for byte (FP64), and
+    a batched run's env file (its comment the two title lines, its datums from
+    the datum line, ReadTrkFile slosh2.c:1940-1990) equals the same env."""
+    root = str(tmp_path)
+    case = helpers.trk1992_case(root, idx)
+    print("trk1992", helpers.TRK1992_CASES[idx], flush=True)
+    bsn = case["basin"]
+    env_r, rex_r = helpers.run_cli(helpers.REF_CLI, root, bsn, "ref")
+    env_g, rex_g = helpers.run_cli(helpers.GPU_CLI, root, bsn, "gpu64", env_extra={"SLG_PRECISION": "64"})
+    assert helpers.same_cli_outputs(env_g, rex_g, env_r, rex_r) == []
+    B = sloshgpu.Basin(case["dta"], case["bntline"])
+    X = sloshgpu.Batch(B, [case["trk"]], precision=64)
+    X.upload()
+    X.step(X.nsteps[0])
+    env, _ = X.envelopes()
+    X.close()
+    B.close()
+    ht1, ht2 = sloshgpu.track_datums(case["trk"])
+    mine = os.path.join(root, "batch.env")
+    sloshgpu.write_env(mine, env[0], ht1, ht2, comment=sloshgpu.track_env_comment(case["trk"]))
+    assert open(mine, "rb").read() == open(env_r, "rb").read()

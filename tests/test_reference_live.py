@@ -219,3 +219,26 @@ def test_dollar_variants_change_the_start(tmp_path):
             assert dried and all(hb[j - 1, i - 1] == -zb[j - 1, i - 1] for i, j in dried)
         else:
             assert np.any(hb[:nj - 1, :ni - 1] == 1.5 - 1.0) and np.any(hb[:nj - 1, :ni - 1] == 0.7)
+
+
+@pytest.mark.parametrize("idx", range(len(helpers.TRK1992_CASES)), ids=helpers.trk1992_id)
+def test_trk1992_host_matches_reference(idx, tmp_path):
+    """Tracks in the original 1992 format (INPUTN2's default branch,
+    inputn2.f:610-627; the landfall time INITAL parses, inital.f:206-215): the
+    product's host initialisation and storm rows equal the reference build's,
+    and the FP64 oracle from them equals it at every checkpoint."""
+    root = str(tmp_path)
+    case = helpers.trk1992_case(root, idx)
+    ref = helpers.run_reference(case, root + "/ref", 300, ",".join(map(str, CHECKS)))
+    rows, tabs, _ = helpers.product_export(case, root)
+    ri = helpers.read_rec(ref + "/init.bin")
+    mi = helpers.read_rec(root + "/init.bin")
+    for k in mi:
+        assert np.array_equal(ri[k], mi[k]), "init record %s differs" % k
+    rrows = np.fromfile(ref + "/storm.bin").reshape(-1, 90)
+    assert np.array_equal(rows[:len(rrows)], rrows)
+    snaps = helpers.run_oracle("64", root + "/init.bin", rows, tabs, 300, set(CHECKS))
+    for c in CHECKS:
+        st = helpers.read_rec(ref + "/s%06d.bin" % c)
+        for f in helpers.FIELDS:
+            assert np.array_equal(snaps[c][f].ravel(), st[f]), "step %d %s differs" % (c, f)
