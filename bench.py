@@ -678,12 +678,12 @@ def main():
             td = synth.write_tide_db(os.path.join(root, "tide"), b, "cfg3", seed=9, adj=True)
             tide = (os.path.join(td, "ft03.dta"), os.path.join(td, "cfg3.bhc"), os.path.join(td, "cfg3.adj"), 21, -5)
             log("batched with the gridded tide, FP%d" % prec)
-            rt = batched_run(paths["dta"], paths["bntline"], trks, prec, a.steps, a.warmup, 1, False,
+            rt = batched_run(paths["dta"], paths["bntline"], trks, prec, a.steps, a.warmup, a.streams, False,
                              path, a.tile_rows, a.span, tide=tide)
             vt = a.storms * rt["cells"] * a.steps / rt["dt"]
             out["tide"] = {"workload": "the batched ensemble with -f_tide V2.1.-5 (37 constituents)",
                            "value": vt, "ms_per_step": rt["dt"] * 1e3 / a.steps, "set_tide_s": rt["tide_s"],
-                           "tune": rt["tune"], "windows": rt["windows"]}
+                           "tune": rt["tune"], "windows": rt["windows"], "streams": rt["streams"]}
             if prec in rate:
                 out["tide"]["vs_without_tide"] = vt / rate[prec]
             log("tide: %.3e storm.cell.steps/s, set_tide %.2f s" % (vt, rt["tide_s"]))

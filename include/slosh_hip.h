@@ -383,9 +383,9 @@ int slg_batch_static_height(const slg_batch *x, int storm, int i, int j, double 
  * (ParseTide, usrparse.c:14-52; squares with ZB <= thresh take no tide in
  * modes 21, 22, 24).  Each run's model clock starts at its start date and
  * moves by its DELT each step; the tables are its year's.  Call once, before
- * slg_batch_upload.  hipGraphs and storm parts on several streams are not
- * used while the tide is on (the steps launch one by one on the batch's
- * stream).  SLG_EUNSUP for the hsf basins' 6-minute tide cadence
+ * slg_batch_upload.  hipGraphs are not used while the tide is on (the steps
+ * launch one by one); storm parts on several streams are (each part sums its
+ * own tide fields).  SLG_EUNSUP for the hsf basins' 6-minute tide cadence
  * (slosh2.c:1565-1577).  The tide spin-up (-spinUp) is not provided.
  * datums_tide: slg_track_datums for such a run (InitWater_TideModeOverride
  * with f_tide = mode). */

@@ -1091,6 +1091,13 @@ static int batch_upload(slg_batch *X, char *err, int errlen)
       const int krc = dev_tide(&I.dev, &D, &X->tdev, D.hb0, 0, 1, X->st);
       HIPOK(krc < 0 ? (hipError_t) -krc : (krc > 0 ? hipErrorInvalidValue : hipSuccess));
       HIPOK(hipStreamSynchronize(X->st));
+      // every storm part's fields start from the same step-0 tide (slot 0)
+      const SlTideDev &T = X->tdev;
+      const size_t fg = (size_t) T.ngrp * 2 * T.ncell;
+      for (int p = 1; p < SL_MAXSPLIT; p++)
+         for (int g = 0; g < T.ngrp; g++)
+            HIPOK(hipMemcpy(T.field + p * fg + (size_t) g * 2 * T.ncell, T.field + (size_t) g * 2 * T.ncell,
+                            (size_t) T.ncell * 8, hipMemcpyDeviceToDevice));
    }
    return 0;
 }
@@ -1152,6 +1159,13 @@ static int launch_split(slg_batch *X, const SlBasinDevT<R> &B, const SlRunDevT<R
       if (p && first && hipStreamWaitEvent(st, prev[2], 0) != hipSuccess) return -(int) hipErrorUnknown;
       r = dev_launch(&B, &V, smpt, filt, dyn, X->parity, X->path, X->tile_rows, st, evp);
       if (r < 0) return r;
+      if (X->tide_mode) {   // each part sums its own fields: the parts run staggered
+         SlTideDev Tp = X->tdev;
+         Tp.sgrp += s0;
+         Tp.field += (size_t) p * Tp.ngrp * 2 * Tp.ncell;
+         const int t = dev_tide(&B, &V, &Tp, r ? V.hb1 : V.hb0, X->step + 1, 0, st);
+         if (t < 0) return t;
+      }
       if (X->hbmax || X->hbmax2) {
          const size_t o = (size_t) s0 * D.fstride;
          const int m = dev_maxstat(&B, &V, r ? V.hb1 : V.hb0, img(X->b, R()).zbd, X->hbmax ? X->hbmax + o : nullptr,
@@ -1181,7 +1195,7 @@ static int launch_whole(slg_batch *X, const SlBasinDevT<R> &B, const SlRunDevT<R
 
 static int launch_one(slg_batch *X, int smpt, int filt, hipEvent_t *ev, int dyn = 0, bool first = false)
 {
-   if (X->nsplit > 1 && X->path == SLG_PATH_FUSED && !X->tide_mode) {
+   if (X->nsplit > 1 && X->path == SLG_PATH_FUSED) {
       if (X->prec == 64) return launch_split(X, X->b->i64.dev, X->d64, smpt, filt, ev, dyn, first);
       return launch_split(X, X->b->i32.dev, X->d32, smpt, filt, ev, dyn, first);
    }
@@ -1215,6 +1229,15 @@ extern "C" int slg_batch_set_streams(slg_batch *X, int nstreams)
          if (!e && hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return SLG_EHIP;
    // steps already queued on the batch stream come first
    if (join_streams(X)) return SLG_EHIP;
+   if (X->tide_mode) {
+      // the tide fields of every part as part 0 holds them (all parts are at
+      // the same step; a part that did not step since holds older ones)
+      const SlTideDev &T = X->tdev;
+      const size_t fg = (size_t) T.ngrp * 2 * T.ncell;
+      if (hipStreamSynchronize(X->st) != hipSuccess) return SLG_EHIP;
+      for (int p = 1; p < SL_MAXSPLIT; p++)
+         if (hipMemcpy(T.field + p * fg, T.field, fg * 8, hipMemcpyDeviceToDevice) != hipSuccess) return SLG_EHIP;
+   }
    X->nsplit = nstreams;
    return 0;
 }
@@ -1719,7 +1742,7 @@ extern "C" int slg_batch_set_tide(slg_batch *X, const char *ft03, const char *bh
    add(ncs * 8); add((size_t) G * 8);                        // casa goff
    add((size_t) G * 4); add((size_t) G * 4); add((size_t) S * 4);   // gyear gmax sgrp
    const size_t foff = tot;
-   tot += (size_t) G * 2 * nc * 8;                          // field
+   tot += (size_t) SL_MAXSPLIT * G * 2 * nc * 8;           // field, one per storm part
    char *d = nullptr;
    std::vector<double> amp(tab), cv(tab), sv(tab);
    hipError_t he = hipMalloc((void **) &d, tot);

@@ -33,13 +33,16 @@ def _tide_files(case):
     return os.path.join(d, "ft03.dta"), os.path.join(d, case["basin"] + ".bhc"), adj if os.path.exists(adj) else None
 
 
-def _batched_envs(case, trks, flag, precision, root, tag, sched_host=False):
+def _batched_envs(case, trks, flag, precision, root, tag, sched_host=False, streams=1):
     mode, thresh = sloshgpu.parse_tide(flag)
     ft03, bhc, adj = _tide_files(case)
     B = sloshgpu.Basin(case["dta"], case["bntline"])
     X = sloshgpu.Batch(B, trks, precision=precision, sched_host=sched_host)
     X.set_tide(ft03, bhc, adj, mode, thresh)
     X.upload()
+    if streams > 1:   # some steps on one stream, the rest in parts on several
+        X.step(100)
+        X.set_streams(streams)
     X.step(max(X.nsteps))
     env, _ = X.envelopes()
     X.close()
@@ -83,10 +86,10 @@ def test_batch_tide_env_equals_reference_cli(idx, tmp_path):
 _RUNS = [((2025, 9, 1, 0, 0), 6), ((2025, 9, 1, 0, 0), 6), ((2024, 2, 29, 13, 30), 8), ((2031, 12, 31, 22, 45), 10)]
 
 
-@pytest.mark.parametrize("precision", [64, 32])
-def test_batch_tide_runs_of_several_clocks(precision, tmp_path):
+@pytest.mark.parametrize("precision,streams", [(64, 1), (32, 1), (64, 3)])
+def test_batch_tide_runs_of_several_clocks(precision, streams, tmp_path):
     root = str(tmp_path)
-    print("batch tide, several clocks, FP%d" % precision, flush=True)
+    print("batch tide, several clocks, FP%d, %d streams" % (precision, streams), flush=True)
     b, _ = helpers.tide_basin("polar")
     rng = np.random.default_rng(7)
     cases = []
@@ -99,7 +102,8 @@ def test_batch_tide_runs_of_several_clocks(precision, tmp_path):
         case = helpers.make_case(r, b, t)
         case["tidedir"] = synth.write_tide_db(r, b, case["basin"], seed=50, adj=True)
         cases.append(case)
-    envs = _batched_envs(cases[0], [c["trk"] for c in cases], "V2", precision, root, "multi%d" % precision)
+    envs = _batched_envs(cases[0], [c["trk"] for c in cases], "V2", precision, root, "multi%d" % precision,
+                         streams=streams)
     cli = helpers.REF_CLI if precision == 64 else helpers.ORC_CLI["32"]
     for k, case in enumerate(cases):
         r = os.path.join(root, "run%d" % k)
