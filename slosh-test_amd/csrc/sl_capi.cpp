@@ -1725,12 +1725,17 @@ extern "C" int slg_batch_set_tide(slg_batch *X, const char *ft03, const char *bh
       cfl[c] = ((mode == 21 || mode == 22) && z >= 290) ? 2 : use ? 1 : 0;
       zb[c] = z;
    }
+   std::vector<int> lst;   // the squares a step changes
+   for (int c = 0; c < nc; c++)
+      if (cfl[c]) lst.push_back(c);
+   const int nl = (int) lst.size();
+   if (lst.empty()) lst.push_back(0);
    const int nb = b->nbcpt;
    std::vector<int> bcc((size_t) std::max(nb, 1), 0);
    for (int n = 0; n < nb; n++) bcc[n] = (b->jsh[n] - 1) * ni + (b->ish[n] - 1);
    // one allocation: year tables, per-square values, groups, fields
    const size_t tab = (size_t) SL_NUMT * nc;
-   size_t off[12], tot = 0;
+   size_t off[13], tot = 0;
    int n = 0;
    auto add = [&](size_t bytes) {
       off[n++] = tot;
@@ -1741,6 +1746,7 @@ extern "C" int slg_batch_set_tide(slg_batch *X, const char *ft03, const char *bh
    add(bcc.size() * 4);                                     // bccell
    add(ncs * 8); add((size_t) G * 8);                        // casa goff
    add((size_t) G * 4); add((size_t) G * 4); add((size_t) S * 4);   // gyear gmax sgrp
+   add(lst.size() * 4);                                     // list
    const size_t foff = tot;
    tot += (size_t) SL_MAXSPLIT * G * 2 * nc * 8;           // field, one per storm part
    char *d = nullptr;
@@ -1757,11 +1763,11 @@ extern "C" int slg_batch_set_tide(slg_batch *X, const char *ft03, const char *bh
       if (he == hipSuccess) he = hipMemcpy(d + off[1] + y * tab * 8, cv.data(), tab * 8, hipMemcpyHostToDevice);
       if (he == hipSuccess) he = hipMemcpy(d + off[2] + y * tab * 8, sv.data(), tab * 8, hipMemcpyHostToDevice);
    }
-   const void *src[12] = {nullptr, nullptr, nullptr, t.mtl, zb.data(), cfl.data(), bcc.data(), casa.data(),
-                          goff.data(), gyear.data(), gmax.data(), sgrp.data()};
-   const size_t len[12] = {0, 0, 0, (size_t) nc * 8, (size_t) nc * 8, (size_t) nc, bcc.size() * 4, ncs * 8,
-                           (size_t) G * 8, (size_t) G * 4, (size_t) G * 4, (size_t) S * 4};
-   for (int k = 3; k < 12 && he == hipSuccess; k++) he = hipMemcpy(d + off[k], src[k], len[k], hipMemcpyHostToDevice);
+   const void *src[13] = {nullptr, nullptr, nullptr, t.mtl, zb.data(), cfl.data(), bcc.data(), casa.data(),
+                          goff.data(), gyear.data(), gmax.data(), sgrp.data(), lst.data()};
+   const size_t len[13] = {0, 0, 0, (size_t) nc * 8, (size_t) nc * 8, (size_t) nc, bcc.size() * 4, ncs * 8,
+                           (size_t) G * 8, (size_t) G * 4, (size_t) G * 4, (size_t) S * 4, lst.size() * 4};
+   for (int k = 3; k < 13 && he == hipSuccess; k++) he = hipMemcpy(d + off[k], src[k], len[k], hipMemcpyHostToDevice);
    sl_tide_free(&t);
    if (he != hipSuccess) {
       if (d) (void) hipFree(d);
@@ -1785,6 +1791,8 @@ extern "C" int slg_batch_set_tide(slg_batch *X, const char *ft03, const char *bh
    T.gyear = (const int *) (d + off[9]);
    T.gmax = (const int *) (d + off[10]);
    T.sgrp = (const int *) (d + off[11]);
+   T.list = (const int *) (d + off[12]);
+   T.nlist = nl;
    T.field = (double *) (d + foff);
    X->tide_mem = d;
    X->tide_mode = mode;

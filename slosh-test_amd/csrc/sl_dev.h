@@ -145,6 +145,8 @@ struct SlTideDev {
    const uint8_t *cfl;        /* [ncell]: 1 the tide is added (SetTideUsableFlag),
                                  2 HB = STATICHT + tide (V2.1 / V2.2, ZB >= 290) */
    const int *bccell;         /* [nbcpt]: the square of each BDRYHT point */
+   const int *list;           /* [nlist]: the squares with a flag (the only ones a step changes) */
+   int nlist;
    const double *casa;        /* group g from goff[g]: per step 0 .. gmax[g] the cos, then
                                  the sin, of each constituent's speed x hours */
    const long *goff;          /* [ngrp] */

@@ -2431,18 +2431,20 @@ __device__ real static_height(const SlBasinDev &B, const SlRunDev &R, int s, int
 // device keeps it.  The
 // old tide of a square is the group's field of step n-1: the driver's stored
 // value and its recomputation from the previous call's angles are that sum.
+// A step's threads take the listed squares only (those SetTideUsableFlag or
+// the deep rule flag), the first call's every square.
 // One thread takes one square (or BDRYHT point) of SL_TSPT runs: the square's
 // ZB and flags are read once, and the runs' HB loads are in flight together.
 #define SL_TSPT 4
 __global__ void __launch_bounds__(256) k_tide_apply(SlBasinDev B, SlRunDev R, SlTideDev T, real *hb, int n,
                                                     int first)
 {
-   const int p = blockIdx.x * 256 + threadIdx.x, s0 = blockIdx.y * SL_TSPT;
-   if (p >= T.ncell + T.nbcpt) return;
+   const int nsq = first ? T.ncell : T.nlist, t0 = blockIdx.x * 256 + threadIdx.x, s0 = blockIdx.y * SL_TSPT;
+   if (t0 >= nsq + T.nbcpt) return;
    const int ns = min(SL_TSPT, R.nstorm - s0);
    const size_t now = (size_t) (n & 1) * T.ncell, old = (size_t) ((n - 1) & 1) * T.ncell;
-   if (p >= T.ncell) {   // the boundary squares: TIDESH = the tide at step n
-      const int b = p - T.ncell, c = T.bccell[b];
+   if (t0 >= nsq) {   // the boundary squares: TIDESH = the tide at step n
+      const int b = t0 - nsq, c = T.bccell[b];
       for (int t = 0; t < ns; t++) {
          const int s = s0 + t;
          if (first || R.nsteps[s] >= n)
@@ -2450,6 +2452,7 @@ __global__ void __launch_bounds__(256) k_tide_apply(SlBasinDev B, SlRunDev R, Sl
       }
       return;
    }
+   const int p = first ? t0 : T.list[t0];    // a step: only the flagged squares
    const uint8_t f = first ? 1 : T.cfl[p];   // the first call: every wet square (slosh2.c:859-896)
    if (!(f & 3)) return;                      // not usable: HB keeps its value
    const int i = p % T.ni, j = p / T.ni;
@@ -2675,8 +2678,10 @@ extern "C" int SL_TIDE(const SlBasinDev *B, const SlRunDev *R, const SlTideDev *
        T->ni < 1 || T->ncell % T->ni || T->ni > B->imxb - 1 || T->ncell / T->ni > B->jmxb - 1)
       return -1;
    hipLaunchKernelGGL(k_tide_field, dim3((T->ncell + 255) / 256, T->ngrp), dim3(256), 0, st, *T, n);
-   hipLaunchKernelGGL(k_tide_apply, dim3((T->ncell + T->nbcpt + 255) / 256, (R->nstorm + SL_TSPT - 1) / SL_TSPT),
-                      dim3(256), 0, st, *B, *R, *T, hb, n, first);
+   const int nsq = first ? T->ncell : T->nlist;
+   if (nsq + T->nbcpt > 0)
+      hipLaunchKernelGGL(k_tide_apply, dim3((nsq + T->nbcpt + 255) / 256, (R->nstorm + SL_TSPT - 1) / SL_TSPT),
+                         dim3(256), 0, st, *B, *R, *T, hb, n, first);
    hipError_t e = hipGetLastError();
    return e == hipSuccess ? 0 : -(int) e;
 }
