@@ -385,8 +385,9 @@ int slg_batch_static_height(const slg_batch *x, int storm, int i, int j, double 
  * moves by its DELT each step; the tables are its year's.  Call once, before
  * slg_batch_upload.  hipGraphs are not used while the tide is on (the steps
  * launch one by one); storm parts on several streams are (each part sums its
- * own tide fields).  SLG_EUNSUP for the hsf basins' 6-minute tide cadence
- * (slosh2.c:1565-1577).  The tide spin-up (-spinUp) is not provided.
+ * own tide fields).  The hsf basins call the tide every 6 model minutes, as
+ * RunLoopStep and PerformRun do there (slosh2.c:1565-1577, 2583-2586).  The
+ * tide spin-up (-spinUp) is not provided.
  * datums_tide: slg_track_datums for such a run (InitWater_TideModeOverride
  * with f_tide = mode). */
 int slg_batch_set_tide(slg_batch *x, const char *ft03, const char *bhc, const char *adj, int mode, int thresh,

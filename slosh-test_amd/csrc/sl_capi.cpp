@@ -1644,11 +1644,12 @@ extern "C" int slg_batch_set_tide(slg_batch *X, const char *ft03, const char *bh
       seterr(err, errlen, "%s", e);
       return SLG_EINPUT;
    }
-   if (!strncmp(t.abbrev, "hsf", 3)) {   // RunLoopStep calls CalcTideGrid2 every 6 minutes there
-      sl_tide_free(&t);
-      seterr(err, errlen, "%s", "the 6-minute tide cadence of the hsf basins is not supported");
-      return SLG_EUNSUP;
-   }
+   // the hsf basins: RunLoopStep calls CalcTideGrid2 only once the clock has
+   // reached tideClock, which PerformRun then moves 6 minutes on
+   // (slosh2.c:1565-1577, 2388, 2583-2586)
+   bool six = false;
+   for (const char *h : {"hsf1", "hsfa", "hsfb", "hsfc", "hsfd", "hsfe", "hsff"})
+      six = six || (!strncmp(t.abbrev, h, 4) && (t.abbrev[4] == 0 || t.abbrev[4] == ' '));
    const int ni = t.ni, nc = t.ncell;
    // clock groups: key = year, then the clock at every step
    std::map<std::vector<double>, int> gidx;
@@ -1694,6 +1695,21 @@ extern "C" int slg_batch_set_tide(slg_batch *X, const char *ft03, const char *bh
    // (the reference's build computes each pair with one sincos call, and
    // glibc's sincos differs from separate sin and cos in about 0.1 % of
    // arguments, so it is taken the same way here)
+   // the number of the tide call after each step of each group (-1: none)
+   std::vector<int> calls(ncs / (2 * SL_NUMT));
+   for (int g = 0; g < G; g++) {
+      int *c = &calls[goff[g] / (2 * SL_NUMT)], k = 0;
+      double tclk = gclk[g][0];
+      c[0] = 0;
+      for (size_t n = 1; n < gclk[g].size(); n++) {
+         if (!six) c[n] = (int) n;
+         else if (gclk[g][n] >= tclk) {
+            c[n] = ++k;
+            tclk += 6 * 60;
+         } else
+            c[n] = -1;
+      }
+   }
    std::vector<double> casa(ncs);
    for (int g = 0; g < G; g++) {
       const double ac = sl_clock_scan_date(years[gyear[g]], 1, 1);
@@ -1735,7 +1751,7 @@ extern "C" int slg_batch_set_tide(slg_batch *X, const char *ft03, const char *bh
    for (int n = 0; n < nb; n++) bcc[n] = (b->jsh[n] - 1) * ni + (b->ish[n] - 1);
    // one allocation: year tables, per-square values, groups, fields
    const size_t tab = (size_t) SL_NUMT * nc;
-   size_t off[13], tot = 0;
+   size_t off[14], tot = 0;
    int n = 0;
    auto add = [&](size_t bytes) {
       off[n++] = tot;
@@ -1747,6 +1763,7 @@ extern "C" int slg_batch_set_tide(slg_batch *X, const char *ft03, const char *bh
    add(ncs * 8); add((size_t) G * 8);                        // casa goff
    add((size_t) G * 4); add((size_t) G * 4); add((size_t) S * 4);   // gyear gmax sgrp
    add(lst.size() * 4);                                     // list
+   add(calls.size() * 4);                                   // call
    const size_t foff = tot;
    tot += (size_t) SL_MAXSPLIT * G * 2 * nc * 8;           // field, one per storm part
    char *d = nullptr;
@@ -1763,11 +1780,12 @@ extern "C" int slg_batch_set_tide(slg_batch *X, const char *ft03, const char *bh
       if (he == hipSuccess) he = hipMemcpy(d + off[1] + y * tab * 8, cv.data(), tab * 8, hipMemcpyHostToDevice);
       if (he == hipSuccess) he = hipMemcpy(d + off[2] + y * tab * 8, sv.data(), tab * 8, hipMemcpyHostToDevice);
    }
-   const void *src[13] = {nullptr, nullptr, nullptr, t.mtl, zb.data(), cfl.data(), bcc.data(), casa.data(),
-                          goff.data(), gyear.data(), gmax.data(), sgrp.data(), lst.data()};
-   const size_t len[13] = {0, 0, 0, (size_t) nc * 8, (size_t) nc * 8, (size_t) nc, bcc.size() * 4, ncs * 8,
-                           (size_t) G * 8, (size_t) G * 4, (size_t) G * 4, (size_t) S * 4, lst.size() * 4};
-   for (int k = 3; k < 13 && he == hipSuccess; k++) he = hipMemcpy(d + off[k], src[k], len[k], hipMemcpyHostToDevice);
+   const void *src[14] = {nullptr, nullptr, nullptr, t.mtl, zb.data(), cfl.data(), bcc.data(), casa.data(),
+                          goff.data(), gyear.data(), gmax.data(), sgrp.data(), lst.data(), calls.data()};
+   const size_t len[14] = {0, 0, 0, (size_t) nc * 8, (size_t) nc * 8, (size_t) nc, bcc.size() * 4, ncs * 8,
+                           (size_t) G * 8, (size_t) G * 4, (size_t) G * 4, (size_t) S * 4, lst.size() * 4,
+                           calls.size() * 4};
+   for (int k = 3; k < 14 && he == hipSuccess; k++) he = hipMemcpy(d + off[k], src[k], len[k], hipMemcpyHostToDevice);
    sl_tide_free(&t);
    if (he != hipSuccess) {
       if (d) (void) hipFree(d);
@@ -1792,6 +1810,7 @@ extern "C" int slg_batch_set_tide(slg_batch *X, const char *ft03, const char *bh
    T.gmax = (const int *) (d + off[10]);
    T.sgrp = (const int *) (d + off[11]);
    T.list = (const int *) (d + off[12]);
+   T.call = (const int *) (d + off[13]);
    T.nlist = nl;
    T.field = (double *) (d + foff);
    X->tide_mem = d;

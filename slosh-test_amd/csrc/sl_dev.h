@@ -146,6 +146,9 @@ struct SlTideDev {
                                  2 HB = STATICHT + tide (V2.1 / V2.2, ZB >= 290) */
    const int *bccell;         /* [nbcpt]: the square of each BDRYHT point */
    const int *list;           /* [nlist]: the squares with a flag (the only ones a step changes) */
+   const int *call;           /* group g from goff[g] / (2 SL_NTIDE): per step 0 .. gmax[g] the
+                                 number of the tide call made after it (0 before the first
+                                 step), or -1: no call (the hsf basins call every 6 minutes) */
    int nlist;
    const double *casa;        /* group g from goff[g]: per step 0 .. gmax[g] the cos, then
                                  the sin, of each constituent's speed x hours */

@@ -2396,6 +2396,8 @@ __global__ void __launch_bounds__(256) k_tide_field(SlTideDev T, int n)
 {
    const int c = blockIdx.x * 256 + threadIdx.x, g = blockIdx.y;
    if (c >= T.ncell || n > T.gmax[g]) return;
+   const int call = T.call[T.goff[g] / (2 * SL_NTIDE) + n];
+   if (call < 0) return;
    const double *ca = T.casa + T.goff[g] + (size_t) n * 2 * SL_NTIDE, *sa = ca + SL_NTIDE;
    const size_t yo = (size_t) T.gyear[g] * SL_NTIDE * T.ncell + c;
    const double *a = T.amp + yo, *cv = T.cv + yo, *sv = T.sv + yo;
@@ -2404,7 +2406,7 @@ __global__ void __launch_bounds__(256) k_tide_field(SlTideDev T, int n)
       const size_t o = (size_t) k * T.ncell;
       t = t + a[o] * (ca[k] * cv[o] - sa[k] * sv[o]);
    }
-   T.field[((size_t) g * 2 + (n & 1)) * T.ncell + c] = t;
+   T.field[((size_t) g * 2 + (call & 1)) * T.ncell + c] = t;
 }
 
 // STATICHT (staticht.f:14-41) at square (i,j) as the tide coupling calls it
@@ -2442,13 +2444,25 @@ __global__ void __launch_bounds__(256) k_tide_apply(SlBasinDev B, SlRunDev R, Sl
    const int nsq = first ? T.ncell : T.nlist, t0 = blockIdx.x * 256 + threadIdx.x, s0 = blockIdx.y * SL_TSPT;
    if (t0 >= nsq + T.nbcpt) return;
    const int ns = min(SL_TSPT, R.nstorm - s0);
-   const size_t now = (size_t) (n & 1) * T.ncell, old = (size_t) ((n - 1) & 1) * T.ncell;
-   if (t0 >= nsq) {   // the boundary squares: TIDESH = the tide at step n
+   // the field slots of this call and the one before, per run: -1 where its
+   // group makes no call after step n
+   int cs[SL_TSPT];
+#pragma unroll
+   for (int t = 0; t < SL_TSPT; t++) {
+      const int s = s0 + t;
+      cs[t] = -1;
+      if (t < ns && (first || R.nsteps[s] >= n)) {
+         const int g = T.sgrp[s];
+         cs[t] = T.call[T.goff[g] / (2 * SL_NTIDE) + n];
+      }
+   }
+   if (t0 >= nsq) {   // the boundary squares: TIDESH = the tide of this call
       const int b = t0 - nsq, c = T.bccell[b];
       for (int t = 0; t < ns; t++) {
          const int s = s0 + t;
-         if (first || R.nsteps[s] >= n)
-            R.tidesh[(size_t) s * T.nbcpt + b] = (real) T.field[(size_t) T.sgrp[s] * 2 * T.ncell + now + c];
+         if (cs[t] >= 0)
+            R.tidesh[(size_t) s * T.nbcpt + b] =
+               (real) T.field[((size_t) T.sgrp[s] * 2 + (cs[t] & 1)) * T.ncell + c];
       }
       return;
    }
@@ -2463,7 +2477,7 @@ __global__ void __launch_bounds__(256) k_tide_apply(SlBasinDev B, SlRunDev R, Sl
 #pragma unroll
    for (int t = 0; t < SL_TSPT; t++) {
       const int s = s0 + t;
-      on[t] = t < ns && (first || R.nsteps[s] >= n);
+      on[t] = cs[t] >= 0;
       h[t] = 0.;
       if (on[t] && !(f & 2)) h[t] = (double) hb[(size_t) s * R.fstride + q];
    }
@@ -2473,6 +2487,7 @@ __global__ void __launch_bounds__(256) k_tide_apply(SlBasinDev B, SlRunDev R, Sl
       const int s = s0 + t;
       const size_t o = (size_t) s * R.fstride + q;
       const double *fg = T.field + (size_t) T.sgrp[s] * 2 * T.ncell + p;
+      const size_t now = (size_t) (cs[t] & 1) * T.ncell, old = (size_t) ((cs[t] - 1) & 1) * T.ncell;
       if (f & 2) {   // V2.1 / V2.2 deep squares (slosh2.c:904-931)
          hb[o] = (real) ((double) static_height(B, R, s, n, i + 1, j + 1) + fg[now]);
       } else if (h[t] + zb != 0.) {   // wet: subtract the old tide (none before the first call), add the new

@@ -124,3 +124,23 @@ def test_batch_tide_across_a_delt_switch(tmp_path):
     env_r, _ = helpers.run_cli(helpers.REF_CLI, root, case["basin"], "ref", extra=["-f_tide", "V2.1.-5"])
     (env_g,) = _batched_envs(case, [case["trk"]], "V2.1.-5", 64, root, "b64")
     _same_env(env_g, env_r)
+
+
+def test_batch_tide_hsf_six_minute_cadence(tmp_path):
+    """The hsf basins call the tide only every 6 model minutes (RunLoopStep's
+    tidetime and PerformRun's tideClock, slosh2.c:1565-1577, 2583-2586): the
+    old tide is the previous call's and TIDESH holds between calls."""
+    root = str(tmp_path)
+    print("batch tide, hsf cadence", flush=True)
+    # a hyperbolic basin 'sfa' goes by 'hsfa' (setup.c:41-64)
+    b = synth.make_basin("sfa", "+", 30, 26, seed=45, land_frac=0.25, n_barriers=12, dt=40.0,
+                         size_km=(1.0, 3.0), corr=4.0)
+    t = synth.straight_track(b, 12.0, 330.0, 60.0, 15.0, hours=12, landfall=6)
+    case = helpers.make_case(root, b, t)
+    case["tidedir"] = synth.write_tide_db(root, b, case["basin"], seed=58, adj=True)
+    env_r, _ = helpers.run_cli(helpers.REF_CLI, root, case["basin"], "ref", extra=["-f_tide", "V2"])
+    (env_g,) = _batched_envs(case, [case["trk"]], "V2", 64, root, "b64")
+    _same_env(env_g, env_r)
+    env_o, _ = helpers.run_cli(helpers.ORC_CLI["32"], root, case["basin"], "orc32", extra=["-f_tide", "V2"])
+    (env_h,) = _batched_envs(case, [case["trk"]], "V2", 32, root, "b32")
+    _same_env(env_h, env_o)

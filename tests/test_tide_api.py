@@ -45,11 +45,6 @@ def test_set_tide_refuses_bad_input_before_the_device(tmp_path):
         open(other, "wb").write(synth.bhc_bytes(synth.tiny(seed=41, ni=20, nj=18)[0], case["basin"], 50))
         with pytest.raises(sloshgpu.SloshError, match="harmonic constants for 20 x 18 squares, the basin has 30 x 26"):
             X.set_tide(ft03, other, None, 2)
-        # the hsf basins' 6-minute cadence
-        hsf = os.path.join(root, "hsf1.bhc")
-        open(hsf, "wb").write(synth.bhc_bytes(b, "hsf1", 50))
-        with pytest.raises(sloshgpu.SloshError, match="6-minute"):
-            X.set_tide(ft03, hsf, None, 2)
     finally:
         X.close()
         B.close()
